@@ -1,0 +1,253 @@
+#!/usr/bin/env python3
+"""Incoherent-ray throughput of the traversal kernels on one MI355X.
+
+Workload: 2,000 subdivision-3 icospheres (2.56M triangles, centres U[0,100]^3,
+radii lognormal(0, 0.5), seed 2) in one SAH BVH built by the CPU constructor.
+A 1920x1080 G-buffer comes from primary closest hits; every pixel that hit
+spawns one cosine-weighted RTAO any-hit ray (tmax 2.0) and one cosine-weighted
+GI closest-hit ray (tmax 1e6), origins offset by 1e-4 along the face normal,
+directions from a per-pixel PCG hash.  A step traces both batches (all rays
+resident in HBM before timing starts); value = rays traced / second.
+
+The dominant kernel (GI closest hit) is priced against HBM bandwidth with the
+algorithmic bytes of every ray counted exactly by the counting build of the
+kernel: 32 B per stackless node visit (64 B per stack node), 64 B per triangle
+test (16-B triangle + three 16-B positions), 32 B ray read + 32 B hit write.
+
+cpu_baseline: the host restatement of the traversal (oracle/, the reference's
+GLSL walk in C) timed on a bounded sample of the same rays.
+
+    python bench.py [--steps K] [--warmup W] [--format stackless|stack]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "incoherent Mrays/s, batched vs plain launch on 1 MI355X; % of HBM roofline"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+class Events:
+    def __init__(self, crt, n):
+        import ctypes as C
+        self.crt = crt
+        self.C = C
+        self.ev = []
+        for _ in range(n):
+            h = C.c_void_p()
+            crt._lib.check(crt.lib().crt_event_create(C.byref(h)))
+            self.ev.append(h.value)
+
+    def record(self, i, stream):
+        self.crt._lib.check(self.crt.lib().crt_event_record(self.ev[i], stream))
+
+    def ms(self, a, b) -> float:
+        out = self.C.c_float()
+        self.crt._lib.check(self.crt.lib().crt_event_elapsed_ms(self.ev[a], self.ev[b], self.C.byref(out)))
+        return float(out.value)
+
+    def close(self):
+        for e in self.ev:
+            self.crt.lib().crt_event_destroy(e)
+
+
+def bytes_per_ray(fmt: int, visits: np.ndarray, tests: np.ndarray) -> np.ndarray:
+    node = 32 if fmt == 0 else 64
+    return node * visits.astype(np.float64) + 64.0 * tests.astype(np.float64) + 64.0
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--format", choices=("stackless", "stack"), default="stackless")
+    ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--lds-stack", type=int, default=0)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline duration")
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline (profiling runs)")
+    ap.add_argument("--spheres", type=int, default=2000)
+    args = ap.parse_args(argv)
+
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); rays shard across GPUs with no exchange, "
+              "so the figure of N GPUs is N independent copies of this run", file=sys.stderr)
+        return 2
+
+    import candela_amd as crt
+    from candela_amd import scenes
+
+    fmt = crt.FORMAT_STACKLESS if args.format == "stackless" else crt.FORMAT_STACK
+    if crt.device_count() < 1:
+        raise SystemExit("no HIP device visible")
+
+    # --- scene + BVH (CPU constructor) -------------------------------------------------
+    t0 = time.perf_counter()
+    meshes = scenes.icosphere_cloud(count=args.spheres, subdivisions=3, seed=2)
+    t1 = time.perf_counter()
+    r = crt.RayIntersector(fmt)
+    r.add_object(0, meshes)
+    t2 = time.perf_counter()
+    r.push_entity(0)
+    r.buffer_entities()
+    r.buffer_data()
+    hb = r.host_buffers()
+    ntri = int(hb["triangles"].size)
+    log(f"scene {ntri} tris built in {t2 - t1:.2f}s (generation {t1 - t0:.2f}s), {hb['nodes'].size} nodes")
+
+    # --- G-buffer and secondary rays --------------------------------------------------
+    W, H = 1920, 1080
+    iv, ip = scenes.camera_inverses(aspect=W / H, **scenes.CLOUD_CAMERA)
+    drays, dhits = r.intersect_primary(W, H, iv, ip)
+    crt.lib().crt_device_sync()
+    prays, phits = drays.download(), dhits.download()
+    pos = hb["vertices"]["position"][:, :3]
+    ao, gi = scenes.secondary_rays(prays, phits, pos, hb["triangles"], seed=2)
+    n_ao, n_gi = ao.size, gi.size
+    log(f"primary hits {n_ao}/{W * H}; {n_ao} AO + {n_gi} GI rays per step")
+    d_ao, d_gi = crt.DeviceArray.from_host(ao), crt.DeviceArray.from_host(gi)
+    h_ao, h_gi = crt.DeviceArray(crt.HIT_DTYPE, n_ao), crt.DeviceArray(crt.HIT_DTYPE, n_gi)
+    scene = r.scene()
+    stream = crt.Stream()
+    sh = stream.handle
+
+    def step():
+        crt.trace(scene, crt.QUERY_ANY, d_ao, h_ao, block_size=args.block, lds_stack=args.lds_stack, stream=stream)
+        crt.trace(scene, crt.QUERY_CLOSEST, d_gi, h_gi, block_size=args.block, lds_stack=args.lds_stack,
+                  stream=stream)
+
+    for _ in range(args.warmup):
+        step()
+    stream.sync()
+    crt.lib().crt_device_sync()
+
+    K = args.steps
+    ev = Events(crt, 3 * K + 2)
+    crt.lib().crt_device_sync()
+    w0 = time.perf_counter()
+    ev.record(0, sh)
+    for k in range(K):
+        ev.record(1 + 3 * k, sh)
+        crt.trace(scene, crt.QUERY_ANY, d_ao, h_ao, block_size=args.block, lds_stack=args.lds_stack, stream=stream)
+        ev.record(2 + 3 * k, sh)
+        crt.trace(scene, crt.QUERY_CLOSEST, d_gi, h_gi, block_size=args.block, lds_stack=args.lds_stack,
+                  stream=stream)
+        ev.record(3 + 3 * k, sh)
+    ev.record(3 * K + 1, sh)
+    stream.sync()
+    crt.lib().crt_device_sync()
+    w1 = time.perf_counter()
+    total_ms = ev.ms(0, 3 * K + 1)
+    ao_ms = [ev.ms(1 + 3 * k, 2 + 3 * k) for k in range(K)]
+    gi_ms = [ev.ms(2 + 3 * k, 3 + 3 * k) for k in range(K)]
+    ev.close()
+    wall_ms = (w1 - w0) * 1e3
+    ms_step = total_ms / K
+    rays_step = n_ao + n_gi
+    value = rays_step / (ms_step * 1e-3) / 1e6
+    log(f"{K} steps: {ms_step:.3f} ms/step (wall {wall_ms / K:.3f}), AO {np.mean(ao_ms):.3f} ms, "
+        f"GI {np.mean(gi_ms):.3f} ms -> {value:.1f} Mrays/s")
+
+    # --- algorithmic bytes (counting build, untimed) ------------------------------------
+    out = {}
+    for name, dr, dh, q, n in (("ao", d_ao, h_ao, crt.QUERY_ANY, n_ao), ("gi", d_gi, h_gi, crt.QUERY_CLOSEST, n_gi)):
+        cnt = crt.DeviceArray(np.uint32, n)
+        import ctypes as C
+        cfg = crt.TraceConfig(args.block, args.lds_stack, cnt.ptr)
+        crt._lib.check(crt.lib().crt_trace(C.byref(scene), q, 0, dr.ptr, dh.ptr, n, C.byref(cfg), None))
+        crt.lib().crt_device_sync()
+        hits = dh.download()
+        tests = cnt.download()
+        visits = hits["status"] & crt.STATUS_VISITS_MASK
+        b = bytes_per_ray(fmt, visits, tests)
+        out[name] = dict(hits=hits, bytes=float(b.sum()), visits=float(visits.mean()), tests=float(tests.mean()),
+                         capped=float(np.mean((hits["status"] & crt.STATUS_ITER_CAP) != 0)),
+                         hit_rate=float(np.mean(hits["t"] > 0)))
+    gi_s = np.mean(gi_ms) * 1e-3
+    achieved = out["gi"]["bytes"] / gi_s / 1e9
+    log(f"GI: {out['gi']['visits']:.1f} node visits, {out['gi']['tests']:.1f} tri tests per ray, "
+        f"{out['gi']['bytes'] / n_gi:.0f} B/ray, {achieved:.0f} GB/s algorithmic; "
+        f"iteration-capped {out['gi']['capped'] * 100:.2f}%")
+
+    # --- CPU baseline on a bounded sample --------------------------------------------------
+    cpu = None
+    parity = None
+    if not args.no_cpu:
+        import oracle
+        threads = max(1, min(args.cpu_threads, os.cpu_count() or 1))
+        nodes, tris, verts, ents = hb["nodes"], hb["triangles"], hb["vertices"], hb["entities"]
+        probe = 4096
+        ts = time.perf_counter()
+        oracle.trace(fmt, nodes, tris, verts, ents, 0, gi[:probe], threads=threads)
+        per_ray = (time.perf_counter() - ts) / probe
+        n_s = int(min(n_gi, max(probe, args.cpu_seconds / 2 / max(per_ray, 1e-9))))
+        sample_gi, sample_ao = gi[:n_s], ao[:n_s]
+        ts = time.perf_counter()
+        ohg = oracle.trace(fmt, nodes, tris, verts, ents, 0, sample_gi, threads=threads)
+        oha = oracle.trace(fmt, nodes, tris, verts, ents, 1, sample_ao, threads=threads)
+        cpu_s = time.perf_counter() - ts
+        cpu_value = 2 * n_s / cpu_s / 1e6
+        cpu = {"value": round(cpu_value, 4), "unit": "Mrays/s", "cores": threads, "kind": "port",
+               "sample": f"first {n_s} GI closest-hit + first {n_s} RTAO any-hit rays of the same step, "
+                         f"{cpu_s:.1f} s on {threads} threads (C restatement of the GLSL walk, -O3)"}
+        g, a = out["gi"]["hits"][:n_s], out["ao"]["hits"][:n_s]
+        parity = {"rays": 2 * n_s,
+                  "exact": float((np.concatenate([g, a]).tobytes() == np.concatenate([ohg, oha]).tobytes())),
+                  "id_agreement": float(np.mean(np.concatenate([g["tri"] == ohg["tri"], a["tri"] == oha["tri"]])))}
+        log(f"cpu baseline {cpu_value:.3f} Mrays/s on {threads} threads; parity {parity}")
+
+    line = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "incoherent GI closest-hit + RTAO any-hit rays from a 1920x1080 G-buffer, "
+                        "2000 icospheres (2.56M tris)",
+            "triangles": ntri,
+            "rays_per_step": rays_step,
+            "gi_rays": n_gi,
+            "ao_rays": n_ao,
+            "format": args.format,
+            "kernel": "plain (one batch per launch)",
+            "parallelism": "single GPU",
+        },
+        "plain_mrays_s": round(value, 2),
+        "batched_mrays_s": None,
+        "roofline": {"bound": "hbm", "kernel": "trace_kernel (GI closest hit)", "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None,
+                     "bytes_per_launch": out["gi"]["bytes"], "launch_ms": round(gi_s * 1e3, 4),
+                     "bytes_per_ray": round(out["gi"]["bytes"] / n_gi, 1)},
+        "cpu_baseline": cpu,
+        "parity": parity,
+        "detail": {k: {kk: vv for kk, vv in v.items() if kk != "hits"} for k, v in out.items()},
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,243 @@
+"""candela_amd -- MI355X-native ray traversal for Candela.
+
+Python face of libcandela_rt (include/candela_rt.h).  It mirrors the
+reference's host API, ``RayIntersector<T>`` (Source/Core/BVH/Intersector.h:60-124),
+and its CPU constructor, ``BuildBVH`` (Source/Core/BVH/BVHConstructor.cpp:951-1108).
+Every call goes to the compiled library; there is no Python fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from ._lib import (BOUNDS_DTYPE, ENTITY_DTYPE, FLAG_IGNORE_TRANSPARENT, FLAG_TRI0_IS_MISS, FORMAT_STACK,
+                   FORMAT_STACKLESS, HIT_DTYPE, QUERY_ANY, QUERY_CLOSEST, RAY_DTYPE, STACK_NODE_DTYPE,
+                   STACKLESS_NODE_DTYPE, STATUS_BAD_INDEX, STATUS_ITER_CAP, STATUS_STACK_OVERFLOW,
+                   STATUS_VISITS_MASK, SWAP_NONE, SWAP_SEEDED, TRIANGLE_DTYPE, VERTEX_DTYPE, BuildOptions,
+                   BvhStats, CrtError, Mesh, SceneView, TraceConfig, check, lib, node_dtype, ptr)
+
+__all__ = [
+    "VERTEX_DTYPE", "TRIANGLE_DTYPE", "STACKLESS_NODE_DTYPE", "STACK_NODE_DTYPE", "BOUNDS_DTYPE", "ENTITY_DTYPE",
+    "RAY_DTYPE", "HIT_DTYPE", "FORMAT_STACKLESS", "FORMAT_STACK", "QUERY_CLOSEST", "QUERY_ANY",
+    "FLAG_IGNORE_TRANSPARENT", "FLAG_TRI0_IS_MISS", "STATUS_ITER_CAP", "STATUS_STACK_OVERFLOW", "STATUS_BAD_INDEX",
+    "STATUS_VISITS_MASK", "SWAP_NONE", "SWAP_SEEDED", "CrtError", "DeviceArray", "Stream", "BuiltBVH", "build_bvh",
+    "RayIntersector", "trace", "device_count", "node_dtype", "lib",
+]
+
+
+def device_count() -> int:
+    return int(lib().crt_device_count())
+
+
+class Stream:
+    """A non-blocking HIP stream owned by the library."""
+
+    def __init__(self):
+        h = C.c_void_p()
+        check(lib().crt_stream_create(C.byref(h)))
+        self.handle = h.value
+
+    def sync(self) -> None:
+        check(lib().crt_stream_sync(self.handle))
+
+    def close(self) -> None:
+        if self.handle:
+            lib().crt_stream_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _stream_handle(stream) -> int | None:
+    if stream is None:
+        return None
+    return stream.handle if isinstance(stream, Stream) else int(stream)
+
+
+class DeviceArray:
+    """A device allocation holding `count` records of `dtype`."""
+
+    def __init__(self, dtype, count: int):
+        self.dtype = np.dtype(dtype)
+        self.count = int(count)
+        self.nbytes = self.dtype.itemsize * self.count
+        p = C.c_void_p()
+        check(lib().crt_device_alloc(C.byref(p), max(1, self.nbytes)))
+        self.ptr = p.value
+
+    @classmethod
+    def from_host(cls, a: np.ndarray, stream=None) -> "DeviceArray":
+        a = np.ascontiguousarray(a)
+        d = cls(a.dtype, a.size)
+        d.upload(a, stream)
+        return d
+
+    def upload(self, a: np.ndarray, stream=None) -> None:
+        a = np.ascontiguousarray(a)
+        if a.nbytes != self.nbytes:
+            raise ValueError(f"upload size {a.nbytes} != allocation {self.nbytes}")
+        check(lib().crt_memcpy_h2d(self.ptr, ptr(a), self.nbytes, _stream_handle(stream)))
+
+    def download(self, stream=None) -> np.ndarray:
+        out = np.empty(self.count, dtype=self.dtype)
+        check(lib().crt_memcpy_d2h(ptr(out), self.ptr, self.nbytes, _stream_handle(stream)))
+        return out
+
+    def fill_bytes(self, value: int, stream=None) -> None:
+        check(lib().crt_memset_d(self.ptr, int(value), self.nbytes, _stream_handle(stream)))
+
+    def free(self) -> None:
+        if self.ptr:
+            lib().crt_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def _meshes(meshes):
+    """[(vertices VERTEX_DTYPE, indices uint32, global_mesh_number)] -> ctypes array + keep-alive list."""
+    keep = []
+    arr = (Mesh * max(1, len(meshes)))()
+    for i, (v, idx, num) in enumerate(meshes):
+        v = np.ascontiguousarray(v, dtype=VERTEX_DTYPE)
+        idx = np.ascontiguousarray(idx, dtype=np.uint32).ravel()
+        keep += [v, idx]
+        arr[i] = Mesh(ptr(v), v.size, ptr(idx), idx.size, int(num))
+    return arr, keep
+
+
+@dataclass
+class BuiltBVH:
+    format: int
+    nodes: np.ndarray
+    triangles: np.ndarray
+    vertices: np.ndarray
+    stats: dict
+
+
+def build_bvh(fmt: int, meshes, t_offset: int = 0, child_swap: int = SWAP_NONE, seed: int = 0,
+              threads: int = 0) -> BuiltBVH:
+    """BuildBVH (BVHConstructor.cpp:951-1108) on the CPU; returns host copies of the flattened buffers."""
+    arr, keep = _meshes(meshes)
+    opts = BuildOptions(child_swap, seed & 0xFFFFFFFF, threads)
+    h = C.c_void_p()
+    check(lib().crt_build_bvh(fmt, arr, len(meshes), int(t_offset), C.byref(opts), C.byref(h)))
+    try:
+        st = BvhStats()
+        check(lib().crt_bvh_get_stats(h, C.byref(st)))
+        n_nodes, n_tri, n_vtx = int(st.array_length), int(st.triangle_count), int(st.vertex_count)
+        nd = node_dtype(fmt)
+        nodes = np.frombuffer((C.c_char * (n_nodes * nd.itemsize)).from_address(lib().crt_bvh_nodes(h)),
+                              dtype=nd).copy()
+        tris = np.frombuffer((C.c_char * (n_tri * 16)).from_address(lib().crt_bvh_triangles(h)),
+                             dtype=TRIANGLE_DTYPE).copy() if n_tri else np.zeros(0, TRIANGLE_DTYPE)
+        verts = np.frombuffer((C.c_char * (n_vtx * 32)).from_address(lib().crt_bvh_vertices(h)),
+                              dtype=VERTEX_DTYPE).copy() if n_vtx else np.zeros(0, VERTEX_DTYPE)
+        stats = {k: int(getattr(st, k)) for k, _ in BvhStats._fields_}
+    finally:
+        lib().crt_bvh_free(h)
+    del keep
+    return BuiltBVH(fmt, nodes, tris, verts, stats)
+
+
+def _colmajor(m) -> np.ndarray:
+    """4x4 matrix in math convention (m[row, col]) -> 16 floats column-major (glm layout)."""
+    m = np.asarray(m, dtype=np.float32)
+    if m.shape == (16,):
+        return np.ascontiguousarray(m)
+    return np.ascontiguousarray(m.T.reshape(16))
+
+
+def trace(scene: SceneView, query: int, rays: DeviceArray, hits: DeviceArray, n: int | None = None, flags: int = 0,
+          block_size: int = 0, lds_stack: int = 0, stream=None) -> None:
+    """crt_trace: closest- or any-hit for every ray of a device ray buffer (asynchronous)."""
+    n = rays.count if n is None else int(n)
+    if n > rays.count or n > hits.count:
+        raise ValueError("ray/hit buffers smaller than n")
+    cfg = TraceConfig(block_size, lds_stack)
+    check(lib().crt_trace(C.byref(scene), query, flags, rays.ptr, hits.ptr, n, C.byref(cfg), _stream_handle(stream)))
+
+
+class RayIntersector:
+    """RayIntersector<T> (Intersector.h:60-351); fmt selects T (stackless / stack node)."""
+
+    def __init__(self, fmt: int = FORMAT_STACKLESS):
+        h = C.c_void_p()
+        check(lib().crt_create(fmt, C.byref(h)))
+        self.handle = h.value
+        self.format = fmt
+
+    def close(self) -> None:
+        if self.handle:
+            lib().crt_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_object(self, object_id: int, meshes, child_swap: int = SWAP_NONE, seed: int = 0, threads: int = 0):
+        arr, keep = _meshes(meshes)
+        opts = BuildOptions(child_swap, seed & 0xFFFFFFFF, threads)
+        check(lib().crt_add_object(self.handle, object_id, arr, len(meshes), C.byref(opts)))
+        del keep
+
+    def add_object_prebuilt(self, object_id: int, nodes: np.ndarray, triangles: np.ndarray, vertices: np.ndarray):
+        nodes = np.ascontiguousarray(nodes, dtype=node_dtype(self.format))
+        triangles = np.ascontiguousarray(triangles, dtype=TRIANGLE_DTYPE)
+        vertices = np.ascontiguousarray(vertices, dtype=VERTEX_DTYPE)
+        check(lib().crt_add_object_prebuilt(self.handle, object_id, ptr(nodes), nodes.size, ptr(triangles),
+                                            triangles.size, ptr(vertices), vertices.size))
+
+    def push_entity(self, object_id: int, model=None, emissive: float = 0.0, translucency: float = 0.0):
+        m = _colmajor(np.eye(4, dtype=np.float32) if model is None else model)
+        check(lib().crt_push_entity(self.handle, object_id, ptr(m), float(emissive), float(translucency)))
+
+    def buffer_entities(self):
+        check(lib().crt_buffer_entities(self.handle))
+
+    def buffer_data(self, clear_cpu_data: bool = False):
+        check(lib().crt_buffer_data(self.handle, int(bool(clear_cpu_data))))
+
+    def host_buffers(self) -> dict:
+        p = [C.c_void_p() for _ in range(4)]
+        n = [C.c_int64() for _ in range(4)]
+        check(lib().crt_context_host_buffers(self.handle, C.byref(p[0]), C.byref(n[0]), C.byref(p[1]),
+                                             C.byref(n[1]), C.byref(p[2]), C.byref(n[2]), C.byref(p[3]),
+                                             C.byref(n[3])))
+        out = {}
+        for key, pp, nn, dt in zip(("nodes", "triangles", "vertices", "entities"), p, n,
+                                   (node_dtype(self.format), TRIANGLE_DTYPE, VERTEX_DTYPE, ENTITY_DTYPE)):
+            cnt = int(nn.value)
+            out[key] = (np.frombuffer((C.c_char * (cnt * dt.itemsize)).from_address(pp.value), dtype=dt).copy()
+                        if cnt else np.zeros(0, dt))
+        return out
+
+    def scene(self) -> SceneView:
+        s = SceneView()
+        check(lib().crt_context_scene(self.handle, C.byref(s)))
+        return s
+
+    def intersect(self, query: int, rays: DeviceArray, hits: DeviceArray, flags: int = 0, stream=None):
+        check(lib().crt_intersect(self.handle, query, flags, rays.ptr, hits.ptr, rays.count,
+                                  _stream_handle(stream)))
+
+    def intersect_primary(self, width: int, height: int, inv_view, inv_proj, flags: int = 0, stream=None):
+        rays = DeviceArray(RAY_DTYPE, width * height)
+        hits = DeviceArray(HIT_DTYPE, width * height)
+        iv, ip = _colmajor(inv_view), _colmajor(inv_proj)
+        check(lib().crt_intersect_primary(self.handle, rays.ptr, hits.ptr, width, height, ptr(iv), ptr(ip), flags,
+                                          _stream_handle(stream)))
+        return rays, hits

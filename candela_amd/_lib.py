@@ -1,0 +1,152 @@
+"""ctypes binding of libcandela_rt.so (the C ABI declared in include/candela_rt.h).
+
+The library is the product: there is no Python or CPU fallback.  Loading fails
+loudly when the shared object has not been built.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcandela_rt.so")
+
+# --- constants (include/candela_rt.h) -----------------------------------------
+CRT_OK = 0
+FORMAT_STACKLESS = 0
+FORMAT_STACK = 1
+QUERY_CLOSEST = 0
+QUERY_ANY = 1
+FLAG_IGNORE_TRANSPARENT = 1 << 0
+FLAG_TRI0_IS_MISS = 1 << 1
+STATUS_ITER_CAP = 1 << 24
+STATUS_STACK_OVERFLOW = 1 << 25
+STATUS_BAD_INDEX = 1 << 26
+STATUS_VISITS_MASK = 0x00FFFFFF
+SWAP_NONE = 0
+SWAP_SEEDED = 1
+
+# --- record dtypes (byte layouts of the reference buffers) ---------------------
+VERTEX_DTYPE = np.dtype([("position", "<f4", 4), ("normal_tangent_data", "<u4", 3), ("texcoords", "<u4")])
+TRIANGLE_DTYPE = np.dtype([("packed", "<i4", 4)])
+STACKLESS_NODE_DTYPE = np.dtype([("min", "<f4", 4), ("max", "<f4", 4)])
+BOUNDS_DTYPE = np.dtype([("min", "<f4", 4), ("max", "<f4", 4)])
+STACK_NODE_DTYPE = np.dtype([("left", BOUNDS_DTYPE), ("right", BOUNDS_DTYPE)])
+ENTITY_DTYPE = np.dtype([("model", "<f4", 16), ("inverse", "<f4", 16), ("node_offset", "<i4"),
+                         ("node_count", "<i4"), ("data", "<i4", 14)])
+RAY_DTYPE = np.dtype([("org", "<f4", 3), ("tmax", "<f4"), ("dir", "<f4", 3), ("id", "<u4")])
+HIT_DTYPE = np.dtype([("t", "<f4"), ("u", "<f4"), ("v", "<f4"), ("w", "<f4"), ("tri", "<i4"),
+                      ("mesh", "<i4"), ("entity", "<i4"), ("status", "<u4")])
+assert VERTEX_DTYPE.itemsize == 32 and TRIANGLE_DTYPE.itemsize == 16
+assert STACKLESS_NODE_DTYPE.itemsize == 32 and STACK_NODE_DTYPE.itemsize == 64
+assert ENTITY_DTYPE.itemsize == 192 and RAY_DTYPE.itemsize == 32 and HIT_DTYPE.itemsize == 32
+
+
+def node_dtype(fmt: int) -> np.dtype:
+    return STACKLESS_NODE_DTYPE if fmt == FORMAT_STACKLESS else STACK_NODE_DTYPE
+
+
+# --- ctypes structs ---------------------------------------------------------------
+class Mesh(C.Structure):
+    _fields_ = [("vertices", C.c_void_p), ("vertex_count", C.c_uint32), ("indices", C.c_void_p),
+                ("index_count", C.c_uint32), ("global_mesh_number", C.c_int32)]
+
+
+class BuildOptions(C.Structure):
+    _fields_ = [("child_swap", C.c_int32), ("seed", C.c_uint32), ("threads", C.c_int32)]
+
+
+class BvhStats(C.Structure):
+    _fields_ = [("triangle_count", C.c_uint64), ("node_count", C.c_uint64), ("leaf_count", C.c_uint64),
+                ("split_fails", C.c_uint64), ("max_depth", C.c_uint32), ("array_length", C.c_uint64),
+                ("vertex_count", C.c_uint64)]
+
+
+class SceneView(C.Structure):
+    _fields_ = [("format", C.c_int32), ("entity_count", C.c_int32), ("total_nodes", C.c_int64),
+                ("nodes", C.c_void_p), ("triangles", C.c_void_p), ("vertices", C.c_void_p),
+                ("entities", C.c_void_p)]
+
+
+class TraceConfig(C.Structure):
+    _fields_ = [("block_size", C.c_int32), ("lds_stack", C.c_int32), ("tri_tests", C.c_void_p)]
+
+
+class CrtError(RuntimeError):
+    pass
+
+
+_lib = None
+
+# name -> (restype, argtypes); the list is also the export check of the tests.
+SIGNATURES = {
+    "crt_last_error": (C.c_char_p, []),
+    "crt_abi_version": (C.c_int, []),
+    "crt_device_count": (C.c_int, []),
+    "crt_build_bvh": (C.c_int, [C.c_int, C.c_void_p, C.c_int, C.c_int32, C.c_void_p, C.POINTER(C.c_void_p)]),
+    "crt_bvh_free": (None, [C.c_void_p]),
+    "crt_bvh_get_stats": (C.c_int, [C.c_void_p, C.POINTER(BvhStats)]),
+    "crt_bvh_nodes": (C.c_void_p, [C.c_void_p]),
+    "crt_bvh_triangles": (C.c_void_p, [C.c_void_p]),
+    "crt_bvh_vertices": (C.c_void_p, [C.c_void_p]),
+    "crt_bvh_format": (C.c_int, [C.c_void_p]),
+    "crt_device_alloc": (C.c_int, [C.POINTER(C.c_void_p), C.c_size_t]),
+    "crt_device_free": (C.c_int, [C.c_void_p]),
+    "crt_memcpy_h2d": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
+    "crt_memcpy_d2h": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
+    "crt_memset_d": (C.c_int, [C.c_void_p, C.c_int, C.c_size_t, C.c_void_p]),
+    "crt_stream_create": (C.c_int, [C.POINTER(C.c_void_p)]),
+    "crt_stream_destroy": (C.c_int, [C.c_void_p]),
+    "crt_stream_sync": (C.c_int, [C.c_void_p]),
+    "crt_device_sync": (C.c_int, []),
+    "crt_event_create": (C.c_int, [C.POINTER(C.c_void_p)]),
+    "crt_event_destroy": (C.c_int, [C.c_void_p]),
+    "crt_event_record": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "crt_event_elapsed_ms": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_float)]),
+    "crt_trace": (C.c_int, [C.POINTER(SceneView), C.c_int, C.c_uint32, C.c_void_p, C.c_void_p, C.c_int64,
+                            C.POINTER(TraceConfig), C.c_void_p]),
+    "crt_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
+    "crt_destroy": (None, [C.c_void_p]),
+    "crt_context_format": (C.c_int, [C.c_void_p]),
+    "crt_add_object": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int, C.c_void_p]),
+    "crt_add_object_prebuilt": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64,
+                                          C.c_void_p, C.c_int64]),
+    "crt_push_entity": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_float, C.c_float]),
+    "crt_buffer_entities": (C.c_int, [C.c_void_p]),
+    "crt_buffer_data": (C.c_int, [C.c_void_p, C.c_int]),
+    "crt_context_host_buffers": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_int64),
+                                           C.POINTER(C.c_void_p), C.POINTER(C.c_int64), C.POINTER(C.c_void_p),
+                                           C.POINTER(C.c_int64), C.POINTER(C.c_void_p), C.POINTER(C.c_int64)]),
+    "crt_context_scene": (C.c_int, [C.c_void_p, C.POINTER(SceneView)]),
+    "crt_intersect": (C.c_int, [C.c_void_p, C.c_int, C.c_uint32, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]),
+    "crt_intersect_primary": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p,
+                                        C.c_void_p, C.c_uint32, C.c_void_p]),
+}
+
+
+def lib() -> C.CDLL:
+    """Loads libcandela_rt.so once; raises if it was not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it with `python -m candela_amd.build` "
+                              "(there is no fallback implementation)")
+        l = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(l, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = l
+    return _lib
+
+
+def check(rc: int) -> None:
+    if rc != CRT_OK:
+        msg = lib().crt_last_error()
+        raise CrtError(f"libcandela_rt error {rc}: {msg.decode() if msg else ''}")
+
+
+def ptr(a: np.ndarray) -> int:
+    return a.ctypes.data

@@ -1,0 +1,470 @@
+// Host runtime of libcandela_rt: errors, device memory, the raw trace entry
+// and the RayIntersector<T> context (Source/Core/BVH/Intersector.h:60-351).
+#include <hip/hip_runtime_api.h>
+
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "candela_rt.h"
+#include "crt_internal.h"
+#include "trace_launch.h"
+
+namespace crt {
+
+static thread_local std::string g_last_error;
+
+int set_error(int code, const std::string& msg) {
+    g_last_error = msg;
+    return code;
+}
+
+static int hip_fail(hipError_t e, const char* what) {
+    return set_error(CRT_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define CRT_HIP(call, what)                      \
+    do {                                         \
+        hipError_t e_ = (call);                  \
+        if (e_ != hipSuccess) return ::crt::hip_fail(e_, what); \
+    } while (0)
+
+static int validate_trace_config(const crt_trace_config* cfg, int& block, int& depth) {
+    block = 256;
+    depth = 16;
+    if (cfg) {
+        if (cfg->block_size != 0) block = cfg->block_size;
+        if (cfg->lds_stack != 0) depth = cfg->lds_stack;
+    }
+    if (block < 64 || block > 1024 || block % 64 != 0)
+        return set_error(CRT_E_INVALID, "trace config: block_size must be a multiple of 64 in [64, 1024]");
+    if (depth != 8 && depth != 16 && depth != 32 && depth != 48 && depth != 64)
+        return set_error(CRT_E_INVALID, "trace config: lds_stack must be one of 8, 16, 32, 48, 64");
+    if (size_t(depth) * size_t(block) * 4 > 64 * 1024)
+        return set_error(CRT_E_INVALID, "trace config: lds_stack * block_size * 4 exceeds 64 KiB of LDS");
+    return CRT_OK;
+}
+
+static int validate_scene(const crt_scene_view* s) {
+    if (!s) return set_error(CRT_E_INVALID, "scene is null");
+    if (s->format != CRT_FORMAT_STACKLESS && s->format != CRT_FORMAT_STACK)
+        return set_error(CRT_E_INVALID, "scene: unknown node format");
+    if (s->entity_count < 0) return set_error(CRT_E_INVALID, "scene: negative entity count");
+    if (s->entity_count > 0 && (!s->nodes || !s->triangles || !s->vertices || !s->entities || s->total_nodes <= 0))
+        return set_error(CRT_E_INVALID, "scene: null device buffer");
+    return CRT_OK;
+}
+
+int trace(const crt_scene_view* scene, int query, uint32_t flags, const crt_ray* rays, crt_hit* hits, int64_t n,
+          const crt_trace_config* cfg, void* stream) {
+    int rc = validate_scene(scene);
+    if (rc) return rc;
+    if (query != CRT_QUERY_CLOSEST && query != CRT_QUERY_ANY) return set_error(CRT_E_INVALID, "crt_trace: bad query");
+    if (n < 0) return set_error(CRT_E_INVALID, "crt_trace: negative ray count");
+    if (n > 0 && (!rays || !hits)) return set_error(CRT_E_INVALID, "crt_trace: null ray or hit buffer");
+    if (n > (int64_t(1) << 31) * 256) return set_error(CRT_E_INVALID, "crt_trace: too many rays for one launch");
+    int block, depth;
+    rc = validate_trace_config(cfg, block, depth);
+    if (rc) return rc;
+    SceneArgs a{scene->format, scene->entity_count, scene->total_nodes, scene->nodes,
+                scene->triangles, scene->vertices, scene->entities};
+    CRT_HIP(launch_trace(a, query, rays, hits, n, flags, kMaxIterations, block, depth, cfg ? cfg->tri_tests : nullptr,
+                         static_cast<hipStream_t>(stream)),
+            "crt_trace: kernel launch");
+    return CRT_OK;
+}
+
+}  // namespace crt
+
+// ---------------------------------------------------------------------------
+// RayIntersector<T> context
+// ---------------------------------------------------------------------------
+struct ObjectData {  // _ObjectData, Intersector.h:51-56
+    int64_t triangle_offset;
+    int64_t vertices_offset;
+    int64_t node_offset;
+    int64_t node_count;
+};
+
+struct crt_context {
+    int format = CRT_FORMAT_STACKLESS;
+    // Host copies (m_BVHNodes, m_BVHVertices, m_BVHTriangles, m_BVHEntities).
+    std::vector<crt_stackless_node> stackless;
+    std::vector<crt_stack_node> stack;
+    std::vector<crt_vertex> vertices;
+    std::vector<crt_triangle> triangles;
+    std::vector<crt_entity> entities;         // buffered (m_BVHEntities)
+    std::vector<crt_entity> pending;          // m_Entities
+    std::map<int32_t, ObjectData> objects;    // m_ObjectData
+    uint32_t index_offset = 0;                // m_IndexOffset
+    // Device buffers.
+    void* d_nodes = nullptr;
+    void* d_tris = nullptr;
+    void* d_verts = nullptr;
+    void* d_entities = nullptr;
+    int64_t nodes_buffered = 0;               // m_NodeCountBuffered
+    int64_t entities_buffered = 0;            // m_EntityPushed
+    bool data_buffered = false;
+
+    int64_t node_count() const {
+        return format == CRT_FORMAT_STACKLESS ? int64_t(stackless.size()) : int64_t(stack.size());
+    }
+    void free_device() {
+        if (d_nodes) (void)hipFree(d_nodes);
+        if (d_tris) (void)hipFree(d_tris);
+        if (d_verts) (void)hipFree(d_verts);
+        if (d_entities) (void)hipFree(d_entities);
+        d_nodes = d_tris = d_verts = d_entities = nullptr;
+    }
+};
+
+namespace {
+
+// 4x4 inverse (column-major) by cofactor expansion in double, rounded once.
+bool invert4(const float* m, float* out) {
+    double a[16];
+    for (int i = 0; i < 16; ++i) a[i] = m[i];
+    double inv[16];
+    inv[0] = a[5] * a[10] * a[15] - a[5] * a[11] * a[14] - a[9] * a[6] * a[15] + a[9] * a[7] * a[14] +
+             a[13] * a[6] * a[11] - a[13] * a[7] * a[10];
+    inv[4] = -a[4] * a[10] * a[15] + a[4] * a[11] * a[14] + a[8] * a[6] * a[15] - a[8] * a[7] * a[14] -
+             a[12] * a[6] * a[11] + a[12] * a[7] * a[10];
+    inv[8] = a[4] * a[9] * a[15] - a[4] * a[11] * a[13] - a[8] * a[5] * a[15] + a[8] * a[7] * a[13] +
+             a[12] * a[5] * a[11] - a[12] * a[7] * a[9];
+    inv[12] = -a[4] * a[9] * a[14] + a[4] * a[10] * a[13] + a[8] * a[5] * a[14] - a[8] * a[6] * a[13] -
+              a[12] * a[5] * a[10] + a[12] * a[6] * a[9];
+    inv[1] = -a[1] * a[10] * a[15] + a[1] * a[11] * a[14] + a[9] * a[2] * a[15] - a[9] * a[3] * a[14] -
+             a[13] * a[2] * a[11] + a[13] * a[3] * a[10];
+    inv[5] = a[0] * a[10] * a[15] - a[0] * a[11] * a[14] - a[8] * a[2] * a[15] + a[8] * a[3] * a[14] +
+             a[12] * a[2] * a[11] - a[12] * a[3] * a[10];
+    inv[9] = -a[0] * a[9] * a[15] + a[0] * a[11] * a[13] + a[8] * a[1] * a[15] - a[8] * a[3] * a[13] -
+             a[12] * a[1] * a[11] + a[12] * a[3] * a[9];
+    inv[13] = a[0] * a[9] * a[14] - a[0] * a[10] * a[13] - a[8] * a[1] * a[14] + a[8] * a[2] * a[13] +
+              a[12] * a[1] * a[10] - a[12] * a[2] * a[9];
+    inv[2] = a[1] * a[6] * a[15] - a[1] * a[7] * a[14] - a[5] * a[2] * a[15] + a[5] * a[3] * a[14] +
+             a[13] * a[2] * a[7] - a[13] * a[3] * a[6];
+    inv[6] = -a[0] * a[6] * a[15] + a[0] * a[7] * a[14] + a[4] * a[2] * a[15] - a[4] * a[3] * a[14] -
+             a[12] * a[2] * a[7] + a[12] * a[3] * a[6];
+    inv[10] = a[0] * a[5] * a[15] - a[0] * a[7] * a[13] - a[4] * a[1] * a[15] + a[4] * a[3] * a[13] +
+              a[12] * a[1] * a[7] - a[12] * a[3] * a[5];
+    inv[14] = -a[0] * a[5] * a[14] + a[0] * a[6] * a[13] + a[4] * a[1] * a[14] - a[4] * a[2] * a[13] -
+              a[12] * a[1] * a[6] + a[12] * a[2] * a[5];
+    inv[3] = -a[1] * a[6] * a[11] + a[1] * a[7] * a[10] + a[5] * a[2] * a[11] - a[5] * a[3] * a[10] -
+             a[9] * a[2] * a[7] + a[9] * a[3] * a[6];
+    inv[7] = a[0] * a[6] * a[11] - a[0] * a[7] * a[10] - a[4] * a[2] * a[11] + a[4] * a[3] * a[10] +
+             a[8] * a[2] * a[7] - a[8] * a[3] * a[6];
+    inv[11] = -a[0] * a[5] * a[11] + a[0] * a[7] * a[9] + a[4] * a[1] * a[11] - a[4] * a[3] * a[9] -
+              a[8] * a[1] * a[7] + a[8] * a[3] * a[5];
+    inv[15] = a[0] * a[5] * a[10] - a[0] * a[6] * a[9] - a[4] * a[1] * a[10] + a[4] * a[2] * a[9] +
+              a[8] * a[1] * a[6] - a[8] * a[2] * a[5];
+    const double det = a[0] * inv[0] + a[1] * inv[4] + a[2] * inv[8] + a[3] * inv[12];
+    if (det == 0.0 || !std::isfinite(det)) return false;
+    for (int i = 0; i < 16; ++i) out[i] = float(inv[i] / det);
+    return true;
+}
+
+template <typename T>
+int upload(void*& dst, const std::vector<T>& src) {
+    if (dst) {
+        (void)hipFree(dst);
+        dst = nullptr;
+    }
+    const size_t bytes = std::max<size_t>(sizeof(T), src.size() * sizeof(T));
+    CRT_HIP(hipMalloc(&dst, bytes), "hipMalloc");
+    if (!src.empty()) CRT_HIP(hipMemcpy(dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice), "hipMemcpy");
+    return CRT_OK;
+}
+
+// Appends one object's buffers with the offset rules of AddObject
+// (Intersector.h:178-197).
+int append_object(crt_context* ctx, int32_t object_id, const void* nodes, int64_t node_count,
+                  const crt_triangle* tris, int64_t tri_count, const crt_vertex* verts, int64_t vert_count) {
+    ObjectData od;
+    od.node_offset = ctx->node_count();
+    od.triangle_offset = int64_t(ctx->triangles.size());
+    od.vertices_offset = int64_t(ctx->vertices.size());
+    od.node_count = node_count;
+    if (ctx->format == CRT_FORMAT_STACKLESS) {
+        const auto* n = static_cast<const crt_stackless_node*>(nodes);
+        ctx->stackless.insert(ctx->stackless.end(), n, n + node_count);
+    } else {
+        const auto* n = static_cast<const crt_stack_node*>(nodes);
+        ctx->stack.insert(ctx->stack.end(), n, n + node_count);
+    }
+    ctx->vertices.insert(ctx->vertices.end(), verts, verts + vert_count);
+    ctx->triangles.reserve(ctx->triangles.size() + size_t(tri_count));
+    for (int64_t i = 0; i < tri_count; ++i) {
+        crt_triangle t = tris[i];
+        t.packed[0] += int32_t(ctx->index_offset);
+        t.packed[1] += int32_t(ctx->index_offset);
+        t.packed[2] += int32_t(ctx->index_offset);
+        ctx->triangles.push_back(t);
+    }
+    ctx->index_offset += uint32_t(vert_count);
+    ctx->objects[object_id] = od;
+    ctx->data_buffered = false;
+    return CRT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* crt_last_error(void) { return crt::g_last_error.c_str(); }
+int crt_abi_version(void) { return CRT_ABI_VERSION; }
+
+int crt_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int crt_device_alloc(void** ptr, size_t bytes) {
+    if (!ptr) return crt::set_error(CRT_E_INVALID, "crt_device_alloc: ptr is null");
+    *ptr = nullptr;
+    if (bytes == 0) bytes = 1;
+    hipError_t e = hipMalloc(ptr, bytes);
+    if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation)
+        return crt::set_error(CRT_E_NOMEM, "crt_device_alloc: out of device memory");
+    if (e != hipSuccess) return crt::hip_fail(e, "crt_device_alloc");
+    return CRT_OK;
+}
+
+int crt_device_free(void* ptr) {
+    if (!ptr) return CRT_OK;
+    CRT_HIP(hipFree(ptr), "crt_device_free");
+    return CRT_OK;
+}
+
+int crt_memcpy_h2d(void* dst, const void* src, size_t bytes, void* stream) {
+    if (bytes == 0) return CRT_OK;
+    if (!dst || !src) return crt::set_error(CRT_E_INVALID, "crt_memcpy_h2d: null pointer");
+    CRT_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, static_cast<hipStream_t>(stream)), "hipMemcpyAsync");
+    CRT_HIP(hipStreamSynchronize(static_cast<hipStream_t>(stream)), "hipStreamSynchronize");
+    return CRT_OK;
+}
+
+int crt_memcpy_d2h(void* dst, const void* src, size_t bytes, void* stream) {
+    if (bytes == 0) return CRT_OK;
+    if (!dst || !src) return crt::set_error(CRT_E_INVALID, "crt_memcpy_d2h: null pointer");
+    CRT_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, static_cast<hipStream_t>(stream)), "hipMemcpyAsync");
+    CRT_HIP(hipStreamSynchronize(static_cast<hipStream_t>(stream)), "hipStreamSynchronize");
+    return CRT_OK;
+}
+
+int crt_memset_d(void* dst, int value, size_t bytes, void* stream) {
+    if (bytes == 0) return CRT_OK;
+    if (!dst) return crt::set_error(CRT_E_INVALID, "crt_memset_d: null pointer");
+    CRT_HIP(hipMemsetAsync(dst, value, bytes, static_cast<hipStream_t>(stream)), "hipMemsetAsync");
+    return CRT_OK;
+}
+
+int crt_stream_create(void** stream) {
+    if (!stream) return crt::set_error(CRT_E_INVALID, "crt_stream_create: null out");
+    hipStream_t s = nullptr;
+    CRT_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreateWithFlags");
+    *stream = s;
+    return CRT_OK;
+}
+
+int crt_stream_destroy(void* stream) {
+    if (!stream) return CRT_OK;
+    CRT_HIP(hipStreamDestroy(static_cast<hipStream_t>(stream)), "hipStreamDestroy");
+    return CRT_OK;
+}
+
+int crt_stream_sync(void* stream) {
+    CRT_HIP(hipStreamSynchronize(static_cast<hipStream_t>(stream)), "hipStreamSynchronize");
+    return CRT_OK;
+}
+
+int crt_device_sync(void) {
+    CRT_HIP(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    return CRT_OK;
+}
+
+int crt_event_create(void** event) {
+    if (!event) return crt::set_error(CRT_E_INVALID, "crt_event_create: null out");
+    hipEvent_t e = nullptr;
+    CRT_HIP(hipEventCreate(&e), "hipEventCreate");
+    *event = e;
+    return CRT_OK;
+}
+
+int crt_event_destroy(void* event) {
+    if (!event) return CRT_OK;
+    CRT_HIP(hipEventDestroy(static_cast<hipEvent_t>(event)), "hipEventDestroy");
+    return CRT_OK;
+}
+
+int crt_event_record(void* event, void* stream) {
+    if (!event) return crt::set_error(CRT_E_INVALID, "crt_event_record: null event");
+    CRT_HIP(hipEventRecord(static_cast<hipEvent_t>(event), static_cast<hipStream_t>(stream)), "hipEventRecord");
+    return CRT_OK;
+}
+
+int crt_event_elapsed_ms(void* start, void* end, float* ms) {
+    if (!start || !end || !ms) return crt::set_error(CRT_E_INVALID, "crt_event_elapsed_ms: null argument");
+    CRT_HIP(hipEventSynchronize(static_cast<hipEvent_t>(end)), "hipEventSynchronize");
+    CRT_HIP(hipEventElapsedTime(ms, static_cast<hipEvent_t>(start), static_cast<hipEvent_t>(end)),
+            "hipEventElapsedTime");
+    return CRT_OK;
+}
+
+int crt_trace(const crt_scene_view* scene, int query, uint32_t flags, const crt_ray* rays, crt_hit* hits,
+              int64_t nrays, const crt_trace_config* config, void* stream) {
+    return crt::trace(scene, query, flags, rays, hits, nrays, config, stream);
+}
+
+// --- context ----------------------------------------------------------------
+
+int crt_create(int format, crt_context** out) {
+    if (!out) return crt::set_error(CRT_E_INVALID, "crt_create: out is null");
+    *out = nullptr;
+    if (format != CRT_FORMAT_STACKLESS && format != CRT_FORMAT_STACK)
+        return crt::set_error(CRT_E_INVALID, "crt_create: unknown node format");
+    crt_context* c = new (std::nothrow) crt_context();
+    if (!c) return crt::set_error(CRT_E_NOMEM, "crt_create: out of host memory");
+    c->format = format;
+    *out = c;
+    return CRT_OK;
+}
+
+void crt_destroy(crt_context* ctx) {
+    if (!ctx) return;
+    ctx->free_device();
+    delete ctx;
+}
+
+int crt_context_format(const crt_context* ctx) { return ctx ? ctx->format : CRT_E_INVALID; }
+
+int crt_add_object(crt_context* ctx, int32_t object_id, const crt_mesh* meshes, int mesh_count,
+                   const crt_build_options* options) {
+    if (!ctx) return crt::set_error(CRT_E_INVALID, "crt_add_object: ctx is null");
+    crt::BvhResult r;
+    // t_offset = triangles already added (Intersector.h:183).
+    int rc = crt::build_bvh(ctx->format, meshes, mesh_count, int32_t(ctx->triangles.size()), options, r);
+    if (rc) return rc;
+    const void* nodes = ctx->format == CRT_FORMAT_STACKLESS ? static_cast<const void*>(r.stackless.data())
+                                                            : static_cast<const void*>(r.stack.data());
+    return append_object(ctx, object_id, nodes, int64_t(r.stats.array_length), r.triangles.data(),
+                         int64_t(r.triangles.size()), r.vertices.data(), int64_t(r.vertices.size()));
+}
+
+int crt_add_object_prebuilt(crt_context* ctx, int32_t object_id, const void* nodes, int64_t node_count,
+                            const crt_triangle* triangles, int64_t triangle_count, const crt_vertex* vertices,
+                            int64_t vertex_count) {
+    if (!ctx) return crt::set_error(CRT_E_INVALID, "crt_add_object_prebuilt: ctx is null");
+    if (node_count <= 0 || !nodes || triangle_count < 0 || (triangle_count && !triangles) || vertex_count < 0 ||
+        (vertex_count && !vertices))
+        return crt::set_error(CRT_E_INVALID, "crt_add_object_prebuilt: bad buffers");
+    return append_object(ctx, object_id, nodes, node_count, triangles, triangle_count, vertices, vertex_count);
+}
+
+int crt_push_entity(crt_context* ctx, int32_t object_id, const float model[16], float emissive, float translucency) {
+    if (!ctx || !model) return crt::set_error(CRT_E_INVALID, "crt_push_entity: null argument");
+    auto it = ctx->objects.find(object_id);
+    if (it == ctx->objects.end())
+        return crt::set_error(CRT_E_INVALID,
+                              "Trying to push entity whose parent object hasn't been added to global BVH");
+    crt_entity e;
+    std::memset(&e, 0, sizeof(e));
+    std::memcpy(e.model, model, sizeof(e.model));
+    if (!invert4(model, e.inverse)) return crt::set_error(CRT_E_INVALID, "crt_push_entity: singular model matrix");
+    e.node_offset = int32_t(it->second.node_offset);
+    e.node_count = int32_t(it->second.node_count);
+    const float alpha = 1.0f - translucency;
+    std::memcpy(&e.data[0], &emissive, 4);
+    std::memcpy(&e.data[1], &alpha, 4);
+    ctx->pending.push_back(e);
+    return CRT_OK;
+}
+
+int crt_buffer_entities(crt_context* ctx) {
+    if (!ctx) return crt::set_error(CRT_E_INVALID, "crt_buffer_entities: ctx is null");
+    int rc = upload(ctx->d_entities, ctx->pending);
+    if (rc) return rc;
+    ctx->entities_buffered = int64_t(ctx->pending.size());
+    ctx->entities = ctx->pending;
+    ctx->pending.clear();
+    return CRT_OK;
+}
+
+int crt_buffer_data(crt_context* ctx, int clear_cpu_data) {
+    if (!ctx) return crt::set_error(CRT_E_INVALID, "crt_buffer_data: ctx is null");
+    int rc = upload(ctx->d_tris, ctx->triangles);
+    if (!rc) rc = ctx->format == CRT_FORMAT_STACKLESS ? upload(ctx->d_nodes, ctx->stackless) : upload(ctx->d_nodes, ctx->stack);
+    if (!rc) rc = upload(ctx->d_verts, ctx->vertices);
+    if (rc) return rc;
+    ctx->nodes_buffered = ctx->node_count();
+    ctx->data_buffered = true;
+    if (clear_cpu_data) {
+        ctx->stackless.clear();
+        ctx->stack.clear();
+        ctx->vertices.clear();
+        ctx->triangles.clear();
+    }
+    return CRT_OK;
+}
+
+int crt_context_host_buffers(const crt_context* ctx, const void** nodes, int64_t* node_count,
+                             const crt_triangle** tris, int64_t* tri_count, const crt_vertex** verts,
+                             int64_t* vert_count, const crt_entity** entities, int64_t* entity_count) {
+    if (!ctx) return crt::set_error(CRT_E_INVALID, "crt_context_host_buffers: ctx is null");
+    if (nodes)
+        *nodes = ctx->format == CRT_FORMAT_STACKLESS ? static_cast<const void*>(ctx->stackless.data())
+                                                     : static_cast<const void*>(ctx->stack.data());
+    if (node_count) *node_count = ctx->node_count();
+    if (tris) *tris = ctx->triangles.data();
+    if (tri_count) *tri_count = int64_t(ctx->triangles.size());
+    if (verts) *verts = ctx->vertices.data();
+    if (vert_count) *vert_count = int64_t(ctx->vertices.size());
+    if (entities) *entities = ctx->entities.data();
+    if (entity_count) *entity_count = int64_t(ctx->entities.size());
+    return CRT_OK;
+}
+
+int crt_context_scene(const crt_context* ctx, crt_scene_view* out) {
+    if (!ctx || !out) return crt::set_error(CRT_E_INVALID, "crt_context_scene: null argument");
+    if (!ctx->data_buffered || !ctx->d_entities)
+        return crt::set_error(CRT_E_STATE, "crt_context_scene: call crt_buffer_data and crt_buffer_entities first");
+    out->format = ctx->format;
+    out->entity_count = int32_t(ctx->entities_buffered);
+    out->total_nodes = ctx->nodes_buffered;
+    out->nodes = ctx->d_nodes;
+    out->triangles = static_cast<const crt_triangle*>(ctx->d_tris);
+    out->vertices = static_cast<const crt_vertex*>(ctx->d_verts);
+    out->entities = static_cast<const crt_entity*>(ctx->d_entities);
+    return CRT_OK;
+}
+
+int crt_intersect(crt_context* ctx, int query, uint32_t flags, const crt_ray* rays, crt_hit* hits, int64_t nrays,
+                  void* stream) {
+    crt_scene_view s;
+    int rc = crt_context_scene(ctx, &s);
+    if (rc) return rc;
+    return crt::trace(&s, query, flags, rays, hits, nrays, nullptr, stream);
+}
+
+int crt_intersect_primary(crt_context* ctx, crt_ray* rays, crt_hit* hits, int width, int height,
+                          const float inv_view[16], const float inv_proj[16], uint32_t flags, void* stream) {
+    crt_scene_view s;
+    int rc = crt_context_scene(ctx, &s);
+    if (rc) return rc;
+    if (width <= 0 || height <= 0 || !rays || !hits || !inv_view || !inv_proj)
+        return crt::set_error(CRT_E_INVALID, "crt_intersect_primary: bad arguments");
+    crt::Camera cam;
+    std::memcpy(cam.inv_view, inv_view, sizeof(cam.inv_view));
+    std::memcpy(cam.inv_proj, inv_proj, sizeof(cam.inv_proj));
+    cam.tmax = 1000000.0f;  // IntersectScene's TMax (TraverseBVHStackless.glsl:309)
+    crt::SceneArgs a{s.format, s.entity_count, s.total_nodes, s.nodes, s.triangles, s.vertices, s.entities};
+    CRT_HIP(crt::launch_primary(a, cam, rays, hits, width, height, flags, crt::kMaxIterations, 256,
+                                static_cast<hipStream_t>(stream)),
+            "crt_intersect_primary: kernel launch");
+    return CRT_OK;
+}
+
+}  // extern "C"

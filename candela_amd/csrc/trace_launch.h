@@ -1,0 +1,39 @@
+// Kernel arguments and launchers shared by the host runtime and trace.hip.
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <cstdint>
+
+#include "candela_rt.h"
+
+namespace crt {
+
+// Device scene passed by value to every kernel (SSBO bindings 16..19 plus
+// u_EntityCount / u_TotalNodes of the reference's traversal include).
+struct SceneArgs {
+    int32_t format;
+    int32_t entity_count;
+    int64_t total_nodes;
+    const void* nodes;
+    const crt_triangle* tris;
+    const crt_vertex* verts;
+    const crt_entity* entities;
+};
+
+struct Camera {
+    float inv_view[16];
+    float inv_proj[16];
+    float tmax;
+};
+
+constexpr int32_t kMaxIterations = 1024;  // TraverseBVHStackless.glsl:184
+
+hipError_t launch_trace(const SceneArgs& s, int query, const crt_ray* rays, crt_hit* hits, int64_t n,
+                        uint32_t flags, int32_t max_iter, int block, int lds_depth, uint32_t* tri_tests,
+                        hipStream_t stream);
+
+hipError_t launch_primary(const SceneArgs& s, const Camera& cam, crt_ray* rays, crt_hit* hits, int width, int height,
+                          uint32_t flags, int32_t max_iter, int block, hipStream_t stream);
+
+}  // namespace crt

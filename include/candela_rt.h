@@ -1,0 +1,315 @@
+/*
+ * candela_rt.h -- C ABI of the MI355X ray-traversal engine for Candela.
+ *
+ * Everything that crosses this boundary is a plain C type: fixed-layout structs,
+ * pointers and sizes.  Device pointers are plain `void*` / typed pointers that were
+ * obtained from crt_device_alloc() (or any HIP allocation of the same process).
+ * Streams are passed as `void*` holding a hipStream_t (NULL = the null stream).
+ *
+ * Every function returns CRT_OK (0) or a negative CRT_E_* code; crt_last_error()
+ * gives the message of the last failure on the calling thread.  No C++ exception
+ * crosses the ABI.
+ *
+ * Reference interfaces replaced (paths relative to the Candela source tree):
+ *   buffer layouts ......... Source/Core/Utils/Vertex.h:7-12,
+ *                            Source/Core/BVH/BVHConstructor.h:61-84,
+ *                            Source/Core/BVH/Intersector.h:43-49
+ *   CPU constructor ........ Source/Core/BVH/BVHConstructor.cpp:951-1108 (BuildBVH x2)
+ *   host Ray Intersection API Source/Core/BVH/Intersector.h:60-124 (RayIntersector<T>)
+ *   device Ray Intersection API
+ *                            Source/Core/Shaders/Intersectors/Include/TraverseBVHStackless.glsl:175-579
+ *                            Source/Core/Shaders/Intersectors/Include/TraverseBVHStack.glsl:168-681
+ */
+#ifndef CANDELA_RT_H
+#define CANDELA_RT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CRT_ABI_VERSION 1
+
+/* ------------------------------------------------------------------------- */
+/* Status codes                                                              */
+/* ------------------------------------------------------------------------- */
+enum {
+    CRT_OK = 0,
+    CRT_E_INVALID = -1,   /* bad argument (null pointer, out-of-range enum, size) */
+    CRT_E_HIP = -2,       /* a HIP runtime call failed (message names it)         */
+    CRT_E_NOMEM = -3,     /* host or device allocation failed                     */
+    CRT_E_STATE = -4,     /* call out of order (e.g. trace before buffer_data)    */
+    CRT_E_NODEVICE = -5   /* no HIP device visible                                */
+};
+
+/* ------------------------------------------------------------------------- */
+/* Buffer layouts -- byte-identical to the reference's std430 SSBO contents    */
+/* ------------------------------------------------------------------------- */
+
+/* Vertex.h:7-12 -- vec4 position; uvec3 normal_tangent_data; uint texcoords (32 B).
+ * Only position.xyz is read by traversal. */
+typedef struct crt_vertex {
+    float position[4];
+    uint32_t normal_tangent_data[3];
+    uint32_t texcoords;
+} crt_vertex;
+
+/* BVHConstructor.h:79-84 -- {vertex index 0,1,2 ; mesh id} (16 B).  The
+ * triangle's position in the flattened array is its "triangle id". */
+typedef struct crt_triangle {
+    int32_t packed[4];
+} crt_triangle;
+
+/* BVHConstructor.h:66-70 -- stackless node (32 B), pre-order.
+ *   min.w : int bits; -1 for an inner node, (start<<4 | count) for a leaf
+ *   max.w : int bits; object-local index of the escape ("miss") node, -1 = end */
+typedef struct crt_stackless_node {
+    float min[4];
+    float max[4];
+} crt_stackless_node;
+
+/* BVHConstructor.h:61-64 */
+typedef struct crt_bounds {
+    float min[4];
+    float max[4];
+} crt_bounds;
+
+/* BVHConstructor.h:72-77 -- stack node (64 B), BFS order.  Entry k holds both
+ * children of inner node k:
+ *   X.min.w : int bits; -1 if child X is inner, (start<<4 | count) if a leaf
+ *   X.max.w : int bits; object-local entry index of child X (inner children) */
+typedef struct crt_stack_node {
+    crt_bounds left;
+    crt_bounds right;
+} crt_stack_node;
+
+/* Intersector.h:43-49 -- per-instance record (192 B).  Matrices are column-major
+ * (glm / GLSL).  data[0] = float bits of emissive amount, data[1] = float bits
+ * of alpha = 1 - translucency. */
+typedef struct crt_entity {
+    float model[16];
+    float inverse[16];
+    int32_t node_offset;
+    int32_t node_count;
+    int32_t data[14];
+} crt_entity;
+
+/* Ray record (32 B).  tmax is the TMax the reference's scene loop starts from
+ * (1e6 in the reference, TraverseBVHStackless.glsl:284).  `id` is not read by
+ * traversal; callers may keep a pixel index there. */
+typedef struct crt_ray {
+    float org[3];
+    float tmax;
+    float dir[3];
+    uint32_t id;
+} crt_ray;
+
+/* Hit record (32 B): the reference's TUVW + Mesh + TriangleIdx + Entity outs
+ * (TraverseBVHStackless.glsl:280-319).
+ *   closest-hit : t,u,v,w = TUVW (u<->vertex A, v<->B, w<->C); all -1 on a miss
+ *   any-hit     : t = first accepted distance or -1; u=v=w=-1
+ *   status      : bits 0-23 node visits summed over entities (saturating),
+ *                 bits 24-31 CRT_STATUS_* flags */
+typedef struct crt_hit {
+    float t, u, v, w;
+    int32_t tri;
+    int32_t mesh;
+    int32_t entity;
+    uint32_t status;
+} crt_hit;
+
+enum { CRT_FORMAT_STACKLESS = 0, CRT_FORMAT_STACK = 1 };
+enum { CRT_QUERY_CLOSEST = 0, CRT_QUERY_ANY = 1 };
+
+/* Trace flags */
+enum {
+    /* IntersectSceneIgnoreTransparent: skip entities whose alpha < 0.99
+     * (TraverseBVHStackless.glsl:321-349) */
+    CRT_FLAG_IGNORE_TRANSPARENT = 1u << 0,
+    /* Reproduce the scene-level test `TriangleIdx > 0`
+     * (TraverseBVHStackless.glsl:304): a closest hit on flattened triangle 0 is
+     * reported with t=u=v=w=-1 (tri/mesh/entity still set). */
+    CRT_FLAG_TRI0_IS_MISS = 1u << 1
+};
+
+/* Per-ray status flags (hit.status bits 24-31) */
+enum {
+    CRT_STATUS_ITER_CAP = 1u << 24,       /* an entity walk stopped at 1024 iterations  */
+    CRT_STATUS_STACK_OVERFLOW = 1u << 25, /* stack walk aborted at 63 pushed entries    */
+    CRT_STATUS_BAD_INDEX = 1u << 26       /* node index left the entity's range         */
+};
+#define CRT_STATUS_VISITS_MASK 0x00FFFFFFu
+
+/* ------------------------------------------------------------------------- */
+/* Library                                                                   */
+/* ------------------------------------------------------------------------- */
+const char* crt_last_error(void);
+int crt_abi_version(void);
+/* Number of visible HIP devices (0 without a GPU; never fails). */
+int crt_device_count(void);
+
+/* ------------------------------------------------------------------------- */
+/* CPU SAH constructor -- BVHConstructor.cpp:951-1108                        */
+/* ------------------------------------------------------------------------- */
+
+/* One mesh of an Object (Object.h:62, Mesh.h:30-31,58). */
+typedef struct crt_mesh {
+    const crt_vertex* vertices;
+    uint32_t vertex_count;
+    const uint32_t* indices;
+    uint32_t index_count; /* multiple of 3 */
+    int32_t global_mesh_number;
+} crt_mesh;
+
+/* Child order of the stackless build.  The reference swaps the two children of
+ * every split with p = 1/2 from mt19937(random_device())
+ * (BVHConstructor.cpp:389-391,599-609).  SEEDED draws the same sequence from
+ * mt19937(seed); NONE keeps the split order. */
+enum { CRT_SWAP_NONE = 0, CRT_SWAP_SEEDED = 1 };
+
+typedef struct crt_build_options {
+    int32_t child_swap;  /* CRT_SWAP_* (stackless format only) */
+    uint32_t seed;       /* mt19937 seed for CRT_SWAP_SEEDED   */
+    int32_t threads;     /* worker threads; 0 = hardware concurrency */
+} crt_build_options;
+
+typedef struct crt_bvh_stats {
+    uint64_t triangle_count;
+    uint64_t node_count;     /* LastNodeIndex: tree nodes excluding the root */
+    uint64_t leaf_count;
+    uint64_t split_fails;
+    uint32_t max_depth;      /* max size of the build stack (MaxBVHDepth) */
+    uint64_t array_length;   /* flattened node array length = node_count + 1 */
+    uint64_t vertex_count;
+} crt_bvh_stats;
+
+typedef struct crt_bvh crt_bvh;
+
+/* BuildBVH(object, nodes, vertices, tris, t_offset): concatenates the meshes,
+ * builds the binned-SAH tree and flattens it into `format`.  Fails (rather than
+ * crashing) where the reference would dereference a leaf root in the stack
+ * flattener (BVHConstructor.cpp:873): a stack-format object needs >= 3
+ * triangles. */
+int crt_build_bvh(int format, const crt_mesh* meshes, int mesh_count, int32_t t_offset,
+                  const crt_build_options* options, crt_bvh** out);
+void crt_bvh_free(crt_bvh* bvh);
+int crt_bvh_get_stats(const crt_bvh* bvh, crt_bvh_stats* out);
+/* Node array: crt_stackless_node[array_length] or crt_stack_node[array_length]. */
+const void* crt_bvh_nodes(const crt_bvh* bvh);
+const crt_triangle* crt_bvh_triangles(const crt_bvh* bvh);
+const crt_vertex* crt_bvh_vertices(const crt_bvh* bvh);
+int crt_bvh_format(const crt_bvh* bvh);
+
+/* ------------------------------------------------------------------------- */
+/* Device memory (thin wrappers so callers need no HIP headers)              */
+/* ------------------------------------------------------------------------- */
+int crt_device_alloc(void** ptr, size_t bytes);
+int crt_device_free(void* ptr);
+int crt_memcpy_h2d(void* dst, const void* src, size_t bytes, void* stream);
+int crt_memcpy_d2h(void* dst, const void* src, size_t bytes, void* stream);
+int crt_memset_d(void* dst, int value, size_t bytes, void* stream);
+int crt_stream_create(void** stream);
+int crt_stream_destroy(void* stream);
+int crt_stream_sync(void* stream);
+int crt_device_sync(void);
+/* HIP events for timing work on a stream. */
+int crt_event_create(void** event);
+int crt_event_destroy(void* event);
+int crt_event_record(void* event, void* stream);
+/* Waits for `end` and returns the milliseconds between the two events. */
+int crt_event_elapsed_ms(void* start, void* end, float* ms);
+
+/* ------------------------------------------------------------------------- */
+/* Raw device scene + trace (SSBO bindings 16..19, Intersector.h:275-281)    */
+/* ------------------------------------------------------------------------- */
+
+/* Device-resident scene: the four buffers the traversal include binds plus
+ * u_EntityCount / u_TotalNodes. */
+typedef struct crt_scene_view {
+    int32_t format;              /* CRT_FORMAT_*                              */
+    int32_t entity_count;        /* u_EntityCount                             */
+    int64_t total_nodes;         /* u_TotalNodes (entries of `nodes`)         */
+    const void* nodes;           /* crt_stackless_node[] or crt_stack_node[]  */
+    const crt_triangle* triangles;
+    const crt_vertex* vertices;
+    const crt_entity* entities;
+} crt_scene_view;
+
+/* Launch shape of the one-ray-per-lane kernels. */
+typedef struct crt_trace_config {
+    int32_t block_size;    /* threads per workgroup: 64..1024, multiple of 64; 0 = 256 */
+    int32_t lds_stack;     /* stack format: entries kept in LDS, one of 8, 16, 32, 48,
+                              64 (0 = 16); deeper entries live in per-lane scratch */
+    uint32_t* tri_tests;   /* optional device array [nrays]: the counting build of the
+                              kernel writes the triangle tests of each ray (node
+                              visits are in hit.status); NULL = no counting */
+} crt_trace_config;
+
+/* IntersectScene / IntersectSceneIgnoreTransparent (closest) or the occlusion
+ * IntersectScene (any) for `nrays` rays; all pointers are device pointers.
+ * Asynchronous on `stream`. */
+int crt_trace(const crt_scene_view* scene, int query, uint32_t flags,
+              const crt_ray* rays, crt_hit* hits, int64_t nrays,
+              const crt_trace_config* config, void* stream);
+
+/* ------------------------------------------------------------------------- */
+/* RayIntersector<T> -- Intersector.h:60-351                                  */
+/* ------------------------------------------------------------------------- */
+typedef struct crt_context crt_context;
+
+/* RayIntersector<T>() + Initialize(); format picks T. */
+int crt_create(int format, crt_context** out);
+void crt_destroy(crt_context* ctx);
+int crt_context_format(const crt_context* ctx);
+
+/* AddObject (Intersector.h:169-198): builds the object's BVH on the CPU with
+ * t_offset = triangles already added, appends nodes (object-local links),
+ * triangles (vertex indices + running vertex offset) and vertices. */
+int crt_add_object(crt_context* ctx, int32_t object_id, const crt_mesh* meshes, int mesh_count,
+                   const crt_build_options* options);
+/* AddObject for buffers that were already built (e.g. by the reference
+ * constructor): same offset rules, no build. */
+int crt_add_object_prebuilt(crt_context* ctx, int32_t object_id, const void* nodes,
+                            int64_t node_count, const crt_triangle* triangles,
+                            int64_t triangle_count, const crt_vertex* vertices,
+                            int64_t vertex_count);
+/* PushEntity (Intersector.h:200-216): model matrix (column-major), emissive,
+ * translucency; inverse is computed here. */
+int crt_push_entity(crt_context* ctx, int32_t object_id, const float model[16], float emissive,
+                    float translucency);
+/* BufferEntities (Intersector.h:226-239) */
+int crt_buffer_entities(crt_context* ctx);
+/* BufferData (Intersector.h:321-351) */
+int crt_buffer_data(crt_context* ctx, int clear_cpu_data);
+
+/* Host copies (the public m_BVHNodes / m_BVHVertices / m_BVHTriangles /
+ * m_BVHEntities members).  Sizes in elements; pointers valid until the next
+ * mutating call. */
+int crt_context_host_buffers(const crt_context* ctx, const void** nodes, int64_t* node_count,
+                             const crt_triangle** tris, int64_t* tri_count,
+                             const crt_vertex** verts, int64_t* vert_count,
+                             const crt_entity** entities, int64_t* entity_count);
+/* Device scene of the context (valid after buffer_data + buffer_entities). */
+int crt_context_scene(const crt_context* ctx, crt_scene_view* out);
+
+/* IntersectRay / IntersectRayIgnoreTransparent / IntersectRay(ro,rd) over a
+ * device ray buffer. */
+int crt_intersect(crt_context* ctx, int query, uint32_t flags, const crt_ray* rays, crt_hit* hits,
+                  int64_t nrays, void* stream);
+
+/* IntersectPrimary (Intersector.h:241-266 + TraverseBVHStackless.glsl:371-388):
+ * one pinhole ray per pixel, pixel-corner uv = pixel/dims, written to
+ * `rays` (width*height device records, row-major) and traced closest-hit into
+ * `hits`.  inv_view / inv_proj are column-major.  Unlike the reference
+ * dispatch (floor(W/16) x floor(H/16) groups), every pixel is covered. */
+int crt_intersect_primary(crt_context* ctx, crt_ray* rays, crt_hit* hits, int width, int height,
+                          const float inv_view[16], const float inv_proj[16], uint32_t flags,
+                          void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CANDELA_RT_H */

@@ -1,0 +1,169 @@
+"""Python access to the checkers in oracle/ (TEST INFRASTRUCTURE ONLY).
+
+  liboracle.so   C restatement of the reference's GLSL traversal (oracle_trace.c)
+  _ref/ref_bvh   the reference's own BVHConstructor.cpp (built from the
+                 reference tree by oracle/Makefile when that tree is present)
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
+this module; libcandela_rt never does.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import struct
+import subprocess
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_LIB = os.path.join(HERE, "liboracle.so")
+REF_BVH = os.path.join(HERE, "_ref", "ref_bvh")
+
+VERTEX_DTYPE = np.dtype([("position", "<f4", 4), ("normal_tangent_data", "<u4", 3), ("texcoords", "<u4")])
+TRIANGLE_DTYPE = np.dtype([("packed", "<i4", 4)])
+STACKLESS_NODE_DTYPE = np.dtype([("min", "<f4", 4), ("max", "<f4", 4)])
+BOUNDS_DTYPE = np.dtype([("min", "<f4", 4), ("max", "<f4", 4)])
+STACK_NODE_DTYPE = np.dtype([("left", BOUNDS_DTYPE), ("right", BOUNDS_DTYPE)])
+ENTITY_DTYPE = np.dtype([("model", "<f4", 16), ("inverse", "<f4", 16), ("node_offset", "<i4"),
+                         ("node_count", "<i4"), ("data", "<i4", 14)])
+RAY_DTYPE = np.dtype([("org", "<f4", 3), ("tmax", "<f4"), ("dir", "<f4", 3), ("id", "<u4")])
+HIT_DTYPE = np.dtype([("t", "<f4"), ("u", "<f4"), ("v", "<f4"), ("w", "<f4"), ("tri", "<i4"),
+                      ("mesh", "<i4"), ("entity", "<i4"), ("status", "<u4")])
+
+_lib = None
+
+
+def available() -> bool:
+    return os.path.exists(ORACLE_LIB)
+
+
+def ref_available() -> bool:
+    return os.path.exists(REF_BVH)
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not available():
+            raise ImportError(f"{ORACLE_LIB} missing: run `make -C oracle`")
+        l = C.CDLL(ORACLE_LIB)
+        vp, i32, i64, u32, f32 = C.c_void_p, C.c_int32, C.c_int64, C.c_uint32, C.c_float
+        l.oracle_trace.restype = C.c_int
+        l.oracle_trace.argtypes = [i32, i32, i64, vp, vp, vp, vp, C.c_int, u32, vp, vp, i64, C.c_int, vp, i32]
+        l.oracle_bruteforce_closest.restype = C.c_int
+        l.oracle_bruteforce_closest.argtypes = [vp, vp, vp, i64, i64, vp, vp, vp, i64]
+        l.oracle_primary_rays.restype = None
+        l.oracle_primary_rays.argtypes = [C.c_int, C.c_int, vp, vp, f32, vp]
+        l.oracle_ray_triangle.restype = None
+        l.oracle_ray_triangle.argtypes = [vp, vp, vp, vp, vp, vp]
+        l.oracle_bounds_stackless.restype = f32
+        l.oracle_bounds_stackless.argtypes = [vp, vp, vp, vp, f32, f32]
+        l.oracle_bounds_stack.restype = f32
+        l.oracle_bounds_stack.argtypes = [vp, vp, vp, vp, f32]
+        _lib = l
+    return _lib
+
+
+def _p(a: np.ndarray) -> int:
+    return a.ctypes.data
+
+
+def trace(fmt: int, nodes: np.ndarray, tris: np.ndarray, verts: np.ndarray, ents: np.ndarray, query: int,
+          rays: np.ndarray, flags: int = 0, threads: int = 0, max_iter: int = 0, with_tests: bool = False):
+    """Host traversal of `rays` (RAY_DTYPE) -> hits (HIT_DTYPE) [, triangle tests per ray]."""
+    nodes = np.ascontiguousarray(nodes)
+    tris = np.ascontiguousarray(tris, TRIANGLE_DTYPE)
+    verts = np.ascontiguousarray(verts, VERTEX_DTYPE)
+    ents = np.ascontiguousarray(ents, ENTITY_DTYPE)
+    rays = np.ascontiguousarray(rays, RAY_DTYPE)
+    hits = np.empty(rays.size, HIT_DTYPE)
+    tests = np.zeros(rays.size, np.uint32) if with_tests else None
+    if threads <= 0:
+        threads = os.cpu_count() or 1
+    rc = lib().oracle_trace(fmt, ents.size, nodes.size, _p(nodes), _p(tris), _p(verts), _p(ents), query, flags,
+                            _p(rays), _p(hits), rays.size, threads, _p(tests) if tests is not None else None,
+                            max_iter)
+    if rc != 0:
+        raise RuntimeError("oracle_trace failed")
+    return (hits, tests) if with_tests else hits
+
+
+def bruteforce_closest(tris, verts, ent, rays, tri_begin: int = 0, tri_end: int | None = None):
+    tris = np.ascontiguousarray(tris, TRIANGLE_DTYPE)
+    verts = np.ascontiguousarray(verts, VERTEX_DTYPE)
+    ent = np.ascontiguousarray(np.atleast_1d(ent), ENTITY_DTYPE)
+    rays = np.ascontiguousarray(rays, RAY_DTYPE)
+    tri_end = tris.size if tri_end is None else tri_end
+    t = np.empty(rays.size, np.float32)
+    tri = np.empty(rays.size, np.int32)
+    lib().oracle_bruteforce_closest(_p(tris), _p(verts), _p(ent), tri_begin, tri_end, _p(rays), _p(t), _p(tri),
+                                    rays.size)
+    return t, tri
+
+
+def primary_rays(width: int, height: int, inv_view_colmajor: np.ndarray, inv_proj_colmajor: np.ndarray,
+                 tmax: float = 1e6) -> np.ndarray:
+    iv = np.ascontiguousarray(inv_view_colmajor, np.float32)
+    ip = np.ascontiguousarray(inv_proj_colmajor, np.float32)
+    rays = np.empty(width * height, RAY_DTYPE)
+    lib().oracle_primary_rays(width, height, _p(iv), _p(ip), tmax, _p(rays))
+    return rays
+
+
+def ray_triangle(ro, rd, a, b, c):
+    arrs = [np.ascontiguousarray(x, np.float32) for x in (ro, rd, a, b, c)]
+    out = np.empty(3, np.float32)
+    lib().oracle_ray_triangle(*[_p(x) for x in arrs], _p(out))
+    return out
+
+
+def bounds_stackless(bmin, bmax, ro, inv, t_min, t_max) -> float:
+    arrs = [np.ascontiguousarray(x, np.float32) for x in (bmin, bmax, ro, inv)]
+    return float(np.float32(lib().oracle_bounds_stackless(*[_p(x) for x in arrs], t_min, t_max)))
+
+
+def bounds_stack(bmin, bmax, ro, inv, maxt) -> float:
+    arrs = [np.ascontiguousarray(x, np.float32) for x in (bmin, bmax, ro, inv)]
+    return float(np.float32(lib().oracle_bounds_stack(*[_p(x) for x in arrs], maxt)))
+
+
+def identity_entity(node_count: int, node_offset: int = 0) -> np.ndarray:
+    e = np.zeros(1, ENTITY_DTYPE)
+    eye = np.eye(4, dtype=np.float32).T.reshape(16)
+    e["model"] = eye
+    e["inverse"] = eye
+    e["node_offset"] = node_offset
+    e["node_count"] = node_count
+    e["data"][0, 1] = np.float32(1.0).view(np.int32)
+    return e
+
+
+def ref_build(fmt: int, verts: np.ndarray, indices: np.ndarray, mesh_ids: np.ndarray):
+    """Runs the reference constructor (one process per build) -> (nodes, triangles, info)."""
+    if not ref_available():
+        raise FileNotFoundError(f"{REF_BVH} not built (needs the reference tree)")
+    verts = np.ascontiguousarray(verts, VERTEX_DTYPE)
+    indices = np.ascontiguousarray(indices, np.uint32).ravel()
+    mesh_ids = np.ascontiguousarray(mesh_ids, np.int32).ravel()
+    assert mesh_ids.size == indices.size // 3
+    with tempfile.TemporaryDirectory() as d:
+        fin, fout = os.path.join(d, "in.bin"), os.path.join(d, "out.bin")
+        with open(fin, "wb") as f:
+            f.write(struct.pack("<II", verts.size, indices.size))
+            f.write(verts.tobytes())
+            f.write(indices.tobytes())
+            f.write(mesh_ids.tobytes())
+        mode = "stackless" if fmt == 0 else "stack"
+        subprocess.run([REF_BVH, mode, fin, fout], check=True, stdout=subprocess.DEVNULL)
+        raw = open(fout, "rb").read()
+    hdr = struct.unpack("<6Q", raw[:48])
+    n_nodes, n_tri = hdr[0], hdr[1]
+    nd = STACKLESS_NODE_DTYPE if fmt == 0 else STACK_NODE_DTYPE
+    off = 48
+    nodes = np.frombuffer(raw, nd, n_nodes, off).copy()
+    off += n_nodes * nd.itemsize
+    tris = np.frombuffer(raw, TRIANGLE_DTYPE, n_tri, off).copy()
+    return nodes, tris, {"array_length": hdr[0], "triangle_count": hdr[1], "node_count": hdr[2],
+                         "leaf_count": hdr[3]}

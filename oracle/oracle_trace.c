@@ -1,0 +1,550 @@
+/*
+ * oracle_trace.c -- CPU restatement of Candela's GLSL ray traversal.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker that the HIP kernels of
+ * libcandela_rt are compared against.  Only tests/, __graft_entry__.smoke() and
+ * bench.py's cpu_baseline leg may load it; the product never calls it.
+ *
+ * Every function follows one GLSL function of the reference, operation for
+ * operation, in FP32 with no FMA contraction (build with -ffp-contract=off):
+ *   ray_triangle ........... Include/TraverseBVHStackless.glsl:79-97
+ *                            (identical: Include/TraverseBVHStack.glsl:87-105)
+ *   bounds_stackless ....... Include/TraverseBVHStackless.glsl:100-109
+ *   bounds_stack ........... Include/TraverseBVHStack.glsl:107-116
+ *   barycentrics ........... Include/TraverseBVHStackless.glsl:156-172
+ *   walk_stackless ......... Include/TraverseBVHStackless.glsl:175-278 (closest)
+ *                            Include/TraverseBVHStackless.glsl:463-556 (any)
+ *   walk_stack ............. Include/TraverseBVHStack.glsl:168-324 (closest)
+ *                            Include/TraverseBVHStack.glsl:509-657 (any)
+ *   scene_closest .......... Include/TraverseBVHStackless.glsl:280-366
+ *   scene_any .............. Include/TraverseBVHStackless.glsl:558-579,
+ *                            Include/TraverseBVHStack.glsl:659-681
+ * (shader paths relative to Source/Core/Shaders/Intersectors/)
+ *
+ * Semantics the GLSL leaves open, fixed here and in the HIP kernels alike:
+ *   - min/max of a NaN operand returns the other operand (IEEE minNum, what
+ *     fminf/fmaxf and AMD's v_min_f32/v_max_f32 do); NaN arises as 0*inf when
+ *     a ray origin lies on a slab plane of an axis-parallel ray;
+ *   - mat4*vec4 sums the four column terms left to right;
+ *   - the node-index guard breaks on index >= u_TotalNodes (the GLSL tests
+ *     `> u_TotalNodes`, which lets one read past the array).
+ *
+ * Pins (tests/test_oracle*.py): closed-form triangles and boxes, brute force
+ * over all triangles on small scenes, stack == stackless closest-t on the same
+ * tree, any-hit <=> closest-hit within tmax, crafted buffers for every guard.
+ */
+#include <math.h>
+#include <pthread.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../include/candela_rt.h"
+
+#define MAX_ITER_DEFAULT 1024
+#define STACK_SIZE 64
+
+typedef struct {
+    float x, y, z;
+} v3;
+
+static inline v3 mk(float x, float y, float z) {
+    v3 r = {x, y, z};
+    return r;
+}
+static inline v3 sub(v3 a, v3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+static inline v3 neg(v3 a) { return mk(-a.x, -a.y, -a.z); }
+static inline float dot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static inline v3 cross(v3 a, v3 b) {
+    return mk(a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y);
+}
+static inline float as_float(int32_t i) {
+    float f;
+    memcpy(&f, &i, 4);
+    return f;
+}
+static inline int32_t as_int(float f) {
+    int32_t i;
+    memcpy(&i, &f, 4);
+    return i;
+}
+static inline float mn(float a, float b) { return fminf(a, b); }
+static inline float mx(float a, float b) { return fmaxf(a, b); }
+
+/* InverseMatrix * vec4(p, w), column-major, terms summed left to right. */
+static inline v3 xform(const float* m, v3 p, float w) {
+    v3 r;
+    r.x = m[0] * p.x + m[4] * p.y + m[8] * p.z + m[12] * w;
+    r.y = m[1] * p.x + m[5] * p.y + m[9] * p.z + m[13] * w;
+    r.z = m[2] * p.x + m[6] * p.y + m[10] * p.z + m[14] * w;
+    return r;
+}
+
+/* RayTriangle: returns t (or -1 when outside), u, v. */
+void oracle_ray_triangle(const float* ro_, const float* rd_, const float* a, const float* b,
+                         const float* c, float* out_tuv) {
+    v3 ro = mk(ro_[0], ro_[1], ro_[2]), rd = mk(rd_[0], rd_[1], rd_[2]);
+    v3 v0 = mk(a[0], a[1], a[2]), v1 = mk(b[0], b[1], b[2]), v2 = mk(c[0], c[1], c[2]);
+    v3 v1v0 = sub(v1, v0);
+    v3 v2v0 = sub(v2, v0);
+    v3 rov0 = sub(ro, v0);
+    v3 n = cross(v1v0, v2v0);
+    v3 q = cross(rov0, rd);
+    float d = 1.0f / dot(rd, n);
+    float u = d * dot(neg(q), v2v0);
+    float v = d * dot(q, v1v0);
+    float t = d * dot(neg(n), rov0);
+    if (u < 0.0f || v < 0.0f || (u + v) > 1.0f) t = -1.0f;
+    out_tuv[0] = t;
+    out_tuv[1] = u;
+    out_tuv[2] = v;
+}
+
+static inline float tri_t(v3 ro, v3 rd, v3 v0, v3 v1, v3 v2) {
+    v3 v1v0 = sub(v1, v0);
+    v3 v2v0 = sub(v2, v0);
+    v3 rov0 = sub(ro, v0);
+    v3 n = cross(v1v0, v2v0);
+    v3 q = cross(rov0, rd);
+    float d = 1.0f / dot(rd, n);
+    float u = d * dot(neg(q), v2v0);
+    float v = d * dot(q, v1v0);
+    float t = d * dot(neg(n), rov0);
+    if (u < 0.0f || v < 0.0f || (u + v) > 1.0f) t = -1.0f;
+    return t;
+}
+
+/* RayBounds of the stackless include (t_min = 0.0001). */
+static inline float bounds_stackless(const float* bmin, const float* bmax, v3 ro, v3 inv, float t_min,
+                                     float t_max) {
+    float t0x = (bmin[0] - ro.x) * inv.x, t0y = (bmin[1] - ro.y) * inv.y, t0z = (bmin[2] - ro.z) * inv.z;
+    float t1x = (bmax[0] - ro.x) * inv.x, t1y = (bmax[1] - ro.y) * inv.y, t1z = (bmax[2] - ro.z) * inv.z;
+    float tmin = mx(mx(mx(mn(t0x, t1x), mn(t0y, t1y)), mn(t0z, t1z)), t_min);
+    float tmax = mn(mn(mx(t0x, t1x), mn(mx(t0y, t1y), mx(t0z, t1z))), t_max);
+    return (tmax >= tmin) ? tmin : -1.0f;
+}
+
+float oracle_bounds_stackless(const float* bmin, const float* bmax, const float* ro, const float* inv,
+                              float t_min, float t_max) {
+    return bounds_stackless(bmin, bmax, mk(ro[0], ro[1], ro[2]), mk(inv[0], inv[1], inv[2]), t_min, t_max);
+}
+
+/* RayBounds of the stack include: entry distance, or exit distance when the
+ * origin is inside. */
+static inline float bounds_stack(const crt_bounds* box, v3 ro, v3 inv, float maxt) {
+    float fx = (box->max[0] - ro.x) * inv.x, fy = (box->max[1] - ro.y) * inv.y, fz = (box->max[2] - ro.z) * inv.z;
+    float nx = (box->min[0] - ro.x) * inv.x, ny = (box->min[1] - ro.y) * inv.y, nz = (box->min[2] - ro.z) * inv.z;
+    float tmaxx = mx(fx, nx), tmaxy = mx(fy, ny), tmaxz = mx(fz, nz);
+    float tminx = mn(fx, nx), tminy = mn(fy, ny), tminz = mn(fz, nz);
+    float t1 = mn(mn(tmaxx, mn(tmaxy, tmaxz)), maxt);
+    float t0 = mx(mx(tminx, mx(tminy, tminz)), 0.0f);
+    return (t1 >= t0) ? (t0 > 0.0f ? t0 : t1) : -1.0f;
+}
+
+float oracle_bounds_stack(const float* bmin, const float* bmax, const float* ro, const float* inv, float maxt) {
+    crt_bounds b;
+    memcpy(b.min, bmin, 12);
+    memcpy(b.max, bmax, 12);
+    b.min[3] = b.max[3] = 0.0f;
+    return bounds_stack(&b, mk(ro[0], ro[1], ro[2]), mk(inv[0], inv[1], inv[2]), maxt);
+}
+
+/* ComputeBarycentrics: (u, v, w) of p in triangle (a, b, c). */
+static inline v3 barycentrics(v3 p, v3 a, v3 b, v3 c) {
+    v3 v0 = sub(b, a), v1 = sub(c, a), v2 = sub(p, a);
+    float d00 = dot(v0, v0);
+    float d01 = dot(v0, v1);
+    float d11 = dot(v1, v1);
+    float d20 = dot(v2, v0);
+    float d21 = dot(v2, v1);
+    float denom = d00 * d11 - d01 * d01;
+    float v = (d11 * d20 - d01 * d21) / denom;
+    float w = (d00 * d21 - d01 * d20) / denom;
+    float u = 1.0f - v - w;
+    return mk(u, v, w);
+}
+
+typedef struct {
+    int32_t format;
+    int32_t entity_count;
+    int64_t total_nodes;
+    const void* nodes;
+    const crt_triangle* tris;
+    const crt_vertex* verts;
+    const crt_entity* ents;
+    int32_t max_iter;
+} scene_t;
+
+typedef struct {
+    float t;
+    int32_t tri;
+    int32_t mesh;
+    uint32_t iters;
+    uint32_t status;
+    uint32_t tri_tests;
+} walk_t;
+
+static inline v3 vpos(const scene_t* s, int32_t idx) {
+    const float* p = s->verts[idx].position;
+    return mk(p[0], p[1], p[2]);
+}
+
+/* Tests triangles [start, start+len); returns 1 when an any-hit query is done. */
+static inline int test_leaf(const scene_t* s, v3 ro, v3 rd, int32_t start, int32_t len, int any, float* tmax,
+                            walk_t* w) {
+    for (int32_t idx = start; idx < start + len; ++idx) {
+        const crt_triangle* tr = &s->tris[idx];
+        v3 a = vpos(s, tr->packed[0]), b = vpos(s, tr->packed[1]), c = vpos(s, tr->packed[2]);
+        float t = tri_t(ro, rd, a, b, c);
+        w->tri_tests++;
+        if (t > 0.0f && t < *tmax) {
+            *tmax = t;
+            w->t = t;
+            w->mesh = tr->packed[3];
+            w->tri = idx;
+            if (any) return 1;
+        }
+    }
+    return 0;
+}
+
+/* IntersectBVHStackless / IntersectBVHStacklessOcclusion */
+static void walk_stackless(const scene_t* s, v3 ro, v3 rd, int32_t start, int32_t count, float tmax, int any,
+                           walk_t* w) {
+    const crt_stackless_node* nodes = (const crt_stackless_node*)s->nodes;
+    v3 inv = mk(1.0f / rd.x, 1.0f / rd.y, 1.0f / rd.z);
+    int32_t it = 0;
+    int32_t ptr = start;
+    int stopped = 0; /* left the loop through a break, not through its condition */
+    w->t = -1.0f;
+    w->tri = -1;
+    w->mesh = -1;
+    while (ptr >= 0 && it < s->max_iter) {
+        if (ptr < start || ptr > start + count || ptr < 0 || (int64_t)ptr >= s->total_nodes) {
+            w->status |= CRT_STATUS_BAD_INDEX;
+            stopped = 1;
+            break;
+        }
+        it++;
+        const crt_stackless_node* n = &nodes[ptr];
+        float tb = bounds_stackless(n->min, n->max, ro, inv, 0.0001f, tmax);
+        if (tb > 0.0f && tb < tmax) {
+            int32_t packed = as_int(n->min[3]);
+            if (packed != -1) {
+                if (test_leaf(s, ro, rd, packed >> 4, packed & 0xF, any, &tmax, w)) {
+                    w->iters += (uint32_t)it;
+                    return; /* occlusion: first accepted t */
+                }
+                ptr = as_int(n->max[3]);
+                if (ptr < 0) {
+                    stopped = 1;
+                    break;
+                }
+                ptr += start;
+                continue;
+            }
+            ptr++;
+            continue;
+        }
+        ptr = as_int(n->max[3]);
+        if (ptr < 0) {
+            stopped = 1;
+            break;
+        }
+        ptr += start;
+    }
+    if (!stopped && ptr >= 0) w->status |= CRT_STATUS_ITER_CAP;
+    w->iters += (uint32_t)it;
+}
+
+/* IntersectBVHStack / IntersectBVHStackOcclusion */
+static void walk_stack(const scene_t* s, v3 ro, v3 rd, int32_t start, int32_t count, float tmax, int any,
+                       walk_t* w) {
+    const crt_stack_node* nodes = (const crt_stack_node*)s->nodes;
+    v3 inv = mk(1.0f / rd.x, 1.0f / rd.y, 1.0f / rd.z);
+    int32_t stack[STACK_SIZE];
+    int32_t sp = 0;
+    int32_t it = 0;
+    int32_t cur = start;
+    int stopped = 0;
+    w->t = -1.0f;
+    w->tri = -1;
+    w->mesh = -1;
+    while (it < s->max_iter) {
+        if (sp >= 64 || sp < 0 || cur < start || cur > start + count || cur < 0 || (int64_t)cur >= s->total_nodes) {
+            w->status |= CRT_STATUS_BAD_INDEX;
+            stopped = 1;
+            break;
+        }
+        it++;
+        const crt_stack_node* n = &nodes[cur];
+        int32_t lp = as_int(n->left.min[3]), rp = as_int(n->right.min[3]);
+        int lleaf = lp != -1, rleaf = rp != -1;
+        float lt = lleaf ? -1.0f : bounds_stack(&n->left, ro, inv, tmax);
+        float rt = rleaf ? -1.0f : bounds_stack(&n->right, ro, inv, tmax);
+        if (lleaf && test_leaf(s, ro, rd, lp >> 4, lp & 0xF, any, &tmax, w)) {
+            w->iters += (uint32_t)it;
+            return;
+        }
+        if (rleaf && test_leaf(s, ro, rd, rp >> 4, rp & 0xF, any, &tmax, w)) {
+            w->iters += (uint32_t)it;
+            return;
+        }
+        if (lt > 0.0f && rt > 0.0f) {
+            int32_t next = as_int(n->left.max[3]) + start;
+            int32_t post = as_int(n->right.max[3]) + start;
+            if (rt < lt) {
+                int32_t tmp = next;
+                next = post;
+                post = tmp;
+            }
+            cur = next;
+            if (sp >= 63) {
+                w->status |= CRT_STATUS_STACK_OVERFLOW;
+                stopped = 1;
+                break;
+            }
+            stack[sp++] = post;
+            continue;
+        } else if (lt > 0.0f) {
+            cur = as_int(n->left.max[3]) + start;
+            continue;
+        } else if (rt > 0.0f) {
+            cur = as_int(n->right.max[3]) + start;
+            continue;
+        }
+        if (sp <= 0) {
+            stopped = 1;
+            break;
+        }
+        cur = stack[--sp];
+    }
+    if (!stopped) w->status |= CRT_STATUS_ITER_CAP;
+    w->iters += (uint32_t)it;
+}
+
+static inline void walk(const scene_t* s, v3 ro, v3 rd, const crt_entity* e, float tmax, int any, walk_t* w) {
+    v3 lro = xform(e->inverse, ro, 1.0f);
+    v3 lrd = xform(e->inverse, rd, 0.0f);
+    if (s->format == CRT_FORMAT_STACKLESS)
+        walk_stackless(s, lro, lrd, e->node_offset, e->node_count, tmax, any, w);
+    else
+        walk_stack(s, lro, lrd, e->node_offset, e->node_count, tmax, any, w);
+}
+
+static inline uint32_t pack_status(uint32_t visits, uint32_t flags) {
+    if (visits > CRT_STATUS_VISITS_MASK) visits = CRT_STATUS_VISITS_MASK;
+    return visits | flags;
+}
+
+/* IntersectScene / IntersectSceneIgnoreTransparent (closest hit). */
+static void scene_closest(const scene_t* s, const crt_ray* r, uint32_t flags, crt_hit* h, uint32_t* tri_tests) {
+    v3 ro = mk(r->org[0], r->org[1], r->org[2]);
+    v3 rd = mk(r->dir[0], r->dir[1], r->dir[2]);
+    float closest = -1.0f;
+    float tmax = r->tmax;
+    int32_t mesh = -1, tri = -1, ent = -1;
+    walk_t w;
+    memset(&w, 0, sizeof(w));
+    uint32_t visits = 0, status = 0, tests = 0;
+    for (int32_t i = 0; i < s->entity_count; ++i) {
+        const crt_entity* e = &s->ents[i];
+        if (flags & CRT_FLAG_IGNORE_TRANSPARENT) {
+            float alpha = as_float(e->data[1]);
+            if (alpha < 0.99f) continue;
+        }
+        w.iters = 0;
+        w.status = 0;
+        w.tri_tests = 0;
+        walk(s, ro, rd, e, tmax, 0, &w);
+        visits += w.iters;
+        status |= w.status;
+        tests += w.tri_tests;
+        if (w.t > 0.0f && w.t < tmax) {
+            tmax = w.t;
+            closest = w.t;
+            mesh = w.mesh;
+            tri = w.tri;
+            ent = i;
+        }
+    }
+    h->tri = tri;
+    h->mesh = mesh;
+    h->entity = ent;
+    h->status = pack_status(visits, status);
+    if (tri_tests) *tri_tests = tests;
+    int ok = closest > 0.0f && ((flags & CRT_FLAG_TRI0_IS_MISS) ? tri > 0 : tri >= 0);
+    if (ok) {
+        const crt_entity* e = &s->ents[ent];
+        v3 lro = xform(e->inverse, ro, 1.0f);
+        v3 lrd = xform(e->inverse, rd, 0.0f);
+        const crt_triangle* tr = &s->tris[tri];
+        v3 a = vpos(s, tr->packed[0]), b = vpos(s, tr->packed[1]), c = vpos(s, tr->packed[2]);
+        v3 p = mk(lro.x + lrd.x * closest, lro.y + lrd.y * closest, lro.z + lrd.z * closest);
+        v3 uvw = barycentrics(p, a, b, c);
+        h->t = closest;
+        h->u = uvw.x;
+        h->v = uvw.y;
+        h->w = uvw.z;
+    } else {
+        h->t = h->u = h->v = h->w = -1.0f;
+    }
+}
+
+/* Occlusion IntersectScene: first entity reporting a hit wins. */
+static void scene_any(const scene_t* s, const crt_ray* r, uint32_t flags, crt_hit* h, uint32_t* tri_tests) {
+    v3 ro = mk(r->org[0], r->org[1], r->org[2]);
+    v3 rd = mk(r->dir[0], r->dir[1], r->dir[2]);
+    walk_t w;
+    memset(&w, 0, sizeof(w));
+    uint32_t visits = 0, status = 0, tests = 0;
+    h->t = h->u = h->v = h->w = -1.0f;
+    h->tri = h->mesh = h->entity = -1;
+    for (int32_t i = 0; i < s->entity_count; ++i) {
+        const crt_entity* e = &s->ents[i];
+        if (flags & CRT_FLAG_IGNORE_TRANSPARENT) {
+            float alpha = as_float(e->data[1]);
+            if (alpha < 0.99f) continue;
+        }
+        w.iters = 0;
+        w.status = 0;
+        w.tri_tests = 0;
+        walk(s, ro, rd, e, r->tmax, 1, &w);
+        visits += w.iters;
+        status |= w.status;
+        tests += w.tri_tests;
+        if (w.t > 0.0f) {
+            h->t = w.t;
+            h->tri = w.tri;
+            h->mesh = w.mesh;
+            h->entity = i;
+            break;
+        }
+    }
+    h->status = pack_status(visits, status);
+    if (tri_tests) *tri_tests = tests;
+}
+
+typedef struct {
+    const scene_t* s;
+    int query;
+    uint32_t flags;
+    const crt_ray* rays;
+    crt_hit* hits;
+    uint32_t* tri_tests;
+    int64_t begin, end;
+} job_t;
+
+static void* run_job(void* arg) {
+    job_t* j = (job_t*)arg;
+    for (int64_t i = j->begin; i < j->end; ++i) {
+        uint32_t* tt = j->tri_tests ? &j->tri_tests[i] : NULL;
+        if (j->query == CRT_QUERY_ANY)
+            scene_any(j->s, &j->rays[i], j->flags, &j->hits[i], tt);
+        else
+            scene_closest(j->s, &j->rays[i], j->flags, &j->hits[i], tt);
+    }
+    return NULL;
+}
+
+/* Traces `n` rays on the host with `threads` POSIX threads.  `tri_tests`
+ * (optional) receives the triangle tests per ray; node visits are in
+ * hit.status.  max_iter <= 0 selects the reference's 1024.  Returns 0. */
+int oracle_trace(int32_t format, int32_t entity_count, int64_t total_nodes, const void* nodes,
+                 const crt_triangle* tris, const crt_vertex* verts, const crt_entity* ents, int query,
+                 uint32_t flags, const crt_ray* rays, crt_hit* hits, int64_t n, int threads, uint32_t* tri_tests,
+                 int32_t max_iter) {
+    scene_t s;
+    s.format = format;
+    s.entity_count = entity_count;
+    s.total_nodes = total_nodes;
+    s.nodes = nodes;
+    s.tris = tris;
+    s.verts = verts;
+    s.ents = ents;
+    s.max_iter = max_iter > 0 ? max_iter : MAX_ITER_DEFAULT;
+    if (format != CRT_FORMAT_STACKLESS && format != CRT_FORMAT_STACK) return -1;
+    if (threads < 1) threads = 1;
+    if (threads > 256) threads = 256;
+    if ((int64_t)threads > n) threads = (int)(n > 0 ? n : 1);
+    pthread_t th[256];
+    job_t jobs[256];
+    int64_t per = (n + threads - 1) / threads;
+    for (int t = 0; t < threads; ++t) {
+        jobs[t].s = &s;
+        jobs[t].query = query;
+        jobs[t].flags = flags;
+        jobs[t].rays = rays;
+        jobs[t].hits = hits;
+        jobs[t].tri_tests = tri_tests;
+        jobs[t].begin = (int64_t)t * per;
+        jobs[t].end = jobs[t].begin + per < n ? jobs[t].begin + per : n;
+        if (jobs[t].begin > n) jobs[t].begin = n;
+    }
+    for (int t = 1; t < threads; ++t) pthread_create(&th[t], NULL, run_job, &jobs[t]);
+    run_job(&jobs[0]);
+    for (int t = 1; t < threads; ++t) pthread_join(th[t], NULL);
+    return 0;
+}
+
+/* Brute force: for every ray, the smallest t in (0, tmax) over triangles
+ * [tri_begin, tri_end) of entity 0's space, with the same RayTriangle
+ * arithmetic.  The first triangle in index order wins exact ties. */
+int oracle_bruteforce_closest(const crt_triangle* tris, const crt_vertex* verts, const crt_entity* ent,
+                              int64_t tri_begin, int64_t tri_end, const crt_ray* rays, float* out_t,
+                              int32_t* out_tri, int64_t n) {
+    scene_t s;
+    memset(&s, 0, sizeof(s));
+    s.tris = tris;
+    s.verts = verts;
+    for (int64_t i = 0; i < n; ++i) {
+        v3 ro = mk(rays[i].org[0], rays[i].org[1], rays[i].org[2]);
+        v3 rd = mk(rays[i].dir[0], rays[i].dir[1], rays[i].dir[2]);
+        v3 lro = xform(ent->inverse, ro, 1.0f);
+        v3 lrd = xform(ent->inverse, rd, 0.0f);
+        float tmax = rays[i].tmax;
+        float best = -1.0f;
+        int32_t bt = -1;
+        for (int64_t k = tri_begin; k < tri_end; ++k) {
+            const crt_triangle* tr = &tris[k];
+            float t = tri_t(lro, lrd, vpos(&s, tr->packed[0]), vpos(&s, tr->packed[1]), vpos(&s, tr->packed[2]));
+            if (t > 0.0f && t < tmax) {
+                tmax = t;
+                best = t;
+                bt = (int32_t)k;
+            }
+        }
+        out_t[i] = best;
+        out_tri[i] = bt;
+    }
+    return 0;
+}
+
+/* GetRayDirectionAt + main() of Intersectors/TraverseBVHStackless.glsl:143-148,
+ * 371-388: pixel-corner uv, clip (uv*2-1, -1, 1) -> inv_proj -> (xy, -1, 0) ->
+ * inv_view, normalised; origin = inv_view[3].xyz.  normalize(v) = v * (1 /
+ * sqrt(dot(v, v))) with a correctly rounded sqrt and division. */
+void oracle_primary_rays(int width, int height, const float* inv_view, const float* inv_proj, float tmax,
+                         crt_ray* rays) {
+    for (int y = 0; y < height; ++y)
+        for (int x = 0; x < width; ++x) {
+            float u = (float)x / (float)width, v = (float)y / (float)height;
+            float cx = u * 2.0f - 1.0f, cy = v * 2.0f - 1.0f;
+            /* inv_proj * (cx, cy, -1, 1): x, y components */
+            float ex = inv_proj[0] * cx + inv_proj[4] * cy + inv_proj[8] * -1.0f + inv_proj[12] * 1.0f;
+            float ey = inv_proj[1] * cx + inv_proj[5] * cy + inv_proj[9] * -1.0f + inv_proj[13] * 1.0f;
+            v3 e = mk(ex, ey, -1.0f);
+            v3 d = xform(inv_view, e, 0.0f);
+            float len2 = dot(d, d);
+            float il = 1.0f / sqrtf(len2);
+            crt_ray* r = &rays[(int64_t)y * width + x];
+            r->org[0] = inv_view[12];
+            r->org[1] = inv_view[13];
+            r->org[2] = inv_view[14];
+            r->tmax = tmax;
+            r->dir[0] = d.x * il;
+            r->dir[1] = d.y * il;
+            r->dir[2] = d.z * il;
+            r->id = (uint32_t)((int64_t)y * width + x);
+        }
+}

@@ -1,0 +1,80 @@
+"""The C ABI library loads, exports what include/candela_rt.h declares and rejects bad input (no GPU needed)."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+
+
+def header_functions():
+    text = open(os.path.join(ROOT, "include", "candela_rt.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(crt_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_symbols_exported(crt):
+    import candela_amd._lib as L
+    names = header_functions()
+    assert len(names) >= 30
+    lib = crt.lib()
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    # Every declared function has a ctypes signature in the binding.
+    assert sorted(L.SIGNATURES) == names
+
+
+def test_abi_version(crt):
+    assert crt.lib().crt_abi_version() == 1
+
+
+def test_struct_sizes(crt):
+    assert crt.VERTEX_DTYPE.itemsize == 32
+    assert crt.TRIANGLE_DTYPE.itemsize == 16
+    assert crt.STACKLESS_NODE_DTYPE.itemsize == 32
+    assert crt.STACK_NODE_DTYPE.itemsize == 64
+    assert crt.ENTITY_DTYPE.itemsize == 192
+    assert crt.RAY_DTYPE.itemsize == 32
+    assert crt.HIT_DTYPE.itemsize == 32
+
+
+def test_errors_without_gpu(crt):
+    lib = crt.lib()
+    h = C.c_void_p()
+    assert lib.crt_build_bvh(7, None, 0, 0, None, C.byref(h)) == -1
+    assert b"format" in lib.crt_last_error()
+    assert lib.crt_create(9, C.byref(h)) == -1
+    # trace with a null scene is rejected before any device work
+    assert lib.crt_trace(None, 0, 0, None, None, 0, None, None) == -1
+    s = crt.SceneView(0, 1, 10, None, None, None, None)
+    assert lib.crt_trace(C.byref(s), 0, 0, None, None, 1, None, None) == -1
+    s2 = crt.SceneView(0, 0, 0, None, None, None, None)
+    cfg = crt.TraceConfig(100, 0)
+    assert lib.crt_trace(C.byref(s2), 0, 0, None, None, 0, C.byref(cfg), None) == -1
+    assert b"block_size" in lib.crt_last_error()
+
+
+def test_context_host_side(crt):
+    from candela_amd import scenes
+    r = crt.RayIntersector(crt.FORMAT_STACKLESS)
+    with pytest.raises(crt.CrtError):
+        r.push_entity(5)          # object 5 was never added (Intersector.h:203-205)
+    r.add_object(0, scenes.boxes_scene())
+    r.add_object(1, scenes.boxes_scene(seed=7))
+    r.push_entity(0)
+    r.push_entity(1, np.diag([2.0, 2.0, 2.0, 1.0]))
+    hb = r.host_buffers()
+    b0 = crt.build_bvh(crt.FORMAT_STACKLESS, scenes.boxes_scene())
+    b1 = crt.build_bvh(crt.FORMAT_STACKLESS, scenes.boxes_scene(seed=7), t_offset=b0.triangles.size)
+    n0 = b0.nodes.size
+    assert hb["nodes"].size == n0 + b1.nodes.size
+    assert hb["nodes"][:n0].tobytes() == b0.nodes.tobytes()
+    assert hb["nodes"][n0:].tobytes() == b1.nodes.tobytes()   # node links stay object-local
+    # vertex indices of object 1 are offset by object 0's vertex count (Intersector.h:190-197)
+    t1 = hb["triangles"][b0.triangles.size:]["packed"]
+    assert np.array_equal(t1[:, :3], b1.triangles["packed"][:, :3] + b0.vertices.size)
+    assert np.array_equal(t1[:, 3], b1.triangles["packed"][:, 3])
+    with pytest.raises(crt.CrtError):
+        r.scene()   # nothing buffered yet
