@@ -69,6 +69,27 @@ def bytes_per_ray(fmt: int, visits: np.ndarray, tests: np.ndarray) -> np.ndarray
     return node * visits.astype(np.float64) + 64.0 * tests.astype(np.float64) + 64.0
 
 
+def copy_probe(crt, args) -> int:
+    """Streaming copy of a buffer of known size: measured HBM read+write bandwidth."""
+    nbytes = (args.copy_probe // 16) * 16
+    src, dst = crt.DeviceArray(np.uint8, nbytes), crt.DeviceArray(np.uint8, nbytes)
+    src.fill_bytes(1)
+    stream = crt.Stream()
+    for _ in range(max(1, args.warmup)):
+        crt._lib.check(crt.lib().crt_device_copy(dst.ptr, src.ptr, nbytes, stream.handle))
+    stream.sync()
+    ev = Events(crt, 2)
+    ev.record(0, stream.handle)
+    for _ in range(args.steps):
+        crt._lib.check(crt.lib().crt_device_copy(dst.ptr, src.ptr, nbytes, stream.handle))
+    ev.record(1, stream.handle)
+    ms = ev.ms(0, 1) / args.steps
+    gbs = 2.0 * nbytes / (ms * 1e-3) / 1e9
+    print(json.dumps({"probe": "copy16_kernel", "bytes_read": nbytes, "bytes_written": nbytes,
+                      "ms_per_copy": round(ms, 4), "read_plus_write_GBs": round(gbs, 1)}), flush=True)
+    return 0
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -81,6 +102,9 @@ def main(argv=None) -> int:
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline (profiling runs)")
     ap.add_argument("--spheres", type=int, default=2000)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="instead of the workload, time the streaming copy kernel over BYTES (HBM probe and "
+                         "counter calibration)")
     args = ap.parse_args(argv)
 
     if args.gpus != 1:
@@ -94,6 +118,8 @@ def main(argv=None) -> int:
     fmt = crt.FORMAT_STACKLESS if args.format == "stackless" else crt.FORMAT_STACK
     if crt.device_count() < 1:
         raise SystemExit("no HIP device visible")
+    if args.copy_probe:
+        return copy_probe(crt, args)
 
     # --- scene + BVH (CPU constructor) -------------------------------------------------
     t0 = time.perf_counter()
@@ -125,50 +151,56 @@ def main(argv=None) -> int:
     stream = crt.Stream()
     sh = stream.handle
 
-    def step():
-        crt.trace(scene, crt.QUERY_ANY, d_ao, h_ao, block_size=args.block, lds_stack=args.lds_stack, stream=stream)
-        crt.trace(scene, crt.QUERY_CLOSEST, d_gi, h_gi, block_size=args.block, lds_stack=args.lds_stack,
-                  stream=stream)
-
-    for _ in range(args.warmup):
-        step()
-    stream.sync()
-    crt.lib().crt_device_sync()
-
     K = args.steps
-    ev = Events(crt, 3 * K + 2)
-    crt.lib().crt_device_sync()
-    w0 = time.perf_counter()
-    ev.record(0, sh)
-    for k in range(K):
-        ev.record(1 + 3 * k, sh)
-        crt.trace(scene, crt.QUERY_ANY, d_ao, h_ao, block_size=args.block, lds_stack=args.lds_stack, stream=stream)
-        ev.record(2 + 3 * k, sh)
-        crt.trace(scene, crt.QUERY_CLOSEST, d_gi, h_gi, block_size=args.block, lds_stack=args.lds_stack,
-                  stream=stream)
-        ev.record(3 + 3 * k, sh)
-    ev.record(3 * K + 1, sh)
-    stream.sync()
-    crt.lib().crt_device_sync()
-    w1 = time.perf_counter()
-    total_ms = ev.ms(0, 3 * K + 1)
-    ao_ms = [ev.ms(1 + 3 * k, 2 + 3 * k) for k in range(K)]
-    gi_ms = [ev.ms(2 + 3 * k, 3 + 3 * k) for k in range(K)]
-    ev.close()
-    wall_ms = (w1 - w0) * 1e3
-    ms_step = total_ms / K
+
+    def run(kernel: int, variant: int) -> dict:
+        """W warm-up + K timed steps of (AO any-hit, GI closest-hit) with one kernel choice."""
+        kw = dict(block_size=args.block, lds_stack=args.lds_stack, kernel=kernel, variant=variant, stream=stream)
+        for _ in range(args.warmup):
+            crt.trace(scene, crt.QUERY_ANY, d_ao, h_ao, **kw)
+            crt.trace(scene, crt.QUERY_CLOSEST, d_gi, h_gi, **kw)
+        stream.sync()
+        crt.lib().crt_device_sync()
+        ev = Events(crt, 3 * K + 2)
+        w0 = time.perf_counter()
+        ev.record(0, sh)
+        for k in range(K):
+            ev.record(1 + 3 * k, sh)
+            crt.trace(scene, crt.QUERY_ANY, d_ao, h_ao, **kw)
+            ev.record(2 + 3 * k, sh)
+            crt.trace(scene, crt.QUERY_CLOSEST, d_gi, h_gi, **kw)
+            ev.record(3 + 3 * k, sh)
+        ev.record(3 * K + 1, sh)
+        stream.sync()
+        crt.lib().crt_device_sync()
+        w1 = time.perf_counter()
+        total_ms = ev.ms(0, 3 * K + 1)
+        ao_ms = float(np.mean([ev.ms(1 + 3 * k, 2 + 3 * k) for k in range(K)]))
+        gi_ms = float(np.mean([ev.ms(2 + 3 * k, 3 + 3 * k) for k in range(K)]))
+        ev.close()
+        ms_step = total_ms / K
+        res = dict(kernel=kernel, variant=variant, ms_step=ms_step, wall_ms_step=(w1 - w0) * 1e3 / K, ao_ms=ao_ms,
+                   gi_ms=gi_ms, mrays=(n_ao + n_gi) / (ms_step * 1e-3) / 1e6)
+        log(f"kernel={kernel} variant={variant}: {ms_step:.3f} ms/step (wall {res['wall_ms_step']:.3f}), "
+            f"AO {ao_ms:.3f} ms, GI {gi_ms:.3f} ms -> {res['mrays']:.1f} Mrays/s")
+        return res
+
+    candidates = [(crt.KERNEL_THREAD_PER_RAY, 0)]
+    if fmt == crt.FORMAT_STACKLESS:
+        candidates += [(crt.KERNEL_PERSISTENT, v) for v in range(4)]
+    results = [run(k, v) for k, v in candidates]
+    thread = results[0]
+    best = max(results, key=lambda r: r["mrays"])
+    ms_step = best["ms_step"]
     rays_step = n_ao + n_gi
-    value = rays_step / (ms_step * 1e-3) / 1e6
-    log(f"{K} steps: {ms_step:.3f} ms/step (wall {wall_ms / K:.3f}), AO {np.mean(ao_ms):.3f} ms, "
-        f"GI {np.mean(gi_ms):.3f} ms -> {value:.1f} Mrays/s")
+    value = best["mrays"]
 
     # --- algorithmic bytes (counting build, untimed) ------------------------------------
     out = {}
     for name, dr, dh, q, n in (("ao", d_ao, h_ao, crt.QUERY_ANY, n_ao), ("gi", d_gi, h_gi, crt.QUERY_CLOSEST, n_gi)):
         cnt = crt.DeviceArray(np.uint32, n)
-        import ctypes as C
-        cfg = crt.TraceConfig(args.block, args.lds_stack, cnt.ptr)
-        crt._lib.check(crt.lib().crt_trace(C.byref(scene), q, 0, dr.ptr, dh.ptr, n, C.byref(cfg), None))
+        crt.trace(scene, q, dr, dh, block_size=args.block, lds_stack=args.lds_stack, kernel=best["kernel"],
+                  variant=best["variant"], tri_tests=cnt)
         crt.lib().crt_device_sync()
         hits = dh.download()
         tests = cnt.download()
@@ -177,7 +209,7 @@ def main(argv=None) -> int:
         out[name] = dict(hits=hits, bytes=float(b.sum()), visits=float(visits.mean()), tests=float(tests.mean()),
                          capped=float(np.mean((hits["status"] & crt.STATUS_ITER_CAP) != 0)),
                          hit_rate=float(np.mean(hits["t"] > 0)))
-    gi_s = np.mean(gi_ms) * 1e-3
+    gi_s = best["gi_ms"] * 1e-3
     achieved = out["gi"]["bytes"] / gi_s / 1e9
     log(f"GI: {out['gi']['visits']:.1f} node visits, {out['gi']['tests']:.1f} tri tests per ray, "
         f"{out['gi']['bytes'] / n_gi:.0f} B/ray, {achieved:.0f} GB/s algorithmic; "
@@ -197,13 +229,20 @@ def main(argv=None) -> int:
         n_s = int(min(n_gi, max(probe, args.cpu_seconds / 2 / max(per_ray, 1e-9))))
         sample_gi, sample_ao = gi[:n_s], ao[:n_s]
         ts = time.perf_counter()
-        ohg = oracle.trace(fmt, nodes, tris, verts, ents, 0, sample_gi, threads=threads)
-        oha = oracle.trace(fmt, nodes, tris, verts, ents, 1, sample_ao, threads=threads)
-        cpu_s = time.perf_counter() - ts
-        cpu_value = 2 * n_s / cpu_s / 1e6
+        passes = 0
+        while True:   # whole passes over the sample until the target duration is reached
+            ohg = oracle.trace(fmt, nodes, tris, verts, ents, 0, sample_gi, threads=threads)
+            oha = oracle.trace(fmt, nodes, tris, verts, ents, 1, sample_ao, threads=threads)
+            passes += 1
+            cpu_s = time.perf_counter() - ts
+            if cpu_s >= args.cpu_seconds or passes >= 100:
+                break
+        n_traced = 2 * n_s * passes
+        cpu_value = n_traced / cpu_s / 1e6
         cpu = {"value": round(cpu_value, 4), "unit": "Mrays/s", "cores": threads, "kind": "port",
                "sample": f"first {n_s} GI closest-hit + first {n_s} RTAO any-hit rays of the same step, "
-                         f"{cpu_s:.1f} s on {threads} threads (C restatement of the GLSL walk, -O3)"}
+                         f"{passes} pass(es), {cpu_s:.1f} s on {threads} threads "
+                         f"(C restatement of the GLSL walk, -O3)"}
         g, a = out["gi"]["hits"][:n_s], out["ao"]["hits"][:n_s]
         parity = {"rays": 2 * n_s,
                   "exact": float((np.concatenate([g, a]).tobytes() == np.concatenate([ohg, oha]).tobytes())),
@@ -231,12 +270,17 @@ def main(argv=None) -> int:
             "gi_rays": n_gi,
             "ao_rays": n_ao,
             "format": args.format,
-            "kernel": "plain (one batch per launch)",
+            "kernel": "plain (one batch per launch): " + ("persistent refill, variant %d" % best["variant"]
+                                                         if best["kernel"] == crt.KERNEL_PERSISTENT
+                                                         else "one ray per lane"),
             "parallelism": "single GPU",
         },
         "plain_mrays_s": round(value, 2),
+        "thread_per_ray_mrays_s": round(thread["mrays"], 2),
         "batched_mrays_s": None,
-        "roofline": {"bound": "hbm", "kernel": "trace_kernel (GI closest hit)", "achieved": round(achieved, 1),
+        "kernels": [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()} for r in results],
+        "roofline": {"bound": "hbm", "kernel": ("persistent_stackless" if best["kernel"] == crt.KERNEL_PERSISTENT
+                                                 else "trace_kernel") + " (GI closest hit)", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": None,
                      "bytes_per_launch": out["gi"]["bytes"], "launch_ms": round(gi_s * 1e3, 4),

@@ -12,7 +12,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from ._lib import (BOUNDS_DTYPE, ENTITY_DTYPE, FLAG_IGNORE_TRANSPARENT, FLAG_TRI0_IS_MISS, FORMAT_STACK,
+from ._lib import (KERNEL_AUTO, KERNEL_PERSISTENT, KERNEL_THREAD_PER_RAY, BOUNDS_DTYPE, ENTITY_DTYPE, FLAG_IGNORE_TRANSPARENT, FLAG_TRI0_IS_MISS, FORMAT_STACK,
                    FORMAT_STACKLESS, HIT_DTYPE, QUERY_ANY, QUERY_CLOSEST, RAY_DTYPE, STACK_NODE_DTYPE,
                    STACKLESS_NODE_DTYPE, STATUS_BAD_INDEX, STATUS_ITER_CAP, STATUS_STACK_OVERFLOW,
                    STATUS_VISITS_MASK, SWAP_NONE, SWAP_SEEDED, TRIANGLE_DTYPE, VERTEX_DTYPE, BuildOptions,
@@ -22,7 +22,7 @@ __all__ = [
     "VERTEX_DTYPE", "TRIANGLE_DTYPE", "STACKLESS_NODE_DTYPE", "STACK_NODE_DTYPE", "BOUNDS_DTYPE", "ENTITY_DTYPE",
     "RAY_DTYPE", "HIT_DTYPE", "FORMAT_STACKLESS", "FORMAT_STACK", "QUERY_CLOSEST", "QUERY_ANY",
     "FLAG_IGNORE_TRANSPARENT", "FLAG_TRI0_IS_MISS", "STATUS_ITER_CAP", "STATUS_STACK_OVERFLOW", "STATUS_BAD_INDEX",
-    "STATUS_VISITS_MASK", "SWAP_NONE", "SWAP_SEEDED", "CrtError", "DeviceArray", "Stream", "BuiltBVH", "build_bvh",
+    "STATUS_VISITS_MASK", "SWAP_NONE", "SWAP_SEEDED", "KERNEL_AUTO", "KERNEL_THREAD_PER_RAY", "KERNEL_PERSISTENT", "CrtError", "DeviceArray", "Stream", "BuiltBVH", "build_bvh",
     "RayIntersector", "trace", "device_count", "node_dtype", "lib",
 ]
 
@@ -159,12 +159,13 @@ def _colmajor(m) -> np.ndarray:
 
 
 def trace(scene: SceneView, query: int, rays: DeviceArray, hits: DeviceArray, n: int | None = None, flags: int = 0,
-          block_size: int = 0, lds_stack: int = 0, stream=None) -> None:
+          block_size: int = 0, lds_stack: int = 0, kernel: int = KERNEL_AUTO, variant: int = 0, grid: int = 0,
+          tri_tests: DeviceArray | None = None, stream=None) -> None:
     """crt_trace: closest- or any-hit for every ray of a device ray buffer (asynchronous)."""
     n = rays.count if n is None else int(n)
-    if n > rays.count or n > hits.count:
-        raise ValueError("ray/hit buffers smaller than n")
-    cfg = TraceConfig(block_size, lds_stack)
+    if n > rays.count or n > hits.count or (tri_tests is not None and n > tri_tests.count):
+        raise ValueError("ray/hit/counter buffers smaller than n")
+    cfg = TraceConfig(block_size, lds_stack, kernel, variant, grid, 0, tri_tests.ptr if tri_tests is not None else None)
     check(lib().crt_trace(C.byref(scene), query, flags, rays.ptr, hits.ptr, n, C.byref(cfg), _stream_handle(stream)))
 
 

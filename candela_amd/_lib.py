@@ -27,6 +27,9 @@ STATUS_BAD_INDEX = 1 << 26
 STATUS_VISITS_MASK = 0x00FFFFFF
 SWAP_NONE = 0
 SWAP_SEEDED = 1
+KERNEL_AUTO = 0
+KERNEL_THREAD_PER_RAY = 1
+KERNEL_PERSISTENT = 2
 
 # --- record dtypes (byte layouts of the reference buffers) ---------------------
 VERTEX_DTYPE = np.dtype([("position", "<f4", 4), ("normal_tangent_data", "<u4", 3), ("texcoords", "<u4")])
@@ -71,7 +74,8 @@ class SceneView(C.Structure):
 
 
 class TraceConfig(C.Structure):
-    _fields_ = [("block_size", C.c_int32), ("lds_stack", C.c_int32), ("tri_tests", C.c_void_p)]
+    _fields_ = [("block_size", C.c_int32), ("lds_stack", C.c_int32), ("kernel", C.c_int32), ("variant", C.c_int32),
+                ("grid", C.c_int32), ("reserved", C.c_int32), ("tri_tests", C.c_void_p)]
 
 
 class CrtError(RuntimeError):
@@ -97,6 +101,7 @@ SIGNATURES = {
     "crt_memcpy_h2d": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
     "crt_memcpy_d2h": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
     "crt_memset_d": (C.c_int, [C.c_void_p, C.c_int, C.c_size_t, C.c_void_p]),
+    "crt_device_copy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
     "crt_stream_create": (C.c_int, [C.POINTER(C.c_void_p)]),
     "crt_stream_destroy": (C.c_int, [C.c_void_p]),
     "crt_stream_sync": (C.c_int, [C.c_void_p]),

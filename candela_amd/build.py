@@ -24,8 +24,8 @@ LIB = os.path.join(HERE, "libcandela_rt.so")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 ARCH = os.environ.get("CRT_OFFLOAD_ARCH", "gfx950")
 
-HOST_SOURCES = ["bvh_build.cpp", "runtime.cpp", "scenes.cpp"]
-HIP_SOURCES = ["trace.hip", "raygen.hip"]
+HOST_SOURCES = ["bvh_build.cpp", "runtime.cpp"]
+HIP_SOURCES = ["trace.hip", "persistent.hip"]
 
 HIP_FLAGS = [
     f"--offload-arch={ARCH}",

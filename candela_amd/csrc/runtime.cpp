@@ -2,8 +2,10 @@
 // and the RayIntersector<T> context (Source/Core/BVH/Intersector.h:60-351).
 #include <hip/hip_runtime_api.h>
 
+#include <atomic>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <map>
 #include <new>
 #include <string>
@@ -58,6 +60,31 @@ static int validate_scene(const crt_scene_view* s) {
     return CRT_OK;
 }
 
+// Device counters of the persistent kernel: a small pool handed out round
+// robin, one slot per launch, zeroed by the launch itself on its stream.
+// Launches in flight on different streams get different slots unless more
+// than kCounterSlots of them overlap.
+constexpr int kCounterSlots = 4096;
+static std::mutex g_counter_mutex;
+static unsigned long long* g_counters[64] = {};
+static std::atomic<uint32_t> g_counter_next{0};
+
+static int counter_slot(unsigned long long** out) {
+    int dev = 0;
+    CRT_HIP(hipGetDevice(&dev), "hipGetDevice");
+    if (dev < 0 || dev >= 64) return set_error(CRT_E_INVALID, "device index out of range");
+    if (!g_counters[dev]) {
+        std::lock_guard<std::mutex> lock(g_counter_mutex);
+        if (!g_counters[dev]) {
+            void* p = nullptr;
+            CRT_HIP(hipMalloc(&p, kCounterSlots * sizeof(unsigned long long)), "hipMalloc(counters)");
+            g_counters[dev] = static_cast<unsigned long long*>(p);
+        }
+    }
+    *out = g_counters[dev] + (g_counter_next.fetch_add(1) % kCounterSlots);
+    return CRT_OK;
+}
+
 int trace(const crt_scene_view* scene, int query, uint32_t flags, const crt_ray* rays, crt_hit* hits, int64_t n,
           const crt_trace_config* cfg, void* stream) {
     int rc = validate_scene(scene);
@@ -69,9 +96,30 @@ int trace(const crt_scene_view* scene, int query, uint32_t flags, const crt_ray*
     int block, depth;
     rc = validate_trace_config(cfg, block, depth);
     if (rc) return rc;
+    const int kernel = cfg ? cfg->kernel : CRT_KERNEL_AUTO;
+    if (kernel < CRT_KERNEL_AUTO || kernel > CRT_KERNEL_PERSISTENT)
+        return set_error(CRT_E_INVALID, "crt_trace: unknown kernel");
+    if (kernel == CRT_KERNEL_PERSISTENT && scene->format != CRT_FORMAT_STACKLESS)
+        return set_error(CRT_E_INVALID, "crt_trace: the persistent kernel walks the stackless format only");
+    if (cfg && (cfg->variant < 0 || cfg->variant > 3 || cfg->grid < 0))
+        return set_error(CRT_E_INVALID, "crt_trace: persistent variant must be 0..3 and grid >= 0");
     SceneArgs a{scene->format, scene->entity_count, scene->total_nodes, scene->nodes,
                 scene->triangles, scene->vertices, scene->entities};
-    CRT_HIP(launch_trace(a, query, rays, hits, n, flags, kMaxIterations, block, depth, cfg ? cfg->tri_tests : nullptr,
+    uint32_t* tri_tests = cfg ? cfg->tri_tests : nullptr;
+    const bool persistent = kernel == CRT_KERNEL_PERSISTENT ||
+                            (kernel == CRT_KERNEL_AUTO && scene->format == CRT_FORMAT_STACKLESS);
+    if (n == 0) return CRT_OK;
+    if (persistent) {
+        unsigned long long* counter = nullptr;
+        rc = counter_slot(&counter);
+        if (rc) return rc;
+        PersistentShape shape{cfg ? cfg->variant : 0, cfg && cfg->grid ? cfg->grid : persistent_fill_grid(query)};
+        CRT_HIP(launch_persistent(a, query, rays, hits, n, flags, kMaxIterations, counter, tri_tests, shape,
+                                  static_cast<hipStream_t>(stream)),
+                "crt_trace: persistent kernel launch");
+        return CRT_OK;
+    }
+    CRT_HIP(launch_trace(a, query, rays, hits, n, flags, kMaxIterations, block, depth, tri_tests,
                          static_cast<hipStream_t>(stream)),
             "crt_trace: kernel launch");
     return CRT_OK;
@@ -259,6 +307,15 @@ int crt_memset_d(void* dst, int value, size_t bytes, void* stream) {
     if (bytes == 0) return CRT_OK;
     if (!dst) return crt::set_error(CRT_E_INVALID, "crt_memset_d: null pointer");
     CRT_HIP(hipMemsetAsync(dst, value, bytes, static_cast<hipStream_t>(stream)), "hipMemsetAsync");
+    return CRT_OK;
+}
+
+int crt_device_copy(void* dst, const void* src, size_t bytes, void* stream) {
+    if (bytes == 0) return CRT_OK;
+    if (!dst || !src || bytes % 16 || (reinterpret_cast<uintptr_t>(dst) | reinterpret_cast<uintptr_t>(src)) % 16)
+        return crt::set_error(CRT_E_INVALID, "crt_device_copy: needs 16-byte aligned pointers and size");
+    CRT_HIP(crt::launch_copy16(dst, src, int64_t(bytes / 16), static_cast<hipStream_t>(stream)),
+            "crt_device_copy: kernel launch");
     return CRT_OK;
 }
 

@@ -452,3 +452,25 @@ hipError_t launch_primary(const SceneArgs& s, const Camera& cam, crt_ray* rays, 
 }
 
 }  // namespace crt
+
+// ---------------------------------------------------------------------------
+// Streaming copy (bandwidth / counter calibration)
+// ---------------------------------------------------------------------------
+namespace crt {
+namespace dev {
+__global__ void __launch_bounds__(256) copy16_kernel(float4* __restrict__ dst, const float4* __restrict__ src,
+                                                     int64_t n) {
+    const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+}  // namespace dev
+
+hipError_t launch_copy16(void* dst, const void* src, int64_t n16, hipStream_t stream) {
+    if (n16 <= 0) return hipSuccess;
+    const int64_t want = (n16 + 255) / 256;
+    const unsigned grid = unsigned(want < 256 * 64 ? want : 256 * 64);
+    hipLaunchKernelGGL(dev::copy16_kernel, dim3(grid), dim3(256), 0, stream, static_cast<float4*>(dst),
+                       static_cast<const float4*>(src), n16);
+    return hipGetLastError();
+}
+}  // namespace crt

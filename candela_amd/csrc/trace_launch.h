@@ -36,4 +36,19 @@ hipError_t launch_trace(const SceneArgs& s, int query, const crt_ray* rays, crt_
 hipError_t launch_primary(const SceneArgs& s, const Camera& cam, crt_ray* rays, crt_hit* hits, int width, int height,
                           uint32_t flags, int32_t max_iter, int block, hipStream_t stream);
 
+// Tuning of the persistent kernel.
+struct PersistentShape {
+    int variant;  // refill / inner / leaf-batch preset
+    int grid;     // workgroups launched
+};
+
+hipError_t launch_persistent(const SceneArgs& s, int query, const crt_ray* rays, crt_hit* hits, int64_t n,
+                             uint32_t flags, int32_t max_iter, unsigned long long* counter, uint32_t* tri_tests,
+                             const PersistentShape& shape, hipStream_t stream);
+
+// Workgroups of 256 threads that fill every CU with the persistent kernel.
+int persistent_fill_grid(int query);
+
+hipError_t launch_copy16(void* dst, const void* src, int64_t n16, hipStream_t stream);
+
 }  // namespace crt

@@ -210,6 +210,10 @@ int crt_device_free(void* ptr);
 int crt_memcpy_h2d(void* dst, const void* src, size_t bytes, void* stream);
 int crt_memcpy_d2h(void* dst, const void* src, size_t bytes, void* stream);
 int crt_memset_d(void* dst, int value, size_t bytes, void* stream);
+/* Device-to-device copy by a streaming kernel (16-byte loads and stores, one
+ * pass): the copy whose counters calibrate the HBM-traffic measurements.
+ * `bytes` must be a multiple of 16 and both pointers 16-byte aligned. */
+int crt_device_copy(void* dst, const void* src, size_t bytes, void* stream);
 int crt_stream_create(void** stream);
 int crt_stream_destroy(void* stream);
 int crt_stream_sync(void* stream);
@@ -237,11 +241,23 @@ typedef struct crt_scene_view {
     const crt_entity* entities;
 } crt_scene_view;
 
-/* Launch shape of the one-ray-per-lane kernels. */
+/* Kernel choice and launch shape. */
+enum {
+    CRT_KERNEL_AUTO = 0,        /* persistent where available, else thread-per-ray   */
+    CRT_KERNEL_THREAD_PER_RAY = 1,  /* one lane per ray, grid = ceil(n / block)     */
+    CRT_KERNEL_PERSISTENT = 2   /* device-filling waves that refill idle lanes from a
+                                   ray counter and batch leaf tests (stackless only) */
+};
+
 typedef struct crt_trace_config {
-    int32_t block_size;    /* threads per workgroup: 64..1024, multiple of 64; 0 = 256 */
+    int32_t block_size;    /* thread-per-ray: threads per workgroup, 64..1024, multiple
+                              of 64; 0 = 256 (persistent kernels use 256) */
     int32_t lds_stack;     /* stack format: entries kept in LDS, one of 8, 16, 32, 48,
                               64 (0 = 16); deeper entries live in per-lane scratch */
+    int32_t kernel;        /* CRT_KERNEL_* */
+    int32_t variant;       /* persistent: refill / inner-step / leaf-batch preset 0..3 */
+    int32_t grid;          /* persistent: workgroups; 0 = enough to fill every CU */
+    int32_t reserved;
     uint32_t* tri_tests;   /* optional device array [nrays]: the counting build of the
                               kernel writes the triangle tests of each ray (node
                               visits are in hit.status); NULL = no counting */

@@ -58,12 +58,18 @@ def device_scene(crt, fmt, nodes, tris, verts, ents):
     return s, d
 
 
-def gpu_trace(crt, scene, query, rays, flags=0, block=0, lds=0):
+def gpu_trace(crt, scene, query, rays, flags=0, block=0, lds=0, kernel=0, variant=0, grid=0):
     dr = crt.DeviceArray.from_host(rays)
     dh = crt.DeviceArray(crt.HIT_DTYPE, rays.size)
-    crt.trace(scene, query, dr, dh, flags=flags, block_size=block, lds_stack=lds)
+    crt.trace(scene, query, dr, dh, flags=flags, block_size=block, lds_stack=lds, kernel=kernel, variant=variant,
+              grid=grid)
     crt.lib().crt_device_sync()
     return dh.download()
+
+
+# (format, kernel) pairs: the one-ray-per-lane kernel walks both formats, the
+# persistent refill kernel the stackless one.
+WALKERS = [(0, 1), (0, 2), (1, 1)]
 
 
 @pytest.mark.parametrize("fmt", [0, 1])
@@ -88,20 +94,20 @@ def test_primary_boxes_512(gpu, orc, fmt):
     assert 0.5 < np.mean(oh["t"] > 0) < 1.0
 
 
-@pytest.mark.parametrize("fmt", [0, 1])
+@pytest.mark.parametrize("fmt,kernel", WALKERS)
 @pytest.mark.parametrize("query", [0, 1])
-def test_random_rays_cloud(gpu, orc, fmt, query):
+def test_random_rays_cloud(gpu, orc, fmt, kernel, query):
     from candela_amd import scenes
-    log(f"[case] random rays cloud fmt={fmt} query={query}")
+    log(f"[case] random rays cloud fmt={fmt} kernel={kernel} query={query}")
     b = gpu.build_bvh(fmt, scenes.icosphere_cloud(count=300, subdivisions=3, seed=12, box=40.0))
     import oracle
     ents = oracle.identity_entity(b.nodes.size)
     s, keep = device_scene(gpu, fmt, b.nodes, b.triangles, b.vertices, ents)
     rays = np.concatenate([random_rays(60000, [-5, -5, -5], [45, 45, 45], seed=21, tmax=3.0 if query else 1e6),
                            axis_aligned_rays(4000, [0, 0, 0], [40, 40, 40], seed=22)])
-    gh = gpu_trace(gpu, s, query, rays)
+    gh = gpu_trace(gpu, s, query, rays, kernel=kernel)
     oh = orc.trace(fmt, b.nodes, b.triangles, b.vertices, ents, query, rays)
-    compare(gh, oh, f"cloud fmt={fmt} q={query}")
+    compare(gh, oh, f"cloud fmt={fmt} kernel={kernel} q={query}")
 
 
 @pytest.mark.parametrize("fmt", [0, 1])
@@ -133,6 +139,9 @@ def test_instances_and_flags(gpu, orc, fmt):
             oh = orc.trace(fmt, hb["nodes"], hb["triangles"], hb["vertices"], hb["entities"], query, rays,
                            flags=flags)
             compare(dh.download(), oh, f"instances fmt={fmt} flags={flags} q={query}")
+            gpu.trace(r.scene(), query, dr, dh, flags=flags, kernel=gpu.KERNEL_THREAD_PER_RAY)
+            gpu.lib().crt_device_sync()
+            compare(dh.download(), oh, f"instances thread-per-ray fmt={fmt} flags={flags} q={query}")
 
 
 def test_launch_shapes_do_not_change_results(gpu, orc):
@@ -148,8 +157,15 @@ def test_launch_shapes_do_not_change_results(gpu, orc):
         shapes = [(64, 8), (128, 16), (256, 32), (512, 16), (1024, 8), (256, 64), (256, 48)] if fmt else \
             [(64, 0), (192, 0), (256, 0), (1024, 0)]
         for block, lds in shapes:
-            gh = gpu_trace(gpu, s, 0, rays, block=block, lds=lds)
+            gh = gpu_trace(gpu, s, 0, rays, block=block, lds=lds, kernel=1)
             compare(gh, oh, f"fmt={fmt} block={block} lds={lds}")
+        if fmt == 0:
+            for q in (0, 1):
+                oq = oh if q == 0 else orc.trace(fmt, b.nodes, b.triangles, b.vertices, ents, 1, rays)
+                for variant in range(4):
+                    for grid in (0, 1, 7, 33):
+                        gh = gpu_trace(gpu, s, q, rays, kernel=2, variant=variant, grid=grid)
+                        compare(gh, oq, f"persistent q={q} variant={variant} grid={grid}")
 
 
 def test_guards_on_gpu(gpu, orc):
@@ -164,11 +180,11 @@ def test_guards_on_gpu(gpu, orc):
     for fmt, nodes, flag in cases:
         ents = oracle.identity_entity(nodes.size)
         s, keep = device_scene(gpu, fmt, nodes, tris, v, ents)
-        for lds in ((0,) if fmt == 0 else (8, 16, 64)):
-            gh = gpu_trace(gpu, s, 0, ray, lds=lds)
+        for kernel, lds in (((1, 0), (2, 0)) if fmt == 0 else ((1, 8), (1, 16), (1, 64))):
+            gh = gpu_trace(gpu, s, 0, ray, lds=lds, kernel=kernel)
             oh = orc.trace(fmt, nodes, tris, v, ents, 0, ray)
             assert (oh["status"] & 0xFF000000 == flag).all()
-            compare(gh, oh, f"guard fmt={fmt} flag={flag:#x} lds={lds}")
+            compare(gh, oh, f"guard fmt={fmt} flag={flag:#x} kernel={kernel} lds={lds}")
 
 
 def test_empty_and_ragged_batches(gpu, orc):
@@ -184,11 +200,12 @@ def test_empty_and_ragged_batches(gpu, orc):
         if n:
             dr.upload(np.concatenate([rays]))
         dh = gpu.DeviceArray(gpu.HIT_DTYPE, max(n, 1))
-        dh.fill_bytes(0x7F)
-        gpu.trace(s, 0, dr, dh, n=n)
-        gpu.lib().crt_device_sync()
-        out = dh.download()
-        if n == 0:
-            assert out.view(np.uint8).tolist() == [0x7F] * 32   # nothing written
-            continue
-        compare(out[:n], orc.trace(0, b.nodes, b.triangles, b.vertices, ents, 0, rays), f"n={n}")
+        for kernel in (1, 2):
+            dh.fill_bytes(0x7F)
+            gpu.trace(s, 0, dr, dh, n=n, kernel=kernel)
+            gpu.lib().crt_device_sync()
+            out = dh.download()
+            if n == 0:
+                assert out.view(np.uint8).tolist() == [0x7F] * 32   # nothing written
+                continue
+            compare(out[:n], orc.trace(0, b.nodes, b.triangles, b.vertices, ents, 0, rays), f"n={n} kernel={kernel}")
