@@ -102,6 +102,8 @@ def main(argv=None) -> int:
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline (profiling runs)")
     ap.add_argument("--spheres", type=int, default=2000)
+    ap.add_argument("--only", default="", metavar="KERNEL:VARIANT",
+                    help="time one kernel choice only (e.g. 1:0 one ray per lane, 2:3 persistent preset 3)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="instead of the workload, time the streaming copy kernel over BYTES (HBM probe and "
                          "counter calibration)")
@@ -187,7 +189,10 @@ def main(argv=None) -> int:
 
     candidates = [(crt.KERNEL_THREAD_PER_RAY, 0)]
     if fmt == crt.FORMAT_STACKLESS:
-        candidates += [(crt.KERNEL_PERSISTENT, v) for v in range(4)]
+        candidates += [(crt.KERNEL_PERSISTENT, v) for v in range(6)]
+    if args.only:
+        k, v = (int(x) for x in args.only.split(":"))
+        candidates = [(k, v)]
     results = [run(k, v) for k, v in candidates]
     thread = results[0]
     best = max(results, key=lambda r: r["mrays"])

@@ -104,7 +104,7 @@ __device__ __forceinline__ uint32_t pack_status(uint32_t visits, uint32_t flags)
 
 // Per-lane state of one ray in the scene loop.
 struct Lane {
-    int64_t slot;      // ray index
+    int32_t slot;      // ray index (a launch holds < 2^31 rays)
     f3 lro, lrd, inv;  // ray in the current entity's space
     float ray_tmax;
     int32_t ent;       // current entity
@@ -226,9 +226,9 @@ __global__ void __launch_bounds__(256) persistent_stackless(SceneArgs s, const c
                 if (base >= (unsigned long long)n && idle == ~0ull) break;  // batch drained
                 if (!active) {
                     const int64_t my = int64_t(base) + int64_t(lane_rank(idle));
-                    if (my < n) {
+                    if (my < n) {  // n < 2^31, so `my` fits the 32-bit slot
                         active = true;
-                        L.slot = my;
+                        L.slot = int32_t(my);
                         const float4 a = reinterpret_cast<const float4*>(rays + my)[0];
                         L.ray_tmax = a.w;
                         L.ent = -1;
@@ -357,6 +357,9 @@ __global__ void reset_counter(unsigned long long* c) { *c = 0ull; }
 
 }  // namespace pdev
 
+// Presets (REFILL, INNER, LEAF_BATCH).  REFILL = 65 never refills single lanes:
+// a wave takes a fresh batch of 64 rays only when all of its lanes are idle,
+// which keeps the leaf batching and drops the per-lane refill.
 template <bool ANY>
 static hipError_t launch_p(const SceneArgs& s, const crt_ray* rays, crt_hit* hits, int64_t n, uint32_t flags,
                            int32_t max_iter, unsigned long long* counter, uint32_t* tri_tests, int grid,
@@ -365,14 +368,14 @@ static hipError_t launch_p(const SceneArgs& s, const crt_ray* rays, crt_hit* hit
 #define CRT_PS_LAUNCH(R, I, LB)                                                                               \
     hipLaunchKernelGGL((pdev::persistent_stackless<ANY, R, I, LB>), dim3(unsigned(grid)), dim3(B), 0, stream, s, \
                        rays, hits, n, flags, max_iter, counter, tri_tests)
-    if (shape.variant == 1)
-        CRT_PS_LAUNCH(32, 16, 32);
-    else if (shape.variant == 2)
-        CRT_PS_LAUNCH(8, 8, 8);
-    else if (shape.variant == 3)
-        CRT_PS_LAUNCH(16, 32, 48);
-    else
-        CRT_PS_LAUNCH(16, 16, 16);
+    switch (shape.variant) {
+        case 1: CRT_PS_LAUNCH(32, 16, 32); break;
+        case 2: CRT_PS_LAUNCH(65, 16, 16); break;
+        case 3: CRT_PS_LAUNCH(65, 32, 32); break;
+        case 4: CRT_PS_LAUNCH(16, 32, 48); break;
+        case 5: CRT_PS_LAUNCH(65, 8, 8); break;
+        default: CRT_PS_LAUNCH(16, 16, 16); break;
+    }
 #undef CRT_PS_LAUNCH
     return hipGetLastError();
 }

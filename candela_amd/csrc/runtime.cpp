@@ -101,8 +101,8 @@ int trace(const crt_scene_view* scene, int query, uint32_t flags, const crt_ray*
         return set_error(CRT_E_INVALID, "crt_trace: unknown kernel");
     if (kernel == CRT_KERNEL_PERSISTENT && scene->format != CRT_FORMAT_STACKLESS)
         return set_error(CRT_E_INVALID, "crt_trace: the persistent kernel walks the stackless format only");
-    if (cfg && (cfg->variant < 0 || cfg->variant > 3 || cfg->grid < 0))
-        return set_error(CRT_E_INVALID, "crt_trace: persistent variant must be 0..3 and grid >= 0");
+    if (cfg && (cfg->variant < 0 || cfg->variant > 5 || cfg->grid < 0))
+        return set_error(CRT_E_INVALID, "crt_trace: persistent variant must be 0..5 and grid >= 0");
     SceneArgs a{scene->format, scene->entity_count, scene->total_nodes, scene->nodes,
                 scene->triangles, scene->vertices, scene->entities};
     uint32_t* tri_tests = cfg ? cfg->tri_tests : nullptr;
@@ -110,6 +110,8 @@ int trace(const crt_scene_view* scene, int query, uint32_t flags, const crt_ray*
                             (kernel == CRT_KERNEL_AUTO && scene->format == CRT_FORMAT_STACKLESS);
     if (n == 0) return CRT_OK;
     if (persistent) {
+        if (n >= (int64_t(1) << 31))
+            return set_error(CRT_E_INVALID, "crt_trace: the persistent kernel takes < 2^31 rays per launch");
         unsigned long long* counter = nullptr;
         rc = counter_slot(&counter);
         if (rc) return rc;

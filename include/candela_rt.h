@@ -255,7 +255,7 @@ typedef struct crt_trace_config {
     int32_t lds_stack;     /* stack format: entries kept in LDS, one of 8, 16, 32, 48,
                               64 (0 = 16); deeper entries live in per-lane scratch */
     int32_t kernel;        /* CRT_KERNEL_* */
-    int32_t variant;       /* persistent: refill / inner-step / leaf-batch preset 0..3 */
+    int32_t variant;       /* persistent: refill / inner-step / leaf-batch preset 0..5 */
     int32_t grid;          /* persistent: workgroups; 0 = enough to fill every CU */
     int32_t reserved;
     uint32_t* tri_tests;   /* optional device array [nrays]: the counting build of the

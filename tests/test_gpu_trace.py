@@ -162,7 +162,7 @@ def test_launch_shapes_do_not_change_results(gpu, orc):
         if fmt == 0:
             for q in (0, 1):
                 oq = oh if q == 0 else orc.trace(fmt, b.nodes, b.triangles, b.vertices, ents, 1, rays)
-                for variant in range(4):
+                for variant in range(6):
                     for grid in (0, 1, 7, 33):
                         gh = gpu_trace(gpu, s, q, rays, kernel=2, variant=variant, grid=grid)
                         compare(gh, oq, f"persistent q={q} variant={variant} grid={grid}")
