@@ -69,6 +69,20 @@ def bytes_per_ray(fmt: int, visits: np.ndarray, tests: np.ndarray) -> np.ndarray
     return node * visits.astype(np.float64) + 64.0 * tests.astype(np.float64) + 64.0
 
 
+def measured_traffic(best: dict, args, fmt: int):
+    """HBM bytes per GI launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs
+    (profiles/traffic_*.json) when they were taken on this kernel choice and workload."""
+    import glob
+    choice = f"{best['kernel']}:{best['variant']}"
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "traffic_*.json"))):
+        with open(path) as f:
+            d = json.load(f)
+        if d.get("kernel_choice") == choice and d.get("workload") == "cloud2000_s3_gi" and args.spheres == 2000 \
+                and fmt == 0:
+            return float(d["traffic_bytes"]), os.path.relpath(path, ROOT)
+    return None, None
+
+
 def copy_probe(crt, args) -> int:
     """Streaming copy of a buffer of known size: measured HBM read+write bandwidth."""
     nbytes = (args.copy_probe // 16) * 16
@@ -216,6 +230,7 @@ def main(argv=None) -> int:
                          hit_rate=float(np.mean(hits["t"] > 0)))
     gi_s = best["gi_ms"] * 1e-3
     achieved = out["gi"]["bytes"] / gi_s / 1e9
+    traffic, traffic_src = measured_traffic(best, args, fmt)
     log(f"GI: {out['gi']['visits']:.1f} node visits, {out['gi']['tests']:.1f} tri tests per ray, "
         f"{out['gi']['bytes'] / n_gi:.0f} B/ray, {achieved:.0f} GB/s algorithmic; "
         f"iteration-capped {out['gi']['capped'] * 100:.2f}%")
@@ -287,7 +302,7 @@ def main(argv=None) -> int:
         "roofline": {"bound": "hbm", "kernel": ("persistent_stackless" if best["kernel"] == crt.KERNEL_PERSISTENT
                                                  else "trace_kernel") + " (GI closest hit)", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None,
+                     "traffic": traffic, "traffic_source": traffic_src,
                      "bytes_per_launch": out["gi"]["bytes"], "launch_ms": round(gi_s * 1e3, 4),
                      "bytes_per_ray": round(out["gi"]["bytes"] / n_gi, 1)},
         "cpu_baseline": cpu,
