@@ -34,6 +34,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "incoherent Mrays/s, batched vs plain launch on 1 MI355X; % of HBM roofline"
 HBM_PEAK_GBS = 8000.0
+KERNEL_NAMES = {1: "trace_kernel", 2: "persistent_stackless", 3: "grid_stackless"}
 
 
 def log(msg: str) -> None:
@@ -203,7 +204,7 @@ def main(argv=None) -> int:
 
     candidates = [(crt.KERNEL_THREAD_PER_RAY, 0)]
     if fmt == crt.FORMAT_STACKLESS:
-        candidates += [(crt.KERNEL_PERSISTENT, v) for v in range(6)]
+        candidates += [(crt.KERNEL_WAVE, v) for v in range(6)] + [(crt.KERNEL_PERSISTENT, v) for v in range(6)]
     if args.only:
         k, v = (int(x) for x in args.only.split(":"))
         candidates = [(k, v)]
@@ -290,17 +291,15 @@ def main(argv=None) -> int:
             "gi_rays": n_gi,
             "ao_rays": n_ao,
             "format": args.format,
-            "kernel": "plain (one batch per launch): " + ("persistent refill, variant %d" % best["variant"]
-                                                         if best["kernel"] == crt.KERNEL_PERSISTENT
-                                                         else "one ray per lane"),
+            "kernel": "plain (one batch per launch): %s, preset %d" % (KERNEL_NAMES[best["kernel"]],
+                                                                       best["variant"]),
             "parallelism": "single GPU",
         },
         "plain_mrays_s": round(value, 2),
         "thread_per_ray_mrays_s": round(thread["mrays"], 2),
         "batched_mrays_s": None,
         "kernels": [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()} for r in results],
-        "roofline": {"bound": "hbm", "kernel": ("persistent_stackless" if best["kernel"] == crt.KERNEL_PERSISTENT
-                                                 else "trace_kernel") + " (GI closest hit)", "achieved": round(achieved, 1),
+        "roofline": {"bound": "hbm", "kernel": KERNEL_NAMES[best["kernel"]] + " (GI closest hit)", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "traffic_source": traffic_src,
                      "bytes_per_launch": out["gi"]["bytes"], "launch_ms": round(gi_s * 1e3, 4),

@@ -30,6 +30,7 @@ SWAP_SEEDED = 1
 KERNEL_AUTO = 0
 KERNEL_THREAD_PER_RAY = 1
 KERNEL_PERSISTENT = 2
+KERNEL_WAVE = 3
 
 # --- record dtypes (byte layouts of the reference buffers) ---------------------
 VERTEX_DTYPE = np.dtype([("position", "<f4", 4), ("normal_tangent_data", "<u4", 3), ("texcoords", "<u4")])

@@ -97,18 +97,26 @@ int trace(const crt_scene_view* scene, int query, uint32_t flags, const crt_ray*
     rc = validate_trace_config(cfg, block, depth);
     if (rc) return rc;
     const int kernel = cfg ? cfg->kernel : CRT_KERNEL_AUTO;
-    if (kernel < CRT_KERNEL_AUTO || kernel > CRT_KERNEL_PERSISTENT)
+    if (kernel < CRT_KERNEL_AUTO || kernel > CRT_KERNEL_WAVE)
         return set_error(CRT_E_INVALID, "crt_trace: unknown kernel");
-    if (kernel == CRT_KERNEL_PERSISTENT && scene->format != CRT_FORMAT_STACKLESS)
-        return set_error(CRT_E_INVALID, "crt_trace: the persistent kernel walks the stackless format only");
+    if ((kernel == CRT_KERNEL_PERSISTENT || kernel == CRT_KERNEL_WAVE) && scene->format != CRT_FORMAT_STACKLESS)
+        return set_error(CRT_E_INVALID, "crt_trace: the persistent and wave kernels walk the stackless format only");
     if (cfg && (cfg->variant < 0 || cfg->variant > 5 || cfg->grid < 0))
-        return set_error(CRT_E_INVALID, "crt_trace: persistent variant must be 0..5 and grid >= 0");
+        return set_error(CRT_E_INVALID, "crt_trace: variant must be 0..5 and grid >= 0");
     SceneArgs a{scene->format, scene->entity_count, scene->total_nodes, scene->nodes,
                 scene->triangles, scene->vertices, scene->entities};
     uint32_t* tri_tests = cfg ? cfg->tri_tests : nullptr;
-    const bool persistent = kernel == CRT_KERNEL_PERSISTENT ||
-                            (kernel == CRT_KERNEL_AUTO && scene->format == CRT_FORMAT_STACKLESS);
+    const bool persistent = kernel == CRT_KERNEL_PERSISTENT;
+    const bool wave = kernel == CRT_KERNEL_WAVE || (kernel == CRT_KERNEL_AUTO && scene->format == CRT_FORMAT_STACKLESS);
     if (n == 0) return CRT_OK;
+    if (wave) {
+        if (n >= (int64_t(1) << 31) * 256 - 2048)
+            return set_error(CRT_E_INVALID, "crt_trace: too many rays for one launch");
+        CRT_HIP(launch_grid(a, query, rays, hits, n, flags, kMaxIterations, tri_tests, cfg ? cfg->variant : 0,
+                            static_cast<hipStream_t>(stream)),
+                "crt_trace: wave kernel launch");
+        return CRT_OK;
+    }
     if (persistent) {
         if (n >= (int64_t(1) << 31))
             return set_error(CRT_E_INVALID, "crt_trace: the persistent kernel takes < 2^31 rays per launch");

@@ -46,6 +46,9 @@ hipError_t launch_persistent(const SceneArgs& s, int query, const crt_ray* rays,
                              uint32_t flags, int32_t max_iter, unsigned long long* counter, uint32_t* tri_tests,
                              const PersistentShape& shape, hipStream_t stream);
 
+hipError_t launch_grid(const SceneArgs& s, int query, const crt_ray* rays, crt_hit* hits, int64_t n, uint32_t flags,
+                       int32_t max_iter, uint32_t* tri_tests, int variant, hipStream_t stream);
+
 // Workgroups of 256 threads that fill every CU with the persistent kernel.
 int persistent_fill_grid(int query);
 

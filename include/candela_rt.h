@@ -245,8 +245,10 @@ typedef struct crt_scene_view {
 enum {
     CRT_KERNEL_AUTO = 0,        /* persistent where available, else thread-per-ray   */
     CRT_KERNEL_THREAD_PER_RAY = 1,  /* one lane per ray, grid = ceil(n / block)     */
-    CRT_KERNEL_PERSISTENT = 2   /* device-filling waves that refill idle lanes from a
-                                   ray counter and batch leaf tests (stackless only) */
+    CRT_KERNEL_PERSISTENT = 2,  /* device-filling waves claiming 64 rays at a time from a
+                                   counter; leaf batching + node window (stackless)  */
+    CRT_KERNEL_WAVE = 3         /* one ray per lane, leaf batching + node window,
+                                   XCD-aware block order (stackless)                 */
 };
 
 typedef struct crt_trace_config {
@@ -255,7 +257,7 @@ typedef struct crt_trace_config {
     int32_t lds_stack;     /* stack format: entries kept in LDS, one of 8, 16, 32, 48,
                               64 (0 = 16); deeper entries live in per-lane scratch */
     int32_t kernel;        /* CRT_KERNEL_* */
-    int32_t variant;       /* persistent: refill / inner-step / leaf-batch preset 0..5 */
+    int32_t variant;       /* persistent / wave: inner-step, leaf-batch, window preset 0..5 */
     int32_t grid;          /* persistent: workgroups; 0 = enough to fill every CU */
     int32_t reserved;
     uint32_t* tri_tests;   /* optional device array [nrays]: the counting build of the

@@ -69,7 +69,7 @@ def gpu_trace(crt, scene, query, rays, flags=0, block=0, lds=0, kernel=0, varian
 
 # (format, kernel) pairs: the one-ray-per-lane kernel walks both formats, the
 # persistent refill kernel the stackless one.
-WALKERS = [(0, 1), (0, 2), (1, 1)]
+WALKERS = [(0, 1), (0, 2), (0, 3), (1, 1)]
 
 
 @pytest.mark.parametrize("fmt", [0, 1])
@@ -163,6 +163,8 @@ def test_launch_shapes_do_not_change_results(gpu, orc):
             for q in (0, 1):
                 oq = oh if q == 0 else orc.trace(fmt, b.nodes, b.triangles, b.vertices, ents, 1, rays)
                 for variant in range(6):
+                    gh = gpu_trace(gpu, s, q, rays, kernel=3, variant=variant)
+                    compare(gh, oq, f"wave q={q} variant={variant}")
                     for grid in (0, 1, 7, 33):
                         gh = gpu_trace(gpu, s, q, rays, kernel=2, variant=variant, grid=grid)
                         compare(gh, oq, f"persistent q={q} variant={variant} grid={grid}")
@@ -180,7 +182,7 @@ def test_guards_on_gpu(gpu, orc):
     for fmt, nodes, flag in cases:
         ents = oracle.identity_entity(nodes.size)
         s, keep = device_scene(gpu, fmt, nodes, tris, v, ents)
-        for kernel, lds in (((1, 0), (2, 0)) if fmt == 0 else ((1, 8), (1, 16), (1, 64))):
+        for kernel, lds in (((1, 0), (2, 0), (3, 0)) if fmt == 0 else ((1, 8), (1, 16), (1, 64))):
             gh = gpu_trace(gpu, s, 0, ray, lds=lds, kernel=kernel)
             oh = orc.trace(fmt, nodes, tris, v, ents, 0, ray)
             assert (oh["status"] & 0xFF000000 == flag).all()
@@ -200,7 +202,7 @@ def test_empty_and_ragged_batches(gpu, orc):
         if n:
             dr.upload(np.concatenate([rays]))
         dh = gpu.DeviceArray(gpu.HIT_DTYPE, max(n, 1))
-        for kernel in (1, 2):
+        for kernel in (1, 2, 3):
             dh.fill_bytes(0x7F)
             gpu.trace(s, 0, dr, dh, n=n, kernel=kernel)
             gpu.lib().crt_device_sync()
