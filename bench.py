@@ -83,6 +83,69 @@ def measured_traffic(best: dict, args, fmt: int):
             return float(d["traffic_bytes"]), os.path.relpath(path, ROOT)
     return None, None
 
+def time_batched(crt, batches, steps: int, warmup: int, stream, candidates=None) -> dict:
+    """Autotunes the batched kernel on `batches`, then times `steps` hipGraph replays of the best plan."""
+    best, tune_ms = crt.autotune_batched(batches, candidates or default_candidates(crt), repeats=2, stream=stream)
+    plan = crt.BatchPlan(batches, best)
+    for _ in range(max(1, warmup)):
+        plan.launch(stream=stream, graph=True)
+    stream.sync()
+    ev = Events(crt, 2)
+    ev.record(0, stream.handle)
+    for _ in range(steps):
+        plan.launch(stream=stream, graph=True)
+    ev.record(1, stream.handle)
+    ms = ev.ms(0, 1) / steps
+    ev.close()
+    cfg = {k: getattr(best, k) for k in ("variant", "tiles_per_claim", "waves_per_group", "order")}
+    return {"ms": ms, "config": cfg, "tune_ms": tune_ms, "rays": plan.rays}
+
+
+def default_candidates(crt):
+    return [crt.BatchConfig(v, t, w, 0, o, 0) for v in (0, 1, 4) for t in (1, 2) for w in (1, 4) for o in (0, 1)]
+
+
+def batched_vs_plain(crt, args, best_plain: dict, stream) -> dict:
+    """64 scenes (log-uniform 1k..2M triangles) x (primary, GI, shadow) batches: plain launches vs one launch."""
+    from candela_amd import workloads
+    t0 = time.perf_counter()
+    work = workloads.batched_scenes(crt, count=args.batched_scenes, seed=3)
+    log(f"batched workload: {len(work.scenes)} scenes, {work.triangles} triangles, {work.rays} rays "
+        f"(BVH builds {work.build_s:.1f}s, setup {time.perf_counter() - t0:.1f}s)")
+    K = args.batched_steps
+    kw = dict(kernel=best_plain["kernel"], variant=best_plain["variant"], stream=stream)
+
+    def plain_step():
+        for s in work.scenes:   # one launch per (scene, query): primary+GI closest, shadow any
+            crt.trace(s.scene, crt.QUERY_CLOSEST, s.d_closest, s.h_closest, **kw)
+            crt.trace(s.scene, crt.QUERY_ANY, s.d_shadow, s.h_shadow, **kw)
+
+    for _ in range(max(1, args.warmup)):
+        plain_step()
+    stream.sync()
+    ev = Events(crt, 2)
+    ev.record(0, stream.handle)
+    for _ in range(K):
+        plain_step()
+    ev.record(1, stream.handle)
+    plain_ms = ev.ms(0, 1) / K
+    ev.close()
+    plain_hits = [(s.h_closest.download(), s.h_shadow.download()) for s in work.scenes]
+    b = time_batched(crt, work.batches(crt), K, args.warmup, stream)
+    same = all(s.h_closest.download().tobytes() == ph[0].tobytes() and
+               s.h_shadow.download().tobytes() == ph[1].tobytes() for s, ph in zip(work.scenes, plain_hits))
+    plain_mr = work.rays / (plain_ms * 1e-3) / 1e6
+    batched_mr = work.rays / (b["ms"] * 1e-3) / 1e6
+    log(f"batched vs plain: plain {2 * len(work.scenes)} launches {plain_ms:.3f} ms ({plain_mr:.1f} Mrays/s), "
+        f"batched one launch {b['ms']:.3f} ms ({batched_mr:.1f} Mrays/s) -> {plain_ms / b['ms']:.2f}x; "
+        f"identical hits: {same}")
+    return {"workload": f"{len(work.scenes)} icosphere clouds, {work.triangles} triangles, {work.rays} rays "
+                        "(40% primary, 40% GI closest, 20% shadow any-hit)",
+            "plain_launches": 2 * len(work.scenes), "plain_ms": round(plain_ms, 4),
+            "plain_mrays_s": round(plain_mr, 2), "batched_ms": round(b["ms"], 4),
+            "batched_mrays_s": round(batched_mr, 2), "speedup": round(plain_ms / b["ms"], 3),
+            "batched_config": b["config"], "hits_identical": bool(same)}
+
 
 def copy_probe(crt, args) -> int:
     """Streaming copy of a buffer of known size: measured HBM read+write bandwidth."""
@@ -117,6 +180,9 @@ def main(argv=None) -> int:
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline (profiling runs)")
     ap.add_argument("--spheres", type=int, default=2000)
+    ap.add_argument("--skip-batched", action="store_true", help="skip the batched-vs-plain comparison (64 scenes)")
+    ap.add_argument("--batched-scenes", type=int, default=64)
+    ap.add_argument("--batched-steps", type=int, default=5)
     ap.add_argument("--only", default="", metavar="KERNEL:VARIANT",
                     help="time one kernel choice only (e.g. 1:0 one ray per lane, 2:3 persistent preset 3)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
@@ -236,6 +302,16 @@ def main(argv=None) -> int:
         f"{out['gi']['bytes'] / n_gi:.0f} B/ray, {achieved:.0f} GB/s algorithmic; "
         f"iteration-capped {out['gi']['capped'] * 100:.2f}%")
 
+    # --- batched persistent kernel: config-2 batches and 64 heterogeneous scenes ---------------
+    batched = None
+    c2_batched = None
+    if not args.skip_batched and fmt == crt.FORMAT_STACKLESS:
+        c2_batched = time_batched(crt, [(scene, crt.QUERY_ANY, d_ao, h_ao), (scene, crt.QUERY_CLOSEST, d_gi, h_gi)],
+                                  K, args.warmup, stream)
+        log(f"config-2 batches in one persistent launch: {c2_batched['ms']:.3f} ms/step -> "
+            f"{rays_step / (c2_batched['ms'] * 1e-3) / 1e6:.1f} Mrays/s ({c2_batched['config']})")
+        batched = batched_vs_plain(crt, args, best, stream)
+
     # --- CPU baseline on a bounded sample --------------------------------------------------
     cpu = None
     parity = None
@@ -297,7 +373,8 @@ def main(argv=None) -> int:
         },
         "plain_mrays_s": round(value, 2),
         "thread_per_ray_mrays_s": round(thread["mrays"], 2),
-        "batched_mrays_s": None,
+        "batched_mrays_s": (round(rays_step / (c2_batched["ms"] * 1e-3) / 1e6, 2) if c2_batched else None),
+        "batched_vs_plain": batched,
         "kernels": [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()} for r in results],
         "roofline": {"bound": "hbm", "kernel": KERNEL_NAMES[best["kernel"]] + " (GI closest hit)", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),

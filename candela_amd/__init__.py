@@ -16,7 +16,7 @@ from ._lib import (KERNEL_AUTO, KERNEL_PERSISTENT, KERNEL_THREAD_PER_RAY, KERNEL
                    FORMAT_STACKLESS, HIT_DTYPE, QUERY_ANY, QUERY_CLOSEST, RAY_DTYPE, STACK_NODE_DTYPE,
                    STACKLESS_NODE_DTYPE, STATUS_BAD_INDEX, STATUS_ITER_CAP, STATUS_STACK_OVERFLOW,
                    STATUS_VISITS_MASK, SWAP_NONE, SWAP_SEEDED, TRIANGLE_DTYPE, VERTEX_DTYPE, BuildOptions,
-                   BvhStats, CrtError, Mesh, SceneView, TraceConfig, check, lib, node_dtype, ptr)
+                   BvhStats, CrtError, Mesh, SceneView, TraceConfig, Batch, BatchConfig, check, lib, node_dtype, ptr)
 
 __all__ = [
     "VERTEX_DTYPE", "TRIANGLE_DTYPE", "STACKLESS_NODE_DTYPE", "STACK_NODE_DTYPE", "BOUNDS_DTYPE", "ENTITY_DTYPE",
@@ -24,7 +24,8 @@ __all__ = [
     "FLAG_IGNORE_TRANSPARENT", "FLAG_TRI0_IS_MISS", "STATUS_ITER_CAP", "STATUS_STACK_OVERFLOW", "STATUS_BAD_INDEX",
     "STATUS_VISITS_MASK", "SWAP_NONE", "SWAP_SEEDED", "KERNEL_AUTO", "KERNEL_THREAD_PER_RAY", "KERNEL_PERSISTENT",
     "KERNEL_WAVE", "CrtError", "DeviceArray", "Stream", "BuiltBVH", "build_bvh",
-    "RayIntersector", "trace", "device_count", "node_dtype", "lib",
+    "RayIntersector", "trace", "device_count", "node_dtype", "lib", "Batch", "BatchConfig", "BatchPlan",
+    "autotune_batched",
 ]
 
 
@@ -243,3 +244,58 @@ class RayIntersector:
         check(lib().crt_intersect_primary(self.handle, rays.ptr, hits.ptr, width, height, ptr(iv), ptr(ip), flags,
                                           _stream_handle(stream)))
         return rays, hits
+
+
+class BatchPlan:
+    """crt_batch_plan: every (scene, query, rays, hits) batch in one persistent launch.
+
+    batches: list of (SceneView, query, rays DeviceArray, hits DeviceArray[, flags[, tri_tests DeviceArray]])."""
+
+    def __init__(self, batches, config: BatchConfig | None = None):
+        self._arr = _batch_array(batches)
+        h = C.c_void_p()
+        check(lib().crt_batch_plan_create(self._arr, len(batches), C.byref(config) if config else None,
+                                          C.byref(h)))
+        self.handle = h.value
+        self.rays = int(lib().crt_batch_plan_rays(self.handle))
+
+    def launch(self, stream=None, graph: bool = False) -> None:
+        fn = lib().crt_batch_plan_launch_graph if graph else lib().crt_batch_plan_launch
+        check(fn(self.handle, _stream_handle(stream)))
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            lib().crt_batch_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _batch_array(batches):
+    arr = (Batch * max(1, len(batches)))()
+    for i, b in enumerate(batches):
+        scene, query, rays, hits = b[:4]
+        flags = b[4] if len(b) > 4 else 0
+        tt = b[5] if len(b) > 5 else None
+        arr[i] = Batch(scene, query, flags, rays.ptr, hits.ptr, min(rays.count, hits.count),
+                       tt.ptr if tt is not None else None)
+    return arr
+
+
+def autotune_batched(batches, candidates=None, repeats: int = 2, stream=None):
+    """crt_autotune_batched: the fastest BatchConfig for these batches and its time per launch (ms)."""
+    arr = _batch_array(batches)
+    best = BatchConfig()
+    ms = C.c_float()
+    if candidates:
+        carr = (BatchConfig * len(candidates))(*candidates)
+        check(lib().crt_autotune_batched(arr, len(batches), carr, len(candidates), repeats, _stream_handle(stream),
+                                         C.byref(best), C.byref(ms)))
+    else:
+        check(lib().crt_autotune_batched(arr, len(batches), None, 0, repeats, _stream_handle(stream), C.byref(best),
+                                         C.byref(ms)))
+    return best, float(ms.value)

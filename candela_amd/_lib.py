@@ -79,6 +79,16 @@ class TraceConfig(C.Structure):
                 ("grid", C.c_int32), ("reserved", C.c_int32), ("tri_tests", C.c_void_p)]
 
 
+class Batch(C.Structure):
+    _fields_ = [("scene", SceneView), ("query", C.c_int32), ("flags", C.c_uint32), ("rays", C.c_void_p),
+                ("hits", C.c_void_p), ("count", C.c_int64), ("tri_tests", C.c_void_p)]
+
+
+class BatchConfig(C.Structure):
+    _fields_ = [("variant", C.c_int32), ("tiles_per_claim", C.c_int32), ("waves_per_group", C.c_int32),
+                ("grid", C.c_int32), ("order", C.c_int32), ("reserved", C.c_int32)]
+
+
 class CrtError(RuntimeError):
     pass
 
@@ -129,6 +139,13 @@ SIGNATURES = {
     "crt_intersect": (C.c_int, [C.c_void_p, C.c_int, C.c_uint32, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]),
     "crt_intersect_primary": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p,
                                         C.c_void_p, C.c_uint32, C.c_void_p]),
+    "crt_batch_plan_create": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(BatchConfig), C.POINTER(C.c_void_p)]),
+    "crt_batch_plan_destroy": (None, [C.c_void_p]),
+    "crt_batch_plan_launch": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "crt_batch_plan_launch_graph": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "crt_batch_plan_rays": (C.c_int64, [C.c_void_p]),
+    "crt_autotune_batched": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_void_p,
+                                       C.POINTER(BatchConfig), C.POINTER(C.c_float)]),
 }
 
 

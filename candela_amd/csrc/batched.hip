@@ -1,0 +1,361 @@
+// Batched traversal: many scenes and ray batches in one persistent launch.
+//
+// The plain path traces one (scene, query) batch per launch; a frame with many
+// small batches then pays a launch and a draining tail per batch, and a batch
+// of a few thousand rays cannot fill 256 CUs at all.  Here every batch is cut
+// into 64-ray tiles, one tile per wave, and device-filling waves claim tiles
+// from one atomic counter over a tile table that spans all batches (optionally
+// heaviest tiles first, so the launch ends on short tiles).  A wave traces its
+// tile with the same wave routine as the plain stackless kernels
+// (wave_walk.h), so every hit record is bit-identical to crt_trace on the same
+// batch.  The launch is a counter-reset kernel plus the batched kernel -- no
+// memset and no allocation -- so it can be captured once into a hipGraph and
+// replayed.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <new>
+#include <numeric>
+#include <vector>
+
+#include "candela_rt.h"
+#include "crt_internal.h"
+#include "trace_launch.h"
+#include "wave_walk.h"
+
+namespace crt {
+
+struct BatchDesc {
+    SceneArgs scene;
+    const crt_ray* rays;
+    crt_hit* hits;
+    uint32_t* tri_tests;
+    int64_t count;
+    int32_t query;
+    uint32_t flags;
+};
+
+namespace pdev {
+
+template <int INNER, int LEAF_BATCH, int K, int WPG>
+__global__ void __launch_bounds__(64 * WPG) batched_stackless(const BatchDesc* __restrict__ descs,
+                                                              const int2* __restrict__ tiles, int32_t ntiles,
+                                                              int32_t per_claim, unsigned* __restrict__ counter,
+                                                              int32_t max_iter) {
+    WaveWalk<false, INNER, LEAF_BATCH, K> closest;
+    WaveWalk<true, INNER, LEAF_BATCH, K> any;
+    while (true) {
+        unsigned base = 0;
+        if (__lane_id() == 0) base = atomicAdd(counter, unsigned(per_claim));
+        base = unsigned(__builtin_amdgcn_readfirstlane(int(__shfl(int(base), 0))));
+        if (base >= unsigned(ntiles)) break;
+        for (int32_t k = 0; k < per_claim; ++k) {
+            const int32_t t = int32_t(base) + k;
+            if (t >= ntiles) break;
+            const int2 tile = tiles[t];
+            const int32_t b = __builtin_amdgcn_readfirstlane(tile.x);
+            const int32_t first = __builtin_amdgcn_readfirstlane(tile.y);
+            const BatchDesc d = descs[b];
+            const int64_t i = int64_t(first) + __lane_id();
+            const int64_t slot = i < d.count ? i : -1;
+            if (d.query == CRT_QUERY_ANY)
+                any.run(d.scene, d.rays, d.hits, slot, d.flags, max_iter, d.tri_tests);
+            else
+                closest.run(d.scene, d.rays, d.hits, slot, d.flags, max_iter, d.tri_tests);
+        }
+    }
+}
+
+__global__ void reset_u32(unsigned* c) { *c = 0u; }
+
+}  // namespace pdev
+
+using BatchedFn = void (*)(const BatchDesc*, const int2*, int32_t, int32_t, unsigned*, int32_t);
+
+template <int WPG>
+static const void* batched_symbol(int variant) {
+    switch (variant) {
+        case 1: return reinterpret_cast<const void*>(&pdev::batched_stackless<8, 8, 1, WPG>);
+        case 2: return reinterpret_cast<const void*>(&pdev::batched_stackless<8, 8, 4, WPG>);
+        case 3: return reinterpret_cast<const void*>(&pdev::batched_stackless<16, 16, 2, WPG>);
+        case 4: return reinterpret_cast<const void*>(&pdev::batched_stackless<4, 4, 2, WPG>);
+        case 5: return reinterpret_cast<const void*>(&pdev::batched_stackless<8, 12, 4, WPG>);
+        default: return reinterpret_cast<const void*>(&pdev::batched_stackless<8, 8, 2, WPG>);
+    }
+}
+
+static const void* batched_kernel(int variant, int wpg) {
+    switch (wpg) {
+        case 1: return batched_symbol<1>(variant);
+        case 2: return batched_symbol<2>(variant);
+        default: return batched_symbol<4>(variant);
+    }
+}
+
+}  // namespace crt
+
+struct crt_batch_plan {
+    std::vector<crt_batch> batches;
+    crt_batch_config cfg{};
+    crt::BatchDesc* d_descs = nullptr;
+    int2* d_tiles = nullptr;
+    unsigned* d_counter = nullptr;
+    int32_t ntiles = 0;
+    int64_t rays = 0;
+    int grid = 0;
+    const void* kernel = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph = nullptr;
+
+    ~crt_batch_plan() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (d_descs) (void)hipFree(d_descs);
+        if (d_tiles) (void)hipFree(d_tiles);
+        if (d_counter) (void)hipFree(d_counter);
+    }
+};
+
+namespace {
+
+int hip_err(hipError_t e, const char* what) {
+    return crt::set_error(CRT_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define PLAN_HIP(call, what)                      \
+    do {                                          \
+        hipError_t e_ = (call);                   \
+        if (e_ != hipSuccess) return hip_err(e_, what); \
+    } while (0)
+
+int validate_config(const crt_batch_config& c) {
+    if (c.variant < 0 || c.variant > 5) return crt::set_error(CRT_E_INVALID, "batch config: variant must be 0..5");
+    if (c.tiles_per_claim != 1 && c.tiles_per_claim != 2 && c.tiles_per_claim != 4)
+        return crt::set_error(CRT_E_INVALID, "batch config: tiles_per_claim must be 1, 2 or 4");
+    if (c.waves_per_group != 1 && c.waves_per_group != 2 && c.waves_per_group != 4)
+        return crt::set_error(CRT_E_INVALID, "batch config: waves_per_group must be 1, 2 or 4");
+    if (c.grid < 0 || c.order < 0 || c.order > 1) return crt::set_error(CRT_E_INVALID, "batch config: bad grid/order");
+    return CRT_OK;
+}
+
+crt_batch_config default_config() {
+    crt_batch_config c{};
+    c.variant = 0;
+    c.tiles_per_claim = 1;
+    c.waves_per_group = 4;
+    c.grid = 0;
+    c.order = 1;
+    return c;
+}
+
+// Relative cost of one tile: any-hit walks stop at the first hit; a walk's
+// length grows with the depth of the tree.
+double tile_cost(const crt_batch& b) {
+    const double depth = std::log2(double(std::max<int64_t>(2, b.scene.total_nodes)));
+    return (b.query == CRT_QUERY_ANY ? 1.0 : 2.5) * depth;
+}
+
+int build_plan(const crt_batch* batches, int nbatches, const crt_batch_config& cfg, crt_batch_plan& p) {
+    p.batches.assign(batches, batches + nbatches);
+    p.cfg = cfg;
+    std::vector<crt::BatchDesc> descs(static_cast<size_t>(nbatches));
+    for (int i = 0; i < nbatches; ++i) {
+        const crt_batch& b = batches[i];
+        if (b.scene.format != CRT_FORMAT_STACKLESS)
+            return crt::set_error(CRT_E_INVALID, "batched traversal walks the stackless format only");
+        if (b.query != CRT_QUERY_CLOSEST && b.query != CRT_QUERY_ANY)
+            return crt::set_error(CRT_E_INVALID, "batch: bad query");
+        if (b.count < 0 || (b.count > 0 && (!b.rays || !b.hits)))
+            return crt::set_error(CRT_E_INVALID, "batch: bad ray/hit buffers");
+        if (b.count > 0 && b.scene.entity_count > 0 &&
+            (!b.scene.nodes || !b.scene.triangles || !b.scene.vertices || !b.scene.entities || b.scene.total_nodes <= 0))
+            return crt::set_error(CRT_E_INVALID, "batch: null scene buffer");
+        if (b.count >= (int64_t(1) << 31)) return crt::set_error(CRT_E_INVALID, "batch: >= 2^31 rays");
+        crt::BatchDesc& d = descs[size_t(i)];
+        d.scene = crt::SceneArgs{b.scene.format, b.scene.entity_count, b.scene.total_nodes, b.scene.nodes,
+                                 b.scene.triangles, b.scene.vertices, b.scene.entities};
+        d.rays = b.rays;
+        d.hits = b.hits;
+        d.tri_tests = b.tri_tests;
+        d.count = b.count;
+        d.query = b.query;
+        d.flags = b.flags;
+    }
+    std::vector<int> order(static_cast<size_t>(nbatches));
+    std::iota(order.begin(), order.end(), 0);
+    if (cfg.order == 1)
+        std::stable_sort(order.begin(), order.end(),
+                         [&](int a, int b) { return tile_cost(batches[a]) > tile_cost(batches[b]); });
+    std::vector<int2> tiles;
+    p.rays = 0;
+    for (int bi : order) {
+        const int64_t n = batches[bi].count;
+        p.rays += n;
+        for (int64_t first = 0; first < n; first += 64) tiles.push_back(make_int2(bi, int(first)));
+    }
+    if (tiles.size() >= size_t(1u << 31)) return crt::set_error(CRT_E_INVALID, "batch plan: too many tiles");
+    p.ntiles = int32_t(tiles.size());
+    PLAN_HIP(hipMalloc(&p.d_descs, std::max<size_t>(1, descs.size()) * sizeof(crt::BatchDesc)), "hipMalloc(descs)");
+    PLAN_HIP(hipMalloc(&p.d_tiles, std::max<size_t>(1, tiles.size()) * sizeof(int2)), "hipMalloc(tiles)");
+    PLAN_HIP(hipMalloc(&p.d_counter, sizeof(unsigned)), "hipMalloc(counter)");
+    if (!descs.empty())
+        PLAN_HIP(hipMemcpy(p.d_descs, descs.data(), descs.size() * sizeof(crt::BatchDesc), hipMemcpyHostToDevice),
+                 "hipMemcpy(descs)");
+    if (!tiles.empty())
+        PLAN_HIP(hipMemcpy(p.d_tiles, tiles.data(), tiles.size() * sizeof(int2), hipMemcpyHostToDevice),
+                 "hipMemcpy(tiles)");
+    p.kernel = crt::batched_kernel(cfg.variant, cfg.waves_per_group);
+    int grid = cfg.grid;
+    if (grid == 0) {
+        int dev = 0, cus = 0, per_cu = 0;
+        PLAN_HIP(hipGetDevice(&dev), "hipGetDevice");
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 256;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, p.kernel, 64 * cfg.waves_per_group, 0) !=
+                hipSuccess ||
+            per_cu <= 0)
+            per_cu = 16 / cfg.waves_per_group;
+        grid = cus * per_cu;
+    }
+    // no more waves than tiles
+    const int64_t max_groups = (int64_t(p.ntiles) + cfg.waves_per_group - 1) / cfg.waves_per_group;
+    p.grid = int(std::max<int64_t>(1, std::min<int64_t>(grid, max_groups)));
+    return CRT_OK;
+}
+
+int launch_plan(crt_batch_plan& p, hipStream_t stream) {
+    if (p.ntiles == 0) return CRT_OK;
+    hipLaunchKernelGGL(crt::pdev::reset_u32, dim3(1), dim3(1), 0, stream, p.d_counter);
+    PLAN_HIP(hipGetLastError(), "batched: reset launch");
+    const crt::BatchDesc* descs = p.d_descs;
+    const int2* tiles = p.d_tiles;
+    int32_t ntiles = p.ntiles;
+    int32_t per_claim = p.cfg.tiles_per_claim;
+    unsigned* counter = p.d_counter;
+    int32_t max_iter = crt::kMaxIterations;
+    void* args[] = {&descs, &tiles, &ntiles, &per_claim, &counter, &max_iter};
+    PLAN_HIP(hipLaunchKernel(p.kernel, dim3(unsigned(p.grid)), dim3(unsigned(64 * p.cfg.waves_per_group)), args, 0,
+                             stream),
+             "batched: kernel launch");
+    return CRT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int crt_batch_plan_create(const crt_batch* batches, int nbatches, const crt_batch_config* config,
+                          crt_batch_plan** out) {
+    if (!out) return crt::set_error(CRT_E_INVALID, "crt_batch_plan_create: out is null");
+    *out = nullptr;
+    if (nbatches < 0 || (nbatches > 0 && !batches))
+        return crt::set_error(CRT_E_INVALID, "crt_batch_plan_create: bad batch list");
+    const crt_batch_config cfg = config ? *config : default_config();
+    int rc = validate_config(cfg);
+    if (rc) return rc;
+    crt_batch_plan* p = new (std::nothrow) crt_batch_plan();
+    if (!p) return crt::set_error(CRT_E_NOMEM, "crt_batch_plan_create: out of host memory");
+    rc = build_plan(batches, nbatches, cfg, *p);
+    if (rc) {
+        delete p;
+        return rc;
+    }
+    *out = p;
+    return CRT_OK;
+}
+
+void crt_batch_plan_destroy(crt_batch_plan* plan) { delete plan; }
+
+int crt_batch_plan_launch(crt_batch_plan* plan, void* stream) {
+    if (!plan) return crt::set_error(CRT_E_INVALID, "crt_batch_plan_launch: plan is null");
+    return launch_plan(*plan, static_cast<hipStream_t>(stream));
+}
+
+int crt_batch_plan_launch_graph(crt_batch_plan* plan, void* stream) {
+    if (!plan) return crt::set_error(CRT_E_INVALID, "crt_batch_plan_launch_graph: plan is null");
+    if (!plan->exec) {
+        // Stream capture needs a non-default stream; the graph is then launched
+        // on the caller's stream.
+        hipStream_t cap = nullptr;
+        PLAN_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking), "hipStreamCreateWithFlags");
+        hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+        int rc = CRT_OK;
+        if (e == hipSuccess) {
+            rc = launch_plan(*plan, cap);
+            hipGraph_t g = nullptr;
+            const hipError_t e2 = hipStreamEndCapture(cap, &g);
+            if (rc == CRT_OK && e2 != hipSuccess) rc = hip_err(e2, "hipStreamEndCapture");
+            if (rc == CRT_OK) {
+                plan->graph = g;
+                const hipError_t e3 = hipGraphInstantiate(&plan->exec, g, nullptr, nullptr, 0);
+                if (e3 != hipSuccess) rc = hip_err(e3, "hipGraphInstantiate");
+            } else if (g) {
+                (void)hipGraphDestroy(g);
+            }
+        } else {
+            rc = hip_err(e, "hipStreamBeginCapture");
+        }
+        (void)hipStreamDestroy(cap);
+        if (rc) return rc;
+    }
+    PLAN_HIP(hipGraphLaunch(plan->exec, static_cast<hipStream_t>(stream)), "hipGraphLaunch");
+    return CRT_OK;
+}
+
+int64_t crt_batch_plan_rays(const crt_batch_plan* plan) { return plan ? plan->rays : 0; }
+
+int crt_autotune_batched(const crt_batch* batches, int nbatches, const crt_batch_config* candidates,
+                         int ncandidates, int repeats, void* stream, crt_batch_config* best, float* best_ms) {
+    if (!best || !best_ms) return crt::set_error(CRT_E_INVALID, "crt_autotune_batched: null output");
+    if (repeats < 1) repeats = 1;
+    std::vector<crt_batch_config> cands;
+    if (candidates && ncandidates > 0) {
+        cands.assign(candidates, candidates + ncandidates);
+    } else {
+        for (int v = 0; v < 6; ++v)
+            for (int t : {1, 2, 4})
+                for (int w : {1, 4})
+                    for (int o : {0, 1}) {
+                        crt_batch_config c{};
+                        c.variant = v;
+                        c.tiles_per_claim = t;
+                        c.waves_per_group = w;
+                        c.order = o;
+                        cands.push_back(c);
+                    }
+    }
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    PLAN_HIP(hipEventCreate(&e0), "hipEventCreate");
+    PLAN_HIP(hipEventCreate(&e1), "hipEventCreate");
+    float best_t = -1.0f;
+    int rc = CRT_OK;
+    for (const crt_batch_config& c : cands) {
+        crt_batch_plan* p = nullptr;
+        rc = crt_batch_plan_create(batches, nbatches, &c, &p);
+        if (rc) break;
+        rc = launch_plan(*p, s);  // warm-up
+        if (!rc && hipEventRecord(e0, s) != hipSuccess) rc = crt::set_error(CRT_E_HIP, "hipEventRecord");
+        for (int r = 0; !rc && r < repeats; ++r) rc = launch_plan(*p, s);
+        if (!rc && hipEventRecord(e1, s) != hipSuccess) rc = crt::set_error(CRT_E_HIP, "hipEventRecord");
+        float ms = 0.0f;
+        if (!rc && (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+            rc = crt::set_error(CRT_E_HIP, "hipEventElapsedTime");
+        delete p;
+        if (rc) break;
+        ms /= float(repeats);
+        if (best_t < 0.0f || ms < best_t) {
+            best_t = ms;
+            *best = c;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    *best_ms = best_t;
+    return CRT_OK;
+}
+
+}  // extern "C"

@@ -326,6 +326,55 @@ int crt_intersect_primary(crt_context* ctx, crt_ray* rays, crt_hit* hits, int wi
                           const float inv_view[16], const float inv_proj[16], uint32_t flags,
                           void* stream);
 
+/* ------------------------------------------------------------------------- */
+/* Batched traversal: many scenes and ray batches in one persistent launch    */
+/* ------------------------------------------------------------------------- */
+
+/* One ray batch: a device scene (stackless format), a query, device rays and
+ * hits.  The batch is traced exactly as crt_trace would trace it. */
+typedef struct crt_batch {
+    crt_scene_view scene;
+    int32_t query;          /* CRT_QUERY_* */
+    uint32_t flags;         /* CRT_FLAG_*  */
+    const crt_ray* rays;
+    crt_hit* hits;
+    int64_t count;
+    uint32_t* tri_tests;    /* optional counting output [count] */
+} crt_batch;
+
+/* Tuning of the batched kernel (crt_autotune_batched searches these). */
+typedef struct crt_batch_config {
+    int32_t variant;        /* wave-walk preset 0..5 (inner steps, leaf batch, window) */
+    int32_t tiles_per_claim;/* 64-ray tiles a wave claims per atomic: 1, 2 or 4 */
+    int32_t waves_per_group;/* 1, 2 or 4 waves per workgroup */
+    int32_t grid;           /* workgroups; 0 = enough to fill every CU */
+    int32_t order;          /* 0 = batch order, 1 = heaviest tiles first (cost estimate) */
+    int32_t reserved;
+} crt_batch_config;
+
+typedef struct crt_batch_plan crt_batch_plan;
+
+/* Uploads the batch descriptors and the tile table once.  The plan keeps
+ * the device pointers of the batches; their contents may change between
+ * launches (new rays, same sizes). */
+int crt_batch_plan_create(const crt_batch* batches, int nbatches, const crt_batch_config* config,
+                          crt_batch_plan** out);
+void crt_batch_plan_destroy(crt_batch_plan* plan);
+/* One persistent launch over every batch (a counter-reset kernel + the
+ * batched kernel; no memset, no allocation -- capturable). */
+int crt_batch_plan_launch(crt_batch_plan* plan, void* stream);
+/* Captures crt_batch_plan_launch into a hipGraph (once) and replays it. */
+int crt_batch_plan_launch_graph(crt_batch_plan* plan, void* stream);
+/* Total rays of the plan. */
+int64_t crt_batch_plan_rays(const crt_batch_plan* plan);
+
+/* Times every candidate configuration on the batches (`repeats` launches
+ * each, after one warm-up) and returns the fastest in *best and its time per
+ * launch in *best_ms.  candidates == NULL searches the built-in grid:
+ * variants 0..5 x tiles_per_claim {1,2,4} x waves_per_group {1,4} x order {0,1}. */
+int crt_autotune_batched(const crt_batch* batches, int nbatches, const crt_batch_config* candidates,
+                         int ncandidates, int repeats, void* stream, crt_batch_config* best, float* best_ms);
+
 #ifdef __cplusplus
 }
 #endif
