@@ -16,7 +16,8 @@ from ._lib import (KERNEL_AUTO, KERNEL_PERSISTENT, KERNEL_THREAD_PER_RAY, KERNEL
                    FORMAT_STACKLESS, HIT_DTYPE, QUERY_ANY, QUERY_CLOSEST, RAY_DTYPE, STACK_NODE_DTYPE,
                    STACKLESS_NODE_DTYPE, STATUS_BAD_INDEX, STATUS_ITER_CAP, STATUS_STACK_OVERFLOW,
                    STATUS_VISITS_MASK, SWAP_NONE, SWAP_SEEDED, TRIANGLE_DTYPE, VERTEX_DTYPE, BuildOptions,
-                   BvhStats, CrtError, Mesh, SceneView, TraceConfig, Batch, BatchConfig, check, lib, node_dtype, ptr)
+                   BvhStats, CrtError, Mesh, SceneView, TraceConfig, Batch, BatchConfig, ShadeParams,
+                   SHADE_RECORD_DTYPE, check, lib, node_dtype, ptr)
 
 __all__ = [
     "VERTEX_DTYPE", "TRIANGLE_DTYPE", "STACKLESS_NODE_DTYPE", "STACK_NODE_DTYPE", "BOUNDS_DTYPE", "ENTITY_DTYPE",
@@ -25,7 +26,7 @@ __all__ = [
     "STATUS_VISITS_MASK", "SWAP_NONE", "SWAP_SEEDED", "KERNEL_AUTO", "KERNEL_THREAD_PER_RAY", "KERNEL_PERSISTENT",
     "KERNEL_WAVE", "CrtError", "DeviceArray", "Stream", "BuiltBVH", "build_bvh",
     "RayIntersector", "trace", "device_count", "node_dtype", "lib", "Batch", "BatchConfig", "BatchPlan",
-    "autotune_batched",
+    "autotune_batched", "ShadeParams", "SHADE_RECORD_DTYPE", "ShadingModel",
 ]
 
 
@@ -299,3 +300,40 @@ def autotune_batched(batches, candidates=None, repeats: int = 2, stream=None):
         check(lib().crt_autotune_batched(arr, len(batches), None, 0, repeats, _stream_handle(stream), C.byref(best),
                                          C.byref(ms)))
     return best, float(ms.value)
+
+
+class ShadingModel:
+    """Differentiable Cook-Torrance + one-bounce shading (crt_shade_forward / crt_shade_backward).
+
+    albedo [ntri, 3], roughness [ntri], light_pos [nl, 3], intensity [nl] live on the device."""
+
+    def __init__(self, albedo, roughness, light_pos, intensity):
+        self.ntri = int(np.asarray(roughness).size)
+        self.nl = int(np.asarray(intensity).size)
+        self.d_albedo = DeviceArray.from_host(np.ascontiguousarray(albedo, np.float32).reshape(-1))
+        self.d_rough = DeviceArray.from_host(np.ascontiguousarray(roughness, np.float32))
+        self.d_lpos = DeviceArray.from_host(np.ascontiguousarray(light_pos, np.float32).reshape(-1))
+        self.d_inten = DeviceArray.from_host(np.ascontiguousarray(intensity, np.float32))
+
+    def params(self) -> ShadeParams:
+        return ShadeParams(self.d_albedo.ptr, self.d_rough.ptr, self.d_lpos.ptr, self.d_inten.ptr, self.ntri, self.nl)
+
+    def set(self, albedo=None, roughness=None, intensity=None):
+        if albedo is not None:
+            self.d_albedo.upload(np.ascontiguousarray(albedo, np.float32).reshape(-1))
+        if roughness is not None:
+            self.d_rough.upload(np.ascontiguousarray(roughness, np.float32))
+        if intensity is not None:
+            self.d_inten.upload(np.ascontiguousarray(intensity, np.float32))
+
+    def forward(self, scene: SceneView, rays: DeviceArray, hits: DeviceArray, color: DeviceArray,
+                records: DeviceArray, seed: int = 4, stream=None) -> None:
+        p = self.params()
+        check(lib().crt_shade_forward(C.byref(scene), rays.ptr, hits.ptr, rays.count, C.byref(p), seed & 0xFFFFFFFF,
+                                      color.ptr, records.ptr, _stream_handle(stream)))
+
+    def backward(self, scene: SceneView, records: DeviceArray, color: DeviceArray, target: DeviceArray,
+                 g_albedo: DeviceArray, g_rough: DeviceArray, g_inten: DeviceArray, stream=None) -> None:
+        p = self.params()
+        check(lib().crt_shade_backward(C.byref(scene), records.ptr, records.count, C.byref(p), color.ptr, target.ptr,
+                                       g_albedo.ptr, g_rough.ptr, g_inten.ptr, _stream_handle(stream)))

@@ -89,6 +89,17 @@ class BatchConfig(C.Structure):
                 ("grid", C.c_int32), ("order", C.c_int32), ("reserved", C.c_int32)]
 
 
+class ShadeParams(C.Structure):
+    _fields_ = [("albedo", C.c_void_p), ("roughness", C.c_void_p), ("light_pos", C.c_void_p),
+                ("intensity", C.c_void_p), ("ntri", C.c_int32), ("nlights", C.c_int32)]
+
+
+SHADE_RECORD_DTYPE = np.dtype([("p", "<f4", 3), ("tri0", "<i4"), ("n", "<f4", 3), ("tri1", "<i4"),
+                               ("eye", "<f4", 3), ("vis", "<u4"), ("q", "<f4", 3), ("pad0", "<f4"),
+                               ("nq", "<f4", 3), ("pad1", "<f4")])
+assert SHADE_RECORD_DTYPE.itemsize == 80
+
+
 class CrtError(RuntimeError):
     pass
 
@@ -144,6 +155,10 @@ SIGNATURES = {
     "crt_batch_plan_launch": (C.c_int, [C.c_void_p, C.c_void_p]),
     "crt_batch_plan_launch_graph": (C.c_int, [C.c_void_p, C.c_void_p]),
     "crt_batch_plan_rays": (C.c_int64, [C.c_void_p]),
+    "crt_shade_forward": (C.c_int, [C.POINTER(SceneView), C.c_void_p, C.c_void_p, C.c_int64, C.POINTER(ShadeParams),
+                                    C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "crt_shade_backward": (C.c_int, [C.POINTER(SceneView), C.c_void_p, C.c_int64, C.POINTER(ShadeParams), C.c_void_p,
+                                     C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "crt_autotune_batched": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_void_p,
                                        C.POINTER(BatchConfig), C.POINTER(C.c_float)]),
 }

@@ -209,3 +209,45 @@ def _surface_rays(prays, phits, positions, tris_flat, n: int, seed: int, box: fl
         out["tmax"] = np.float32(1e6)
     out["id"] = np.arange(n, dtype=np.uint32)
     return out
+
+
+@dataclass
+class ShadingProblem:
+    intersector: object
+    scene: object
+    host: dict
+    rays: object          # primary rays (device)
+    hits: object          # primary hits (device)
+    model: object         # ShadingModel with the parameters theta
+    albedo: np.ndarray
+    roughness: np.ndarray
+    light_pos: np.ndarray
+    intensity: np.ndarray
+    target_albedo: np.ndarray
+    target_roughness: np.ndarray
+    target_intensity: np.ndarray
+
+
+def shading_problem(crt, size: int = 1024, seed: int = 4, nlights: int = 8, fmt: int = FORMAT_STACKLESS):
+    """The 16x16 boxes scene at size x size with per-triangle albedo U[0.05,0.95]^3 and roughness
+    U[0.05,1], `nlights` point lights (intensity U[1,10]) above the grid; the target parameters are
+    theta + N(0, 0.1) (roughness kept >= 0.02)."""
+    rng = np.random.default_rng(seed)
+    r = crt.RayIntersector(fmt)
+    r.add_object(0, scenes.boxes_scene())
+    r.push_entity(0)
+    r.buffer_entities()
+    r.buffer_data()
+    hb = r.host_buffers()
+    ntri = int(hb["triangles"].size)
+    iv, ip = scenes.camera_inverses(aspect=1.0, **scenes.BOXES_CAMERA)
+    drays, dhits = r.intersect_primary(size, size, iv, ip)
+    albedo = rng.uniform(0.05, 0.95, size=(ntri, 3))
+    rough = rng.uniform(0.05, 1.0, size=ntri)
+    lpos = rng.uniform([-4.0, 6.0, -4.0], [20.0, 14.0, 20.0], size=(nlights, 3))
+    inten = rng.uniform(1.0, 10.0, size=nlights)
+    t_alb = albedo + rng.normal(0.0, 0.1, size=albedo.shape)
+    t_rough = np.maximum(rough + rng.normal(0.0, 0.1, size=rough.shape), 0.02)
+    t_inten = inten + rng.normal(0.0, 0.1, size=inten.shape)
+    model = crt.ShadingModel(albedo, rough, lpos, inten)
+    return ShadingProblem(r, r.scene(), hb, drays, dhits, model, albedo, rough, lpos, inten, t_alb, t_rough, t_inten)

@@ -375,6 +375,53 @@ int64_t crt_batch_plan_rays(const crt_batch_plan* plan);
 int crt_autotune_batched(const crt_batch* batches, int nbatches, const crt_batch_config* candidates,
                          int ncandidates, int repeats, void* stream, crt_batch_config* best, float* best_ms);
 
+/* ------------------------------------------------------------------------- */
+/* Differentiable shading: Cook-Torrance direct light + one diffuse bounce    */
+/* ------------------------------------------------------------------------- */
+
+/* Per-pixel record of the forward pass (80 B): the primary hit and its bounce
+ * hit, the geometry the backward pass reuses.  Visibility is not reused: the
+ * backward pass re-traces every shadow ray; `vis` holds the forward pass's
+ * visibility bits for inspection only (bit l: light l visible from p, bit
+ * 16+l: from q). */
+typedef struct crt_shade_record {
+    float p[3];     int32_t tri0;   /* primary hit point; hit triangle, -1 on a miss  */
+    float n[3];     int32_t tri1;   /* face normal at p toward the eye; bounce triangle */
+    float eye[3];   uint32_t vis;   /* primary ray origin; forward visibility bits      */
+    float q[3];     float pad0;     /* bounce hit point                                  */
+    float nq[3];    float pad1;     /* face normal at q toward p                         */
+} crt_shade_record;
+
+/* Shading parameters (device pointers).  Cook-Torrance as in
+ * Source/Core/Shaders/Include/CookTorranceBRDF.glsl:37-79 with metalness 0,
+ * per-triangle albedo and roughness, white point lights of radiance I/d^2. */
+typedef struct crt_shade_params {
+    const float* albedo;      /* [ntri * 3] */
+    const float* roughness;   /* [ntri]     */
+    const float* light_pos;   /* [nlights * 3] */
+    const float* intensity;   /* [nlights]  */
+    int32_t ntri;
+    int32_t nlights;          /* 1..16 */
+} crt_shade_params;
+
+/* Forward: for every pixel with a primary hit, traces one cosine-weighted
+ * bounce ray (closest hit, per-pixel PCG seeded by `seed`) and the shadow rays
+ * toward every light from p and from the bounce point (any hit); writes the
+ * RGB color [npix * 3] and the per-pixel records.  Pixels whose primary ray
+ * missed are black. */
+int crt_shade_forward(const crt_scene_view* scene, const crt_ray* primary_rays, const crt_hit* primary_hits,
+                      int64_t npix, const crt_shade_params* params, uint32_t seed, float* color,
+                      crt_shade_record* records, void* stream);
+
+/* Backward (adjoint) of loss = sum over pixels and channels of (color -
+ * target)^2 with geometry and visibility held fixed: re-traces the shadow
+ * rays of every record, then ADDS d(loss)/d(albedo) [ntri*3],
+ * d(loss)/d(roughness) [ntri] and d(loss)/d(intensity) [nlights] into the
+ * gradient buffers (zero them first). */
+int crt_shade_backward(const crt_scene_view* scene, const crt_shade_record* records, int64_t npix,
+                       const crt_shade_params* params, const float* color, const float* target,
+                       float* grad_albedo, float* grad_roughness, float* grad_intensity, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

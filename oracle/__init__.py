@@ -62,6 +62,15 @@ def lib() -> C.CDLL:
         l.oracle_bounds_stackless.argtypes = [vp, vp, vp, vp, f32, f32]
         l.oracle_bounds_stack.restype = f32
         l.oracle_bounds_stack.argtypes = [vp, vp, vp, vp, f32]
+        f64 = C.c_double
+        l.oracle_shade_forward.restype = None
+        l.oracle_shade_forward.argtypes = [i64, vp, vp, vp, vp, vp, C.c_int, vp]
+        l.oracle_shade_adjoint.restype = f64
+        l.oracle_shade_adjoint.argtypes = [i64, vp, vp, vp, vp, vp, C.c_int, vp, C.c_int, vp, vp, vp]
+        l.oracle_shade_fd.restype = None
+        l.oracle_shade_fd.argtypes = [i64, vp, vp, vp, vp, vp, C.c_int, vp, C.c_int, vp, vp, C.c_int, f64, vp]
+        l.oracle_cook_torrance.restype = None
+        l.oracle_cook_torrance.argtypes = [vp, vp, vp, vp, f64, f64, vp]
         _lib = l
     return _lib
 
@@ -167,3 +176,54 @@ def ref_build(fmt: int, verts: np.ndarray, indices: np.ndarray, mesh_ids: np.nda
     tris = np.frombuffer(raw, TRIANGLE_DTYPE, n_tri, off).copy()
     return nodes, tris, {"array_length": hdr[0], "triangle_count": hdr[1], "node_count": hdr[2],
                          "leaf_count": hdr[3]}
+
+
+SHADE_RECORD_DTYPE = np.dtype([("p", "<f4", 3), ("tri0", "<i4"), ("n", "<f4", 3), ("tri1", "<i4"),
+                               ("eye", "<f4", 3), ("vis", "<u4"), ("q", "<f4", 3), ("pad0", "<f4"),
+                               ("nq", "<f4", 3), ("pad1", "<f4")])
+
+
+def _d(a):
+    return np.ascontiguousarray(a, np.float64)
+
+
+def shade_forward(rec, albedo, rough, lpos, inten):
+    """FP64 colors [n, 3] of the records (geometry and visibility from the records)."""
+    rec = np.ascontiguousarray(rec, SHADE_RECORD_DTYPE)
+    a, r, lp, it = _d(albedo), _d(rough), _d(lpos), _d(inten)
+    out = np.empty((rec.size, 3), np.float64)
+    lib().oracle_shade_forward(rec.size, _p(rec), _p(a), _p(r), _p(lp), _p(it), it.size, _p(out))
+    return out
+
+
+def shade_adjoint(rec, albedo, rough, lpos, inten, target):
+    """(loss, d/d albedo [ntri*3], d/d roughness [ntri], d/d intensity [nl]) in FP64."""
+    rec = np.ascontiguousarray(rec, SHADE_RECORD_DTYPE)
+    a, r, lp, it = _d(albedo), _d(rough), _d(lpos), _d(inten)
+    t = np.ascontiguousarray(target, np.float32)
+    ntri = r.size
+    ga, gr, gi = np.empty(3 * ntri), np.empty(ntri), np.empty(it.size)
+    loss = lib().oracle_shade_adjoint(rec.size, _p(rec), _p(a), _p(r), _p(lp), _p(it), it.size, _p(t), ntri,
+                                      _p(ga), _p(gr), _p(gi))
+    return float(loss), ga, gr, gi
+
+
+def shade_fd(rec, albedo, rough, lpos, inten, target, kind, index, h=1e-3):
+    """Central finite differences of the FP64 loss for parameters (kind, index)."""
+    rec = np.ascontiguousarray(rec, SHADE_RECORD_DTYPE)
+    a, r, lp, it = _d(albedo), _d(rough), _d(lpos), _d(inten)
+    t = np.ascontiguousarray(target, np.float32)
+    kind = np.ascontiguousarray(kind, np.int32)
+    index = np.ascontiguousarray(index, np.int32)
+    out = np.empty(kind.size, np.float64)
+    lib().oracle_shade_fd(rec.size, _p(rec), _p(a), _p(r), _p(lp), _p(it), it.size, _p(t), r.size, _p(kind),
+                          _p(index), kind.size, h, _p(out))
+    return out
+
+
+def cook_torrance(x, n, eye, light, intensity, r):
+    """(A, kd, spec, dspec/dr) of one light, FP64."""
+    arrs = [_d(v) for v in (x, n, eye, light)]
+    out = np.empty(4, np.float64)
+    lib().oracle_cook_torrance(*[_p(v) for v in arrs], float(intensity), float(r), _p(out))
+    return out
