@@ -3,7 +3,9 @@
 Checks, in order:
   1. the forward pass's shadow visibility equals the host traversal oracle on
      the same shadow rays (bit-exact traversal);
-  2. the FP32 GPU colors equal the FP64 oracle forward to 1e-4 relative;
+  2. the FP32 GPU colors equal the FP64 oracle forward to 5e-4 relative (the
+     GGX NDF at roughness r is steep: an FP32 half vector off by delta ~ 1e-7
+     rad moves D by up to 2 delta / r^2, 1e-4 at r = 0.05);
   3. the GPU adjoint equals the FP64 CPU adjoint (1e-3 relative);
   4. the GPU adjoint equals FP64 central finite differences with h = 1e-3 on
      256 random parameters (1e-3 relative).  Where the h = 1e-3 difference has
@@ -89,7 +91,7 @@ def test_forward_matches_fp64(problem, orc):
     rel = np.abs(got - ref)[big] / np.abs(ref)[big]
     log(f"[shade] forward: max rel err {rel.max():.2e} over {big.sum()} channels, "
         f"hit pixels {np.mean(rec['tri0'] >= 0):.3f}, bounce hits {np.mean(rec['tri1'] >= 0):.3f}")
-    assert rel.max() < 1e-4
+    assert rel.max() < 5e-4
     assert np.abs(got - ref)[~big].max() < 1e-6 * np.abs(ref).max()
 
 
