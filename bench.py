@@ -9,10 +9,14 @@ GI closest-hit ray (tmax 1e6), origins offset by 1e-4 along the face normal,
 directions from a per-pixel PCG hash.  A step traces both batches (all rays
 resident in HBM before timing starts); value = rays traced / second.
 
-The dominant kernel (GI closest hit) is priced against HBM bandwidth with the
-algorithmic bytes of every ray counted exactly by the counting build of the
-kernel: 32 B per stackless node visit (64 B per stack node), 64 B per triangle
-test (16-B triangle + three 16-B positions), 32 B ray read + 32 B hit write.
+The candidates are the BVH2 kernels (one ray per lane; wave walk and
+persistent presets) and the one-ray-per-lane walk over the BVH4 / BVH8 layouts
+derived from the same BVH2; value is the best.  The dominant kernel (GI
+closest hit) is priced against HBM bandwidth with the algorithmic bytes of
+every ray counted exactly by the counting build of the kernel: 32 B per
+stackless node visit (64 B per stack node, 128 B per BVH4 node, 256 B per
+BVH8 node), 64 B per triangle test (16-B triangle + three 16-B positions; 48
+B per triangle record in the wide layouts), 32 B ray read + 32 B hit write.
 
 cpu_baseline: the host restatement of the traversal (oracle/, the reference's
 GLSL walk in C) timed on a bounded sample of the same rays.
@@ -66,15 +70,44 @@ class Events:
 
 
 def bytes_per_ray(fmt: int, visits: np.ndarray, tests: np.ndarray) -> np.ndarray:
-    node = 32 if fmt == 0 else 64
-    return node * visits.astype(np.float64) + 64.0 * tests.astype(np.float64) + 64.0
+    """Algorithmic bytes: node bytes per visit + triangle bytes per test + 32-B ray read + 32-B hit write.
+    BVH2: 32-B stackless / 64-B stack node, 64 B per test (16-B triangle + three 16-B positions).
+    Wide: 128 B per group of four children (one group for BVH4, two for BVH8), 48-B triangle record."""
+    if fmt in (2, 3):
+        node, tri = (128.0 if fmt == 2 else 256.0), 48.0
+    else:
+        node, tri = (32.0 if fmt == 0 else 64.0), 64.0
+    return node * visits.astype(np.float64) + tri * tests.astype(np.float64) + 64.0
+
+
+def kernel_label(c: dict) -> str:
+    if c["layout"] == "bvh2":
+        return "plain (one batch per launch): %s, preset %d, BVH2 stackless" % (KERNEL_NAMES[c["kernel"]], c["variant"])
+    return "plain (one batch per launch): trace_kernel over the BVH%d layout derived from the BVH2, %d-triangle " \
+           "leaves" % (c["width"], c["max_leaf"])
+
+
+def choice_name(c: dict) -> str:
+    """'K:V' for a BVH2 kernel choice, 'wW:L' for the wide layout of width W and leaf size L."""
+    if c["layout"] == "bvh2":
+        return f"{c['kernel']}:{c['variant']}"
+    return f"w{c['width']}:{c['max_leaf']}"
+
+
+def wide_scene(crt, hb: dict, d_tris, d_verts, width: int, max_leaf: int):
+    """Derives the wide layout from the host BVH2 buffers and uploads it; returns (SceneView, keep-alive, stats)."""
+    w = crt.wide_build(crt.FORMAT_STACKLESS, hb["nodes"], hb["triangles"], hb["vertices"], hb["entities"], width,
+                       max_leaf)
+    dg, dr, de = (crt.DeviceArray.from_host(a) for a in (w.groups, w.records, w.entities))
+    view = crt.SceneView(w.format, w.entities.size, w.stats["node_count"], dg.ptr, d_tris, d_verts, de.ptr, dr.ptr)
+    return view, (dg, dr, de), w
 
 
 def measured_traffic(best: dict, args, fmt: int):
     """HBM bytes per GI launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs
     (profiles/traffic_*.json) when they were taken on this kernel choice and workload."""
     import glob
-    choice = f"{best['kernel']}:{best['variant']}"
+    choice = choice_name(best)
     for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "traffic_*.json"))):
         with open(path) as f:
             d = json.load(f)
@@ -184,7 +217,9 @@ def main(argv=None) -> int:
     ap.add_argument("--batched-scenes", type=int, default=64)
     ap.add_argument("--batched-steps", type=int, default=5)
     ap.add_argument("--only", default="", metavar="KERNEL:VARIANT",
-                    help="time one kernel choice only (e.g. 1:0 one ray per lane, 2:3 persistent preset 3)")
+                    help="time one kernel choice only (e.g. 1:0 one ray per lane, 3:1 wave walk preset 1, "
+                         "w8:2 the BVH8 layout with 2-triangle leaves)")
+    ap.add_argument("--no-wide", action="store_true", help="time the BVH2 kernels only")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="instead of the workload, time the streaming copy kernel over BYTES (HBM probe and "
                          "counter calibration)")
@@ -236,8 +271,24 @@ def main(argv=None) -> int:
 
     K = args.steps
 
-    def run(kernel: int, variant: int) -> dict:
+    wides = {}
+
+    def scene_of(c: dict):
+        if c["layout"] == "bvh2":
+            return scene
+        key = (c["width"], c["max_leaf"])
+        if key not in wides:
+            t = time.perf_counter()
+            wides[key] = wide_scene(crt, hb, scene.triangles, scene.vertices, *key)
+            st = wides[key][2].stats
+            log(f"wide layout {key}: {st['node_count']} nodes, fill {st['mean_fill']:.2f}, depth {st['max_depth']}, "
+                f"built in {time.perf_counter() - t:.2f}s")
+        return wides[key][0]
+
+    def run(c: dict) -> dict:
         """W warm-up + K timed steps of (AO any-hit, GI closest-hit) with one kernel choice."""
+        kernel, variant = c["kernel"], c["variant"]
+        scene = scene_of(c)
         kw = dict(block_size=args.block, lds_stack=args.lds_stack, kernel=kernel, variant=variant, stream=stream)
         for _ in range(args.warmup):
             crt.trace(scene, crt.QUERY_ANY, d_ao, h_ao, **kw)
@@ -262,21 +313,31 @@ def main(argv=None) -> int:
         gi_ms = float(np.mean([ev.ms(2 + 3 * k, 3 + 3 * k) for k in range(K)]))
         ev.close()
         ms_step = total_ms / K
-        res = dict(kernel=kernel, variant=variant, ms_step=ms_step, wall_ms_step=(w1 - w0) * 1e3 / K, ao_ms=ao_ms,
+        res = dict(c, choice=choice_name(c), ms_step=ms_step, wall_ms_step=(w1 - w0) * 1e3 / K, ao_ms=ao_ms,
                    gi_ms=gi_ms, mrays=(n_ao + n_gi) / (ms_step * 1e-3) / 1e6)
-        log(f"kernel={kernel} variant={variant}: {ms_step:.3f} ms/step (wall {res['wall_ms_step']:.3f}), "
+        log(f"{choice_name(c)}: {ms_step:.3f} ms/step (wall {res['wall_ms_step']:.3f}), "
             f"AO {ao_ms:.3f} ms, GI {gi_ms:.3f} ms -> {res['mrays']:.1f} Mrays/s")
         return res
 
-    candidates = [(crt.KERNEL_THREAD_PER_RAY, 0)]
+    def bvh2(k, v):
+        return dict(layout="bvh2", kernel=k, variant=v)
+
+    def wide(w, leaf):
+        return dict(layout=f"wide{w}", width=w, max_leaf=leaf, kernel=crt.KERNEL_THREAD_PER_RAY, variant=0)
+
+    candidates = [bvh2(crt.KERNEL_THREAD_PER_RAY, 0)]
     if fmt == crt.FORMAT_STACKLESS:
-        candidates += [(crt.KERNEL_WAVE, v) for v in range(6)] + [(crt.KERNEL_PERSISTENT, v) for v in range(6)]
+        candidates += [bvh2(crt.KERNEL_WAVE, v) for v in range(6)] + [bvh2(crt.KERNEL_PERSISTENT, v) for v in range(6)]
+        if not args.no_wide:
+            candidates += [wide(w, leaf) for w in (4, 8) for leaf in (2, 4)]
     if args.only:
-        k, v = (int(x) for x in args.only.split(":"))
-        candidates = [(k, v)]
-    results = [run(k, v) for k, v in candidates]
+        a, b = args.only.split(":")
+        candidates = [wide(int(a[1:]), int(b)) if a.startswith("w") else bvh2(int(a), int(b))]
+    results = [run(c) for c in candidates]
     thread = results[0]
     best = max(results, key=lambda r: r["mrays"])
+    best_scene = scene_of(best)
+    best_fmt = best_scene.format
     ms_step = best["ms_step"]
     rays_step = n_ao + n_gi
     value = best["mrays"]
@@ -285,13 +346,13 @@ def main(argv=None) -> int:
     out = {}
     for name, dr, dh, q, n in (("ao", d_ao, h_ao, crt.QUERY_ANY, n_ao), ("gi", d_gi, h_gi, crt.QUERY_CLOSEST, n_gi)):
         cnt = crt.DeviceArray(np.uint32, n)
-        crt.trace(scene, q, dr, dh, block_size=args.block, lds_stack=args.lds_stack, kernel=best["kernel"],
+        crt.trace(best_scene, q, dr, dh, block_size=args.block, lds_stack=args.lds_stack, kernel=best["kernel"],
                   variant=best["variant"], tri_tests=cnt)
         crt.lib().crt_device_sync()
         hits = dh.download()
         tests = cnt.download()
         visits = hits["status"] & crt.STATUS_VISITS_MASK
-        b = bytes_per_ray(fmt, visits, tests)
+        b = bytes_per_ray(best_fmt, visits, tests)
         out[name] = dict(hits=hits, bytes=float(b.sum()), visits=float(visits.mean()), tests=float(tests.mean()),
                          capped=float(np.mean((hits["status"] & crt.STATUS_ITER_CAP) != 0)),
                          hit_rate=float(np.mean(hits["t"] > 0)))
@@ -306,11 +367,12 @@ def main(argv=None) -> int:
     batched = None
     c2_batched = None
     if not args.skip_batched and fmt == crt.FORMAT_STACKLESS:
+        plain_bvh2 = max((r for r in results if r["layout"] == "bvh2"), key=lambda r: r["mrays"])
         c2_batched = time_batched(crt, [(scene, crt.QUERY_ANY, d_ao, h_ao), (scene, crt.QUERY_CLOSEST, d_gi, h_gi)],
                                   K, args.warmup, stream)
         log(f"config-2 batches in one persistent launch: {c2_batched['ms']:.3f} ms/step -> "
             f"{rays_step / (c2_batched['ms'] * 1e-3) / 1e6:.1f} Mrays/s ({c2_batched['config']})")
-        batched = batched_vs_plain(crt, args, best, stream)
+        batched = batched_vs_plain(crt, args, plain_bvh2, stream)
 
     # --- CPU baseline on a bounded sample --------------------------------------------------
     cpu = None
@@ -341,9 +403,18 @@ def main(argv=None) -> int:
                          f"{passes} pass(es), {cpu_s:.1f} s on {threads} threads "
                          f"(C restatement of the GLSL walk, -O3)"}
         g, a = out["gi"]["hits"][:n_s], out["ao"]["hits"][:n_s]
+        if best["layout"] == "bvh2":
+            twin_g, twin_a = ohg, oha
+        else:   # the host twin of the wide walk, on the same wide layout
+            w = wides[(best["width"], best["max_leaf"])][2]
+            twin_g = oracle.trace_wide(w.width, w.groups, w.records, w.entities, 0, sample_gi, threads=threads)
+            twin_a = oracle.trace_wide(w.width, w.groups, w.records, w.entities, 1, sample_ao, threads=threads)
         parity = {"rays": 2 * n_s,
-                  "exact": float((np.concatenate([g, a]).tobytes() == np.concatenate([ohg, oha]).tobytes())),
-                  "id_agreement": float(np.mean(np.concatenate([g["tri"] == ohg["tri"], a["tri"] == oha["tri"]])))}
+                  "exact": float((np.concatenate([g, a]).tobytes() == np.concatenate([twin_g, twin_a]).tobytes())),
+                  "id_agreement_vs_reference_walk": float(np.mean(np.concatenate([g["tri"] == ohg["tri"],
+                                                                                  a["tri"] == oha["tri"]]))),
+                  "t_agreement_vs_reference_walk": float(np.mean(np.concatenate([g["t"] == ohg["t"],
+                                                                                 (a["t"] > 0) == (oha["t"] > 0)])))}
         log(f"cpu baseline {cpu_value:.3f} Mrays/s on {threads} threads; parity {parity}")
 
     line = {
@@ -367,8 +438,7 @@ def main(argv=None) -> int:
             "gi_rays": n_gi,
             "ao_rays": n_ao,
             "format": args.format,
-            "kernel": "plain (one batch per launch): %s, preset %d" % (KERNEL_NAMES[best["kernel"]],
-                                                                       best["variant"]),
+            "kernel": kernel_label(best),
             "parallelism": "single GPU",
         },
         "plain_mrays_s": round(value, 2),
@@ -376,7 +446,7 @@ def main(argv=None) -> int:
         "batched_mrays_s": (round(rays_step / (c2_batched["ms"] * 1e-3) / 1e6, 2) if c2_batched else None),
         "batched_vs_plain": batched,
         "kernels": [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()} for r in results],
-        "roofline": {"bound": "hbm", "kernel": KERNEL_NAMES[best["kernel"]] + " (GI closest hit)", "achieved": round(achieved, 1),
+        "roofline": {"bound": "hbm", "kernel": kernel_label(best) + " (GI closest hit)", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "traffic_source": traffic_src,
                      "bytes_per_launch": out["gi"]["bytes"], "launch_ms": round(gi_s * 1e3, 4),

@@ -13,7 +13,8 @@ from dataclasses import dataclass
 import numpy as np
 
 from ._lib import (KERNEL_AUTO, KERNEL_PERSISTENT, KERNEL_THREAD_PER_RAY, KERNEL_WAVE, BOUNDS_DTYPE, ENTITY_DTYPE, FLAG_IGNORE_TRANSPARENT, FLAG_TRI0_IS_MISS, FORMAT_STACK,
-                   FORMAT_STACKLESS, HIT_DTYPE, QUERY_ANY, QUERY_CLOSEST, RAY_DTYPE, STACK_NODE_DTYPE,
+                   FORMAT_STACKLESS, FORMAT_WIDE4, FORMAT_WIDE8, WIDE_GROUP_DTYPE, TRI_RECORD_DTYPE, WideStats,
+                   HIT_DTYPE, QUERY_ANY, QUERY_CLOSEST, RAY_DTYPE, STACK_NODE_DTYPE,
                    STACKLESS_NODE_DTYPE, STATUS_BAD_INDEX, STATUS_ITER_CAP, STATUS_STACK_OVERFLOW,
                    STATUS_VISITS_MASK, SWAP_NONE, SWAP_SEEDED, TRIANGLE_DTYPE, VERTEX_DTYPE, BuildOptions,
                    BvhStats, CrtError, Mesh, SceneView, TraceConfig, Batch, BatchConfig, ShadeParams,
@@ -26,7 +27,8 @@ __all__ = [
     "STATUS_VISITS_MASK", "SWAP_NONE", "SWAP_SEEDED", "KERNEL_AUTO", "KERNEL_THREAD_PER_RAY", "KERNEL_PERSISTENT",
     "KERNEL_WAVE", "CrtError", "DeviceArray", "Stream", "BuiltBVH", "build_bvh",
     "RayIntersector", "trace", "device_count", "node_dtype", "lib", "Batch", "BatchConfig", "BatchPlan",
-    "autotune_batched", "ShadeParams", "SHADE_RECORD_DTYPE", "ShadingModel",
+    "autotune_batched", "ShadeParams", "SHADE_RECORD_DTYPE", "ShadingModel", "FORMAT_WIDE4", "FORMAT_WIDE8",
+    "WIDE_GROUP_DTYPE", "TRI_RECORD_DTYPE", "WideBVH", "wide_build",
 ]
 
 
@@ -153,6 +155,47 @@ def build_bvh(fmt: int, meshes, t_offset: int = 0, child_swap: int = SWAP_NONE, 
     return BuiltBVH(fmt, nodes, tris, verts, stats)
 
 
+@dataclass
+class WideBVH:
+    width: int
+    groups: np.ndarray     # WIDE_GROUP_DTYPE [node_count * width / 4]
+    records: np.ndarray    # TRI_RECORD_DTYPE [triangles]
+    entities: np.ndarray   # ENTITY_DTYPE, node_offset / node_count = wide root / node count
+    stats: dict
+
+    @property
+    def format(self) -> int:
+        return FORMAT_WIDE8 if self.width == 8 else FORMAT_WIDE4
+
+
+def _view(addr: int, count: int, dt: np.dtype) -> np.ndarray:
+    if not count:
+        return np.zeros(0, dt)
+    return np.frombuffer((C.c_char * (count * dt.itemsize)).from_address(addr), dtype=dt).copy()
+
+
+def wide_build(src_format: int, nodes: np.ndarray, triangles: np.ndarray, vertices: np.ndarray,
+               entities: np.ndarray, width: int = 4, max_leaf: int = 2) -> WideBVH:
+    """crt_wide_build: the BVH4 / BVH8 layout derived from BVH2 buffers (host copies)."""
+    nodes = np.ascontiguousarray(nodes, dtype=node_dtype(src_format))
+    triangles = np.ascontiguousarray(triangles, dtype=TRIANGLE_DTYPE)
+    vertices = np.ascontiguousarray(vertices, dtype=VERTEX_DTYPE)
+    entities = np.ascontiguousarray(entities, dtype=ENTITY_DTYPE)
+    h = C.c_void_p()
+    check(lib().crt_wide_build(src_format, ptr(nodes), nodes.size, ptr(triangles), triangles.size, ptr(vertices),
+                               vertices.size, ptr(entities), entities.size, int(width), int(max_leaf), C.byref(h)))
+    try:
+        st = WideStats()
+        check(lib().crt_wide_get_stats(h, C.byref(st)))
+        stats = {k: (float(getattr(st, k)) if k == "mean_fill" else int(getattr(st, k))) for k, _ in WideStats._fields_}
+        groups = _view(lib().crt_wide_groups(h), stats["node_count"] * width // 4, WIDE_GROUP_DTYPE)
+        records = _view(lib().crt_wide_records(h), stats["record_count"], TRI_RECORD_DTYPE)
+        ents = _view(lib().crt_wide_entities(h), entities.size, ENTITY_DTYPE)
+    finally:
+        lib().crt_wide_free(h)
+    return WideBVH(int(width), groups, records, ents, stats)
+
+
 def _colmajor(m) -> np.ndarray:
     """4x4 matrix in math convention (m[row, col]) -> 16 floats column-major (glm layout)."""
     m = np.asarray(m, dtype=np.float32)
@@ -232,6 +275,15 @@ class RayIntersector:
     def scene(self) -> SceneView:
         s = SceneView()
         check(lib().crt_context_scene(self.handle, C.byref(s)))
+        return s
+
+    def build_wide(self, width: int = 4, max_leaf: int = 2) -> None:
+        """crt_context_build_wide: derive and upload the BVH4 / BVH8 layout of the buffered scene."""
+        check(lib().crt_context_build_wide(self.handle, int(width), int(max_leaf)))
+
+    def wide_scene(self) -> SceneView:
+        s = SceneView()
+        check(lib().crt_context_wide_scene(self.handle, C.byref(s)))
         return s
 
     def intersect(self, query: int, rays: DeviceArray, hits: DeviceArray, flags: int = 0, stream=None):

@@ -17,6 +17,8 @@ LIB_PATH = os.path.join(HERE, "libcandela_rt.so")
 CRT_OK = 0
 FORMAT_STACKLESS = 0
 FORMAT_STACK = 1
+FORMAT_WIDE4 = 2
+FORMAT_WIDE8 = 3
 QUERY_CLOSEST = 0
 QUERY_ANY = 1
 FLAG_IGNORE_TRANSPARENT = 1 << 0
@@ -43,6 +45,11 @@ ENTITY_DTYPE = np.dtype([("model", "<f4", 16), ("inverse", "<f4", 16), ("node_of
 RAY_DTYPE = np.dtype([("org", "<f4", 3), ("tmax", "<f4"), ("dir", "<f4", 3), ("id", "<u4")])
 HIT_DTYPE = np.dtype([("t", "<f4"), ("u", "<f4"), ("v", "<f4"), ("w", "<f4"), ("tri", "<i4"),
                       ("mesh", "<i4"), ("entity", "<i4"), ("status", "<u4")])
+WIDE_GROUP_DTYPE = np.dtype([("lo_x", "<f4", 4), ("hi_x", "<f4", 4), ("lo_y", "<f4", 4), ("hi_y", "<f4", 4),
+                             ("lo_z", "<f4", 4), ("hi_z", "<f4", 4), ("child", "<i4", 4), ("count", "<i4", 4)])
+TRI_RECORD_DTYPE = np.dtype([("v0", "<f4", 3), ("tri", "<i4"), ("v1", "<f4", 3), ("mesh", "<i4"),
+                             ("v2", "<f4", 3), ("pad", "<i4")])
+assert WIDE_GROUP_DTYPE.itemsize == 128 and TRI_RECORD_DTYPE.itemsize == 48
 assert VERTEX_DTYPE.itemsize == 32 and TRIANGLE_DTYPE.itemsize == 16
 assert STACKLESS_NODE_DTYPE.itemsize == 32 and STACK_NODE_DTYPE.itemsize == 64
 assert ENTITY_DTYPE.itemsize == 192 and RAY_DTYPE.itemsize == 32 and HIT_DTYPE.itemsize == 32
@@ -71,7 +78,12 @@ class BvhStats(C.Structure):
 class SceneView(C.Structure):
     _fields_ = [("format", C.c_int32), ("entity_count", C.c_int32), ("total_nodes", C.c_int64),
                 ("nodes", C.c_void_p), ("triangles", C.c_void_p), ("vertices", C.c_void_p),
-                ("entities", C.c_void_p)]
+                ("entities", C.c_void_p), ("tri_records", C.c_void_p)]
+
+
+class WideStats(C.Structure):
+    _fields_ = [("node_count", C.c_int64), ("leaf_count", C.c_int64), ("record_count", C.c_int64),
+                ("width", C.c_int32), ("max_depth", C.c_int32), ("mean_fill", C.c_double)]
 
 
 class TraceConfig(C.Structure):
@@ -161,6 +173,15 @@ SIGNATURES = {
                                      C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "crt_autotune_batched": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_void_p,
                                        C.POINTER(BatchConfig), C.POINTER(C.c_float)]),
+    "crt_wide_build": (C.c_int, [C.c_int, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64,
+                                 C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_void_p)]),
+    "crt_wide_free": (None, [C.c_void_p]),
+    "crt_wide_get_stats": (C.c_int, [C.c_void_p, C.POINTER(WideStats)]),
+    "crt_wide_groups": (C.c_void_p, [C.c_void_p]),
+    "crt_wide_records": (C.c_void_p, [C.c_void_p]),
+    "crt_wide_entities": (C.c_void_p, [C.c_void_p]),
+    "crt_context_build_wide": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
+    "crt_context_wide_scene": (C.c_int, [C.c_void_p, C.POINTER(SceneView)]),
 }
 
 

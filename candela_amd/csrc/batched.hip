@@ -175,8 +175,7 @@ int build_plan(const crt_batch* batches, int nbatches, const crt_batch_config& c
             return crt::set_error(CRT_E_INVALID, "batch: null scene buffer");
         if (b.count >= (int64_t(1) << 31)) return crt::set_error(CRT_E_INVALID, "batch: >= 2^31 rays");
         crt::BatchDesc& d = descs[size_t(i)];
-        d.scene = crt::SceneArgs{b.scene.format, b.scene.entity_count, b.scene.total_nodes, b.scene.nodes,
-                                 b.scene.triangles, b.scene.vertices, b.scene.entities};
+        d.scene = crt::scene_args(b.scene);
         d.rays = b.rays;
         d.hits = b.hits;
         d.tri_tests = b.tri_tests;

@@ -50,12 +50,18 @@ static int validate_trace_config(const crt_trace_config* cfg, int& block, int& d
     return CRT_OK;
 }
 
+static bool is_wide(int format) { return format == CRT_FORMAT_WIDE4 || format == CRT_FORMAT_WIDE8; }
+
 static int validate_scene(const crt_scene_view* s) {
     if (!s) return set_error(CRT_E_INVALID, "scene is null");
-    if (s->format != CRT_FORMAT_STACKLESS && s->format != CRT_FORMAT_STACK)
+    if (s->format != CRT_FORMAT_STACKLESS && s->format != CRT_FORMAT_STACK && !is_wide(s->format))
         return set_error(CRT_E_INVALID, "scene: unknown node format");
     if (s->entity_count < 0) return set_error(CRT_E_INVALID, "scene: negative entity count");
-    if (s->entity_count > 0 && (!s->nodes || !s->triangles || !s->vertices || !s->entities || s->total_nodes <= 0))
+    if (s->entity_count > 0 && is_wide(s->format) &&
+        (!s->nodes || !s->tri_records || !s->entities || s->total_nodes <= 0))
+        return set_error(CRT_E_INVALID, "scene: null device buffer (wide layout)");
+    if (s->entity_count > 0 && !is_wide(s->format) &&
+        (!s->nodes || !s->triangles || !s->vertices || !s->entities || s->total_nodes <= 0))
         return set_error(CRT_E_INVALID, "scene: null device buffer");
     return CRT_OK;
 }
@@ -101,10 +107,13 @@ int trace(const crt_scene_view* scene, int query, uint32_t flags, const crt_ray*
         return set_error(CRT_E_INVALID, "crt_trace: unknown kernel");
     if ((kernel == CRT_KERNEL_PERSISTENT || kernel == CRT_KERNEL_WAVE) && scene->format != CRT_FORMAT_STACKLESS)
         return set_error(CRT_E_INVALID, "crt_trace: the persistent and wave kernels walk the stackless format only");
+    if (is_wide(scene->format) && (depth != 8 && depth != 16 && depth != 32))
+        return set_error(CRT_E_INVALID, "crt_trace: wide formats take lds_stack 8, 16 or 32");
+    if (is_wide(scene->format) && size_t(depth) * size_t(block) * 8 > 64 * 1024)
+        return set_error(CRT_E_INVALID, "trace config: 2 * lds_stack * block_size * 4 exceeds 64 KiB of LDS");
     if (cfg && (cfg->variant < 0 || cfg->variant > 5 || cfg->grid < 0))
         return set_error(CRT_E_INVALID, "crt_trace: variant must be 0..5 and grid >= 0");
-    SceneArgs a{scene->format, scene->entity_count, scene->total_nodes, scene->nodes,
-                scene->triangles, scene->vertices, scene->entities};
+    const SceneArgs a = scene_args(*scene);
     uint32_t* tri_tests = cfg ? cfg->tri_tests : nullptr;
     const bool persistent = kernel == CRT_KERNEL_PERSISTENT;
     const bool wave = kernel == CRT_KERNEL_WAVE || (kernel == CRT_KERNEL_AUTO && scene->format == CRT_FORMAT_STACKLESS);
@@ -166,6 +175,13 @@ struct crt_context {
     int64_t nodes_buffered = 0;               // m_NodeCountBuffered
     int64_t entities_buffered = 0;            // m_EntityPushed
     bool data_buffered = false;
+    // Wide layout derived from the buffered data and entities.
+    void* d_wide_groups = nullptr;
+    void* d_wide_records = nullptr;
+    void* d_wide_entities = nullptr;
+    int32_t wide_width = 0;                   // 0 = not built or stale
+    int64_t wide_nodes = 0;
+    int64_t wide_entities = 0;
 
     int64_t node_count() const {
         return format == CRT_FORMAT_STACKLESS ? int64_t(stackless.size()) : int64_t(stack.size());
@@ -176,6 +192,14 @@ struct crt_context {
         if (d_verts) (void)hipFree(d_verts);
         if (d_entities) (void)hipFree(d_entities);
         d_nodes = d_tris = d_verts = d_entities = nullptr;
+        free_wide();
+    }
+    void free_wide() {
+        if (d_wide_groups) (void)hipFree(d_wide_groups);
+        if (d_wide_records) (void)hipFree(d_wide_records);
+        if (d_wide_entities) (void)hipFree(d_wide_entities);
+        d_wide_groups = d_wide_records = d_wide_entities = nullptr;
+        wide_width = 0;
     }
 };
 
@@ -457,6 +481,7 @@ int crt_buffer_entities(crt_context* ctx) {
     ctx->entities_buffered = int64_t(ctx->pending.size());
     ctx->entities = ctx->pending;
     ctx->pending.clear();
+    ctx->free_wide();
     return CRT_OK;
 }
 
@@ -468,6 +493,7 @@ int crt_buffer_data(crt_context* ctx, int clear_cpu_data) {
     if (rc) return rc;
     ctx->nodes_buffered = ctx->node_count();
     ctx->data_buffered = true;
+    ctx->free_wide();
     if (clear_cpu_data) {
         ctx->stackless.clear();
         ctx->stack.clear();
@@ -505,6 +531,59 @@ int crt_context_scene(const crt_context* ctx, crt_scene_view* out) {
     out->triangles = static_cast<const crt_triangle*>(ctx->d_tris);
     out->vertices = static_cast<const crt_vertex*>(ctx->d_verts);
     out->entities = static_cast<const crt_entity*>(ctx->d_entities);
+    out->tri_records = nullptr;
+    return CRT_OK;
+}
+
+int crt_context_build_wide(crt_context* ctx, int32_t width, int32_t max_leaf) {
+    if (!ctx) return crt::set_error(CRT_E_INVALID, "crt_context_build_wide: ctx is null");
+    if (!ctx->data_buffered || !ctx->d_entities)
+        return crt::set_error(CRT_E_STATE, "crt_context_build_wide: call crt_buffer_data and crt_buffer_entities first");
+    if (ctx->node_count() != ctx->nodes_buffered)
+        return crt::set_error(CRT_E_STATE,
+                              "crt_context_build_wide: the host copies were cleared or changed since crt_buffer_data");
+    const void* nodes = ctx->format == CRT_FORMAT_STACKLESS ? static_cast<const void*>(ctx->stackless.data())
+                                                            : static_cast<const void*>(ctx->stack.data());
+    crt_wide* w = nullptr;
+    int rc = crt_wide_build(ctx->format, nodes, ctx->node_count(), ctx->triangles.data(),
+                            int64_t(ctx->triangles.size()), ctx->vertices.data(), int64_t(ctx->vertices.size()),
+                            ctx->entities.data(), int32_t(ctx->entities.size()), width, max_leaf, &w);
+    if (rc) return rc;
+    crt_wide_stats st;
+    crt_wide_get_stats(w, &st);
+    ctx->free_wide();
+    const crt_wide_group* g = crt_wide_groups(w);
+    const crt_tri_record* r = crt_wide_records(w);
+    const crt_entity* e = crt_wide_entities(w);
+    std::vector<crt_wide_group> vg(g, g + st.node_count * (width / 4));
+    std::vector<crt_tri_record> vr(r, r + st.record_count);
+    std::vector<crt_entity> ve(e, e + ctx->entities.size());
+    crt_wide_free(w);
+    rc = upload(ctx->d_wide_groups, vg);
+    if (!rc) rc = upload(ctx->d_wide_records, vr);
+    if (!rc) rc = upload(ctx->d_wide_entities, ve);
+    if (rc) {
+        ctx->free_wide();
+        return rc;
+    }
+    ctx->wide_width = width;
+    ctx->wide_nodes = st.node_count;
+    ctx->wide_entities = int64_t(ve.size());
+    return CRT_OK;
+}
+
+int crt_context_wide_scene(const crt_context* ctx, crt_scene_view* out) {
+    if (!ctx || !out) return crt::set_error(CRT_E_INVALID, "crt_context_wide_scene: null argument");
+    if (!ctx->wide_width)
+        return crt::set_error(CRT_E_STATE, "crt_context_wide_scene: call crt_context_build_wide first");
+    out->format = ctx->wide_width == 8 ? CRT_FORMAT_WIDE8 : CRT_FORMAT_WIDE4;
+    out->entity_count = int32_t(ctx->wide_entities);
+    out->total_nodes = ctx->wide_nodes;
+    out->nodes = ctx->d_wide_groups;
+    out->triangles = static_cast<const crt_triangle*>(ctx->d_tris);
+    out->vertices = static_cast<const crt_vertex*>(ctx->d_verts);
+    out->entities = static_cast<const crt_entity*>(ctx->d_wide_entities);
+    out->tri_records = static_cast<const crt_tri_record*>(ctx->d_wide_records);
     return CRT_OK;
 }
 
@@ -527,7 +606,7 @@ int crt_intersect_primary(crt_context* ctx, crt_ray* rays, crt_hit* hits, int wi
     std::memcpy(cam.inv_view, inv_view, sizeof(cam.inv_view));
     std::memcpy(cam.inv_proj, inv_proj, sizeof(cam.inv_proj));
     cam.tmax = 1000000.0f;  // IntersectScene's TMax (TraverseBVHStackless.glsl:309)
-    crt::SceneArgs a{s.format, s.entity_count, s.total_nodes, s.nodes, s.triangles, s.vertices, s.entities};
+    const crt::SceneArgs a = crt::scene_args(s);
     CRT_HIP(crt::launch_primary(a, cam, rays, hits, width, height, flags, crt::kMaxIterations, 256,
                                 static_cast<hipStream_t>(stream)),
             "crt_intersect_primary: kernel launch");

@@ -354,8 +354,7 @@ int shade_args(const crt_scene_view* scene, const crt_shade_params* params, crt:
         return crt::set_error(CRT_E_INVALID, "shade: nlights must be 1..16");
     if (!params->albedo || !params->roughness || !params->light_pos || !params->intensity || params->ntri <= 0)
         return crt::set_error(CRT_E_INVALID, "shade: null parameter buffer");
-    a = crt::SceneArgs{scene->format, scene->entity_count, scene->total_nodes, scene->nodes,
-                       scene->triangles, scene->vertices, scene->entities};
+    a = crt::scene_args(*scene);
     p = crt::shade::Params{params->albedo, params->roughness, params->light_pos, params->intensity, params->ntri,
                            params->nlights};
     return CRT_OK;

@@ -14,7 +14,8 @@
 //
 // Memory: a stackless node is two 16-byte loads, a stack node four, a
 // triangle one 16-byte load and each vertex position one 16-byte load of the
-// first half of its 32-byte record.  The stack walk keeps its first
+// first half of its 32-byte record.  A wide node is eight 16-byte loads per
+// group of four children and a triangle record three.  The stack walk keeps its first
 // LDS_DEPTH entries in LDS (entry-major, so lane i of a wave touches bank i)
 // and the rest in a per-lane scratch array.
 #include <hip/hip_runtime.h>
@@ -89,7 +90,9 @@ template <int FORMAT, bool ANY, int D>
 static hipError_t launch_trace_d(const SceneArgs& s, const crt_ray* rays, crt_hit* hits, int64_t n, uint32_t flags,
                                  int32_t max_iter, int block, uint32_t* tri_tests, hipStream_t stream) {
     const int64_t grid = (n + block - 1) / block;
-    const size_t lds = FORMAT == CRT_FORMAT_STACK ? size_t(D) * block * sizeof(int32_t) : 0;
+    // Stack format: one int per entry; wide formats: node + entry distance.
+    const size_t per_entry = FORMAT == CRT_FORMAT_STACK ? 1 : FORMAT == CRT_FORMAT_STACKLESS ? 0 : 2;
+    const size_t lds = per_entry * size_t(D) * block * sizeof(int32_t);
     hipLaunchKernelGGL((dev::trace_kernel<FORMAT, ANY, D>), dim3(unsigned(grid)), dim3(unsigned(block)), lds, stream,
                        s, rays, hits, n, flags, max_iter, tri_tests);
     return hipGetLastError();
@@ -100,6 +103,12 @@ static hipError_t launch_trace_f(const SceneArgs& s, const crt_ray* rays, crt_hi
                                  int32_t max_iter, int block, int depth, uint32_t* tri_tests, hipStream_t stream) {
     if constexpr (FORMAT == CRT_FORMAT_STACKLESS) {
         return launch_trace_d<FORMAT, ANY, 0>(s, rays, hits, n, flags, max_iter, block, tri_tests, stream);
+    } else if constexpr (FORMAT == CRT_FORMAT_WIDE4 || FORMAT == CRT_FORMAT_WIDE8) {
+        switch (depth) {
+            case 8: return launch_trace_d<FORMAT, ANY, 8>(s, rays, hits, n, flags, max_iter, block, tri_tests, stream);
+            case 32: return launch_trace_d<FORMAT, ANY, 32>(s, rays, hits, n, flags, max_iter, block, tri_tests, stream);
+            default: return launch_trace_d<FORMAT, ANY, 16>(s, rays, hits, n, flags, max_iter, block, tri_tests, stream);
+        }
     } else {
         switch (depth) {
             case 8: return launch_trace_d<FORMAT, ANY, 8>(s, rays, hits, n, flags, max_iter, block, tri_tests, stream);
@@ -115,6 +124,16 @@ hipError_t launch_trace(const SceneArgs& s, int query, const crt_ray* rays, crt_
                         uint32_t flags, int32_t max_iter, int block, int depth, uint32_t* tri_tests,
                         hipStream_t stream) {
     if (n <= 0) return hipSuccess;
+    if (s.format == CRT_FORMAT_WIDE4) {
+        return query == CRT_QUERY_ANY
+                   ? launch_trace_f<CRT_FORMAT_WIDE4, true>(s, rays, hits, n, flags, max_iter, block, depth, tri_tests, stream)
+                   : launch_trace_f<CRT_FORMAT_WIDE4, false>(s, rays, hits, n, flags, max_iter, block, depth, tri_tests, stream);
+    }
+    if (s.format == CRT_FORMAT_WIDE8) {
+        return query == CRT_QUERY_ANY
+                   ? launch_trace_f<CRT_FORMAT_WIDE8, true>(s, rays, hits, n, flags, max_iter, block, depth, tri_tests, stream)
+                   : launch_trace_f<CRT_FORMAT_WIDE8, false>(s, rays, hits, n, flags, max_iter, block, depth, tri_tests, stream);
+    }
     if (s.format == CRT_FORMAT_STACKLESS) {
         return query == CRT_QUERY_ANY
                    ? launch_trace_f<CRT_FORMAT_STACKLESS, true>(s, rays, hits, n, flags, max_iter, block, depth, tri_tests, stream)

@@ -19,7 +19,14 @@ struct SceneArgs {
     const crt_triangle* tris;
     const crt_vertex* verts;
     const crt_entity* entities;
+    const crt_tri_record* tri_records;  // wide formats only
 };
+
+// SceneArgs of a scene view.
+inline SceneArgs scene_args(const crt_scene_view& v) {
+    return SceneArgs{v.format, v.entity_count, v.total_nodes, v.nodes, v.triangles, v.vertices, v.entities,
+                     v.tri_records};
+}
 
 struct Camera {
     float inv_view[16];

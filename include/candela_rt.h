@@ -120,7 +120,11 @@ typedef struct crt_hit {
     uint32_t status;
 } crt_hit;
 
-enum { CRT_FORMAT_STACKLESS = 0, CRT_FORMAT_STACK = 1 };
+/* STACKLESS and STACK are the reference's two layouts (FlattenedNode,
+ * FlattenedStackNode).  WIDE4 and WIDE8 are layouts derived from either of
+ * them by crt_wide_build (no reference counterpart; the reference only has
+ * BVH2): 4- or 8-wide nodes and triangle records in the same leaf order. */
+enum { CRT_FORMAT_STACKLESS = 0, CRT_FORMAT_STACK = 1, CRT_FORMAT_WIDE4 = 2, CRT_FORMAT_WIDE8 = 3 };
 enum { CRT_QUERY_CLOSEST = 0, CRT_QUERY_ANY = 1 };
 
 /* Trace flags */
@@ -230,7 +234,11 @@ int crt_event_elapsed_ms(void* start, void* end, float* ms);
 /* ------------------------------------------------------------------------- */
 
 /* Device-resident scene: the four buffers the traversal include binds plus
- * u_EntityCount / u_TotalNodes. */
+ * u_EntityCount / u_TotalNodes.  For the wide formats `nodes` holds
+ * crt_wide_group[total_nodes * width / 4], `tri_records` the triangle records
+ * and `entities` the entity copy whose node_offset / node_count name the
+ * object's wide root and wide node count; `triangles` and `vertices` are not
+ * read. */
 typedef struct crt_scene_view {
     int32_t format;              /* CRT_FORMAT_*                              */
     int32_t entity_count;        /* u_EntityCount                             */
@@ -239,6 +247,7 @@ typedef struct crt_scene_view {
     const crt_triangle* triangles;
     const crt_vertex* vertices;
     const crt_entity* entities;
+    const struct crt_tri_record* tri_records;  /* wide formats only, else NULL */
 } crt_scene_view;
 
 /* Kernel choice and launch shape. */
@@ -271,6 +280,65 @@ typedef struct crt_trace_config {
 int crt_trace(const crt_scene_view* scene, int query, uint32_t flags,
               const crt_ray* rays, crt_hit* hits, int64_t nrays,
               const crt_trace_config* config, void* stream);
+
+/* ------------------------------------------------------------------------- */
+/* Wide BVH (BVH4 / BVH8) derived from the reference's BVH2                   */
+/* ------------------------------------------------------------------------- */
+
+/* Four children of a wide node (128 B), structure of arrays so each field of
+ * the four children is one 16-byte load.  A BVH4 node is one group, a BVH8
+ * node two consecutive groups.  count[k]: 0 = inner child (child[k] = wide
+ * node index, absolute), n > 0 = leaf of n triangles starting at record
+ * child[k], -1 = empty slot.  The child boxes are the BVH2 node boxes,
+ * unchanged. */
+typedef struct crt_wide_group {
+    float lo_x[4], hi_x[4];
+    float lo_y[4], hi_y[4];
+    float lo_z[4], hi_z[4];
+    int32_t child[4];
+    int32_t count[4];
+} crt_wide_group;
+
+/* One triangle in leaf order (48 B): the three vertex positions, the
+ * triangle's own index (= its index in the reference's Triangle array) and
+ * its mesh id, so a test needs no index or vertex fetch. */
+typedef struct crt_tri_record {
+    float v0[3];
+    int32_t tri;
+    float v1[3];
+    int32_t mesh;
+    float v2[3];
+    int32_t pad;
+} crt_tri_record;
+
+typedef struct crt_wide crt_wide;
+
+typedef struct crt_wide_stats {
+    int64_t node_count;      /* wide nodes (each width / 4 groups)              */
+    int64_t leaf_count;      /* leaf slots                                      */
+    int64_t record_count;    /* triangle records (= triangles)                  */
+    int32_t width;           /* 4 or 8                                          */
+    int32_t max_depth;       /* deepest wide node (root = 1)                    */
+    double  mean_fill;       /* mean used slots per wide node                   */
+} crt_wide_stats;
+
+/* Collapses the BVH2 of every distinct object the entities reference
+ * (src_format STACKLESS or STACK buffers, object-local node links, global
+ * triangle starts as AddObject concatenates them) into `width`-wide nodes:
+ * each wide node repeatedly opens the child BVH2 node of largest surface
+ * area until it holds `width` children.  A subtree of at most `max_leaf`
+ * triangles (2..15) whose triangles are contiguous becomes one leaf (2 keeps
+ * the BVH2's own leaves).  Validates every link, leaf range and vertex index
+ * of the input. */
+int crt_wide_build(int src_format, const void* nodes, int64_t node_count, const crt_triangle* triangles,
+                   int64_t triangle_count, const crt_vertex* vertices, int64_t vertex_count,
+                   const crt_entity* entities, int32_t entity_count, int32_t width, int32_t max_leaf,
+                   crt_wide** out);
+void crt_wide_free(crt_wide* wide);
+int crt_wide_get_stats(const crt_wide* wide, crt_wide_stats* out);
+const crt_wide_group* crt_wide_groups(const crt_wide* wide);       /* node_count * width / 4 */
+const crt_tri_record* crt_wide_records(const crt_wide* wide);      /* record_count */
+const crt_entity* crt_wide_entities(const crt_wide* wide);         /* entity_count */
 
 /* ------------------------------------------------------------------------- */
 /* RayIntersector<T> -- Intersector.h:60-351                                  */
@@ -311,6 +379,12 @@ int crt_context_host_buffers(const crt_context* ctx, const void** nodes, int64_t
                              const crt_entity** entities, int64_t* entity_count);
 /* Device scene of the context (valid after buffer_data + buffer_entities). */
 int crt_context_scene(const crt_context* ctx, crt_scene_view* out);
+/* Builds the wide layout (crt_wide_build) from the context's host copies of
+ * the buffered data and entities and uploads it (needs buffer_data without
+ * clearing the CPU data, and buffer_entities).  No reference counterpart. */
+int crt_context_build_wide(crt_context* ctx, int32_t width, int32_t max_leaf);
+/* Device scene of the wide layout (CRT_FORMAT_WIDE4 / WIDE8). */
+int crt_context_wide_scene(const crt_context* ctx, crt_scene_view* out);
 
 /* IntersectRay / IntersectRayIgnoreTransparent / IntersectRay(ro,rd) over a
  * device ray buffer. */

@@ -31,6 +31,10 @@ ENTITY_DTYPE = np.dtype([("model", "<f4", 16), ("inverse", "<f4", 16), ("node_of
 RAY_DTYPE = np.dtype([("org", "<f4", 3), ("tmax", "<f4"), ("dir", "<f4", 3), ("id", "<u4")])
 HIT_DTYPE = np.dtype([("t", "<f4"), ("u", "<f4"), ("v", "<f4"), ("w", "<f4"), ("tri", "<i4"),
                       ("mesh", "<i4"), ("entity", "<i4"), ("status", "<u4")])
+WIDE_GROUP_DTYPE = np.dtype([("lo_x", "<f4", 4), ("hi_x", "<f4", 4), ("lo_y", "<f4", 4), ("hi_y", "<f4", 4),
+                             ("lo_z", "<f4", 4), ("hi_z", "<f4", 4), ("child", "<i4", 4), ("count", "<i4", 4)])
+TRI_RECORD_DTYPE = np.dtype([("v0", "<f4", 3), ("tri", "<i4"), ("v1", "<f4", 3), ("mesh", "<i4"),
+                             ("v2", "<f4", 3), ("pad", "<i4")])
 
 _lib = None
 
@@ -52,6 +56,8 @@ def lib() -> C.CDLL:
         vp, i32, i64, u32, f32 = C.c_void_p, C.c_int32, C.c_int64, C.c_uint32, C.c_float
         l.oracle_trace.restype = C.c_int
         l.oracle_trace.argtypes = [i32, i32, i64, vp, vp, vp, vp, C.c_int, u32, vp, vp, i64, C.c_int, vp, i32]
+        l.oracle_trace_wide.restype = C.c_int
+        l.oracle_trace_wide.argtypes = [i32, i32, i64, vp, vp, vp, C.c_int, u32, vp, vp, i64, C.c_int, vp, i32, i32]
         l.oracle_bruteforce_closest.restype = C.c_int
         l.oracle_bruteforce_closest.argtypes = [vp, vp, vp, i64, i64, vp, vp, vp, i64]
         l.oracle_primary_rays.restype = None
@@ -96,6 +102,26 @@ def trace(fmt: int, nodes: np.ndarray, tris: np.ndarray, verts: np.ndarray, ents
                             max_iter)
     if rc != 0:
         raise RuntimeError("oracle_trace failed")
+    return (hits, tests) if with_tests else hits
+
+
+def trace_wide(width: int, groups: np.ndarray, records: np.ndarray, ents: np.ndarray, query: int, rays: np.ndarray,
+               flags: int = 0, threads: int = 0, max_iter: int = 0, stack_cap: int = 80, with_tests: bool = False):
+    """Host walk of the wide layout (the twin of the GPU wide walk; stack_cap = LDS depth + 64)."""
+    groups = np.ascontiguousarray(groups, WIDE_GROUP_DTYPE)
+    records = np.ascontiguousarray(records, TRI_RECORD_DTYPE)
+    ents = np.ascontiguousarray(ents, ENTITY_DTYPE)
+    rays = np.ascontiguousarray(rays, RAY_DTYPE)
+    hits = np.empty(rays.size, HIT_DTYPE)
+    tests = np.zeros(rays.size, np.uint32) if with_tests else None
+    if threads <= 0:
+        threads = min(16, os.cpu_count() or 1)
+    nodes = groups.size * 4 // width
+    rc = lib().oracle_trace_wide(width, ents.size, nodes, _p(groups), _p(records), _p(ents), query, flags, _p(rays),
+                                 _p(hits), rays.size, threads, _p(tests) if tests is not None else None, max_iter,
+                                 stack_cap)
+    if rc != 0:
+        raise RuntimeError("oracle_trace_wide failed")
     return (hits, tests) if with_tests else hits
 
 

@@ -32,6 +32,11 @@
  * Pins (tests/test_oracle*.py): closed-form triangles and boxes, brute force
  * over all triangles on small scenes, stack == stackless closest-t on the same
  * tree, any-hit <=> closest-hit within tmax, crafted buffers for every guard.
+ *
+ * walk_wide walks the wide layout (crt_wide_build), which has no reference
+ * counterpart: it is the host twin of the HIP wide walk, used to check that a
+ * collapsed tree gives the same nearest hit as brute force and as the BVH2
+ * walks (tests/test_wide.py), and to check the GPU wide walk bit for bit.
  */
 #include <math.h>
 #include <pthread.h>
@@ -172,6 +177,8 @@ typedef struct {
     const crt_triangle* tris;
     const crt_vertex* verts;
     const crt_entity* ents;
+    const crt_tri_record* recs; /* wide formats */
+    int32_t stack_cap;          /* wide formats: stack entries before STACK_OVERFLOW */
     int32_t max_iter;
 } scene_t;
 
@@ -323,13 +330,132 @@ static void walk_stack(const scene_t* s, v3 ro, v3 rd, int32_t start, int32_t co
     w->iters += (uint32_t)it;
 }
 
+/* Triangle records [start, start+len) of the wide layout. */
+static inline int test_records(const scene_t* s, v3 ro, v3 rd, int32_t start, int32_t len, int any, float* tmax,
+                               walk_t* w) {
+    for (int32_t idx = start; idx < start + len; ++idx) {
+        const crt_tri_record* r = &s->recs[idx];
+        float t = tri_t(ro, rd, mk(r->v0[0], r->v0[1], r->v0[2]), mk(r->v1[0], r->v1[1], r->v1[2]),
+                        mk(r->v2[0], r->v2[1], r->v2[2]));
+        w->tri_tests++;
+        if (t > 0.0f && t < *tmax) {
+            *tmax = t;
+            w->t = t;
+            w->tri = r->tri;
+            w->mesh = r->mesh;
+            if (any) return 1;
+        }
+    }
+    return 0;
+}
+
+/* Wide walk: per node, every non-empty child box gets the slab arithmetic of
+ * the stack include's RayBounds against the current tmax, with the exit
+ * distance and tmax grown by WIDE_GROW = 1 + 2^-19 (so that a leaf whose
+ * triangle lies on the box face, a few ulps below the computed entry, is not
+ * culled by a closer-by-ulps hit); hit leaves are tested at once, the nearest
+ * hit inner child is visited next and the other hit inner children are pushed
+ * with their entry distance t0; a popped entry with t0 > tmax * WIDE_GROW is
+ * skipped (closest hit). */
+#define WIDE_GROW (1.0f + 1.0f / 524288.0f)
+static void walk_wide(const scene_t* s, v3 ro, v3 rd, int32_t root, int32_t count, float tmax, int any,
+                      walk_t* w) {
+    const int G = s->format == CRT_FORMAT_WIDE8 ? 2 : 1;
+    const crt_wide_group* groups = (const crt_wide_group*)s->nodes;
+    v3 inv = mk(1.0f / rd.x, 1.0f / rd.y, 1.0f / rd.z);
+    int32_t st_node[256];
+    float st_t[256];
+    int cap = s->stack_cap < 256 ? s->stack_cap : 256;
+    int sp = 0, it = 0, stopped = 0;
+    int32_t cur = root;
+    w->t = -1.0f;
+    w->tri = -1;
+    w->mesh = -1;
+    while (it < s->max_iter) {
+        if (cur < root || cur >= root + count || (int64_t)cur >= s->total_nodes) {
+            w->status |= CRT_STATUS_BAD_INDEX;
+            stopped = 1;
+            break;
+        }
+        it++;
+        int32_t near_node = -1;
+        float near_t = 0.0f;
+        int overflow = 0;
+        for (int g = 0; g < G; ++g) {
+            const crt_wide_group* gp = &groups[(int64_t)cur * G + g];
+            for (int k = 0; k < 4; ++k) {
+                if (gp->count[k] < 0) continue;
+                float fx = (gp->hi_x[k] - ro.x) * inv.x, fy = (gp->hi_y[k] - ro.y) * inv.y,
+                      fz = (gp->hi_z[k] - ro.z) * inv.z;
+                float nx = (gp->lo_x[k] - ro.x) * inv.x, ny = (gp->lo_y[k] - ro.y) * inv.y,
+                      nz = (gp->lo_z[k] - ro.z) * inv.z;
+                float t1 = mn(mn(mx(fx, nx), mn(mx(fy, ny), mx(fz, nz))) * WIDE_GROW, tmax * WIDE_GROW);
+                float t0 = mx(mx(mn(fx, nx), mx(mn(fy, ny), mn(fz, nz))), 0.0f);
+                if (!(t1 >= t0) || !(t0 > 0.0f || t1 > 0.0f)) continue;
+                if (gp->count[k] > 0) {
+                    if (test_records(s, ro, rd, gp->child[k], gp->count[k], any, &tmax, w)) {
+                        w->iters += (uint32_t)it;
+                        return;
+                    }
+                    continue;
+                }
+                int32_t push_node = gp->child[k];
+                float push_t = t0;
+                if (near_node < 0) {
+                    near_node = gp->child[k];
+                    near_t = t0;
+                    continue;
+                }
+                if (t0 < near_t) {
+                    push_node = near_node;
+                    push_t = near_t;
+                    near_node = gp->child[k];
+                    near_t = t0;
+                }
+                if (sp >= cap) {
+                    overflow = 1;
+                    continue;
+                }
+                st_node[sp] = push_node;
+                st_t[sp] = push_t;
+                ++sp;
+            }
+        }
+        if (overflow) {
+            w->status |= CRT_STATUS_STACK_OVERFLOW;
+            stopped = 1;
+            break;
+        }
+        if (near_node >= 0 && (any || !(near_t > tmax * WIDE_GROW))) {
+            cur = near_node;
+            continue;
+        }
+        int found = 0;
+        while (sp > 0) {
+            --sp;
+            if (!any && st_t[sp] > tmax * WIDE_GROW) continue;
+            cur = st_node[sp];
+            found = 1;
+            break;
+        }
+        if (!found) {
+            stopped = 1;
+            break;
+        }
+    }
+    if (!stopped) w->status |= CRT_STATUS_ITER_CAP;
+    w->iters += (uint32_t)it;
+}
+
 static inline void walk(const scene_t* s, v3 ro, v3 rd, const crt_entity* e, float tmax, int any, walk_t* w) {
     v3 lro = xform(e->inverse, ro, 1.0f);
     v3 lrd = xform(e->inverse, rd, 0.0f);
     if (s->format == CRT_FORMAT_STACKLESS)
         walk_stackless(s, lro, lrd, e->node_offset, e->node_count, tmax, any, w);
-    else
+    else if (s->format == CRT_FORMAT_STACK)
         walk_stack(s, lro, lrd, e->node_offset, e->node_count, tmax, any, w);
+    else
+        walk_wide(s, lro, lrd, e->node_offset, e->node_count, tmax, any, w);
 }
 
 static inline uint32_t pack_status(uint32_t visits, uint32_t flags) {
@@ -378,8 +504,18 @@ static void scene_closest(const scene_t* s, const crt_ray* r, uint32_t flags, cr
         const crt_entity* e = &s->ents[ent];
         v3 lro = xform(e->inverse, ro, 1.0f);
         v3 lrd = xform(e->inverse, rd, 0.0f);
-        const crt_triangle* tr = &s->tris[tri];
-        v3 a = vpos(s, tr->packed[0]), b = vpos(s, tr->packed[1]), c = vpos(s, tr->packed[2]);
+        v3 a, b, c;
+        if (s->format == CRT_FORMAT_WIDE4 || s->format == CRT_FORMAT_WIDE8) {
+            const crt_tri_record* r = &s->recs[tri];
+            a = mk(r->v0[0], r->v0[1], r->v0[2]);
+            b = mk(r->v1[0], r->v1[1], r->v1[2]);
+            c = mk(r->v2[0], r->v2[1], r->v2[2]);
+        } else {
+            const crt_triangle* tr = &s->tris[tri];
+            a = vpos(s, tr->packed[0]);
+            b = vpos(s, tr->packed[1]);
+            c = vpos(s, tr->packed[2]);
+        }
         v3 p = mk(lro.x + lrd.x * closest, lro.y + lrd.y * closest, lro.z + lrd.z * closest);
         v3 uvw = barycentrics(p, a, b, c);
         h->t = closest;
@@ -450,11 +586,15 @@ static void* run_job(void* arg) {
 /* Traces `n` rays on the host with `threads` POSIX threads.  `tri_tests`
  * (optional) receives the triangle tests per ray; node visits are in
  * hit.status.  max_iter <= 0 selects the reference's 1024.  Returns 0. */
+static int run_scene(const scene_t* sp, int query, uint32_t flags, const crt_ray* rays, crt_hit* hits, int64_t n,
+                     int threads, uint32_t* tri_tests);
+
 int oracle_trace(int32_t format, int32_t entity_count, int64_t total_nodes, const void* nodes,
                  const crt_triangle* tris, const crt_vertex* verts, const crt_entity* ents, int query,
                  uint32_t flags, const crt_ray* rays, crt_hit* hits, int64_t n, int threads, uint32_t* tri_tests,
                  int32_t max_iter) {
     scene_t s;
+    memset(&s, 0, sizeof(s));
     s.format = format;
     s.entity_count = entity_count;
     s.total_nodes = total_nodes;
@@ -464,6 +604,33 @@ int oracle_trace(int32_t format, int32_t entity_count, int64_t total_nodes, cons
     s.ents = ents;
     s.max_iter = max_iter > 0 ? max_iter : MAX_ITER_DEFAULT;
     if (format != CRT_FORMAT_STACKLESS && format != CRT_FORMAT_STACK) return -1;
+    return run_scene(&s, query, flags, rays, hits, n, threads, tri_tests);
+}
+
+/* The wide layout: groups = crt_wide_group[total_nodes * width / 4],
+ * records, and the entity copy whose node_offset / node_count are wide
+ * roots and counts.  stack_cap = the GPU walk's LDS depth + 64. */
+int oracle_trace_wide(int32_t width, int32_t entity_count, int64_t total_nodes, const crt_wide_group* groups,
+                      const crt_tri_record* records, const crt_entity* ents, int query, uint32_t flags,
+                      const crt_ray* rays, crt_hit* hits, int64_t n, int threads, uint32_t* tri_tests,
+                      int32_t max_iter, int32_t stack_cap) {
+    scene_t s;
+    memset(&s, 0, sizeof(s));
+    if (width != 4 && width != 8) return -1;
+    s.format = width == 8 ? CRT_FORMAT_WIDE8 : CRT_FORMAT_WIDE4;
+    s.entity_count = entity_count;
+    s.total_nodes = total_nodes;
+    s.nodes = groups;
+    s.recs = records;
+    s.ents = ents;
+    s.stack_cap = stack_cap > 0 ? stack_cap : 80;
+    s.max_iter = max_iter > 0 ? max_iter : MAX_ITER_DEFAULT;
+    return run_scene(&s, query, flags, rays, hits, n, threads, tri_tests);
+}
+
+static int run_scene(const scene_t* sp, int query, uint32_t flags, const crt_ray* rays, crt_hit* hits, int64_t n,
+                     int threads, uint32_t* tri_tests) {
+    const scene_t s = *sp;
     if (threads < 1) threads = 1;
     if (threads > 256) threads = 256;
     if ((int64_t)threads > n) threads = (int)(n > 0 ? n : 1);

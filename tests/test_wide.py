@@ -1,0 +1,265 @@
+"""The wide (BVH4 / BVH8) layout derived from the reference's BVH2 buffers: host checks.
+
+* structure: leaf slots partition the triangles, every wide node is reached
+  exactly once, every slot box is one of the BVH2's node boxes bit for bit and
+  contains what hangs below it, records carry the right vertex positions;
+* traversal: the host walk of the wide layout reports the brute-force nearest
+  hit (t exact, triangle exact up to equal-distance ties) and the same hits as
+  the BVH2 walks on instanced scenes; any-hit agrees with closest-hit within
+  tmax;
+* errors: bad width / leaf size and broken links are rejected.
+"""
+import numpy as np
+import pytest
+
+from oracle_cases import axis_aligned_rays, crafted_stackless, random_rays
+
+
+def _context(crt, fmt, meshes):
+    """Host buffers of a one-object context plus its identity entity (no device needed)."""
+    import oracle
+    r = crt.RayIntersector(fmt)
+    r.add_object(0, meshes)
+    hb = r.host_buffers()
+    hb["entities"] = oracle.identity_entity(hb["nodes"].size)
+    return r, hb
+
+
+def _bvh2_boxes(fmt, nodes):
+    """Set of (lo, hi) float32 byte strings of every BVH2 node box (children boxes for the stack format)."""
+    if fmt == 0:
+        bx = [np.concatenate([nodes["min"][:, :3], nodes["max"][:, :3]], axis=1)]
+    else:
+        bx = [np.concatenate([nodes[s]["min"][:, :3], nodes[s]["max"][:, :3]], axis=1) for s in ("left", "right")]
+    b = np.ascontiguousarray(np.concatenate(bx).astype(np.float32))
+    return {row.tobytes() for row in b}
+
+
+def _slots(w):
+    g = w.groups
+    per = w.width // 4
+    n = g.size // per
+    lo = np.stack([g["lo_x"], g["lo_y"], g["lo_z"]], axis=-1).reshape(n, w.width, 3)
+    hi = np.stack([g["hi_x"], g["hi_y"], g["hi_z"]], axis=-1).reshape(n, w.width, 3)
+    return lo, hi, g["child"].reshape(n, w.width), g["count"].reshape(n, w.width)
+
+
+@pytest.mark.parametrize("fmt", [0, 1])
+@pytest.mark.parametrize("width,max_leaf", [(4, 2), (8, 2), (4, 6), (8, 4)])
+def test_structure(crt, fmt, width, max_leaf):
+    from candela_amd import scenes
+    meshes = scenes.icosphere_cloud(count=60, subdivisions=2, seed=11, box=20.0)
+    _, hb = _context(crt, fmt, meshes)
+    w = crt.wide_build(fmt, hb["nodes"], hb["triangles"], hb["vertices"], hb["entities"], width, max_leaf)
+    lo, hi, child, count = _slots(w)
+    n = lo.shape[0]
+    assert w.stats["node_count"] == n and w.stats["width"] == width
+    root, cnt = int(w.entities["node_offset"][0]), int(w.entities["node_count"][0])
+    assert root == 0 and cnt == n
+    # every wide node reached exactly once from the root
+    seen = np.zeros(n, np.int32)
+    stack = [root]
+    covered = np.zeros(hb["triangles"].size, np.int32)
+    boxes = _bvh2_boxes(fmt, hb["nodes"])
+    verts = hb["vertices"]["position"][:, :3]
+    tris = hb["triangles"]["packed"]
+    while stack:
+        k = stack.pop()
+        seen[k] += 1
+        used = count[k] >= 0
+        assert used.any()
+        assert (used[:used.sum()]).all()                          # used slots come first
+        for j in np.nonzero(used)[0]:
+            box = np.concatenate([lo[k, j], hi[k, j]]).astype(np.float32)
+            assert box.tobytes() in boxes                          # a BVH2 box, bit for bit
+            if count[k, j] > 0:
+                s, c = int(child[k, j]), int(count[k, j])
+                assert c <= max(max_leaf, 15)
+                covered[s:s + c] += 1
+                p = verts[tris[s:s + c, :3].ravel()]
+                assert (p >= lo[k, j] - 0).all() and (p <= hi[k, j] + 0).all()
+            else:
+                c = int(child[k, j])
+                assert root <= c < root + cnt
+                sub_used = count[c] >= 0
+                assert (lo[c][sub_used] >= lo[k, j]).all() and (hi[c][sub_used] <= hi[k, j]).all()
+                stack.append(c)
+    assert (seen == 1).all()
+    assert (covered == 1).all()
+    assert w.stats["leaf_count"] == int((count > 0).sum())
+    assert w.stats["mean_fill"] > (2.5 if width == 4 else 3.5)
+    # records: positions, index and mesh of every triangle
+    rec = w.records
+    assert np.array_equal(rec["tri"], np.arange(tris.shape[0], dtype=np.int32))
+    assert np.array_equal(rec["mesh"], tris[:, 3])
+    assert np.array_equal(rec["v0"], verts[tris[:, 0]]) and np.array_equal(rec["v2"], verts[tris[:, 2]])
+    if max_leaf == 2:
+        assert int((count > 0).sum()) == int(np.count_nonzero(_bvh2_leaf_mask(fmt, hb["nodes"])))
+
+
+def _bvh2_leaf_mask(fmt, nodes):
+    """One entry per BVH2 leaf (stack format: children reachable from entry 0)."""
+    if fmt == 0:
+        return nodes["min"][:, 3].view(np.int32) != -1
+    stack = [0]
+    out = []
+    while stack:
+        e = stack.pop()
+        for s in ("left", "right"):
+            packed = nodes[s]["min"][e, 3].view(np.int32)
+            if packed != -1:
+                out.append(1)
+            else:
+                stack.append(int(nodes[s]["max"][e, 3].view(np.int32)))
+    return np.array(out)
+
+
+@pytest.mark.parametrize("width,max_leaf", [(4, 2), (8, 2), (8, 6)])
+def test_wide_walk_equals_brute_force(crt, orc, width, max_leaf):
+    from candela_amd import scenes
+    meshes = scenes.boxes_scene()
+    _, hb = _context(crt, 0, meshes)
+    w = crt.wide_build(0, hb["nodes"], hb["triangles"], hb["vertices"], hb["entities"], width, max_leaf)
+    rays = np.concatenate([random_rays(3000, [-4, 0.2, -4], [20, 8, 20], seed=3),
+                           random_rays(2000, [-10, 5, -10], [-5, 9, -5], seed=4, toward=[8, 1, 8]),
+                           axis_aligned_rays(500, [-2, 0.1, -2], [18, 5, 18], seed=5)])
+    h = orc.trace_wide(width, w.groups, w.records, w.entities, 0, rays)
+    bt, btri = orc.bruteforce_closest(hb["triangles"], hb["vertices"], hb["entities"], rays)
+    assert (h["status"] & 0xFF000000 == 0).all()
+    assert np.array_equal(h["t"], bt)
+    # where the triangle differs (coplanar faces of touching boxes, shared
+    # edges), the reported triangle lies at exactly the same distance
+    tie = np.nonzero(h["tri"] != btri)[0]
+    assert tie.size < 0.05 * rays.size
+    for i in tie[:300]:
+        k = int(h["tri"][i])
+        t1, _ = orc.bruteforce_closest(hb["triangles"], hb["vertices"], hb["entities"], rays[i:i + 1], k, k + 1)
+        assert t1[0] == h["t"][i]
+    hit = h["t"] > 0
+    s = h["u"][hit] + h["v"][hit] + h["w"][hit]
+    assert np.abs(s - 1).max() < 1e-5
+    # the same barycentrics as the BVH2 stack walk where the triangle is the same
+    st = orc.trace(1, *_stack_scene(crt, meshes), 0, rays)
+    same = st["tri"] == h["tri"]
+    for f in ("t", "u", "v", "w"):
+        assert np.array_equal(st[f][same], h[f][same])
+
+
+def _stack_scene(crt, meshes):
+    _, hb = _context(crt, 1, meshes)
+    return hb["nodes"], hb["triangles"], hb["vertices"], hb["entities"]
+
+
+@pytest.mark.parametrize("src", [0, 1])
+@pytest.mark.parametrize("width", [4, 8])
+def test_wide_walk_equals_bvh2_on_cloud(crt, orc, src, width):
+    """Closest and any hit over a denser cloud, against the BVH2 stack walk (which is brute force)."""
+    from candela_amd import scenes
+    meshes = scenes.icosphere_cloud(count=120, subdivisions=3, seed=13, box=30.0)
+    _, hb = _context(crt, src, meshes)
+    w = crt.wide_build(src, hb["nodes"], hb["triangles"], hb["vertices"], hb["entities"], width, 2)
+    rays = random_rays(30000, [-2, -2, -2], [32, 32, 32], seed=14)
+    ref = orc.trace(1, *_stack_scene(crt, meshes), 0, rays)
+    h, tests = orc.trace_wide(width, w.groups, w.records, w.entities, 0, rays, with_tests=True)
+    assert np.array_equal(h["t"], ref["t"])
+    assert ((h["tri"] == ref["tri"]) | (h["t"] == ref["t"])).all()
+    assert (h["tri"] == ref["tri"]).mean() >= 0.9999
+    # a wide node visit replaces several BVH2 visits
+    v_w = (h["status"] & 0xFFFFFF).astype(np.float64).mean()
+    v_2 = (ref["status"] & 0xFFFFFF).astype(np.float64).mean()
+    assert v_w < 0.75 * v_2
+    # any hit within tmax <=> closest hit within tmax
+    short = rays.copy()
+    short["tmax"] = 3.0
+    c = orc.trace_wide(width, w.groups, w.records, w.entities, 0, short)
+    a = orc.trace_wide(width, w.groups, w.records, w.entities, 1, short)
+    assert np.array_equal(c["t"] > 0, a["t"] > 0)
+    assert (a["t"][a["t"] > 0] < 3.0).all()
+
+
+def _two_objects(crt, fmt):
+    """Boxes (object 0) and a small cloud (object 1); entities: 0, 1 raised by 6, 0 raised by 6."""
+    import oracle
+    from candela_amd import scenes
+    r = crt.RayIntersector(fmt)
+    r.add_object(0, scenes.boxes_scene())
+    n0 = r.host_buffers()["nodes"].size
+    r.add_object(1, scenes.icosphere_cloud(count=20, subdivisions=2, seed=5, box=10.0))
+    hb = r.host_buffers()
+    n1 = hb["nodes"].size - n0
+    up = np.eye(4, dtype=np.float32)
+    up[1, 3] = 6.0
+    ents = np.concatenate([oracle.identity_entity(n0), oracle.identity_entity(n1, n0), oracle.identity_entity(n0)])
+    for k in (1, 2):
+        ents["model"][k] = up.T.reshape(16)
+        ents["inverse"][k] = np.linalg.inv(up).astype(np.float32).T.reshape(16)
+    hb["entities"] = ents
+    return hb
+
+
+def test_instances_and_objects(crt, orc):
+    """Two objects, three entities with transforms: one wide tree per object, entity roots remapped."""
+    hb = _two_objects(crt, 0)
+    w = crt.wide_build(0, hb["nodes"], hb["triangles"], hb["vertices"], hb["entities"], 4, 2)
+    e = w.entities
+    assert e["node_offset"][0] == e["node_offset"][2] and e["node_count"][0] == e["node_count"][2]
+    assert e["node_offset"][1] == e["node_offset"][0] + e["node_count"][0]
+    assert np.array_equal(e["model"], hb["entities"]["model"]) and np.array_equal(e["data"], hb["entities"]["data"])
+    rays = random_rays(20000, [-2, 0.2, -2], [18, 14, 18], seed=21)
+    st = _two_objects(crt, 1)
+    ref = orc.trace(1, st["nodes"], st["triangles"], st["vertices"], st["entities"], 0, rays)
+    h = orc.trace_wide(4, w.groups, w.records, w.entities, 0, rays)
+    assert np.array_equal(h["t"], ref["t"])
+    assert np.array_equal(h["entity"], ref["entity"])
+    assert set(np.unique(h["entity"][h["t"] > 0])) == {0, 1, 2}
+    for flags in (1, 2):
+        alpha = w.entities.copy()
+        alpha["data"][1, 1] = np.float32(0.5).view(np.int32)
+        f = orc.trace_wide(4, w.groups, w.records, alpha, 0, rays, flags=flags)
+        if flags == 1:
+            assert (f["entity"][f["t"] > 0] != 1).all()
+        else:
+            z = f["tri"] == 0
+            assert (f["t"][z] == -1).all()
+
+
+def test_errors(crt):
+    from candela_amd import scenes
+    _, hb = _context(crt, 0, scenes.boxes_scene())
+    args = (hb["nodes"], hb["triangles"], hb["vertices"], hb["entities"])
+    with pytest.raises(crt.CrtError):
+        crt.wide_build(0, *args, 5, 2)
+    with pytest.raises(crt.CrtError):
+        crt.wide_build(0, *args, 4, 1)
+    with pytest.raises(crt.CrtError):
+        crt.wide_build(2, *args, 4, 2)
+    import oracle
+    from oracle_cases import single_triangle_scene
+    v, tris = single_triangle_scene()
+    for kind in ("bad_link", "cycle"):
+        nodes = crafted_stackless(kind)
+        with pytest.raises(crt.CrtError):
+            crt.wide_build(0, nodes, tris, v, oracle.identity_entity(nodes.size), 4, 2)
+    bad = hb["entities"].copy()
+    bad["node_count"][0] = hb["nodes"].size + 1
+    with pytest.raises(crt.CrtError):
+        crt.wide_build(0, hb["nodes"], hb["triangles"], hb["vertices"], bad, 4, 2)
+
+
+def test_single_leaf_root(crt, orc):
+    """A one-leaf object (the stackless root is a leaf) becomes a root with one leaf slot."""
+    import oracle
+    from oracle_cases import single_triangle_scene
+    from candela_amd import scenes
+    v, tris = single_triangle_scene()
+    nodes = np.zeros(1, crt.STACKLESS_NODE_DTYPE)
+    nodes["min"][0, :3] = [0, 0, 0]
+    nodes["max"][0, :3] = [1, 1, 0]
+    nodes["min"][0, 3] = np.array([(0 << 4) | 1], np.int32).view(np.float32)[0]
+    nodes["max"][0, 3] = np.array([-1], np.int32).view(np.float32)[0]
+    w = crt.wide_build(0, nodes, tris, v, oracle.identity_entity(1), 8, 2)
+    assert w.stats["node_count"] == 1 and w.stats["leaf_count"] == 1
+    ray = random_rays(1, [0.2, 0.2, 1], [0.2, 0.2, 1], seed=0)
+    ray["dir"] = [0, 0, -1]
+    h = orc.trace_wide(8, w.groups, w.records, w.entities, 0, ray)
+    assert h["t"][0] == 1.0 and h["tri"][0] == 0 and h["mesh"][0] == 7
