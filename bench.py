@@ -83,24 +83,17 @@ def bytes_per_ray(fmt: int, visits: np.ndarray, tests: np.ndarray) -> np.ndarray
 def kernel_label(c: dict) -> str:
     if c["layout"] == "bvh2":
         return "plain (one batch per launch): %s, preset %d, BVH2 stackless" % (KERNEL_NAMES[c["kernel"]], c["variant"])
-    return "plain (one batch per launch): trace_kernel over the BVH%d layout derived from the BVH2, %d-triangle " \
-           "leaves" % (c["width"], c["max_leaf"])
+    name = "wide_grid, preset %d" % c["variant"] if c["kernel"] == 3 else "trace_kernel"
+    return "plain (one batch per launch): %s over the BVH%d layout derived from the BVH2, %d-triangle leaves" % (
+        name, c["width"], c["max_leaf"])
 
 
 def choice_name(c: dict) -> str:
-    """'K:V' for a BVH2 kernel choice, 'wW:L' for the wide layout of width W and leaf size L."""
+    """'K:V' for a BVH2 kernel choice, 'wWlL:K:V' for kernel K, variant V over the wide layout of width W
+    and leaf size L."""
     if c["layout"] == "bvh2":
         return f"{c['kernel']}:{c['variant']}"
-    return f"w{c['width']}:{c['max_leaf']}"
-
-
-def wide_scene(crt, hb: dict, d_tris, d_verts, width: int, max_leaf: int):
-    """Derives the wide layout from the host BVH2 buffers and uploads it; returns (SceneView, keep-alive, stats)."""
-    w = crt.wide_build(crt.FORMAT_STACKLESS, hb["nodes"], hb["triangles"], hb["vertices"], hb["entities"], width,
-                       max_leaf)
-    dg, dr, de = (crt.DeviceArray.from_host(a) for a in (w.groups, w.records, w.entities))
-    view = crt.SceneView(w.format, w.entities.size, w.stats["node_count"], dg.ptr, d_tris, d_verts, de.ptr, dr.ptr)
-    return view, (dg, dr, de), w
+    return f"w{c['width']}l{c['max_leaf']}:{c['kernel']}:{c['variant']}"
 
 
 def measured_traffic(best: dict, args, fmt: int):
@@ -138,46 +131,63 @@ def default_candidates(crt):
     return [crt.BatchConfig(v, t, w, 0, o, 0) for v in (0, 1, 4) for t in (1, 2) for w in (1, 4) for o in (0, 1)]
 
 
-def batched_vs_plain(crt, args, best_plain: dict, stream) -> dict:
-    """64 scenes (log-uniform 1k..2M triangles) x (primary, GI, shadow) batches: plain launches vs one launch."""
+def batched_vs_plain(crt, args, plain_choice: dict, stream) -> dict:
+    """64 scenes (log-uniform 1k..2M triangles) x (primary, GI, shadow) batches: plain launches vs one launch,
+    over the BVH2 and over the BVH4 / BVH8 layouts derived from it (plain_choice: layout key -> (kernel, variant))."""
     from candela_amd import workloads
     t0 = time.perf_counter()
     work = workloads.batched_scenes(crt, count=args.batched_scenes, seed=3)
     log(f"batched workload: {len(work.scenes)} scenes, {work.triangles} triangles, {work.rays} rays "
         f"(BVH builds {work.build_s:.1f}s, setup {time.perf_counter() - t0:.1f}s)")
     K = args.batched_steps
-    kw = dict(kernel=best_plain["kernel"], variant=best_plain["variant"], stream=stream)
+    per_layout = {}
+    for layout, (kernel, variant) in plain_choice.items():
+        if layout is not None:
+            log(f"wide layouts {layout} of all scenes built in {work.build_wide(crt, *layout):.1f}s")
+        kw = dict(kernel=kernel, variant=variant, stream=stream)
 
-    def plain_step():
-        for s in work.scenes:   # one launch per (scene, query): primary+GI closest, shadow any
-            crt.trace(s.scene, crt.QUERY_CLOSEST, s.d_closest, s.h_closest, **kw)
-            crt.trace(s.scene, crt.QUERY_ANY, s.d_shadow, s.h_shadow, **kw)
+        def plain_step():
+            for s in work.scenes:   # one launch per (scene, query): primary+GI closest, shadow any
+                sc = s.scene_for(layout)
+                crt.trace(sc, crt.QUERY_CLOSEST, s.d_closest, s.h_closest, **kw)
+                crt.trace(sc, crt.QUERY_ANY, s.d_shadow, s.h_shadow, **kw)
 
-    for _ in range(max(1, args.warmup)):
-        plain_step()
-    stream.sync()
-    ev = Events(crt, 2)
-    ev.record(0, stream.handle)
-    for _ in range(K):
-        plain_step()
-    ev.record(1, stream.handle)
-    plain_ms = ev.ms(0, 1) / K
-    ev.close()
-    plain_hits = [(s.h_closest.download(), s.h_shadow.download()) for s in work.scenes]
-    b = time_batched(crt, work.batches(crt), K, args.warmup, stream)
-    same = all(s.h_closest.download().tobytes() == ph[0].tobytes() and
-               s.h_shadow.download().tobytes() == ph[1].tobytes() for s, ph in zip(work.scenes, plain_hits))
-    plain_mr = work.rays / (plain_ms * 1e-3) / 1e6
-    batched_mr = work.rays / (b["ms"] * 1e-3) / 1e6
-    log(f"batched vs plain: plain {2 * len(work.scenes)} launches {plain_ms:.3f} ms ({plain_mr:.1f} Mrays/s), "
-        f"batched one launch {b['ms']:.3f} ms ({batched_mr:.1f} Mrays/s) -> {plain_ms / b['ms']:.2f}x; "
-        f"identical hits: {same}")
+        for _ in range(max(1, args.warmup)):
+            plain_step()
+        stream.sync()
+        ev = Events(crt, 2)
+        ev.record(0, stream.handle)
+        for _ in range(K):
+            plain_step()
+        ev.record(1, stream.handle)
+        plain_ms = ev.ms(0, 1) / K
+        ev.close()
+        plain_hits = [(s.h_closest.download(), s.h_shadow.download()) for s in work.scenes]
+        b = time_batched(crt, work.batches(crt, layout), K, args.warmup, stream)
+        same = all(s.h_closest.download().tobytes() == ph[0].tobytes() and
+                   s.h_shadow.download().tobytes() == ph[1].tobytes() for s, ph in zip(work.scenes, plain_hits))
+        name = "bvh2" if layout is None else f"wide{layout[0]}"
+        per_layout[name] = {"plain_ms": round(plain_ms, 4), "plain_mrays_s": round(work.rays / plain_ms / 1e3, 2),
+                            "plain_kernel": f"{kernel}:{variant}", "batched_ms": round(b["ms"], 4),
+                            "batched_mrays_s": round(work.rays / b["ms"] / 1e3, 2),
+                            "speedup": round(plain_ms / b["ms"], 3), "batched_config": b["config"],
+                            "hits_identical": bool(same)}
+        log(f"{name}: plain {2 * len(work.scenes)} launches {plain_ms:.3f} ms ({work.rays / plain_ms / 1e3:.1f} "
+            f"Mrays/s), batched one launch {b['ms']:.3f} ms ({work.rays / b['ms'] / 1e3:.1f} Mrays/s) -> "
+            f"{plain_ms / b['ms']:.2f}x; identical hits: {same}")
+    best_plain = min(per_layout.items(), key=lambda kv: kv[1]["plain_ms"])
+    best_batched = min(per_layout.items(), key=lambda kv: kv[1]["batched_ms"])
     return {"workload": f"{len(work.scenes)} icosphere clouds, {work.triangles} triangles, {work.rays} rays "
                         "(40% primary, 40% GI closest, 20% shadow any-hit)",
-            "plain_launches": 2 * len(work.scenes), "plain_ms": round(plain_ms, 4),
-            "plain_mrays_s": round(plain_mr, 2), "batched_ms": round(b["ms"], 4),
-            "batched_mrays_s": round(batched_mr, 2), "speedup": round(plain_ms / b["ms"], 3),
-            "batched_config": b["config"], "hits_identical": bool(same)}
+            "plain_launches": 2 * len(work.scenes),
+            "plain_ms": best_plain[1]["plain_ms"], "plain_layout": best_plain[0],
+            "plain_mrays_s": best_plain[1]["plain_mrays_s"],
+            "batched_ms": best_batched[1]["batched_ms"], "batched_layout": best_batched[0],
+            "batched_mrays_s": best_batched[1]["batched_mrays_s"],
+            "speedup": round(best_plain[1]["plain_ms"] / best_batched[1]["batched_ms"], 3),
+            "speedup_same_layout": {k: v["speedup"] for k, v in per_layout.items()},
+            "hits_identical": all(v["hits_identical"] for v in per_layout.values()),
+            "layouts": per_layout}
 
 
 def copy_probe(crt, args) -> int:
@@ -218,7 +228,7 @@ def main(argv=None) -> int:
     ap.add_argument("--batched-steps", type=int, default=5)
     ap.add_argument("--only", default="", metavar="KERNEL:VARIANT",
                     help="time one kernel choice only (e.g. 1:0 one ray per lane, 3:1 wave walk preset 1, "
-                         "w8:2 the BVH8 layout with 2-triangle leaves)")
+                         "w4l2:3:0 the wave walk preset 0 over the BVH4 layout with 2-triangle leaves)")
     ap.add_argument("--no-wide", action="store_true", help="time the BVH2 kernels only")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="instead of the workload, time the streaming copy kernel over BYTES (HBM probe and "
@@ -279,11 +289,11 @@ def main(argv=None) -> int:
         key = (c["width"], c["max_leaf"])
         if key not in wides:
             t = time.perf_counter()
-            wides[key] = wide_scene(crt, hb, scene.triangles, scene.vertices, *key)
-            st = wides[key][2].stats
+            wides[key] = crt.upload_wide(fmt, hb, *key)
+            st = wides[key].host.stats
             log(f"wide layout {key}: {st['node_count']} nodes, fill {st['mean_fill']:.2f}, depth {st['max_depth']}, "
                 f"built in {time.perf_counter() - t:.2f}s")
-        return wides[key][0]
+        return wides[key].scene
 
     def run(c: dict) -> dict:
         """W warm-up + K timed steps of (AO any-hit, GI closest-hit) with one kernel choice."""
@@ -322,17 +332,22 @@ def main(argv=None) -> int:
     def bvh2(k, v):
         return dict(layout="bvh2", kernel=k, variant=v)
 
-    def wide(w, leaf):
-        return dict(layout=f"wide{w}", width=w, max_leaf=leaf, kernel=crt.KERNEL_THREAD_PER_RAY, variant=0)
+    def wide(w, leaf, kernel=crt.KERNEL_WAVE, variant=0):
+        return dict(layout=f"wide{w}", width=w, max_leaf=leaf, kernel=kernel, variant=variant)
 
     candidates = [bvh2(crt.KERNEL_THREAD_PER_RAY, 0)]
     if fmt == crt.FORMAT_STACKLESS:
         candidates += [bvh2(crt.KERNEL_WAVE, v) for v in range(6)] + [bvh2(crt.KERNEL_PERSISTENT, v) for v in range(6)]
         if not args.no_wide:
-            candidates += [wide(w, leaf) for w in (4, 8) for leaf in (2, 4)]
+            candidates += [wide(w, 2, crt.KERNEL_THREAD_PER_RAY) for w in (4, 8)]
+            candidates += [wide(w, leaf, crt.KERNEL_WAVE, v) for w in (4, 8) for leaf in (2, 4) for v in range(6)]
     if args.only:
-        a, b = args.only.split(":")
-        candidates = [wide(int(a[1:]), int(b)) if a.startswith("w") else bvh2(int(a), int(b))]
+        parts = args.only.split(":")
+        if parts[0].startswith("w"):
+            w, leaf = (int(x) for x in parts[0][1:].split("l"))
+            candidates = [wide(w, leaf, int(parts[1]), int(parts[2]))]
+        else:
+            candidates = [bvh2(int(parts[0]), int(parts[1]))]
     results = [run(c) for c in candidates]
     thread = results[0]
     best = max(results, key=lambda r: r["mrays"])
@@ -367,12 +382,23 @@ def main(argv=None) -> int:
     batched = None
     c2_batched = None
     if not args.skip_batched and fmt == crt.FORMAT_STACKLESS:
-        plain_bvh2 = max((r for r in results if r["layout"] == "bvh2"), key=lambda r: r["mrays"])
-        c2_batched = time_batched(crt, [(scene, crt.QUERY_ANY, d_ao, h_ao), (scene, crt.QUERY_CLOSEST, d_gi, h_gi)],
+        def best_of(pred):
+            rs = [r for r in results if pred(r)]
+            return max(rs, key=lambda r: r["mrays"]) if rs else None
+
+        plain_choice = {}
+        b2 = best_of(lambda r: r["layout"] == "bvh2")
+        plain_choice[None] = (b2["kernel"], b2["variant"])
+        for w in (4, 8):
+            bw = best_of(lambda r: r["layout"] == f"wide{w}" and r["max_leaf"] == 2)
+            if bw is not None:
+                plain_choice[(w, 2)] = (bw["kernel"], bw["variant"])
+        c2_batched = time_batched(crt, [(best_scene, crt.QUERY_ANY, d_ao, h_ao),
+                                        (best_scene, crt.QUERY_CLOSEST, d_gi, h_gi)],
                                   K, args.warmup, stream)
         log(f"config-2 batches in one persistent launch: {c2_batched['ms']:.3f} ms/step -> "
             f"{rays_step / (c2_batched['ms'] * 1e-3) / 1e6:.1f} Mrays/s ({c2_batched['config']})")
-        batched = batched_vs_plain(crt, args, plain_bvh2, stream)
+        batched = batched_vs_plain(crt, args, plain_choice, stream)
 
     # --- CPU baseline on a bounded sample --------------------------------------------------
     cpu = None
@@ -406,7 +432,7 @@ def main(argv=None) -> int:
         if best["layout"] == "bvh2":
             twin_g, twin_a = ohg, oha
         else:   # the host twin of the wide walk, on the same wide layout
-            w = wides[(best["width"], best["max_leaf"])][2]
+            w = wides[(best["width"], best["max_leaf"])].host
             twin_g = oracle.trace_wide(w.width, w.groups, w.records, w.entities, 0, sample_gi, threads=threads)
             twin_a = oracle.trace_wide(w.width, w.groups, w.records, w.entities, 1, sample_ao, threads=threads)
         parity = {"rays": 2 * n_s,

@@ -28,7 +28,7 @@ __all__ = [
     "KERNEL_WAVE", "CrtError", "DeviceArray", "Stream", "BuiltBVH", "build_bvh",
     "RayIntersector", "trace", "device_count", "node_dtype", "lib", "Batch", "BatchConfig", "BatchPlan",
     "autotune_batched", "ShadeParams", "SHADE_RECORD_DTYPE", "ShadingModel", "FORMAT_WIDE4", "FORMAT_WIDE8",
-    "WIDE_GROUP_DTYPE", "TRI_RECORD_DTYPE", "WideBVH", "wide_build",
+    "WIDE_GROUP_DTYPE", "TRI_RECORD_DTYPE", "WideBVH", "wide_build", "DeviceWide", "upload_wide",
 ]
 
 
@@ -194,6 +194,26 @@ def wide_build(src_format: int, nodes: np.ndarray, triangles: np.ndarray, vertic
     finally:
         lib().crt_wide_free(h)
     return WideBVH(int(width), groups, records, ents, stats)
+
+
+class DeviceWide:
+    """A wide layout uploaded to the device, with its crt_scene_view (scene)."""
+
+    def __init__(self, w: WideBVH, triangles=None, vertices=None):
+        self.host = w
+        self.groups = DeviceArray.from_host(w.groups)
+        self.records = DeviceArray.from_host(w.records)
+        self.entities = DeviceArray.from_host(w.entities)
+        self.scene = SceneView(w.format, w.entities.size, w.stats["node_count"], self.groups.ptr,
+                               triangles.ptr if triangles is not None else None,
+                               vertices.ptr if vertices is not None else None, self.entities.ptr, self.records.ptr)
+
+
+def upload_wide(src_format: int, host_buffers: dict, width: int = 4, max_leaf: int = 2) -> DeviceWide:
+    """Derives the wide layout from a context's host buffers and uploads it."""
+    hb = host_buffers
+    return DeviceWide(wide_build(src_format, hb["nodes"], hb["triangles"], hb["vertices"], hb["entities"], width,
+                                 max_leaf))
 
 
 def _colmajor(m) -> np.ndarray:

@@ -25,6 +25,7 @@
 #include "crt_internal.h"
 #include "trace_launch.h"
 #include "wave_walk.h"
+#include "wide_walk.h"
 
 namespace crt {
 
@@ -69,6 +70,48 @@ __global__ void __launch_bounds__(64 * WPG) batched_stackless(const BatchDesc* _
     }
 }
 
+// The same tile queue over the wide layouts (BVH4 / BVH8): each claimed tile
+// runs the wave-scheduled wide walk (wide_walk.h), so every hit record equals
+// crt_trace's wide wave kernel on the same batch.
+template <int W, int LDS_DEPTH, int INNER, int LEAF_BATCH, int WPG>
+__global__ void __launch_bounds__(64 * WPG) batched_wide(const BatchDesc* __restrict__ descs,
+                                                         const int2* __restrict__ tiles, int32_t ntiles,
+                                                         int32_t per_claim, unsigned* __restrict__ counter,
+                                                         int32_t max_iter) {
+    __shared__ int32_t lds[wide::lds_ints<W, LDS_DEPTH>() * 64 * WPG];
+    int32_t spill_node[wide::kSpill];
+    float spill_t[wide::kSpill];
+    wide::Stack<LDS_DEPTH> st;
+    st.lds = lds + threadIdx.x;
+    st.stride = 64 * WPG;
+    st.spill_node = spill_node;
+    st.spill_t = spill_t;
+    while (true) {
+        unsigned base = 0;
+        if (__lane_id() == 0) base = atomicAdd(counter, unsigned(per_claim));
+        base = unsigned(__builtin_amdgcn_readfirstlane(int(__shfl(int(base), 0))));
+        if (base >= unsigned(ntiles)) break;
+        for (int32_t k = 0; k < per_claim; ++k) {
+            const int32_t t = int32_t(base) + k;
+            if (t >= ntiles) break;
+            const int2 tile = tiles[t];
+            const int32_t b = __builtin_amdgcn_readfirstlane(tile.x);
+            const int32_t first = __builtin_amdgcn_readfirstlane(tile.y);
+            const BatchDesc d = descs[b];
+            const int64_t i = int64_t(first) + __lane_id();
+            const int64_t slot = i < d.count ? i : -1;
+            // one walk object per branch: their registers need not coexist
+            if (d.query == CRT_QUERY_ANY) {
+                wide::WideWave<true, W, LDS_DEPTH, INNER, LEAF_BATCH> w;
+                w.run(d.scene, d.rays, d.hits, slot, d.flags, max_iter, d.tri_tests, st);
+            } else {
+                wide::WideWave<false, W, LDS_DEPTH, INNER, LEAF_BATCH> w;
+                w.run(d.scene, d.rays, d.hits, slot, d.flags, max_iter, d.tri_tests, st);
+            }
+        }
+    }
+}
+
 __global__ void reset_u32(unsigned* c) { *c = 0u; }
 
 }  // namespace pdev
@@ -87,11 +130,32 @@ static const void* batched_symbol(int variant) {
     }
 }
 
-static const void* batched_kernel(int variant, int wpg) {
+// Wide presets (LDS stack entries, INNER, LEAF_BATCH) -- the same as the
+// plain wide wave kernel's (wide.hip).
+template <int W, int WPG>
+static const void* batched_wide_symbol(int variant) {
+    switch (variant) {
+        case 1: return reinterpret_cast<const void*>(&pdev::batched_wide<W, 8, 8, 16, WPG>);
+        case 2: return reinterpret_cast<const void*>(&pdev::batched_wide<W, 8, 2, 8, WPG>);
+        case 3: return reinterpret_cast<const void*>(&pdev::batched_wide<W, 16, 4, 16, WPG>);
+        case 4: return reinterpret_cast<const void*>(&pdev::batched_wide<W, 8, 4, 32, WPG>);
+        case 5: return reinterpret_cast<const void*>(&pdev::batched_wide<W, 8, 16, 48, WPG>);
+        default: return reinterpret_cast<const void*>(&pdev::batched_wide<W, 8, 4, 16, WPG>);
+    }
+}
+
+template <int WPG>
+static const void* kernel_for(int format, int variant) {
+    if (format == CRT_FORMAT_WIDE4) return batched_wide_symbol<4, WPG>(variant);
+    if (format == CRT_FORMAT_WIDE8) return batched_wide_symbol<8, WPG>(variant);
+    return batched_symbol<WPG>(variant);
+}
+
+static const void* batched_kernel(int format, int variant, int wpg) {
     switch (wpg) {
-        case 1: return batched_symbol<1>(variant);
-        case 2: return batched_symbol<2>(variant);
-        default: return batched_symbol<4>(variant);
+        case 1: return kernel_for<1>(format, variant);
+        case 2: return kernel_for<2>(format, variant);
+        default: return kernel_for<4>(format, variant);
     }
 }
 
@@ -162,17 +226,25 @@ int build_plan(const crt_batch* batches, int nbatches, const crt_batch_config& c
     p.batches.assign(batches, batches + nbatches);
     p.cfg = cfg;
     std::vector<crt::BatchDesc> descs(static_cast<size_t>(nbatches));
+    const int format = nbatches > 0 ? batches[0].scene.format : CRT_FORMAT_STACKLESS;
+    const bool wide = format == CRT_FORMAT_WIDE4 || format == CRT_FORMAT_WIDE8;
     for (int i = 0; i < nbatches; ++i) {
         const crt_batch& b = batches[i];
-        if (b.scene.format != CRT_FORMAT_STACKLESS)
-            return crt::set_error(CRT_E_INVALID, "batched traversal walks the stackless format only");
+        if (b.scene.format != CRT_FORMAT_STACKLESS && b.scene.format != CRT_FORMAT_WIDE4 &&
+            b.scene.format != CRT_FORMAT_WIDE8)
+            return crt::set_error(CRT_E_INVALID, "batched traversal walks the stackless and wide formats");
+        if (b.scene.format != format)
+            return crt::set_error(CRT_E_INVALID, "batch plan: every batch must use the same node format");
         if (b.query != CRT_QUERY_CLOSEST && b.query != CRT_QUERY_ANY)
             return crt::set_error(CRT_E_INVALID, "batch: bad query");
         if (b.count < 0 || (b.count > 0 && (!b.rays || !b.hits)))
             return crt::set_error(CRT_E_INVALID, "batch: bad ray/hit buffers");
-        if (b.count > 0 && b.scene.entity_count > 0 &&
+        if (b.count > 0 && b.scene.entity_count > 0 && !wide &&
             (!b.scene.nodes || !b.scene.triangles || !b.scene.vertices || !b.scene.entities || b.scene.total_nodes <= 0))
             return crt::set_error(CRT_E_INVALID, "batch: null scene buffer");
+        if (b.count > 0 && b.scene.entity_count > 0 && wide &&
+            (!b.scene.nodes || !b.scene.tri_records || !b.scene.entities || b.scene.total_nodes <= 0))
+            return crt::set_error(CRT_E_INVALID, "batch: null scene buffer (wide layout)");
         if (b.count >= (int64_t(1) << 31)) return crt::set_error(CRT_E_INVALID, "batch: >= 2^31 rays");
         crt::BatchDesc& d = descs[size_t(i)];
         d.scene = crt::scene_args(b.scene);
@@ -206,7 +278,7 @@ int build_plan(const crt_batch* batches, int nbatches, const crt_batch_config& c
     if (!tiles.empty())
         PLAN_HIP(hipMemcpy(p.d_tiles, tiles.data(), tiles.size() * sizeof(int2), hipMemcpyHostToDevice),
                  "hipMemcpy(tiles)");
-    p.kernel = crt::batched_kernel(cfg.variant, cfg.waves_per_group);
+    p.kernel = crt::batched_kernel(format, cfg.variant, cfg.waves_per_group);
     int grid = cfg.grid;
     if (grid == 0) {
         int dev = 0, cus = 0, per_cu = 0;

@@ -105,19 +105,31 @@ int trace(const crt_scene_view* scene, int query, uint32_t flags, const crt_ray*
     const int kernel = cfg ? cfg->kernel : CRT_KERNEL_AUTO;
     if (kernel < CRT_KERNEL_AUTO || kernel > CRT_KERNEL_WAVE)
         return set_error(CRT_E_INVALID, "crt_trace: unknown kernel");
-    if ((kernel == CRT_KERNEL_PERSISTENT || kernel == CRT_KERNEL_WAVE) && scene->format != CRT_FORMAT_STACKLESS)
-        return set_error(CRT_E_INVALID, "crt_trace: the persistent and wave kernels walk the stackless format only");
+    if (kernel == CRT_KERNEL_PERSISTENT && scene->format != CRT_FORMAT_STACKLESS)
+        return set_error(CRT_E_INVALID, "crt_trace: the persistent kernel walks the stackless format only");
+    if (kernel == CRT_KERNEL_WAVE && scene->format == CRT_FORMAT_STACK)
+        return set_error(CRT_E_INVALID, "crt_trace: the wave kernels walk the stackless and wide formats");
     if (is_wide(scene->format) && (depth != 8 && depth != 16 && depth != 32))
         return set_error(CRT_E_INVALID, "crt_trace: wide formats take lds_stack 8, 16 or 32");
-    if (is_wide(scene->format) && size_t(depth) * size_t(block) * 8 > 64 * 1024)
-        return set_error(CRT_E_INVALID, "trace config: 2 * lds_stack * block_size * 4 exceeds 64 KiB of LDS");
+    if (is_wide(scene->format) && size_t(2 * depth + 8) * size_t(block) * 4 > 64 * 1024)
+        return set_error(CRT_E_INVALID,
+                         "trace config: (2 * lds_stack + width) * block_size * 4 exceeds 64 KiB of LDS");
     if (cfg && (cfg->variant < 0 || cfg->variant > 5 || cfg->grid < 0))
         return set_error(CRT_E_INVALID, "crt_trace: variant must be 0..5 and grid >= 0");
     const SceneArgs a = scene_args(*scene);
     uint32_t* tri_tests = cfg ? cfg->tri_tests : nullptr;
     const bool persistent = kernel == CRT_KERNEL_PERSISTENT;
-    const bool wave = kernel == CRT_KERNEL_WAVE || (kernel == CRT_KERNEL_AUTO && scene->format == CRT_FORMAT_STACKLESS);
+    const bool wave = kernel == CRT_KERNEL_WAVE ||
+                      (kernel == CRT_KERNEL_AUTO && (scene->format == CRT_FORMAT_STACKLESS || is_wide(scene->format)));
     if (n == 0) return CRT_OK;
+    if (wave && is_wide(scene->format)) {
+        if (n >= (int64_t(1) << 31) * 256 - 2048)
+            return set_error(CRT_E_INVALID, "crt_trace: too many rays for one launch");
+        CRT_HIP(launch_wide_grid(a, query, rays, hits, n, flags, kMaxIterations, tri_tests, cfg ? cfg->variant : 0,
+                                 static_cast<hipStream_t>(stream)),
+                "crt_trace: wide wave kernel launch");
+        return CRT_OK;
+    }
     if (wave) {
         if (n >= (int64_t(1) << 31) * 256 - 2048)
             return set_error(CRT_E_INVALID, "crt_trace: too many rays for one launch");

@@ -4,8 +4,8 @@
 // function of Source/Core/Shaders/Intersectors/Include/:
 //   walk_stackless<ANY>  TraverseBVHStackless.glsl:175-278 / :463-556
 //   walk_stack<ANY>      TraverseBVHStack.glsl:168-324 / :509-657
-//   walk_wide<ANY, W>    the same closest / any-hit contract over the wide
-//                        layout derived from either (no reference counterpart)
+//   wide::walk_wide_thread  the same closest / any-hit contract over the wide
+//                        layout derived from either (wide_walk.h)
 //   trace_one            IntersectScene / IntersectSceneIgnoreTransparent
 //                        (TraverseBVHStackless.glsl:280-366) and the
 //                        occlusion IntersectScene (:558-579)
@@ -20,6 +20,7 @@
 #include "candela_rt.h"
 #include "device_math.h"
 #include "trace_launch.h"
+#include "wide_walk.h"
 
 namespace crt {
 namespace dev {
@@ -167,160 +168,6 @@ __device__ __forceinline__ uint32_t walk_stack(const SceneArgs& s, f3 ro, f3 rd,
     return uint32_t(it);
 }
 
-// Triangles [start, start+len) of the wide layout's records (three 16-byte
-// loads each: v0 + triangle index, v1 + mesh id, v2).  Same test and
-// acceptance as test_leaf.
-template <bool ANY>
-__device__ __forceinline__ bool test_records(const SceneArgs& s, f3 ro, f3 rd, int32_t start, int32_t len,
-                                             float& tmax, Walk& w) {
-    const float4* __restrict__ rec = reinterpret_cast<const float4*>(s.tri_records);
-    for (int32_t idx = start; idx < start + len; ++idx) {
-        const float4 a = rec[3 * int64_t(idx)], b = rec[3 * int64_t(idx) + 1], c = rec[3 * int64_t(idx) + 2];
-        const float t = tri_t(ro, rd, mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), mk(c.x, c.y, c.z));
-        w.tests++;
-        if (t > 0.0f && t < tmax) {
-            tmax = t;
-            w.t = t;
-            w.tri = __float_as_int(a.w);
-            w.mesh = __float_as_int(b.w);
-            if (ANY) return true;
-        }
-    }
-    return false;
-}
-
-// Relative margin of the wide walk's box tests.  A triangle lying on a face
-// of its leaf box (touching boxes, axis-aligned geometry) can have a computed
-// t a few ulps below the box's computed entry distance; with a closer-by-ulps
-// hit already found, an exact comparison would cull the leaf holding the true
-// nearest triangle.  Growing the exit distance and tmax by 2^-19 (16 ulps)
-// keeps such leaves, so the walk reports the brute-force nearest hit.
-constexpr float kWideGrow = 1.0f + 1.0f / 524288.0f;
-
-// Walk of a wide tree (CRT_FORMAT_WIDE4 / WIDE8, W children per node, W / 4
-// groups of 128 bytes).  Per node: every non-empty child box is tested with
-// the slab arithmetic of the stack include's RayBounds (entry t0 =
-// max(near, 0), exit t1 = min(far, tmax), both upper ends grown by
-// kWideGrow; hit when t1 >= t0 and t0 or t1 is positive); hit leaves are
-// tested at once; of the hit inner children the nearest is visited next and
-// the others are pushed with their entry distance.  A popped entry whose
-// entry distance exceeds the grown current closest t is skipped -- exactly
-// the entries whose box test would now fail.  Stack entries (node, entry
-// distance) live in LDS for the first LDS_DEPTH, then in per-lane scratch;
-// beyond LDS_DEPTH + 64 the walk stops with STACK_OVERFLOW.
-template <bool ANY, int W, int LDS_DEPTH>
-__device__ __forceinline__ uint32_t walk_wide(const SceneArgs& s, f3 ro, f3 rd, int32_t root, int32_t count,
-                                              float tmax, int32_t max_iter, Walk& w, uint32_t& status, int32_t* lds,
-                                              int32_t lds_stride) {
-    constexpr int G = W / 4;
-    constexpr int kSpill = 64;
-    const float4* __restrict__ groups = reinterpret_cast<const float4*>(s.nodes);
-    const f3 inv = mk(1.0f / rd.x, 1.0f / rd.y, 1.0f / rd.z);
-    int32_t spill_node[kSpill];
-    float spill_t[kSpill];
-    int32_t sp = 0;
-    int32_t it = 0;
-    int32_t cur = root;
-    bool stopped = false;
-    w.t = -1.0f;
-    w.tri = -1;
-    w.mesh = -1;
-    while (it < max_iter) {
-        if (cur < root || cur >= root + count || int64_t(cur) >= s.total_nodes) {
-            status |= CRT_STATUS_BAD_INDEX;
-            stopped = true;
-            break;
-        }
-        it++;
-        int32_t near_node = -1;
-        float near_t = 0.0f;
-        bool overflow = false;
-#pragma unroll
-        for (int g = 0; g < G; ++g) {
-            const float4* p = groups + (int64_t(cur) * G + g) * 8;
-            const float4 lox = p[0], hix = p[1], loy = p[2], hiy = p[3], loz = p[4], hiz = p[5];
-            const float4 chf = p[6], cnf = p[7];
-            const float lx[4] = {lox.x, lox.y, lox.z, lox.w}, hx[4] = {hix.x, hix.y, hix.z, hix.w};
-            const float ly[4] = {loy.x, loy.y, loy.z, loy.w}, hy[4] = {hiy.x, hiy.y, hiy.z, hiy.w};
-            const float lz[4] = {loz.x, loz.y, loz.z, loz.w}, hz[4] = {hiz.x, hiz.y, hiz.z, hiz.w};
-            const int32_t ch[4] = {__float_as_int(chf.x), __float_as_int(chf.y), __float_as_int(chf.z),
-                                   __float_as_int(chf.w)};
-            const int32_t cn[4] = {__float_as_int(cnf.x), __float_as_int(cnf.y), __float_as_int(cnf.z),
-                                   __float_as_int(cnf.w)};
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                if (cn[k] < 0) continue;
-                const float fx = (hx[k] - ro.x) * inv.x, fy = (hy[k] - ro.y) * inv.y, fz = (hz[k] - ro.z) * inv.z;
-                const float nx = (lx[k] - ro.x) * inv.x, ny = (ly[k] - ro.y) * inv.y, nz = (lz[k] - ro.z) * inv.z;
-                const float t1 = mn(mn(mx(fx, nx), mn(mx(fy, ny), mx(fz, nz))) * kWideGrow, tmax * kWideGrow);
-                const float t0 = mx(mx(mn(fx, nx), mx(mn(fy, ny), mn(fz, nz))), 0.0f);
-                if (!(t1 >= t0) || !(t0 > 0.0f || t1 > 0.0f)) continue;
-                if (cn[k] > 0) {
-                    if (test_records<ANY>(s, ro, rd, ch[k], cn[k], tmax, w)) return uint32_t(it);
-                    continue;
-                }
-                int32_t push_node = ch[k];
-                float push_t = t0;
-                if (near_node < 0) {
-                    near_node = ch[k];
-                    near_t = t0;
-                    continue;
-                }
-                if (t0 < near_t) {
-                    push_node = near_node;
-                    push_t = near_t;
-                    near_node = ch[k];
-                    near_t = t0;
-                }
-                if (sp >= LDS_DEPTH + kSpill) {
-                    overflow = true;
-                    continue;
-                }
-                if (sp < LDS_DEPTH) {
-                    lds[(2 * sp) * lds_stride] = push_node;
-                    lds[(2 * sp + 1) * lds_stride] = __float_as_int(push_t);
-                } else {
-                    spill_node[sp - LDS_DEPTH] = push_node;
-                    spill_t[sp - LDS_DEPTH] = push_t;
-                }
-                ++sp;
-            }
-        }
-        if (overflow) {
-            status |= CRT_STATUS_STACK_OVERFLOW;
-            stopped = true;
-            break;
-        }
-        if (near_node >= 0 && (ANY || !(near_t > tmax * kWideGrow))) {
-            cur = near_node;
-            continue;
-        }
-        bool found = false;
-        while (sp > 0) {
-            --sp;
-            int32_t node;
-            float t;
-            if (sp < LDS_DEPTH) {
-                node = lds[(2 * sp) * lds_stride];
-                t = __int_as_float(lds[(2 * sp + 1) * lds_stride]);
-            } else {
-                node = spill_node[sp - LDS_DEPTH];
-                t = spill_t[sp - LDS_DEPTH];
-            }
-            if (!ANY && t > tmax * kWideGrow) continue;
-            cur = node;
-            found = true;
-            break;
-        }
-        if (!found) {
-            stopped = true;
-            break;
-        }
-    }
-    if (!stopped) status |= CRT_STATUS_ITER_CAP;
-    return uint32_t(it);
-}
-
 template <int FORMAT, bool ANY, int LDS_DEPTH>
 __device__ __forceinline__ uint32_t walk_entity(const SceneArgs& s, f3 ro, f3 rd, int32_t start, int32_t count,
                                                 float tmax, int32_t max_iter, Walk& w, uint32_t& status,
@@ -330,8 +177,8 @@ __device__ __forceinline__ uint32_t walk_entity(const SceneArgs& s, f3 ro, f3 rd
     else if constexpr (FORMAT == CRT_FORMAT_STACK)
         return walk_stack<ANY, LDS_DEPTH>(s, ro, rd, start, count, tmax, max_iter, w, status, lds, lds_stride);
     else
-        return walk_wide<ANY, FORMAT == CRT_FORMAT_WIDE8 ? 8 : 4, LDS_DEPTH>(s, ro, rd, start, count, tmax,
-                                                                              max_iter, w, status, lds, lds_stride);
+        return wide::walk_wide_thread<ANY, FORMAT == CRT_FORMAT_WIDE8 ? 8 : 4, LDS_DEPTH>(
+            s, ro, rd, start, count, tmax, max_iter, w.t, w.tri, w.mesh, w.tests, status, lds, lds_stride);
 }
 
 

@@ -90,9 +90,13 @@ template <int FORMAT, bool ANY, int D>
 static hipError_t launch_trace_d(const SceneArgs& s, const crt_ray* rays, crt_hit* hits, int64_t n, uint32_t flags,
                                  int32_t max_iter, int block, uint32_t* tri_tests, hipStream_t stream) {
     const int64_t grid = (n + block - 1) / block;
-    // Stack format: one int per entry; wide formats: node + entry distance.
-    const size_t per_entry = FORMAT == CRT_FORMAT_STACK ? 1 : FORMAT == CRT_FORMAT_STACKLESS ? 0 : 2;
-    const size_t lds = per_entry * size_t(D) * block * sizeof(int32_t);
+    // Stack format: one int per entry; wide formats: node + entry distance per
+    // entry, then the pending leaves of one node.
+    const size_t per_lane = FORMAT == CRT_FORMAT_STACK       ? size_t(D)
+                            : FORMAT == CRT_FORMAT_STACKLESS ? 0
+                            : FORMAT == CRT_FORMAT_WIDE8     ? size_t(wide::lds_ints<8, D>())
+                                                             : size_t(wide::lds_ints<4, D>());
+    const size_t lds = per_lane * block * sizeof(int32_t);
     hipLaunchKernelGGL((dev::trace_kernel<FORMAT, ANY, D>), dim3(unsigned(grid)), dim3(unsigned(block)), lds, stream,
                        s, rays, hits, n, flags, max_iter, tri_tests);
     return hipGetLastError();

@@ -77,6 +77,11 @@ class SceneBatch:
     d_shadow: object = None
     h_closest: object = None
     h_shadow: object = None
+    wide: dict = field(default_factory=dict)   # (width, max_leaf) -> DeviceWide
+
+    def scene_for(self, layout):
+        """The BVH2 scene (layout None) or the wide layout (width, max_leaf)."""
+        return self.scene if layout is None else self.wide[layout].scene
 
 
 @dataclass
@@ -92,16 +97,27 @@ class BatchedWorkload:
     def triangles(self) -> int:
         return sum(s.triangles for s in self.scenes)
 
-    def batches(self, crt):
-        """(scene, query, rays, hits) per ray type: primary, GI (closest) and shadow (any) of every scene."""
+    def batches(self, crt, layout=None):
+        """(scene, query, rays, hits) per ray type: primary, GI (closest) and shadow (any) of every scene,
+        over the BVH2 (layout None) or the wide layout (width, max_leaf) built by build_wide."""
         out = []
         for s in self.scenes:
-            out.append((s.scene, QUERY_CLOSEST, DeviceSpan(s.d_closest, 0, s.primary),
+            sc = s.scene_for(layout)
+            out.append((sc, QUERY_CLOSEST, DeviceSpan(s.d_closest, 0, s.primary),
                         DeviceSpan(s.h_closest, 0, s.primary)))
-            out.append((s.scene, QUERY_CLOSEST, DeviceSpan(s.d_closest, s.primary, s.gi),
+            out.append((sc, QUERY_CLOSEST, DeviceSpan(s.d_closest, s.primary, s.gi),
                         DeviceSpan(s.h_closest, s.primary, s.gi)))
-            out.append((s.scene, QUERY_ANY, s.d_shadow, s.h_shadow))
+            out.append((sc, QUERY_ANY, s.d_shadow, s.h_shadow))
         return out
+
+    def build_wide(self, crt, width: int, max_leaf: int = 2) -> float:
+        """Derives and uploads the wide layout of every scene; returns the host seconds spent."""
+        import time
+        t0 = time.perf_counter()
+        for s in self.scenes:
+            if (width, max_leaf) not in s.wide:
+                s.wide[(width, max_leaf)] = crt.upload_wide(FORMAT_STACKLESS, s.host, width, max_leaf)
+        return time.perf_counter() - t0
 
 
 def _log_uniform(rng, lo: float, hi: float, n: int) -> np.ndarray:

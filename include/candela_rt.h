@@ -252,12 +252,14 @@ typedef struct crt_scene_view {
 
 /* Kernel choice and launch shape. */
 enum {
-    CRT_KERNEL_AUTO = 0,        /* persistent where available, else thread-per-ray   */
+    CRT_KERNEL_AUTO = 0,        /* the wave kernel for stackless and wide formats,
+                                   thread-per-ray for the stack format              */
     CRT_KERNEL_THREAD_PER_RAY = 1,  /* one lane per ray, grid = ceil(n / block)     */
     CRT_KERNEL_PERSISTENT = 2,  /* device-filling waves claiming 64 rays at a time from a
                                    counter; leaf batching + node window (stackless)  */
-    CRT_KERNEL_WAVE = 3         /* one ray per lane, leaf batching + node window,
-                                   XCD-aware block order (stackless)                 */
+    CRT_KERNEL_WAVE = 3         /* one ray per lane, leaf batching, XCD-aware block
+                                   order (stackless: + node window; wide formats:
+                                   leaves of a node parked and tested by the wave) */
 };
 
 typedef struct crt_trace_config {
@@ -404,8 +406,10 @@ int crt_intersect_primary(crt_context* ctx, crt_ray* rays, crt_hit* hits, int wi
 /* Batched traversal: many scenes and ray batches in one persistent launch    */
 /* ------------------------------------------------------------------------- */
 
-/* One ray batch: a device scene (stackless format), a query, device rays and
- * hits.  The batch is traced exactly as crt_trace would trace it. */
+/* One ray batch: a device scene (stackless, WIDE4 or WIDE8 format; every
+ * batch of a plan uses the same one), a query, device rays and hits.  The
+ * batch is traced exactly as crt_trace's wave kernel would trace it.  Node
+ * width is tuned by planning the same batches over each layout. */
 typedef struct crt_batch {
     crt_scene_view scene;
     int32_t query;          /* CRT_QUERY_* */
@@ -418,7 +422,8 @@ typedef struct crt_batch {
 
 /* Tuning of the batched kernel (crt_autotune_batched searches these). */
 typedef struct crt_batch_config {
-    int32_t variant;        /* wave-walk preset 0..5 (inner steps, leaf batch, window) */
+    int32_t variant;        /* wave-walk preset 0..5 (stackless: inner steps, leaf batch,
+                               window; wide: LDS stack depth, inner steps, leaf batch) */
     int32_t tiles_per_claim;/* 64-ray tiles a wave claims per atomic: 1, 2 or 4 */
     int32_t waves_per_group;/* 1, 2 or 4 waves per workgroup */
     int32_t grid;           /* workgroups; 0 = enough to fill every CU */

@@ -349,15 +349,26 @@ static inline int test_records(const scene_t* s, v3 ro, v3 rd, int32_t start, in
     return 0;
 }
 
-/* Wide walk: per node, every non-empty child box gets the slab arithmetic of
- * the stack include's RayBounds against the current tmax, with the exit
- * distance and tmax grown by WIDE_GROW = 1 + 2^-19 (so that a leaf whose
- * triangle lies on the box face, a few ulps below the computed entry, is not
- * culled by a closer-by-ulps hit); hit leaves are tested at once, the nearest
- * hit inner child is visited next and the other hit inner children are pushed
- * with their entry distance t0; a popped entry with t0 > tmax * WIDE_GROW is
- * skipped (closest hit). */
+/* Wide walk (the host twin of libcandela_rt's wide_walk.h, same per-ray
+ * sequence):
+ *   pop      with no next node, pop (node, entry distance) entries until one
+ *            lies within tmax * WIDE_GROW (closest hit; any hit does not cull);
+ *            an empty stack ends the walk;
+ *   guards   iteration cap, then the node index range;
+ *   node     every non-empty child box gets the slab arithmetic of the stack
+ *            include's RayBounds with the exit distance and tmax grown by
+ *            WIDE_GROW = 1 + 2^-19 (a leaf whose triangle lies on the box
+ *            face, a few ulps below the computed entry, is not culled by a
+ *            closer-by-ulps hit); in slot order a hit leaf is appended to the
+ *            pending list, a hit inner child competes for "near" (strictly
+ *            smaller entry distance wins, the loser is pushed);
+ *   leaves   with pending leaves the near child is pushed too, the pending
+ *            triangles are tested in slot order and the next node is popped;
+ *            otherwise the walk goes to the near child, else pops;
+ *   overflow a push beyond stack_cap entries ends the walk before any leaf of
+ *            that node is tested. */
 #define WIDE_GROW (1.0f + 1.0f / 524288.0f)
+
 static void walk_wide(const scene_t* s, v3 ro, v3 rd, int32_t root, int32_t count, float tmax, int any,
                       walk_t* w) {
     const int G = s->format == CRT_FORMAT_WIDE8 ? 2 : 1;
@@ -366,37 +377,52 @@ static void walk_wide(const scene_t* s, v3 ro, v3 rd, int32_t root, int32_t coun
     int32_t st_node[256];
     float st_t[256];
     int cap = s->stack_cap < 256 ? s->stack_cap : 256;
-    int sp = 0, it = 0, stopped = 0;
+    int sp = 0, it = 0;
     int32_t cur = root;
     w->t = -1.0f;
     w->tri = -1;
     w->mesh = -1;
-    while (it < s->max_iter) {
+    for (;;) {
+        if (cur < 0) {
+            int found = 0;
+            while (sp > 0) {
+                --sp;
+                if (!any && st_t[sp] > tmax * WIDE_GROW) continue;
+                cur = st_node[sp];
+                found = 1;
+                break;
+            }
+            if (!found) break;
+        }
+        if (it >= s->max_iter) {
+            w->status |= CRT_STATUS_ITER_CAP;
+            break;
+        }
         if (cur < root || cur >= root + count || (int64_t)cur >= s->total_nodes) {
             w->status |= CRT_STATUS_BAD_INDEX;
-            stopped = 1;
             break;
         }
         it++;
         int32_t near_node = -1;
         float near_t = 0.0f;
         int overflow = 0;
+        int32_t pend_start[8], pend_len[8];
+        int npend = 0;
         for (int g = 0; g < G; ++g) {
             const crt_wide_group* gp = &groups[(int64_t)cur * G + g];
             for (int k = 0; k < 4; ++k) {
-                if (gp->count[k] < 0) continue;
                 float fx = (gp->hi_x[k] - ro.x) * inv.x, fy = (gp->hi_y[k] - ro.y) * inv.y,
                       fz = (gp->hi_z[k] - ro.z) * inv.z;
                 float nx = (gp->lo_x[k] - ro.x) * inv.x, ny = (gp->lo_y[k] - ro.y) * inv.y,
                       nz = (gp->lo_z[k] - ro.z) * inv.z;
                 float t1 = mn(mn(mx(fx, nx), mn(mx(fy, ny), mx(fz, nz))) * WIDE_GROW, tmax * WIDE_GROW);
                 float t0 = mx(mx(mn(fx, nx), mx(mn(fy, ny), mn(fz, nz))), 0.0f);
-                if (!(t1 >= t0) || !(t0 > 0.0f || t1 > 0.0f)) continue;
+                int hit = gp->count[k] >= 0 && t1 >= t0 && (t0 > 0.0f || t1 > 0.0f);
+                if (!hit) continue;
                 if (gp->count[k] > 0) {
-                    if (test_records(s, ro, rd, gp->child[k], gp->count[k], any, &tmax, w)) {
-                        w->iters += (uint32_t)it;
-                        return;
-                    }
+                    pend_start[npend] = gp->child[k];
+                    pend_len[npend] = gp->count[k];
+                    npend++;
                     continue;
                 }
                 int32_t push_node = gp->child[k];
@@ -423,27 +449,29 @@ static void walk_wide(const scene_t* s, v3 ro, v3 rd, int32_t root, int32_t coun
         }
         if (overflow) {
             w->status |= CRT_STATUS_STACK_OVERFLOW;
-            stopped = 1;
             break;
         }
-        if (near_node >= 0 && (any || !(near_t > tmax * WIDE_GROW))) {
+        if (npend) {
+            if (near_node >= 0) {
+                if (sp >= cap) {
+                    w->status |= CRT_STATUS_STACK_OVERFLOW;
+                    break;
+                }
+                st_node[sp] = near_node;
+                st_t[sp] = near_t;
+                ++sp;
+            }
+            for (int j = 0; j < npend; ++j) {
+                if (test_records(s, ro, rd, pend_start[j], pend_len[j], any, &tmax, w)) {
+                    w->iters += (uint32_t)it;
+                    return;
+                }
+            }
+            cur = -1;
+        } else {
             cur = near_node;
-            continue;
-        }
-        int found = 0;
-        while (sp > 0) {
-            --sp;
-            if (!any && st_t[sp] > tmax * WIDE_GROW) continue;
-            cur = st_node[sp];
-            found = 1;
-            break;
-        }
-        if (!found) {
-            stopped = 1;
-            break;
         }
     }
-    if (!stopped) w->status |= CRT_STATUS_ITER_CAP;
     w->iters += (uint32_t)it;
 }
 

@@ -116,3 +116,36 @@ def test_autotune_returns_a_working_config(work, orc):
     crt.BatchPlan(work.batches(crt), best).launch()
     crt.lib().crt_device_sync()
     check_all(work, orc, "autotuned config")
+
+
+def wide_oracle_hits(orc, s, layout, which):
+    w = s.wide[layout].host
+    rays, q = (s.closest, 0) if which == "closest" else (s.shadow, 1)
+    return orc.trace_wide(w.width, w.groups, w.records, w.entities, q, rays)
+
+
+@pytest.mark.parametrize("width", [4, 8])
+def test_batched_wide(work, orc, width):
+    """The tile queue over the BVH4 / BVH8 layouts: every preset, with and without graph replay."""
+    import candela_amd as crt
+    layout = (width, 2)
+    work.build_wide(crt, width, 2)
+    for variant, tpc, wpg, order in ((0, 1, 4, 1), (1, 2, 1, 0), (2, 4, 2, 1), (3, 1, 4, 0), (4, 2, 4, 1),
+                                     (5, 1, 1, 1)):
+        for s in work.scenes:
+            s.h_closest.fill_bytes(0xAB)
+            s.h_shadow.fill_bytes(0xAB)
+        plan = crt.BatchPlan(work.batches(crt, layout), crt.BatchConfig(variant, tpc, wpg, 0, order, 0))
+        for replay in range(2):
+            plan.launch(graph=replay == 1)
+            crt.lib().crt_device_sync()
+            for s in work.scenes:
+                for which, h in (("closest", s.h_closest), ("shadow", s.h_shadow)):
+                    got, ref = h.download(), wide_oracle_hits(orc, s, layout, which)
+                    assert got.tobytes() == ref.tobytes(), f"wide{width} v{variant} scene {s.index} {which}"
+        log(f"[parity] batched wide{width} variant={variant} tpc={tpc} wpg={wpg} order={order}: "
+            f"{work.rays} rays bit-identical to the host wide walk (eager and graph)")
+    # mixing node formats in one plan is rejected
+    mixed = work.batches(crt, layout)[:2] + work.batches(crt)[2:4]
+    with pytest.raises(crt.CrtError):
+        crt.BatchPlan(mixed)

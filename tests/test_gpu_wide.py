@@ -95,7 +95,27 @@ def test_wide_cloud_incoherent(gpu, orc, src, width):
         assert np.array_equal(dt.download(), ot)
 
 
-@pytest.mark.parametrize("block,lds", [(64, 8), (256, 16), (256, 32), (1024, 8)])
+@pytest.mark.parametrize("width", [4, 8])
+def test_wide_wave_presets(gpu, orc, width):
+    """The wave-scheduled wide walk (every preset) against the host twin and the one-ray-per-lane walk."""
+    from candela_amd import scenes
+    log(f"[case] wide wave presets width={width}")
+    meshes = scenes.icosphere_cloud(count=150, subdivisions=3, seed=23, box=35.0)
+    r = _wide_context(gpu, 0, meshes, width, 2)
+    hb, w = _host_wide(gpu, r, width, 2)
+    rays = np.concatenate([random_rays(50000, [-2, -2, -2], [37, 37, 37], seed=24),
+                           random_rays(20000, [-2, -2, -2], [37, 37, 37], seed=25, tmax=2.0)])
+    scene = r.wide_scene()
+    for q in (0, 1):
+        o = orc.trace_wide(width, w.groups, w.records, w.entities, q, rays, stack_cap=8 + 64)
+        o16 = orc.trace_wide(width, w.groups, w.records, w.entities, q, rays, stack_cap=16 + 64)
+        assert np.array_equal(o, o16)                   # no ray comes near the stack capacity
+        compare(gpu_trace(gpu, scene, q, rays, kernel=1), o, f"wide{width} thread q={q}")
+        for v in range(6):
+            compare(gpu_trace(gpu, scene, q, rays, kernel=3, variant=v), o, f"wide{width} wave v{v} q={q}")
+
+
+@pytest.mark.parametrize("block,lds", [(64, 8), (256, 16), (128, 32), (512, 8)])
 def test_wide_launch_shapes(gpu, orc, block, lds):
     from candela_amd import scenes
     log(f"[case] wide launch block={block} lds={lds}")
@@ -137,11 +157,13 @@ def test_wide_errors(gpu):
     r = _wide_context(gpu, 0, scenes.boxes_scene(), 4, 2)
     rays = random_rays(10, [0, 0, 0], [1, 1, 1], seed=0)
     with pytest.raises(gpu.CrtError):
-        gpu_trace(gpu, r.wide_scene(), 0, rays, kernel=3)          # the wave kernel walks stackless only
+        gpu_trace(gpu, r.wide_scene(), 0, rays, kernel=2)          # the persistent kernel walks stackless only
+    with pytest.raises(gpu.CrtError):
+        gpu_trace(gpu, r.wide_scene(), 0, rays, kernel=3, variant=6)
     with pytest.raises(gpu.CrtError):
         gpu_trace(gpu, r.wide_scene(), 0, rays, lds=64)            # wide: 8, 16 or 32
     with pytest.raises(gpu.CrtError):
-        gpu_trace(gpu, r.wide_scene(), 0, rays, block=1024, lds=16)  # 2 * 16 * 1024 * 4 > 64 KiB
+        gpu_trace(gpu, r.wide_scene(), 0, rays, block=1024, lds=16)  # (2 * 16 + 8) * 1024 * 4 > 64 KiB
     r.buffer_entities()                                            # makes the wide layout stale
     with pytest.raises(gpu.CrtError):
         r.wide_scene()
