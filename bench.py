@@ -83,7 +83,8 @@ def bytes_per_ray(fmt: int, visits: np.ndarray, tests: np.ndarray) -> np.ndarray
 def kernel_label(c: dict) -> str:
     if c["layout"] == "bvh2":
         return "plain (one batch per launch): %s, preset %d, BVH2 stackless" % (KERNEL_NAMES[c["kernel"]], c["variant"])
-    name = "wide_grid, preset %d" % c["variant"] if c["kernel"] == 3 else "trace_kernel"
+    name = {3: "wide_grid, preset %d" % c["variant"], 2: "wide_persistent (lane refill), preset %d" % c["variant"]}.get(
+        c["kernel"], "trace_kernel")
     return "plain (one batch per launch): %s over the BVH%d layout derived from the BVH2, %d-triangle leaves" % (
         name, c["width"], c["max_leaf"])
 
@@ -340,7 +341,8 @@ def main(argv=None) -> int:
         candidates += [bvh2(crt.KERNEL_WAVE, v) for v in range(6)] + [bvh2(crt.KERNEL_PERSISTENT, v) for v in range(6)]
         if not args.no_wide:
             candidates += [wide(w, 2, crt.KERNEL_THREAD_PER_RAY) for w in (4, 8)]
-            candidates += [wide(w, leaf, crt.KERNEL_WAVE, v) for w in (4, 8) for leaf in (2, 4) for v in range(6)]
+            candidates += [wide(w, leaf, k, v) for w in (4, 8) for leaf in (2, 4)
+                           for k in (crt.KERNEL_WAVE, crt.KERNEL_PERSISTENT) for v in range(6)]
     if args.only:
         parts = args.only.split(":")
         if parts[0].startswith("w"):

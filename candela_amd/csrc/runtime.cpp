@@ -105,8 +105,8 @@ int trace(const crt_scene_view* scene, int query, uint32_t flags, const crt_ray*
     const int kernel = cfg ? cfg->kernel : CRT_KERNEL_AUTO;
     if (kernel < CRT_KERNEL_AUTO || kernel > CRT_KERNEL_WAVE)
         return set_error(CRT_E_INVALID, "crt_trace: unknown kernel");
-    if (kernel == CRT_KERNEL_PERSISTENT && scene->format != CRT_FORMAT_STACKLESS)
-        return set_error(CRT_E_INVALID, "crt_trace: the persistent kernel walks the stackless format only");
+    if (kernel == CRT_KERNEL_PERSISTENT && scene->format == CRT_FORMAT_STACK)
+        return set_error(CRT_E_INVALID, "crt_trace: the persistent kernels walk the stackless and wide formats");
     if (kernel == CRT_KERNEL_WAVE && scene->format == CRT_FORMAT_STACK)
         return set_error(CRT_E_INVALID, "crt_trace: the wave kernels walk the stackless and wide formats");
     if (is_wide(scene->format) && (depth != 8 && depth != 16 && depth != 32))
@@ -136,6 +136,17 @@ int trace(const crt_scene_view* scene, int query, uint32_t flags, const crt_ray*
         CRT_HIP(launch_grid(a, query, rays, hits, n, flags, kMaxIterations, tri_tests, cfg ? cfg->variant : 0,
                             static_cast<hipStream_t>(stream)),
                 "crt_trace: wave kernel launch");
+        return CRT_OK;
+    }
+    if (persistent && is_wide(scene->format)) {
+        if (n >= (int64_t(1) << 31))
+            return set_error(CRT_E_INVALID, "crt_trace: the persistent kernel takes < 2^31 rays per launch");
+        unsigned long long* counter = nullptr;
+        rc = counter_slot(&counter);
+        if (rc) return rc;
+        CRT_HIP(launch_wide_persistent(a, query, rays, hits, n, flags, kMaxIterations, counter, tri_tests,
+                                       cfg ? cfg->variant : 0, cfg ? cfg->grid : 0, static_cast<hipStream_t>(stream)),
+                "crt_trace: wide persistent kernel launch");
         return CRT_OK;
     }
     if (persistent) {

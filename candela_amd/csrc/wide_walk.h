@@ -13,15 +13,16 @@
 //             entity's [root, root + count) (CRT_STATUS_BAD_INDEX)
 //   node      every non-empty child box gets the slab arithmetic of the stack
 //             include's RayBounds (Include/TraverseBVHStack.glsl:107-116), with
-//             the exit distance and tmax grown by kWideGrow; in slot order a
-//             hit leaf is appended to the pending list and a hit inner child
-//             competes for "near" (strictly smaller entry distance wins; the
-//             loser is pushed)
-//   leaves    with pending leaves the near child is pushed too, the pending
-//             triangles are tested in slot order, and the next node is popped
-//             (with the tmax those tests produced); without, the walk goes to
-//             the near child, else pops
-//   overflow  a push beyond LDS_DEPTH + 64 entries ends the walk with
+//             the exit distance and tmax grown by kWideGrow; hit leaves are
+//             appended to the pending list in slot order; hit inner children
+//             are sorted by entry distance with a fixed compare-exchange
+//             network (sort_children) and pushed farthest first, all but the
+//             nearest
+//   leaves    with pending leaves the nearest child is pushed too (on top),
+//             the pending triangles are tested in slot order, and the next
+//             node is popped (with the tmax those tests produced); without,
+//             the walk goes to the nearest child, else pops
+//   overflow  pushes that would pass LDS_DEPTH + 64 entries end the walk with
 //             CRT_STATUS_STACK_OVERFLOW before any leaf of that node is tested
 //
 // Triangle tests and acceptance (0 < t < tmax, first found wins a tie) are
@@ -153,75 +154,94 @@ struct Core {
         np = 0;
         pi = 0;
         const float4* __restrict__ groups = reinterpret_cast<const float4*>(s.nodes);
-        int32_t near_node = -1;
-        float near_t = 0.0f;
-        bool overflow = false;
-#pragma unroll
-        for (int g = 0; g < G; ++g) {
-            const float4* p = groups + (int64_t(cur) * G + g) * 8;
-            const float4 lox = p[0], hix = p[1], loy = p[2], hiy = p[3], loz = p[4], hiz = p[5];
-            const float4 chf = p[6], cnf = p[7];
-            const float lx[4] = {lox.x, lox.y, lox.z, lox.w}, hx[4] = {hix.x, hix.y, hix.z, hix.w};
-            const float ly[4] = {loy.x, loy.y, loy.z, loy.w}, hy[4] = {hiy.x, hiy.y, hiy.z, hiy.w};
-            const float lz[4] = {loz.x, loz.y, loz.z, loz.w}, hz[4] = {hiz.x, hiz.y, hiz.z, hiz.w};
-            const int32_t ch[4] = {__float_as_int(chf.x), __float_as_int(chf.y), __float_as_int(chf.z),
-                                   __float_as_int(chf.w)};
-            const int32_t cn[4] = {__float_as_int(cnf.x), __float_as_int(cnf.y), __float_as_int(cnf.z),
-                                   __float_as_int(cnf.w)};
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const float fx = (hx[k] - ro.x) * inv.x, fy = (hy[k] - ro.y) * inv.y, fz = (hz[k] - ro.z) * inv.z;
-                const float nx = (lx[k] - ro.x) * inv.x, ny = (ly[k] - ro.y) * inv.y, nz = (lz[k] - ro.z) * inv.z;
-                const float t1 = mn(mn(mx(fx, nx), mn(mx(fy, ny), mx(fz, nz))) * kWideGrow, tmax * kWideGrow);
-                const float t0 = mx(mx(mn(fx, nx), mx(mn(fy, ny), mn(fz, nz))), 0.0f);
-                const bool hit = cn[k] >= 0 && t1 >= t0 && (t0 > 0.0f || t1 > 0.0f);
-                if (hit && cn[k] > 0) {
-                    st.put_leaf(np, (ch[k] << 4) | cn[k]);
-                    ++np;
-                } else if (hit) {
-                    int32_t push_node = ch[k];
-                    float push_t = t0;
-                    bool push = true;
-                    if (near_node < 0) {
-                        near_node = ch[k];
-                        near_t = t0;
-                        push = false;
-                    } else if (t0 < near_t) {
-                        push_node = near_node;
-                        push_t = near_t;
-                        near_node = ch[k];
-                        near_t = t0;
-                    }
-                    if (push) {
-                        if (sp >= kCap) {
-                            overflow = true;
-                        } else {
-                            st.put(sp, push_node, push_t);
-                            ++sp;
-                        }
-                    }
-                }
-            }
+        // hit inner children as (entry distance, node) in named registers t0..t7 /
+        // n0..n7 (no arrays: every index is a compile-time name); misses and
+        // leaves are (+inf, -1)
+        const float inf = __builtin_inff();
+        const float tgrow = tmax * kWideGrow;
+        float t0 = inf, t1 = inf, t2 = inf, t3 = inf, t4 = inf, t5 = inf, t6 = inf, t7 = inf;
+        int32_t n0 = -1, n1 = -1, n2 = -1, n3 = -1, n4 = -1, n5 = -1, n6 = -1, n7 = -1;
+#define CRT_WIDE_CHILD(LO_X, HI_X, LO_Y, HI_Y, LO_Z, HI_Z, CH, CN, T, N)                                       \
+    {                                                                                                       \
+        const float fx = (HI_X - ro.x) * inv.x, fy = (HI_Y - ro.y) * inv.y, fz = (HI_Z - ro.z) * inv.z;     \
+        const float nx = (LO_X - ro.x) * inv.x, ny = (LO_Y - ro.y) * inv.y, nz = (LO_Z - ro.z) * inv.z;     \
+        const float e1 = mn(mn(mx(fx, nx), mn(mx(fy, ny), mx(fz, nz))) * kWideGrow, tgrow);                \
+        const float e0 = mx(mx(mn(fx, nx), mx(mn(fy, ny), mn(fz, nz))), 0.0f);                              \
+        const int32_t cnt = __float_as_int(CN), chi = __float_as_int(CH);                                   \
+        const bool hit = cnt >= 0 && e1 >= e0 && (e0 > 0.0f || e1 > 0.0f);                                  \
+        if (hit && cnt > 0) {                                                                               \
+            st.put_leaf(np, (chi << 4) | cnt);                                                              \
+            ++np;                                                                                           \
+        }                                                                                                   \
+        const bool inner = hit && cnt == 0;                                                                 \
+        T = inner ? e0 : inf;                                                                               \
+        N = inner ? chi : -1;                                                                               \
+    }
+#define CRT_WIDE_GROUP(G_, TA, NA, TB, NB, TC, NC, TD, ND)                                                  \
+    {                                                                                                       \
+        const float4* p = groups + (int64_t(cur) * G + G_) * 8;                                             \
+        const float4 lox = p[0], hix = p[1], loy = p[2], hiy = p[3], loz = p[4], hiz = p[5];                \
+        const float4 chf = p[6], cnf = p[7];                                                                \
+        CRT_WIDE_CHILD(lox.x, hix.x, loy.x, hiy.x, loz.x, hiz.x, chf.x, cnf.x, TA, NA)                      \
+        CRT_WIDE_CHILD(lox.y, hix.y, loy.y, hiy.y, loz.y, hiz.y, chf.y, cnf.y, TB, NB)                      \
+        CRT_WIDE_CHILD(lox.z, hix.z, loy.z, hiy.z, loz.z, hiz.z, chf.z, cnf.z, TC, NC)                      \
+        CRT_WIDE_CHILD(lox.w, hix.w, loy.w, hiy.w, loz.w, hiz.w, chf.w, cnf.w, TD, ND)                      \
+    }
+// compare-exchange: swap when the second entry is strictly nearer
+#define CRT_CX(TI, NI, TJ, NJ)              \
+    {                                       \
+        const bool sw = TJ < TI;            \
+        const float ta = TI, tb = TJ;       \
+        const int32_t na = NI, nb = NJ;     \
+        TI = sw ? tb : ta;                  \
+        TJ = sw ? ta : tb;                  \
+        NI = sw ? nb : na;                  \
+        NJ = sw ? na : nb;                  \
+    }
+        CRT_WIDE_GROUP(0, t0, n0, t1, n1, t2, n2, t3, n3)
+        if constexpr (W == 8) CRT_WIDE_GROUP(1, t4, n4, t5, n5, t6, n6, t7, n7)
+        // ascending entry distance: W = 4 a 5-exchange network, W = 8 Batcher's
+        // odd-even merge sort (19 exchanges)
+        if constexpr (W == 4) {
+            CRT_CX(t0, n0, t1, n1) CRT_CX(t2, n2, t3, n3) CRT_CX(t0, n0, t2, n2) CRT_CX(t1, n1, t3, n3)
+            CRT_CX(t1, n1, t2, n2)
+        } else {
+            CRT_CX(t0, n0, t1, n1) CRT_CX(t2, n2, t3, n3) CRT_CX(t4, n4, t5, n5) CRT_CX(t6, n6, t7, n7)
+            CRT_CX(t0, n0, t2, n2) CRT_CX(t1, n1, t3, n3) CRT_CX(t4, n4, t6, n6) CRT_CX(t5, n5, t7, n7)
+            CRT_CX(t1, n1, t2, n2) CRT_CX(t5, n5, t6, n6)
+            CRT_CX(t0, n0, t4, n4) CRT_CX(t1, n1, t5, n5) CRT_CX(t2, n2, t6, n6) CRT_CX(t3, n3, t7, n7)
+            CRT_CX(t2, n2, t4, n4) CRT_CX(t3, n3, t5, n5)
+            CRT_CX(t1, n1, t2, n2) CRT_CX(t3, n3, t4, n4) CRT_CX(t5, n5, t6, n6)
         }
-        if (overflow) {
+#undef CRT_CX
+#undef CRT_WIDE_GROUP
+#undef CRT_WIDE_CHILD
+        const int32_t m = (n0 >= 0) + (n1 >= 0) + (n2 >= 0) + (n3 >= 0) + (n4 >= 0) + (n5 >= 0) + (n6 >= 0) +
+                          (n7 >= 0);
+        // with leaves pending the nearest child is pushed too (on top)
+        const int32_t pushes = np ? m : (m > 0 ? m - 1 : 0);
+        if (sp + pushes > kCap) {
             status |= CRT_STATUS_STACK_OVERFLOW;
             np = 0;
             return kEnd;
         }
+        const int32_t lowest = np ? 0 : 1;   // sorted index pushed last
+        // farthest first, so that pops come nearest first
+        if (W == 8) {
+            if (7 >= lowest && 7 < m) st.put(sp++, n7, t7);
+            if (6 >= lowest && 6 < m) st.put(sp++, n6, t6);
+            if (5 >= lowest && 5 < m) st.put(sp++, n5, t5);
+            if (4 >= lowest && 4 < m) st.put(sp++, n4, t4);
+        }
+        if (3 >= lowest && 3 < m) st.put(sp++, n3, t3);
+        if (2 >= lowest && 2 < m) st.put(sp++, n2, t2);
+        if (1 < m) st.put(sp++, n1, t1);
+        if (lowest == 0 && 0 < m) st.put(sp++, n0, t0);
         if (np) {
-            if (near_node >= 0) {
-                if (sp >= kCap) {
-                    status |= CRT_STATUS_STACK_OVERFLOW;
-                    np = 0;
-                    return kEnd;
-                }
-                st.put(sp, near_node, near_t);
-                ++sp;
-            }
             cur = -1;
             return kLeaves;
         }
-        cur = near_node;   // -1 (none hit): pop first
+        cur = n0;   // -1 (no inner child hit): pop first
         return kNext;
     }
 
@@ -356,88 +376,141 @@ struct WideWave {
         if (tri_tests) tri_tests[slot] = c.tests;
     }
 
+    // Lane takes ray i.
+    __device__ __forceinline__ void start(const SceneArgs& s, const crt_ray* __restrict__ rays,
+                                          crt_hit* __restrict__ hits, int64_t i, uint32_t flags,
+                                          uint32_t* __restrict__ tri_tests) {
+        active = true;
+        leafing = false;
+        walk_done = false;
+        slot = int32_t(i);
+        ray_tmax = reinterpret_cast<const float4*>(rays + i)[0].w;
+        ent = -1;
+        hit_ent = -1;
+        visits = 0;
+        c.tmax = ray_tmax;
+        c.tri = c.mesh = -1;
+        c.tests = c.status = 0;
+        if (!begin_walk(s, rays, flags)) {
+            finish(s, rays, hits, tri_tests, flags);
+            active = false;
+        }
+    }
+
+    // One round of the wave: node phase, leaf phase, walk ends.
+    __device__ __forceinline__ void round(const SceneArgs& s, const crt_ray* __restrict__ rays,
+                                          crt_hit* __restrict__ hits, uint32_t flags, int32_t max_iter,
+                                          uint32_t* __restrict__ tri_tests, Stack<LDS_DEPTH>& st) {
+        const float4* __restrict__ rec = reinterpret_cast<const float4*>(s.tri_records);
+        // --- node phase ----------------------------------------------------------
+        for (int k = 0; k < INNER; ++k) {
+            if (active && !walk_done && !leafing) {
+                const int32_t r = c.step(s, max_iter, st);
+                if (r == kEnd) {
+                    walk_done = true;
+                } else if (r == kLeaves) {
+                    leafing = true;
+                    leaf = st.leaf(0);
+                    c.pi = 1;
+                    off = 0;
+                }
+            }
+            const unsigned long long stepping = __ballot(active && !walk_done && !leafing);
+            if (stepping == 0 || __popcll(__ballot(leafing)) >= LEAF_BATCH) break;
+        }
+        // --- leaf phase: pending triangles, two records per fetch ----------------
+        while (__ballot(leafing)) {
+            if (leafing) {
+                const int32_t first = (leaf >> 4) + off, left = (leaf & 0xF) - off;
+                const float4* p = rec + 3 * int64_t(first);
+                const float4 a0 = p[0], b0 = p[1], c0 = p[2];
+                float4 a1 = a0, b1 = b0, c1 = c0;
+                if (left > 1) {
+                    a1 = p[3];
+                    b1 = p[4];
+                    c1 = p[5];
+                }
+                const float before = c.tmax;
+                bool done = c.test(first, a0, b0, c0);
+                if (!done && left > 1) done = c.test(first + 1, a1, b1, c1);
+                if (c.tmax < before) hit_ent = ent;
+                off += left > 1 ? 2 : 1;
+                if (done) {
+                    leafing = false;
+                    walk_done = true;
+                } else if (off >= (leaf & 0xF)) {
+                    if (c.pi < c.np) {
+                        leaf = st.leaf(c.pi);
+                        ++c.pi;
+                        off = 0;
+                    } else {
+                        leafing = false;
+                    }
+                }
+            }
+        }
+        // --- walk ends: next entity or the ray's result --------------------------
+        if (__ballot(walk_done)) {
+            if (walk_done) {
+                walk_done = false;
+                visits += uint32_t(c.it);
+                bool fin = ANY && c.tri >= 0;
+                if (!fin) fin = !begin_walk(s, rays, flags);
+                if (fin) {
+                    finish(s, rays, hits, tri_tests, flags);
+                    active = false;
+                }
+            }
+        }
+    }
+
+    // Rays i of 64 lanes (i < 0: lane idle) until all are done.
     __device__ __forceinline__ void run(const SceneArgs& s, const crt_ray* __restrict__ rays,
                                         crt_hit* __restrict__ hits, int64_t i, uint32_t flags, int32_t max_iter,
                                         uint32_t* __restrict__ tri_tests, Stack<LDS_DEPTH> st) {
-        const float4* __restrict__ rec = reinterpret_cast<const float4*>(s.tri_records);
-        active = i >= 0;
+        active = false;
         leafing = false;
         walk_done = false;
-        if (active) {
-            slot = int32_t(i);
-            ray_tmax = reinterpret_cast<const float4*>(rays + i)[0].w;
-            ent = -1;
-            hit_ent = -1;
-            visits = 0;
-            c.tmax = ray_tmax;
-            c.tri = c.mesh = -1;
-            c.tests = c.status = 0;
-            if (!begin_walk(s, rays, flags)) {
-                finish(s, rays, hits, tri_tests, flags);
-                active = false;
-            }
-        }
-        while (__ballot(active)) {
-            // --- node phase ------------------------------------------------------
-            for (int k = 0; k < INNER; ++k) {
-                if (active && !walk_done && !leafing) {
-                    const int32_t r = c.step(s, max_iter, st);
-                    if (r == kEnd) {
-                        walk_done = true;
-                    } else if (r == kLeaves) {
-                        leafing = true;
-                        leaf = st.leaf(0);
-                        c.pi = 1;
-                        off = 0;
-                    }
-                }
-                const unsigned long long stepping = __ballot(active && !walk_done && !leafing);
-                if (stepping == 0 || __popcll(__ballot(leafing)) >= LEAF_BATCH) break;
-            }
-            // --- leaf phase: pending triangles, two records per fetch ------------
-            while (__ballot(leafing)) {
-                if (leafing) {
-                    const int32_t first = (leaf >> 4) + off, left = (leaf & 0xF) - off;
-                    const float4* p = rec + 3 * int64_t(first);
-                    const float4 a0 = p[0], b0 = p[1], c0 = p[2];
-                    float4 a1 = a0, b1 = b0, c1 = c0;
-                    if (left > 1) {
-                        a1 = p[3];
-                        b1 = p[4];
-                        c1 = p[5];
-                    }
-                    const float before = c.tmax;
-                    bool done = c.test(first, a0, b0, c0);
-                    if (!done && left > 1) done = c.test(first + 1, a1, b1, c1);
-                    if (c.tmax < before) hit_ent = ent;
-                    off += left > 1 ? 2 : 1;
-                    if (done) {
-                        leafing = false;
-                        walk_done = true;
-                    } else if (off >= (leaf & 0xF)) {
-                        if (c.pi < c.np) {
-                            leaf = st.leaf(c.pi);
-                            ++c.pi;
-                            off = 0;
-                        } else {
-                            leafing = false;
-                        }
+        if (i >= 0) start(s, rays, hits, i, flags, tri_tests);
+        while (__ballot(active)) round(s, rays, hits, flags, max_iter, tri_tests, st);
+    }
+
+    // Rays [0, n) claimed from `counter` by the whole wave: between rounds,
+    // once at least `refill` lanes are idle, one atomicAdd claims a ray for
+    // each idle lane (lane ranks by mbcnt), so a wave keeps its lanes busy
+    // instead of waiting for its longest ray.  Ends when the counter has
+    // passed n and every lane is done.
+    __device__ __forceinline__ void run_refill(const SceneArgs& s, const crt_ray* __restrict__ rays,
+                                               crt_hit* __restrict__ hits, int64_t n, uint32_t flags,
+                                               int32_t max_iter, uint32_t* __restrict__ tri_tests,
+                                               Stack<LDS_DEPTH> st, unsigned long long* __restrict__ counter,
+                                               int32_t refill) {
+        active = false;
+        leafing = false;
+        walk_done = false;
+        bool more = true;   // wave-uniform: the counter may still hold rays
+        while (true) {
+            if (more) {
+                const unsigned long long idle = __ballot(!active);
+                const int32_t cnt = __popcll(idle);
+                if (cnt >= refill) {
+                    unsigned long long base = 0;
+                    if (__lane_id() == 0) base = atomicAdd(counter, (unsigned long long)cnt);
+                    base = (unsigned long long)__shfl((long long)base, 0);
+                    if (base + (unsigned long long)cnt >= (unsigned long long)n) more = false;
+                    if (!active) {
+                        const int32_t rank = __builtin_amdgcn_mbcnt_hi(uint32_t(idle >> 32),
+                                                                       __builtin_amdgcn_mbcnt_lo(uint32_t(idle), 0));
+                        const unsigned long long idx = base + (unsigned long long)rank;
+                        if (idx < (unsigned long long)n) start(s, rays, hits, int64_t(idx), flags, tri_tests);
                     }
                 }
             }
-            // --- walk ends: next entity or the ray's result ----------------------
-            if (__ballot(walk_done)) {
-                if (walk_done) {
-                    walk_done = false;
-                    visits += uint32_t(c.it);
-                    bool fin = ANY && c.tri >= 0;
-                    if (!fin) fin = !begin_walk(s, rays, flags);
-                    if (fin) {
-                        finish(s, rays, hits, tri_tests, flags);
-                        active = false;
-                    }
-                }
+            if (!__ballot(active)) {
+                if (!more) break;
+                continue;
             }
+            round(s, rays, hits, flags, max_iter, tri_tests, st);
         }
     }
 };

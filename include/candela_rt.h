@@ -255,8 +255,10 @@ enum {
     CRT_KERNEL_AUTO = 0,        /* the wave kernel for stackless and wide formats,
                                    thread-per-ray for the stack format              */
     CRT_KERNEL_THREAD_PER_RAY = 1,  /* one lane per ray, grid = ceil(n / block)     */
-    CRT_KERNEL_PERSISTENT = 2,  /* device-filling waves claiming 64 rays at a time from a
-                                   counter; leaf batching + node window (stackless)  */
+    CRT_KERNEL_PERSISTENT = 2,  /* device-filling waves claiming rays from a counter:
+                                   stackless, 64 at a time (leaf batching + node
+                                   window); wide formats, one per idle lane whenever
+                                   enough lanes are idle                            */
     CRT_KERNEL_WAVE = 3         /* one ray per lane, leaf batching, XCD-aware block
                                    order (stackless: + node window; wide formats:
                                    leaves of a node parked and tested by the wave) */

@@ -359,15 +359,28 @@ static inline int test_records(const scene_t* s, v3 ro, v3 rd, int32_t start, in
  *            include's RayBounds with the exit distance and tmax grown by
  *            WIDE_GROW = 1 + 2^-19 (a leaf whose triangle lies on the box
  *            face, a few ulps below the computed entry, is not culled by a
- *            closer-by-ulps hit); in slot order a hit leaf is appended to the
- *            pending list, a hit inner child competes for "near" (strictly
- *            smaller entry distance wins, the loser is pushed);
- *   leaves   with pending leaves the near child is pushed too, the pending
- *            triangles are tested in slot order and the next node is popped;
- *            otherwise the walk goes to the near child, else pops;
- *   overflow a push beyond stack_cap entries ends the walk before any leaf of
- *            that node is tested. */
+ *            closer-by-ulps hit); hit leaves are appended to the pending list
+ *            in slot order; hit inner children are sorted by entry distance
+ *            with the kernel's compare-exchange network and pushed farthest
+ *            first, all but the nearest;
+ *   leaves   with pending leaves the nearest child is pushed too (on top), the
+ *            pending triangles are tested in slot order and the next node is
+ *            popped; otherwise the walk goes to the nearest child, else pops;
+ *   overflow pushes that would pass stack_cap entries end the walk before any
+ *            leaf of that node is tested. */
 #define WIDE_GROW (1.0f + 1.0f / 524288.0f)
+
+/* compare-exchange: swap when entry j is strictly nearer than entry i */
+static inline void cx(float* t, int32_t* n, int i, int j) {
+    if (t[j] < t[i]) {
+        float tt = t[i];
+        int32_t nn = n[i];
+        t[i] = t[j];
+        n[i] = n[j];
+        t[j] = tt;
+        n[j] = nn;
+    }
+}
 
 static void walk_wide(const scene_t* s, v3 ro, v3 rd, int32_t root, int32_t count, float tmax, int any,
                       walk_t* w) {
@@ -403,11 +416,11 @@ static void walk_wide(const scene_t* s, v3 ro, v3 rd, int32_t root, int32_t coun
             break;
         }
         it++;
-        int32_t near_node = -1;
-        float near_t = 0.0f;
-        int overflow = 0;
+        float ct[8];
+        int32_t cn[8];
         int32_t pend_start[8], pend_len[8];
         int npend = 0;
+        const float tgrow = tmax * WIDE_GROW;
         for (int g = 0; g < G; ++g) {
             const crt_wide_group* gp = &groups[(int64_t)cur * G + g];
             for (int k = 0; k < 4; ++k) {
@@ -415,52 +428,45 @@ static void walk_wide(const scene_t* s, v3 ro, v3 rd, int32_t root, int32_t coun
                       fz = (gp->hi_z[k] - ro.z) * inv.z;
                 float nx = (gp->lo_x[k] - ro.x) * inv.x, ny = (gp->lo_y[k] - ro.y) * inv.y,
                       nz = (gp->lo_z[k] - ro.z) * inv.z;
-                float t1 = mn(mn(mx(fx, nx), mn(mx(fy, ny), mx(fz, nz))) * WIDE_GROW, tmax * WIDE_GROW);
+                float t1 = mn(mn(mx(fx, nx), mn(mx(fy, ny), mx(fz, nz))) * WIDE_GROW, tgrow);
                 float t0 = mx(mx(mn(fx, nx), mx(mn(fy, ny), mn(fz, nz))), 0.0f);
                 int hit = gp->count[k] >= 0 && t1 >= t0 && (t0 > 0.0f || t1 > 0.0f);
-                if (!hit) continue;
-                if (gp->count[k] > 0) {
+                if (hit && gp->count[k] > 0) {
                     pend_start[npend] = gp->child[k];
                     pend_len[npend] = gp->count[k];
                     npend++;
-                    continue;
                 }
-                int32_t push_node = gp->child[k];
-                float push_t = t0;
-                if (near_node < 0) {
-                    near_node = gp->child[k];
-                    near_t = t0;
-                    continue;
-                }
-                if (t0 < near_t) {
-                    push_node = near_node;
-                    push_t = near_t;
-                    near_node = gp->child[k];
-                    near_t = t0;
-                }
-                if (sp >= cap) {
-                    overflow = 1;
-                    continue;
-                }
-                st_node[sp] = push_node;
-                st_t[sp] = push_t;
-                ++sp;
+                int inner = hit && gp->count[k] == 0;
+                ct[4 * g + k] = inner ? t0 : INFINITY;
+                cn[4 * g + k] = inner ? gp->child[k] : -1;
             }
         }
-        if (overflow) {
+        /* ascending entry distance by the kernel's compare-exchange networks */
+        if (G == 1) {
+            static const int net4[5][2] = {{0, 1}, {2, 3}, {0, 2}, {1, 3}, {1, 2}};
+            for (int e = 0; e < 5; ++e) cx(ct, cn, net4[e][0], net4[e][1]);
+        } else {
+            static const int net8[19][2] = {{0, 1}, {2, 3}, {4, 5}, {6, 7}, {0, 2}, {1, 3}, {4, 6},
+                                            {5, 7}, {1, 2}, {5, 6}, {0, 4}, {1, 5}, {2, 6}, {3, 7},
+                                            {2, 4}, {3, 5}, {1, 2}, {3, 4}, {5, 6}};
+            for (int e = 0; e < 19; ++e) cx(ct, cn, net8[e][0], net8[e][1]);
+        }
+        int m = 0;
+        for (int j = 0; j < 4 * G; ++j) m += cn[j] >= 0;
+        int pushes = npend ? m : (m > 0 ? m - 1 : 0);
+        if (sp + pushes > cap) {
             w->status |= CRT_STATUS_STACK_OVERFLOW;
             break;
         }
-        if (npend) {
-            if (near_node >= 0) {
-                if (sp >= cap) {
-                    w->status |= CRT_STATUS_STACK_OVERFLOW;
-                    break;
-                }
-                st_node[sp] = near_node;
-                st_t[sp] = near_t;
+        int lowest = npend ? 0 : 1;
+        for (int j = 4 * G - 1; j >= lowest; --j) {
+            if (j < m) {
+                st_node[sp] = cn[j];
+                st_t[sp] = ct[j];
                 ++sp;
             }
+        }
+        if (npend) {
             for (int j = 0; j < npend; ++j) {
                 if (test_records(s, ro, rd, pend_start[j], pend_len[j], any, &tmax, w)) {
                     w->iters += (uint32_t)it;
@@ -469,7 +475,7 @@ static void walk_wide(const scene_t* s, v3 ro, v3 rd, int32_t root, int32_t coun
             }
             cur = -1;
         } else {
-            cur = near_node;
+            cur = cn[0];
         }
     }
     w->iters += (uint32_t)it;
