@@ -14,6 +14,7 @@ import numpy as np
 
 from ._lib import (KERNEL_AUTO, KERNEL_PERSISTENT, KERNEL_THREAD_PER_RAY, KERNEL_WAVE, BOUNDS_DTYPE, ENTITY_DTYPE, FLAG_IGNORE_TRANSPARENT, FLAG_TRI0_IS_MISS, FORMAT_STACK,
                    FORMAT_STACKLESS, FORMAT_WIDE4, FORMAT_WIDE8, WIDE_GROUP_DTYPE, TRI_RECORD_DTYPE, WideStats,
+                   TEXTURE_REF_DTYPE, TEXTURE_DTYPE, HIT_ATTR_DTYPE,
                    HIT_DTYPE, QUERY_ANY, QUERY_CLOSEST, RAY_DTYPE, STACK_NODE_DTYPE,
                    STACKLESS_NODE_DTYPE, STATUS_BAD_INDEX, STATUS_ITER_CAP, STATUS_STACK_OVERFLOW,
                    STATUS_VISITS_MASK, SWAP_NONE, SWAP_SEEDED, TRIANGLE_DTYPE, VERTEX_DTYPE, BuildOptions,
@@ -29,6 +30,7 @@ __all__ = [
     "RayIntersector", "trace", "device_count", "node_dtype", "lib", "Batch", "BatchConfig", "BatchPlan",
     "autotune_batched", "ShadeParams", "SHADE_RECORD_DTYPE", "ShadingModel", "FORMAT_WIDE4", "FORMAT_WIDE8",
     "WIDE_GROUP_DTYPE", "TRI_RECORD_DTYPE", "WideBVH", "wide_build", "DeviceWide", "upload_wide",
+    "TEXTURE_REF_DTYPE", "TEXTURE_DTYPE", "HIT_ATTR_DTYPE", "MaterialTable", "hit_attributes",
 ]
 
 
@@ -194,6 +196,32 @@ def wide_build(src_format: int, nodes: np.ndarray, triangles: np.ndarray, vertic
     finally:
         lib().crt_wide_free(h)
     return WideBVH(int(width), groups, records, ents, stats)
+
+
+class MaterialTable:
+    """Per-mesh material records (crt_texture_ref) and RGBA8 sRGB textures, on the device."""
+
+    def __init__(self, refs: np.ndarray, textures=()):
+        """refs: TEXTURE_REF_DTYPE per mesh id; textures: (texels uint32 [h, w], ...) in table order."""
+        self.refs = DeviceArray.from_host(np.ascontiguousarray(refs, TEXTURE_REF_DTYPE))
+        self.texels = [DeviceArray.from_host(np.ascontiguousarray(t, np.uint32).reshape(-1)) for t in textures]
+        desc = np.zeros(max(1, len(self.texels)), TEXTURE_DTYPE)
+        for k, (t, d) in enumerate(zip(textures, self.texels)):
+            h, w = np.asarray(t).shape
+            desc[k] = (d.ptr, w, h)
+        self.desc = DeviceArray.from_host(desc)
+        self.nrefs = int(np.asarray(refs).size)
+        self.ntextures = len(self.texels)
+
+
+def hit_attributes(scene: SceneView, materials: MaterialTable, hits: DeviceArray, out: DeviceArray,
+                   n: int | None = None, stream=None) -> None:
+    """crt_hit_attributes: GetData for n closest-hit records (asynchronous)."""
+    n = hits.count if n is None else int(n)
+    if n > hits.count or n > out.count:
+        raise ValueError("hit / attribute buffers smaller than n")
+    check(lib().crt_hit_attributes(C.byref(scene), materials.refs.ptr, materials.nrefs, materials.desc.ptr,
+                                   materials.ntextures, hits.ptr, n, out.ptr, _stream_handle(stream)))
 
 
 class DeviceWide:

@@ -50,6 +50,10 @@ WIDE_GROUP_DTYPE = np.dtype([("lo_x", "<f4", 4), ("hi_x", "<f4", 4), ("lo_y", "<
 TRI_RECORD_DTYPE = np.dtype([("v0", "<f4", 3), ("tri", "<i4"), ("v1", "<f4", 3), ("mesh", "<i4"),
                              ("v2", "<f4", 3), ("pad", "<i4")])
 assert WIDE_GROUP_DTYPE.itemsize == 128 and TRI_RECORD_DTYPE.itemsize == 48
+TEXTURE_REF_DTYPE = np.dtype([("model_color", "<f4", 4), ("albedo", "<i4"), ("normal", "<i4"), ("pad", "<i4", 2)])
+TEXTURE_DTYPE = np.dtype([("texels", "<u8"), ("width", "<i4"), ("height", "<i4")])
+HIT_ATTR_DTYPE = np.dtype([("normal", "<f4", 3), ("emissivity", "<f4"), ("albedo", "<f4", 3), ("alpha", "<f4")])
+assert TEXTURE_REF_DTYPE.itemsize == 32 and TEXTURE_DTYPE.itemsize == 16 and HIT_ATTR_DTYPE.itemsize == 32
 assert VERTEX_DTYPE.itemsize == 32 and TRIANGLE_DTYPE.itemsize == 16
 assert STACKLESS_NODE_DTYPE.itemsize == 32 and STACK_NODE_DTYPE.itemsize == 64
 assert ENTITY_DTYPE.itemsize == 192 and RAY_DTYPE.itemsize == 32 and HIT_DTYPE.itemsize == 32
@@ -181,6 +185,8 @@ SIGNATURES = {
     "crt_wide_records": (C.c_void_p, [C.c_void_p]),
     "crt_wide_entities": (C.c_void_p, [C.c_void_p]),
     "crt_context_build_wide": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
+    "crt_hit_attributes": (C.c_int, [C.POINTER(SceneView), C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, C.c_void_p,
+                                     C.c_int64, C.c_void_p, C.c_void_p]),
     "crt_context_wide_scene": (C.c_int, [C.c_void_p, C.POINTER(SceneView)]),
 }
 

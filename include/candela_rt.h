@@ -405,6 +405,48 @@ int crt_intersect_primary(crt_context* ctx, crt_ray* rays, crt_hit* hits, int wi
                           void* stream);
 
 /* ------------------------------------------------------------------------- */
+/* Hit attributes -- GetData (TraverseBVHStackless.glsl:375-408)              */
+/* ------------------------------------------------------------------------- */
+
+/* Per-mesh material record, indexed by the hit's mesh id (BVH::TextureReferences,
+ * TraverseBVHStackless.glsl:44-49, SSBO binding 20; 32 B). albedo / normal:
+ * index into the texture table, -1 = none. */
+typedef struct crt_texture_ref {
+    float model_color[4];
+    int32_t albedo;
+    int32_t normal;
+    int32_t pad[2];
+} crt_texture_ref;
+
+/* One texture of the table (the reference's bindless sampler2D Textures[]):
+ * width * height RGBA8 texels, sRGB-encoded color, row-major from v = 0. */
+typedef struct crt_texture {
+    const uint32_t* texels;   /* device pointer */
+    int32_t width;
+    int32_t height;
+} crt_texture;
+
+/* GetData's outputs for one hit (32 B). */
+typedef struct crt_hit_attr {
+    float normal[3];      /* normalize(sum of half-unpacked vertex normals x barycentrics); -1 on a miss */
+    float emissivity;     /* entity Data[0]                                                            */
+    float albedo[3];      /* albedo texture at the interpolated UV, else the mesh's model color        */
+    float alpha;          /* entity Data[1] = 1 - translucency (0 on a miss)                           */
+} crt_hit_attr;
+
+/* GetData for n closest-hit records (crt_trace / crt_intersect output of
+ * this scene): interpolated normal, albedo and the entity's emissivity and
+ * alpha.  `refs` [nrefs] and `textures` [ntextures] are device arrays (the
+ * table itself and every texel array).  A miss (t < 0 or mesh < 0) gives
+ * normal -1, albedo 0, emissivity 0, alpha 0.  Texture sampling: level 0,
+ * bilinear, repeat wrap, sRGB decoded per texel before filtering.  A mesh id
+ * outside the table gives albedo 0; a texture index outside the table, or a
+ * texture without texels, falls back to the model color. */
+int crt_hit_attributes(const crt_scene_view* scene, const crt_texture_ref* refs, int32_t nrefs,
+                       const crt_texture* textures, int32_t ntextures, const crt_hit* hits, int64_t n,
+                       crt_hit_attr* out, void* stream);
+
+/* ------------------------------------------------------------------------- */
 /* Batched traversal: many scenes and ray batches in one persistent launch    */
 /* ------------------------------------------------------------------------- */
 

@@ -35,6 +35,9 @@ WIDE_GROUP_DTYPE = np.dtype([("lo_x", "<f4", 4), ("hi_x", "<f4", 4), ("lo_y", "<
                              ("lo_z", "<f4", 4), ("hi_z", "<f4", 4), ("child", "<i4", 4), ("count", "<i4", 4)])
 TRI_RECORD_DTYPE = np.dtype([("v0", "<f4", 3), ("tri", "<i4"), ("v1", "<f4", 3), ("mesh", "<i4"),
                              ("v2", "<f4", 3), ("pad", "<i4")])
+TEXTURE_REF_DTYPE = np.dtype([("model_color", "<f4", 4), ("albedo", "<i4"), ("normal", "<i4"), ("pad", "<i4", 2)])
+TEXTURE_DTYPE = np.dtype([("texels", "<u8"), ("width", "<i4"), ("height", "<i4")])
+HIT_ATTR_DTYPE = np.dtype([("normal", "<f4", 3), ("emissivity", "<f4"), ("albedo", "<f4", 3), ("alpha", "<f4")])
 
 _lib = None
 
@@ -56,6 +59,14 @@ def lib() -> C.CDLL:
         vp, i32, i64, u32, f32 = C.c_void_p, C.c_int32, C.c_int64, C.c_uint32, C.c_float
         l.oracle_trace.restype = C.c_int
         l.oracle_trace.argtypes = [i32, i32, i64, vp, vp, vp, vp, C.c_int, u32, vp, vp, i64, C.c_int, vp, i32]
+        l.oracle_hit_attributes.restype = None
+        l.oracle_hit_attributes.argtypes = [vp, vp, vp, vp, i32, vp, i32, vp, i64, vp]
+        l.oracle_half_to_float.restype = f32
+        l.oracle_half_to_float.argtypes = [u32]
+        l.oracle_sample.restype = None
+        l.oracle_sample.argtypes = [vp, i32, i32, f32, f32, vp]
+        l.oracle_srgb_table.restype = None
+        l.oracle_srgb_table.argtypes = [vp]
         l.oracle_trace_wide.restype = C.c_int
         l.oracle_trace_wide.argtypes = [i32, i32, i64, vp, vp, vp, C.c_int, u32, vp, vp, i64, C.c_int, vp, i32, i32]
         l.oracle_bruteforce_closest.restype = C.c_int
@@ -123,6 +134,40 @@ def trace_wide(width: int, groups: np.ndarray, records: np.ndarray, ents: np.nda
     if rc != 0:
         raise RuntimeError("oracle_trace_wide failed")
     return (hits, tests) if with_tests else hits
+
+
+def hit_attributes(tris, verts, ents, refs, textures, hits):
+    """GetData on the host.  textures: list of (texels uint32 [h, w] or [h*w], width, height)."""
+    tris = np.ascontiguousarray(tris, TRIANGLE_DTYPE)
+    verts = np.ascontiguousarray(verts, VERTEX_DTYPE)
+    ents = np.ascontiguousarray(ents, ENTITY_DTYPE)
+    refs = np.ascontiguousarray(refs, TEXTURE_REF_DTYPE)
+    hits = np.ascontiguousarray(hits, HIT_DTYPE)
+    keep = [np.ascontiguousarray(t[0], np.uint32) for t in textures]
+    desc = np.zeros(max(1, len(textures)), TEXTURE_DTYPE)
+    for k, (t, (_, w, h)) in enumerate(zip(keep, textures)):
+        desc[k] = (t.ctypes.data, w, h)
+    out = np.empty(hits.size, HIT_ATTR_DTYPE)
+    lib().oracle_hit_attributes(_p(tris), _p(verts), _p(ents), _p(refs), refs.size, _p(desc), len(textures),
+                                _p(hits), hits.size, _p(out))
+    return out
+
+
+def half_to_float(h: int) -> float:
+    return float(lib().oracle_half_to_float(int(h)))
+
+
+def sample(texels, width: int, height: int, u: float, v: float) -> np.ndarray:
+    t = np.ascontiguousarray(texels, np.uint32)
+    out = np.empty(3, np.float32)
+    lib().oracle_sample(_p(t), width, height, float(u), float(v), _p(out))
+    return out
+
+
+def srgb_table() -> np.ndarray:
+    out = np.empty(256, np.float32)
+    lib().oracle_srgb_table(_p(out))
+    return out
 
 
 def bruteforce_closest(tris, verts, ent, rays, tri_begin: int = 0, tri_end: int | None = None):
