@@ -191,6 +191,39 @@ def batched_vs_plain(crt, args, plain_choice: dict, stream) -> dict:
             "layouts": per_layout}
 
 
+def time_attributes(crt, scene, d_hits, n: int, hits: np.ndarray, stream, steps: int) -> dict:
+    """crt_hit_attributes over the GI hits: every sphere (mesh) gets a model colour; every other one an
+    albedo texture (8 textures of 256x256 RGBA8).  Algorithmic bytes per hit: 32 (hit) + 32 (written
+    record), and for a hit: 16 (triangle) + 3 x 16 (packed vertex halves) + 32 (material) + 8 (entity
+    data), + 16 (texture descriptor) + 4 x 4 (texels) when textured."""
+    rng = np.random.default_rng(7)
+    nmesh = int(hits["mesh"].max()) + 1 if (hits["t"] > 0).any() else 1
+    refs = np.zeros(nmesh, crt.TEXTURE_REF_DTYPE)
+    refs["model_color"] = rng.uniform(0, 1, size=(nmesh, 4)).astype(np.float32)
+    refs["albedo"] = np.where(np.arange(nmesh) % 2 == 0, np.arange(nmesh) % 8, -1)
+    refs["normal"] = -1
+    textures = [rng.integers(0, 2 ** 32, size=(256, 256), dtype=np.uint64).astype(np.uint32) for _ in range(8)]
+    mats = crt.MaterialTable(refs, textures)
+    out = crt.DeviceArray(crt.HIT_ATTR_DTYPE, n)
+    crt.hit_attributes(scene, mats, d_hits, out, n, stream=stream)
+    stream.sync()
+    ev = Events(crt, 2)
+    ev.record(0, stream.handle)
+    for _ in range(steps):
+        crt.hit_attributes(scene, mats, d_hits, out, n, stream=stream)
+    ev.record(1, stream.handle)
+    ms = ev.ms(0, 1) / steps
+    ev.close()
+    hit = hits["t"] > 0
+    textured = hit & (refs["albedo"][np.clip(hits["mesh"], 0, nmesh - 1)] >= 0)
+    nbytes = 64.0 * n + 136.0 * hit.sum() + 32.0 * textured.sum()
+    res = {"hits": int(n), "hit_fraction": round(float(hit.mean()), 4), "textured_fraction": round(float(textured.mean()), 4),
+           "ms": round(ms, 4), "mhits_s": round(n / ms / 1e3, 1), "bytes_per_launch": nbytes,
+           "achieved_GBs": round(nbytes / (ms * 1e-3) / 1e9, 1)}
+    log(f"hit attributes: {n} GI hits in {ms:.4f} ms -> {res['mhits_s']} Mhits/s, {res['achieved_GBs']} GB/s algorithmic")
+    return res
+
+
 def copy_probe(crt, args) -> int:
     """Streaming copy of a buffer of known size: measured HBM read+write bandwidth."""
     nbytes = (args.copy_probe // 16) * 16
@@ -290,7 +323,7 @@ def main(argv=None) -> int:
         key = (c["width"], c["max_leaf"])
         if key not in wides:
             t = time.perf_counter()
-            wides[key] = crt.upload_wide(fmt, hb, *key)
+            wides[key] = crt.upload_wide(fmt, hb, *key, triangles=scene.triangles, vertices=scene.vertices)
             st = wides[key].host.stats
             log(f"wide layout {key}: {st['node_count']} nodes, fill {st['mean_fill']:.2f}, depth {st['max_depth']}, "
                 f"built in {time.perf_counter() - t:.2f}s")
@@ -379,6 +412,9 @@ def main(argv=None) -> int:
     log(f"GI: {out['gi']['visits']:.1f} node visits, {out['gi']['tests']:.1f} tri tests per ray, "
         f"{out['gi']['bytes'] / n_gi:.0f} B/ray, {achieved:.0f} GB/s algorithmic; "
         f"iteration-capped {out['gi']['capped'] * 100:.2f}%")
+
+    # --- hit attributes (GetData) of the GI hits: one launch, HIP events -----------------------
+    attrs = time_attributes(crt, best_scene, h_gi, n_gi, out["gi"]["hits"], stream, K)
 
     # --- batched persistent kernel: config-2 batches and 64 heterogeneous scenes ---------------
     batched = None
@@ -479,6 +515,7 @@ def main(argv=None) -> int:
                      "traffic": traffic, "traffic_source": traffic_src,
                      "bytes_per_launch": out["gi"]["bytes"], "launch_ms": round(gi_s * 1e3, 4),
                      "bytes_per_ray": round(out["gi"]["bytes"] / n_gi, 1)},
+        "attributes": attrs,
         "cpu_baseline": cpu,
         "parity": parity,
         "detail": {k: {kk: vv for kk, vv in v.items() if kk != "hits"} for k, v in out.items()},

@@ -228,20 +228,23 @@ class DeviceWide:
     """A wide layout uploaded to the device, with its crt_scene_view (scene)."""
 
     def __init__(self, w: WideBVH, triangles=None, vertices=None):
+        """triangles / vertices: the BVH2 scene's device buffers (DeviceArray or device address), needed only
+        by crt_hit_attributes."""
         self.host = w
         self.groups = DeviceArray.from_host(w.groups)
         self.records = DeviceArray.from_host(w.records)
         self.entities = DeviceArray.from_host(w.entities)
-        self.scene = SceneView(w.format, w.entities.size, w.stats["node_count"], self.groups.ptr,
-                               triangles.ptr if triangles is not None else None,
-                               vertices.ptr if vertices is not None else None, self.entities.ptr, self.records.ptr)
+        addr = lambda b: b.ptr if isinstance(b, DeviceArray) else b
+        self.scene = SceneView(w.format, w.entities.size, w.stats["node_count"], self.groups.ptr, addr(triangles),
+                               addr(vertices), self.entities.ptr, self.records.ptr)
 
 
-def upload_wide(src_format: int, host_buffers: dict, width: int = 4, max_leaf: int = 2) -> DeviceWide:
+def upload_wide(src_format: int, host_buffers: dict, width: int = 4, max_leaf: int = 2, triangles=None,
+                vertices=None) -> DeviceWide:
     """Derives the wide layout from a context's host buffers and uploads it."""
     hb = host_buffers
     return DeviceWide(wide_build(src_format, hb["nodes"], hb["triangles"], hb["vertices"], hb["entities"], width,
-                                 max_leaf))
+                                 max_leaf), triangles, vertices)
 
 
 def _colmajor(m) -> np.ndarray:

@@ -116,7 +116,8 @@ class BatchedWorkload:
         t0 = time.perf_counter()
         for s in self.scenes:
             if (width, max_leaf) not in s.wide:
-                s.wide[(width, max_leaf)] = crt.upload_wide(FORMAT_STACKLESS, s.host, width, max_leaf)
+                s.wide[(width, max_leaf)] = crt.upload_wide(FORMAT_STACKLESS, s.host, width, max_leaf,
+                                                            s.scene.triangles, s.scene.vertices)
         return time.perf_counter() - t0
 
 
