@@ -14,7 +14,7 @@ import numpy as np
 
 from ._lib import (KERNEL_AUTO, KERNEL_PERSISTENT, KERNEL_THREAD_PER_RAY, KERNEL_WAVE, BOUNDS_DTYPE, ENTITY_DTYPE, FLAG_IGNORE_TRANSPARENT, FLAG_TRI0_IS_MISS, FORMAT_STACK,
                    FORMAT_STACKLESS, FORMAT_WIDE4, FORMAT_WIDE8, WIDE_GROUP_DTYPE, TRI_RECORD_DTYPE, WideStats,
-                   TEXTURE_REF_DTYPE, TEXTURE_DTYPE, HIT_ATTR_DTYPE,
+                   TEXTURE_REF_DTYPE, TEXTURE_DTYPE, HIT_ATTR_DTYPE, PATH_PARAMS_DTYPE, PATH_SAMPLE_DTYPE,
                    HIT_DTYPE, QUERY_ANY, QUERY_CLOSEST, RAY_DTYPE, STACK_NODE_DTYPE,
                    STACKLESS_NODE_DTYPE, STATUS_BAD_INDEX, STATUS_ITER_CAP, STATUS_STACK_OVERFLOW,
                    STATUS_VISITS_MASK, SWAP_NONE, SWAP_SEEDED, TRIANGLE_DTYPE, VERTEX_DTYPE, BuildOptions,
@@ -31,6 +31,7 @@ __all__ = [
     "autotune_batched", "ShadeParams", "SHADE_RECORD_DTYPE", "ShadingModel", "FORMAT_WIDE4", "FORMAT_WIDE8",
     "WIDE_GROUP_DTYPE", "TRI_RECORD_DTYPE", "WideBVH", "wide_build", "DeviceWide", "upload_wide",
     "TEXTURE_REF_DTYPE", "TEXTURE_DTYPE", "HIT_ATTR_DTYPE", "MaterialTable", "hit_attributes",
+    "PATH_PARAMS_DTYPE", "PATH_SAMPLE_DTYPE", "path_params", "trace_paths",
 ]
 
 
@@ -222,6 +223,29 @@ def hit_attributes(scene: SceneView, materials: MaterialTable, hits: DeviceArray
         raise ValueError("hit / attribute buffers smaller than n")
     check(lib().crt_hit_attributes(C.byref(scene), materials.refs.ptr, materials.nrefs, materials.desc.ptr,
                                    materials.ntextures, hits.ptr, n, out.ptr, _stream_handle(stream)))
+
+
+def path_params(width: int, height: int, spp: int, inv_view, inv_proj, albedo=None, seed: int = 5,
+                max_bounces: int = 2, rr: float = 0.8, tmax: float = 1e6, sky=(1.0, 1.0, 1.0)) -> np.ndarray:
+    """A crt_path_params record; `albedo` is a device array of nmesh * 3 floats (or None)."""
+    p = np.zeros(1, PATH_PARAMS_DTYPE)
+    p["inv_view"], p["inv_proj"] = _colmajor(inv_view), _colmajor(inv_proj)
+    p["width"], p["height"], p["spp"], p["seed"] = width, height, spp, seed & 0xFFFFFFFF
+    p["max_bounces"], p["rr"], p["tmax"], p["sky"] = max_bounces, rr, tmax, sky
+    if albedo is not None:
+        p["albedo"], p["nmesh"] = albedo.ptr, albedo.count // 3
+    return p
+
+
+def trace_paths(scene: SceneView, params: np.ndarray, out: DeviceArray, npaths: int | None = None, variant: int = 0,
+                grid: int = 0, stream=None) -> None:
+    """crt_trace_paths: paths [0, npaths) into out (PATH_SAMPLE_DTYPE), asynchronous."""
+    params = np.ascontiguousarray(params, PATH_PARAMS_DTYPE).reshape(1)
+    npaths = out.count if npaths is None else int(npaths)
+    if npaths > out.count:
+        raise ValueError("output smaller than npaths")
+    cfg = TraceConfig(0, 0, 0, variant, grid, 0, None)
+    check(lib().crt_trace_paths(C.byref(scene), ptr(params), out.ptr, npaths, C.byref(cfg), _stream_handle(stream)))
 
 
 class DeviceWide:

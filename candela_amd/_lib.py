@@ -54,6 +54,12 @@ TEXTURE_REF_DTYPE = np.dtype([("model_color", "<f4", 4), ("albedo", "<i4"), ("no
 TEXTURE_DTYPE = np.dtype([("texels", "<u8"), ("width", "<i4"), ("height", "<i4")])
 HIT_ATTR_DTYPE = np.dtype([("normal", "<f4", 3), ("emissivity", "<f4"), ("albedo", "<f4", 3), ("alpha", "<f4")])
 assert TEXTURE_REF_DTYPE.itemsize == 32 and TEXTURE_DTYPE.itemsize == 16 and HIT_ATTR_DTYPE.itemsize == 32
+PATH_PARAMS_DTYPE = np.dtype([("inv_view", "<f4", 16), ("inv_proj", "<f4", 16), ("width", "<i4"), ("height", "<i4"),
+                              ("spp", "<i4"), ("seed", "<u4"), ("max_bounces", "<i4"), ("rr", "<f4"), ("tmax", "<f4"),
+                              ("sky", "<f4", 3), ("albedo", "<u8"), ("nmesh", "<i4"), ("pad", "<i4")])
+PATH_SAMPLE_DTYPE = np.dtype([("radiance", "<f4", 3), ("segments", "<i4"), ("visits", "<u4"), ("tests", "<u4"),
+                              ("status", "<u4"), ("pad", "<u4")])
+assert PATH_PARAMS_DTYPE.itemsize == 184 and PATH_SAMPLE_DTYPE.itemsize == 32
 assert VERTEX_DTYPE.itemsize == 32 and TRIANGLE_DTYPE.itemsize == 16
 assert STACKLESS_NODE_DTYPE.itemsize == 32 and STACK_NODE_DTYPE.itemsize == 64
 assert ENTITY_DTYPE.itemsize == 192 and RAY_DTYPE.itemsize == 32 and HIT_DTYPE.itemsize == 32
@@ -187,6 +193,8 @@ SIGNATURES = {
     "crt_context_build_wide": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
     "crt_hit_attributes": (C.c_int, [C.POINTER(SceneView), C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, C.c_void_p,
                                      C.c_int64, C.c_void_p, C.c_void_p]),
+    "crt_trace_paths": (C.c_int, [C.POINTER(SceneView), C.c_void_p, C.c_void_p, C.c_int64, C.POINTER(TraceConfig),
+                                  C.c_void_p]),
     "crt_context_wide_scene": (C.c_int, [C.c_void_p, C.POINTER(SceneView)]),
 }
 

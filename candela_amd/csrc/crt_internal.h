@@ -24,4 +24,8 @@ struct BvhResult {
 int build_bvh(int format, const crt_mesh* meshes, int mesh_count, int32_t t_offset,
               const crt_build_options* options, BvhResult& res);
 
+// A zeroed-by-the-launch device counter from the per-device pool of the
+// persistent kernels (one slot per launch, handed out round robin).
+int counter_slot(unsigned long long** out);
+
 }  // namespace crt

@@ -1,0 +1,418 @@
+// Multi-bounce diffuse path tracing with wavefront path regeneration in one
+// persistent kernel (the large-scene workload: a displaced terrain plus
+// icospheres, 3840 x 2160 x 4 samples, 2 diffuse bounces, Russian roulette
+// after the first bounce).
+//
+// Every lane owns one path.  A path's segments are walked by the wide walk of
+// wide_walk.h (Core, same node step, same pending-leaf parking); when a
+// segment's entity loop ends, the lane shades in place -- a miss adds
+// throughput x sky and ends the path, a hit multiplies the throughput by the
+// mesh albedo and, unless the path is done, spawns the next segment -- and
+// keeps stepping.  A lane whose path ended claims the next path index from a
+// counter together with the other idle lanes of its wave (one atomic per
+// wave, ranks by mbcnt) and generates that path's camera ray on the spot, so
+// no ray or hit buffer exists between segments and no wave waits for its
+// longest path.
+//
+// Per path p (all FP32, no contraction; the host twin in oracle/oracle_path.c
+// follows the same sequence bit for bit):
+//   rng      state = pcg(p ^ (seed * 0x9E3779B9)); each draw state = pcg(state),
+//            uniform = (state >> 8) * 2^-24
+//   camera   pixel = p / spp, x = pixel % width, y = pixel / width; jitter
+//            (draw, draw); IntersectPrimary's GetRayDirectionAt
+//            (TraverseBVHStackless.glsl:143-148) at uv = ((x + jx) / width,
+//            (y + jy) / height), direction normalised
+//   segment  closest hit over the scene (entity loop, tmax as given)
+//   miss     radiance += throughput * sky; end
+//   hit      throughput *= albedo[mesh] (0 for a mesh beyond the table);
+//            end after `max_bounces` bounces; from the second bounce on,
+//            continue with probability rr (draw < rr) and throughput /= rr;
+//            next direction cosine-weighted about the face normal turned
+//            against the ray (basis of CosWeightedHemisphere,
+//            Include/Sampling.glsl:1-12; sin and cos of the azimuth from a
+//            quarter-angle Taylor polynomial so host and device agree),
+//            origin = hit + normal * 1e-4
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+
+#include "candela_rt.h"
+#include "crt_internal.h"
+#include "trace_launch.h"
+#include "wide_walk.h"
+
+#pragma clang fp contract(off)
+
+namespace crt {
+namespace path {
+
+using namespace ::crt::dm;
+using wide::Core;
+using wide::kEnd;
+using wide::kLeaves;
+using wide::Stack;
+
+__device__ __forceinline__ uint32_t pcg(uint32_t v) {
+    const uint32_t state = v * 747796405u + 2891336453u;
+    const uint32_t word = ((state >> ((state >> 28u) + 4u)) ^ state) * 277803737u;
+    return (word >> 22u) ^ word;
+}
+
+__device__ __forceinline__ float draw(uint32_t& st) {
+    st = pcg(st);
+    return float(st >> 8) * 5.9604644775390625e-8f;   // 2^-24
+}
+
+__device__ __forceinline__ f3 add(f3 a, f3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ f3 scale(f3 a, float s) { return mk(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ f3 normalize(f3 a) { return scale(a, 1.0f / sqrtf(dot(a, a))); }
+
+// sin and cos of 2 pi u, u in [0, 1): quadrant q = floor(4u), angle b = (4u - q)
+// pi/2 in [0, pi/2), a = b - pi/4 in [-pi/4, pi/4); Taylor polynomials of sin a
+// (to a^9) and cos a (to a^10), error below 4e-9; sin b = (sin a + cos a) / sqrt2,
+// cos b = (cos a - sin a) / sqrt2, then the exact quadrant rotation.
+__device__ __forceinline__ void sincos_turn(float u, float& s, float& c) {
+    const float u4 = u * 4.0f;
+    float q = floorf(u4);
+    if (q > 3.0f) q = 3.0f;
+    const float a = (u4 - q) * 1.57079632679f - 0.785398163397f;
+    const float a2 = a * a;
+    const float sa = a * (1.0f + a2 * (-1.66666667e-1f + a2 * (8.33333333e-3f + a2 * (-1.98412698e-4f +
+                                                                                    a2 * 2.75573192e-6f))));
+    const float ca = 1.0f + a2 * (-0.5f + a2 * (4.16666667e-2f + a2 * (-1.38888889e-3f + a2 * (2.48015873e-5f +
+                                                                                            a2 * -2.75573192e-7f))));
+    const float r = 0.707106781187f;
+    const float sb = (sa + ca) * r, cb = (ca - sa) * r;
+    const int qi = int(q);
+    s = qi == 0 ? sb : qi == 1 ? cb : qi == 2 ? -sb : -cb;
+    c = qi == 0 ? cb : qi == 1 ? -sb : qi == 2 ? -cb : sb;
+}
+
+__device__ __forceinline__ f3 cosine_dir(f3 n, float u1, float u2) {
+    float s, c;
+    sincos_turn(u1, s, c);
+    const float ra = sqrtf(u2);
+    const float rx = ra * c, ry = ra * s, rz = sqrtf(1.0f - u2);
+    f3 uu = cross(n, mk(0.0f, 1.0f, 1.0f));
+    if (dot(uu, uu) < 1e-12f) uu = cross(n, mk(1.0f, 0.0f, 0.0f));
+    uu = normalize(uu);
+    const f3 vv = cross(uu, n);
+    return add(add(scale(uu, rx), scale(vv, ry)), scale(n, rz));
+}
+
+struct Params {
+    Camera cam;
+    int32_t width, height, spp;
+    uint32_t seed;
+    int32_t max_bounces;
+    float rr;
+    float sky[3];
+    const float* albedo;   // nmesh * 3
+    int32_t nmesh;
+};
+
+template <int W, int LDS_DEPTH, int INNER, int LEAF_BATCH>
+struct PathWave {
+    Core<false, W, LDS_DEPTH> c;
+    int64_t path;
+    f3 org, dir;
+    f3 thr, rad;
+    uint32_t rng;
+    int32_t bounce, segs;
+    uint32_t visits, tests;
+    int32_t ent, hit_ent;
+    int32_t leaf, off;
+    bool active, leafing, walk_done;
+
+    __device__ __forceinline__ bool begin_walk(const SceneArgs& s) {
+        const int32_t e = ent + 1;
+        ent = e;
+        if (e >= s.entity_count) return false;
+        const crt_entity& en = s.entities[e];
+        c.begin(xform(en.inverse, org, 1.0f), xform(en.inverse, dir, 0.0f), c.tmax, en.node_offset, en.node_count);
+        walk_done = false;
+        return true;
+    }
+
+    // A new segment from (org, dir); false when the scene has no entity.
+    __device__ __forceinline__ bool start_segment(const SceneArgs& s, float tmax) {
+        ent = -1;
+        hit_ent = -1;
+        c.tmax = tmax;
+        c.tri = c.mesh = -1;
+        c.tests = 0;
+        segs++;
+        return begin_walk(s);
+    }
+
+    // The segment's entity loop is over: shade.  True when the path continues
+    // (org, dir hold the next segment's ray).
+    __device__ __forceinline__ bool shade(const SceneArgs& s, const Params& P) {
+        tests += c.tests;
+        if (c.tri < 0) {
+            rad = add(rad, mk(thr.x * P.sky[0], thr.y * P.sky[1], thr.z * P.sky[2]));
+            return false;
+        }
+        const float t = c.tmax;
+        const int32_t mesh = c.mesh;
+        f3 alb = mk(0.0f, 0.0f, 0.0f);
+        if (mesh >= 0 && mesh < P.nmesh) alb = mk(P.albedo[3 * mesh], P.albedo[3 * mesh + 1], P.albedo[3 * mesh + 2]);
+        thr = mk(thr.x * alb.x, thr.y * alb.y, thr.z * alb.z);
+        if (bounce >= P.max_bounces) return false;
+        if (bounce >= 1) {
+            if (!(draw(rng) < P.rr)) return false;
+            thr = mk(thr.x / P.rr, thr.y / P.rr, thr.z / P.rr);
+        }
+        // face normal of the hit triangle in world space (inverse-transpose of the entity's model)
+        const float4* r = reinterpret_cast<const float4*>(s.tri_records) + 3 * int64_t(c.tri);
+        const float4 va = r[0], vb = r[1], vc = r[2];
+        const f3 a = mk(va.x, va.y, va.z);
+        const f3 no = cross(sub(mk(vb.x, vb.y, vb.z), a), sub(mk(vc.x, vc.y, vc.z), a));
+        const float* inv = s.entities[hit_ent].inverse;
+        f3 n = normalize(mk(inv[0] * no.x + inv[1] * no.y + inv[2] * no.z, inv[4] * no.x + inv[5] * no.y + inv[6] * no.z,
+                            inv[8] * no.x + inv[9] * no.y + inv[10] * no.z));
+        if (dot(n, dir) > 0.0f) n = neg(n);
+        const f3 p = add(org, scale(dir, t));
+        const float u1 = draw(rng), u2 = draw(rng);
+        org = add(p, scale(n, 1e-4f));
+        dir = cosine_dir(n, u1, u2);
+        bounce++;
+        return true;
+    }
+
+    __device__ __forceinline__ void finish(crt_path_sample* __restrict__ out) {
+        float4* o = reinterpret_cast<float4*>(out + path);
+        o[0] = make_float4(rad.x, rad.y, rad.z, __int_as_float(segs));
+        o[1] = make_float4(__uint_as_float(visits), __uint_as_float(tests), __uint_as_float(c.status), 0.0f);
+        active = false;
+    }
+
+    // Segment over: shade, then the next segment or the path's record.
+    __device__ __forceinline__ void segment_done(const SceneArgs& s, const Params& P, float tmax,
+                                                 crt_path_sample* __restrict__ out) {
+        while (shade(s, P)) {
+            if (start_segment(s, tmax)) return;
+            c.tri = -1;   // no entity: the new segment misses
+        }
+        finish(out);
+    }
+
+    __device__ __forceinline__ void start_path(const SceneArgs& s, const Params& P, int64_t p, float tmax,
+                                               crt_path_sample* __restrict__ out) {
+        active = true;
+        leafing = false;
+        walk_done = false;
+        path = p;
+        rng = pcg(uint32_t(p) ^ (P.seed * 0x9E3779B9u));
+        const int64_t pixel = p / P.spp;
+        const int32_t x = int32_t(pixel % P.width), y = int32_t(pixel / P.width);
+        const float jx = draw(rng), jy = draw(rng);
+        const float u = (float(x) + jx) / float(P.width), v = (float(y) + jy) / float(P.height);
+        const float cx = u * 2.0f - 1.0f, cy = v * 2.0f - 1.0f;
+        const float* ip = P.cam.inv_proj;
+        const float ex = ip[0] * cx + ip[4] * cy + ip[8] * -1.0f + ip[12] * 1.0f;
+        const float ey = ip[1] * cx + ip[5] * cy + ip[9] * -1.0f + ip[13] * 1.0f;
+        const f3 d = xform(P.cam.inv_view, mk(ex, ey, -1.0f), 0.0f);
+        dir = scale(d, 1.0f / sqrtf(dot(d, d)));
+        org = mk(P.cam.inv_view[12], P.cam.inv_view[13], P.cam.inv_view[14]);
+        thr = mk(1.0f, 1.0f, 1.0f);
+        rad = mk(0.0f, 0.0f, 0.0f);
+        bounce = 0;
+        segs = 0;
+        visits = 0;
+        tests = 0;
+        c.status = 0;
+        if (!start_segment(s, tmax)) {
+            c.tri = -1;
+            segment_done(s, P, tmax, out);
+        }
+    }
+
+    __device__ __forceinline__ void round(const SceneArgs& s, const Params& P, float tmax, int32_t max_iter,
+                                          Stack<LDS_DEPTH>& st, crt_path_sample* __restrict__ out) {
+        const float4* __restrict__ rec = reinterpret_cast<const float4*>(s.tri_records);
+        for (int k = 0; k < INNER; ++k) {
+            if (active && !walk_done && !leafing) {
+                const int32_t r = c.step(s, max_iter, st);
+                if (r == kEnd) {
+                    walk_done = true;
+                } else if (r == kLeaves) {
+                    leafing = true;
+                    leaf = st.leaf(0);
+                    c.pi = 1;
+                    off = 0;
+                }
+            }
+            const unsigned long long stepping = __ballot(active && !walk_done && !leafing);
+            if (stepping == 0 || __popcll(__ballot(leafing)) >= LEAF_BATCH) break;
+        }
+        while (__ballot(leafing)) {
+            if (leafing) {
+                const int32_t first = (leaf >> 4) + off, left = (leaf & 0xF) - off;
+                const float4* p = rec + 3 * int64_t(first);
+                const float4 a0 = p[0], b0 = p[1], c0 = p[2];
+                float4 a1 = a0, b1 = b0, c1 = c0;
+                if (left > 1) {
+                    a1 = p[3];
+                    b1 = p[4];
+                    c1 = p[5];
+                }
+                const float before = c.tmax;
+                c.test(first, a0, b0, c0);
+                if (left > 1) c.test(first + 1, a1, b1, c1);
+                if (c.tmax < before) hit_ent = ent;
+                off += left > 1 ? 2 : 1;
+                if (off >= (leaf & 0xF)) {
+                    if (c.pi < c.np) {
+                        leaf = st.leaf(c.pi);
+                        ++c.pi;
+                        off = 0;
+                    } else {
+                        leafing = false;
+                    }
+                }
+            }
+        }
+        if (__ballot(walk_done)) {
+            if (walk_done) {
+                walk_done = false;
+                visits += uint32_t(c.it);
+                if (!begin_walk(s)) segment_done(s, P, tmax, out);
+            }
+        }
+    }
+
+    __device__ __forceinline__ void run(const SceneArgs& s, const Params& P, int64_t npaths, float tmax,
+                                        int32_t max_iter, Stack<LDS_DEPTH> st, unsigned long long* __restrict__ counter,
+                                        int32_t refill, crt_path_sample* __restrict__ out) {
+        active = false;
+        leafing = false;
+        walk_done = false;
+        bool more = true;
+        while (true) {
+            if (more) {
+                const unsigned long long idle = __ballot(!active);
+                const int32_t cnt = __popcll(idle);
+                if (cnt >= refill) {
+                    unsigned long long base = 0;
+                    if (__lane_id() == 0) base = atomicAdd(counter, (unsigned long long)cnt);
+                    base = (unsigned long long)__shfl((long long)base, 0);
+                    if (base + (unsigned long long)cnt >= (unsigned long long)npaths) more = false;
+                    if (!active) {
+                        const int32_t rank = __builtin_amdgcn_mbcnt_hi(uint32_t(idle >> 32),
+                                                                       __builtin_amdgcn_mbcnt_lo(uint32_t(idle), 0));
+                        const unsigned long long idx = base + (unsigned long long)rank;
+                        if (idx < (unsigned long long)npaths) start_path(s, P, int64_t(idx), tmax, out);
+                    }
+                }
+            }
+            if (!__ballot(active)) {
+                if (!more) break;
+                continue;
+            }
+            round(s, P, tmax, max_iter, st, out);
+        }
+    }
+};
+
+template <int W, int LDS_DEPTH, int INNER, int LEAF_BATCH, int REFILL>
+__global__ void __launch_bounds__(256) path_kernel(SceneArgs s, Params P, int64_t npaths, float tmax,
+                                                   int32_t max_iter, unsigned long long* __restrict__ counter,
+                                                   crt_path_sample* __restrict__ out) {
+    __shared__ int32_t lds[wide::lds_ints<W, LDS_DEPTH>() * 256];
+    int32_t spill_node[wide::kSpill];
+    float spill_t[wide::kSpill];
+    Stack<LDS_DEPTH> st;
+    st.lds = lds + threadIdx.x;
+    st.stride = 256;
+    st.spill_node = spill_node;
+    st.spill_t = spill_t;
+    PathWave<W, LDS_DEPTH, INNER, LEAF_BATCH> w;
+    w.run(s, P, npaths, tmax, max_iter, st, counter, REFILL, out);
+}
+
+__global__ void reset64(unsigned long long* c) { *c = 0ull; }
+
+// Presets (variant, LDS stack entries, INNER, LEAF_BATCH, REFILL).
+#define CRT_PATH_PRESETS(X) \
+    X(0, 8, 4, 32, 16)      \
+    X(1, 8, 4, 16, 8)       \
+    X(2, 8, 8, 32, 32)      \
+    X(3, 8, 4, 48, 24)
+
+template <int W>
+const void* path_fn(int variant) {
+#define CRT_PATH_FN(V, D, I, LB, R) \
+    case V: return reinterpret_cast<const void*>(&path_kernel<W, D, I, LB, R>);
+    switch (variant) {
+        CRT_PATH_PRESETS(CRT_PATH_FN)
+        default: return nullptr;
+    }
+#undef CRT_PATH_FN
+}
+
+}  // namespace path
+}  // namespace crt
+
+extern "C" int crt_trace_paths(const crt_scene_view* scene, const crt_path_params* params, crt_path_sample* out,
+                               int64_t npaths, const crt_trace_config* config, void* stream) {
+    if (!scene || !params) return crt::set_error(CRT_E_INVALID, "crt_trace_paths: null scene or params");
+    if (scene->format != CRT_FORMAT_WIDE4 && scene->format != CRT_FORMAT_WIDE8)
+        return crt::set_error(CRT_E_INVALID, "crt_trace_paths: the path kernel walks the wide layouts");
+    if (scene->entity_count > 0 && (!scene->nodes || !scene->tri_records || !scene->entities || scene->total_nodes <= 0))
+        return crt::set_error(CRT_E_INVALID, "crt_trace_paths: null scene buffer");
+    const crt_path_params& p = *params;
+    if (p.width <= 0 || p.height <= 0 || p.spp <= 0 || p.max_bounces < 0 || !(p.rr > 0.0f && p.rr <= 1.0f))
+        return crt::set_error(CRT_E_INVALID, "crt_trace_paths: bad image size, samples, bounces or roulette");
+    if (p.nmesh < 0 || (p.nmesh > 0 && !p.albedo)) return crt::set_error(CRT_E_INVALID, "crt_trace_paths: bad albedo table");
+    if (npaths < 0 || npaths > int64_t(p.width) * p.height * p.spp)
+        return crt::set_error(CRT_E_INVALID, "crt_trace_paths: npaths exceeds width * height * spp");
+    if (npaths > 0 && !out) return crt::set_error(CRT_E_INVALID, "crt_trace_paths: out is null");
+    if (npaths >= (int64_t(1) << 32)) return crt::set_error(CRT_E_INVALID, "crt_trace_paths: >= 2^32 paths");
+    const int variant = config ? config->variant : 0;
+    if (variant < 0 || variant > 3) return crt::set_error(CRT_E_INVALID, "crt_trace_paths: variant must be 0..3");
+    if (npaths == 0) return CRT_OK;
+    crt::path::Params P;
+    std::memcpy(P.cam.inv_view, p.inv_view, sizeof(P.cam.inv_view));
+    std::memcpy(P.cam.inv_proj, p.inv_proj, sizeof(P.cam.inv_proj));
+    P.cam.tmax = p.tmax;
+    P.width = p.width;
+    P.height = p.height;
+    P.spp = p.spp;
+    P.seed = p.seed;
+    P.max_bounces = p.max_bounces;
+    P.rr = p.rr;
+    P.sky[0] = p.sky[0];
+    P.sky[1] = p.sky[1];
+    P.sky[2] = p.sky[2];
+    P.albedo = p.albedo;
+    P.nmesh = p.nmesh;
+    unsigned long long* counter = nullptr;
+    int rc = crt::counter_slot(&counter);
+    if (rc) return rc;
+    const void* fn = scene->format == CRT_FORMAT_WIDE8 ? crt::path::path_fn<8>(variant) : crt::path::path_fn<4>(variant);
+    int grid = config && config->grid > 0 ? config->grid : 0;
+    if (grid == 0) {
+        int dev = 0, cus = 0, per_cu = 0;
+        if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 256;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, 0) != hipSuccess || per_cu <= 0) per_cu = 4;
+        grid = cus * per_cu;
+    }
+    const int64_t need = (npaths + 255) / 256;
+    if (need < grid) grid = int(need);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(crt::path::reset64, dim3(1), dim3(1), 0, st, counter);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return crt::set_error(CRT_E_HIP, std::string("crt_trace_paths: reset: ") + hipGetErrorString(e));
+    crt::SceneArgs a = crt::scene_args(*scene);
+    int64_t n = npaths;
+    float tmax = p.tmax;
+    int32_t max_iter = crt::kMaxIterations;
+    crt_path_sample* o = out;
+    void* args[] = {&a, &P, &n, &tmax, &max_iter, &counter, &o};
+    e = hipLaunchKernel(fn, dim3(unsigned(grid)), dim3(256), args, 0, st);
+    if (e != hipSuccess) return crt::set_error(CRT_E_HIP, std::string("crt_trace_paths: launch: ") + hipGetErrorString(e));
+    return CRT_OK;
+}

@@ -75,7 +75,7 @@ static std::mutex g_counter_mutex;
 static unsigned long long* g_counters[64] = {};
 static std::atomic<uint32_t> g_counter_next{0};
 
-static int counter_slot(unsigned long long** out) {
+int counter_slot(unsigned long long** out) {
     int dev = 0;
     CRT_HIP(hipGetDevice(&dev), "hipGetDevice");
     if (dev < 0 || dev >= 64) return set_error(CRT_E_INVALID, "device index out of range");

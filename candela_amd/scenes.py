@@ -131,6 +131,88 @@ def icosphere_cloud(count: int = 2000, subdivisions: int = 3, seed: int = 2, box
     return [(make_vertices(pos[k], normals[k]), flat_faces, k) for k in range(count)]
 
 
+def _lattice(ix: np.ndarray, iz: np.ndarray, seed: int) -> np.ndarray:
+    """Value-noise lattice values in [0, 1): a PCG hash of (x, z, seed)."""
+    h = (ix.astype(np.uint32) * np.uint32(73856093)) ^ (iz.astype(np.uint32) * np.uint32(19349663)) ^ \
+        np.uint32((seed * 83492791) & 0xFFFFFFFF)
+    return pcg_hash(h).astype(np.float64) / 4294967296.0
+
+
+def value_noise(x: np.ndarray, z: np.ndarray, seed: int) -> np.ndarray:
+    """Smoothly interpolated lattice noise in [0, 1) (smoothstep weights)."""
+    x0, z0 = np.floor(x), np.floor(z)
+    fx, fz = x - x0, z - z0
+    wx, wz = fx * fx * (3 - 2 * fx), fz * fz * (3 - 2 * fz)
+    ix, iz = x0.astype(np.int64), z0.astype(np.int64)
+    a = _lattice(ix, iz, seed)
+    b = _lattice(ix + 1, iz, seed)
+    c = _lattice(ix, iz + 1, seed)
+    d = _lattice(ix + 1, iz + 1, seed)
+    return (a * (1 - wx) + b * wx) * (1 - wz) + (c * (1 - wx) + d * wx) * wz
+
+
+def fbm(x: np.ndarray, z: np.ndarray, octaves: int = 6, seed: int = 5) -> np.ndarray:
+    """Fractal Brownian motion: sum over octaves of 0.5^o * value_noise(2^o * p), normalised to [0, 1)."""
+    total = np.zeros_like(x, dtype=np.float64)
+    norm = 0.0
+    for o in range(octaves):
+        amp, freq = 0.5 ** o, 2.0 ** o
+        total += amp * value_noise(x * freq, z * freq, seed + 101 * o)
+        norm += amp
+    return total / norm
+
+
+def terrain(n: int = 2048, size: float = 2048.0, height: float = 240.0, octaves: int = 6, seed: int = 5,
+            features: float = 8.0, mesh: int = 0):
+    """Config-5 terrain: an n x n grid of quads (2 n^2 triangles) over [0, size]^2 in x, z, displaced in y
+    by height * fBm(6 octaves of seed-driven value noise, `features` base cells across).  Normals from
+    central differences of the height field, UV = grid position / n."""
+    g = np.arange(n + 1, dtype=np.float64)
+    gx, gz = np.meshgrid(g, g)                         # gz rows, gx columns
+    hgt = height * fbm(gx * (features / n), gz * (features / n), octaves, seed)
+    step = size / n
+    pos = np.stack([gx * step, hgt, gz * step], axis=-1).reshape(-1, 3).astype(np.float32)
+    dx = np.gradient(hgt, axis=1) / step
+    dz = np.gradient(hgt, axis=0) / step
+    nrm = np.stack([-dx, np.ones_like(dx), -dz], axis=-1)
+    nrm /= np.linalg.norm(nrm, axis=-1, keepdims=True)
+    uv = np.stack([gx / n, gz / n], axis=-1).reshape(-1, 2)
+    v = make_vertices(pos, nrm.reshape(-1, 3).astype(np.float32), uv.astype(np.float32))
+    i = np.arange(n, dtype=np.uint32)
+    qi, qj = np.meshgrid(i, i)                         # qj rows (z), qi columns (x)
+    a = (qj * (n + 1) + qi).ravel()
+    b, c, d = a + 1, a + (n + 1), a + (n + 2)
+    idx = np.stack([a, c, b, b, c, d], axis=-1).reshape(-1).astype(np.uint32)
+    return (v, idx, mesh), hgt
+
+
+def large_scene(n: int = 2048, spheres: int = 1400, subdivisions: int = 4, seed: int = 5, size: float = 2048.0,
+                height: float = 240.0):
+    """Config-5 scene: the terrain (mesh 0) plus `spheres` icospheres (meshes 1..spheres) with radii
+    lognormal(ln 6, 0.5) resting 0..40 units above the terrain at uniform x, z.  2 n^2 + spheres * 20 * 4^s
+    triangles: 8,388,608 + 7,168,000 = 15,556,608 at the defaults."""
+    rng = np.random.default_rng(seed)
+    ter, hgt = terrain(n, size, height, seed=seed)
+    unit, faces = icosphere(subdivisions)
+    xs = rng.uniform(0.0, size, spheres)
+    zs = rng.uniform(0.0, size, spheres)
+    radii = rng.lognormal(math.log(6.0), 0.5, spheres)
+    gi = np.clip((xs / size * n).astype(np.int64), 0, n)
+    gj = np.clip((zs / size * n).astype(np.int64), 0, n)
+    ys = hgt[gj, gi] + radii + rng.uniform(0.0, 40.0, spheres)
+    centres = np.stack([xs, ys, zs], axis=-1)
+    meshes = [ter]
+    flat = faces.ravel()
+    nrm = unit.astype(np.float32)
+    for k in range(spheres):
+        pos = (unit * radii[k] + centres[k]).astype(np.float32)
+        meshes.append((make_vertices(pos, nrm), flat, 1 + k))
+    return meshes
+
+
+LARGE_CAMERA = dict(eye=(1024.0, 420.0, -260.0), center=(1024.0, 60.0, 900.0), fovy_deg=60.0)
+
+
 def scene_arrays(meshes):
     """Concatenated (positions [V,3] f32, triangles [N,3] u32) of a mesh list."""
     pos, tris, off = [], [], 0

@@ -447,6 +447,46 @@ int crt_hit_attributes(const crt_scene_view* scene, const crt_texture_ref* refs,
                        crt_hit_attr* out, void* stream);
 
 /* ------------------------------------------------------------------------- */
+/* Multi-bounce diffuse paths with wavefront regeneration (one persistent      */
+/* kernel; no reference counterpart -- the reference's nearest analogue is the */
+/* fixed two-bounce loop of DiffuseTrace.glsl:508-533)                         */
+/* ------------------------------------------------------------------------- */
+
+typedef struct crt_path_params {
+    float inv_view[16];     /* column-major camera inverses (IntersectPrimary's)   */
+    float inv_proj[16];
+    int32_t width, height;  /* image                                               */
+    int32_t spp;            /* samples per pixel: path p is sample p % spp of pixel p / spp */
+    uint32_t seed;
+    int32_t max_bounces;    /* diffuse bounces after the camera segment            */
+    float rr;               /* continue probability from the second bounce on (0, 1] */
+    float tmax;             /* every segment's TMax                                */
+    float sky[3];           /* radiance of rays that leave the scene               */
+    const float* albedo;    /* device, nmesh * 3 (per mesh id)                     */
+    int32_t nmesh;
+    int32_t pad;
+} crt_path_params;
+
+/* One path's result (32 B). */
+typedef struct crt_path_sample {
+    float radiance[3];
+    int32_t segments;       /* rays traced for this path (camera + bounces)  */
+    uint32_t visits;        /* node visits over all its segments             */
+    uint32_t tests;         /* triangle tests over all its segments          */
+    uint32_t status;        /* CRT_STATUS_* of any of its walks              */
+    uint32_t pad;
+} crt_path_sample;
+
+/* Traces paths [0, npaths) over a wide scene (CRT_FORMAT_WIDE4 / WIDE8): per
+ * path a jittered camera ray, closest hits, diffuse bounces with Russian
+ * roulette, sky radiance on escape; out[p] per path.  Device-filling waves
+ * claim new paths whenever enough lanes are idle, so segments of different
+ * paths and bounces share the waves.  config: variant 0..3 (stack depth,
+ * inner steps, leaf batch, refill threshold), grid (0 = fill the device). */
+int crt_trace_paths(const crt_scene_view* scene, const crt_path_params* params, crt_path_sample* out,
+                    int64_t npaths, const crt_trace_config* config, void* stream);
+
+/* ------------------------------------------------------------------------- */
 /* Batched traversal: many scenes and ray batches in one persistent launch    */
 /* ------------------------------------------------------------------------- */
 

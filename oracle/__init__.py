@@ -38,6 +38,11 @@ TRI_RECORD_DTYPE = np.dtype([("v0", "<f4", 3), ("tri", "<i4"), ("v1", "<f4", 3),
 TEXTURE_REF_DTYPE = np.dtype([("model_color", "<f4", 4), ("albedo", "<i4"), ("normal", "<i4"), ("pad", "<i4", 2)])
 TEXTURE_DTYPE = np.dtype([("texels", "<u8"), ("width", "<i4"), ("height", "<i4")])
 HIT_ATTR_DTYPE = np.dtype([("normal", "<f4", 3), ("emissivity", "<f4"), ("albedo", "<f4", 3), ("alpha", "<f4")])
+PATH_PARAMS_DTYPE = np.dtype([("inv_view", "<f4", 16), ("inv_proj", "<f4", 16), ("width", "<i4"), ("height", "<i4"),
+                              ("spp", "<i4"), ("seed", "<u4"), ("max_bounces", "<i4"), ("rr", "<f4"), ("tmax", "<f4"),
+                              ("sky", "<f4", 3), ("albedo", "<u8"), ("nmesh", "<i4"), ("pad", "<i4")])
+PATH_SAMPLE_DTYPE = np.dtype([("radiance", "<f4", 3), ("segments", "<i4"), ("visits", "<u4"), ("tests", "<u4"),
+                              ("status", "<u4"), ("pad", "<u4")])
 
 _lib = None
 
@@ -59,6 +64,10 @@ def lib() -> C.CDLL:
         vp, i32, i64, u32, f32 = C.c_void_p, C.c_int32, C.c_int64, C.c_uint32, C.c_float
         l.oracle_trace.restype = C.c_int
         l.oracle_trace.argtypes = [i32, i32, i64, vp, vp, vp, vp, C.c_int, u32, vp, vp, i64, C.c_int, vp, i32]
+        l.oracle_trace_paths.restype = None
+        l.oracle_trace_paths.argtypes = [vp, i32, i64, vp, vp, vp, i32, vp, vp, i64, vp, i32]
+        l.oracle_sincos_turn.restype = None
+        l.oracle_sincos_turn.argtypes = [f32, vp, vp]
         l.oracle_hit_attributes.restype = None
         l.oracle_hit_attributes.argtypes = [vp, vp, vp, vp, i32, vp, i32, vp, i64, vp]
         l.oracle_half_to_float.restype = f32
@@ -134,6 +143,28 @@ def trace_wide(width: int, groups: np.ndarray, records: np.ndarray, ents: np.nda
     if rc != 0:
         raise RuntimeError("oracle_trace_wide failed")
     return (hits, tests) if with_tests else hits
+
+
+def trace_paths(params: np.ndarray, width: int, groups, records, ents, albedo, paths, stack_cap: int = 72):
+    """The path kernel's per-path sequence on the host, for the path indices `paths`.
+    params: one PATH_PARAMS_DTYPE record (its albedo pointer is ignored; `albedo` [nmesh, 3] is used)."""
+    params = np.ascontiguousarray(params, PATH_PARAMS_DTYPE).reshape(1).copy()
+    albedo = np.ascontiguousarray(albedo, np.float32)
+    params["nmesh"] = albedo.size // 3
+    groups = np.ascontiguousarray(groups, WIDE_GROUP_DTYPE)
+    records = np.ascontiguousarray(records, TRI_RECORD_DTYPE)
+    ents = np.ascontiguousarray(ents, ENTITY_DTYPE)
+    paths = np.ascontiguousarray(paths, np.int64)
+    out = np.empty(paths.size, PATH_SAMPLE_DTYPE)
+    lib().oracle_trace_paths(_p(params), width, groups.size * 4 // width, _p(groups), _p(records), _p(ents), ents.size,
+                             _p(albedo), _p(paths), paths.size, _p(out), stack_cap)
+    return out
+
+
+def sincos_turn(u: float):
+    s, c = C.c_float(), C.c_float()
+    lib().oracle_sincos_turn(float(u), C.byref(s), C.byref(c))
+    return s.value, c.value
 
 
 def hit_attributes(tris, verts, ents, refs, textures, hits):
