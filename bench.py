@@ -224,6 +224,83 @@ def time_attributes(crt, scene, d_hits, n: int, hits: np.ndarray, stream, steps:
     return res
 
 
+def time_large(crt, args) -> dict:
+    """The large-scene workload: a 2048^2 fBm terrain (8,388,608 triangles) plus 1,400 subdivision-4
+    icospheres (7,168,000), 3840 x 2160 pixels x 4 samples, 2 diffuse bounces, roulette p = 0.8 from the
+    second bounce, sky light; one persistent launch traces every path (crt_trace_paths over the BVH4
+    layout).  Mrays/s counts every traced segment.  2,000 sampled paths are checked against the host twin."""
+    from candela_amd import scenes
+    d = max(1, args.large_scale)
+    t0 = time.perf_counter()
+    meshes = scenes.large_scene(n=2048 // d, spheres=1400 // d)
+    t1 = time.perf_counter()
+    r = crt.RayIntersector(crt.FORMAT_STACKLESS)
+    r.add_object(0, meshes)
+    del meshes
+    t2 = time.perf_counter()
+    r.push_entity(0)
+    r.buffer_entities()                              # materialises the entity record
+    hb = r.host_buffers()
+    dw = crt.upload_wide(crt.FORMAT_STACKLESS, hb, 4, 2)     # one host collapse, reused by the parity sample
+    t3 = time.perf_counter()
+    ntri = int(hb["triangles"].size)
+    nmesh = int(hb["triangles"]["packed"][:, 3].max()) + 1
+    rng = np.random.default_rng(5)
+    albedo = rng.uniform(0.2, 0.8, size=(nmesh, 3)).astype(np.float32)
+    albedo[0] = (0.45, 0.4, 0.3)                     # terrain
+    dalb = crt.DeviceArray.from_host(albedo.ravel())
+    W, H, spp = 3840 // d, 2160 // d, 4
+    iv, ip = scenes.camera_inverses(aspect=W / H, **scenes.LARGE_CAMERA)
+    p = crt.path_params(W, H, spp, iv, ip, albedo=dalb, seed=5, max_bounces=2, rr=0.8, sky=(1.0, 1.0, 1.0))
+    n = W * H * spp
+    out = crt.DeviceArray(crt.PATH_SAMPLE_DTYPE, n)
+    stream = crt.Stream()
+    scene = dw.scene
+    best = None
+    for v in range(4):
+        crt.trace_paths(scene, p, out, n, variant=v, stream=stream)   # warm-up
+        stream.sync()
+        ev = Events(crt, 2)
+        ev.record(0, stream.handle)
+        reps = 3
+        for _ in range(reps):
+            crt.trace_paths(scene, p, out, n, variant=v, stream=stream)
+        ev.record(1, stream.handle)
+        ms = ev.ms(0, 1) / reps
+        ev.close()
+        log(f"large scene: path preset {v}: {ms:.3f} ms per frame")
+        if best is None or ms < best[1]:
+            best = (v, ms)
+    crt.trace_paths(scene, p, out, n, variant=best[0], stream=stream)
+    stream.sync()
+    res = out.download()
+    segs = int(res["segments"].sum())
+    visits = res["visits"].astype(np.float64).sum()
+    tests = res["tests"].astype(np.float64).sum()
+    nbytes = 128.0 * visits + 48.0 * tests + 32.0 * n
+    parity = None
+    try:
+        import oracle
+        w = dw.host
+        idx = np.random.default_rng(6).choice(n, 2000, replace=False)
+        ref = oracle.trace_paths(p, 4, w.groups, w.records, w.entities, albedo, idx)
+        parity = {"paths": int(idx.size), "exact": bool(res[idx].tobytes() == ref.tobytes())}
+    except ImportError:
+        parity = None
+    ms = best[1]
+    out_d = {"workload": f"terrain {2048 // d}^2 + {1400 // d} icospheres ({ntri} triangles), {W}x{H}x{spp} paths, "
+                         "2 diffuse bounces, roulette 0.8, BVH4 layout",
+             "triangles": ntri, "paths": n, "segments": segs, "ms_per_frame": round(ms, 3),
+             "mrays_s": round(segs / ms / 1e3, 1), "mpaths_s": round(n / ms / 1e3, 1), "preset": best[0],
+             "mean_segments": round(segs / n, 4), "bytes_per_frame": nbytes,
+             "achieved_GBs": round(nbytes / (ms * 1e-3) / 1e9, 1),
+             "status_flags": int(np.count_nonzero(res["status"])),
+             "setup_s": {"scene": round(t1 - t0, 1), "bvh": round(t2 - t1, 1), "upload_and_wide": round(t3 - t2, 1)},
+             "parity": parity}
+    log(f"large scene: {out_d['mrays_s']} Mrays/s ({segs} segments in {ms:.2f} ms), parity {parity}")
+    return out_d
+
+
 def copy_probe(crt, args) -> int:
     """Streaming copy of a buffer of known size: measured HBM read+write bandwidth."""
     nbytes = (args.copy_probe // 16) * 16
@@ -264,6 +341,12 @@ def main(argv=None) -> int:
                     help="time one kernel choice only (e.g. 1:0 one ray per lane, 3:1 wave walk preset 1, "
                          "w4l2:3:0 the wave walk preset 0 over the BVH4 layout with 2-triangle leaves)")
     ap.add_argument("--no-wide", action="store_true", help="time the BVH2 kernels only")
+    ap.add_argument("--no-large", action="store_true",
+                    help="skip the large-scene path-tracing workload (15.6M triangles, 3840x2160x4 paths)")
+    ap.add_argument("--large-scale", type=int, default=1, metavar="D",
+                    help="divide the large scene's terrain grid, sphere count and image size by D (tests)")
+    ap.add_argument("--large-only", action="store_true",
+                    help="run only the large-scene path-tracing workload and print its record as one JSON line")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="instead of the workload, time the streaming copy kernel over BYTES (HBM probe and "
                          "counter calibration)")
@@ -282,6 +365,9 @@ def main(argv=None) -> int:
         raise SystemExit("no HIP device visible")
     if args.copy_probe:
         return copy_probe(crt, args)
+    if args.large_only:
+        print(json.dumps(time_large(crt, args)), flush=True)
+        return 0
 
     # --- scene + BVH (CPU constructor) -------------------------------------------------
     t0 = time.perf_counter()
@@ -416,6 +502,15 @@ def main(argv=None) -> int:
     # --- hit attributes (GetData) of the GI hits: one launch, HIP events -----------------------
     attrs = time_attributes(crt, best_scene, h_gi, n_gi, out["gi"]["hits"], stream, K)
 
+    # --- large scene: multi-bounce paths with in-kernel regeneration --------------------------
+    large = None
+    if not args.no_large and not args.only:
+        try:
+            large = time_large(crt, args)
+        except Exception as e:   # reported, never fatal to the headline line
+            large = {"error": f"{type(e).__name__}: {e}"}
+            log(f"large scene failed: {large['error']}")
+
     # --- batched persistent kernel: config-2 batches and 64 heterogeneous scenes ---------------
     batched = None
     c2_batched = None
@@ -516,6 +611,7 @@ def main(argv=None) -> int:
                      "bytes_per_launch": out["gi"]["bytes"], "launch_ms": round(gi_s * 1e3, 4),
                      "bytes_per_ray": round(out["gi"]["bytes"] / n_gi, 1)},
         "attributes": attrs,
+        "large_scene": large,
         "cpu_baseline": cpu,
         "parity": parity,
         "detail": {k: {kk: vv for kk, vv in v.items() if kk != "hits"} for k, v in out.items()},

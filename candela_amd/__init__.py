@@ -8,6 +8,7 @@ Every call goes to the compiled library; there is no Python fallback.
 from __future__ import annotations
 
 import ctypes as C
+import os
 from dataclasses import dataclass
 
 import numpy as np
@@ -19,7 +20,7 @@ from ._lib import (KERNEL_AUTO, KERNEL_PERSISTENT, KERNEL_THREAD_PER_RAY, KERNEL
                    STACKLESS_NODE_DTYPE, STATUS_BAD_INDEX, STATUS_ITER_CAP, STATUS_STACK_OVERFLOW,
                    STATUS_VISITS_MASK, SWAP_NONE, SWAP_SEEDED, TRIANGLE_DTYPE, VERTEX_DTYPE, BuildOptions,
                    BvhStats, CrtError, Mesh, SceneView, TraceConfig, Batch, BatchConfig, ShadeParams,
-                   SHADE_RECORD_DTYPE, check, lib, node_dtype, ptr)
+                   SHADE_RECORD_DTYPE, CacheHeader, E_CORRUPT, E_IO, E_STALE, check, lib, node_dtype, ptr)
 
 __all__ = [
     "VERTEX_DTYPE", "TRIANGLE_DTYPE", "STACKLESS_NODE_DTYPE", "STACK_NODE_DTYPE", "BOUNDS_DTYPE", "ENTITY_DTYPE",
@@ -31,7 +32,8 @@ __all__ = [
     "autotune_batched", "ShadeParams", "SHADE_RECORD_DTYPE", "ShadingModel", "FORMAT_WIDE4", "FORMAT_WIDE8",
     "WIDE_GROUP_DTYPE", "TRI_RECORD_DTYPE", "WideBVH", "wide_build", "DeviceWide", "upload_wide",
     "TEXTURE_REF_DTYPE", "TEXTURE_DTYPE", "HIT_ATTR_DTYPE", "MaterialTable", "hit_attributes",
-    "PATH_PARAMS_DTYPE", "PATH_SAMPLE_DTYPE", "path_params", "trace_paths",
+    "PATH_PARAMS_DTYPE", "PATH_SAMPLE_DTYPE", "path_params", "trace_paths", "hash64", "bvh_key", "load_bvh",
+    "CacheHeader", "E_IO", "E_CORRUPT", "E_STALE",
 ]
 
 
@@ -133,29 +135,85 @@ class BuiltBVH:
     stats: dict
 
 
+def _bvh_copies(h, fmt: int) -> BuiltBVH:
+    st = BvhStats()
+    check(lib().crt_bvh_get_stats(h, C.byref(st)))
+    n_nodes, n_tri, n_vtx = int(st.array_length), int(st.triangle_count), int(st.vertex_count)
+    nd = node_dtype(fmt)
+    nodes = np.frombuffer((C.c_char * (n_nodes * nd.itemsize)).from_address(lib().crt_bvh_nodes(h)),
+                          dtype=nd).copy() if n_nodes else np.zeros(0, nd)
+    tris = np.frombuffer((C.c_char * (n_tri * 16)).from_address(lib().crt_bvh_triangles(h)),
+                         dtype=TRIANGLE_DTYPE).copy() if n_tri else np.zeros(0, TRIANGLE_DTYPE)
+    verts = np.frombuffer((C.c_char * (n_vtx * 32)).from_address(lib().crt_bvh_vertices(h)),
+                          dtype=VERTEX_DTYPE).copy() if n_vtx else np.zeros(0, VERTEX_DTYPE)
+    stats = {k: int(getattr(st, k)) for k, _ in BvhStats._fields_}
+    return BuiltBVH(fmt, nodes, tris, verts, stats)
+
+
+def hash64(data, seed: int = 0) -> int:
+    """crt_hash64: XXH64 of a bytes-like object or contiguous array."""
+    b = np.frombuffer(memoryview(data).cast("B"), np.uint8) if not isinstance(data, np.ndarray) else \
+        np.ascontiguousarray(data).view(np.uint8).reshape(-1)
+    return int(lib().crt_hash64(ptr(b) if b.size else None, b.size, seed & 0xFFFFFFFFFFFFFFFF))
+
+
+def bvh_key(fmt: int, meshes, child_swap: int = SWAP_NONE, seed: int = 0) -> int:
+    """crt_bvh_key: the cache key of a build's inputs."""
+    arr, keep = _meshes(meshes)
+    opts = BuildOptions(child_swap, seed & 0xFFFFFFFF, 0)
+    k = C.c_uint64()
+    check(lib().crt_bvh_key(fmt, arr, len(meshes), C.byref(opts), C.byref(k)))
+    del keep
+    return int(k.value)
+
+
+def _header_dict(hd: CacheHeader) -> dict:
+    out = {}
+    for name, _ in CacheHeader._fields_:
+        v = getattr(hd, name)
+        if name in ("magic", "toolchain"):
+            v = bytes(v).decode()
+        elif name.startswith("section_"):
+            v = [int(x) for x in v]
+        elif name.startswith("reserved"):
+            continue
+        out[name] = v
+    return out
+
+
+def load_bvh(path: str, expect_key: int = 0, t_offset: int | None = None) -> tuple[BuiltBVH, dict]:
+    """crt_bvh_load (+ crt_bvh_rebase when t_offset is given): host copies and the header."""
+    h = C.c_void_p()
+    hd = CacheHeader()
+    check(lib().crt_bvh_load(os.fsencode(path), expect_key & 0xFFFFFFFFFFFFFFFF, C.byref(h), C.byref(hd)))
+    try:
+        if t_offset is not None:
+            check(lib().crt_bvh_rebase(h, int(t_offset)))
+        return _bvh_copies(h, int(hd.format)), _header_dict(hd)
+    finally:
+        lib().crt_bvh_free(h)
+
+
 def build_bvh(fmt: int, meshes, t_offset: int = 0, child_swap: int = SWAP_NONE, seed: int = 0,
-              threads: int = 0) -> BuiltBVH:
-    """BuildBVH (BVHConstructor.cpp:951-1108) on the CPU; returns host copies of the flattened buffers."""
+              threads: int = 0, save_to: str | None = None, rebase_to: int | None = None) -> BuiltBVH:
+    """BuildBVH (BVHConstructor.cpp:951-1108) on the CPU; returns host copies of the flattened buffers.
+    save_to: also write a cache file (crt_bvh_save with the inputs' crt_bvh_key), before any rebase.
+    rebase_to: move the leaf starts to this t_offset (crt_bvh_rebase) before copying."""
     arr, keep = _meshes(meshes)
     opts = BuildOptions(child_swap, seed & 0xFFFFFFFF, threads)
     h = C.c_void_p()
     check(lib().crt_build_bvh(fmt, arr, len(meshes), int(t_offset), C.byref(opts), C.byref(h)))
     try:
-        st = BvhStats()
-        check(lib().crt_bvh_get_stats(h, C.byref(st)))
-        n_nodes, n_tri, n_vtx = int(st.array_length), int(st.triangle_count), int(st.vertex_count)
-        nd = node_dtype(fmt)
-        nodes = np.frombuffer((C.c_char * (n_nodes * nd.itemsize)).from_address(lib().crt_bvh_nodes(h)),
-                              dtype=nd).copy()
-        tris = np.frombuffer((C.c_char * (n_tri * 16)).from_address(lib().crt_bvh_triangles(h)),
-                             dtype=TRIANGLE_DTYPE).copy() if n_tri else np.zeros(0, TRIANGLE_DTYPE)
-        verts = np.frombuffer((C.c_char * (n_vtx * 32)).from_address(lib().crt_bvh_vertices(h)),
-                              dtype=VERTEX_DTYPE).copy() if n_vtx else np.zeros(0, VERTEX_DTYPE)
-        stats = {k: int(getattr(st, k)) for k, _ in BvhStats._fields_}
+        if save_to is not None:
+            k = C.c_uint64()
+            check(lib().crt_bvh_key(fmt, arr, len(meshes), C.byref(opts), C.byref(k)))
+            check(lib().crt_bvh_save(h, k.value, os.fsencode(save_to)))
+        if rebase_to is not None:
+            check(lib().crt_bvh_rebase(h, int(rebase_to)))
+        return _bvh_copies(h, fmt)
     finally:
         lib().crt_bvh_free(h)
-    del keep
-    return BuiltBVH(fmt, nodes, tris, verts, stats)
+        del keep
 
 
 @dataclass
@@ -310,11 +368,20 @@ class RayIntersector:
         except Exception:
             pass
 
-    def add_object(self, object_id: int, meshes, child_swap: int = SWAP_NONE, seed: int = 0, threads: int = 0):
+    def add_object(self, object_id: int, meshes, child_swap: int = SWAP_NONE, seed: int = 0, threads: int = 0,
+                   cache_dir: str | None = None) -> int | None:
+        """AddObject.  With cache_dir, crt_add_object_cached: returns 1 (loaded), 0 (built and stored) or -1
+        (built, not stored)."""
         arr, keep = _meshes(meshes)
         opts = BuildOptions(child_swap, seed & 0xFFFFFFFF, threads)
-        check(lib().crt_add_object(self.handle, object_id, arr, len(meshes), C.byref(opts)))
+        if cache_dir is None:
+            check(lib().crt_add_object(self.handle, object_id, arr, len(meshes), C.byref(opts)))
+            return None
+        hit = C.c_int32()
+        check(lib().crt_add_object_cached(self.handle, object_id, arr, len(meshes), C.byref(opts),
+                                          os.fsencode(cache_dir), C.byref(hit)))
         del keep
+        return int(hit.value)
 
     def add_object_prebuilt(self, object_id: int, nodes: np.ndarray, triangles: np.ndarray, vertices: np.ndarray):
         nodes = np.ascontiguousarray(nodes, dtype=node_dtype(self.format))

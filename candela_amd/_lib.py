@@ -29,6 +29,7 @@ STATUS_BAD_INDEX = 1 << 26
 STATUS_VISITS_MASK = 0x00FFFFFF
 SWAP_NONE = 0
 SWAP_SEEDED = 1
+E_INVALID, E_HIP, E_NOMEM, E_STATE, E_NODEVICE, E_IO, E_CORRUPT, E_STALE = -1, -2, -3, -4, -5, -6, -7, -8
 KERNEL_AUTO = 0
 KERNEL_THREAD_PER_RAY = 1
 KERNEL_PERSISTENT = 2
@@ -85,6 +86,17 @@ class BvhStats(C.Structure):
                 ("vertex_count", C.c_uint64)]
 
 
+class CacheHeader(C.Structure):
+    _fields_ = [("magic", C.c_char * 8), ("header_bytes", C.c_uint32), ("version", C.c_uint32),
+                ("format", C.c_int32), ("t_offset", C.c_int32), ("key", C.c_uint64),
+                ("triangle_count", C.c_uint64), ("array_length", C.c_uint64), ("vertex_count", C.c_uint64),
+                ("node_count", C.c_uint64), ("leaf_count", C.c_uint64), ("split_fails", C.c_uint64),
+                ("max_depth", C.c_uint32), ("child_swap", C.c_int32), ("seed", C.c_uint32), ("bins", C.c_uint32),
+                ("max_leaf", C.c_uint32), ("reserved0", C.c_uint32), ("section_offset", C.c_uint64 * 3),
+                ("section_bytes", C.c_uint64 * 3), ("section_hash", C.c_uint64 * 3), ("toolchain", C.c_char * 32),
+                ("reserved1", C.c_uint8 * 40), ("header_hash", C.c_uint64)]
+
+
 class SceneView(C.Structure):
     _fields_ = [("format", C.c_int32), ("entity_count", C.c_int32), ("total_nodes", C.c_int64),
                 ("nodes", C.c_void_p), ("triangles", C.c_void_p), ("vertices", C.c_void_p),
@@ -123,7 +135,9 @@ assert SHADE_RECORD_DTYPE.itemsize == 80
 
 
 class CrtError(RuntimeError):
-    pass
+    def __init__(self, msg: str, code: int = 0):
+        super().__init__(msg)
+        self.code = code
 
 
 _lib = None
@@ -140,6 +154,14 @@ SIGNATURES = {
     "crt_bvh_triangles": (C.c_void_p, [C.c_void_p]),
     "crt_bvh_vertices": (C.c_void_p, [C.c_void_p]),
     "crt_bvh_format": (C.c_int, [C.c_void_p]),
+    "crt_bvh_t_offset": (C.c_int32, [C.c_void_p]),
+    "crt_hash64": (C.c_uint64, [C.c_void_p, C.c_size_t, C.c_uint64]),
+    "crt_bvh_key": (C.c_int, [C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.POINTER(C.c_uint64)]),
+    "crt_bvh_save": (C.c_int, [C.c_void_p, C.c_uint64, C.c_char_p]),
+    "crt_bvh_load": (C.c_int, [C.c_char_p, C.c_uint64, C.POINTER(C.c_void_p), C.POINTER(CacheHeader)]),
+    "crt_bvh_rebase": (C.c_int, [C.c_void_p, C.c_int32]),
+    "crt_add_object_cached": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int, C.c_void_p, C.c_char_p,
+                                        C.POINTER(C.c_int32)]),
     "crt_device_alloc": (C.c_int, [C.POINTER(C.c_void_p), C.c_size_t]),
     "crt_device_free": (C.c_int, [C.c_void_p]),
     "crt_memcpy_h2d": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
@@ -218,7 +240,7 @@ def lib() -> C.CDLL:
 def check(rc: int) -> None:
     if rc != CRT_OK:
         msg = lib().crt_last_error()
-        raise CrtError(f"libcandela_rt error {rc}: {msg.decode() if msg else ''}")
+        raise CrtError(f"libcandela_rt error {rc}: {msg.decode() if msg else ''}", rc)
 
 
 def ptr(a: np.ndarray) -> int:

@@ -52,8 +52,6 @@ namespace {
 
 constexpr float kArbitraryMax = 10000000.0f;   // BVHConstructor.h:20
 constexpr float kArbitraryMin = -10000000.0f;  // BVHConstructor.h:21
-constexpr int kBinCount = 64;                  // BVHConstructor.cpp:45
-constexpr uint32_t kMaxLeafTris = 2;           // BVHConstructor.cpp:49
 constexpr float kInfCost = 1e29f;              // BVHConstructor.cpp:56
 
 // glm::min / glm::max (func_common.inl:15-28)
@@ -664,9 +662,6 @@ int build_bvh(int format, const crt_mesh* meshes, int mesh_count, int32_t t_offs
 // --------------------------------------------------------------------------
 // C ABI
 // --------------------------------------------------------------------------
-struct crt_bvh {
-    crt::BvhResult r;
-};
 
 extern "C" {
 
@@ -681,6 +676,8 @@ int crt_build_bvh(int format, const crt_mesh* meshes, int mesh_count, int32_t t_
         delete b;
         return rc;
     }
+    b->t_offset = t_offset;
+    if (options) b->options = *options;
     *out = b;
     return CRT_OK;
 }
@@ -701,5 +698,6 @@ const void* crt_bvh_nodes(const crt_bvh* bvh) {
 const crt_triangle* crt_bvh_triangles(const crt_bvh* bvh) { return bvh ? bvh->r.triangles.data() : nullptr; }
 const crt_vertex* crt_bvh_vertices(const crt_bvh* bvh) { return bvh ? bvh->r.vertices.data() : nullptr; }
 int crt_bvh_format(const crt_bvh* bvh) { return bvh ? bvh->r.format : CRT_E_INVALID; }
+int32_t crt_bvh_t_offset(const crt_bvh* bvh) { return bvh ? bvh->t_offset : 0; }
 
 }  // extern "C"

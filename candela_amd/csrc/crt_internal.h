@@ -7,6 +7,8 @@
 
 #include "candela_rt.h"
 
+struct crt_bvh;
+
 namespace crt {
 
 // Records `msg` as the calling thread's last error and returns `code`.
@@ -24,8 +26,27 @@ struct BvhResult {
 int build_bvh(int format, const crt_mesh* meshes, int mesh_count, int32_t t_offset,
               const crt_build_options* options, BvhResult& res);
 
+// Binned-SAH constants of the constructor (BVHConstructor.cpp:45,49).
+constexpr int kBinCount = 64;
+constexpr uint32_t kMaxLeafTris = 2;
+
+// BVH cache files (scene_cache.cpp).
+int bvh_key(int format, const crt_mesh* meshes, int mesh_count, const crt_build_options* options, uint64_t* key);
+int save_bvh(const crt_bvh& b, uint64_t key, const std::string& path);
+int load_bvh(const std::string& path, uint64_t expect_key, crt_bvh& b, crt_cache_header* header_out);
+int rebase_bvh(crt_bvh& b, int32_t t_offset);
+// True when the header's toolchain is this library's (CRT_SWAP_SEEDED draws match).
+bool same_toolchain(const crt_cache_header& hd);
+
 // A zeroed-by-the-launch device counter from the per-device pool of the
 // persistent kernels (one slot per launch, handed out round robin).
 int counter_slot(unsigned long long** out);
 
 }  // namespace crt
+
+// A built object: its flattened buffers and what they were built with.
+struct crt_bvh {
+    crt::BvhResult r;
+    int32_t t_offset = 0;
+    crt_build_options options{CRT_SWAP_NONE, 0u, 0};
+};

@@ -4,6 +4,7 @@
 
 #include <atomic>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <mutex>
 #include <map>
@@ -476,6 +477,47 @@ int crt_add_object_prebuilt(crt_context* ctx, int32_t object_id, const void* nod
         (vertex_count && !vertices))
         return crt::set_error(CRT_E_INVALID, "crt_add_object_prebuilt: bad buffers");
     return append_object(ctx, object_id, nodes, node_count, triangles, triangle_count, vertices, vertex_count);
+}
+
+int crt_add_object_cached(crt_context* ctx, int32_t object_id, const crt_mesh* meshes, int mesh_count,
+                          const crt_build_options* options, const char* cache_dir, int32_t* cache_hit) {
+    if (cache_hit) *cache_hit = 0;
+    if (!ctx || !cache_dir) return crt::set_error(CRT_E_INVALID, "crt_add_object_cached: null argument");
+    uint64_t key = 0;
+    int rc = crt::bvh_key(ctx->format, meshes, mesh_count, options, &key);
+    if (rc) return rc;
+    char name[32];
+    std::snprintf(name, sizeof(name), "%016llx.crtbvh", static_cast<unsigned long long>(key));
+    std::string path(cache_dir);
+    if (!path.empty() && path.back() != '/') path += '/';
+    path += name;
+    const int32_t t_offset = int32_t(ctx->triangles.size());
+    const bool seeded = options && options->child_swap == CRT_SWAP_SEEDED;
+
+    crt_bvh b;
+    crt_cache_header hd;
+    bool hit = crt::load_bvh(path, key, b, &hd) == CRT_OK && hd.format == ctx->format &&
+               (!seeded || crt::same_toolchain(hd));
+    int32_t status = 1;
+    if (!hit) {
+        b = crt_bvh();
+        rc = crt::build_bvh(ctx->format, meshes, mesh_count, 0, options, b.r);
+        if (rc) return rc;
+        b.t_offset = 0;
+        if (options) b.options = *options;
+        status = crt::save_bvh(b, key, path) == CRT_OK ? 0 : -1;
+    }
+    std::string save_error = status < 0 ? crt_last_error() : std::string();
+    rc = crt::rebase_bvh(b, t_offset);
+    if (rc) return rc;
+    const void* nodes = ctx->format == CRT_FORMAT_STACKLESS ? static_cast<const void*>(b.r.stackless.data())
+                                                            : static_cast<const void*>(b.r.stack.data());
+    rc = append_object(ctx, object_id, nodes, int64_t(b.r.stats.array_length), b.r.triangles.data(),
+                       int64_t(b.r.triangles.size()), b.r.vertices.data(), int64_t(b.r.vertices.size()));
+    if (rc) return rc;
+    if (status < 0) crt::set_error(CRT_E_IO, save_error);
+    if (cache_hit) *cache_hit = status;
+    return CRT_OK;
 }
 
 int crt_push_entity(crt_context* ctx, int32_t object_id, const float model[16], float emissive, float translucency) {

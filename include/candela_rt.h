@@ -41,7 +41,10 @@ enum {
     CRT_E_HIP = -2,       /* a HIP runtime call failed (message names it)         */
     CRT_E_NOMEM = -3,     /* host or device allocation failed                     */
     CRT_E_STATE = -4,     /* call out of order (e.g. trace before buffer_data)    */
-    CRT_E_NODEVICE = -5   /* no HIP device visible                                */
+    CRT_E_NODEVICE = -5,  /* no HIP device visible                                */
+    CRT_E_IO = -6,        /* a file could not be opened, read or written          */
+    CRT_E_CORRUPT = -7,   /* a cache file failed its magic, size or hash checks   */
+    CRT_E_STALE = -8      /* a cache file was built from other inputs             */
 };
 
 /* ------------------------------------------------------------------------- */
@@ -205,6 +208,69 @@ const void* crt_bvh_nodes(const crt_bvh* bvh);
 const crt_triangle* crt_bvh_triangles(const crt_bvh* bvh);
 const crt_vertex* crt_bvh_vertices(const crt_bvh* bvh);
 int crt_bvh_format(const crt_bvh* bvh);
+/* The t_offset the buffers were built (or rebased) with. */
+int32_t crt_bvh_t_offset(const crt_bvh* bvh);
+
+/* ------------------------------------------------------------------------- */
+/* BVH cache files.  No reference counterpart: AddObject rebuilds every      */
+/* object at every launch (Intersector.h:169-198 -> BuildBVH,                */
+/* BVHConstructor.cpp:951-1108).  A cache file holds one object's flattened  */
+/* buffers, byte for byte as BuildBVH leaves them, behind this header.       */
+/* ------------------------------------------------------------------------- */
+
+/* File layout (little-endian): the 256-byte header, then the node, triangle
+ * and vertex sections, each at a 64-byte aligned offset.  Every hash is XXH64
+ * (seed 0); header_hash covers bytes [0, 248) of the header. */
+#define CRT_CACHE_MAGIC "CRTBVH1"
+#define CRT_CACHE_VERSION 1
+typedef struct crt_cache_header {
+    char magic[8];               /* CRT_CACHE_MAGIC, NUL-terminated        */
+    uint32_t header_bytes;       /* 256                                    */
+    uint32_t version;            /* CRT_CACHE_VERSION                      */
+    int32_t format;              /* CRT_FORMAT_STACKLESS or _STACK         */
+    int32_t t_offset;            /* leaf starts are relative to this       */
+    uint64_t key;                /* crt_bvh_key of the build's inputs      */
+    uint64_t triangle_count;     /* N                                      */
+    uint64_t array_length;       /* node records (2L - 1)                  */
+    uint64_t vertex_count;       /* V                                      */
+    uint64_t node_count;         /* crt_bvh_stats.node_count               */
+    uint64_t leaf_count;         /* L                                      */
+    uint64_t split_fails;
+    uint32_t max_depth;
+    int32_t child_swap;          /* crt_build_options.child_swap           */
+    uint32_t seed;               /* crt_build_options.seed                 */
+    uint32_t bins;               /* binned-SAH bin count (64)              */
+    uint32_t max_leaf;           /* triangles per leaf (2)                 */
+    uint32_t reserved0;
+    uint64_t section_offset[3];  /* nodes, triangles, vertices             */
+    uint64_t section_bytes[3];
+    uint64_t section_hash[3];
+    char toolchain[32];          /* C++ library of the build ("libstdc++ 20220324"):
+                                    it fixes mt19937's uniform_int draws for
+                                    CRT_SWAP_SEEDED                         */
+    uint8_t reserved1[40];
+    uint64_t header_hash;
+} crt_cache_header;
+
+/* XXH64 of `bytes` bytes (Y. Collet's published xxHash64). */
+uint64_t crt_hash64(const void* data, size_t bytes, uint64_t seed);
+/* The cache key of a build: XXH64 over "crt-bvh-key/1", then format,
+ * child_swap, seed (0 unless CRT_SWAP_SEEDED), mesh_count as int32, then per
+ * mesh its vertex_count, index_count and global_mesh_number as 32-bit ints,
+ * its vertex records and its indices.  t_offset and threads do not enter it:
+ * they do not change the tree. */
+int crt_bvh_key(int format, const crt_mesh* meshes, int mesh_count, const crt_build_options* options,
+                uint64_t* key);
+/* Writes `bvh` to `path` (a temporary file renamed over it, so readers never
+ * see a partial file).  `key` is stored in the header. */
+int crt_bvh_save(const crt_bvh* bvh, uint64_t key, const char* path);
+/* Reads a cache file.  CRT_E_IO if it cannot be read, CRT_E_CORRUPT if a
+ * check fails, CRT_E_STALE if expect_key != 0 and differs from the stored
+ * key.  `header` (may be NULL) receives the header. */
+int crt_bvh_load(const char* path, uint64_t expect_key, crt_bvh** out, crt_cache_header* header);
+/* Moves every leaf start from the bvh's t_offset to `t_offset`; the result is
+ * byte-identical to a build with that t_offset. */
+int crt_bvh_rebase(crt_bvh* bvh, int32_t t_offset);
 
 /* ------------------------------------------------------------------------- */
 /* Device memory (thin wrappers so callers need no HIP headers)              */
@@ -365,6 +431,16 @@ int crt_add_object_prebuilt(crt_context* ctx, int32_t object_id, const void* nod
                             int64_t node_count, const crt_triangle* triangles,
                             int64_t triangle_count, const crt_vertex* vertices,
                             int64_t vertex_count);
+/* crt_add_object through a cache directory: loads `<cache_dir>/<key>.crtbvh`
+ * (key as 16 hex digits) when it holds this object's build, else builds with
+ * t_offset 0 and stores the file.  Either way the appended buffers are
+ * byte-identical to crt_add_object's.  *cache_hit (may be NULL): 1 loaded,
+ * 0 built and stored, -1 built but the file could not be stored (the object
+ * is still added; crt_last_error() says why).  A corrupt or stale file, or a
+ * CRT_SWAP_SEEDED file written under another C++ library, is rebuilt and
+ * overwritten. */
+int crt_add_object_cached(crt_context* ctx, int32_t object_id, const crt_mesh* meshes, int mesh_count,
+                          const crt_build_options* options, const char* cache_dir, int32_t* cache_hit);
 /* PushEntity (Intersector.h:200-216): model matrix (column-major), emissive,
  * translucency; inverse is computed here. */
 int crt_push_entity(crt_context* ctx, int32_t object_id, const float model[16], float emissive,

@@ -38,6 +38,8 @@ def test_struct_sizes(crt):
     assert crt.ENTITY_DTYPE.itemsize == 192
     assert crt.RAY_DTYPE.itemsize == 32
     assert crt.HIT_DTYPE.itemsize == 32
+    import ctypes
+    assert ctypes.sizeof(crt.CacheHeader) == 256 and crt.CacheHeader.header_hash.offset == 248
 
 
 def test_errors_without_gpu(crt):

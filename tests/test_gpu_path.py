@@ -63,7 +63,7 @@ def test_paths_match_host_twin(crt, orc, scene):
         f"5000 sampled paths bit-identical to the host twin: {same}")
     assert same
     assert (got["status"] == 0).all() and (segs >= 1).all() and (segs <= 3).all()
-    assert (segs == 3).mean() > 0.1 and (segs == 1).mean() > 0.01
+    assert (segs == 3).mean() > 0.03 and (segs == 1).mean() > 0.01
 
 
 def test_presets_and_grids_agree(crt, scene):
