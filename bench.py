@@ -224,6 +224,41 @@ def time_attributes(crt, scene, d_hits, n: int, hits: np.ndarray, stream, steps:
     return res
 
 
+def time_raygen(crt, scene, d_rays, d_hits, rays: np.ndarray, hits: np.ndarray, hb: dict, stream, steps: int) -> dict:
+    """crt_gen_diffuse_rays over the 1920x1080 primary G-buffer: AO + GI rays of every pixel that hit, compacted
+    (count, scan, emit).  Algorithmic bytes: 32 (hit, count pass) + 64 (ray + hit, emit pass) per record, and
+    per hit 16 (triangle) + 3 x 16 (positions) + 2 x 32 (two rays written).  The rays are checked against the
+    host twin."""
+    n = rays.size
+    gen = crt.RayGen(n)
+    ao, gi = crt.DeviceArray(crt.RAY_DTYPE, n), crt.DeviceArray(crt.RAY_DTYPE, n)
+    p = crt.raygen_params(seed=2, offset=1e-4)
+    gen.diffuse(scene, d_rays, d_hits, p, ao, gi, stream=stream)
+    stream.sync()
+    ev = Events(crt, 2)
+    ev.record(0, stream.handle)
+    for _ in range(steps):
+        gen.diffuse(scene, d_rays, d_hits, p, ao, gi, stream=stream)
+    ev.record(1, stream.handle)
+    ms = ev.ms(0, 1) / steps
+    ev.close()
+    k = gen.last_count(stream)
+    nbytes = 96.0 * n + 128.0 * k
+    exact = None
+    try:
+        import oracle
+        oao, ogi = oracle.gen_diffuse_rays(p, rays, hits, hb["entities"], tris=hb["triangles"], verts=hb["vertices"])
+        exact = bool(k == oao.size and ao.download()[:k].tobytes() == oao.tobytes()
+                     and gi.download()[:k].tobytes() == ogi.tobytes())
+    except ImportError:
+        pass
+    res = {"records": int(n), "rays": 2 * k, "ms": round(ms, 4), "mrays_s": round(2 * k / ms / 1e3, 1),
+           "bytes_per_launch": nbytes, "achieved_GBs": round(nbytes / (ms * 1e-3) / 1e9, 1), "exact": exact}
+    log(f"raygen: {2 * k} AO + GI rays from {n} primary hits in {ms:.4f} ms -> {res['mrays_s']} Mrays/s, "
+        f"{res['achieved_GBs']} GB/s algorithmic, exact {exact}")
+    return res
+
+
 def time_large(crt, args) -> dict:
     """The large-scene workload: a 2048^2 fBm terrain (8,388,608 triangles) plus 1,400 subdivision-4
     icospheres (7,168,000), 3840 x 2160 pixels x 4 samples, 2 diffuse bounces, roulette p = 0.8 from the
@@ -502,6 +537,9 @@ def main(argv=None) -> int:
     # --- hit attributes (GetData) of the GI hits: one launch, HIP events -----------------------
     attrs = time_attributes(crt, best_scene, h_gi, n_gi, out["gi"]["hits"], stream, K)
 
+    # --- wavefront ray generation from the primary G-buffer: one launch chain, HIP events -------
+    raygen = time_raygen(crt, scene, drays, dhits, prays, phits, hb, stream, K)
+
     # --- large scene: multi-bounce paths with in-kernel regeneration --------------------------
     large = None
     if not args.no_large and not args.only:
@@ -611,6 +649,7 @@ def main(argv=None) -> int:
                      "bytes_per_launch": out["gi"]["bytes"], "launch_ms": round(gi_s * 1e3, 4),
                      "bytes_per_ray": round(out["gi"]["bytes"] / n_gi, 1)},
         "attributes": attrs,
+        "raygen": raygen,
         "large_scene": large,
         "cpu_baseline": cpu,
         "parity": parity,

@@ -20,7 +20,7 @@ from ._lib import (KERNEL_AUTO, KERNEL_PERSISTENT, KERNEL_THREAD_PER_RAY, KERNEL
                    STACKLESS_NODE_DTYPE, STATUS_BAD_INDEX, STATUS_ITER_CAP, STATUS_STACK_OVERFLOW,
                    STATUS_VISITS_MASK, SWAP_NONE, SWAP_SEEDED, TRIANGLE_DTYPE, VERTEX_DTYPE, BuildOptions,
                    BvhStats, CrtError, Mesh, SceneView, TraceConfig, Batch, BatchConfig, ShadeParams,
-                   SHADE_RECORD_DTYPE, CacheHeader, E_CORRUPT, E_IO, E_STALE, check, lib, node_dtype, ptr)
+                   SHADE_RECORD_DTYPE, RAYGEN_PARAMS_DTYPE, PROBE_GRID_DTYPE, CacheHeader, E_CORRUPT, E_IO, E_STALE, check, lib, node_dtype, ptr)
 
 __all__ = [
     "VERTEX_DTYPE", "TRIANGLE_DTYPE", "STACKLESS_NODE_DTYPE", "STACK_NODE_DTYPE", "BOUNDS_DTYPE", "ENTITY_DTYPE",
@@ -33,7 +33,8 @@ __all__ = [
     "WIDE_GROUP_DTYPE", "TRI_RECORD_DTYPE", "WideBVH", "wide_build", "DeviceWide", "upload_wide",
     "TEXTURE_REF_DTYPE", "TEXTURE_DTYPE", "HIT_ATTR_DTYPE", "MaterialTable", "hit_attributes",
     "PATH_PARAMS_DTYPE", "PATH_SAMPLE_DTYPE", "path_params", "trace_paths", "hash64", "bvh_key", "load_bvh",
-    "CacheHeader", "E_IO", "E_CORRUPT", "E_STALE",
+    "CacheHeader", "E_IO", "E_CORRUPT", "E_STALE", "RAYGEN_PARAMS_DTYPE", "PROBE_GRID_DTYPE", "raygen_params",
+    "RayGen", "probe_grid", "gen_probe_rays",
 ]
 
 
@@ -304,6 +305,76 @@ def trace_paths(scene: SceneView, params: np.ndarray, out: DeviceArray, npaths: 
         raise ValueError("output smaller than npaths")
     cfg = TraceConfig(0, 0, 0, variant, grid, 0, None)
     check(lib().crt_trace_paths(C.byref(scene), ptr(params), out.ptr, npaths, C.byref(cfg), _stream_handle(stream)))
+
+
+def raygen_params(seed: int = 2, offset: float = 1e-4, ao_tmax: float = 2.0, gi_tmax: float = 1e6,
+                  spec_offset=(0.05, 0.1), spec_tmax: float = 1e6, roughness_scale: float = 0.9,
+                  roughness=None) -> np.ndarray:
+    """A crt_raygen_params record; `roughness` is a device array of one float per mesh id (or None)."""
+    p = np.zeros(1, RAYGEN_PARAMS_DTYPE)
+    p["seed"], p["offset"], p["ao_tmax"], p["gi_tmax"] = seed & 0xFFFFFFFF, offset, ao_tmax, gi_tmax
+    p["spec_offset"], p["spec_tmax"], p["roughness_scale"] = spec_offset, spec_tmax, roughness_scale
+    if roughness is not None:
+        p["roughness"], p["nmesh"] = roughness.ptr, roughness.count
+    return p
+
+
+class RayGen:
+    """Wavefront ray generation (crt_gen_diffuse_rays / crt_gen_specular_rays) with its workspace and
+    device ray counter, for inputs of up to `capacity` records."""
+
+    def __init__(self, capacity: int):
+        self.capacity = int(capacity)
+        self.workspace = DeviceArray(np.uint8, int(lib().crt_raygen_workspace_bytes(self.capacity)))
+        self.count = DeviceArray(np.int64, 1)
+
+    def _n(self, hits, n):
+        n = hits.count if n is None else int(n)
+        if n > self.capacity:
+            raise ValueError(f"{n} records exceed the generator's capacity {self.capacity}")
+        return n
+
+    def diffuse(self, scene: SceneView, primary: DeviceArray, hits: DeviceArray, params: np.ndarray,
+                ao_out: DeviceArray | None, gi_out: DeviceArray | None, n: int | None = None, stream=None) -> None:
+        """AO and GI rays of the hit records, compacted in input order (asynchronous; see last_count)."""
+        n = self._n(hits, n)
+        for o in (ao_out, gi_out):
+            if o is not None and o.count < n:
+                raise ValueError("output smaller than the input")
+        params = np.ascontiguousarray(params, RAYGEN_PARAMS_DTYPE).reshape(1)
+        check(lib().crt_gen_diffuse_rays(C.byref(scene), primary.ptr, hits.ptr, n, ptr(params),
+                                         ao_out.ptr if ao_out is not None else None,
+                                         gi_out.ptr if gi_out is not None else None, self.count.ptr,
+                                         self.workspace.ptr, self.workspace.nbytes, _stream_handle(stream)))
+
+    def specular(self, scene: SceneView, primary: DeviceArray, hits: DeviceArray, params: np.ndarray,
+                 out: DeviceArray, n: int | None = None, stream=None) -> None:
+        """Reflection rays of the hit records, compacted in input order (asynchronous; see last_count)."""
+        n = self._n(hits, n)
+        if out.count < n:
+            raise ValueError("output smaller than the input")
+        params = np.ascontiguousarray(params, RAYGEN_PARAMS_DTYPE).reshape(1)
+        check(lib().crt_gen_specular_rays(C.byref(scene), primary.ptr, hits.ptr, n, ptr(params), out.ptr,
+                                          self.count.ptr, self.workspace.ptr, self.workspace.nbytes,
+                                          _stream_handle(stream)))
+
+    def last_count(self, stream=None) -> int:
+        """Rays written by the last call (waits for it)."""
+        return int(self.count.download(stream)[0])
+
+
+def probe_grid(res, box_origin, size, seed: int = 0, tmax: float = 1e6) -> np.ndarray:
+    g = np.zeros(1, PROBE_GRID_DTYPE)
+    g["res"], g["box_origin"], g["size"], g["seed"], g["tmax"] = res, box_origin, size, seed & 0xFFFFFFFF, tmax
+    return g
+
+
+def gen_probe_rays(grid: np.ndarray, out: DeviceArray, stream=None) -> None:
+    """crt_gen_probe_rays: one ray per probe into `out` (asynchronous)."""
+    grid = np.ascontiguousarray(grid, PROBE_GRID_DTYPE).reshape(1)
+    if out.count < int(np.prod(grid["res"][0].astype(np.int64))):
+        raise ValueError("output smaller than the probe count")
+    check(lib().crt_gen_probe_rays(ptr(grid), out.ptr, _stream_handle(stream)))
 
 
 class DeviceWide:

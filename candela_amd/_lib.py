@@ -64,6 +64,11 @@ assert PATH_PARAMS_DTYPE.itemsize == 184 and PATH_SAMPLE_DTYPE.itemsize == 32
 assert VERTEX_DTYPE.itemsize == 32 and TRIANGLE_DTYPE.itemsize == 16
 assert STACKLESS_NODE_DTYPE.itemsize == 32 and STACK_NODE_DTYPE.itemsize == 64
 assert ENTITY_DTYPE.itemsize == 192 and RAY_DTYPE.itemsize == 32 and HIT_DTYPE.itemsize == 32
+RAYGEN_PARAMS_DTYPE = np.dtype([("seed", "<u4"), ("offset", "<f4"), ("ao_tmax", "<f4"), ("gi_tmax", "<f4"),
+                                ("spec_offset", "<f4", 2), ("spec_tmax", "<f4"), ("roughness_scale", "<f4"),
+                                ("roughness", "<u8"), ("nmesh", "<i4"), ("pad", "<i4")])
+PROBE_GRID_DTYPE = np.dtype([("res", "<i4", 3), ("seed", "<u4"), ("box_origin", "<f4", 3), ("tmax", "<f4"),
+                             ("size", "<f4", 3), ("pad", "<i4")])
 
 
 def node_dtype(fmt: int) -> np.dtype:
@@ -155,6 +160,12 @@ SIGNATURES = {
     "crt_bvh_vertices": (C.c_void_p, [C.c_void_p]),
     "crt_bvh_format": (C.c_int, [C.c_void_p]),
     "crt_bvh_t_offset": (C.c_int32, [C.c_void_p]),
+    "crt_raygen_workspace_bytes": (C.c_size_t, [C.c_int64]),
+    "crt_gen_diffuse_rays": (C.c_int, [C.POINTER(SceneView), C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p,
+                                       C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
+    "crt_gen_specular_rays": (C.c_int, [C.POINTER(SceneView), C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p,
+                                        C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
+    "crt_gen_probe_rays": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "crt_hash64": (C.c_uint64, [C.c_void_p, C.c_size_t, C.c_uint64]),
     "crt_bvh_key": (C.c_int, [C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.POINTER(C.c_uint64)]),
     "crt_bvh_save": (C.c_int, [C.c_void_p, C.c_uint64, C.c_char_p]),

@@ -563,6 +563,69 @@ int crt_trace_paths(const crt_scene_view* scene, const crt_path_params* params, 
                     int64_t npaths, const crt_trace_config* config, void* stream);
 
 /* ------------------------------------------------------------------------- */
+/* Wavefront ray generation for the reference's consumer passes: the rays     */
+/* DiffuseTrace.glsl:441-446, SpecularTrace.glsl:509-513 and                  */
+/* UpdateRadianceProbes.glsl:408-423 build per pixel / probe before calling   */
+/* IntersectRay, produced as dense ray buffers for crt_trace.                 */
+/* ------------------------------------------------------------------------- */
+
+typedef struct crt_raygen_params {
+    uint32_t seed;            /* per-ray hash chain h0 = pcg(id ^ seed * 0x9E3779B9), h(k+1) = pcg(hk);
+                                 uniform k = (hk >> 8) * 2^-24 (replaces the reference's fract(sin) hash2) */
+    float offset;             /* diffuse / AO origin offset along the normal (DiffuseTrace.glsl:445: 0.05) */
+    float ao_tmax;            /* AO any-hit TMax                                                           */
+    float gi_tmax;            /* GI closest-hit TMax                                                       */
+    float spec_offset[2];     /* specular origin offset mix(lo, hi, clamp(roughness * 1.4, 0, 1))
+                                 (SpecularTrace.glsl:512: 0.05, 0.1)                                       */
+    float spec_tmax;          /* specular closest-hit TMax                                                 */
+    float roughness_scale;    /* roughness passed to the sampler = table value * scale (0.9 at :513)       */
+    const float* roughness;   /* device, per mesh id (the G-buffer's PBR.x); NULL = every mesh 1.0         */
+    int32_t nmesh;            /* entries in `roughness`; a mesh id outside gives 1.0                       */
+    int32_t pad;
+} crt_raygen_params;
+
+/* Bytes of device workspace crt_gen_diffuse_rays / crt_gen_specular_rays need
+ * for n input records. */
+size_t crt_raygen_workspace_bytes(int64_t n);
+
+/* For every closest-hit record i with t > 0 (in input order, compacted):
+ * P = org + dir * t, N = the hit triangle's face normal in world space
+ * (inverse-transpose of the entity's model matrix) turned against the ray,
+ * origin = P + N * offset; AO ray (u0, u1) and GI ray (u2, u3) with
+ * CosWeightedHemisphere (Include/Sampling.glsl:1-12) directions, hash key and
+ * output id = primary[i].id.  *d_count (device int64) receives the number of
+ * rays written to each of ao_out / gi_out (either may be NULL).  The scene
+ * view is the one the hits were traced on (BVH2, or wide with tri_records). */
+int crt_gen_diffuse_rays(const crt_scene_view* scene, const crt_ray* primary, const crt_hit* hits, int64_t n,
+                         const crt_raygen_params* params, crt_ray* ao_out, crt_ray* gi_out, int64_t* d_count,
+                         void* workspace, size_t workspace_bytes, void* stream);
+
+/* Reflection rays (StochasticReflectionDirection, SpecularTrace.glsl:102-135):
+ * incident I = normalize(dir), roughness r = table[mesh] * roughness_scale;
+ * r < 0.01 reflects about N, else up to 12 GGX half-vector draws
+ * (SampleGGXVNDF, Include/Sampling.glsl:63-83, with uniforms scaled by
+ * (0.8, 0.7)) until one has N.H > 0.001, then reflect(I, H).  Same
+ * compaction, origin offset and outputs as crt_gen_diffuse_rays. */
+int crt_gen_specular_rays(const crt_scene_view* scene, const crt_ray* primary, const crt_hit* hits, int64_t n,
+                          const crt_raygen_params* params, crt_ray* out, int64_t* d_count, void* workspace,
+                          size_t workspace_bytes, void* stream);
+
+/* One ray per probe of a res[0] x res[1] x res[2] grid
+ * (UpdateRadianceProbes.glsl:408-423): probe (x, y, z), index
+ * x + res[0] * (y + res[1] * z) = the ray's id; origin = box_origin +
+ * (probe / res * 2 - 1) * size; direction uniform on the sphere (LambertBRDF
+ * at :351-362 normalized: azimuth 2 pi u0, cos(theta) = 2 u1 - 1). */
+typedef struct crt_probe_grid {
+    int32_t res[3];
+    uint32_t seed;
+    float box_origin[3];
+    float tmax;
+    float size[3];
+    int32_t pad;
+} crt_probe_grid;
+int crt_gen_probe_rays(const crt_probe_grid* grid, crt_ray* out, void* stream);
+
+/* ------------------------------------------------------------------------- */
 /* Batched traversal: many scenes and ray batches in one persistent launch    */
 /* ------------------------------------------------------------------------- */
 

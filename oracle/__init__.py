@@ -44,6 +44,12 @@ PATH_PARAMS_DTYPE = np.dtype([("inv_view", "<f4", 16), ("inv_proj", "<f4", 16), 
 PATH_SAMPLE_DTYPE = np.dtype([("radiance", "<f4", 3), ("segments", "<i4"), ("visits", "<u4"), ("tests", "<u4"),
                               ("status", "<u4"), ("pad", "<u4")])
 
+RAYGEN_PARAMS_DTYPE = np.dtype([("seed", "<u4"), ("offset", "<f4"), ("ao_tmax", "<f4"), ("gi_tmax", "<f4"),
+                                ("spec_offset", "<f4", 2), ("spec_tmax", "<f4"), ("roughness_scale", "<f4"),
+                                ("roughness", "<u8"), ("nmesh", "<i4"), ("pad", "<i4")])
+PROBE_GRID_DTYPE = np.dtype([("res", "<i4", 3), ("seed", "<u4"), ("box_origin", "<f4", 3), ("tmax", "<f4"),
+                             ("size", "<f4", 3), ("pad", "<i4")])
+
 _lib = None
 
 
@@ -66,6 +72,10 @@ def lib() -> C.CDLL:
         l.oracle_trace.argtypes = [i32, i32, i64, vp, vp, vp, vp, C.c_int, u32, vp, vp, i64, C.c_int, vp, i32]
         l.oracle_trace_paths.restype = None
         l.oracle_trace_paths.argtypes = [vp, i32, i64, vp, vp, vp, i32, vp, vp, i64, vp, i32]
+        l.oracle_gen_rays.restype = i64
+        l.oracle_gen_rays.argtypes = [C.c_int, vp, vp, vp, vp, vp, vp, i64, vp, vp, vp, vp]
+        l.oracle_gen_probe_rays.restype = None
+        l.oracle_gen_probe_rays.argtypes = [vp, vp]
         l.oracle_sincos_turn.restype = None
         l.oracle_sincos_turn.argtypes = [f32, vp, vp]
         l.oracle_hit_attributes.restype = None
@@ -328,4 +338,39 @@ def cook_torrance(x, n, eye, light, intensity, r):
     arrs = [_d(v) for v in (x, n, eye, light)]
     out = np.empty(4, np.float64)
     lib().oracle_cook_torrance(*[_p(v) for v in arrs], float(intensity), float(r), _p(out))
+    return out
+
+
+def _gen(kind, params, primary, hits, ents, records=None, tris=None, verts=None, roughness=None):
+    params = np.ascontiguousarray(params, RAYGEN_PARAMS_DTYPE).reshape(1).copy()
+    primary = np.ascontiguousarray(primary, RAY_DTYPE)
+    hits = np.ascontiguousarray(hits, HIT_DTYPE)
+    ents = np.ascontiguousarray(ents, ENTITY_DTYPE)
+    rough = None if roughness is None else np.ascontiguousarray(roughness, np.float32)
+    params["nmesh"] = 0 if rough is None else rough.size
+    recs = None if records is None else np.ascontiguousarray(records, TRI_RECORD_DTYPE)
+    tr = None if tris is None else np.ascontiguousarray(tris, TRIANGLE_DTYPE)
+    vt = None if verts is None else np.ascontiguousarray(verts, VERTEX_DTYPE)
+    out0 = np.zeros(hits.size, RAY_DTYPE)
+    out1 = np.zeros(hits.size, RAY_DTYPE)
+    k = lib().oracle_gen_rays(kind, _p(recs) if recs is not None else None, _p(tr) if tr is not None else None,
+                              _p(vt) if vt is not None else None, _p(ents), _p(primary), _p(hits), hits.size,
+                              _p(params), _p(rough) if rough is not None else None, _p(out0), _p(out1))
+    return out0[:k], out1[:k]
+
+
+def gen_diffuse_rays(params, primary, hits, ents, records=None, tris=None, verts=None):
+    """(AO rays, GI rays) of the records with t > 0, in input order (raygen.hip's host twin)."""
+    return _gen(0, params, primary, hits, ents, records, tris, verts)
+
+
+def gen_specular_rays(params, primary, hits, ents, records=None, tris=None, verts=None, roughness=None):
+    """Reflection rays of the records with t > 0, in input order."""
+    return _gen(1, params, primary, hits, ents, records, tris, verts, roughness)[0]
+
+
+def gen_probe_rays(grid) -> np.ndarray:
+    grid = np.ascontiguousarray(grid, PROBE_GRID_DTYPE).reshape(1)
+    out = np.zeros(int(np.prod(grid["res"][0].astype(np.int64))), RAY_DTYPE)
+    lib().oracle_gen_probe_rays(_p(grid), _p(out))
     return out
