@@ -80,28 +80,52 @@ def bytes_per_ray(fmt: int, visits: np.ndarray, tests: np.ndarray) -> np.ndarray
     return node * visits.astype(np.float64) + tri * tests.astype(np.float64) + 64.0
 
 
-def kernel_label(c: dict) -> str:
+def _kernel_name(layout: str, kernel: int, variant: int) -> str:
+    if layout == "bvh2":
+        return "%s, preset %d" % (KERNEL_NAMES[kernel], variant)
+    return {3: "wide_grid, preset %d" % variant, 2: "wide_persistent (lane refill), preset %d" % variant}.get(
+        kernel, "trace_kernel")
+
+
+def kernel_label(c: dict, gi_only: bool = False) -> str:
+    name = _kernel_name(c["layout"], c["kernel"], c["variant"])
     if c["layout"] == "bvh2":
-        return "plain (one batch per launch): %s, preset %d, BVH2 stackless" % (KERNEL_NAMES[c["kernel"]], c["variant"])
-    name = {3: "wide_grid, preset %d" % c["variant"], 2: "wide_persistent (lane refill), preset %d" % c["variant"]}.get(
-        c["kernel"], "trace_kernel")
-    return "plain (one batch per launch): %s over the BVH%d layout derived from the BVH2, %d-triangle leaves" % (
-        name, c["width"], c["max_leaf"])
+        label = "plain (one batch per launch): %s, BVH2 stackless" % name
+    else:
+        label = "plain (one batch per launch): %s over the BVH%d layout derived from the BVH2, %d-triangle leaves" % (
+            name, c["width"], c["max_leaf"])
+    if "ao_kernel" in c and not gi_only:
+        label += "; AO launch: " + _kernel_name(c["layout"], c["ao_kernel"], c["ao_variant"])
+    return label
 
 
 def choice_name(c: dict) -> str:
     """'K:V' for a BVH2 kernel choice, 'wWlL:K:V' for kernel K, variant V over the wide layout of width W
-    and leaf size L."""
+    and leaf size L; a choice whose AO launch uses another kernel gets '+ao:K:V'."""
     if c["layout"] == "bvh2":
-        return f"{c['kernel']}:{c['variant']}"
-    return f"w{c['width']}l{c['max_leaf']}:{c['kernel']}:{c['variant']}"
+        name = f"{c['kernel']}:{c['variant']}"
+    else:
+        name = f"w{c['width']}l{c['max_leaf']}:{c['kernel']}:{c['variant']}"
+    if "ao_kernel" in c:
+        name += f"+ao:{c['ao_kernel']}:{c['ao_variant']}"
+    return name
+
+
+def gi_choice_name(c: dict) -> str:
+    """The choice of the GI launch alone."""
+    return choice_name({k: v for k, v in c.items() if not k.startswith("ao_")})
+
+
+def ao_of(c: dict) -> tuple:
+    """(kernel, variant) of the AO launch."""
+    return c.get("ao_kernel", c["kernel"]), c.get("ao_variant", c["variant"])
 
 
 def measured_traffic(best: dict, args, fmt: int):
     """HBM bytes per GI launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs
     (profiles/traffic_*.json) when they were taken on this kernel choice and workload."""
     import glob
-    choice = choice_name(best)
+    choice = gi_choice_name(best)
     for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "traffic_*.json"))):
         with open(path) as f:
             d = json.load(f)
@@ -112,7 +136,7 @@ def measured_traffic(best: dict, args, fmt: int):
 
 def time_batched(crt, batches, steps: int, warmup: int, stream, candidates=None) -> dict:
     """Autotunes the batched kernel on `batches`, then times `steps` hipGraph replays of the best plan."""
-    best, tune_ms = crt.autotune_batched(batches, candidates or default_candidates(crt), repeats=2, stream=stream)
+    best, tune_ms = crt.autotune_batched(batches, candidates or default_candidates(crt), repeats=3, stream=stream)
     plan = crt.BatchPlan(batches, best)
     for _ in range(max(1, warmup)):
         plan.launch(stream=stream, graph=True)
@@ -124,17 +148,19 @@ def time_batched(crt, batches, steps: int, warmup: int, stream, candidates=None)
     ev.record(1, stream.handle)
     ms = ev.ms(0, 1) / steps
     ev.close()
-    cfg = {k: getattr(best, k) for k in ("variant", "tiles_per_claim", "waves_per_group", "order")}
+    cfg = {k: getattr(best, k) for k in ("variant", "tiles_per_claim", "waves_per_group", "order", "shards")}
     return {"ms": ms, "config": cfg, "tune_ms": tune_ms, "rays": plan.rays}
 
 
 def default_candidates(crt):
-    return [crt.BatchConfig(v, t, w, 0, o, 0) for v in (0, 1, 4) for t in (1, 2) for w in (1, 4) for o in (0, 1)]
+    return [crt.BatchConfig(v, t, w, 0, o, q) for v in (0, 1, 4) for t in (1, 2) for w in (1, 4) for o in (0, 1)
+            for q in (1, 16)]
 
 
 def batched_vs_plain(crt, args, plain_choice: dict, stream) -> dict:
     """64 scenes (log-uniform 1k..2M triangles) x (primary, GI, shadow) batches: plain launches vs one launch,
-    over the BVH2 and over the BVH4 / BVH8 layouts derived from it (plain_choice: layout key -> (kernel, variant))."""
+    over the BVH2 and over the BVH4 / BVH8 layouts derived from it (plain_choice: layout key -> (kernel, variant
+    of the closest-hit launches, kernel, variant of the any-hit launches))."""
     from candela_amd import workloads
     t0 = time.perf_counter()
     work = workloads.batched_scenes(crt, count=args.batched_scenes, seed=3)
@@ -142,16 +168,17 @@ def batched_vs_plain(crt, args, plain_choice: dict, stream) -> dict:
         f"(BVH builds {work.build_s:.1f}s, setup {time.perf_counter() - t0:.1f}s)")
     K = args.batched_steps
     per_layout = {}
-    for layout, (kernel, variant) in plain_choice.items():
+    for layout, (kernel, variant, ao_kernel, ao_variant) in plain_choice.items():
         if layout is not None:
             log(f"wide layouts {layout} of all scenes built in {work.build_wide(crt, *layout):.1f}s")
         kw = dict(kernel=kernel, variant=variant, stream=stream)
+        kw_any = dict(kernel=ao_kernel, variant=ao_variant, stream=stream)
 
         def plain_step():
             for s in work.scenes:   # one launch per (scene, query): primary+GI closest, shadow any
                 sc = s.scene_for(layout)
                 crt.trace(sc, crt.QUERY_CLOSEST, s.d_closest, s.h_closest, **kw)
-                crt.trace(sc, crt.QUERY_ANY, s.d_shadow, s.h_shadow, **kw)
+                crt.trace(sc, crt.QUERY_ANY, s.d_shadow, s.h_shadow, **kw_any)
 
         for _ in range(max(1, args.warmup)):
             plain_step()
@@ -169,7 +196,9 @@ def batched_vs_plain(crt, args, plain_choice: dict, stream) -> dict:
                    s.h_shadow.download().tobytes() == ph[1].tobytes() for s, ph in zip(work.scenes, plain_hits))
         name = "bvh2" if layout is None else f"wide{layout[0]}"
         per_layout[name] = {"plain_ms": round(plain_ms, 4), "plain_mrays_s": round(work.rays / plain_ms / 1e3, 2),
-                            "plain_kernel": f"{kernel}:{variant}", "batched_ms": round(b["ms"], 4),
+                            "plain_kernel": f"{kernel}:{variant}" + (f"+any:{ao_kernel}:{ao_variant}" if
+                                                                     (ao_kernel, ao_variant) != (kernel, variant) else ""),
+                            "batched_ms": round(b["ms"], 4),
                             "batched_mrays_s": round(work.rays / b["ms"] / 1e3, 2),
                             "speedup": round(plain_ms / b["ms"], 3), "batched_config": b["config"],
                             "hits_identical": bool(same)}
@@ -292,7 +321,7 @@ def time_large(crt, args) -> dict:
     stream = crt.Stream()
     scene = dw.scene
     best = None
-    for v in range(4):
+    for v in range(6):
         crt.trace_paths(scene, p, out, n, variant=v, stream=stream)   # warm-up
         stream.sync()
         ev = Events(crt, 2)
@@ -455,8 +484,10 @@ def main(argv=None) -> int:
         kernel, variant = c["kernel"], c["variant"]
         scene = scene_of(c)
         kw = dict(block_size=args.block, lds_stack=args.lds_stack, kernel=kernel, variant=variant, stream=stream)
+        ak, av = ao_of(c)
+        kw_ao = dict(kw, kernel=ak, variant=av)
         for _ in range(args.warmup):
-            crt.trace(scene, crt.QUERY_ANY, d_ao, h_ao, **kw)
+            crt.trace(scene, crt.QUERY_ANY, d_ao, h_ao, **kw_ao)
             crt.trace(scene, crt.QUERY_CLOSEST, d_gi, h_gi, **kw)
         stream.sync()
         crt.lib().crt_device_sync()
@@ -465,7 +496,7 @@ def main(argv=None) -> int:
         ev.record(0, sh)
         for k in range(K):
             ev.record(1 + 3 * k, sh)
-            crt.trace(scene, crt.QUERY_ANY, d_ao, h_ao, **kw)
+            crt.trace(scene, crt.QUERY_ANY, d_ao, h_ao, **kw_ao)
             ev.record(2 + 3 * k, sh)
             crt.trace(scene, crt.QUERY_CLOSEST, d_gi, h_gi, **kw)
             ev.record(3 + 3 * k, sh)
@@ -496,7 +527,7 @@ def main(argv=None) -> int:
         if not args.no_wide:
             candidates += [wide(w, 2, crt.KERNEL_THREAD_PER_RAY) for w in (4, 8)]
             candidates += [wide(w, leaf, k, v) for w in (4, 8) for leaf in (2, 4)
-                           for k in (crt.KERNEL_WAVE, crt.KERNEL_PERSISTENT) for v in range(6)]
+                           for k in (crt.KERNEL_WAVE, crt.KERNEL_PERSISTENT) for v in range(10)]
     if args.only:
         parts = args.only.split(":")
         if parts[0].startswith("w"):
@@ -505,6 +536,16 @@ def main(argv=None) -> int:
         else:
             candidates = [bvh2(int(parts[0]), int(parts[1]))]
     results = [run(c) for c in candidates]
+    # The AO and GI launches may use different kernels over the same layout: the fastest GI choice with the
+    # fastest AO choice of its layout, timed as one more candidate.
+    fastest_gi = min(results, key=lambda r: r["gi_ms"])
+    same_layout = [r for r in results if scene_of(r) is scene_of(fastest_gi) and "ao_kernel" not in r]
+    fastest_ao = min(same_layout, key=lambda r: r["ao_ms"])
+    if (fastest_ao["kernel"], fastest_ao["variant"]) != (fastest_gi["kernel"], fastest_gi["variant"]):
+        mixed = {k: v for k, v in fastest_gi.items()
+                 if k in ("layout", "width", "max_leaf", "kernel", "variant")}
+        mixed.update(ao_kernel=fastest_ao["kernel"], ao_variant=fastest_ao["variant"])
+        results.append(run(mixed))
     thread = results[0]
     best = max(results, key=lambda r: r["mrays"])
     best_scene = scene_of(best)
@@ -517,8 +558,9 @@ def main(argv=None) -> int:
     out = {}
     for name, dr, dh, q, n in (("ao", d_ao, h_ao, crt.QUERY_ANY, n_ao), ("gi", d_gi, h_gi, crt.QUERY_CLOSEST, n_gi)):
         cnt = crt.DeviceArray(np.uint32, n)
-        crt.trace(best_scene, q, dr, dh, block_size=args.block, lds_stack=args.lds_stack, kernel=best["kernel"],
-                  variant=best["variant"], tri_tests=cnt)
+        k_, v_ = ao_of(best) if q == crt.QUERY_ANY else (best["kernel"], best["variant"])
+        crt.trace(best_scene, q, dr, dh, block_size=args.block, lds_stack=args.lds_stack, kernel=k_, variant=v_,
+                  tri_tests=cnt)
         crt.lib().crt_device_sync()
         hits = dh.download()
         tests = cnt.download()
@@ -559,11 +601,11 @@ def main(argv=None) -> int:
 
         plain_choice = {}
         b2 = best_of(lambda r: r["layout"] == "bvh2")
-        plain_choice[None] = (b2["kernel"], b2["variant"])
+        plain_choice[None] = (b2["kernel"], b2["variant"], *ao_of(b2))
         for w in (4, 8):
             bw = best_of(lambda r: r["layout"] == f"wide{w}" and r["max_leaf"] == 2)
             if bw is not None:
-                plain_choice[(w, 2)] = (bw["kernel"], bw["variant"])
+                plain_choice[(w, 2)] = (bw["kernel"], bw["variant"], *ao_of(bw))
         c2_batched = time_batched(crt, [(best_scene, crt.QUERY_ANY, d_ao, h_ao),
                                         (best_scene, crt.QUERY_CLOSEST, d_gi, h_gi)],
                                   K, args.warmup, stream)
@@ -643,7 +685,7 @@ def main(argv=None) -> int:
         "batched_mrays_s": (round(rays_step / (c2_batched["ms"] * 1e-3) / 1e6, 2) if c2_batched else None),
         "batched_vs_plain": batched,
         "kernels": [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()} for r in results],
-        "roofline": {"bound": "hbm", "kernel": kernel_label(best) + " (GI closest hit)", "achieved": round(achieved, 1),
+        "roofline": {"bound": "hbm", "kernel": kernel_label(best, gi_only=True) + " (GI closest hit)", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "traffic_source": traffic_src,
                      "bytes_per_launch": out["gi"]["bytes"], "launch_ms": round(gi_s * 1e3, 4),

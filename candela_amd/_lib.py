@@ -125,7 +125,7 @@ class Batch(C.Structure):
 
 class BatchConfig(C.Structure):
     _fields_ = [("variant", C.c_int32), ("tiles_per_claim", C.c_int32), ("waves_per_group", C.c_int32),
-                ("grid", C.c_int32), ("order", C.c_int32), ("reserved", C.c_int32)]
+                ("grid", C.c_int32), ("order", C.c_int32), ("shards", C.c_int32)]
 
 
 class ShadeParams(C.Structure):

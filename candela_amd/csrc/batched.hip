@@ -4,11 +4,13 @@
 // small batches then pays a launch and a draining tail per batch, and a batch
 // of a few thousand rays cannot fill 256 CUs at all.  Here every batch is cut
 // into 64-ray tiles, one tile per wave, and device-filling waves claim tiles
-// from one atomic counter over a tile table that spans all batches (optionally
-// heaviest tiles first, so the launch ends on short tiles).  A wave traces its
+// from a tile table that spans all batches (optionally heaviest tiles first,
+// so the launch ends on short tiles) through the sharded queue of queue.h:
+// shard k holds tiles k, k + 16, ..., so every shard keeps the table's order
+// and no counter sees more than a sixteenth of the claims.  A wave traces its
 // tile with the same wave routine as the plain stackless kernels
 // (wave_walk.h), so every hit record is bit-identical to crt_trace on the same
-// batch.  The launch is a counter-reset kernel plus the batched kernel -- no
+// batch.  The launch is a queue-reset kernel plus the batched kernel -- no
 // memset and no allocation -- so it can be captured once into a hipGraph and
 // replayed.
 #include <hip/hip_runtime.h>
@@ -23,6 +25,7 @@
 
 #include "candela_rt.h"
 #include "crt_internal.h"
+#include "queue.h"
 #include "trace_launch.h"
 #include "wave_walk.h"
 #include "wide_walk.h"
@@ -44,18 +47,18 @@ namespace pdev {
 template <int INNER, int LEAF_BATCH, int K, int WPG>
 __global__ void __launch_bounds__(64 * WPG) batched_stackless(const BatchDesc* __restrict__ descs,
                                                               const int2* __restrict__ tiles, int32_t ntiles,
-                                                              int32_t per_claim, unsigned* __restrict__ counter,
-                                                              int32_t max_iter) {
+                                                              int32_t per_claim, unsigned long long* __restrict__ counters,
+                                                              int32_t max_iter, int32_t shards) {
     WaveWalk<false, INNER, LEAF_BATCH, K> closest;
     WaveWalk<true, INNER, LEAF_BATCH, K> any;
+    ShardQueue q;
+    q.init(counters, ntiles, true, shards);
     while (true) {
-        unsigned base = 0;
-        if (__lane_id() == 0) base = atomicAdd(counter, unsigned(per_claim));
-        base = unsigned(__builtin_amdgcn_readfirstlane(int(__shfl(int(base), 0))));
-        if (base >= unsigned(ntiles)) break;
-        for (int32_t k = 0; k < per_claim; ++k) {
-            const int32_t t = int32_t(base) + k;
-            if (t >= ntiles) break;
+        int32_t got = 0;
+        const int64_t base = q.claim(per_claim, got);
+        if (got == 0) break;
+        for (int32_t k = 0; k < got; ++k) {
+            const int32_t t = __builtin_amdgcn_readfirstlane(int32_t(q.at(base + k)));
             const int2 tile = tiles[t];
             const int32_t b = __builtin_amdgcn_readfirstlane(tile.x);
             const int32_t first = __builtin_amdgcn_readfirstlane(tile.y);
@@ -76,8 +79,8 @@ __global__ void __launch_bounds__(64 * WPG) batched_stackless(const BatchDesc* _
 template <int W, int LDS_DEPTH, int INNER, int LEAF_BATCH, int WPG>
 __global__ void __launch_bounds__(64 * WPG) batched_wide(const BatchDesc* __restrict__ descs,
                                                          const int2* __restrict__ tiles, int32_t ntiles,
-                                                         int32_t per_claim, unsigned* __restrict__ counter,
-                                                         int32_t max_iter) {
+                                                         int32_t per_claim, unsigned long long* __restrict__ counters,
+                                                         int32_t max_iter, int32_t shards) {
     __shared__ int32_t lds[wide::lds_ints<W, LDS_DEPTH>() * 64 * WPG];
     int32_t spill_node[wide::kSpill];
     float spill_t[wide::kSpill];
@@ -86,14 +89,14 @@ __global__ void __launch_bounds__(64 * WPG) batched_wide(const BatchDesc* __rest
     st.stride = 64 * WPG;
     st.spill_node = spill_node;
     st.spill_t = spill_t;
+    ShardQueue q;
+    q.init(counters, ntiles, true, shards);
     while (true) {
-        unsigned base = 0;
-        if (__lane_id() == 0) base = atomicAdd(counter, unsigned(per_claim));
-        base = unsigned(__builtin_amdgcn_readfirstlane(int(__shfl(int(base), 0))));
-        if (base >= unsigned(ntiles)) break;
-        for (int32_t k = 0; k < per_claim; ++k) {
-            const int32_t t = int32_t(base) + k;
-            if (t >= ntiles) break;
+        int32_t got = 0;
+        const int64_t base = q.claim(per_claim, got);
+        if (got == 0) break;
+        for (int32_t k = 0; k < got; ++k) {
+            const int32_t t = __builtin_amdgcn_readfirstlane(int32_t(q.at(base + k)));
             const int2 tile = tiles[t];
             const int32_t b = __builtin_amdgcn_readfirstlane(tile.x);
             const int32_t first = __builtin_amdgcn_readfirstlane(tile.y);
@@ -112,11 +115,9 @@ __global__ void __launch_bounds__(64 * WPG) batched_wide(const BatchDesc* __rest
     }
 }
 
-__global__ void reset_u32(unsigned* c) { *c = 0u; }
-
 }  // namespace pdev
 
-using BatchedFn = void (*)(const BatchDesc*, const int2*, int32_t, int32_t, unsigned*, int32_t);
+using BatchedFn = void (*)(const BatchDesc*, const int2*, int32_t, int32_t, unsigned long long*, int32_t, int32_t);
 
 template <int WPG>
 static const void* batched_symbol(int variant) {
@@ -166,7 +167,7 @@ struct crt_batch_plan {
     crt_batch_config cfg{};
     crt::BatchDesc* d_descs = nullptr;
     int2* d_tiles = nullptr;
-    unsigned* d_counter = nullptr;
+    unsigned long long* d_counters = nullptr;   // one sharded tile queue (queue.h)
     int32_t ntiles = 0;
     int64_t rays = 0;
     int grid = 0;
@@ -179,7 +180,7 @@ struct crt_batch_plan {
         if (graph) (void)hipGraphDestroy(graph);
         if (d_descs) (void)hipFree(d_descs);
         if (d_tiles) (void)hipFree(d_tiles);
-        if (d_counter) (void)hipFree(d_counter);
+        if (d_counters) (void)hipFree(d_counters);
     }
 };
 
@@ -202,6 +203,8 @@ int validate_config(const crt_batch_config& c) {
     if (c.waves_per_group != 1 && c.waves_per_group != 2 && c.waves_per_group != 4)
         return crt::set_error(CRT_E_INVALID, "batch config: waves_per_group must be 1, 2 or 4");
     if (c.grid < 0 || c.order < 0 || c.order > 1) return crt::set_error(CRT_E_INVALID, "batch config: bad grid/order");
+    if (c.shards != 0 && c.shards != 1 && c.shards != crt::kQueueShards)
+        return crt::set_error(CRT_E_INVALID, "batch config: shards must be 0, 1 or 16");
     return CRT_OK;
 }
 
@@ -271,7 +274,7 @@ int build_plan(const crt_batch* batches, int nbatches, const crt_batch_config& c
     p.ntiles = int32_t(tiles.size());
     PLAN_HIP(hipMalloc(&p.d_descs, std::max<size_t>(1, descs.size()) * sizeof(crt::BatchDesc)), "hipMalloc(descs)");
     PLAN_HIP(hipMalloc(&p.d_tiles, std::max<size_t>(1, tiles.size()) * sizeof(int2)), "hipMalloc(tiles)");
-    PLAN_HIP(hipMalloc(&p.d_counter, sizeof(unsigned)), "hipMalloc(counter)");
+    PLAN_HIP(hipMalloc(&p.d_counters, size_t(crt::kQueueWords) * sizeof(unsigned long long)), "hipMalloc(queue)");
     if (!descs.empty())
         PLAN_HIP(hipMemcpy(p.d_descs, descs.data(), descs.size() * sizeof(crt::BatchDesc), hipMemcpyHostToDevice),
                  "hipMemcpy(descs)");
@@ -298,15 +301,16 @@ int build_plan(const crt_batch* batches, int nbatches, const crt_batch_config& c
 
 int launch_plan(crt_batch_plan& p, hipStream_t stream) {
     if (p.ntiles == 0) return CRT_OK;
-    hipLaunchKernelGGL(crt::pdev::reset_u32, dim3(1), dim3(1), 0, stream, p.d_counter);
+    hipLaunchKernelGGL(crt::reset_queue, dim3(1), dim3(64), 0, stream, p.d_counters);
     PLAN_HIP(hipGetLastError(), "batched: reset launch");
     const crt::BatchDesc* descs = p.d_descs;
     const int2* tiles = p.d_tiles;
     int32_t ntiles = p.ntiles;
     int32_t per_claim = p.cfg.tiles_per_claim;
-    unsigned* counter = p.d_counter;
+    unsigned long long* counters = p.d_counters;
     int32_t max_iter = crt::kMaxIterations;
-    void* args[] = {&descs, &tiles, &ntiles, &per_claim, &counter, &max_iter};
+    int32_t shards = p.cfg.shards > 1 ? p.cfg.shards : 1;
+    void* args[] = {&descs, &tiles, &ntiles, &per_claim, &counters, &max_iter, &shards};
     PLAN_HIP(hipLaunchKernel(p.kernel, dim3(unsigned(p.grid)), dim3(unsigned(64 * p.cfg.waves_per_group)), args, 0,
                              stream),
              "batched: kernel launch");
@@ -388,14 +392,16 @@ int crt_autotune_batched(const crt_batch* batches, int nbatches, const crt_batch
         for (int v = 0; v < 6; ++v)
             for (int t : {1, 2, 4})
                 for (int w : {1, 4})
-                    for (int o : {0, 1}) {
-                        crt_batch_config c{};
-                        c.variant = v;
-                        c.tiles_per_claim = t;
-                        c.waves_per_group = w;
-                        c.order = o;
-                        cands.push_back(c);
-                    }
+                    for (int o : {0, 1})
+                        for (int q : {1, crt::kQueueShards}) {
+                            crt_batch_config c{};
+                            c.variant = v;
+                            c.tiles_per_claim = t;
+                            c.waves_per_group = w;
+                            c.order = o;
+                            c.shards = q;
+                            cands.push_back(c);
+                        }
     }
     hipStream_t s = static_cast<hipStream_t>(stream);
     hipEvent_t e0 = nullptr, e1 = nullptr;

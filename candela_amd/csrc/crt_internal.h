@@ -38,6 +38,13 @@ int rebase_bvh(crt_bvh& b, int32_t t_offset);
 // True when the header's toolchain is this library's (CRT_SWAP_SEEDED draws match).
 bool same_toolchain(const crt_cache_header& hd);
 
+// Counters of one sharded work queue (queue.h): kQueueBlockWords u64, zeroed
+// by the launch that uses them; blocks are handed out round robin from a
+// per-device pool of kQueueSlots.
+constexpr int kQueueBlockWords = 256;
+constexpr int kQueueSlots = 64;
+int queue_block(unsigned long long** out);
+
 // A zeroed-by-the-launch device counter from the per-device pool of the
 // persistent kernels (one slot per launch, handed out round robin).
 int counter_slot(unsigned long long** out);

@@ -8,11 +8,11 @@
 // segment's entity loop ends, the lane shades in place -- a miss adds
 // throughput x sky and ends the path, a hit multiplies the throughput by the
 // mesh albedo and, unless the path is done, spawns the next segment -- and
-// keeps stepping.  A lane whose path ended claims the next path index from a
-// counter together with the other idle lanes of its wave (one atomic per
-// wave, ranks by mbcnt) and generates that path's camera ray on the spot, so
-// no ray or hit buffer exists between segments and no wave waits for its
-// longest path.
+// keeps stepping.  A lane whose path ended claims the next path index from
+// the sharded queue (queue.h) together with the other idle lanes of its wave
+// (one atomic per claim, ranks by mbcnt) and generates that path's camera ray
+// on the spot, so no ray or hit buffer exists between segments and no wave
+// waits for its longest path.
 //
 // Per path p (all FP32, no contraction; the host twin in oracle/oracle_path.c
 // follows the same sequence bit for bit):
@@ -243,31 +243,29 @@ struct PathWave {
     }
 
     __device__ __forceinline__ void run(const SceneArgs& s, const Params& P, int64_t npaths, float tmax,
-                                        int32_t max_iter, Stack<LDS_DEPTH> st, unsigned long long* __restrict__ counter,
+                                        int32_t max_iter, Stack<LDS_DEPTH> st, unsigned long long* __restrict__ counters,
                                         int32_t refill, crt_path_sample* __restrict__ out) {
         active = false;
         leafing = false;
         walk_done = false;
-        bool more = true;
+        ShardQueue q;
+        q.init(counters, npaths);
         while (true) {
-            if (more) {
+            if (q.more) {
                 const unsigned long long idle = __ballot(!active);
                 const int32_t cnt = __popcll(idle);
                 if (cnt >= refill) {
-                    unsigned long long base = 0;
-                    if (__lane_id() == 0) base = atomicAdd(counter, (unsigned long long)cnt);
-                    base = (unsigned long long)__shfl((long long)base, 0);
-                    if (base + (unsigned long long)cnt >= (unsigned long long)npaths) more = false;
+                    int32_t got = 0;
+                    const int64_t first = q.claim(cnt, got);
                     if (!active) {
                         const int32_t rank = __builtin_amdgcn_mbcnt_hi(uint32_t(idle >> 32),
                                                                        __builtin_amdgcn_mbcnt_lo(uint32_t(idle), 0));
-                        const unsigned long long idx = base + (unsigned long long)rank;
-                        if (idx < (unsigned long long)npaths) start_path(s, P, int64_t(idx), tmax, out);
+                        if (rank < got) start_path(s, P, q.at(first + rank), tmax, out);
                     }
                 }
             }
             if (!__ballot(active)) {
-                if (!more) break;
+                if (!q.more) break;
                 continue;
             }
             round(s, P, tmax, max_iter, st, out);
@@ -276,10 +274,9 @@ struct PathWave {
 };
 
 template <int W, int LDS_DEPTH, int INNER, int LEAF_BATCH, int REFILL>
-__global__ void __launch_bounds__(256) path_kernel(SceneArgs s, Params P, int64_t npaths, float tmax,
-                                                   int32_t max_iter, unsigned long long* __restrict__ counter,
-                                                   crt_path_sample* __restrict__ out) {
-    __shared__ int32_t lds[wide::lds_ints<W, LDS_DEPTH>() * 256];
+__device__ __forceinline__ void path_body(const SceneArgs& s, const Params& P, int64_t npaths, float tmax,
+                                          int32_t max_iter, unsigned long long* __restrict__ counters,
+                                          crt_path_sample* __restrict__ out, int32_t* lds) {
     int32_t spill_node[wide::kSpill];
     float spill_t[wide::kSpill];
     Stack<LDS_DEPTH> st;
@@ -288,22 +285,49 @@ __global__ void __launch_bounds__(256) path_kernel(SceneArgs s, Params P, int64_
     st.spill_node = spill_node;
     st.spill_t = spill_t;
     PathWave<W, LDS_DEPTH, INNER, LEAF_BATCH> w;
-    w.run(s, P, npaths, tmax, max_iter, st, counter, REFILL, out);
+    w.run(s, P, npaths, tmax, max_iter, st, counters, REFILL, out);
 }
 
-__global__ void reset64(unsigned long long* c) { *c = 0ull; }
+template <int W, int LDS_DEPTH, int INNER, int LEAF_BATCH, int REFILL>
+__global__ void __launch_bounds__(256) path_kernel(SceneArgs s, Params P, int64_t npaths, float tmax,
+                                                   int32_t max_iter, unsigned long long* __restrict__ counters,
+                                                   crt_path_sample* __restrict__ out) {
+    __shared__ int32_t lds[wide::lds_ints<W, LDS_DEPTH>() * 256];
+    path_body<W, LDS_DEPTH, INNER, LEAF_BATCH, REFILL>(s, P, npaths, tmax, max_iter, counters, out, lds);
+}
 
-// Presets (variant, LDS stack entries, INNER, LEAF_BATCH, REFILL).
+// The same kernel compiled for at least OCC waves per SIMD (register budget 512 / OCC).
+template <int W, int LDS_DEPTH, int INNER, int LEAF_BATCH, int REFILL, int OCC>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC, 8)))
+path_kernel_occ(SceneArgs s, Params P, int64_t npaths, float tmax, int32_t max_iter,
+                unsigned long long* __restrict__ counters, crt_path_sample* __restrict__ out) {
+    __shared__ int32_t lds[wide::lds_ints<W, LDS_DEPTH>() * 256];
+    path_body<W, LDS_DEPTH, INNER, LEAF_BATCH, REFILL>(s, P, npaths, tmax, max_iter, counters, out, lds);
+}
+
+template <int W, int D, int I, int LB, int R, int OCC>
+const void* path_kernel_ptr() {
+    if constexpr (OCC == 0) {
+        return reinterpret_cast<const void*>(&path_kernel<W, D, I, LB, R>);
+    } else {
+        return reinterpret_cast<const void*>(&path_kernel_occ<W, D, I, LB, R, OCC>);
+    }
+}
+
+// Presets (variant, LDS stack entries, INNER, LEAF_BATCH, REFILL, OCC waves per
+// SIMD the register budget must allow; 0 = the compiler's choice).
 #define CRT_PATH_PRESETS(X) \
-    X(0, 8, 4, 32, 16)      \
-    X(1, 8, 4, 16, 8)       \
-    X(2, 8, 8, 32, 32)      \
-    X(3, 8, 4, 48, 24)
+    X(0, 8, 4, 32, 16, 0)   \
+    X(1, 8, 4, 16, 8, 0)    \
+    X(2, 8, 8, 32, 32, 0)   \
+    X(3, 8, 4, 48, 24, 0)   \
+    X(4, 8, 4, 32, 16, 4)   \
+    X(5, 8, 4, 48, 24, 4)
 
 template <int W>
 const void* path_fn(int variant) {
-#define CRT_PATH_FN(V, D, I, LB, R) \
-    case V: return reinterpret_cast<const void*>(&path_kernel<W, D, I, LB, R>);
+#define CRT_PATH_FN(V, D, I, LB, R, OCC) \
+    case V: return path_kernel_ptr<W, D, I, LB, R, OCC>();
     switch (variant) {
         CRT_PATH_PRESETS(CRT_PATH_FN)
         default: return nullptr;
@@ -330,7 +354,7 @@ extern "C" int crt_trace_paths(const crt_scene_view* scene, const crt_path_param
     if (npaths > 0 && !out) return crt::set_error(CRT_E_INVALID, "crt_trace_paths: out is null");
     if (npaths >= (int64_t(1) << 32)) return crt::set_error(CRT_E_INVALID, "crt_trace_paths: >= 2^32 paths");
     const int variant = config ? config->variant : 0;
-    if (variant < 0 || variant > 3) return crt::set_error(CRT_E_INVALID, "crt_trace_paths: variant must be 0..3");
+    if (variant < 0 || variant > 5) return crt::set_error(CRT_E_INVALID, "crt_trace_paths: variant must be 0..5");
     if (npaths == 0) return CRT_OK;
     crt::path::Params P;
     std::memcpy(P.cam.inv_view, p.inv_view, sizeof(P.cam.inv_view));
@@ -347,8 +371,8 @@ extern "C" int crt_trace_paths(const crt_scene_view* scene, const crt_path_param
     P.sky[2] = p.sky[2];
     P.albedo = p.albedo;
     P.nmesh = p.nmesh;
-    unsigned long long* counter = nullptr;
-    int rc = crt::counter_slot(&counter);
+    unsigned long long* counters = nullptr;
+    int rc = crt::queue_block(&counters);
     if (rc) return rc;
     const void* fn = scene->format == CRT_FORMAT_WIDE8 ? crt::path::path_fn<8>(variant) : crt::path::path_fn<4>(variant);
     int grid = config && config->grid > 0 ? config->grid : 0;
@@ -362,7 +386,7 @@ extern "C" int crt_trace_paths(const crt_scene_view* scene, const crt_path_param
     const int64_t need = (npaths + 255) / 256;
     if (need < grid) grid = int(need);
     hipStream_t st = static_cast<hipStream_t>(stream);
-    hipLaunchKernelGGL(crt::path::reset64, dim3(1), dim3(1), 0, st, counter);
+    hipLaunchKernelGGL(crt::reset_queue, dim3(1), dim3(64), 0, st, counters);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return crt::set_error(CRT_E_HIP, std::string("crt_trace_paths: reset: ") + hipGetErrorString(e));
     crt::SceneArgs a = crt::scene_args(*scene);
@@ -370,7 +394,7 @@ extern "C" int crt_trace_paths(const crt_scene_view* scene, const crt_path_param
     float tmax = p.tmax;
     int32_t max_iter = crt::kMaxIterations;
     crt_path_sample* o = out;
-    void* args[] = {&a, &P, &n, &tmax, &max_iter, &counter, &o};
+    void* args[] = {&a, &P, &n, &tmax, &max_iter, &counters, &o};
     e = hipLaunchKernel(fn, dim3(unsigned(grid)), dim3(256), args, 0, st);
     if (e != hipSuccess) return crt::set_error(CRT_E_HIP, std::string("crt_trace_paths: launch: ") + hipGetErrorString(e));
     return CRT_OK;

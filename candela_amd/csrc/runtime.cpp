@@ -92,6 +92,26 @@ int counter_slot(unsigned long long** out) {
     return CRT_OK;
 }
 
+static unsigned long long* g_queues[64] = {};
+static std::atomic<uint32_t> g_queue_next{0};
+
+int queue_block(unsigned long long** out) {
+    int dev = 0;
+    CRT_HIP(hipGetDevice(&dev), "hipGetDevice");
+    if (dev < 0 || dev >= 64) return set_error(CRT_E_INVALID, "device index out of range");
+    if (!g_queues[dev]) {
+        std::lock_guard<std::mutex> lock(g_counter_mutex);
+        if (!g_queues[dev]) {
+            void* p = nullptr;
+            CRT_HIP(hipMalloc(&p, size_t(kQueueSlots) * kQueueBlockWords * sizeof(unsigned long long)),
+                    "hipMalloc(queues)");
+            g_queues[dev] = static_cast<unsigned long long*>(p);
+        }
+    }
+    *out = g_queues[dev] + size_t(g_queue_next.fetch_add(1) % kQueueSlots) * kQueueBlockWords;
+    return CRT_OK;
+}
+
 int trace(const crt_scene_view* scene, int query, uint32_t flags, const crt_ray* rays, crt_hit* hits, int64_t n,
           const crt_trace_config* cfg, void* stream) {
     int rc = validate_scene(scene);
@@ -115,8 +135,8 @@ int trace(const crt_scene_view* scene, int query, uint32_t flags, const crt_ray*
     if (is_wide(scene->format) && size_t(2 * depth + 8) * size_t(block) * 4 > 64 * 1024)
         return set_error(CRT_E_INVALID,
                          "trace config: (2 * lds_stack + width) * block_size * 4 exceeds 64 KiB of LDS");
-    if (cfg && (cfg->variant < 0 || cfg->variant > 5 || cfg->grid < 0))
-        return set_error(CRT_E_INVALID, "crt_trace: variant must be 0..5 and grid >= 0");
+    if (cfg && (cfg->variant < 0 || cfg->variant > 9 || cfg->grid < 0))
+        return set_error(CRT_E_INVALID, "crt_trace: variant must be 0..9 and grid >= 0");
     const SceneArgs a = scene_args(*scene);
     uint32_t* tri_tests = cfg ? cfg->tri_tests : nullptr;
     const bool persistent = kernel == CRT_KERNEL_PERSISTENT;
@@ -142,10 +162,10 @@ int trace(const crt_scene_view* scene, int query, uint32_t flags, const crt_ray*
     if (persistent && is_wide(scene->format)) {
         if (n >= (int64_t(1) << 31))
             return set_error(CRT_E_INVALID, "crt_trace: the persistent kernel takes < 2^31 rays per launch");
-        unsigned long long* counter = nullptr;
-        rc = counter_slot(&counter);
+        unsigned long long* counters = nullptr;
+        rc = queue_block(&counters);
         if (rc) return rc;
-        CRT_HIP(launch_wide_persistent(a, query, rays, hits, n, flags, kMaxIterations, counter, tri_tests,
+        CRT_HIP(launch_wide_persistent(a, query, rays, hits, n, flags, kMaxIterations, counters, tri_tests,
                                        cfg ? cfg->variant : 0, cfg ? cfg->grid : 0, static_cast<hipStream_t>(stream)),
                 "crt_trace: wide persistent kernel launch");
         return CRT_OK;

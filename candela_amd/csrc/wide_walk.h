@@ -36,6 +36,7 @@
 
 #include "candela_rt.h"
 #include "device_math.h"
+#include "queue.h"
 #include "trace_launch.h"
 
 namespace crt {
@@ -475,39 +476,37 @@ struct WideWave {
         while (__ballot(active)) round(s, rays, hits, flags, max_iter, tri_tests, st);
     }
 
-    // Rays [0, n) claimed from `counter` by the whole wave: between rounds,
-    // once at least `refill` lanes are idle, one atomicAdd claims a ray for
-    // each idle lane (lane ranks by mbcnt), so a wave keeps its lanes busy
-    // instead of waiting for its longest ray.  Ends when the counter has
-    // passed n and every lane is done.
+    // Rays [0, n) claimed from the sharded queue (queue.h) by the whole wave:
+    // between rounds, once at least `refill` lanes are idle, one claim takes a
+    // ray for each idle lane (lane ranks by mbcnt), so a wave keeps its lanes
+    // busy instead of waiting for its longest ray.  Ends when the queue is dry
+    // and every lane is done.
     __device__ __forceinline__ void run_refill(const SceneArgs& s, const crt_ray* __restrict__ rays,
                                                crt_hit* __restrict__ hits, int64_t n, uint32_t flags,
                                                int32_t max_iter, uint32_t* __restrict__ tri_tests,
-                                               Stack<LDS_DEPTH> st, unsigned long long* __restrict__ counter,
+                                               Stack<LDS_DEPTH> st, unsigned long long* __restrict__ counters,
                                                int32_t refill) {
         active = false;
         leafing = false;
         walk_done = false;
-        bool more = true;   // wave-uniform: the counter may still hold rays
+        ShardQueue q;
+        q.init(counters, n);
         while (true) {
-            if (more) {
+            if (q.more) {
                 const unsigned long long idle = __ballot(!active);
                 const int32_t cnt = __popcll(idle);
                 if (cnt >= refill) {
-                    unsigned long long base = 0;
-                    if (__lane_id() == 0) base = atomicAdd(counter, (unsigned long long)cnt);
-                    base = (unsigned long long)__shfl((long long)base, 0);
-                    if (base + (unsigned long long)cnt >= (unsigned long long)n) more = false;
+                    int32_t got = 0;
+                    const int64_t first = q.claim(cnt, got);
                     if (!active) {
                         const int32_t rank = __builtin_amdgcn_mbcnt_hi(uint32_t(idle >> 32),
                                                                        __builtin_amdgcn_mbcnt_lo(uint32_t(idle), 0));
-                        const unsigned long long idx = base + (unsigned long long)rank;
-                        if (idx < (unsigned long long)n) start(s, rays, hits, int64_t(idx), flags, tri_tests);
+                        if (rank < got) start(s, rays, hits, q.at(first + rank), flags, tri_tests);
                     }
                 }
             }
             if (!__ballot(active)) {
-                if (!more) break;
+                if (!q.more) break;
                 continue;
             }
             round(s, rays, hits, flags, max_iter, tri_tests, st);

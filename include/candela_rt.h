@@ -336,7 +336,10 @@ typedef struct crt_trace_config {
     int32_t lds_stack;     /* stack format: entries kept in LDS, one of 8, 16, 32, 48,
                               64 (0 = 16); deeper entries live in per-lane scratch */
     int32_t kernel;        /* CRT_KERNEL_* */
-    int32_t variant;       /* persistent / wave: inner-step, leaf-batch, window preset 0..5 */
+    int32_t variant;       /* persistent / wave preset: 0..5 over the BVH2 (inner steps, leaf
+                              batch, window); 0..9 over the wide layouts (stack depth, inner
+                              steps, leaf batch, refill threshold; 6..9 trade register spills
+                              for occupancy) */
     int32_t grid;          /* persistent: workgroups; 0 = enough to fill every CU */
     int32_t reserved;
     uint32_t* tri_tests;   /* optional device array [nrays]: the counting build of the
@@ -557,7 +560,7 @@ typedef struct crt_path_sample {
  * path a jittered camera ray, closest hits, diffuse bounces with Russian
  * roulette, sky radiance on escape; out[p] per path.  Device-filling waves
  * claim new paths whenever enough lanes are idle, so segments of different
- * paths and bounces share the waves.  config: variant 0..3 (stack depth,
+ * paths and bounces share the waves.  config: variant 0..5 (stack depth,
  * inner steps, leaf batch, refill threshold), grid (0 = fill the device). */
 int crt_trace_paths(const crt_scene_view* scene, const crt_path_params* params, crt_path_sample* out,
                     int64_t npaths, const crt_trace_config* config, void* stream);
@@ -651,7 +654,8 @@ typedef struct crt_batch_config {
     int32_t waves_per_group;/* 1, 2 or 4 waves per workgroup */
     int32_t grid;           /* workgroups; 0 = enough to fill every CU */
     int32_t order;          /* 0 = batch order, 1 = heaviest tiles first (cost estimate) */
-    int32_t reserved;
+    int32_t shards;         /* tile-queue counters: 0 or 1 = one, 16 = sixteen interleaved shards
+                               (shard k holds tiles k, k + 16, ...) */
 } crt_batch_config;
 
 typedef struct crt_batch_plan crt_batch_plan;
@@ -673,7 +677,8 @@ int64_t crt_batch_plan_rays(const crt_batch_plan* plan);
 /* Times every candidate configuration on the batches (`repeats` launches
  * each, after one warm-up) and returns the fastest in *best and its time per
  * launch in *best_ms.  candidates == NULL searches the built-in grid:
- * variants 0..5 x tiles_per_claim {1,2,4} x waves_per_group {1,4} x order {0,1}. */
+ * variants 0..5 x tiles_per_claim {1,2,4} x waves_per_group {1,4} x order {0,1}
+ * x shards {1,16}. */
 int crt_autotune_batched(const crt_batch* batches, int nbatches, const crt_batch_config* candidates,
                          int ncandidates, int repeats, void* stream, crt_batch_config* best, float* best_ms);
 

@@ -66,13 +66,14 @@ def test_batched_default_plan(work, orc):
 @pytest.mark.parametrize("variant", range(6))
 def test_batched_configs(work, orc, variant):
     import candela_amd as crt
-    for tpc, wpg, order, grid in ((1, 4, 1, 0), (2, 1, 0, 0), (4, 2, 1, 3)):
+    for tpc, wpg, order, grid, shards in ((1, 4, 1, 0, 0), (2, 1, 0, 0, 16), (4, 2, 1, 3, 16), (1, 1, 1, 5, 16)):
         clear_hits(work)
-        cfg = crt.BatchConfig(variant, tpc, wpg, grid, order, 0)
+        cfg = crt.BatchConfig(variant, tpc, wpg, grid, order, shards)
         plan = crt.BatchPlan(work.batches(crt), cfg)
         plan.launch()
         crt.lib().crt_device_sync()
-        check_all(work, orc, f"batched variant={variant} tpc={tpc} wpg={wpg} order={order} grid={grid}")
+        check_all(work, orc, f"batched variant={variant} tpc={tpc} wpg={wpg} order={order} grid={grid} "
+                             f"shards={shards}")
 
 
 def test_batched_graph_replays(work, orc):
@@ -107,7 +108,8 @@ def test_batched_equals_plain_with_counters(work):
 
 def test_autotune_returns_a_working_config(work, orc):
     import candela_amd as crt
-    cands = [crt.BatchConfig(v, t, w, 0, o, 0) for v in (0, 1) for t in (1, 2) for w in (1, 4) for o in (0, 1)]
+    cands = [crt.BatchConfig(v, t, w, 0, o, q) for v in (0, 1) for t in (1, 2) for w in (1, 4) for o in (0, 1)
+             for q in (1, 16)]
     best, ms = crt.autotune_batched(work.batches(crt), cands, repeats=1)
     log(f"[autotune] best variant={best.variant} tpc={best.tiles_per_claim} wpg={best.waves_per_group} "
         f"order={best.order} {ms:.3f} ms")
@@ -130,12 +132,12 @@ def test_batched_wide(work, orc, width):
     import candela_amd as crt
     layout = (width, 2)
     work.build_wide(crt, width, 2)
-    for variant, tpc, wpg, order in ((0, 1, 4, 1), (1, 2, 1, 0), (2, 4, 2, 1), (3, 1, 4, 0), (4, 2, 4, 1),
-                                     (5, 1, 1, 1)):
+    for variant, tpc, wpg, order, shards in ((0, 1, 4, 1, 16), (1, 2, 1, 0, 1), (2, 4, 2, 1, 16), (3, 1, 4, 0, 0),
+                                             (4, 2, 4, 1, 16), (5, 1, 1, 1, 16)):
         for s in work.scenes:
             s.h_closest.fill_bytes(0xAB)
             s.h_shadow.fill_bytes(0xAB)
-        plan = crt.BatchPlan(work.batches(crt, layout), crt.BatchConfig(variant, tpc, wpg, 0, order, 0))
+        plan = crt.BatchPlan(work.batches(crt, layout), crt.BatchConfig(variant, tpc, wpg, 0, order, shards))
         for replay in range(2):
             plan.launch(graph=replay == 1)
             crt.lib().crt_device_sync()
@@ -143,7 +145,7 @@ def test_batched_wide(work, orc, width):
                 for which, h in (("closest", s.h_closest), ("shadow", s.h_shadow)):
                     got, ref = h.download(), wide_oracle_hits(orc, s, layout, which)
                     assert got.tobytes() == ref.tobytes(), f"wide{width} v{variant} scene {s.index} {which}"
-        log(f"[parity] batched wide{width} variant={variant} tpc={tpc} wpg={wpg} order={order}: "
+        log(f"[parity] batched wide{width} variant={variant} tpc={tpc} wpg={wpg} order={order} shards={shards}: "
             f"{work.rays} rays bit-identical to the host wide walk (eager and graph)")
     # mixing node formats in one plan is rejected
     mixed = work.batches(crt, layout)[:2] + work.batches(crt)[2:4]

@@ -69,7 +69,7 @@ def test_paths_match_host_twin(crt, orc, scene):
 def test_presets_and_grids_agree(crt, scene):
     r, w, albedo = scene
     _, base = _run(crt, r, albedo, 128, 72, 2)
-    for variant, grid in ((1, 0), (2, 3), (3, 0), (0, 1)):
+    for variant, grid in ((1, 0), (2, 3), (3, 0), (0, 1), (4, 0), (5, 2)):
         _, got = _run(crt, r, albedo, 128, 72, 2, variant=variant, grid=grid)
         assert got.tobytes() == base.tobytes(), f"variant {variant} grid {grid}"
     log(f"[parity] paths: presets 0..3 and grids 1, 3 give identical records for {base.size} paths")
@@ -85,4 +85,4 @@ def test_errors(crt, scene):
     with pytest.raises(crt.CrtError):
         crt.trace_paths(r.wide_scene(), crt.path_params(4, 4, 1, iv, ip, rr=0.0), out)
     with pytest.raises(crt.CrtError):
-        crt.trace_paths(r.wide_scene(), crt.path_params(4, 4, 1, iv, ip), out, variant=4)
+        crt.trace_paths(r.wide_scene(), crt.path_params(4, 4, 1, iv, ip), out, variant=6)

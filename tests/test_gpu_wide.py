@@ -97,7 +97,8 @@ def test_wide_cloud_incoherent(gpu, orc, src, width):
 
 @pytest.mark.parametrize("width", [4, 8])
 def test_wide_wave_presets(gpu, orc, width):
-    """The wave-scheduled and the refilling wide walks (every preset, several grids) against the host twin."""
+    """The wave-scheduled and the refilling wide walks (every preset, including the occupancy-hinted builds
+    6..9, several grids) against the host twin."""
     from candela_amd import scenes
     log(f"[case] wide wave presets width={width}")
     meshes = scenes.icosphere_cloud(count=150, subdivisions=3, seed=23, box=35.0)
@@ -111,9 +112,9 @@ def test_wide_wave_presets(gpu, orc, width):
         o16 = orc.trace_wide(width, w.groups, w.records, w.entities, q, rays, stack_cap=16 + 64)
         assert np.array_equal(o, o16)                   # no ray comes near the stack capacity
         compare(gpu_trace(gpu, scene, q, rays, kernel=1), o, f"wide{width} thread q={q}")
-        for v in range(6):
+        for v in range(10):
             compare(gpu_trace(gpu, scene, q, rays, kernel=3, variant=v), o, f"wide{width} wave v{v} q={q}")
-        for v, grid in ((0, 0), (1, 7), (2, 0), (3, 64), (4, 0), (5, 1)):
+        for v, grid in ((0, 0), (1, 7), (2, 0), (3, 64), (4, 0), (5, 1), (6, 0), (7, 3), (8, 0), (9, 0)):
             compare(gpu_trace(gpu, scene, q, rays, kernel=2, variant=v, grid=grid), o,
                     f"wide{width} refill v{v} grid={grid} q={q}")
 
@@ -160,9 +161,9 @@ def test_wide_errors(gpu):
     r = _wide_context(gpu, 0, scenes.boxes_scene(), 4, 2)
     rays = random_rays(10, [0, 0, 0], [1, 1, 1], seed=0)
     with pytest.raises(gpu.CrtError):
-        gpu_trace(gpu, r.wide_scene(), 0, rays, kernel=3, variant=6)
+        gpu_trace(gpu, r.wide_scene(), 0, rays, kernel=3, variant=10)
     with pytest.raises(gpu.CrtError):
-        gpu_trace(gpu, r.wide_scene(), 0, rays, kernel=2, variant=6)
+        gpu_trace(gpu, r.wide_scene(), 0, rays, kernel=2, variant=10)
     with pytest.raises(gpu.CrtError):
         gpu_trace(gpu, r.wide_scene(), 0, rays, lds=64)            # wide: 8, 16 or 32
     with pytest.raises(gpu.CrtError):
