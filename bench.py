@@ -321,7 +321,7 @@ def time_large(crt, args) -> dict:
     stream = crt.Stream()
     scene = dw.scene
     best = None
-    for v in range(6):
+    for v in (range(6) if args.large_preset < 0 else [args.large_preset]):
         crt.trace_paths(scene, p, out, n, variant=v, stream=stream)   # warm-up
         stream.sync()
         ev = Events(crt, 2)
@@ -403,12 +403,15 @@ def main(argv=None) -> int:
     ap.add_argument("--batched-steps", type=int, default=5)
     ap.add_argument("--only", default="", metavar="KERNEL:VARIANT",
                     help="time one kernel choice only (e.g. 1:0 one ray per lane, 3:1 wave walk preset 1, "
-                         "w4l2:3:0 the wave walk preset 0 over the BVH4 layout with 2-triangle leaves)")
+                         "w4l2:3:0 the wave walk preset 0 over the BVH4 layout with 2-triangle leaves; a suffix "
+                         "+ao:K:V runs the AO launches with kernel K, variant V)")
     ap.add_argument("--no-wide", action="store_true", help="time the BVH2 kernels only")
     ap.add_argument("--no-large", action="store_true",
                     help="skip the large-scene path-tracing workload (15.6M triangles, 3840x2160x4 paths)")
     ap.add_argument("--large-scale", type=int, default=1, metavar="D",
                     help="divide the large scene's terrain grid, sphere count and image size by D (tests)")
+    ap.add_argument("--large-preset", type=int, default=-1, metavar="V",
+                    help="time only path-kernel preset V on the large scene (default: every preset)")
     ap.add_argument("--large-only", action="store_true",
                     help="run only the large-scene path-tracing workload and print its record as one JSON line")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
@@ -529,19 +532,25 @@ def main(argv=None) -> int:
             candidates += [wide(w, leaf, k, v) for w in (4, 8) for leaf in (2, 4)
                            for k in (crt.KERNEL_WAVE, crt.KERNEL_PERSISTENT) for v in range(10)]
     if args.only:
-        parts = args.only.split(":")
+        main_part, _, ao_part = args.only.partition("+ao:")
+        parts = main_part.split(":")
         if parts[0].startswith("w"):
             w, leaf = (int(x) for x in parts[0][1:].split("l"))
-            candidates = [wide(w, leaf, int(parts[1]), int(parts[2]))]
+            only = wide(w, leaf, int(parts[1]), int(parts[2]))
         else:
-            candidates = [bvh2(int(parts[0]), int(parts[1]))]
+            only = bvh2(int(parts[0]), int(parts[1]))
+        if ao_part:
+            ak, av = (int(x) for x in ao_part.split(":"))
+            only.update(ao_kernel=ak, ao_variant=av)
+        candidates = [only]
     results = [run(c) for c in candidates]
     # The AO and GI launches may use different kernels over the same layout: the fastest GI choice with the
     # fastest AO choice of its layout, timed as one more candidate.
     fastest_gi = min(results, key=lambda r: r["gi_ms"])
     same_layout = [r for r in results if scene_of(r) is scene_of(fastest_gi) and "ao_kernel" not in r]
-    fastest_ao = min(same_layout, key=lambda r: r["ao_ms"])
-    if (fastest_ao["kernel"], fastest_ao["variant"]) != (fastest_gi["kernel"], fastest_gi["variant"]):
+    fastest_ao = min(same_layout, key=lambda r: r["ao_ms"]) if same_layout else fastest_gi
+    if "ao_kernel" not in fastest_gi and \
+            (fastest_ao["kernel"], fastest_ao["variant"]) != (fastest_gi["kernel"], fastest_gi["variant"]):
         mixed = {k: v for k, v in fastest_gi.items()
                  if k in ("layout", "width", "max_leaf", "kernel", "variant")}
         mixed.update(ao_kernel=fastest_ao["kernel"], ao_variant=fastest_ao["variant"])

@@ -169,7 +169,8 @@ struct Core {
         const float e1 = mn(mn(mx(fx, nx), mn(mx(fy, ny), mx(fz, nz))) * kWideGrow, tgrow);                \
         const float e0 = mx(mx(mn(fx, nx), mx(mn(fy, ny), mn(fz, nz))), 0.0f);                              \
         const int32_t cnt = __float_as_int(CN), chi = __float_as_int(CH);                                   \
-        const bool hit = cnt >= 0 && e1 >= e0 && (e0 > 0.0f || e1 > 0.0f);                                  \
+        /* e0 >= 0, so (e0 > 0 || e1 > 0) given e1 >= e0 is e1 > 0 */                                      \
+        const bool hit = cnt >= 0 && e1 >= e0 && e1 > 0.0f;                                                 \
         if (hit && cnt > 0) {                                                                               \
             st.put_leaf(np, (chi << 4) | cnt);                                                              \
             ++np;                                                                                           \
