@@ -288,6 +288,24 @@ def time_raygen(crt, scene, d_rays, d_hits, rays: np.ndarray, hits: np.ndarray, 
     return res
 
 
+def time_gpu_build(crt, meshes, stream, steps: int) -> dict:
+    """crt_build_bvh_gpu over one object (linear BVH over Morton codes): HIP-event time per build, after one
+    warm-up build, with inputs resident on the device."""
+    from candela_amd import scenes
+    verts, idx, mids = scenes.concat_meshes(meshes)
+    b = crt.GpuBVH(verts, idx, mids, stream=stream)
+    stream.sync()
+    ev = Events(crt, 2)
+    ev.record(0, stream.handle)
+    for _ in range(steps):
+        b.build(stream)
+    ev.record(1, stream.handle)
+    ms = ev.ms(0, 1) / steps
+    ev.close()
+    log(f"gpu build: {b.ntris} triangles in {ms:.3f} ms -> {b.ntris / ms / 1e3:.1f} Mtris/s")
+    return {"triangles": int(b.ntris), "ms": round(ms, 3), "mtris_s": round(b.ntris / ms / 1e3, 1)}, b
+
+
 def time_large(crt, args) -> dict:
     """The large-scene workload: a 2048^2 fBm terrain (8,388,608 triangles) plus 1,400 subdivision-4
     icospheres (7,168,000), 3840 x 2160 pixels x 4 samples, 2 diffuse bounces, roulette p = 0.8 from the
@@ -300,8 +318,15 @@ def time_large(crt, args) -> dict:
     t1 = time.perf_counter()
     r = crt.RayIntersector(crt.FORMAT_STACKLESS)
     r.add_object(0, meshes)
-    del meshes
     t2 = time.perf_counter()
+    gpu_build = None
+    try:
+        gpu_build, gb = time_gpu_build(crt, meshes, crt.Stream(), 5)
+        del gb
+    except Exception as e:   # reported, never fatal to the workload
+        gpu_build = {"error": f"{type(e).__name__}: {e}"}
+    del meshes
+    t2b = time.perf_counter()
     r.push_entity(0)
     r.buffer_entities()                              # materialises the entity record
     hb = r.host_buffers()
@@ -359,7 +384,8 @@ def time_large(crt, args) -> dict:
              "mean_segments": round(segs / n, 4), "bytes_per_frame": nbytes,
              "achieved_GBs": round(nbytes / (ms * 1e-3) / 1e9, 1),
              "status_flags": int(np.count_nonzero(res["status"])),
-             "setup_s": {"scene": round(t1 - t0, 1), "bvh": round(t2 - t1, 1), "upload_and_wide": round(t3 - t2, 1)},
+             "setup_s": {"scene": round(t1 - t0, 1), "bvh": round(t2 - t1, 1), "upload_and_wide": round(t3 - t2b, 1)},
+             "gpu_build": gpu_build,
              "parity": parity}
     log(f"large scene: {out_d['mrays_s']} Mrays/s ({segs} segments in {ms:.2f} ms), parity {parity}")
     return out_d
@@ -588,6 +614,32 @@ def main(argv=None) -> int:
     # --- hit attributes (GetData) of the GI hits: one launch, HIP events -----------------------
     attrs = time_attributes(crt, best_scene, h_gi, n_gi, out["gi"]["hits"], stream, K)
 
+    # --- device BVH builder over the same scene; GI rays over its tree (stack layout) ----------
+    try:
+        gpu_build, gb = time_gpu_build(crt, meshes, stream, K)
+        dh_b = crt.DeviceArray(crt.HIT_DTYPE, n_gi)
+        crt.trace(gb.scene, crt.QUERY_CLOSEST, d_gi, dh_b, kernel=crt.KERNEL_THREAD_PER_RAY, stream=stream)
+        stream.sync()
+        ev = Events(crt, 2)
+        ev.record(0, sh)
+        for _ in range(K):
+            crt.trace(gb.scene, crt.QUERY_CLOSEST, d_gi, dh_b, kernel=crt.KERNEL_THREAD_PER_RAY, stream=stream)
+        ev.record(1, sh)
+        ms_b = ev.ms(0, 1) / K
+        ev.close()
+        same_t = float(np.mean(dh_b.download()["t"] == out["gi"]["hits"]["t"]))
+        sah_ms = thread["gi_ms"] if thread["layout"] == "bvh2" and thread["kernel"] == crt.KERNEL_THREAD_PER_RAY \
+            else None
+        gpu_build.update(gi_ms_thread_kernel=round(ms_b, 4),
+                         gi_ms_thread_kernel_sah=round(sah_ms, 4) if sah_ms is not None else None,
+                         gi_t_agreement_with_sah_tree=same_t)
+        log(f"gpu-built tree: GI {ms_b:.3f} ms one ray per lane (SAH tree {sah_ms} ms), "
+            f"nearest t agreement {same_t:.6f}")
+        del gb, dh_b
+    except Exception as e:   # reported, never fatal to the headline line
+        gpu_build = {"error": f"{type(e).__name__}: {e}"}
+        log(f"gpu build failed: {gpu_build['error']}")
+
     # --- wavefront ray generation from the primary G-buffer: one launch chain, HIP events -------
     raygen = time_raygen(crt, scene, drays, dhits, prays, phits, hb, stream, K)
 
@@ -701,6 +753,7 @@ def main(argv=None) -> int:
                      "bytes_per_ray": round(out["gi"]["bytes"] / n_gi, 1)},
         "attributes": attrs,
         "raygen": raygen,
+        "gpu_build": gpu_build,
         "large_scene": large,
         "cpu_baseline": cpu,
         "parity": parity,

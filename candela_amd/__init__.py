@@ -34,7 +34,7 @@ __all__ = [
     "TEXTURE_REF_DTYPE", "TEXTURE_DTYPE", "HIT_ATTR_DTYPE", "MaterialTable", "hit_attributes",
     "PATH_PARAMS_DTYPE", "PATH_SAMPLE_DTYPE", "path_params", "trace_paths", "hash64", "bvh_key", "load_bvh",
     "CacheHeader", "E_IO", "E_CORRUPT", "E_STALE", "RAYGEN_PARAMS_DTYPE", "PROBE_GRID_DTYPE", "raygen_params",
-    "RayGen", "probe_grid", "gen_probe_rays",
+    "RayGen", "probe_grid", "gen_probe_rays", "GpuBVH",
 ]
 
 
@@ -375,6 +375,46 @@ def gen_probe_rays(grid: np.ndarray, out: DeviceArray, stream=None) -> None:
     if out.count < int(np.prod(grid["res"][0].astype(np.int64))):
         raise ValueError("output smaller than the probe count")
     check(lib().crt_gen_probe_rays(ptr(grid), out.ptr, _stream_handle(stream)))
+
+
+def _identity_entity(node_count: int) -> np.ndarray:
+    e = np.zeros(1, ENTITY_DTYPE)
+    e["model"][0] = np.eye(4, dtype=np.float32).reshape(16)
+    e["inverse"][0] = np.eye(4, dtype=np.float32).reshape(16)
+    e["node_count"] = node_count
+    alpha = np.array([1.0], np.float32).view(np.int32)[0]
+    e["data"][0, 1] = alpha
+    return e
+
+
+class GpuBVH:
+    """crt_build_bvh_gpu: the stack layout of one object built on the device (a linear BVH over Morton codes;
+    not BuildBVH's tree).  `scene` is a crt_scene_view over it with one identity entity."""
+
+    def __init__(self, vertices: np.ndarray, indices: np.ndarray, mesh_ids=None, mesh_id: int = 0, stream=None):
+        v = np.ascontiguousarray(vertices, VERTEX_DTYPE)
+        idx = np.ascontiguousarray(indices, np.uint32).reshape(-1)
+        if idx.size % 3 or idx.size < 6 or int(idx.max()) >= v.size:
+            raise ValueError("GpuBVH needs >= 2 triangles whose vertex indices lie in the vertex array")
+        self.ntris = idx.size // 3
+        self.vertices = DeviceArray.from_host(v)
+        self.indices = DeviceArray.from_host(idx)
+        self.mesh_ids = None if mesh_ids is None else DeviceArray.from_host(np.ascontiguousarray(mesh_ids, np.int32))
+        self.mesh_id = int(mesh_id)
+        self.nodes = DeviceArray(STACK_NODE_DTYPE, max(1, self.ntris - 1))
+        self.triangles = DeviceArray(TRIANGLE_DTYPE, max(1, self.ntris))
+        self.workspace = DeviceArray(np.uint8, max(256, int(lib().crt_build_bvh_gpu_workspace(self.ntris))))
+        self.entities = DeviceArray.from_host(_identity_entity(max(0, self.ntris - 1)))
+        self.build(stream)
+        self.scene = SceneView(FORMAT_STACK, 1, max(0, self.ntris - 1), self.nodes.ptr, self.triangles.ptr,
+                               self.vertices.ptr, self.entities.ptr, None)
+
+    def build(self, stream=None) -> None:
+        """Rebuilds from the current vertex buffer (asynchronous)."""
+        check(lib().crt_build_bvh_gpu(self.vertices.ptr, self.vertices.count, self.indices.ptr, self.ntris,
+                                      self.mesh_ids.ptr if self.mesh_ids is not None else None, self.mesh_id, 0,
+                                      self.nodes.ptr, self.triangles.ptr, self.workspace.ptr, self.workspace.nbytes,
+                                      _stream_handle(stream)))
 
 
 class DeviceWide:

@@ -29,6 +29,7 @@
 #include "crt_internal.h"
 #include "device_math.h"
 #include "sampling.h"
+#include "scan.h"
 #include "trace_launch.h"
 
 #pragma clang fp contract(off)
@@ -39,8 +40,9 @@ namespace rg {
 using namespace ::crt::dm;
 using namespace ::crt::smp;
 
-constexpr int kBlock = 256;
-constexpr int kScanBlock = 1024;
+using scan::kBlock;
+using scan::kScanBlock;
+using scan::block_rank;
 
 struct Params {
     uint32_t seed;
@@ -54,24 +56,6 @@ __device__ __forceinline__ bool live(const crt_hit* __restrict__ hits, int64_t i
     return i < n && hits[i].t > 0.0f;
 }
 
-// Rank of this lane among the flagged lanes of its workgroup, and the
-// workgroup's count.  wave_counts: LDS of kBlock / 64 entries.
-__device__ __forceinline__ uint32_t block_rank(bool flag, uint32_t* wave_counts, uint32_t& total) {
-    const uint64_t ballot = __ballot(flag);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const uint32_t below = __builtin_amdgcn_mbcnt_hi(uint32_t(ballot >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(ballot), 0u));
-    if (lane == 0) wave_counts[wave] = uint32_t(__popcll(ballot));
-    __syncthreads();
-    uint32_t before = 0;
-    total = 0;
-    for (int w = 0; w < kBlock / 64; ++w) {
-        const uint32_t c = wave_counts[w];
-        before += w < wave ? c : 0u;
-        total += c;
-    }
-    return before + below;
-}
-
 __global__ void __launch_bounds__(kBlock) count_kernel(const crt_hit* __restrict__ hits, int64_t n,
                                                        uint32_t* __restrict__ block_counts) {
     __shared__ uint32_t wave_counts[kBlock / 64];
@@ -79,38 +63,6 @@ __global__ void __launch_bounds__(kBlock) count_kernel(const crt_hit* __restrict
     uint32_t total;
     (void)block_rank(live(hits, i, n), wave_counts, total);
     if (threadIdx.x == 0) block_counts[blockIdx.x] = total;
-}
-
-// Exclusive scan of nb block counts by one workgroup; the total goes to *count.
-__global__ void __launch_bounds__(kScanBlock) scan_kernel(const uint32_t* __restrict__ block_counts, int64_t nb,
-                                                          int64_t* __restrict__ offsets, int64_t* __restrict__ count) {
-    __shared__ int64_t wave_sums[kScanBlock / 64];
-    __shared__ int64_t carry_sh;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    if (threadIdx.x == 0) carry_sh = 0;
-    __syncthreads();
-    for (int64_t base = 0; base < nb; base += kScanBlock) {
-        const int64_t k = base + threadIdx.x;
-        const int64_t v = k < nb ? int64_t(block_counts[k]) : 0;
-        int64_t incl = v;
-        for (int d = 1; d < 64; d <<= 1) {
-            const int64_t o = __shfl_up(incl, d, 64);
-            if (lane >= d) incl += o;
-        }
-        if (lane == 63) wave_sums[wave] = incl;
-        __syncthreads();
-        int64_t before = carry_sh;
-        int64_t chunk = 0;
-        for (int w = 0; w < kScanBlock / 64; ++w) {
-            before += w < wave ? wave_sums[w] : 0;
-            chunk += wave_sums[w];
-        }
-        if (k < nb) offsets[k] = before + incl - v;
-        __syncthreads();
-        if (threadIdx.x == 0) carry_sh += chunk;
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) *count = carry_sh;
 }
 
 struct Surface {
@@ -271,7 +223,7 @@ int generate(int kind, const char* who, const crt_scene_view* scene, const crt_r
              params->roughness,     params->nmesh};
     const SceneArgs a = scene_args(*scene);
     if (nb > 0) hipLaunchKernelGGL(count_kernel, dim3(unsigned(nb)), dim3(kBlock), 0, st, hits, n, counts);
-    hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(kScanBlock), 0, st, counts, nb, offsets, d_count);
+    hipLaunchKernelGGL(scan::exclusive_scan, dim3(1), dim3(kScanBlock), 0, st, counts, nb, offsets, d_count);
     if (nb > 0) {
         if (kind == 0)
             hipLaunchKernelGGL(emit_kernel<0>, dim3(unsigned(nb)), dim3(kBlock), 0, st, a, P, primary, hits, n,

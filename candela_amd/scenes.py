@@ -223,6 +223,18 @@ def scene_arrays(meshes):
     return np.concatenate(pos), np.concatenate(tris)
 
 
+def concat_meshes(meshes):
+    """(vertex records [V], indices [3N] u32 with vertex offsets, mesh id per triangle [N]) in mesh order,
+    as BuildBVH concatenates an object's meshes (BVHConstructor.cpp:981-1002)."""
+    vs, idx, mids, off = [], [], [], 0
+    for v, i, m in meshes:
+        vs.append(v)
+        idx.append(np.asarray(i, np.uint32).reshape(-1) + np.uint32(off))
+        mids.append(np.full(len(i) // 3, m, np.int32))
+        off += len(v)
+    return np.concatenate(vs), np.concatenate(idx), np.concatenate(mids)
+
+
 # --- cameras (glm::lookAt / glm::perspective, right-handed, clip z in [-1, 1]) -----
 
 

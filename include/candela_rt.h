@@ -273,6 +273,29 @@ int crt_bvh_load(const char* path, uint64_t expect_key, crt_bvh** out, crt_cache
 int crt_bvh_rebase(crt_bvh* bvh, int32_t t_offset);
 
 /* ------------------------------------------------------------------------- */
+/* Device BVH builder.  No reference counterpart (BuildBVH runs on the CPU,   */
+/* BVHConstructor.cpp:951-1108): a linear BVH over 30-bit Morton codes of the */
+/* triangle centroids (Karras 2012), for scenes rebuilt every frame.  The     */
+/* tree is NOT BuildBVH's, so the buffers are not byte-identical to it; every */
+/* walk over them still reports the nearest hit.                              */
+/* ------------------------------------------------------------------------- */
+
+/* Bytes of device workspace crt_build_bvh_gpu needs for ntris triangles. */
+size_t crt_build_bvh_gpu_workspace(int64_t ntris);
+/* Builds the stack layout of one object on the device, asynchronously on
+ * `stream`: nodes [ntris - 1] (entry i holds both children of inner node i;
+ * a child covering two triangles is written as a 2-triangle leaf) and
+ * triangles [ntris] ({i0, i1, i2, mesh} in leaf order; leaf starts count from
+ * t_offset).  vertices / indices (3 per triangle, indices into vertices) are
+ * device arrays; mesh_ids (device, one per triangle) or, when NULL, mesh_id
+ * for every triangle.  Every index must be < nverts (device data is not
+ * checked).  workspace: 256-byte aligned device memory of
+ * crt_build_bvh_gpu_workspace(ntris) bytes.  Needs ntris >= 2. */
+int crt_build_bvh_gpu(const crt_vertex* vertices, int64_t nverts, const uint32_t* indices, int64_t ntris,
+                      const int32_t* mesh_ids, int32_t mesh_id, int32_t t_offset, crt_stack_node* nodes,
+                      crt_triangle* triangles, void* workspace, size_t workspace_bytes, void* stream);
+
+/* ------------------------------------------------------------------------- */
 /* Device memory (thin wrappers so callers need no HIP headers)              */
 /* ------------------------------------------------------------------------- */
 int crt_device_alloc(void** ptr, size_t bytes);

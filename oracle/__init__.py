@@ -76,6 +76,10 @@ def lib() -> C.CDLL:
         l.oracle_gen_rays.argtypes = [C.c_int, vp, vp, vp, vp, vp, vp, i64, vp, vp, vp, vp]
         l.oracle_gen_probe_rays.restype = None
         l.oracle_gen_probe_rays.argtypes = [vp, vp]
+        l.oracle_build_lbvh.restype = C.c_int
+        l.oracle_build_lbvh.argtypes = [vp, vp, i64, vp, i32, i32, vp, vp]
+        l.oracle_spread3.restype = u32
+        l.oracle_spread3.argtypes = [u32]
         l.oracle_sincos_turn.restype = None
         l.oracle_sincos_turn.argtypes = [f32, vp, vp]
         l.oracle_hit_attributes.restype = None
@@ -374,3 +378,22 @@ def gen_probe_rays(grid) -> np.ndarray:
     out = np.zeros(int(np.prod(grid["res"][0].astype(np.int64))), RAY_DTYPE)
     lib().oracle_gen_probe_rays(_p(grid), _p(out))
     return out
+
+
+def build_lbvh(vertices, indices, mesh_ids=None, mesh_id: int = 0, t_offset: int = 0):
+    """The device builder's host twin: (stack nodes [n - 1], triangles [n]) for n = len(indices) // 3."""
+    v = np.ascontiguousarray(vertices, VERTEX_DTYPE)
+    idx = np.ascontiguousarray(indices, np.uint32).reshape(-1)
+    n = idx.size // 3
+    m = None if mesh_ids is None else np.ascontiguousarray(mesh_ids, np.int32)
+    nodes = np.zeros(max(0, n - 1), STACK_NODE_DTYPE)
+    tris = np.zeros(n, TRIANGLE_DTYPE)
+    rc = lib().oracle_build_lbvh(_p(v), _p(idx), n, _p(m) if m is not None else None, mesh_id, t_offset, _p(nodes),
+                                 _p(tris))
+    if rc != 0:
+        raise ValueError("oracle_build_lbvh failed (needs >= 2 triangles)")
+    return nodes, tris
+
+
+def spread3(v: int) -> int:
+    return int(lib().oracle_spread3(v))
