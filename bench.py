@@ -96,18 +96,23 @@ def kernel_label(c: dict, gi_only: bool = False) -> str:
             name, c["width"], c["max_leaf"])
     if "ao_kernel" in c and not gi_only:
         label += "; AO launch: " + _kernel_name(c["layout"], c["ao_kernel"], c["ao_variant"])
+    if c.get("ao_order") == 1 and not gi_only:
+        label += "; AO children in slot order"
     return label
 
 
 def choice_name(c: dict) -> str:
     """'K:V' for a BVH2 kernel choice, 'wWlL:K:V' for kernel K, variant V over the wide layout of width W
-    and leaf size L; a choice whose AO launch uses another kernel gets '+ao:K:V'."""
+    and leaf size L; a choice whose AO launch uses another kernel gets '+ao:K:V', and ':s' after it when the
+    AO walks take children in slot order (CRT_FLAG_SLOT_ORDER)."""
     if c["layout"] == "bvh2":
         name = f"{c['kernel']}:{c['variant']}"
     else:
         name = f"w{c['width']}l{c['max_leaf']}:{c['kernel']}:{c['variant']}"
     if "ao_kernel" in c:
         name += f"+ao:{c['ao_kernel']}:{c['ao_variant']}"
+    if c.get("ao_order") == 1:
+        name += ":s" if "ao_kernel" in c else f"+ao:{c['kernel']}:{c['variant']}:s"
     return name
 
 
@@ -119,6 +124,11 @@ def gi_choice_name(c: dict) -> str:
 def ao_of(c: dict) -> tuple:
     """(kernel, variant) of the AO launch."""
     return c.get("ao_kernel", c["kernel"]), c.get("ao_variant", c["variant"])
+
+
+def ao_flags(crt, c: dict) -> int:
+    """Trace flags of the AO launch: the child order of its any-hit walks."""
+    return crt.FLAG_SLOT_ORDER if c.get("ao_order") == 1 else 0
 
 
 def measured_traffic(best: dict, args, fmt: int):
@@ -134,9 +144,9 @@ def measured_traffic(best: dict, args, fmt: int):
             return float(d["traffic_bytes"]), os.path.relpath(path, ROOT)
     return None, None
 
-def time_batched(crt, batches, steps: int, warmup: int, stream, candidates=None) -> dict:
+def time_batched(crt, batches, steps: int, warmup: int, stream, wide: bool) -> dict:
     """Autotunes the batched kernel on `batches`, then times `steps` hipGraph replays of the best plan."""
-    best, tune_ms = crt.autotune_batched(batches, candidates or default_candidates(crt), repeats=3, stream=stream)
+    best, tune_ms = crt.autotune_batched(batches, default_candidates(crt, wide), repeats=3, stream=stream)
     plan = crt.BatchPlan(batches, best)
     for _ in range(max(1, warmup)):
         plan.launch(stream=stream, graph=True)
@@ -148,19 +158,22 @@ def time_batched(crt, batches, steps: int, warmup: int, stream, candidates=None)
     ev.record(1, stream.handle)
     ms = ev.ms(0, 1) / steps
     ev.close()
-    cfg = {k: getattr(best, k) for k in ("variant", "tiles_per_claim", "waves_per_group", "order", "shards")}
+    cfg = {k: getattr(best, k) for k in ("variant", "tiles_per_claim", "waves_per_group", "order", "shards",
+                                         "child_order")}
     return {"ms": ms, "config": cfg, "tune_ms": tune_ms, "rays": plan.rays}
 
 
-def default_candidates(crt):
-    return [crt.BatchConfig(v, t, w, 0, o, q) for v in (0, 1, 4) for t in (1, 2) for w in (1, 4) for o in (0, 1)
-            for q in (1, 16)]
+def default_candidates(crt, wide: bool):
+    """Autotuner grid: preset, tiles per claim, waves per group, tile order, queue shards and, over the wide
+    layouts, the child order of the any-hit walks."""
+    return [crt.BatchConfig(v, t, w, 0, o, q, co) for v in (0, 1, 4) for t in (1, 2) for w in (1, 4)
+            for o in (0, 1) for q in (1, 16) for co in ((0, 1) if wide else (0,))]
 
 
 def batched_vs_plain(crt, args, plain_choice: dict, stream) -> dict:
     """64 scenes (log-uniform 1k..2M triangles) x (primary, GI, shadow) batches: plain launches vs one launch,
     over the BVH2 and over the BVH4 / BVH8 layouts derived from it (plain_choice: layout key -> (kernel, variant
-    of the closest-hit launches, kernel, variant of the any-hit launches))."""
+    of the closest-hit launches, kernel, variant and child order of the any-hit launches))."""
     from candela_amd import workloads
     t0 = time.perf_counter()
     work = workloads.batched_scenes(crt, count=args.batched_scenes, seed=3)
@@ -168,11 +181,12 @@ def batched_vs_plain(crt, args, plain_choice: dict, stream) -> dict:
         f"(BVH builds {work.build_s:.1f}s, setup {time.perf_counter() - t0:.1f}s)")
     K = args.batched_steps
     per_layout = {}
-    for layout, (kernel, variant, ao_kernel, ao_variant) in plain_choice.items():
+    for layout, (kernel, variant, ao_kernel, ao_variant, ao_order) in plain_choice.items():
         if layout is not None:
             log(f"wide layouts {layout} of all scenes built in {work.build_wide(crt, *layout):.1f}s")
         kw = dict(kernel=kernel, variant=variant, stream=stream)
-        kw_any = dict(kernel=ao_kernel, variant=ao_variant, stream=stream)
+        kw_any = dict(kernel=ao_kernel, variant=ao_variant, stream=stream,
+                      flags=crt.FLAG_SLOT_ORDER if ao_order == 1 else 0)
 
         def plain_step():
             for s in work.scenes:   # one launch per (scene, query): primary+GI closest, shadow any
@@ -190,14 +204,24 @@ def batched_vs_plain(crt, args, plain_choice: dict, stream) -> dict:
         ev.record(1, stream.handle)
         plain_ms = ev.ms(0, 1) / K
         ev.close()
-        plain_hits = [(s.h_closest.download(), s.h_shadow.download()) for s in work.scenes]
-        b = time_batched(crt, work.batches(crt, layout), K, args.warmup, stream)
-        same = all(s.h_closest.download().tobytes() == ph[0].tobytes() and
-                   s.h_shadow.download().tobytes() == ph[1].tobytes() for s, ph in zip(work.scenes, plain_hits))
+        plain_hits = [[s.h_closest.download(), s.h_shadow.download()] for s in work.scenes]
+        b = time_batched(crt, work.batches(crt, layout), K, args.warmup, stream, wide=layout is not None)
+        batched_hits = [(s.h_closest.download(), s.h_shadow.download()) for s in work.scenes]
+        if layout is not None and b["config"]["child_order"] != ao_order:
+            # the batched any-hit walks took the other child order: compare with plain shadow hits in that order
+            f = crt.FLAG_SLOT_ORDER if b["config"]["child_order"] == 1 else 0
+            for s in work.scenes:
+                crt.trace(s.scene_for(layout), crt.QUERY_ANY, s.d_shadow, s.h_shadow, **dict(kw_any, flags=f))
+            stream.sync()
+            for s, ph in zip(work.scenes, plain_hits):
+                ph[1] = s.h_shadow.download()
+        same = all(bh[0].tobytes() == ph[0].tobytes() and bh[1].tobytes() == ph[1].tobytes()
+                   for bh, ph in zip(batched_hits, plain_hits))
         name = "bvh2" if layout is None else f"wide{layout[0]}"
         per_layout[name] = {"plain_ms": round(plain_ms, 4), "plain_mrays_s": round(work.rays / plain_ms / 1e3, 2),
-                            "plain_kernel": f"{kernel}:{variant}" + (f"+any:{ao_kernel}:{ao_variant}" if
-                                                                     (ao_kernel, ao_variant) != (kernel, variant) else ""),
+                            "plain_kernel": f"{kernel}:{variant}" + (
+                                f"+any:{ao_kernel}:{ao_variant}" + (":s" if ao_order == 1 else "")
+                                if (ao_kernel, ao_variant, ao_order) != (kernel, variant, 0) else ""),
                             "batched_ms": round(b["ms"], 4),
                             "batched_mrays_s": round(work.rays / b["ms"] / 1e3, 2),
                             "speedup": round(plain_ms / b["ms"], 3), "batched_config": b["config"],
@@ -514,7 +538,7 @@ def main(argv=None) -> int:
         scene = scene_of(c)
         kw = dict(block_size=args.block, lds_stack=args.lds_stack, kernel=kernel, variant=variant, stream=stream)
         ak, av = ao_of(c)
-        kw_ao = dict(kw, kernel=ak, variant=av)
+        kw_ao = dict(kw, kernel=ak, variant=av, flags=ao_flags(crt, c))
         for _ in range(args.warmup):
             crt.trace(scene, crt.QUERY_ANY, d_ao, h_ao, **kw_ao)
             crt.trace(scene, crt.QUERY_CLOSEST, d_gi, h_gi, **kw)
@@ -566,8 +590,10 @@ def main(argv=None) -> int:
         else:
             only = bvh2(int(parts[0]), int(parts[1]))
         if ao_part:
-            ak, av = (int(x) for x in ao_part.split(":"))
-            only.update(ao_kernel=ak, ao_variant=av)
+            ap = ao_part.split(":")
+            only.update(ao_kernel=int(ap[0]), ao_variant=int(ap[1]))
+            if ap[2:] == ["s"]:
+                only.update(ao_order=1)
         candidates = [only]
     results = [run(c) for c in candidates]
     # The AO and GI launches may use different kernels over the same layout: the fastest GI choice with the
@@ -581,6 +607,11 @@ def main(argv=None) -> int:
                  if k in ("layout", "width", "max_leaf", "kernel", "variant")}
         mixed.update(ao_kernel=fastest_ao["kernel"], ao_variant=fastest_ao["variant"])
         results.append(run(mixed))
+    # child order of the AO walks: the fastest AO choice of the wide layouts again with slot order
+    if not args.only and fastest_gi["layout"] != "bvh2":
+        base = min((r for r in results if scene_of(r) is scene_of(fastest_gi)), key=lambda r: r["ms_step"])
+        results.append(run(dict({k: v for k, v in base.items() if k in (
+            "layout", "width", "max_leaf", "kernel", "variant", "ao_kernel", "ao_variant")}, ao_order=1)))
     thread = results[0]
     best = max(results, key=lambda r: r["mrays"])
     best_scene = scene_of(best)
@@ -595,7 +626,7 @@ def main(argv=None) -> int:
         cnt = crt.DeviceArray(np.uint32, n)
         k_, v_ = ao_of(best) if q == crt.QUERY_ANY else (best["kernel"], best["variant"])
         crt.trace(best_scene, q, dr, dh, block_size=args.block, lds_stack=args.lds_stack, kernel=k_, variant=v_,
-                  tri_tests=cnt)
+                  tri_tests=cnt, flags=ao_flags(crt, best) if q == crt.QUERY_ANY else 0)
         crt.lib().crt_device_sync()
         hits = dh.download()
         tests = cnt.download()
@@ -662,14 +693,14 @@ def main(argv=None) -> int:
 
         plain_choice = {}
         b2 = best_of(lambda r: r["layout"] == "bvh2")
-        plain_choice[None] = (b2["kernel"], b2["variant"], *ao_of(b2))
+        plain_choice[None] = (b2["kernel"], b2["variant"], *ao_of(b2), 0)
         for w in (4, 8):
             bw = best_of(lambda r: r["layout"] == f"wide{w}" and r["max_leaf"] == 2)
             if bw is not None:
-                plain_choice[(w, 2)] = (bw["kernel"], bw["variant"], *ao_of(bw))
+                plain_choice[(w, 2)] = (bw["kernel"], bw["variant"], *ao_of(bw), bw.get("ao_order", 0))
         c2_batched = time_batched(crt, [(best_scene, crt.QUERY_ANY, d_ao, h_ao),
                                         (best_scene, crt.QUERY_CLOSEST, d_gi, h_gi)],
-                                  K, args.warmup, stream)
+                                  K, args.warmup, stream, wide=best["layout"] != "bvh2")
         log(f"config-2 batches in one persistent launch: {c2_batched['ms']:.3f} ms/step -> "
             f"{rays_step / (c2_batched['ms'] * 1e-3) / 1e6:.1f} Mrays/s ({c2_batched['config']})")
         batched = batched_vs_plain(crt, args, plain_choice, stream)
@@ -718,7 +749,8 @@ def main(argv=None) -> int:
         else:   # the host twin of the wide walk, on the same wide layout
             w = wides[(best["width"], best["max_leaf"])].host
             twin_g = oracle.trace_wide(w.width, w.groups, w.records, w.entities, 0, gi, threads=threads)
-            twin_a = oracle.trace_wide(w.width, w.groups, w.records, w.entities, 1, ao, threads=threads)
+            twin_a = oracle.trace_wide(w.width, w.groups, w.records, w.entities, 1, ao, threads=threads,
+                                       flags=ao_flags(crt, best))
         parity = {"rays": int(g.size + a.size),
                   "exact": float((np.concatenate([g, a]).tobytes() == np.concatenate([twin_g, twin_a]).tobytes())),
                   "id_agreement_vs_reference_walk": float(np.mean(np.concatenate([g["tri"] == ohg["tri"],

@@ -13,7 +13,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from ._lib import (KERNEL_AUTO, KERNEL_PERSISTENT, KERNEL_THREAD_PER_RAY, KERNEL_WAVE, BOUNDS_DTYPE, ENTITY_DTYPE, FLAG_IGNORE_TRANSPARENT, FLAG_TRI0_IS_MISS, FORMAT_STACK,
+from ._lib import (KERNEL_AUTO, KERNEL_PERSISTENT, KERNEL_THREAD_PER_RAY, KERNEL_WAVE, BOUNDS_DTYPE, ENTITY_DTYPE, FLAG_IGNORE_TRANSPARENT, FLAG_TRI0_IS_MISS, FLAG_SLOT_ORDER, FORMAT_STACK,
                    FORMAT_STACKLESS, FORMAT_WIDE4, FORMAT_WIDE8, WIDE_GROUP_DTYPE, TRI_RECORD_DTYPE, WideStats,
                    TEXTURE_REF_DTYPE, TEXTURE_DTYPE, HIT_ATTR_DTYPE, PATH_PARAMS_DTYPE, PATH_SAMPLE_DTYPE,
                    HIT_DTYPE, QUERY_ANY, QUERY_CLOSEST, RAY_DTYPE, STACK_NODE_DTYPE,
@@ -25,7 +25,7 @@ from ._lib import (KERNEL_AUTO, KERNEL_PERSISTENT, KERNEL_THREAD_PER_RAY, KERNEL
 __all__ = [
     "VERTEX_DTYPE", "TRIANGLE_DTYPE", "STACKLESS_NODE_DTYPE", "STACK_NODE_DTYPE", "BOUNDS_DTYPE", "ENTITY_DTYPE",
     "RAY_DTYPE", "HIT_DTYPE", "FORMAT_STACKLESS", "FORMAT_STACK", "QUERY_CLOSEST", "QUERY_ANY",
-    "FLAG_IGNORE_TRANSPARENT", "FLAG_TRI0_IS_MISS", "STATUS_ITER_CAP", "STATUS_STACK_OVERFLOW", "STATUS_BAD_INDEX",
+    "FLAG_IGNORE_TRANSPARENT", "FLAG_TRI0_IS_MISS", "FLAG_SLOT_ORDER", "STATUS_ITER_CAP", "STATUS_STACK_OVERFLOW", "STATUS_BAD_INDEX",
     "STATUS_VISITS_MASK", "SWAP_NONE", "SWAP_SEEDED", "KERNEL_AUTO", "KERNEL_THREAD_PER_RAY", "KERNEL_PERSISTENT",
     "KERNEL_WAVE", "CrtError", "DeviceArray", "Stream", "BuiltBVH", "build_bvh",
     "RayIntersector", "trace", "device_count", "node_dtype", "lib", "Batch", "BatchConfig", "BatchPlan",

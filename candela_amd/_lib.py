@@ -23,6 +23,7 @@ QUERY_CLOSEST = 0
 QUERY_ANY = 1
 FLAG_IGNORE_TRANSPARENT = 1 << 0
 FLAG_TRI0_IS_MISS = 1 << 1
+FLAG_SLOT_ORDER = 1 << 2
 STATUS_ITER_CAP = 1 << 24
 STATUS_STACK_OVERFLOW = 1 << 25
 STATUS_BAD_INDEX = 1 << 26
@@ -125,7 +126,7 @@ class Batch(C.Structure):
 
 class BatchConfig(C.Structure):
     _fields_ = [("variant", C.c_int32), ("tiles_per_claim", C.c_int32), ("waves_per_group", C.c_int32),
-                ("grid", C.c_int32), ("order", C.c_int32), ("shards", C.c_int32)]
+                ("grid", C.c_int32), ("order", C.c_int32), ("shards", C.c_int32), ("child_order", C.c_int32)]
 
 
 class ShadeParams(C.Structure):

@@ -196,6 +196,8 @@ int hip_err(hipError_t e, const char* what) {
         if (e_ != hipSuccess) return hip_err(e_, what); \
     } while (0)
 
+static_assert(sizeof(crt_batch_config) == 28, "crt_batch_config is 7 int32 fields");
+
 int validate_config(const crt_batch_config& c) {
     if (c.variant < 0 || c.variant > 5) return crt::set_error(CRT_E_INVALID, "batch config: variant must be 0..5");
     if (c.tiles_per_claim != 1 && c.tiles_per_claim != 2 && c.tiles_per_claim != 4)
@@ -205,6 +207,8 @@ int validate_config(const crt_batch_config& c) {
     if (c.grid < 0 || c.order < 0 || c.order > 1) return crt::set_error(CRT_E_INVALID, "batch config: bad grid/order");
     if (c.shards != 0 && c.shards != 1 && c.shards != crt::kQueueShards)
         return crt::set_error(CRT_E_INVALID, "batch config: shards must be 0, 1 or 16");
+    if (c.child_order != 0 && c.child_order != 1)
+        return crt::set_error(CRT_E_INVALID, "batch config: child_order must be 0 or 1");
     return CRT_OK;
 }
 
@@ -256,7 +260,7 @@ int build_plan(const crt_batch* batches, int nbatches, const crt_batch_config& c
         d.tri_tests = b.tri_tests;
         d.count = b.count;
         d.query = b.query;
-        d.flags = b.flags;
+        d.flags = b.flags | (cfg.child_order == 1 && b.query == CRT_QUERY_ANY ? uint32_t(CRT_FLAG_SLOT_ORDER) : 0u);
     }
     std::vector<int> order(static_cast<size_t>(nbatches));
     std::iota(order.begin(), order.end(), 0);
@@ -389,19 +393,26 @@ int crt_autotune_batched(const crt_batch* batches, int nbatches, const crt_batch
     if (candidates && ncandidates > 0) {
         cands.assign(candidates, candidates + ncandidates);
     } else {
+        // child order matters only to any-hit walks of the wide formats
+        bool orders = false;
+        for (int i = 0; i < nbatches; ++i)
+            orders |= batches[i].query == CRT_QUERY_ANY &&
+                      (batches[i].scene.format == CRT_FORMAT_WIDE4 || batches[i].scene.format == CRT_FORMAT_WIDE8);
         for (int v = 0; v < 6; ++v)
             for (int t : {1, 2, 4})
                 for (int w : {1, 4})
                     for (int o : {0, 1})
-                        for (int q : {1, crt::kQueueShards}) {
-                            crt_batch_config c{};
-                            c.variant = v;
-                            c.tiles_per_claim = t;
-                            c.waves_per_group = w;
-                            c.order = o;
-                            c.shards = q;
-                            cands.push_back(c);
-                        }
+                        for (int q : {1, crt::kQueueShards})
+                            for (int co = 0; co <= (orders ? 1 : 0); ++co) {
+                                crt_batch_config c{};
+                                c.variant = v;
+                                c.tiles_per_claim = t;
+                                c.waves_per_group = w;
+                                c.order = o;
+                                c.shards = q;
+                                c.child_order = co;
+                                cands.push_back(c);
+                            }
     }
     hipStream_t s = static_cast<hipStream_t>(stream);
     hipEvent_t e0 = nullptr, e1 = nullptr;

@@ -193,7 +193,7 @@ struct PathWave {
         const float4* __restrict__ rec = reinterpret_cast<const float4*>(s.tri_records);
         for (int k = 0; k < INNER; ++k) {
             if (active && !walk_done && !leafing) {
-                const int32_t r = c.step(s, max_iter, st);
+                const int32_t r = c.step(s, max_iter, st, false);
                 if (r == kEnd) {
                     walk_done = true;
                 } else if (r == kLeaves) {

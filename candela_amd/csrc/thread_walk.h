@@ -171,14 +171,15 @@ __device__ __forceinline__ uint32_t walk_stack(const SceneArgs& s, f3 ro, f3 rd,
 template <int FORMAT, bool ANY, int LDS_DEPTH>
 __device__ __forceinline__ uint32_t walk_entity(const SceneArgs& s, f3 ro, f3 rd, int32_t start, int32_t count,
                                                 float tmax, int32_t max_iter, Walk& w, uint32_t& status,
-                                                int32_t* lds, int32_t lds_stride) {
+                                                int32_t* lds, int32_t lds_stride, bool slot_order) {
     if constexpr (FORMAT == CRT_FORMAT_STACKLESS)
         return walk_stackless<ANY>(s, ro, rd, start, count, tmax, max_iter, w, status);
     else if constexpr (FORMAT == CRT_FORMAT_STACK)
         return walk_stack<ANY, LDS_DEPTH>(s, ro, rd, start, count, tmax, max_iter, w, status, lds, lds_stride);
     else
         return wide::walk_wide_thread<ANY, FORMAT == CRT_FORMAT_WIDE8 ? 8 : 4, LDS_DEPTH>(
-            s, ro, rd, start, count, tmax, max_iter, w.t, w.tri, w.mesh, w.tests, status, lds, lds_stride);
+            s, ro, rd, start, count, tmax, max_iter, w.t, w.tri, w.mesh, w.tests, status, lds, lds_stride,
+            slot_order);
 }
 
 
@@ -203,7 +204,8 @@ __device__ __forceinline__ crt_hit scene_trace(const SceneArgs& s, const crt_ray
             const f3 lro = xform(e.inverse, ro, 1.0f);
             const f3 lrd = xform(e.inverse, rd, 0.0f);
             visits += walk_entity<FORMAT, true, LDS_DEPTH>(s, lro, lrd, e.node_offset, e.node_count, ray.tmax,
-                                                           max_iter, w, status, lds, lds_stride);
+                                                           max_iter, w, status, lds, lds_stride,
+                                                           (flags & CRT_FLAG_SLOT_ORDER) != 0);
             if (w.t > 0.0f) {
                 tuvw.x = w.t;
                 tri = w.tri;
@@ -221,7 +223,7 @@ __device__ __forceinline__ crt_hit scene_trace(const SceneArgs& s, const crt_ray
             const f3 lro = xform(e.inverse, ro, 1.0f);
             const f3 lrd = xform(e.inverse, rd, 0.0f);
             visits += walk_entity<FORMAT, false, LDS_DEPTH>(s, lro, lrd, e.node_offset, e.node_count, tmax,
-                                                            max_iter, w, status, lds, lds_stride);
+                                                            max_iter, w, status, lds, lds_stride, false);
             if (w.t > 0.0f && w.t < tmax) {
                 tmax = w.t;
                 closest = w.t;

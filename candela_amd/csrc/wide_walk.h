@@ -16,12 +16,13 @@
 //             the exit distance and tmax grown by kWideGrow; hit leaves are
 //             appended to the pending list in slot order; hit inner children
 //             are sorted by entry distance with a fixed compare-exchange
-//             network (sort_children) and pushed farthest first, all but the
-//             nearest
-//   leaves    with pending leaves the nearest child is pushed too (on top),
+//             network and pushed farthest first, all but the nearest; any-hit
+//             walks with CRT_FLAG_SLOT_ORDER skip the sort: the first hit child
+//             in slot order comes first, the others are pushed last slot first
+//   leaves    with pending leaves the first child is pushed too (on top),
 //             the pending triangles are tested in slot order, and the next
 //             node is popped (with the tmax those tests produced); without,
-//             the walk goes to the nearest child, else pops
+//             the walk goes to the first child, else pops
 //   overflow  pushes that would pass LDS_DEPTH + 64 entries end the walk with
 //             CRT_STATUS_STACK_OVERFLOW before any leaf of that node is tested
 //
@@ -128,7 +129,10 @@ struct Core {
 
     // Pop, guards, one node.  kNext: cur holds the next node; kLeaves: np
     // pending leaves to test before the next step; kEnd: the walk is over.
-    __device__ __forceinline__ int32_t step(const SceneArgs& s, int32_t max_iter, Stack<LDS_DEPTH>& st) {
+    // slot_order (any hit only, CRT_FLAG_SLOT_ORDER): hit children in slot
+    // order instead of nearest first.
+    __device__ __forceinline__ int32_t step(const SceneArgs& s, int32_t max_iter, Stack<LDS_DEPTH>& st,
+                                            bool slot_order) {
         if (cur < 0) {
             bool found = false;
             while (sp > 0) {
@@ -202,8 +206,53 @@ struct Core {
     }
         CRT_WIDE_GROUP(0, t0, n0, t1, n1, t2, n2, t3, n3)
         if constexpr (W == 8) CRT_WIDE_GROUP(1, t4, n4, t5, n5, t6, n6, t7, n7)
-        // ascending entry distance: W = 4 a 5-exchange network, W = 8 Batcher's
-        // odd-even merge sort (19 exchanges)
+#undef CRT_WIDE_GROUP
+#undef CRT_WIDE_CHILD
+        const int32_t m = (n0 >= 0) + (n1 >= 0) + (n2 >= 0) + (n3 >= 0) + (n4 >= 0) + (n5 >= 0) + (n6 >= 0) +
+                          (n7 >= 0);
+        // with leaves pending the first child is pushed too (on top)
+        const int32_t pushes = np ? m : (m > 0 ? m - 1 : 0);
+        if (sp + pushes > kCap) {
+            status |= CRT_STATUS_STACK_OVERFLOW;
+            np = 0;
+            return kEnd;
+        }
+        if (ANY && slot_order) {
+            // slot order: the first hit child in slot order comes first, the
+            // others are pushed last slot first
+            int32_t f = -1, nf = -1;
+            if constexpr (W == 8) {
+                f = n7 >= 0 ? 7 : f;
+                f = n6 >= 0 ? 6 : f;
+                f = n5 >= 0 ? 5 : f;
+                f = n4 >= 0 ? 4 : f;
+            }
+            f = n3 >= 0 ? 3 : f;
+            f = n2 >= 0 ? 2 : f;
+            f = n1 >= 0 ? 1 : f;
+            f = n0 >= 0 ? 0 : f;
+            const int32_t skip = np ? -1 : f;
+            if constexpr (W == 8) {
+                if (n7 >= 0 && skip != 7) st.put(sp++, n7, t7);
+                if (n6 >= 0 && skip != 6) st.put(sp++, n6, t6);
+                if (n5 >= 0 && skip != 5) st.put(sp++, n5, t5);
+                if (n4 >= 0 && skip != 4) st.put(sp++, n4, t4);
+                nf = f == 4 ? n4 : f == 5 ? n5 : f == 6 ? n6 : f == 7 ? n7 : nf;
+            }
+            if (n3 >= 0 && skip != 3) st.put(sp++, n3, t3);
+            if (n2 >= 0 && skip != 2) st.put(sp++, n2, t2);
+            if (n1 >= 0 && skip != 1) st.put(sp++, n1, t1);
+            if (n0 >= 0 && skip != 0) st.put(sp++, n0, t0);
+            nf = f == 0 ? n0 : f == 1 ? n1 : f == 2 ? n2 : f == 3 ? n3 : nf;
+            if (np) {
+                cur = -1;
+                return kLeaves;
+            }
+            cur = nf;   // -1 (no inner child hit): pop first
+            return kNext;
+        }
+        // nearest first: ascending entry distance by a fixed network, W = 4
+        // five exchanges, W = 8 Batcher's odd-even merge sort (19 exchanges)
         if constexpr (W == 4) {
             CRT_CX(t0, n0, t1, n1) CRT_CX(t2, n2, t3, n3) CRT_CX(t0, n0, t2, n2) CRT_CX(t1, n1, t3, n3)
             CRT_CX(t1, n1, t2, n2)
@@ -216,17 +265,6 @@ struct Core {
             CRT_CX(t1, n1, t2, n2) CRT_CX(t3, n3, t4, n4) CRT_CX(t5, n5, t6, n6)
         }
 #undef CRT_CX
-#undef CRT_WIDE_GROUP
-#undef CRT_WIDE_CHILD
-        const int32_t m = (n0 >= 0) + (n1 >= 0) + (n2 >= 0) + (n3 >= 0) + (n4 >= 0) + (n5 >= 0) + (n6 >= 0) +
-                          (n7 >= 0);
-        // with leaves pending the nearest child is pushed too (on top)
-        const int32_t pushes = np ? m : (m > 0 ? m - 1 : 0);
-        if (sp + pushes > kCap) {
-            status |= CRT_STATUS_STACK_OVERFLOW;
-            np = 0;
-            return kEnd;
-        }
         const int32_t lowest = np ? 0 : 1;   // sorted index pushed last
         // farthest first, so that pops come nearest first
         if (W == 8) {
@@ -270,7 +308,7 @@ template <bool ANY, int W, int LDS_DEPTH>
 __device__ __forceinline__ uint32_t walk_wide_thread(const SceneArgs& s, f3 ro, f3 rd, int32_t root, int32_t count,
                                                      float tmax, int32_t max_iter, float& out_t, int32_t& out_tri,
                                                      int32_t& out_mesh, uint32_t& tests, uint32_t& status,
-                                                     int32_t* lds, int32_t stride) {
+                                                     int32_t* lds, int32_t stride, bool slot_order) {
     Core<ANY, W, LDS_DEPTH> c;
     Stack<LDS_DEPTH> st;
     int32_t spill_node[kSpill];
@@ -287,7 +325,7 @@ __device__ __forceinline__ uint32_t walk_wide_thread(const SceneArgs& s, f3 ro, 
     const float4* __restrict__ rec = reinterpret_cast<const float4*>(s.tri_records);
     bool done = false;
     while (!done) {
-        const int32_t r = c.step(s, max_iter, st);
+        const int32_t r = c.step(s, max_iter, st, slot_order);
         if (r == kEnd) break;
         if (r == kLeaves) {
             for (int32_t j = 0; j < c.np && !done; ++j) {
@@ -404,10 +442,11 @@ struct WideWave {
                                           crt_hit* __restrict__ hits, uint32_t flags, int32_t max_iter,
                                           uint32_t* __restrict__ tri_tests, Stack<LDS_DEPTH>& st) {
         const float4* __restrict__ rec = reinterpret_cast<const float4*>(s.tri_records);
+        const bool slot_order = (flags & CRT_FLAG_SLOT_ORDER) != 0;
         // --- node phase ----------------------------------------------------------
         for (int k = 0; k < INNER; ++k) {
             if (active && !walk_done && !leafing) {
-                const int32_t r = c.step(s, max_iter, st);
+                const int32_t r = c.step(s, max_iter, st, slot_order);
                 if (r == kEnd) {
                     walk_done = true;
                 } else if (r == kLeaves) {

@@ -138,7 +138,12 @@ enum {
     /* Reproduce the scene-level test `TriangleIdx > 0`
      * (TraverseBVHStackless.glsl:304): a closest hit on flattened triangle 0 is
      * reported with t=u=v=w=-1 (tri/mesh/entity still set). */
-    CRT_FLAG_TRI0_IS_MISS = 1u << 1
+    CRT_FLAG_TRI0_IS_MISS = 1u << 1,
+    /* Child ordering of any-hit walks of the wide layouts (no reference
+     * counterpart): the hit children of a node are taken in slot order
+     * instead of nearest first, which saves the sort.  Closest-hit walks and
+     * the BVH2 formats ignore it. */
+    CRT_FLAG_SLOT_ORDER = 1u << 2
 };
 
 /* Per-ray status flags (hit.status bits 24-31) */
@@ -679,6 +684,9 @@ typedef struct crt_batch_config {
     int32_t order;          /* 0 = batch order, 1 = heaviest tiles first (cost estimate) */
     int32_t shards;         /* tile-queue counters: 0 or 1 = one, 16 = sixteen interleaved shards
                                (shard k holds tiles k, k + 16, ...) */
+    int32_t child_order;    /* 0 = nearest child first; 1 = the any-hit batches walk children in
+                               slot order (CRT_FLAG_SLOT_ORDER added to their flags; the wide
+                               formats honour it, the stackless format has no choice to make) */
 } crt_batch_config;
 
 typedef struct crt_batch_plan crt_batch_plan;
@@ -701,7 +709,8 @@ int64_t crt_batch_plan_rays(const crt_batch_plan* plan);
  * each, after one warm-up) and returns the fastest in *best and its time per
  * launch in *best_ms.  candidates == NULL searches the built-in grid:
  * variants 0..5 x tiles_per_claim {1,2,4} x waves_per_group {1,4} x order {0,1}
- * x shards {1,16}. */
+ * x shards {1,16}, x child_order {0,1} when the batches use a wide format and
+ * hold any-hit queries. */
 int crt_autotune_batched(const crt_batch* batches, int nbatches, const crt_batch_config* candidates,
                          int ncandidates, int repeats, void* stream, crt_batch_config* best, float* best_ms);
 

@@ -180,6 +180,7 @@ typedef struct {
     const crt_tri_record* recs; /* wide formats */
     int32_t stack_cap;          /* wide formats: stack entries before STACK_OVERFLOW */
     int32_t max_iter;
+    int32_t slot_order;         /* wide formats, any hit: CRT_FLAG_SLOT_ORDER */
 } scene_t;
 
 typedef struct {
@@ -362,10 +363,12 @@ static inline int test_records(const scene_t* s, v3 ro, v3 rd, int32_t start, in
  *            closer-by-ulps hit); hit leaves are appended to the pending list
  *            in slot order; hit inner children are sorted by entry distance
  *            with the kernel's compare-exchange network and pushed farthest
- *            first, all but the nearest;
- *   leaves   with pending leaves the nearest child is pushed too (on top), the
+ *            first, all but the nearest (any hit with CRT_FLAG_SLOT_ORDER:
+ *            no sort, the first hit child in slot order comes first and the
+ *            others are pushed last slot first);
+ *   leaves   with pending leaves the first child is pushed too (on top), the
  *            pending triangles are tested in slot order and the next node is
- *            popped; otherwise the walk goes to the nearest child, else pops;
+ *            popped; otherwise the walk goes to the first child, else pops;
  *   overflow pushes that would pass stack_cap entries end the walk before any
  *            leaf of that node is tested. */
 #define WIDE_GROW (1.0f + 1.0f / 524288.0f)
@@ -441,16 +444,6 @@ static void walk_wide(const scene_t* s, v3 ro, v3 rd, int32_t root, int32_t coun
                 cn[4 * g + k] = inner ? gp->child[k] : -1;
             }
         }
-        /* ascending entry distance by the kernel's compare-exchange networks */
-        if (G == 1) {
-            static const int net4[5][2] = {{0, 1}, {2, 3}, {0, 2}, {1, 3}, {1, 2}};
-            for (int e = 0; e < 5; ++e) cx(ct, cn, net4[e][0], net4[e][1]);
-        } else {
-            static const int net8[19][2] = {{0, 1}, {2, 3}, {4, 5}, {6, 7}, {0, 2}, {1, 3}, {4, 6},
-                                            {5, 7}, {1, 2}, {5, 6}, {0, 4}, {1, 5}, {2, 6}, {3, 7},
-                                            {2, 4}, {3, 5}, {1, 2}, {3, 4}, {5, 6}};
-            for (int e = 0; e < 19; ++e) cx(ct, cn, net8[e][0], net8[e][1]);
-        }
         int m = 0;
         for (int j = 0; j < 4 * G; ++j) m += cn[j] >= 0;
         int pushes = npend ? m : (m > 0 ? m - 1 : 0);
@@ -458,13 +451,41 @@ static void walk_wide(const scene_t* s, v3 ro, v3 rd, int32_t root, int32_t coun
             w->status |= CRT_STATUS_STACK_OVERFLOW;
             break;
         }
-        int lowest = npend ? 0 : 1;
-        for (int j = 4 * G - 1; j >= lowest; --j) {
-            if (j < m) {
-                st_node[sp] = cn[j];
-                st_t[sp] = ct[j];
-                ++sp;
+        int32_t next = -1;
+        if (any && s->slot_order) {
+            /* slot order: the first hit child comes first, the others are
+               pushed last slot first */
+            int first = -1;
+            for (int j = 4 * G - 1; j >= 0; --j)
+                if (cn[j] >= 0) first = j;
+            for (int j = 4 * G - 1; j >= 0; --j) {
+                if (cn[j] >= 0 && (npend || j != first)) {
+                    st_node[sp] = cn[j];
+                    st_t[sp] = ct[j];
+                    ++sp;
+                }
             }
+            next = first >= 0 ? cn[first] : -1;
+        } else {
+            /* ascending entry distance by the kernel's compare-exchange networks */
+            if (G == 1) {
+                static const int net4[5][2] = {{0, 1}, {2, 3}, {0, 2}, {1, 3}, {1, 2}};
+                for (int e = 0; e < 5; ++e) cx(ct, cn, net4[e][0], net4[e][1]);
+            } else {
+                static const int net8[19][2] = {{0, 1}, {2, 3}, {4, 5}, {6, 7}, {0, 2}, {1, 3}, {4, 6},
+                                                {5, 7}, {1, 2}, {5, 6}, {0, 4}, {1, 5}, {2, 6}, {3, 7},
+                                                {2, 4}, {3, 5}, {1, 2}, {3, 4}, {5, 6}};
+                for (int e = 0; e < 19; ++e) cx(ct, cn, net8[e][0], net8[e][1]);
+            }
+            int lowest = npend ? 0 : 1;
+            for (int j = 4 * G - 1; j >= lowest; --j) {
+                if (j < m) {
+                    st_node[sp] = cn[j];
+                    st_t[sp] = ct[j];
+                    ++sp;
+                }
+            }
+            next = cn[0];
         }
         if (npend) {
             for (int j = 0; j < npend; ++j) {
@@ -475,7 +496,7 @@ static void walk_wide(const scene_t* s, v3 ro, v3 rd, int32_t root, int32_t coun
             }
             cur = -1;
         } else {
-            cur = cn[0];
+            cur = next;
         }
     }
     w->iters += (uint32_t)it;
@@ -659,6 +680,7 @@ int oracle_trace_wide(int32_t width, int32_t entity_count, int64_t total_nodes, 
     s.ents = ents;
     s.stack_cap = stack_cap > 0 ? stack_cap : 80;
     s.max_iter = max_iter > 0 ? max_iter : MAX_ITER_DEFAULT;
+    s.slot_order = (flags & CRT_FLAG_SLOT_ORDER) != 0;
     return run_scene(&s, query, flags, rays, hits, n, threads, tri_tests);
 }
 

@@ -40,6 +40,8 @@ def test_struct_sizes(crt):
     assert crt.HIT_DTYPE.itemsize == 32
     import ctypes
     assert ctypes.sizeof(crt.CacheHeader) == 256 and crt.CacheHeader.header_hash.offset == 248
+    assert ctypes.sizeof(crt.BatchConfig) == 28 and crt.BatchConfig.child_order.offset == 24
+    assert (crt.FLAG_IGNORE_TRANSPARENT, crt.FLAG_TRI0_IS_MISS, crt.FLAG_SLOT_ORDER) == (1, 2, 4)
 
 
 def test_errors_without_gpu(crt):

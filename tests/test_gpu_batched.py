@@ -120,34 +120,60 @@ def test_autotune_returns_a_working_config(work, orc):
     check_all(work, orc, "autotuned config")
 
 
-def wide_oracle_hits(orc, s, layout, which):
+def wide_oracle_hits(orc, s, layout, which, child_order=0):
+    """Host wide walk of one batch; child_order 1 walks the any-hit batch with CRT_FLAG_SLOT_ORDER."""
+    import candela_amd as crt
     w = s.wide[layout].host
     rays, q = (s.closest, 0) if which == "closest" else (s.shadow, 1)
-    return orc.trace_wide(w.width, w.groups, w.records, w.entities, q, rays)
+    flags = crt.FLAG_SLOT_ORDER if child_order == 1 and q == 1 else 0
+    return orc.trace_wide(w.width, w.groups, w.records, w.entities, q, rays, flags=flags)
 
 
 @pytest.mark.parametrize("width", [4, 8])
 def test_batched_wide(work, orc, width):
-    """The tile queue over the BVH4 / BVH8 layouts: every preset, with and without graph replay."""
+    """The tile queue over the BVH4 / BVH8 layouts: every preset, both child orders, with and without graph
+    replay."""
     import candela_amd as crt
     layout = (width, 2)
     work.build_wide(crt, width, 2)
-    for variant, tpc, wpg, order, shards in ((0, 1, 4, 1, 16), (1, 2, 1, 0, 1), (2, 4, 2, 1, 16), (3, 1, 4, 0, 0),
-                                             (4, 2, 4, 1, 16), (5, 1, 1, 1, 16)):
+    for variant, tpc, wpg, order, shards, co in ((0, 1, 4, 1, 16, 0), (1, 2, 1, 0, 1, 1), (2, 4, 2, 1, 16, 0),
+                                                 (3, 1, 4, 0, 0, 1), (4, 2, 4, 1, 16, 1), (5, 1, 1, 1, 16, 0)):
         for s in work.scenes:
             s.h_closest.fill_bytes(0xAB)
             s.h_shadow.fill_bytes(0xAB)
-        plan = crt.BatchPlan(work.batches(crt, layout), crt.BatchConfig(variant, tpc, wpg, 0, order, shards))
+        plan = crt.BatchPlan(work.batches(crt, layout), crt.BatchConfig(variant, tpc, wpg, 0, order, shards, co))
         for replay in range(2):
             plan.launch(graph=replay == 1)
             crt.lib().crt_device_sync()
             for s in work.scenes:
                 for which, h in (("closest", s.h_closest), ("shadow", s.h_shadow)):
-                    got, ref = h.download(), wide_oracle_hits(orc, s, layout, which)
-                    assert got.tobytes() == ref.tobytes(), f"wide{width} v{variant} scene {s.index} {which}"
-        log(f"[parity] batched wide{width} variant={variant} tpc={tpc} wpg={wpg} order={order} shards={shards}: "
-            f"{work.rays} rays bit-identical to the host wide walk (eager and graph)")
-    # mixing node formats in one plan is rejected
+                    got, ref = h.download(), wide_oracle_hits(orc, s, layout, which, co)
+                    assert got.tobytes() == ref.tobytes(), f"wide{width} v{variant} co{co} scene {s.index} {which}"
+        log(f"[parity] batched wide{width} variant={variant} tpc={tpc} wpg={wpg} order={order} shards={shards} "
+            f"child_order={co}: {work.rays} rays bit-identical to the host wide walk (eager and graph)")
+    # mixing node formats in one plan is rejected, and so is an unknown child order
     mixed = work.batches(crt, layout)[:2] + work.batches(crt)[2:4]
     with pytest.raises(crt.CrtError):
         crt.BatchPlan(mixed)
+    with pytest.raises(crt.CrtError):
+        crt.BatchPlan(work.batches(crt, layout), crt.BatchConfig(0, 1, 4, 0, 1, 16, 2))
+
+
+def test_autotune_wide_child_order(work, orc):
+    """The built-in candidate grid over a wide layout with any-hit batches includes both child orders; the
+    returned config reproduces the host walk with its child order."""
+    import candela_amd as crt
+    layout = (4, 2)
+    work.build_wide(crt, 4, 2)
+    cands = [crt.BatchConfig(v, 2, 4, 0, 1, 16, co) for v in (0, 2) for co in (0, 1)]
+    best, ms = crt.autotune_batched(work.batches(crt, layout), cands, repeats=1)
+    log(f"[autotune] wide4 best variant={best.variant} child_order={best.child_order} {ms:.3f} ms")
+    assert ms > 0 and best.child_order in (0, 1)
+    for s in work.scenes:
+        s.h_closest.fill_bytes(0xAB)
+        s.h_shadow.fill_bytes(0xAB)
+    crt.BatchPlan(work.batches(crt, layout), best).launch()
+    crt.lib().crt_device_sync()
+    for s in work.scenes:
+        for which, h in (("closest", s.h_closest), ("shadow", s.h_shadow)):
+            assert h.download().tobytes() == wide_oracle_hits(orc, s, layout, which, best.child_order).tobytes()

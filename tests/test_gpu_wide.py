@@ -119,6 +119,31 @@ def test_wide_wave_presets(gpu, orc, width):
                     f"wide{width} refill v{v} grid={grid} q={q}")
 
 
+@pytest.mark.parametrize("width", [4, 8])
+def test_wide_slot_order(gpu, orc, width):
+    """Any-hit walks with CRT_FLAG_SLOT_ORDER (children in slot order) on every wide kernel and preset against
+    the host twin with the same flag; closest-hit launches ignore the flag."""
+    from candela_amd import scenes
+    log(f"[case] wide slot order width={width}")
+    meshes = scenes.icosphere_cloud(count=150, subdivisions=3, seed=33, box=35.0)
+    r = _wide_context(gpu, 0, meshes, width, 2)
+    hb, w = _host_wide(gpu, r, width, 2)
+    rays = np.concatenate([random_rays(40000, [-2, -2, -2], [37, 37, 37], seed=34),
+                           random_rays(20000, [-2, -2, -2], [37, 37, 37], seed=35, tmax=2.0)])
+    scene = r.wide_scene()
+    f = gpu.FLAG_SLOT_ORDER
+    o = orc.trace_wide(width, w.groups, w.records, w.entities, 1, rays, flags=f, stack_cap=8 + 64)
+    assert not np.array_equal(o, orc.trace_wide(width, w.groups, w.records, w.entities, 1, rays, stack_cap=8 + 64))
+    compare(gpu_trace(gpu, scene, 1, rays, kernel=1, flags=f), o, f"wide{width} thread slot order")
+    for v in range(10):
+        compare(gpu_trace(gpu, scene, 1, rays, kernel=3, variant=v, flags=f), o, f"wide{width} wave v{v} slot order")
+    for v, grid in ((0, 0), (2, 5), (6, 0), (8, 0), (9, 0)):
+        compare(gpu_trace(gpu, scene, 1, rays, kernel=2, variant=v, grid=grid, flags=f), o,
+                f"wide{width} refill v{v} grid={grid} slot order")
+    oc = orc.trace_wide(width, w.groups, w.records, w.entities, 0, rays, stack_cap=8 + 64)
+    compare(gpu_trace(gpu, scene, 0, rays, kernel=2, variant=8, flags=f), oc, f"wide{width} closest ignores it")
+
+
 @pytest.mark.parametrize("block,lds", [(64, 8), (256, 16), (128, 32), (512, 8)])
 def test_wide_launch_shapes(gpu, orc, block, lds):
     from candela_amd import scenes
@@ -150,7 +175,7 @@ def test_wide_instances_and_flags(gpu, orc):
     w = gpu.wide_build(0, hb["nodes"], hb["triangles"], hb["vertices"], hb["entities"], 4, 2)
     rays = random_rays(40000, [-2, 0.2, -2], [18, 14, 18], seed=21)
     for q in (0, 1):
-        for flags in (0, 1, 2):
+        for flags in (0, 1, 2, gpu.FLAG_SLOT_ORDER, gpu.FLAG_SLOT_ORDER | 1):
             g = gpu_trace(gpu, r.wide_scene(), q, rays, flags=flags)
             o = orc.trace_wide(4, w.groups, w.records, w.entities, q, rays, flags=flags)
             compare(g, o, f"wide4 instances q={q} flags={flags}")

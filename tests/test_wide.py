@@ -263,3 +263,49 @@ def test_single_leaf_root(crt, orc):
     ray["dir"] = [0, 0, -1]
     h = orc.trace_wide(8, w.groups, w.records, w.entities, 0, ray)
     assert h["t"][0] == 1.0 and h["tri"][0] == 0 and h["mesh"][0] == 7
+
+
+def test_slot_order_any_hit(crt, orc):
+    """CRT_FLAG_SLOT_ORDER: any-hit walks take the hit children in slot order instead of nearest first.
+
+    Four spheres on the x axis hang under one BVH4 root, one per slot.  A ray through all four, started beyond
+    the far end from the root's slot-0 sphere, reports a hit in that sphere (the farthest one) with the flag,
+    and in the nearest sphere without it.  Closest-hit walks ignore the flag, and any hit within tmax still
+    exists exactly when a closest hit does."""
+    from candela_amd import scenes
+    from oracle_cases import RAY_DTYPE
+    pos, faces = scenes.icosphere(1)
+    meshes = [(scenes.make_vertices(pos + np.array([10.0 * k, 0.0, 0.0])), faces, k) for k in range(4)]
+    _, hb = _context(crt, 0, meshes)
+    w = crt.wide_build(0, hb["nodes"], hb["triangles"], hb["vertices"], hb["entities"], 4, 2)
+    g = w.groups[int(w.entities["node_offset"][0])]
+    assert (g["count"] == 0).all(), "four inner children under the root"
+    cx = (g["lo_x"] + g["hi_x"]) / 2
+    assert sorted(np.round(cx).astype(int).tolist()) == [0, 10, 20, 30]
+    c0 = float(cx[0])
+    sign = 1.0 if c0 < 15.0 else -1.0
+    rays = np.zeros(3, RAY_DTYPE)
+    rays["org"] = [[15.0 + 35.0 * sign, 0.05 + 0.11 * k, 0.03 + 0.07 * k] for k in range(3)]
+    rays["dir"] = [-sign, 0.0, 0.0]
+    rays["tmax"] = 1e6
+    near = orc.trace_wide(4, w.groups, w.records, w.entities, 1, rays)
+    slot = orc.trace_wide(4, w.groups, w.records, w.entities, 1, rays, flags=crt.FLAG_SLOT_ORDER)
+    assert (near["t"] > 0).all() and (slot["t"] > 0).all()
+    x_near = rays["org"][:, 0] - sign * near["t"]
+    x_slot = rays["org"][:, 0] - sign * slot["t"]
+    assert (np.abs(x_near - (15.0 + 15.0 * sign)) <= 1.0).all()   # the nearest sphere
+    assert (np.abs(x_slot - c0) <= 1.0).all()                     # the slot-0 sphere, farthest along the ray
+    # closest hits ignore the flag; any-hit existence does not depend on the order
+    cloud = scenes.icosphere_cloud(count=80, subdivisions=2, seed=31, box=25.0)
+    _, hb = _context(crt, 0, cloud)
+    for width in (4, 8):
+        w = crt.wide_build(0, hb["nodes"], hb["triangles"], hb["vertices"], hb["entities"], width, 2)
+        rr = random_rays(20000, [-2, -2, -2], [27, 27, 27], seed=32, tmax=3.0)
+        c = orc.trace_wide(width, w.groups, w.records, w.entities, 0, rr)
+        c1 = orc.trace_wide(width, w.groups, w.records, w.entities, 0, rr, flags=crt.FLAG_SLOT_ORDER)
+        assert c.tobytes() == c1.tobytes()
+        a = orc.trace_wide(width, w.groups, w.records, w.entities, 1, rr, flags=crt.FLAG_SLOT_ORDER)
+        assert np.array_equal(a["t"] > 0, c["t"] > 0)
+        assert (a["t"][a["t"] > 0] < 3.0).all() and (a["t"][a["t"] > 0] >= c["t"][a["t"] > 0]).all()
+        a0 = orc.trace_wide(width, w.groups, w.records, w.entities, 1, rr)
+        assert not np.array_equal(a0["status"], a["status"])        # the visit counts change with the order
