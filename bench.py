@@ -751,12 +751,17 @@ def main(argv=None) -> int:
             twin_g = oracle.trace_wide(w.width, w.groups, w.records, w.entities, 0, gi, threads=threads)
             twin_a = oracle.trace_wide(w.width, w.groups, w.records, w.entities, 1, ao, threads=threads,
                                        flags=ao_flags(crt, best))
+        gi_miss = g["tri"] != ohg["tri"]
         parity = {"rays": int(g.size + a.size),
                   "exact": float((np.concatenate([g, a]).tobytes() == np.concatenate([twin_g, twin_a]).tobytes())),
-                  "id_agreement_vs_reference_walk": float(np.mean(np.concatenate([g["tri"] == ohg["tri"],
-                                                                                  a["tri"] == oha["tri"]]))),
-                  "t_agreement_vs_reference_walk": float(np.mean(np.concatenate([g["t"] == ohg["t"],
-                                                                                 (a["t"] > 0) == (oha["t"] > 0)])))}
+                  # closest hits: the triangle is unique up to exact distance ties (shared edges and vertices)
+                  "gi_id_agreement_vs_reference_walk": float(1.0 - gi_miss.mean()),
+                  "gi_id_mismatches": int(gi_miss.sum()),
+                  "gi_id_mismatches_at_equal_t": int((gi_miss & (g["t"] == ohg["t"])).sum()),
+                  "gi_t_agreement_vs_reference_walk": float(np.mean(g["t"] == ohg["t"])),
+                  # any hits: occluded or not is fixed, which occluder is found first depends on the walk order
+                  "ao_occlusion_agreement_vs_reference_walk": float(np.mean((a["t"] > 0) == (oha["t"] > 0))),
+                  "ao_first_occluder_same_as_reference_walk": float(np.mean(a["tri"] == oha["tri"]))}
         log(f"cpu baseline {cpu_value:.3f} Mrays/s on {threads} threads, {cpu_1:.3f} Mrays/s on 1 thread; "
             f"parity {parity}")
 

@@ -44,13 +44,37 @@ struct BatchDesc {
 
 namespace pdev {
 
+// Pointers read from a descriptor in memory are generic to the compiler, so
+// every access through them would be a flat instruction, which also counts
+// against the LDS wait counter.  All of them point to device memory: rebuilt
+// from their address as global-segment pointers, they give global loads and
+// stores (a plain cast to the global segment and back would be folded away).
+template <class T>
+__device__ __forceinline__ T* global_ptr(T* p) {
+    return (T*)(__attribute__((address_space(1))) T*)reinterpret_cast<uintptr_t>(p);
+}
+
+__device__ __forceinline__ BatchDesc load_desc(const BatchDesc* __restrict__ descs, int32_t b) {
+    BatchDesc d = descs[b];
+    d.scene.nodes = global_ptr(d.scene.nodes);
+    d.scene.tris = global_ptr(d.scene.tris);
+    d.scene.verts = global_ptr(d.scene.verts);
+    d.scene.entities = global_ptr(d.scene.entities);
+    d.scene.tri_records = global_ptr(d.scene.tri_records);
+    d.rays = global_ptr(d.rays);
+    d.hits = global_ptr(d.hits);
+    d.tri_tests = global_ptr(d.tri_tests);
+    return d;
+}
+
 template <int INNER, int LEAF_BATCH, int K, int WPG>
 __global__ void __launch_bounds__(64 * WPG) batched_stackless(const BatchDesc* __restrict__ descs,
                                                               const int2* __restrict__ tiles, int32_t ntiles,
                                                               int32_t per_claim, unsigned long long* __restrict__ counters,
                                                               int32_t max_iter, int32_t shards) {
-    WaveWalk<false, INNER, LEAF_BATCH, K> closest;
-    WaveWalk<true, INNER, LEAF_BATCH, K> any;
+    // one walk for both queries (the query is a per-tile value): two inlined
+    // walks would need the sum of their registers
+    WaveWalk<false, INNER, LEAF_BATCH, K, true> w;
     ShardQueue q;
     q.init(counters, ntiles, true, shards);
     while (true) {
@@ -62,13 +86,11 @@ __global__ void __launch_bounds__(64 * WPG) batched_stackless(const BatchDesc* _
             const int2 tile = tiles[t];
             const int32_t b = __builtin_amdgcn_readfirstlane(tile.x);
             const int32_t first = __builtin_amdgcn_readfirstlane(tile.y);
-            const BatchDesc d = descs[b];
+            const BatchDesc d = load_desc(descs, b);
             const int64_t i = int64_t(first) + __lane_id();
             const int64_t slot = i < d.count ? i : -1;
-            if (d.query == CRT_QUERY_ANY)
-                any.run(d.scene, d.rays, d.hits, slot, d.flags, max_iter, d.tri_tests);
-            else
-                closest.run(d.scene, d.rays, d.hits, slot, d.flags, max_iter, d.tri_tests);
+            w.dyn_any = d.query == CRT_QUERY_ANY;
+            w.run(d.scene, d.rays, d.hits, slot, d.flags, max_iter, d.tri_tests);
         }
     }
 }
@@ -100,17 +122,14 @@ __global__ void __launch_bounds__(64 * WPG) batched_wide(const BatchDesc* __rest
             const int2 tile = tiles[t];
             const int32_t b = __builtin_amdgcn_readfirstlane(tile.x);
             const int32_t first = __builtin_amdgcn_readfirstlane(tile.y);
-            const BatchDesc d = descs[b];
+            const BatchDesc d = load_desc(descs, b);
             const int64_t i = int64_t(first) + __lane_id();
             const int64_t slot = i < d.count ? i : -1;
-            // one walk object per branch: their registers need not coexist
-            if (d.query == CRT_QUERY_ANY) {
-                wide::WideWave<true, W, LDS_DEPTH, INNER, LEAF_BATCH> w;
-                w.run(d.scene, d.rays, d.hits, slot, d.flags, max_iter, d.tri_tests, st);
-            } else {
-                wide::WideWave<false, W, LDS_DEPTH, INNER, LEAF_BATCH> w;
-                w.run(d.scene, d.rays, d.hits, slot, d.flags, max_iter, d.tri_tests, st);
-            }
+            // one walk for both queries (the query is a per-tile value): two
+            // inlined walks would need the sum of their registers
+            wide::WideWave<false, W, LDS_DEPTH, INNER, LEAF_BATCH, true> w;
+            w.c.dyn_any = d.query == CRT_QUERY_ANY;
+            w.run(d.scene, d.rays, d.hits, slot, d.flags, max_iter, d.tri_tests, st);
         }
     }
 }

@@ -15,6 +15,13 @@
 namespace crt {
 namespace dm {
 
+// An int in LDS.  Per-lane stacks address their LDS part through an
+// LDS-qualified pointer: with a generic one the compiler merges the LDS and
+// the scratch branch of a pop into one flat load (a select between the two
+// segment bases), and a flat load waits on every outstanding memory
+// operation.
+using LdsInt = __attribute__((address_space(3))) int32_t;
+
 struct f3 {
     float x, y, z;
 };

@@ -108,6 +108,7 @@ __device__ __forceinline__ uint32_t walk_stack(const SceneArgs& s, f3 ro, f3 rd,
     const f3 inv = mk(1.0f / rd.x, 1.0f / rd.y, 1.0f / rd.z);
     constexpr int kSpill = (64 - LDS_DEPTH) > 0 ? (64 - LDS_DEPTH) : 1;
     int32_t spill[kSpill];
+    LdsInt* const col = (LdsInt*)lds;   // see LdsInt
     int32_t sp = 0;
     int32_t it = 0;
     int32_t cur = start;
@@ -145,7 +146,7 @@ __device__ __forceinline__ uint32_t walk_stack(const SceneArgs& s, f3 ro, f3 rd,
                 break;
             }
             if (sp < LDS_DEPTH)
-                lds[sp * lds_stride] = post;
+                col[sp * lds_stride] = post;
             else
                 spill[sp - LDS_DEPTH] = post;
             sp++;
@@ -162,7 +163,7 @@ __device__ __forceinline__ uint32_t walk_stack(const SceneArgs& s, f3 ro, f3 rd,
             break;
         }
         --sp;
-        cur = sp < LDS_DEPTH ? lds[sp * lds_stride] : spill[sp - LDS_DEPTH];
+        cur = sp < LDS_DEPTH ? col[sp * lds_stride] : spill[sp - LDS_DEPTH];
     }
     if (!stopped) status |= CRT_STATUS_ITER_CAP;
     return uint32_t(it);

@@ -55,12 +55,17 @@ struct Lane {
     int32_t leaf_packed, leaf_link;
 };
 
-template <bool ANY, int INNER, int LEAF_BATCH, int K>
+// The query is ANY, or with DYN the member dyn_any, set before run (one
+// compiled walk serving both queries, for kernels that mix them: two inlined
+// walks would add up their registers).
+template <bool ANY, int INNER, int LEAF_BATCH, int K, bool DYN = false>
 struct WaveWalk {
     static_assert(K >= 1 && K <= 4, "window of 1 to 4 nodes");
 
     Lane L;
     bool active, parked, walk_done;
+    bool dyn_any;
+    __device__ __forceinline__ bool any() const { return DYN ? dyn_any : ANY; }
 
     // Next entity that takes part (IgnoreTransparent skips alpha < 0.99);
     // returns false when the entity loop is over.
@@ -81,7 +86,7 @@ struct WaveWalk {
         L.count = en.node_count;
         L.ptr = L.start;
         L.it = 0;
-        if (ANY) L.tmax = L.ray_tmax;
+        if (any()) L.tmax = L.ray_tmax;
         walk_done = L.ptr < 0;  // `while (Pointer >= 0 ...)` never runs
         return true;
     }
@@ -93,7 +98,7 @@ struct WaveWalk {
         const int32_t tri = L.tri, mesh = L.mesh, ent = L.hit_ent;
         // ClosestT > 0 exactly when a triangle was accepted; its t is then L.tmax
         const float closest = tri >= 0 ? L.tmax : -1.0f;
-        if (ANY) {
+        if (any()) {
             if (tri >= 0) tuvw.x = closest;
         } else {
             const bool ok = closest > 0.0f && ((flags & CRT_FLAG_TRI0_IS_MISS) ? tri > 0 : tri >= 0);
@@ -125,7 +130,7 @@ struct WaveWalk {
             L.mesh = tr.w;
             L.tri = idx;
             L.hit_ent = L.ent;
-            if (ANY) return true;
+            if (any()) return true;
         }
         return false;
     }
@@ -258,7 +263,7 @@ struct WaveWalk {
                     walk_done = false;
                     L.visits += uint32_t(L.it);
                     // any-hit: the first entity with a hit ends the scene loop
-                    bool fin = ANY && L.tri >= 0;
+                    bool fin = any() && L.tri >= 0;
                     if (!fin) fin = !begin_walk(s, rays, flags);
                     if (fin) {
                         finish(s, rays, hits, tri_tests, flags);

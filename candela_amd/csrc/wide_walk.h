@@ -75,13 +75,15 @@ struct Stack {
     int32_t* spill_node;
     float* spill_t;
 
-    __device__ __forceinline__ void put_leaf(int32_t j, int32_t v) { lds[(2 * LDS_DEPTH + j) * stride] = v; }
-    __device__ __forceinline__ int32_t leaf(int32_t j) const { return lds[(2 * LDS_DEPTH + j) * stride]; }
+    __device__ __forceinline__ LdsInt* col() const { return (LdsInt*)lds; }   // see LdsInt
+
+    __device__ __forceinline__ void put_leaf(int32_t j, int32_t v) { col()[(2 * LDS_DEPTH + j) * stride] = v; }
+    __device__ __forceinline__ int32_t leaf(int32_t j) const { return col()[(2 * LDS_DEPTH + j) * stride]; }
 
     __device__ __forceinline__ void put(int32_t sp, int32_t node, float t) {
         if (sp < LDS_DEPTH) {
-            lds[(2 * sp) * stride] = node;
-            lds[(2 * sp + 1) * stride] = __float_as_int(t);
+            col()[(2 * sp) * stride] = node;
+            col()[(2 * sp + 1) * stride] = __float_as_int(t);
         } else {
             spill_node[sp - LDS_DEPTH] = node;
             spill_t[sp - LDS_DEPTH] = t;
@@ -89,8 +91,8 @@ struct Stack {
     }
     __device__ __forceinline__ void get(int32_t sp, int32_t& node, float& t) const {
         if (sp < LDS_DEPTH) {
-            node = lds[(2 * sp) * stride];
-            t = __int_as_float(lds[(2 * sp + 1) * stride]);
+            node = col()[(2 * sp) * stride];
+            t = __int_as_float(col()[(2 * sp + 1) * stride]);
         } else {
             node = spill_node[sp - LDS_DEPTH];
             t = spill_t[sp - LDS_DEPTH];
@@ -98,11 +100,16 @@ struct Stack {
     }
 };
 
-// One ray's walk of one entity's wide tree.
-template <bool ANY, int W, int LDS_DEPTH>
+// One ray's walk of one entity's wide tree.  The query is ANY, or with DYN
+// the member dyn_any, set per walk (one compiled walk serving both queries,
+// for kernels that mix them: two inlined walks would add up their registers).
+template <bool ANY, int W, int LDS_DEPTH, bool DYN = false>
 struct Core {
     static constexpr int G = W / 4;
     static constexpr int32_t kCap = LDS_DEPTH + kSpill;
+
+    bool dyn_any;
+    __device__ __forceinline__ bool any() const { return DYN ? dyn_any : ANY; }
 
     f3 ro, rd, inv;
     float tmax;          // closest: shrinks with every accepted hit
@@ -140,7 +147,7 @@ struct Core {
                 int32_t node;
                 float t;
                 st.get(sp, node, t);
-                if (!ANY && t > tmax * kWideGrow) continue;
+                if (!any() && t > tmax * kWideGrow) continue;
                 cur = node;
                 found = true;
                 break;
@@ -217,7 +224,7 @@ struct Core {
             np = 0;
             return kEnd;
         }
-        if (ANY && slot_order) {
+        if (any() && slot_order) {
             // slot order: the first hit child in slot order comes first, the
             // others are pushed last slot first
             int32_t f = -1, nf = -1;
@@ -295,7 +302,7 @@ struct Core {
             tri = __float_as_int(a.w);
             mesh = __float_as_int(b.w);
             (void)idx;
-            if (ANY) return true;
+            if (any()) return true;
         }
         return false;
     }
@@ -361,9 +368,9 @@ __device__ __forceinline__ uint32_t walk_wide_thread(const SceneArgs& s, f3 ro, 
 //   walk end    next entity (the walk starts from the scene loop's TMax, so a
 //               hit the walk accepts always wins at scene level), or the hit
 //               record.
-template <bool ANY, int W, int LDS_DEPTH, int INNER, int LEAF_BATCH>
+template <bool ANY, int W, int LDS_DEPTH, int INNER, int LEAF_BATCH, bool DYN = false>
 struct WideWave {
-    Core<ANY, W, LDS_DEPTH> c;
+    Core<ANY, W, LDS_DEPTH, DYN> c;
     int32_t slot, ent, hit_ent;
     float ray_tmax;
     uint32_t visits;
@@ -381,7 +388,7 @@ struct WideWave {
         const float4* rp = reinterpret_cast<const float4*>(rays + slot);
         const float4 a = rp[0], b = rp[1];
         c.begin(xform(en.inverse, mk(a.x, a.y, a.z), 1.0f), xform(en.inverse, mk(b.x, b.y, b.z), 0.0f),
-                ANY ? ray_tmax : c.tmax, en.node_offset, en.node_count);
+                c.any() ? ray_tmax : c.tmax, en.node_offset, en.node_count);
         walk_done = false;
         return true;
     }
@@ -392,7 +399,7 @@ struct WideWave {
         float4 tuvw = make_float4(-1.0f, -1.0f, -1.0f, -1.0f);
         const int32_t tri = c.tri;
         const float closest = tri >= 0 ? c.tmax : -1.0f;
-        if (ANY) {
+        if (c.any()) {
             if (tri >= 0) tuvw.x = closest;
         } else {
             const bool ok = closest > 0.0f && ((flags & CRT_FLAG_TRI0_IS_MISS) ? tri > 0 : tri >= 0);
@@ -495,7 +502,7 @@ struct WideWave {
             if (walk_done) {
                 walk_done = false;
                 visits += uint32_t(c.it);
-                bool fin = ANY && c.tri >= 0;
+                bool fin = c.any() && c.tri >= 0;
                 if (!fin) fin = !begin_walk(s, rays, flags);
                 if (fin) {
                     finish(s, rays, hits, tri_tests, flags);
