@@ -210,13 +210,10 @@ struct PathWave {
             if (leafing) {
                 const int32_t first = (leaf >> 4) + off, left = (leaf & 0xF) - off;
                 const float4* p = rec + 3 * int64_t(first);
+                // the second record, or the first again when one is left
+                const float4* p1 = p + (left > 1 ? 3 : 0);
                 const float4 a0 = p[0], b0 = p[1], c0 = p[2];
-                float4 a1 = a0, b1 = b0, c1 = c0;
-                if (left > 1) {
-                    a1 = p[3];
-                    b1 = p[4];
-                    c1 = p[5];
-                }
+                const float4 a1 = p1[0], b1 = p1[1], c1 = p1[2];
                 const float before = c.tmax;
                 c.test(first, a0, b0, c0);
                 if (left > 1) c.test(first + 1, a1, b1, c1);
