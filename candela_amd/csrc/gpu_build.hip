@@ -223,32 +223,41 @@ __global__ void __launch_bounds__(kBlock) karras(const uint32_t* __restrict__ ke
     parent[right] = int32_t(i);
 }
 
-// Boxes bottom up.  Thread per leaf; flags[i] counts the children of internal node i that are done.
-__global__ void __launch_bounds__(kBlock) boxes_up(const crt_vertex* __restrict__ v, const uint32_t* __restrict__ idx,
+// Arrival at internal node i: flags[i] counts its children that are done;
+// the second arrival (old value 1) finishes the node.  Acquire-release at the
+// node's scope: a node whose leaf range lies in one workgroup's leaves has
+// both children computed in that workgroup, so its counter and its children's
+// boxes only need workgroup scope; the nodes that span workgroups (the upper
+// levels, a few per workgroup) use agent scope, which on this chip writes back
+// and invalidates the XCD's L2 -- done for every node, that made this pass
+// twenty times slower than the rest of the build.
+__device__ __forceinline__ bool second_arrival(unsigned* flag, bool local) {
+    const unsigned old = local ? __hip_atomic_fetch_add(flag, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP)
+                               : __hip_atomic_fetch_add(flag, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    return old != 0u;
+}
+
+// Boxes bottom up.  Thread per leaf: it climbs while it is the second
+// arrival, so every internal node's box is written exactly once, after both
+// children's.  Large workgroups keep more of the tree workgroup-local.
+constexpr int kBoxBlock = 1024;
+
+__global__ void __launch_bounds__(kBoxBlock) boxes_up(const crt_vertex* __restrict__ v, const uint32_t* __restrict__ idx,
                                                    const uint32_t* __restrict__ order, int64_t n,
-                                                   const int2* __restrict__ child, const int32_t* __restrict__ parent,
-                                                   unsigned* __restrict__ flags, Box* __restrict__ boxes) {
-    const int64_t l = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+                                                   const int2* __restrict__ child, const int2* __restrict__ range,
+                                                   const int32_t* __restrict__ parent, unsigned* __restrict__ flags,
+                                                   Box* __restrict__ boxes) {
+    const int64_t l = int64_t(blockIdx.x) * kBoxBlock + threadIdx.x;
     if (l >= n) return;
     int32_t node = parent[(n - 1) + l];
     while (node >= 0) {
-        __threadfence();
-        if (atomicAdd(&flags[node], 1u) == 0u) return;   // the sibling's thread finishes this node
-        __threadfence();
+        const int2 r = range[node];
+        if (!second_arrival(&flags[node], r.x / kBoxBlock == r.y / kBoxBlock)) return;
         const int2 c = child[node];
         Box b[2];
         for (int k = 0; k < 2; ++k) {
             const int32_t cc = k == 0 ? c.x : c.y;
-            if (cc >= n - 1) {
-                b[k] = tri_box(v, idx, int64_t(order[cc - (n - 1)]));
-            } else {
-                // written by another thread before its flag increment: read around the cache
-                const volatile float* p = reinterpret_cast<const volatile float*>(&boxes[cc]);
-                for (int q = 0; q < 3; ++q) {
-                    b[k].lo[q] = p[q];
-                    b[k].hi[q] = p[3 + q];
-                }
-            }
+            b[k] = cc >= n - 1 ? tri_box(v, idx, int64_t(order[cc - (n - 1)])) : boxes[cc];
         }
         Box u;
         for (int q = 0; q < 3; ++q) {
@@ -406,8 +415,8 @@ int crt_build_bvh_gpu(const crt_vertex* vertices, int64_t nverts, const uint32_t
     hipLaunchKernelGGL(karras, dim3(ni), dim3(kBlock), 0, st, keys[cur], n, child, range, parent);
     // flags must start at zero: a fill kernel (no memset node, so the chain stays capturable)
     hipLaunchKernelGGL(zero_u32, dim3(ni), dim3(kBlock), 0, st, flags, n - 1);
-    hipLaunchKernelGGL(boxes_up, dim3(nb), dim3(kBlock), 0, st, vertices, indices, vals[cur], n, child, parent, flags,
-                       boxes);
+    hipLaunchKernelGGL(boxes_up, dim3(unsigned((n + kBoxBlock - 1) / kBoxBlock)), dim3(kBoxBlock), 0, st, vertices,
+                       indices, vals[cur], n, child, range, parent, flags, boxes);
     hipLaunchKernelGGL(emit, dim3(nb), dim3(kBlock), 0, st, vertices, indices, mesh_ids, mesh_id, vals[cur], n,
                        t_offset, child, range, boxes, nodes, triangles);
     const hipError_t e = hipGetLastError();
