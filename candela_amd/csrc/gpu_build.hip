@@ -223,49 +223,89 @@ __global__ void __launch_bounds__(kBlock) karras(const uint32_t* __restrict__ ke
     parent[right] = int32_t(i);
 }
 
-// Arrival at internal node i: flags[i] counts its children that are done;
-// the second arrival (old value 1) finishes the node.  Acquire-release at the
-// node's scope: a node whose leaf range lies in one workgroup's leaves has
-// both children computed in that workgroup, so its counter and its children's
-// boxes only need workgroup scope; the nodes that span workgroups (the upper
-// levels, a few per workgroup) use agent scope, which on this chip writes back
-// and invalidates the XCD's L2 -- done for every node, that made this pass
-// twenty times slower than the rest of the build.
-__device__ __forceinline__ bool second_arrival(unsigned* flag, bool local) {
-    const unsigned old = local ? __hip_atomic_fetch_add(flag, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP)
-                               : __hip_atomic_fetch_add(flag, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    return old != 0u;
-}
-
-// Boxes bottom up.  Thread per leaf: it climbs while it is the second
-// arrival, so every internal node's box is written exactly once, after both
-// children's.  Large workgroups keep more of the tree workgroup-local.
+// Boxes bottom up, in two passes.  A thread per leaf climbs while it is the
+// second of a node's two children to finish, so every internal node's box is
+// written once, after both children's.
+//
+// Karras's tree gives internal node i a leaf range with one end at i, so a
+// node whose range lies in one workgroup's kBoxBlock leaves has its index in
+// that workgroup's leaf index range, and its whole subtree is computed by that
+// workgroup.  The local pass keeps those nodes' counters and boxes in LDS
+// (workgroup scope).  A climb that reaches a node spanning workgroups counts
+// its arrival in global memory (relaxed; nothing is read back in this pass),
+// and the second arrival appends the node to the ready list.  The upper pass
+// (one workgroup) then finishes the spanning nodes from that list, reading the
+// local pass's boxes after the kernel boundary and synchronising among its
+// own threads at workgroup scope.  No pass needs agent-scope fences, which on
+// this chip write back and invalidate an XCD's L2.
 constexpr int kBoxBlock = 1024;
 
-__global__ void __launch_bounds__(kBoxBlock) boxes_up(const crt_vertex* __restrict__ v, const uint32_t* __restrict__ idx,
-                                                   const uint32_t* __restrict__ order, int64_t n,
-                                                   const int2* __restrict__ child, const int2* __restrict__ range,
-                                                   const int32_t* __restrict__ parent, unsigned* __restrict__ flags,
-                                                   Box* __restrict__ boxes) {
-    const int64_t l = int64_t(blockIdx.x) * kBoxBlock + threadIdx.x;
-    if (l >= n) return;
+__device__ __forceinline__ Box unite(const Box& a, const Box& b) {
+    Box u;
+    for (int q = 0; q < 3; ++q) {
+        u.lo[q] = fminf(a.lo[q], b.lo[q]);
+        u.hi[q] = fmaxf(a.hi[q], b.hi[q]);
+    }
+    return u;
+}
+
+// flags [n]: arrival counters of the internal nodes [0, n - 1) and, at n - 1,
+// the length of the ready list.
+__global__ void __launch_bounds__(kBoxBlock) boxes_local(const crt_vertex* __restrict__ v,
+                                                         const uint32_t* __restrict__ idx,
+                                                         const uint32_t* __restrict__ order, int64_t n,
+                                                         const int2* __restrict__ child, const int2* __restrict__ range,
+                                                         const int32_t* __restrict__ parent,
+                                                         unsigned* __restrict__ flags, Box* __restrict__ boxes,
+                                                         int32_t* __restrict__ ready) {
+    __shared__ unsigned lflag[kBoxBlock];
+    __shared__ Box lbox[kBoxBlock];
+    const int64_t base = int64_t(blockIdx.x) * kBoxBlock;
+    lflag[threadIdx.x] = 0u;
+    __syncthreads();
+    const int64_t l = base + threadIdx.x;
+    if (l >= n) return;   // no barrier below
     int32_t node = parent[(n - 1) + l];
     while (node >= 0) {
         const int2 r = range[node];
-        if (!second_arrival(&flags[node], r.x / kBoxBlock == r.y / kBoxBlock)) return;
+        if (r.x / kBoxBlock != r.y / kBoxBlock) {
+            if (__hip_atomic_fetch_add(&flags[node], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 1u) {
+                const unsigned k = __hip_atomic_fetch_add(&flags[n - 1], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                ready[k] = node;
+            }
+            return;
+        }
+        const int32_t li = node - int32_t(base);   // in [0, kBoxBlock): node lies in this workgroup's leaf range
+        if (__hip_atomic_fetch_add(&lflag[li], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP) == 0u) return;
         const int2 c = child[node];
-        Box b[2];
-        for (int k = 0; k < 2; ++k) {
-            const int32_t cc = k == 0 ? c.x : c.y;
-            b[k] = cc >= n - 1 ? tri_box(v, idx, int64_t(order[cc - (n - 1)])) : boxes[cc];
-        }
-        Box u;
-        for (int q = 0; q < 3; ++q) {
-            u.lo[q] = fminf(b[0].lo[q], b[1].lo[q]);
-            u.hi[q] = fmaxf(b[0].hi[q], b[1].hi[q]);
-        }
+        const Box b0 = c.x >= n - 1 ? tri_box(v, idx, int64_t(order[c.x - (n - 1)])) : lbox[c.x - int32_t(base)];
+        const Box b1 = c.y >= n - 1 ? tri_box(v, idx, int64_t(order[c.y - (n - 1)])) : lbox[c.y - int32_t(base)];
+        const Box u = unite(b0, b1);
+        lbox[li] = u;
         boxes[node] = u;
         node = node == 0 ? -1 : parent[node];
+    }
+}
+
+__global__ void __launch_bounds__(kBoxBlock) boxes_upper(const crt_vertex* __restrict__ v,
+                                                         const uint32_t* __restrict__ idx,
+                                                         const uint32_t* __restrict__ order, int64_t n,
+                                                         const int2* __restrict__ child,
+                                                         const int32_t* __restrict__ parent,
+                                                         unsigned* __restrict__ flags, Box* __restrict__ boxes,
+                                                         const int32_t* __restrict__ ready) {
+    const unsigned count = flags[n - 1];
+    for (unsigned i = threadIdx.x; i < count; i += kBoxBlock) {
+        int32_t node = ready[i];
+        while (true) {
+            const int2 c = child[node];
+            const Box b0 = c.x >= n - 1 ? tri_box(v, idx, int64_t(order[c.x - (n - 1)])) : boxes[c.x];
+            const Box b1 = c.y >= n - 1 ? tri_box(v, idx, int64_t(order[c.y - (n - 1)])) : boxes[c.y];
+            boxes[node] = unite(b0, b1);
+            if (node == 0) break;
+            node = parent[node];
+            if (__hip_atomic_fetch_add(&flags[node], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP) == 0u) break;
+        }
     }
 }
 
@@ -325,7 +365,7 @@ __global__ void __launch_bounds__(kBlock) emit(const crt_vertex* __restrict__ v,
 inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
 struct Layout {
-    size_t keys[2], vals[2], counts, offsets, total, partial, frame, child, range, parent, flags, boxes, bytes;
+    size_t keys[2], vals[2], counts, offsets, total, partial, frame, child, range, parent, flags, boxes, ready, bytes;
 };
 
 inline Layout layout(int64_t n) {
@@ -352,6 +392,7 @@ inline Layout layout(int64_t n) {
     L.parent = take(2 * un * 4);
     L.flags = take(un * 4);
     L.boxes = take(un * sizeof(Box));
+    L.ready = take(un * 4);
     L.bytes = off;
     return L;
 }
@@ -395,6 +436,7 @@ int crt_build_bvh_gpu(const crt_vertex* vertices, int64_t nverts, const uint32_t
     auto* parent = static_cast<int32_t*>(at(L.parent));
     auto* flags = static_cast<unsigned*>(at(L.flags));
     auto* boxes = static_cast<Box*>(at(L.boxes));
+    auto* ready = static_cast<int32_t*>(at(L.ready));
     const hipStream_t st = static_cast<hipStream_t>(stream);
     const int64_t n = ntris;
     const unsigned nb = unsigned((n + kBlock - 1) / kBlock);
@@ -413,10 +455,13 @@ int crt_build_bvh_gpu(const crt_vertex* vertices, int64_t nverts, const uint32_t
     }
     const unsigned ni = unsigned((n - 1 + kBlock - 1) / kBlock);
     hipLaunchKernelGGL(karras, dim3(ni), dim3(kBlock), 0, st, keys[cur], n, child, range, parent);
-    // flags must start at zero: a fill kernel (no memset node, so the chain stays capturable)
-    hipLaunchKernelGGL(zero_u32, dim3(ni), dim3(kBlock), 0, st, flags, n - 1);
-    hipLaunchKernelGGL(boxes_up, dim3(unsigned((n + kBoxBlock - 1) / kBoxBlock)), dim3(kBoxBlock), 0, st, vertices,
-                       indices, vals[cur], n, child, range, parent, flags, boxes);
+    // flags and the ready count must start at zero: a fill kernel (no memset
+    // node, so the chain stays capturable)
+    hipLaunchKernelGGL(zero_u32, dim3(nb), dim3(kBlock), 0, st, flags, n);
+    hipLaunchKernelGGL(boxes_local, dim3(unsigned((n + kBoxBlock - 1) / kBoxBlock)), dim3(kBoxBlock), 0, st, vertices,
+                       indices, vals[cur], n, child, range, parent, flags, boxes, ready);
+    hipLaunchKernelGGL(boxes_upper, dim3(1), dim3(kBoxBlock), 0, st, vertices, indices, vals[cur], n, child, parent,
+                       flags, boxes, ready);
     hipLaunchKernelGGL(emit, dim3(nb), dim3(kBlock), 0, st, vertices, indices, mesh_ids, mesh_id, vals[cur], n,
                        t_offset, child, range, boxes, nodes, triangles);
     const hipError_t e = hipGetLastError();
