@@ -13,14 +13,16 @@
 //      caches it, BVHConstructor.cpp:404-427); bounds of the centroids by a
 //      two-level min/max reduction;
 //   2. a 30-bit Morton code of the centroid quantized to 1024 cells per axis;
-//   3. a stable LSD radix sort of (code, triangle) one bit per pass: each pass
-//      counts the zeros per workgroup, scans the counts and scatters zeros
-//      then ones in input order (scan.h);
+//   3. a stable LSD radix sort of (code, triangle) by 4-bit digits, 8 passes:
+//      each pass counts the digits per 1024-code tile, scans the digit-major
+//      counts (scan.h) and scatters each tile's codes in input order within
+//      their digit;
 //   4. Karras's radix tree: internal node i finds its range and split by
 //      binary searches over the longest common prefix of neighbouring codes
 //      (ties broken by the sorted index, so every code is distinct);
-//   5. boxes bottom up: each leaf walks to the root; at every internal node
-//      the second of its two children to arrive forms the union;
+//   5. boxes bottom up: each leaf walks up; at every internal node the second
+//      of its two children to arrive forms the union (workgroup-local subtrees
+//      in LDS, then the nodes spanning workgroups in one more workgroup);
 //   6. the stack layout: entry i holds internal node i's two children; a
 //      child that covers two leaves is written as one 2-triangle leaf (the
 //      reference's leaf size), so its own entry is never reached.
@@ -150,37 +152,80 @@ __global__ void __launch_bounds__(kBlock) morton(const crt_vertex* __restrict__ 
     vals[t] = uint32_t(t);
 }
 
-// Radix pass over bit `bit`: zeros per workgroup.
-__global__ void __launch_bounds__(kBlock) count_zeros(const uint32_t* __restrict__ keys, int64_t n, int bit,
-                                                      uint32_t* __restrict__ counts) {
-    __shared__ uint32_t wave_counts[kBlock / 64];
-    const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
-    uint32_t total;
-    (void)scan::block_rank(i < n && ((keys[i] >> bit) & 1u) == 0u, wave_counts, total);
-    if (threadIdx.x == 0) counts[blockIdx.x] = total;
+// Radix sort of the codes by 4-bit digits (LSD, stable), tiles of kTile
+// codes: a workgroup of kBlock threads takes kItems codes each, round r
+// holding codes tile * kTile + r * kBlock + thread, so rounds and threads run
+// in index order.  tile_ranks gives each code its rank among the codes of its
+// digit that precede it in the tile (peers in the wave from one ballot per
+// digit bit; earlier waves and rounds from LDS counts) and leaves the tile's
+// per-digit counts in run[].  Codes past n take digit 15: they rank after
+// every code of the tile, and only the last tile has them, whose digit-15
+// segment ends the order.
+constexpr int kDigitBits = 4;
+constexpr int kRadix = 1 << kDigitBits;
+constexpr int kItems = 4;
+constexpr int kTile = kBlock * kItems;
+constexpr int kWaves = kBlock / 64;
+
+__device__ __forceinline__ void tile_ranks(const uint32_t* __restrict__ keys, int64_t n, int shift, uint32_t key[kItems],
+                                           uint32_t digit[kItems], uint32_t rank[kItems], uint32_t* wc,
+                                           uint32_t* run) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint64_t below = (1ull << lane) - 1ull;
+    if (threadIdx.x < kRadix) run[threadIdx.x] = 0u;
+    for (int r = 0; r < kItems; ++r) {
+        const int64_t i = int64_t(blockIdx.x) * kTile + r * kBlock + threadIdx.x;
+        key[r] = i < n ? keys[i] : 0u;
+        const uint32_t d = i < n ? (key[r] >> shift) & (kRadix - 1) : kRadix - 1;
+        digit[r] = d;
+        uint64_t peers = ~0ull;
+        for (int b = 0; b < kDigitBits; ++b) {
+            const uint64_t bb = __ballot((d >> b) & 1u);
+            peers &= ((d >> b) & 1u) ? bb : ~bb;
+        }
+        if (threadIdx.x < kWaves * kRadix) wc[threadIdx.x] = 0u;
+        __syncthreads();
+        if ((peers & below) == 0ull) wc[wave * kRadix + d] = uint32_t(__popcll(peers));   // the digit's first lane
+        __syncthreads();
+        uint32_t before = run[d];
+        for (int w = 0; w < kWaves; ++w) before += w < wave ? wc[w * kRadix + d] : 0u;
+        rank[r] = before + uint32_t(__popcll(peers & below));
+        __syncthreads();
+        if (threadIdx.x < kRadix) {
+            uint32_t sum = 0u;
+            for (int w = 0; w < kWaves; ++w) sum += wc[w * kRadix + threadIdx.x];
+            run[threadIdx.x] += sum;
+        }
+        __syncthreads();
+    }
 }
 
-// Radix pass: zeros first, then ones, each in input order.
-__global__ void __launch_bounds__(kBlock) scatter_bit(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
-                                                      int64_t n, int bit, const int64_t* __restrict__ zero_offsets,
-                                                      const int64_t* __restrict__ zeros_total,
-                                                      uint32_t* __restrict__ kout, uint32_t* __restrict__ vout) {
-    __shared__ uint32_t wave_counts[kBlock / 64];
-    const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
-    const bool in = i < n;
-    const uint32_t k = in ? kin[i] : 0u;
-    const bool zero = in && ((k >> bit) & 1u) == 0u;
-    uint32_t zcount;
-    const uint32_t zrank = scan::block_rank(zero, wave_counts, zcount);
-    __syncthreads();   // wave_counts is reused for the ones
-    uint32_t ocount;
-    const uint32_t orank = scan::block_rank(in && !zero, wave_counts, ocount);
-    if (!in) return;
-    const int64_t zbefore = zero_offsets[blockIdx.x];
-    const int64_t pos = zero ? zbefore + zrank
-                             : *zeros_total + (int64_t(blockIdx.x) * kBlock - zbefore) + orank;
-    kout[pos] = k;
-    vout[pos] = vin[i];
+// counts[digit * ntiles + tile]: digit-major, so one exclusive scan gives
+// each (digit, tile) its first position in the stable order.
+__global__ void __launch_bounds__(kBlock) count_digits(const uint32_t* __restrict__ keys, int64_t n, int shift,
+                                                       uint32_t* __restrict__ counts) {
+    __shared__ uint32_t wc[kWaves * kRadix];
+    __shared__ uint32_t run[kRadix];
+    uint32_t key[kItems], digit[kItems], rank[kItems];
+    tile_ranks(keys, n, shift, key, digit, rank, wc, run);
+    if (threadIdx.x < kRadix) counts[int64_t(threadIdx.x) * gridDim.x + blockIdx.x] = run[threadIdx.x];
+}
+
+__global__ void __launch_bounds__(kBlock) scatter_digits(const uint32_t* __restrict__ kin,
+                                                         const uint32_t* __restrict__ vin, int64_t n, int shift,
+                                                         const int64_t* __restrict__ offsets,
+                                                         uint32_t* __restrict__ kout, uint32_t* __restrict__ vout) {
+    __shared__ uint32_t wc[kWaves * kRadix];
+    __shared__ uint32_t run[kRadix];
+    uint32_t key[kItems], digit[kItems], rank[kItems];
+    tile_ranks(kin, n, shift, key, digit, rank, wc, run);
+    for (int r = 0; r < kItems; ++r) {
+        const int64_t i = int64_t(blockIdx.x) * kTile + r * kBlock + threadIdx.x;
+        if (i >= n) continue;
+        const int64_t pos = offsets[int64_t(digit[r]) * gridDim.x + blockIdx.x] + rank[r];
+        kout[pos] = key[r];
+        vout[pos] = vin[i];
+    }
 }
 
 // Longest common prefix of sorted codes i and j (Karras's delta); -1 outside [0, n).
@@ -382,8 +427,9 @@ inline Layout layout(int64_t n) {
     L.keys[1] = take(un * 4);
     L.vals[0] = take(un * 4);
     L.vals[1] = take(un * 4);
-    L.counts = take(nb * 4);
-    L.offsets = take(nb * 8);
+    const size_t nt = size_t((n + kTile - 1) / kTile);
+    L.counts = take(kRadix * nt * 4);
+    L.offsets = take(kRadix * nt * 8);
     L.total = take(8);
     L.partial = take(nb * 6 * 4);
     L.frame = take(6 * 4);
@@ -445,11 +491,12 @@ int crt_build_bvh_gpu(const crt_vertex* vertices, int64_t nverts, const uint32_t
     hipLaunchKernelGGL(finish_bounds, dim3(1), dim3(kBlock), 0, st, partial, int64_t(nb), frame);
     hipLaunchKernelGGL(morton, dim3(nb), dim3(kBlock), 0, st, vertices, indices, n, frame, keys[0], vals[0]);
     int cur = 0;
-    for (int bit = 0; bit < kMortonBits; ++bit) {
-        hipLaunchKernelGGL(count_zeros, dim3(nb), dim3(kBlock), 0, st, keys[cur], n, bit, counts);
-        hipLaunchKernelGGL(crt::scan::exclusive_scan, dim3(1), dim3(kScanBlock), 0, st, counts, int64_t(nb), offsets,
-                           total);
-        hipLaunchKernelGGL(scatter_bit, dim3(nb), dim3(kBlock), 0, st, keys[cur], vals[cur], n, bit, offsets, total,
+    const unsigned nt = unsigned((n + kTile - 1) / kTile);
+    for (int shift = 0; shift < kMortonBits; shift += kDigitBits) {
+        hipLaunchKernelGGL(count_digits, dim3(nt), dim3(kBlock), 0, st, keys[cur], n, shift, counts);
+        hipLaunchKernelGGL(crt::scan::exclusive_scan, dim3(1), dim3(kScanBlock), 0, st, counts,
+                           int64_t(kRadix) * nt, offsets, total);
+        hipLaunchKernelGGL(scatter_digits, dim3(nt), dim3(kBlock), 0, st, keys[cur], vals[cur], n, shift, offsets,
                            keys[1 - cur], vals[1 - cur]);
         cur = 1 - cur;
     }
