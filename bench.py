@@ -166,7 +166,7 @@ def time_batched(crt, batches, steps: int, warmup: int, stream, wide: bool) -> d
 def default_candidates(crt, wide: bool):
     """Autotuner grid: preset, tiles per claim, waves per group, tile order, queue shards and, over the wide
     layouts, the child order of the any-hit walks."""
-    return [crt.BatchConfig(v, t, w, 0, o, q, co) for v in (0, 1, 4) for t in (1, 2) for w in (1, 4)
+    return [crt.BatchConfig(v, t, w, 0, o, q, co) for v in (0, 1, 4, 6, 7) for t in (1, 2) for w in (1, 4)
             for o in (0, 1) for q in (1, 16) for co in ((0, 1) if wide else (0,))]
 
 

@@ -67,11 +67,11 @@ __device__ __forceinline__ BatchDesc load_desc(const BatchDesc* __restrict__ des
     return d;
 }
 
-template <int INNER, int LEAF_BATCH, int K, int WPG>
-__global__ void __launch_bounds__(64 * WPG) batched_stackless(const BatchDesc* __restrict__ descs,
-                                                              const int2* __restrict__ tiles, int32_t ntiles,
-                                                              int32_t per_claim, unsigned long long* __restrict__ counters,
-                                                              int32_t max_iter, int32_t shards) {
+template <int INNER, int LEAF_BATCH, int K>
+__device__ __forceinline__ void stackless_body(const BatchDesc* __restrict__ descs, const int2* __restrict__ tiles,
+                                               int32_t ntiles, int32_t per_claim,
+                                               unsigned long long* __restrict__ counters, int32_t max_iter,
+                                               int32_t shards) {
     // one walk for both queries (the query is a per-tile value): two inlined
     // walks would need the sum of their registers
     WaveWalk<false, INNER, LEAF_BATCH, K, true> w;
@@ -95,15 +95,30 @@ __global__ void __launch_bounds__(64 * WPG) batched_stackless(const BatchDesc* _
     }
 }
 
+#define CRT_BATCH_ARGS                                                                                          \
+    const BatchDesc *__restrict__ descs, const int2 *__restrict__ tiles, int32_t ntiles, int32_t per_claim,      \
+        unsigned long long *__restrict__ counters, int32_t max_iter, int32_t shards
+
+template <int INNER, int LEAF_BATCH, int K, int WPG>
+__global__ void __launch_bounds__(64 * WPG) batched_stackless(CRT_BATCH_ARGS) {
+    stackless_body<INNER, LEAF_BATCH, K>(descs, tiles, ntiles, per_claim, counters, max_iter, shards);
+}
+
+// The same kernels compiled for at least OCC waves per SIMD (register budget
+// 512 / OCC), trading spills for occupancy.
+template <int INNER, int LEAF_BATCH, int K, int WPG, int OCC>
+__global__ void __launch_bounds__(64 * WPG) __attribute__((amdgpu_waves_per_eu(OCC, 8)))
+batched_stackless_occ(CRT_BATCH_ARGS) {
+    stackless_body<INNER, LEAF_BATCH, K>(descs, tiles, ntiles, per_claim, counters, max_iter, shards);
+}
+
 // The same tile queue over the wide layouts (BVH4 / BVH8): each claimed tile
 // runs the wave-scheduled wide walk (wide_walk.h), so every hit record equals
 // crt_trace's wide wave kernel on the same batch.
 template <int W, int LDS_DEPTH, int INNER, int LEAF_BATCH, int WPG>
-__global__ void __launch_bounds__(64 * WPG) batched_wide(const BatchDesc* __restrict__ descs,
-                                                         const int2* __restrict__ tiles, int32_t ntiles,
-                                                         int32_t per_claim, unsigned long long* __restrict__ counters,
-                                                         int32_t max_iter, int32_t shards) {
-    __shared__ int32_t lds[wide::lds_ints<W, LDS_DEPTH>() * 64 * WPG];
+__device__ __forceinline__ void wide_body(const BatchDesc* __restrict__ descs, const int2* __restrict__ tiles,
+                                          int32_t ntiles, int32_t per_claim, unsigned long long* __restrict__ counters,
+                                          int32_t max_iter, int32_t shards, int32_t* lds) {
     int32_t spill_node[wide::kSpill];
     float spill_t[wide::kSpill];
     wide::Stack<LDS_DEPTH> st;
@@ -134,6 +149,20 @@ __global__ void __launch_bounds__(64 * WPG) batched_wide(const BatchDesc* __rest
     }
 }
 
+template <int W, int LDS_DEPTH, int INNER, int LEAF_BATCH, int WPG>
+__global__ void __launch_bounds__(64 * WPG) batched_wide(CRT_BATCH_ARGS) {
+    __shared__ int32_t lds[wide::lds_ints<W, LDS_DEPTH>() * 64 * WPG];
+    wide_body<W, LDS_DEPTH, INNER, LEAF_BATCH, WPG>(descs, tiles, ntiles, per_claim, counters, max_iter, shards, lds);
+}
+
+template <int W, int LDS_DEPTH, int INNER, int LEAF_BATCH, int WPG, int OCC>
+__global__ void __launch_bounds__(64 * WPG) __attribute__((amdgpu_waves_per_eu(OCC, 8)))
+batched_wide_occ(CRT_BATCH_ARGS) {
+    __shared__ int32_t lds[wide::lds_ints<W, LDS_DEPTH>() * 64 * WPG];
+    wide_body<W, LDS_DEPTH, INNER, LEAF_BATCH, WPG>(descs, tiles, ntiles, per_claim, counters, max_iter, shards, lds);
+}
+#undef CRT_BATCH_ARGS
+
 }  // namespace pdev
 
 using BatchedFn = void (*)(const BatchDesc*, const int2*, int32_t, int32_t, unsigned long long*, int32_t, int32_t);
@@ -146,6 +175,8 @@ static const void* batched_symbol(int variant) {
         case 3: return reinterpret_cast<const void*>(&pdev::batched_stackless<16, 16, 2, WPG>);
         case 4: return reinterpret_cast<const void*>(&pdev::batched_stackless<4, 4, 2, WPG>);
         case 5: return reinterpret_cast<const void*>(&pdev::batched_stackless<8, 12, 4, WPG>);
+        case 6: return reinterpret_cast<const void*>(&pdev::batched_stackless_occ<8, 8, 2, WPG, 5>);
+        case 7: return reinterpret_cast<const void*>(&pdev::batched_stackless_occ<8, 8, 1, WPG, 6>);
         default: return reinterpret_cast<const void*>(&pdev::batched_stackless<8, 8, 2, WPG>);
     }
 }
@@ -160,6 +191,8 @@ static const void* batched_wide_symbol(int variant) {
         case 3: return reinterpret_cast<const void*>(&pdev::batched_wide<W, 16, 4, 16, WPG>);
         case 4: return reinterpret_cast<const void*>(&pdev::batched_wide<W, 8, 4, 32, WPG>);
         case 5: return reinterpret_cast<const void*>(&pdev::batched_wide<W, 8, 16, 48, WPG>);
+        case 6: return reinterpret_cast<const void*>(&pdev::batched_wide_occ<W, 8, 8, 16, WPG, 5>);
+        case 7: return reinterpret_cast<const void*>(&pdev::batched_wide_occ<W, 8, 4, 16, WPG, 5>);
         default: return reinterpret_cast<const void*>(&pdev::batched_wide<W, 8, 4, 16, WPG>);
     }
 }
@@ -218,7 +251,7 @@ int hip_err(hipError_t e, const char* what) {
 static_assert(sizeof(crt_batch_config) == 28, "crt_batch_config is 7 int32 fields");
 
 int validate_config(const crt_batch_config& c) {
-    if (c.variant < 0 || c.variant > 5) return crt::set_error(CRT_E_INVALID, "batch config: variant must be 0..5");
+    if (c.variant < 0 || c.variant > 7) return crt::set_error(CRT_E_INVALID, "batch config: variant must be 0..7");
     if (c.tiles_per_claim != 1 && c.tiles_per_claim != 2 && c.tiles_per_claim != 4)
         return crt::set_error(CRT_E_INVALID, "batch config: tiles_per_claim must be 1, 2 or 4");
     if (c.waves_per_group != 1 && c.waves_per_group != 2 && c.waves_per_group != 4)
@@ -417,7 +450,7 @@ int crt_autotune_batched(const crt_batch* batches, int nbatches, const crt_batch
         for (int i = 0; i < nbatches; ++i)
             orders |= batches[i].query == CRT_QUERY_ANY &&
                       (batches[i].scene.format == CRT_FORMAT_WIDE4 || batches[i].scene.format == CRT_FORMAT_WIDE8);
-        for (int v = 0; v < 6; ++v)
+        for (int v = 0; v < 8; ++v)
             for (int t : {1, 2, 4})
                 for (int w : {1, 4})
                     for (int o : {0, 1})

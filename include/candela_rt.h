@@ -676,8 +676,9 @@ typedef struct crt_batch {
 
 /* Tuning of the batched kernel (crt_autotune_batched searches these). */
 typedef struct crt_batch_config {
-    int32_t variant;        /* wave-walk preset 0..5 (stackless: inner steps, leaf batch,
-                               window; wide: LDS stack depth, inner steps, leaf batch) */
+    int32_t variant;        /* wave-walk preset 0..7 (stackless: inner steps, leaf batch,
+                               window; wide: LDS stack depth, inner steps, leaf batch; 6 and 7
+                               are compiled for at least 5 or 6 waves per SIMD) */
     int32_t tiles_per_claim;/* 64-ray tiles a wave claims per atomic: 1, 2 or 4 */
     int32_t waves_per_group;/* 1, 2 or 4 waves per workgroup */
     int32_t grid;           /* workgroups; 0 = enough to fill every CU */
@@ -708,7 +709,7 @@ int64_t crt_batch_plan_rays(const crt_batch_plan* plan);
 /* Times every candidate configuration on the batches (`repeats` launches
  * each, after one warm-up) and returns the fastest in *best and its time per
  * launch in *best_ms.  candidates == NULL searches the built-in grid:
- * variants 0..5 x tiles_per_claim {1,2,4} x waves_per_group {1,4} x order {0,1}
+ * variants 0..7 x tiles_per_claim {1,2,4} x waves_per_group {1,4} x order {0,1}
  * x shards {1,16}, x child_order {0,1} when the batches use a wide format and
  * hold any-hit queries. */
 int crt_autotune_batched(const crt_batch* batches, int nbatches, const crt_batch_config* candidates,

@@ -63,7 +63,7 @@ def test_batched_default_plan(work, orc):
     check_all(work, orc, "batched default")
 
 
-@pytest.mark.parametrize("variant", range(6))
+@pytest.mark.parametrize("variant", range(8))
 def test_batched_configs(work, orc, variant):
     import candela_amd as crt
     for tpc, wpg, order, grid, shards in ((1, 4, 1, 0, 0), (2, 1, 0, 0, 16), (4, 2, 1, 3, 16), (1, 1, 1, 5, 16)):
@@ -137,7 +137,8 @@ def test_batched_wide(work, orc, width):
     layout = (width, 2)
     work.build_wide(crt, width, 2)
     for variant, tpc, wpg, order, shards, co in ((0, 1, 4, 1, 16, 0), (1, 2, 1, 0, 1, 1), (2, 4, 2, 1, 16, 0),
-                                                 (3, 1, 4, 0, 0, 1), (4, 2, 4, 1, 16, 1), (5, 1, 1, 1, 16, 0)):
+                                                 (3, 1, 4, 0, 0, 1), (4, 2, 4, 1, 16, 1), (5, 1, 1, 1, 16, 0),
+                                                 (6, 2, 4, 1, 16, 1), (7, 1, 2, 0, 16, 0)):
         for s in work.scenes:
             s.h_closest.fill_bytes(0xAB)
             s.h_shadow.fill_bytes(0xAB)
@@ -157,6 +158,8 @@ def test_batched_wide(work, orc, width):
         crt.BatchPlan(mixed)
     with pytest.raises(crt.CrtError):
         crt.BatchPlan(work.batches(crt, layout), crt.BatchConfig(0, 1, 4, 0, 1, 16, 2))
+    with pytest.raises(crt.CrtError):
+        crt.BatchPlan(work.batches(crt, layout), crt.BatchConfig(8, 1, 4, 0, 1, 16, 0))
 
 
 def test_autotune_wide_child_order(work, orc):
