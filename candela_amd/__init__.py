@@ -29,6 +29,7 @@ __all__ = [
     "STATUS_VISITS_MASK", "SWAP_NONE", "SWAP_SEEDED", "KERNEL_AUTO", "KERNEL_THREAD_PER_RAY", "KERNEL_PERSISTENT",
     "KERNEL_WAVE", "CrtError", "DeviceArray", "Stream", "BuiltBVH", "build_bvh",
     "RayIntersector", "trace", "device_count", "node_dtype", "lib", "Batch", "BatchConfig", "BatchPlan",
+    "GpuWide",
     "autotune_batched", "ShadeParams", "SHADE_RECORD_DTYPE", "ShadingModel", "FORMAT_WIDE4", "FORMAT_WIDE8",
     "WIDE_GROUP_DTYPE", "TRI_RECORD_DTYPE", "WideBVH", "wide_build", "DeviceWide", "upload_wide",
     "TEXTURE_REF_DTYPE", "TEXTURE_DTYPE", "HIT_ATTR_DTYPE", "MaterialTable", "hit_attributes",
@@ -415,6 +416,36 @@ class GpuBVH:
                                       self.mesh_ids.ptr if self.mesh_ids is not None else None, self.mesh_id, 0,
                                       self.nodes.ptr, self.triangles.ptr, self.workspace.ptr, self.workspace.nbytes,
                                       _stream_handle(stream)))
+
+
+class GpuWide:
+    """crt_wide_build_gpu: the wide layout (2-triangle leaves) of a GpuBVH, collapsed on the device and numbered
+    breadth-first; `scene` is its crt_scene_view (one identity entity over the wide nodes)."""
+
+    def __init__(self, bvh: GpuBVH, width: int = 4, stream=None):
+        if width not in (4, 8):
+            raise ValueError("width must be 4 or 8")
+        self.bvh = bvh
+        self.width = width
+        self.groups = DeviceArray(WIDE_GROUP_DTYPE, bvh.ntris * (width // 4))
+        self.records = DeviceArray(TRI_RECORD_DTYPE, bvh.ntris)
+        self.workspace = DeviceArray(np.uint8, max(256, int(lib().crt_wide_build_gpu_workspace(bvh.ntris, width))))
+        self.entities = DeviceArray(ENTITY_DTYPE, 1)
+        self.node_count = 0
+        self.scene = None
+        self.build(stream)
+
+    def build(self, stream=None) -> None:
+        """Collapses the GpuBVH's current tree (synchronizes the stream)."""
+        b = self.bvh
+        n = C.c_int64()
+        check(lib().crt_wide_build_gpu(b.nodes.ptr, b.nodes.count, b.triangles.ptr, b.ntris, b.vertices.ptr,
+                                       b.vertices.count, self.width, self.groups.ptr, self.records.ptr, C.byref(n),
+                                       self.workspace.ptr, self.workspace.nbytes, _stream_handle(stream)))
+        self.node_count = int(n.value)
+        self.entities.upload(_identity_entity(self.node_count))
+        self.scene = SceneView(FORMAT_WIDE4 if self.width == 4 else FORMAT_WIDE8, 1, self.node_count,
+                               self.groups.ptr, b.triangles.ptr, b.vertices.ptr, self.entities.ptr, self.records.ptr)
 
 
 class DeviceWide:

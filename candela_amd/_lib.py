@@ -163,6 +163,10 @@ SIGNATURES = {
     "crt_bvh_t_offset": (C.c_int32, [C.c_void_p]),
     "crt_raygen_workspace_bytes": (C.c_size_t, [C.c_int64]),
     "crt_build_bvh_gpu_workspace": (C.c_size_t, [C.c_int64]),
+    "crt_wide_build_gpu_workspace": (C.c_size_t, [C.c_int64, C.c_int32]),
+    "crt_wide_build_gpu": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_int32,
+                                     C.c_void_p, C.c_void_p, C.POINTER(C.c_int64), C.c_void_p, C.c_size_t,
+                                     C.c_void_p]),
     "crt_build_bvh_gpu": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_void_p, C.c_int32, C.c_int32,
                                     C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
     "crt_gen_diffuse_rays": (C.c_int, [C.POINTER(SceneView), C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p,

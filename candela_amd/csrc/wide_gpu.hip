@@ -1,0 +1,391 @@
+// Device collapse of one object's stack-layout BVH2 into the wide layout
+// (crt_wide_build_gpu), for trees built on the device every frame
+// (crt_build_bvh_gpu) that the wide walks should trace without a host round
+// trip.
+//
+// It is crt_wide_build's collapse (wide_build.cpp) with 2-triangle leaves,
+// level by level.  Each wide node of a level expands its BVH2 node as the host
+// does: starting from the node's two children, the inner child of largest
+// half surface area (the first one on a tie) is replaced by its own two
+// children until the node holds `width` of them; a leaf, or an inner node of
+// two adjacent 1-triangle leaves, stays one slot.  So every wide node has the
+// host's slots, boxes, leaf ranges and slot order.  Nodes are numbered
+// breadth-first, where the host numbers them depth-first as it allocates;
+// walks visit the same boxes in the same order, so hit records, visit and
+// test counts are identical.
+//
+// A level is three launches: expand (a thread per wide node writes its slots
+// and lists the BVH2 entries of its inner children, with per-workgroup
+// totals), one workgroup scans the totals (scan.h), and assign (a
+// workgroup-local scan gives each node the index of its first child; it writes
+// the child fields and the next level's list).  The host reads each level's
+// size back, and reads the root entry first: a tree that is one 2-triangle
+// leaf gets its one-slot root from the host.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <string>
+
+#include "candela_rt.h"
+#include "crt_internal.h"
+#include "scan.h"
+
+namespace crt {
+namespace wg {
+
+constexpr int kBlock = 256;
+
+// A BVH2 child as its parent's stack entry stores it, with its leaf range
+// when it is (or collapses into) a leaf.
+struct Kid {
+    float lo[3], hi[3];
+    int32_t entry;          // its own stack entry when inner, else -1
+    int32_t start, count;   // leaf range when entry < 0
+};
+
+// Bits of *err: a link leaves the node array; the links reach more wide
+// nodes than the output holds (they do not form a tree); a vertex index
+// leaves the vertex array.
+constexpr unsigned kErrLink = 1u, kErrCap = 2u, kErrVertex = 4u;
+
+__device__ __forceinline__ Kid make_kid(const crt_stack_node* __restrict__ nodes, int64_t node_count,
+                                        const crt_bounds& b, unsigned* err) {
+    Kid k;
+    for (int q = 0; q < 3; ++q) {
+        k.lo[q] = b.min[q];
+        k.hi[q] = b.max[q];
+    }
+    const int32_t wmin = __float_as_int(b.min[3]);
+    k.entry = -1;
+    k.start = 0;
+    k.count = 0;
+    if (wmin != -1) {
+        k.start = wmin >> 4;
+        k.count = wmin & 0xF;
+        return k;
+    }
+    const int32_t e = __float_as_int(b.max[3]);
+    if (e < 0 || int64_t(e) >= node_count) {
+        atomicOr(err, kErrLink);
+        return k;   // read as an empty leaf; the call fails
+    }
+    // an inner node of two leaves with two adjacent triangles in all is one
+    // 2-triangle leaf (the host's rule: at most 2 triangles, contiguous)
+    const crt_stack_node& s = nodes[e];
+    const int32_t a = __float_as_int(s.left.min[3]), c = __float_as_int(s.right.min[3]);
+    if (a != -1 && c != -1) {
+        const int32_t sa = a >> 4, na = a & 0xF, sc = c >> 4, nc = c & 0xF;
+        const int32_t first = min(sa, sc), last = max(sa + na, sc + nc);
+        if (na + nc <= 2 && last - first == na + nc) {
+            k.start = first;
+            k.count = na + nc;
+            return k;
+        }
+    }
+    k.entry = e;
+    return k;
+}
+
+// The host's half_area: dx * dy + dy * dz + dz * dx, left to right.
+__device__ __forceinline__ float half_area(const Kid& k) {
+    const float dx = k.hi[0] - k.lo[0], dy = k.hi[1] - k.lo[1], dz = k.hi[2] - k.lo[2];
+    return dx * dy + dy * dz + dz * dx;
+}
+
+// Exclusive prefix of v over the workgroup (kBlock threads) and the total.
+__device__ __forceinline__ uint32_t block_excl(uint32_t v, uint32_t* wave_sums, uint32_t& total) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t incl = v;
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t o = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += o;
+    }
+    if (lane == 63) wave_sums[wave] = incl;
+    __syncthreads();
+    uint32_t before = 0;
+    total = 0;
+    for (int w = 0; w < kBlock / 64; ++w) {
+        before += w < wave ? wave_sums[w] : 0u;
+        total += wave_sums[w];
+    }
+    return before + incl - v;
+}
+
+__global__ void __launch_bounds__(kBlock) seed(int32_t* entry, int32_t* wide, unsigned* err) {
+    entry[0] = 0;
+    wide[0] = 0;
+    *err = 0u;
+}
+
+// Slots of the wide nodes of one level.  kids[item * W + slot]: the stack
+// entry of an inner child, -1 otherwise; counts[item]: inner children;
+// totals[block]: their sum per workgroup.
+template <int W>
+__global__ void __launch_bounds__(kBlock) expand(const crt_stack_node* __restrict__ nodes, int64_t node_count,
+                                                 const int32_t* __restrict__ entries,
+                                                 const int32_t* __restrict__ wides, int32_t nf,
+                                                 crt_wide_group* __restrict__ groups, int32_t* __restrict__ kids_out,
+                                                 uint32_t* __restrict__ counts, uint32_t* __restrict__ totals,
+                                                 unsigned* __restrict__ err) {
+    __shared__ uint32_t wave_sums[kBlock / 64];
+    const int32_t item = int32_t(blockIdx.x) * kBlock + int32_t(threadIdx.x);
+    uint32_t m = 0;
+    if (item < nf) {
+        const crt_stack_node& e = nodes[entries[item]];
+        Kid kids[W];
+        int n = 0;
+        kids[n++] = make_kid(nodes, node_count, e.left, err);
+        kids[n++] = make_kid(nodes, node_count, e.right, err);
+        while (n < W) {
+            int best = -1;
+            float best_area = -1.0f;
+            for (int k = 0; k < n; ++k) {
+                if (kids[k].entry < 0) continue;
+                const float a = half_area(kids[k]);
+                if (a > best_area) {
+                    best_area = a;
+                    best = k;
+                }
+            }
+            if (best < 0) break;
+            const crt_stack_node& c = nodes[kids[best].entry];
+            for (int k = n; k > best + 1; --k) kids[k] = kids[k - 1];
+            kids[best + 1] = make_kid(nodes, node_count, c.right, err);
+            kids[best] = make_kid(nodes, node_count, c.left, err);
+            ++n;
+        }
+        const int64_t w = wides[item];
+        for (int s = 0; s < W; ++s) {
+            crt_wide_group& g = groups[w * (W / 4) + s / 4];
+            const int j = s % 4;
+            const bool used = s < n;
+            const Kid& k = kids[used ? s : 0];
+            g.lo_x[j] = used ? k.lo[0] : 0.0f;
+            g.lo_y[j] = used ? k.lo[1] : 0.0f;
+            g.lo_z[j] = used ? k.lo[2] : 0.0f;
+            g.hi_x[j] = used ? k.hi[0] : 0.0f;
+            g.hi_y[j] = used ? k.hi[1] : 0.0f;
+            g.hi_z[j] = used ? k.hi[2] : 0.0f;
+            const bool inner = used && k.entry >= 0;
+            g.child[j] = used && !inner ? k.start : 0;   // inner: assign writes it
+            g.count[j] = !used ? -1 : (inner ? 0 : k.count);
+            kids_out[int64_t(item) * W + s] = inner ? k.entry : -1;
+            m += inner ? 1u : 0u;
+        }
+        counts[item] = m;
+    }
+    uint32_t total;
+    (void)block_excl(m, wave_sums, total);
+    if (threadIdx.x == 0) totals[blockIdx.x] = total;
+}
+
+// Child indices: a node's inner children are next_base + its offset + 0, 1,
+// ... in slot order; they form the next level's list in that order.
+template <int W>
+__global__ void __launch_bounds__(kBlock) assign(const int32_t* __restrict__ wides, int32_t nf,
+                                                 const int32_t* __restrict__ kids, const uint32_t* __restrict__ counts,
+                                                 const int64_t* __restrict__ block_offsets, int64_t next_base,
+                                                 int64_t cap, crt_wide_group* __restrict__ groups,
+                                                 int32_t* __restrict__ next_entries, int32_t* __restrict__ next_wides,
+                                                 unsigned* __restrict__ err) {
+    __shared__ uint32_t wave_sums[kBlock / 64];
+    const int32_t item = int32_t(blockIdx.x) * kBlock + int32_t(threadIdx.x);
+    const uint32_t m = item < nf ? counts[item] : 0u;
+    uint32_t total;
+    const uint32_t local = block_excl(m, wave_sums, total);
+    if (item >= nf) return;
+    const int64_t off = block_offsets[blockIdx.x] + local;
+    const int64_t w = wides[item];
+    int64_t r = 0;
+    for (int s = 0; s < W; ++s) {
+        const int32_t e = kids[int64_t(item) * W + s];
+        if (e < 0) continue;
+        const int64_t idx = next_base + off + r;
+        if (idx >= cap) {   // not a tree: nothing past the outputs is written
+            atomicOr(err, kErrCap);
+            return;
+        }
+        groups[w * (W / 4) + s / 4].child[s % 4] = int32_t(idx);
+        next_entries[off + r] = e;
+        next_wides[off + r] = int32_t(idx);
+        ++r;
+    }
+}
+
+// Triangle records in leaf order (crt_wide_build's).
+__global__ void __launch_bounds__(kBlock) records(const crt_triangle* __restrict__ tris,
+                                                  const crt_vertex* __restrict__ verts, int64_t nverts, int64_t n,
+                                                  crt_tri_record* __restrict__ out, unsigned* __restrict__ err) {
+    const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (i >= n) return;
+    crt_triangle t = tris[i];
+    for (int k = 0; k < 3; ++k) {
+        if (t.packed[k] < 0 || int64_t(t.packed[k]) >= nverts) {
+            atomicOr(err, kErrVertex);
+            t.packed[k] = 0;
+        }
+    }
+    crt_tri_record r;
+    const float4 a = *reinterpret_cast<const float4*>(verts[t.packed[0]].position);
+    const float4 b = *reinterpret_cast<const float4*>(verts[t.packed[1]].position);
+    const float4 c = *reinterpret_cast<const float4*>(verts[t.packed[2]].position);
+    r.v0[0] = a.x, r.v0[1] = a.y, r.v0[2] = a.z;
+    r.v1[0] = b.x, r.v1[1] = b.y, r.v1[2] = b.z;
+    r.v2[0] = c.x, r.v2[1] = c.y, r.v2[2] = c.z;
+    r.tri = int32_t(i);
+    r.mesh = t.packed[3];
+    r.pad = 0;
+    out[i] = r;
+}
+
+inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+struct Layout {
+    size_t entries[2], wides[2], kids, counts, totals, offsets, total, err, bytes;
+};
+
+// Levels hold at most `cap` wide nodes (a valid tree has fewer wide nodes than triangles).
+inline Layout layout(int64_t cap, int32_t width) {
+    const size_t n = size_t(cap), nb = size_t((cap + kBlock - 1) / kBlock);
+    Layout L{};
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        const size_t at = off;
+        off = align256(off + bytes);
+        return at;
+    };
+    L.entries[0] = take(n * 4);
+    L.entries[1] = take(n * 4);
+    L.wides[0] = take(n * 4);
+    L.wides[1] = take(n * 4);
+    L.kids = take(n * size_t(width) * 4);
+    L.counts = take(n * 4);
+    L.totals = take(nb * 4);
+    L.offsets = take(nb * 8);
+    L.total = take(8);
+    L.err = take(4);
+    L.bytes = off;
+    return L;
+}
+
+inline int32_t host_bits(float f) {
+    int32_t i;
+    std::memcpy(&i, &f, 4);
+    return i;
+}
+
+// A tree whose root entry holds two adjacent 1-triangle leaves is one
+// 2-triangle leaf under a wide root of one slot (crt_wide_build's rule); the
+// root entry was read back, so the host writes that node.
+template <int W>
+int single_leaf(const crt_stack_node& root, crt_wide_group* groups, int64_t* wide_nodes, hipStream_t st) {
+    const int32_t a = host_bits(root.left.min[3]), c = host_bits(root.right.min[3]);
+    crt_wide_group g[W / 4] = {};
+    for (auto& x : g)
+        for (int j = 0; j < 4; ++j) x.count[j] = -1;
+    g[0].lo_x[0] = fminf(root.left.min[0], root.right.min[0]);
+    g[0].lo_y[0] = fminf(root.left.min[1], root.right.min[1]);
+    g[0].lo_z[0] = fminf(root.left.min[2], root.right.min[2]);
+    g[0].hi_x[0] = fmaxf(root.left.max[0], root.right.max[0]);
+    g[0].hi_y[0] = fmaxf(root.left.max[1], root.right.max[1]);
+    g[0].hi_z[0] = fmaxf(root.left.max[2], root.right.max[2]);
+    g[0].child[0] = std::min(a >> 4, c >> 4);
+    g[0].count[0] = (a & 0xF) + (c & 0xF);
+    if (hipMemcpyAsync(groups, g, sizeof(g), hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+        return set_error(CRT_E_HIP, "crt_wide_build_gpu: single-leaf root");
+    *wide_nodes = 1;
+    return CRT_OK;
+}
+
+template <int W>
+int collapse(const crt_stack_node* nodes, int64_t node_count, int64_t cap, crt_wide_group* groups,
+             int64_t* wide_nodes, char* ws, const Layout& L, hipStream_t st) {
+    crt_stack_node root;
+    if (hipMemcpyAsync(&root, nodes, sizeof(root), hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+        return set_error(CRT_E_HIP, "crt_wide_build_gpu: reading the root entry");
+    {
+        const int32_t a = host_bits(root.left.min[3]), c = host_bits(root.right.min[3]);
+        if (a != -1 && c != -1) {
+            const int32_t sa = a >> 4, na = a & 0xF, sc = c >> 4, nc = c & 0xF;
+            if (na + nc <= 2 && std::max(sa + na, sc + nc) - std::min(sa, sc) == na + nc)
+                return single_leaf<W>(root, groups, wide_nodes, st);
+        }
+    }
+    int32_t* entries[2] = {reinterpret_cast<int32_t*>(ws + L.entries[0]), reinterpret_cast<int32_t*>(ws + L.entries[1])};
+    int32_t* wides[2] = {reinterpret_cast<int32_t*>(ws + L.wides[0]), reinterpret_cast<int32_t*>(ws + L.wides[1])};
+    auto* kids = reinterpret_cast<int32_t*>(ws + L.kids);
+    auto* counts = reinterpret_cast<uint32_t*>(ws + L.counts);
+    auto* totals = reinterpret_cast<uint32_t*>(ws + L.totals);
+    auto* offsets = reinterpret_cast<int64_t*>(ws + L.offsets);
+    auto* total = reinterpret_cast<int64_t*>(ws + L.total);
+    auto* err = reinterpret_cast<unsigned*>(ws + L.err);
+    int64_t nf = 1, next_base = 1;
+    int cur = 0;
+    while (nf > 0) {
+        const unsigned nb = unsigned((nf + kBlock - 1) / kBlock);
+        hipLaunchKernelGGL(expand<W>, dim3(nb), dim3(kBlock), 0, st, nodes, node_count, entries[cur], wides[cur],
+                           int32_t(nf), groups, kids, counts, totals, err);
+        hipLaunchKernelGGL(scan::exclusive_scan, dim3(1), dim3(scan::kScanBlock), 0, st, totals, int64_t(nb), offsets,
+                           total);
+        hipLaunchKernelGGL(assign<W>, dim3(nb), dim3(kBlock), 0, st, wides[cur], int32_t(nf), kids, counts, offsets,
+                           next_base, cap, groups, entries[1 - cur], wides[1 - cur], err);
+        int64_t next = 0;
+        unsigned bad = 0;
+        if (hipMemcpyAsync(&next, total, 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipMemcpyAsync(&bad, err, 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess)
+            return set_error(CRT_E_HIP, std::string("crt_wide_build_gpu: ") + hipGetErrorString(hipGetLastError()));
+        if (bad & kErrVertex) return set_error(CRT_E_INVALID, "crt_wide_build_gpu: a vertex index leaves the vertex array");
+        if (bad & kErrLink) return set_error(CRT_E_INVALID, "crt_wide_build_gpu: a stack link leaves the node array");
+        if ((bad & kErrCap) || next_base + next > cap)
+            return set_error(CRT_E_INVALID, "crt_wide_build_gpu: more wide nodes than triangles (the links do not "
+                                            "form a tree)");
+        nf = next;
+        next_base += next;
+        cur = 1 - cur;
+    }
+    *wide_nodes = next_base;
+    return CRT_OK;
+}
+
+}  // namespace wg
+}  // namespace crt
+
+extern "C" {
+
+size_t crt_wide_build_gpu_workspace(int64_t ntris, int32_t width) {
+    if (ntris < 2 || (width != 4 && width != 8)) return 0;
+    return crt::wg::layout(ntris, width).bytes;
+}
+
+int crt_wide_build_gpu(const crt_stack_node* nodes, int64_t node_count, const crt_triangle* triangles, int64_t ntris,
+                       const crt_vertex* vertices, int64_t nverts, int32_t width, crt_wide_group* groups,
+                       crt_tri_record* records, int64_t* wide_nodes, void* workspace, size_t workspace_bytes,
+                       void* stream) {
+    using namespace crt::wg;
+    if (!nodes || !triangles || !vertices || !groups || !records || !wide_nodes || !workspace)
+        return crt::set_error(CRT_E_INVALID, "crt_wide_build_gpu: null argument");
+    if (width != 4 && width != 8) return crt::set_error(CRT_E_INVALID, "crt_wide_build_gpu: width must be 4 or 8");
+    if (ntris < 2 || node_count < 1 || nverts < 1 || ntris > 0x7FFFFFFFll)
+        return crt::set_error(CRT_E_INVALID, "crt_wide_build_gpu: needs 2 to 2^31 - 1 triangles and a root entry");
+    const Layout L = layout(ntris, width);
+    if (workspace_bytes < L.bytes || reinterpret_cast<uintptr_t>(workspace) % 256 != 0)
+        return crt::set_error(CRT_E_INVALID, "crt_wide_build_gpu: workspace must be 256-byte aligned and hold "
+                                             "crt_wide_build_gpu_workspace(ntris, width) = " +
+                                                 std::to_string(L.bytes) + " bytes");
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    char* ws = static_cast<char*>(workspace);
+    hipLaunchKernelGGL(seed, dim3(1), dim3(1), 0, st, reinterpret_cast<int32_t*>(ws + L.entries[0]),
+                       reinterpret_cast<int32_t*>(ws + L.wides[0]), reinterpret_cast<unsigned*>(ws + L.err));
+    hipLaunchKernelGGL(crt::wg::records, dim3(unsigned((ntris + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, triangles,
+                       vertices, nverts, ntris, records, reinterpret_cast<unsigned*>(ws + L.err));
+    return width == 4 ? collapse<4>(nodes, node_count, ntris, groups, wide_nodes, ws, L, st)
+                      : collapse<8>(nodes, node_count, ntris, groups, wide_nodes, ws, L, st);
+}
+
+}  // extern "C"

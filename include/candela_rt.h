@@ -441,6 +441,26 @@ const crt_wide_group* crt_wide_groups(const crt_wide* wide);       /* node_count
 const crt_tri_record* crt_wide_records(const crt_wide* wide);      /* record_count */
 const crt_entity* crt_wide_entities(const crt_wide* wide);         /* entity_count */
 
+/* Bytes of device workspace crt_wide_build_gpu needs (ntris >= 2, width 4 or 8). */
+size_t crt_wide_build_gpu_workspace(int64_t ntris, int32_t width);
+/* crt_wide_build of one object on the device (no reference counterpart; for
+ * trees built every frame by crt_build_bvh_gpu): the object's stack-layout
+ * entries nodes [node_count] (links local to them, entry 0 the root),
+ * triangles [ntris] and vertices [nverts], all device arrays, give the wide
+ * layout with 2-triangle leaves.  Every wide node has crt_wide_build's slots,
+ * boxes and slot order (max_leaf 2); nodes are numbered breadth-first rather
+ * than depth-first, so walks of either layout give identical hit records and
+ * counters.  groups: device room for ntris * width / 4 groups; records
+ * [ntris]; *wide_nodes (host) receives the node count (the entity's
+ * node_count, root 0).  Reads each tree level's size back, so it
+ * synchronizes `stream`; links and vertex indices are checked on the device
+ * (CRT_E_INVALID).  workspace: 256-byte aligned, crt_wide_build_gpu_workspace
+ * bytes. */
+int crt_wide_build_gpu(const crt_stack_node* nodes, int64_t node_count, const crt_triangle* triangles, int64_t ntris,
+                       const crt_vertex* vertices, int64_t nverts, int32_t width, crt_wide_group* groups,
+                       crt_tri_record* records, int64_t* wide_nodes, void* workspace, size_t workspace_bytes,
+                       void* stream);
+
 /* ------------------------------------------------------------------------- */
 /* RayIntersector<T> -- Intersector.h:60-351                                  */
 /* ------------------------------------------------------------------------- */

@@ -110,3 +110,80 @@ def test_build_errors(gpu):
                                         (1 << 28) - 5, b.nodes.ptr, b.triangles.ptr, b.workspace.ptr,
                                         b.workspace.nbytes, None))
     assert lib.crt_build_bvh_gpu_workspace(1) == 0
+
+
+def _bfs_renumber(groups, width, root=0):
+    """The wide layout renumbered breadth-first from `root` (children in slot order), as the device collapse
+    numbers it."""
+    import numpy as np
+    per = width // 4
+    child = groups["child"].reshape(-1, width)
+    count = groups["count"].reshape(-1, width)
+    order = [root]
+    for node in order:
+        for s in range(width):
+            if count[node, s] == 0:
+                order.append(int(child[node, s]))
+    new_of = {old: new for new, old in enumerate(order)}
+    out = np.concatenate([groups[old * per:(old + 1) * per] for old in order])
+    c = out["child"].reshape(-1, width)
+    k = out["count"].reshape(-1, width)
+    for i in range(c.shape[0]):
+        for s in range(width):
+            if k[i, s] == 0:
+                c[i, s] = new_of[int(c[i, s])]
+    out["child"] = c.reshape(out["child"].shape)
+    return out, len(order)
+
+
+@pytest.mark.parametrize("name", ["2", "3", "1000", "200000", "boxes", "cloud", "same", "flat"])
+def test_wide_collapse_on_device(gpu, orc, name):
+    """crt_wide_build_gpu over the device-built tree: node count, groups (renumbered breadth-first) and records
+    byte-identical to crt_wide_build on the host over the same buffers; walks of both layouts give identical hit
+    records and triangle-test counts."""
+    import oracle
+    log(f"[case] device wide collapse {name}")
+    verts, idx, mids = _scene(name)
+    b = gpu.GpuBVH(verts, idx, mids)
+    gpu.lib().crt_device_sync()
+    nodes, tris = b.nodes.download(), b.triangles.download()
+    ent = oracle.identity_entity(nodes.size)
+    for width in (4, 8):
+        w = gpu.GpuWide(b, width)
+        h = gpu.wide_build(1, nodes, tris, verts, ent, width, 2)
+        assert w.node_count == h.stats["node_count"]
+        ref, reached = _bfs_renumber(h.groups, width, int(h.entities["node_offset"][0]))
+        assert reached == w.node_count
+        got = w.groups.download()[:w.node_count * (width // 4)]
+        assert got.tobytes() == ref.tobytes(), f"groups differ ({name}, width {width})"
+        assert w.records.download().tobytes() == h.records.tobytes()
+        # the same visit order in both layouts: ties resolve alike
+        lo, hi = ([-60, -60, -60], [60, 60, 60]) if name.isdigit() or name == "flat" else ([-4, -4, -4], [64, 64, 64])
+        rays = random_rays(20000, lo, hi, seed=31)
+        for q in (0, 1):
+            g = gpu_trace(gpu, w.scene, q, rays, kernel=3, variant=0)
+            o = orc.trace_wide(width, h.groups, h.records, h.entities, q, rays)
+            compare(g, o, f"device collapse {name} w{width} q={q}")
+            assert g.tobytes() == o.tobytes()
+        log(f"[parity] device wide collapse {name} width {width}: {w.node_count} nodes byte-identical (breadth-first)")
+
+
+def test_wide_collapse_errors(gpu):
+    import ctypes as C
+    verts, idx, _ = _soup(10, 1)
+    b = gpu.GpuBVH(verts, idx)
+    lib = gpu.lib()
+    w = gpu.GpuWide(b, 4)
+    n = C.c_int64()
+    args = (b.nodes.ptr, b.nodes.count, b.triangles.ptr, b.ntris, b.vertices.ptr, b.vertices.count)
+    with pytest.raises(gpu.CrtError):           # width
+        gpu.check(lib.crt_wide_build_gpu(*args, 6, w.groups.ptr, w.records.ptr, C.byref(n), w.workspace.ptr,
+                                         w.workspace.nbytes, None))
+    with pytest.raises(gpu.CrtError):           # workspace too small
+        gpu.check(lib.crt_wide_build_gpu(*args, 4, w.groups.ptr, w.records.ptr, C.byref(n), w.workspace.ptr, 64,
+                                         None))
+    with pytest.raises(gpu.CrtError):           # a vertex index past the vertex array
+        gpu.check(lib.crt_wide_build_gpu(b.nodes.ptr, b.nodes.count, b.triangles.ptr, b.ntris, b.vertices.ptr, 2, 4,
+                                         w.groups.ptr, w.records.ptr, C.byref(n), w.workspace.ptr,
+                                         w.workspace.nbytes, None))
+    assert lib.crt_wide_build_gpu_workspace(1, 4) == 0 and lib.crt_wide_build_gpu_workspace(100, 5) == 0
