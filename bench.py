@@ -666,7 +666,31 @@ def main(argv=None) -> int:
                          gi_t_agreement_with_sah_tree=same_t)
         log(f"gpu-built tree: GI {ms_b:.3f} ms one ray per lane (SAH tree {sah_ms} ms), "
             f"nearest t agreement {same_t:.6f}")
-        del gb, dh_b
+        # the device BVH4 collapse of that tree (wall clock: it reads each level's size back), and the best
+        # refilling BVH4 GI kernel over it against the same kernel over the SAH tree's collapse
+        gw = crt.GpuWide(gb, 4, stream=stream)
+        w0 = time.perf_counter()
+        for _ in range(K):
+            gw.build(stream)
+        collapse_ms = (time.perf_counter() - w0) * 1e3 / K
+        k_w, v_w = (best["kernel"], best["variant"]) if best["layout"] == "wide4" else (crt.KERNEL_PERSISTENT, 8)
+        dh_w = crt.DeviceArray(crt.HIT_DTYPE, n_gi)
+        crt.trace(gw.scene, crt.QUERY_CLOSEST, d_gi, dh_w, kernel=k_w, variant=v_w, stream=stream)
+        stream.sync()
+        ev = Events(crt, 2)
+        ev.record(0, sh)
+        for _ in range(K):
+            crt.trace(gw.scene, crt.QUERY_CLOSEST, d_gi, dh_w, kernel=k_w, variant=v_w, stream=stream)
+        ev.record(1, sh)
+        ms_w = ev.ms(0, 1) / K
+        ev.close()
+        same_w = float(np.mean(dh_w.download()["t"] == out["gi"]["hits"]["t"]))
+        gpu_build.update(wide_collapse_ms_wall=round(collapse_ms, 3), wide_nodes=int(gw.node_count),
+                         gi_ms_wide_kernel=round(ms_w, 4), gi_wide_kernel=f"{k_w}:{v_w}",
+                         gi_t_agreement_wide=same_w)
+        log(f"device BVH4 collapse: {gw.node_count} nodes in {collapse_ms:.3f} ms (wall); GI {ms_w:.3f} ms with "
+            f"kernel {k_w}:{v_w} over it, nearest t agreement {same_w:.6f}")
+        del gb, dh_b, gw, dh_w
     except Exception as e:   # reported, never fatal to the headline line
         gpu_build = {"error": f"{type(e).__name__}: {e}"}
         log(f"gpu build failed: {gpu_build['error']}")
