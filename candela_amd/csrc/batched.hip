@@ -29,6 +29,7 @@
 #include "trace_launch.h"
 #include "wave_walk.h"
 #include "wide_walk.h"
+#include "thread_walk.h"
 
 namespace crt {
 
@@ -161,6 +162,52 @@ batched_wide_occ(CRT_BATCH_ARGS) {
     __shared__ int32_t lds[wide::lds_ints<W, LDS_DEPTH>() * 64 * WPG];
     wide_body<W, LDS_DEPTH, INNER, LEAF_BATCH, WPG>(descs, tiles, ntiles, per_claim, counters, max_iter, shards, lds);
 }
+// The stack format (FlattenStackBVH): each claimed tile runs the one-ray-per-
+// lane stack walk of crt_trace's thread kernel (thread_walk.h), its first
+// LDS_DEPTH entries in LDS; the walk's results do not depend on LDS_DEPTH, so
+// every hit record equals crt_trace's.
+template <int LDS_DEPTH, int WPG>
+__device__ __forceinline__ void stack_body(const BatchDesc* __restrict__ descs, const int2* __restrict__ tiles,
+                                           int32_t ntiles, int32_t per_claim, unsigned long long* __restrict__ counters,
+                                           int32_t max_iter, int32_t shards, int32_t* lds) {
+    ShardQueue q;
+    q.init(counters, ntiles, true, shards);
+    while (true) {
+        int32_t got = 0;
+        const int64_t base = q.claim(per_claim, got);
+        if (got == 0) break;
+        for (int32_t k = 0; k < got; ++k) {
+            const int32_t t = __builtin_amdgcn_readfirstlane(int32_t(q.at(base + k)));
+            const int2 tile = tiles[t];
+            const int32_t b = __builtin_amdgcn_readfirstlane(tile.x);
+            const int32_t first = __builtin_amdgcn_readfirstlane(tile.y);
+            const BatchDesc d = load_desc(descs, b);
+            const int64_t i = int64_t(first) + __lane_id();
+            if (i >= d.count) continue;
+            const crt_ray r = dev::load_ray(d.rays, i);
+            uint32_t* tt = d.tri_tests ? d.tri_tests + i : nullptr;
+            if (d.query == CRT_QUERY_ANY)
+                dev::trace_one<CRT_FORMAT_STACK, true, LDS_DEPTH>(d.scene, r, d.flags, max_iter, d.hits + i, tt,
+                                                                 lds + threadIdx.x, 64 * WPG);
+            else
+                dev::trace_one<CRT_FORMAT_STACK, false, LDS_DEPTH>(d.scene, r, d.flags, max_iter, d.hits + i, tt,
+                                                                  lds + threadIdx.x, 64 * WPG);
+        }
+    }
+}
+
+template <int LDS_DEPTH, int WPG>
+__global__ void __launch_bounds__(64 * WPG) batched_stack(CRT_BATCH_ARGS) {
+    __shared__ int32_t lds[LDS_DEPTH * 64 * WPG];
+    stack_body<LDS_DEPTH, WPG>(descs, tiles, ntiles, per_claim, counters, max_iter, shards, lds);
+}
+
+template <int LDS_DEPTH, int WPG, int OCC>
+__global__ void __launch_bounds__(64 * WPG) __attribute__((amdgpu_waves_per_eu(OCC, 8)))
+batched_stack_occ(CRT_BATCH_ARGS) {
+    __shared__ int32_t lds[LDS_DEPTH * 64 * WPG];
+    stack_body<LDS_DEPTH, WPG>(descs, tiles, ntiles, per_claim, counters, max_iter, shards, lds);
+}
 #undef CRT_BATCH_ARGS
 
 }  // namespace pdev
@@ -197,8 +244,25 @@ static const void* batched_wide_symbol(int variant) {
     }
 }
 
+// Stack presets: LDS stack entries per lane (0..5), then 8 entries compiled
+// for at least 5 (6) or 6 (7) waves per SIMD.
+template <int WPG>
+static const void* batched_stack_symbol(int variant) {
+    switch (variant) {
+        case 1: return reinterpret_cast<const void*>(&pdev::batched_stack<16, WPG>);
+        case 2: return reinterpret_cast<const void*>(&pdev::batched_stack<32, WPG>);
+        case 3: return reinterpret_cast<const void*>(&pdev::batched_stack<4, WPG>);
+        case 4: return reinterpret_cast<const void*>(&pdev::batched_stack<12, WPG>);
+        case 5: return reinterpret_cast<const void*>(&pdev::batched_stack<24, WPG>);
+        case 6: return reinterpret_cast<const void*>(&pdev::batched_stack_occ<8, WPG, 5>);
+        case 7: return reinterpret_cast<const void*>(&pdev::batched_stack_occ<8, WPG, 6>);
+        default: return reinterpret_cast<const void*>(&pdev::batched_stack<8, WPG>);
+    }
+}
+
 template <int WPG>
 static const void* kernel_for(int format, int variant) {
+    if (format == CRT_FORMAT_STACK) return batched_stack_symbol<WPG>(variant);
     if (format == CRT_FORMAT_WIDE4) return batched_wide_symbol<4, WPG>(variant);
     if (format == CRT_FORMAT_WIDE8) return batched_wide_symbol<8, WPG>(variant);
     return batched_symbol<WPG>(variant);
@@ -289,9 +353,9 @@ int build_plan(const crt_batch* batches, int nbatches, const crt_batch_config& c
     const bool wide = format == CRT_FORMAT_WIDE4 || format == CRT_FORMAT_WIDE8;
     for (int i = 0; i < nbatches; ++i) {
         const crt_batch& b = batches[i];
-        if (b.scene.format != CRT_FORMAT_STACKLESS && b.scene.format != CRT_FORMAT_WIDE4 &&
-            b.scene.format != CRT_FORMAT_WIDE8)
-            return crt::set_error(CRT_E_INVALID, "batched traversal walks the stackless and wide formats");
+        if (b.scene.format != CRT_FORMAT_STACKLESS && b.scene.format != CRT_FORMAT_STACK &&
+            b.scene.format != CRT_FORMAT_WIDE4 && b.scene.format != CRT_FORMAT_WIDE8)
+            return crt::set_error(CRT_E_INVALID, "batch: unknown node format");
         if (b.scene.format != format)
             return crt::set_error(CRT_E_INVALID, "batch plan: every batch must use the same node format");
         if (b.query != CRT_QUERY_CLOSEST && b.query != CRT_QUERY_ANY)

@@ -680,10 +680,11 @@ int crt_gen_probe_rays(const crt_probe_grid* grid, crt_ray* out, void* stream);
 /* Batched traversal: many scenes and ray batches in one persistent launch    */
 /* ------------------------------------------------------------------------- */
 
-/* One ray batch: a device scene (stackless, WIDE4 or WIDE8 format; every
- * batch of a plan uses the same one), a query, device rays and hits.  The
- * batch is traced exactly as crt_trace's wave kernel would trace it.  Node
- * width is tuned by planning the same batches over each layout. */
+/* One ray batch: a device scene (any format; every batch of a plan uses the
+ * same one), a query, device rays and hits.  The batch is traced exactly as
+ * crt_trace would trace it: the wave walk for the stackless and wide
+ * formats, the one-ray-per-lane stack walk for the stack format.  Node width
+ * is tuned by planning the same batches over each layout. */
 typedef struct crt_batch {
     crt_scene_view scene;
     int32_t query;          /* CRT_QUERY_* */
@@ -696,9 +697,9 @@ typedef struct crt_batch {
 
 /* Tuning of the batched kernel (crt_autotune_batched searches these). */
 typedef struct crt_batch_config {
-    int32_t variant;        /* wave-walk preset 0..7 (stackless: inner steps, leaf batch,
-                               window; wide: LDS stack depth, inner steps, leaf batch; 6 and 7
-                               are compiled for at least 5 or 6 waves per SIMD) */
+    int32_t variant;        /* walk preset 0..7 (stackless: inner steps, leaf batch, window;
+                               wide: LDS stack depth, inner steps, leaf batch; stack: LDS stack
+                               depth; 6 and 7 are compiled for at least 5 or 6 waves per SIMD) */
     int32_t tiles_per_claim;/* 64-ray tiles a wave claims per atomic: 1, 2 or 4 */
     int32_t waves_per_group;/* 1, 2 or 4 waves per workgroup */
     int32_t grid;           /* workgroups; 0 = enough to fill every CU */
