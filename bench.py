@@ -690,7 +690,57 @@ def main(argv=None) -> int:
                          gi_t_agreement_wide=same_w)
         log(f"device BVH4 collapse: {gw.node_count} nodes in {collapse_ms:.3f} ms (wall); GI {ms_w:.3f} ms with "
             f"kernel {k_w}:{v_w} over it, nearest t agreement {same_w:.6f}")
-        del gb, dh_b, gw, dh_w
+        # the asynchronous collapse (nothing read back): 64 levels, then sized to the levels the tree needs
+        # plus 2 (what a per-frame rebuild would launch); the whole per-frame rebuild (build, collapse, AO and
+        # GI traces) captured once into a graph and replayed K times
+        def time_async(levels):
+            g_ = crt.GpuWide(gb, 4, stream=stream, max_levels=levels)
+            stream.sync()
+            ev_ = Events(crt, 2)
+            ev_.record(0, sh)
+            for _ in range(K):
+                g_.build(stream)
+            ev_.record(1, sh)
+            ms_ = ev_.ms(0, 1) / K
+            ev_.close()
+            return g_, ms_, g_.result()
+
+        ga64, ms_async64, _ = time_async(64)
+        need = ga64.levels
+        del ga64
+        ga, ms_async, (n_async, status) = time_async(need + 2)
+        dh_ao = crt.DeviceArray(crt.HIT_DTYPE, n_ao)
+        for _ in range(2):
+            crt.trace(ga.scene, crt.QUERY_ANY, d_ao, dh_ao, kernel=k_w, variant=v_w, stream=stream)
+            crt.trace(ga.scene, crt.QUERY_CLOSEST, d_gi, dh_w, kernel=k_w, variant=v_w, stream=stream)
+        stream.sync()
+        graph = crt.Graph(stream)
+        with graph:
+            gb.build(stream)
+            ga.build(stream)
+            crt.trace(ga.scene, crt.QUERY_ANY, d_ao, dh_ao, kernel=k_w, variant=v_w, stream=stream)
+            crt.trace(ga.scene, crt.QUERY_CLOSEST, d_gi, dh_w, kernel=k_w, variant=v_w, stream=stream)
+        graph.launch()
+        stream.sync()
+        ev = Events(crt, 2)
+        ev.record(0, sh)
+        for _ in range(K):
+            graph.launch()
+        ev.record(1, sh)
+        ms_frame = ev.ms(0, 1) / K
+        ev.close()
+        graph.close()
+        same_frame = float(np.mean(dh_w.download()["t"] == out["gi"]["hits"]["t"]))
+        gpu_build.update(wide_collapse_async_ms_64_levels=round(ms_async64, 3), wide_levels=need,
+                         wide_collapse_async_ms=round(ms_async, 3), wide_collapse_async_levels=need + 2,
+                         wide_collapse_async_ok=bool(status == 0 and n_async == gw.node_count),
+                         rebuild_frame_graph_ms=round(ms_frame, 3),
+                         rebuild_frame_mrays_s=round((n_ao + n_gi) / (ms_frame * 1e-3) / 1e6, 1),
+                         rebuild_frame_gi_t_agreement=same_frame)
+        log(f"async BVH4 collapse: {ms_async64:.3f} ms at 64 levels, {ms_async:.3f} ms at {need + 2} (the tree "
+            f"needs {need}; status {status}, {n_async} nodes); rebuild frame in one "
+            f"graph (build + collapse + AO + GI): {ms_frame:.3f} ms, nearest t agreement {same_frame:.6f}")
+        del gb, dh_b, gw, dh_w, ga, dh_ao
     except Exception as e:   # reported, never fatal to the headline line
         gpu_build = {"error": f"{type(e).__name__}: {e}"}
         log(f"gpu build failed: {gpu_build['error']}")

@@ -20,7 +20,8 @@ from ._lib import (KERNEL_AUTO, KERNEL_PERSISTENT, KERNEL_THREAD_PER_RAY, KERNEL
                    STACKLESS_NODE_DTYPE, STATUS_BAD_INDEX, STATUS_ITER_CAP, STATUS_STACK_OVERFLOW,
                    STATUS_VISITS_MASK, SWAP_NONE, SWAP_SEEDED, TRIANGLE_DTYPE, VERTEX_DTYPE, BuildOptions,
                    BvhStats, CrtError, Mesh, SceneView, TraceConfig, Batch, BatchConfig, ShadeParams,
-                   SHADE_RECORD_DTYPE, RAYGEN_PARAMS_DTYPE, PROBE_GRID_DTYPE, CacheHeader, E_CORRUPT, E_IO, E_STALE, check, lib, node_dtype, ptr)
+                   SHADE_RECORD_DTYPE, RAYGEN_PARAMS_DTYPE, PROBE_GRID_DTYPE, CacheHeader, E_CORRUPT, E_IO, E_STALE, WIDE_GPU_BAD_LINK, WIDE_GPU_NOT_TREE,
+                   WIDE_GPU_BAD_VERTEX, WIDE_GPU_TOO_DEEP, WIDE_GPU_MAX_LEVELS, check, lib, node_dtype, ptr)
 
 __all__ = [
     "VERTEX_DTYPE", "TRIANGLE_DTYPE", "STACKLESS_NODE_DTYPE", "STACK_NODE_DTYPE", "BOUNDS_DTYPE", "ENTITY_DTYPE",
@@ -35,7 +36,8 @@ __all__ = [
     "TEXTURE_REF_DTYPE", "TEXTURE_DTYPE", "HIT_ATTR_DTYPE", "MaterialTable", "hit_attributes",
     "PATH_PARAMS_DTYPE", "PATH_SAMPLE_DTYPE", "path_params", "trace_paths", "hash64", "bvh_key", "load_bvh",
     "CacheHeader", "E_IO", "E_CORRUPT", "E_STALE", "RAYGEN_PARAMS_DTYPE", "PROBE_GRID_DTYPE", "raygen_params",
-    "RayGen", "probe_grid", "gen_probe_rays", "GpuBVH",
+    "RayGen", "probe_grid", "gen_probe_rays", "GpuBVH", "Graph", "WIDE_GPU_BAD_LINK", "WIDE_GPU_NOT_TREE",
+    "WIDE_GPU_BAD_VERTEX", "WIDE_GPU_TOO_DEEP", "WIDE_GPU_MAX_LEVELS",
 ]
 
 
@@ -57,6 +59,40 @@ class Stream:
     def close(self) -> None:
         if self.handle:
             lib().crt_stream_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Graph:
+    """Work captured from a Stream (crt_graph_begin / crt_graph_end) and replayed with launch()."""
+
+    def __init__(self, stream: "Stream"):
+        self.stream = stream
+        self.handle = None
+
+    def __enter__(self):
+        check(lib().crt_graph_begin(self.stream.handle))
+        return self
+
+    def __exit__(self, exc_type, exc, tb):
+        h = C.c_void_p()
+        rc = lib().crt_graph_end(self.stream.handle, C.byref(h))
+        if exc_type is None:
+            check(rc)
+            self.handle = h.value
+        return False
+
+    def launch(self, stream: "Stream | None" = None) -> None:
+        check(lib().crt_graph_launch(self.handle, _stream_handle(stream or self.stream)))
+
+    def close(self) -> None:
+        if self.handle:
+            lib().crt_graph_destroy(self.handle)
             self.handle = None
 
     def __del__(self):
@@ -420,32 +456,63 @@ class GpuBVH:
 
 class GpuWide:
     """crt_wide_build_gpu: the wide layout (2-triangle leaves) of a GpuBVH, collapsed on the device and numbered
-    breadth-first; `scene` is its crt_scene_view (one identity entity over the wide nodes)."""
+    breadth-first; `scene` is its crt_scene_view (one identity entity over the wide nodes).
 
-    def __init__(self, bvh: GpuBVH, width: int = 4, stream=None):
+    max_levels=None collapses with crt_wide_build_gpu (reads each level's size back; node_count is exact).
+    An integer uses crt_wide_build_gpu_async with that many levels: nothing is read back, so build() can be
+    captured into a Graph; the scene then spans ntris nodes (the capacity of `groups`, which bounds every
+    valid link), and result() reads the node count and status words."""
+
+    def __init__(self, bvh: GpuBVH, width: int = 4, stream=None, max_levels: int | None = None):
         if width not in (4, 8):
             raise ValueError("width must be 4 or 8")
         self.bvh = bvh
         self.width = width
+        self.max_levels = max_levels
         self.groups = DeviceArray(WIDE_GROUP_DTYPE, bvh.ntris * (width // 4))
         self.records = DeviceArray(TRI_RECORD_DTYPE, bvh.ntris)
-        self.workspace = DeviceArray(np.uint8, max(256, int(lib().crt_wide_build_gpu_workspace(bvh.ntris, width))))
+        if max_levels is None:
+            ws = lib().crt_wide_build_gpu_workspace(bvh.ntris, width)
+        else:
+            ws = lib().crt_wide_build_gpu_async_workspace(bvh.ntris, width, int(max_levels))
+            self.out = DeviceArray(np.int64, 3)   # [node count, levels needed, status]
+        self.workspace = DeviceArray(np.uint8, max(256, int(ws)))
         self.entities = DeviceArray(ENTITY_DTYPE, 1)
         self.node_count = 0
         self.scene = None
+        if max_levels is not None:
+            self.node_count = bvh.ntris
+            self._set_scene()
         self.build(stream)
 
-    def build(self, stream=None) -> None:
-        """Collapses the GpuBVH's current tree (synchronizes the stream)."""
+    def _set_scene(self) -> None:
         b = self.bvh
+        self.entities.upload(_identity_entity(self.node_count))
+        self.scene = SceneView(FORMAT_WIDE4 if self.width == 4 else FORMAT_WIDE8, 1, self.node_count,
+                               self.groups.ptr, b.triangles.ptr, b.vertices.ptr, self.entities.ptr, self.records.ptr)
+
+    def build(self, stream=None) -> None:
+        """Collapses the GpuBVH's current tree (synchronizes the stream unless max_levels was given)."""
+        b = self.bvh
+        if self.max_levels is not None:
+            check(lib().crt_wide_build_gpu_async(b.nodes.ptr, b.nodes.count, b.triangles.ptr, b.ntris,
+                                                 b.vertices.ptr, b.vertices.count, self.width, int(self.max_levels),
+                                                 self.groups.ptr, self.records.ptr, self.out.ptr, self.out.ptr + 16,
+                                                 self.workspace.ptr, self.workspace.nbytes, _stream_handle(stream)))
+            return
         n = C.c_int64()
         check(lib().crt_wide_build_gpu(b.nodes.ptr, b.nodes.count, b.triangles.ptr, b.ntris, b.vertices.ptr,
                                        b.vertices.count, self.width, self.groups.ptr, self.records.ptr, C.byref(n),
                                        self.workspace.ptr, self.workspace.nbytes, _stream_handle(stream)))
         self.node_count = int(n.value)
-        self.entities.upload(_identity_entity(self.node_count))
-        self.scene = SceneView(FORMAT_WIDE4 if self.width == 4 else FORMAT_WIDE8, 1, self.node_count,
-                               self.groups.ptr, b.triangles.ptr, b.vertices.ptr, self.entities.ptr, self.records.ptr)
+        self._set_scene()
+
+    def result(self) -> tuple[int, int]:
+        """(wide node count, CRT_WIDE_GPU_* status bits) of the last asynchronous build; synchronizes the device."""
+        check(lib().crt_device_sync())
+        out = self.out.download()
+        self.levels = int(out[1])
+        return int(out[0]), int(out[2] & 0xFFFFFFFF)
 
 
 class DeviceWide:

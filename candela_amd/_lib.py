@@ -35,6 +35,8 @@ KERNEL_AUTO = 0
 KERNEL_THREAD_PER_RAY = 1
 KERNEL_PERSISTENT = 2
 KERNEL_WAVE = 3
+WIDE_GPU_BAD_LINK, WIDE_GPU_NOT_TREE, WIDE_GPU_BAD_VERTEX, WIDE_GPU_TOO_DEEP = 1, 2, 4, 8
+WIDE_GPU_MAX_LEVELS = 4096
 
 # --- record dtypes (byte layouts of the reference buffers) ---------------------
 VERTEX_DTYPE = np.dtype([("position", "<f4", 4), ("normal_tangent_data", "<u4", 3), ("texcoords", "<u4")])
@@ -167,6 +169,10 @@ SIGNATURES = {
     "crt_wide_build_gpu": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_int32,
                                      C.c_void_p, C.c_void_p, C.POINTER(C.c_int64), C.c_void_p, C.c_size_t,
                                      C.c_void_p]),
+    "crt_wide_build_gpu_async_workspace": (C.c_size_t, [C.c_int64, C.c_int32, C.c_int32]),
+    "crt_wide_build_gpu_async": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64,
+                                           C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                           C.c_void_p, C.c_size_t, C.c_void_p]),
     "crt_build_bvh_gpu": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_void_p, C.c_int32, C.c_int32,
                                     C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
     "crt_gen_diffuse_rays": (C.c_int, [C.POINTER(SceneView), C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p,
@@ -191,6 +197,10 @@ SIGNATURES = {
     "crt_stream_destroy": (C.c_int, [C.c_void_p]),
     "crt_stream_sync": (C.c_int, [C.c_void_p]),
     "crt_device_sync": (C.c_int, []),
+    "crt_graph_begin": (C.c_int, [C.c_void_p]),
+    "crt_graph_end": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
+    "crt_graph_launch": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "crt_graph_destroy": (C.c_int, [C.c_void_p]),
     "crt_event_create": (C.c_int, [C.POINTER(C.c_void_p)]),
     "crt_event_destroy": (C.c_int, [C.c_void_p]),
     "crt_event_record": (C.c_int, [C.c_void_p, C.c_void_p]),

@@ -421,6 +421,38 @@ int crt_device_sync(void) {
     return CRT_OK;
 }
 
+int crt_graph_begin(void* stream) {
+    if (!stream) return crt::set_error(CRT_E_INVALID, "crt_graph_begin: capture needs a created stream, not the null stream");
+    CRT_HIP(hipStreamBeginCapture(static_cast<hipStream_t>(stream), hipStreamCaptureModeThreadLocal),
+            "hipStreamBeginCapture");
+    return CRT_OK;
+}
+
+int crt_graph_end(void* stream, void** graph) {
+    if (!stream || !graph) return crt::set_error(CRT_E_INVALID, "crt_graph_end: null argument");
+    *graph = nullptr;
+    hipGraph_t g = nullptr;
+    CRT_HIP(hipStreamEndCapture(static_cast<hipStream_t>(stream), &g), "hipStreamEndCapture");
+    hipGraphExec_t exec = nullptr;
+    const hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    CRT_HIP(e, "hipGraphInstantiate");
+    *graph = exec;
+    return CRT_OK;
+}
+
+int crt_graph_launch(void* graph, void* stream) {
+    if (!graph) return crt::set_error(CRT_E_INVALID, "crt_graph_launch: null graph");
+    CRT_HIP(hipGraphLaunch(static_cast<hipGraphExec_t>(graph), static_cast<hipStream_t>(stream)), "hipGraphLaunch");
+    return CRT_OK;
+}
+
+int crt_graph_destroy(void* graph) {
+    if (!graph) return CRT_OK;
+    CRT_HIP(hipGraphExecDestroy(static_cast<hipGraphExec_t>(graph)), "hipGraphExecDestroy");
+    return CRT_OK;
+}
+
 int crt_event_create(void** event) {
     if (!event) return crt::set_error(CRT_E_INVALID, "crt_event_create: null out");
     hipEvent_t e = nullptr;

@@ -120,6 +120,82 @@ __global__ void __launch_bounds__(kBlock) seed(int32_t* entry, int32_t* wide, un
     *err = 0u;
 }
 
+// The slots of one wide node w, the expansion of stack entry `entry`: writes
+// its groups and kids[slot] (the stack entry of an inner child, -1
+// otherwise); returns its number of inner children.
+template <int W>
+__device__ __forceinline__ uint32_t expand_one(const crt_stack_node* __restrict__ nodes, int64_t node_count,
+                                               int32_t entry, int64_t w, crt_wide_group* __restrict__ groups,
+                                               int32_t* __restrict__ kids_out, unsigned* __restrict__ err) {
+    const crt_stack_node& e = nodes[entry];
+    Kid kids[W];
+    int n = 0;
+    kids[n++] = make_kid(nodes, node_count, e.left, err);
+    kids[n++] = make_kid(nodes, node_count, e.right, err);
+    while (n < W) {
+        int best = -1;
+        float best_area = -1.0f;
+        for (int k = 0; k < n; ++k) {
+            if (kids[k].entry < 0) continue;
+            const float a = half_area(kids[k]);
+            if (a > best_area) {
+                best_area = a;
+                best = k;
+            }
+        }
+        if (best < 0) break;
+        const crt_stack_node& c = nodes[kids[best].entry];
+        for (int k = n; k > best + 1; --k) kids[k] = kids[k - 1];
+        kids[best + 1] = make_kid(nodes, node_count, c.right, err);
+        kids[best] = make_kid(nodes, node_count, c.left, err);
+        ++n;
+    }
+    uint32_t m = 0;
+    for (int s = 0; s < W; ++s) {
+        crt_wide_group& g = groups[w * (W / 4) + s / 4];
+        const int j = s % 4;
+        const bool used = s < n;
+        const Kid& k = kids[used ? s : 0];
+        g.lo_x[j] = used ? k.lo[0] : 0.0f;
+        g.lo_y[j] = used ? k.lo[1] : 0.0f;
+        g.lo_z[j] = used ? k.lo[2] : 0.0f;
+        g.hi_x[j] = used ? k.hi[0] : 0.0f;
+        g.hi_y[j] = used ? k.hi[1] : 0.0f;
+        g.hi_z[j] = used ? k.hi[2] : 0.0f;
+        const bool inner = used && k.entry >= 0;
+        g.child[j] = used && !inner ? k.start : 0;   // inner: assign writes it
+        g.count[j] = !used ? -1 : (inner ? 0 : k.count);
+        kids_out[s] = inner ? k.entry : -1;
+        m += inner ? 1u : 0u;
+    }
+    return m;
+}
+
+// Child indices of wide node w: its inner children are next_base + off + 0,
+// 1, ... in slot order, and entries off, off + 1, ... of the next level's
+// list.  False when an index reaches cap (the links do not form a tree).
+template <int W>
+__device__ __forceinline__ bool assign_one(int64_t w, const int32_t* __restrict__ kids, int64_t off,
+                                           int64_t next_base, int64_t cap, crt_wide_group* __restrict__ groups,
+                                           int32_t* __restrict__ next_entries, int32_t* __restrict__ next_wides,
+                                           unsigned* __restrict__ err) {
+    int64_t r = 0;
+    for (int s = 0; s < W; ++s) {
+        const int32_t e = kids[s];
+        if (e < 0) continue;
+        const int64_t idx = next_base + off + r;
+        if (idx >= cap) {   // nothing past the outputs is written
+            atomicOr(err, kErrCap);
+            return false;
+        }
+        groups[w * (W / 4) + s / 4].child[s % 4] = int32_t(idx);
+        next_entries[off + r] = e;
+        next_wides[off + r] = int32_t(idx);
+        ++r;
+    }
+    return true;
+}
+
 // Slots of the wide nodes of one level.  kids[item * W + slot]: the stack
 // entry of an inner child, -1 otherwise; counts[item]: inner children;
 // totals[block]: their sum per workgroup.
@@ -134,47 +210,7 @@ __global__ void __launch_bounds__(kBlock) expand(const crt_stack_node* __restric
     const int32_t item = int32_t(blockIdx.x) * kBlock + int32_t(threadIdx.x);
     uint32_t m = 0;
     if (item < nf) {
-        const crt_stack_node& e = nodes[entries[item]];
-        Kid kids[W];
-        int n = 0;
-        kids[n++] = make_kid(nodes, node_count, e.left, err);
-        kids[n++] = make_kid(nodes, node_count, e.right, err);
-        while (n < W) {
-            int best = -1;
-            float best_area = -1.0f;
-            for (int k = 0; k < n; ++k) {
-                if (kids[k].entry < 0) continue;
-                const float a = half_area(kids[k]);
-                if (a > best_area) {
-                    best_area = a;
-                    best = k;
-                }
-            }
-            if (best < 0) break;
-            const crt_stack_node& c = nodes[kids[best].entry];
-            for (int k = n; k > best + 1; --k) kids[k] = kids[k - 1];
-            kids[best + 1] = make_kid(nodes, node_count, c.right, err);
-            kids[best] = make_kid(nodes, node_count, c.left, err);
-            ++n;
-        }
-        const int64_t w = wides[item];
-        for (int s = 0; s < W; ++s) {
-            crt_wide_group& g = groups[w * (W / 4) + s / 4];
-            const int j = s % 4;
-            const bool used = s < n;
-            const Kid& k = kids[used ? s : 0];
-            g.lo_x[j] = used ? k.lo[0] : 0.0f;
-            g.lo_y[j] = used ? k.lo[1] : 0.0f;
-            g.lo_z[j] = used ? k.lo[2] : 0.0f;
-            g.hi_x[j] = used ? k.hi[0] : 0.0f;
-            g.hi_y[j] = used ? k.hi[1] : 0.0f;
-            g.hi_z[j] = used ? k.hi[2] : 0.0f;
-            const bool inner = used && k.entry >= 0;
-            g.child[j] = used && !inner ? k.start : 0;   // inner: assign writes it
-            g.count[j] = !used ? -1 : (inner ? 0 : k.count);
-            kids_out[int64_t(item) * W + s] = inner ? k.entry : -1;
-            m += inner ? 1u : 0u;
-        }
+        m = expand_one<W>(nodes, node_count, entries[item], wides[item], groups, kids_out + int64_t(item) * W, err);
         counts[item] = m;
     }
     uint32_t total;
@@ -197,22 +233,186 @@ __global__ void __launch_bounds__(kBlock) assign(const int32_t* __restrict__ wid
     uint32_t total;
     const uint32_t local = block_excl(m, wave_sums, total);
     if (item >= nf) return;
-    const int64_t off = block_offsets[blockIdx.x] + local;
-    const int64_t w = wides[item];
-    int64_t r = 0;
-    for (int s = 0; s < W; ++s) {
-        const int32_t e = kids[int64_t(item) * W + s];
-        if (e < 0) continue;
-        const int64_t idx = next_base + off + r;
-        if (idx >= cap) {   // not a tree: nothing past the outputs is written
-            atomicOr(err, kErrCap);
-            return;
+    (void)assign_one<W>(wides[item], kids + int64_t(item) * W, block_offsets[blockIdx.x] + local, next_base, cap,
+                        groups, next_entries, next_wides, err);
+}
+
+// ---- the asynchronous collapse (crt_wide_build_gpu_async) ----------------
+//
+// The host launches a fixed number of levels and reads nothing back, so the
+// whole collapse can be captured into a graph.  Level state lives on the
+// device, two int64 per level l: lvl[2l] is the number of wide nodes numbered
+// through level l (the first index of level l + 1) and lvl[2l + 1] the size of
+// level l.  Each level launches kGrid-or-fewer workgroups over contiguous
+// chunks of the level (so the breadth-first numbering stays the host loop's),
+// and a level of size 0 returns at once.
+
+// Bit of *err: the tree has more levels than were launched.
+constexpr unsigned kErrDepth = 8u;
+static_assert(kErrLink == CRT_WIDE_GPU_BAD_LINK && kErrCap == CRT_WIDE_GPU_NOT_TREE &&
+                  kErrVertex == CRT_WIDE_GPU_BAD_VERTEX && kErrDepth == CRT_WIDE_GPU_TOO_DEEP,
+              "status bits are the header's");
+constexpr int kGridMax = 2048;
+
+// Exclusive prefix of v over the workgroup and the total, then a barrier, so
+// that the call can repeat in a loop.
+__device__ __forceinline__ uint32_t block_excl_loop(uint32_t v, uint32_t* wave_sums, uint32_t& total) {
+    const uint32_t r = block_excl(v, wave_sums, total);
+    __syncthreads();
+    return r;
+}
+
+// Sum over the workgroup (kBlock threads); every thread gets it.
+__device__ __forceinline__ uint64_t block_sum64(uint64_t v, uint64_t* wave_sums) {
+    for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
+    if ((threadIdx.x & 63) == 0) wave_sums[threadIdx.x >> 6] = v;
+    __syncthreads();
+    uint64_t t = 0;
+    for (int w = 0; w < kBlock / 64; ++w) t += wave_sums[w];
+    __syncthreads();
+    return t;
+}
+
+// Items [begin, end) of a level of nf wide nodes for this workgroup of a grid
+// of `grid`; chunks are whole multiples of kBlock.
+__device__ __forceinline__ void chunk_of(int64_t nf, int32_t grid, int64_t& begin, int64_t& end) {
+    const int64_t pieces = (nf + kBlock - 1) / kBlock;
+    const int64_t per = (pieces + grid - 1) / grid * kBlock;
+    begin = int64_t(blockIdx.x) * per;
+    end = min(nf, begin + per);
+}
+
+// Level 0 and the error word.  A root entry that holds two adjacent
+// 1-triangle leaves is one 2-triangle leaf under a root of one slot
+// (crt_wide_build's rule): this kernel writes that root, and level 0 is then
+// empty.  Otherwise level 0 is the root entry.
+template <int W>
+__global__ void __launch_bounds__(64) root_async(const crt_stack_node* __restrict__ nodes, int32_t* __restrict__ entry,
+                                                 int32_t* __restrict__ wide, int64_t* __restrict__ lvl,
+                                                 crt_wide_group* __restrict__ groups, unsigned* __restrict__ err) {
+    if (threadIdx.x != 0) return;
+    *err = 0u;
+    entry[0] = 0;
+    wide[0] = 0;
+    const crt_bounds l = nodes[0].left, r = nodes[0].right;
+    const int32_t a = __float_as_int(l.min[3]), c = __float_as_int(r.min[3]);
+    const int32_t sa = a >> 4, na = a & 0xF, sc = c >> 4, nc = c & 0xF;
+    const bool one = a != -1 && c != -1 && na + nc <= 2 && max(sa + na, sc + nc) - min(sa, sc) == na + nc;
+    if (one) {
+        for (int q = 0; q < W / 4; ++q) {
+            crt_wide_group& g = groups[q];
+            for (int j = 0; j < 4; ++j) {
+                const bool s0 = q == 0 && j == 0;
+                g.lo_x[j] = s0 ? fminf(l.min[0], r.min[0]) : 0.0f;
+                g.lo_y[j] = s0 ? fminf(l.min[1], r.min[1]) : 0.0f;
+                g.lo_z[j] = s0 ? fminf(l.min[2], r.min[2]) : 0.0f;
+                g.hi_x[j] = s0 ? fmaxf(l.max[0], r.max[0]) : 0.0f;
+                g.hi_y[j] = s0 ? fmaxf(l.max[1], r.max[1]) : 0.0f;
+                g.hi_z[j] = s0 ? fmaxf(l.max[2], r.max[2]) : 0.0f;
+                g.child[j] = s0 ? min(sa, sc) : 0;
+                g.count[j] = s0 ? na + nc : -1;
+            }
         }
-        groups[w * (W / 4) + s / 4].child[s % 4] = int32_t(idx);
-        next_entries[off + r] = e;
-        next_wides[off + r] = int32_t(idx);
-        ++r;
     }
+    lvl[0] = 1;
+    lvl[1] = one ? 0 : 1;
+}
+
+// expand over the chunks of level `level`; totals[block] for every block of
+// the grid when the level is not empty.
+template <int W>
+__global__ void __launch_bounds__(kBlock) expand_async(const crt_stack_node* __restrict__ nodes, int64_t node_count,
+                                                       const int64_t* __restrict__ lvl, int32_t level, int32_t grid,
+                                                       const int32_t* __restrict__ entries,
+                                                       const int32_t* __restrict__ wides,
+                                                       crt_wide_group* __restrict__ groups,
+                                                       int32_t* __restrict__ kids_out, uint32_t* __restrict__ counts,
+                                                       uint32_t* __restrict__ totals, unsigned* __restrict__ err) {
+    __shared__ uint32_t wave_sums[kBlock / 64];
+    const int64_t nf = lvl[2 * level + 1];
+    if (nf == 0) return;
+    int64_t begin, end;
+    chunk_of(nf, grid, begin, end);
+    uint32_t run = 0;
+    for (int64_t base = begin; base < end; base += kBlock) {
+        const int64_t item = base + threadIdx.x;
+        uint32_t m = 0;
+        if (item < end) {
+            m = expand_one<W>(nodes, node_count, entries[item], wides[item], groups, kids_out + item * W, err);
+            counts[item] = m;
+        }
+        uint32_t total;
+        (void)block_excl_loop(m, wave_sums, total);
+        run += total;
+    }
+    if (threadIdx.x == 0) totals[blockIdx.x] = run;
+}
+
+// assign over the same chunks; block 0 also writes level + 1's state.  A tree
+// whose next level would pass cap stops there (kErrCap).
+template <int W>
+__global__ void __launch_bounds__(kBlock) assign_async(int64_t* __restrict__ lvl, int32_t level, int32_t grid,
+                                                       int64_t cap, const int32_t* __restrict__ wides,
+                                                       const int32_t* __restrict__ kids,
+                                                       const uint32_t* __restrict__ counts,
+                                                       const uint32_t* __restrict__ totals,
+                                                       crt_wide_group* __restrict__ groups,
+                                                       int32_t* __restrict__ next_entries,
+                                                       int32_t* __restrict__ next_wides, unsigned* __restrict__ err) {
+    __shared__ uint32_t wave_sums[kBlock / 64];
+    __shared__ uint64_t sums[kBlock / 64];
+    const int64_t nf = lvl[2 * level + 1], next_base = lvl[2 * level];
+    if (nf == 0) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            lvl[2 * level + 2] = next_base;
+            lvl[2 * level + 3] = 0;
+        }
+        return;
+    }
+    uint64_t before = 0, all = 0;
+    for (int32_t j = int32_t(threadIdx.x); j < grid; j += kBlock) {
+        const uint64_t v = totals[j];
+        all += v;
+        before += j < int32_t(blockIdx.x) ? v : 0u;
+    }
+    before = block_sum64(before, sums);
+    all = block_sum64(all, sums);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        const bool over = int64_t(next_base + all) > cap;
+        if (over) atomicOr(err, kErrCap);
+        lvl[2 * level + 2] = over ? next_base : next_base + int64_t(all);
+        lvl[2 * level + 3] = over ? 0 : int64_t(all);
+    }
+    int64_t begin, end;
+    chunk_of(nf, grid, begin, end);
+    int64_t run = int64_t(before);
+    for (int64_t base = begin; base < end; base += kBlock) {
+        const int64_t item = base + threadIdx.x;
+        const uint32_t m = item < end ? counts[item] : 0u;
+        uint32_t total;
+        const uint32_t local = block_excl_loop(m, wave_sums, total);
+        if (item < end)
+            (void)assign_one<W>(wides[item], kids + item * W, run + local, next_base, cap, groups, next_entries,
+                                next_wides, err);
+        run += total;
+    }
+}
+
+// The node count, the number of levels the tree needs (the first empty
+// level; max_levels when none was) and the status word for the caller.
+__global__ void __launch_bounds__(64) finish_async(const int64_t* __restrict__ lvl, int32_t levels,
+                                                   const unsigned* __restrict__ err, int64_t* __restrict__ wide_nodes,
+                                                   uint32_t* __restrict__ status) {
+    if (threadIdx.x != 0) return;
+    int32_t used = levels;
+    for (int32_t l = 0; l < levels; ++l)
+        if (lvl[2 * l + 1] == 0) {
+            used = l;
+            break;
+        }
+    wide_nodes[0] = lvl[2 * levels];
+    wide_nodes[1] = used;
+    *status = *err | (lvl[2 * levels + 1] > 0 ? kErrDepth : 0u);
 }
 
 // Triangle records in leaf order (crt_wide_build's).
@@ -244,11 +444,12 @@ __global__ void __launch_bounds__(kBlock) records(const crt_triangle* __restrict
 inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
 struct Layout {
-    size_t entries[2], wides[2], kids, counts, totals, offsets, total, err, bytes;
+    size_t entries[2], wides[2], kids, counts, totals, offsets, total, err, lvl, bytes;
 };
 
-// Levels hold at most `cap` wide nodes (a valid tree has fewer wide nodes than triangles).
-inline Layout layout(int64_t cap, int32_t width) {
+// Levels hold at most `cap` wide nodes (a valid tree has fewer wide nodes
+// than triangles); `levels` > 0 adds the asynchronous collapse's level state.
+inline Layout layout(int64_t cap, int32_t width, int32_t levels = 0) {
     const size_t n = size_t(cap), nb = size_t((cap + kBlock - 1) / kBlock);
     Layout L{};
     size_t off = 0;
@@ -267,6 +468,7 @@ inline Layout layout(int64_t cap, int32_t width) {
     L.offsets = take(nb * 8);
     L.total = take(8);
     L.err = take(4);
+    L.lvl = take(levels > 0 ? size_t(2) * size_t(levels + 1) * 8 : 0);
     L.bytes = off;
     return L;
 }
@@ -353,6 +555,32 @@ int collapse(const crt_stack_node* nodes, int64_t node_count, int64_t cap, crt_w
     return CRT_OK;
 }
 
+template <int W>
+void collapse_async(const crt_stack_node* nodes, int64_t node_count, const crt_triangle* tris,
+                    const crt_vertex* verts, int64_t nverts, int64_t cap, int32_t levels, crt_wide_group* groups,
+                    crt_tri_record* recs, int64_t* wide_nodes, uint32_t* status, char* ws, const Layout& L,
+                    hipStream_t st) {
+    int32_t* entries[2] = {reinterpret_cast<int32_t*>(ws + L.entries[0]), reinterpret_cast<int32_t*>(ws + L.entries[1])};
+    int32_t* wides[2] = {reinterpret_cast<int32_t*>(ws + L.wides[0]), reinterpret_cast<int32_t*>(ws + L.wides[1])};
+    auto* kids = reinterpret_cast<int32_t*>(ws + L.kids);
+    auto* counts = reinterpret_cast<uint32_t*>(ws + L.counts);
+    auto* totals = reinterpret_cast<uint32_t*>(ws + L.totals);
+    auto* lvl = reinterpret_cast<int64_t*>(ws + L.lvl);
+    auto* err = reinterpret_cast<unsigned*>(ws + L.err);
+    const int32_t grid = int32_t(std::min<int64_t>(kGridMax, (cap + kBlock - 1) / kBlock));
+    hipLaunchKernelGGL(root_async<W>, dim3(1), dim3(64), 0, st, nodes, entries[0], wides[0], lvl, groups, err);
+    hipLaunchKernelGGL(records, dim3(unsigned((cap + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, tris, verts, nverts,
+                       cap, recs, err);
+    for (int32_t l = 0; l < levels; ++l) {
+        const int cur = l & 1;
+        hipLaunchKernelGGL(expand_async<W>, dim3(grid), dim3(kBlock), 0, st, nodes, node_count, lvl, l, grid,
+                           entries[cur], wides[cur], groups, kids, counts, totals, err);
+        hipLaunchKernelGGL(assign_async<W>, dim3(grid), dim3(kBlock), 0, st, lvl, l, grid, cap, wides[cur], kids, counts,
+                           totals, groups, entries[1 - cur], wides[1 - cur], err);
+    }
+    hipLaunchKernelGGL(finish_async, dim3(1), dim3(64), 0, st, lvl, levels, err, wide_nodes, status);
+}
+
 }  // namespace wg
 }  // namespace crt
 
@@ -386,6 +614,46 @@ int crt_wide_build_gpu(const crt_stack_node* nodes, int64_t node_count, const cr
                        vertices, nverts, ntris, records, reinterpret_cast<unsigned*>(ws + L.err));
     return width == 4 ? collapse<4>(nodes, node_count, ntris, groups, wide_nodes, ws, L, st)
                       : collapse<8>(nodes, node_count, ntris, groups, wide_nodes, ws, L, st);
+}
+
+size_t crt_wide_build_gpu_async_workspace(int64_t ntris, int32_t width, int32_t max_levels) {
+    if (ntris < 2 || (width != 4 && width != 8) || max_levels < 1 || max_levels > CRT_WIDE_GPU_MAX_LEVELS) return 0;
+    return crt::wg::layout(ntris, width, max_levels).bytes;
+}
+
+int crt_wide_build_gpu_async(const crt_stack_node* nodes, int64_t node_count, const crt_triangle* triangles,
+                             int64_t ntris, const crt_vertex* vertices, int64_t nverts, int32_t width,
+                             int32_t max_levels, crt_wide_group* groups, crt_tri_record* records,
+                             int64_t* wide_nodes, uint32_t* status, void* workspace, size_t workspace_bytes,
+                             void* stream) {
+    using namespace crt::wg;
+    if (!nodes || !triangles || !vertices || !groups || !records || !wide_nodes || !status || !workspace)
+        return crt::set_error(CRT_E_INVALID, "crt_wide_build_gpu_async: null argument");
+    if (width != 4 && width != 8)
+        return crt::set_error(CRT_E_INVALID, "crt_wide_build_gpu_async: width must be 4 or 8");
+    if (max_levels < 1 || max_levels > CRT_WIDE_GPU_MAX_LEVELS)
+        return crt::set_error(CRT_E_INVALID, "crt_wide_build_gpu_async: max_levels must be 1 to " +
+                                                 std::to_string(CRT_WIDE_GPU_MAX_LEVELS));
+    if (ntris < 2 || node_count < 1 || nverts < 1 || ntris > 0x7FFFFFFFll)
+        return crt::set_error(CRT_E_INVALID,
+                              "crt_wide_build_gpu_async: needs 2 to 2^31 - 1 triangles and a root entry");
+    const Layout L = layout(ntris, width, max_levels);
+    if (workspace_bytes < L.bytes || reinterpret_cast<uintptr_t>(workspace) % 256 != 0)
+        return crt::set_error(CRT_E_INVALID, "crt_wide_build_gpu_async: workspace must be 256-byte aligned and hold "
+                                             "crt_wide_build_gpu_async_workspace(ntris, width, max_levels) = " +
+                                                 std::to_string(L.bytes) + " bytes");
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    char* ws = static_cast<char*>(workspace);
+    if (width == 4)
+        collapse_async<4>(nodes, node_count, triangles, vertices, nverts, ntris, max_levels, groups, records,
+                          wide_nodes, status, ws, L, st);
+    else
+        collapse_async<8>(nodes, node_count, triangles, vertices, nverts, ntris, max_levels, groups, records,
+                          wide_nodes, status, ws, L, st);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+        return crt::set_error(CRT_E_HIP, std::string("crt_wide_build_gpu_async: ") + hipGetErrorString(e));
+    return CRT_OK;
 }
 
 }  // extern "C"

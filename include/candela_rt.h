@@ -316,6 +316,17 @@ int crt_stream_create(void** stream);
 int crt_stream_destroy(void* stream);
 int crt_stream_sync(void* stream);
 int crt_device_sync(void);
+/* Stream capture into a replayable graph, for callers without HIP headers
+ * (no reference counterpart).  crt_graph_begin starts capturing `stream`
+ * (thread-local mode): the asynchronous calls made on it afterwards are
+ * recorded, not run.  Calls that synchronize or read back (crt_wide_build_gpu,
+ * crt_context_* uploads) fail while capturing.  crt_graph_end ends the
+ * capture and instantiates it into *graph; crt_graph_launch replays it on a
+ * stream.  A replay reads the buffers' contents at that time. */
+int crt_graph_begin(void* stream);
+int crt_graph_end(void* stream, void** graph);
+int crt_graph_launch(void* graph, void* stream);
+int crt_graph_destroy(void* graph);
 /* HIP events for timing work on a stream. */
 int crt_event_create(void** event);
 int crt_event_destroy(void* event);
@@ -460,6 +471,35 @@ int crt_wide_build_gpu(const crt_stack_node* nodes, int64_t node_count, const cr
                        const crt_vertex* vertices, int64_t nverts, int32_t width, crt_wide_group* groups,
                        crt_tri_record* records, int64_t* wide_nodes, void* workspace, size_t workspace_bytes,
                        void* stream);
+
+/* Status bits of crt_wide_build_gpu_async (0: the layout is complete). */
+#define CRT_WIDE_GPU_BAD_LINK 1u     /* a stack link leaves the node array */
+#define CRT_WIDE_GPU_NOT_TREE 2u     /* more wide nodes than triangles */
+#define CRT_WIDE_GPU_BAD_VERTEX 4u   /* a vertex index leaves the vertex array */
+#define CRT_WIDE_GPU_TOO_DEEP 8u     /* the wide tree has more than max_levels levels */
+#define CRT_WIDE_GPU_MAX_LEVELS 4096
+
+/* Bytes of device workspace crt_wide_build_gpu_async needs (0 for invalid
+ * arguments). */
+size_t crt_wide_build_gpu_async_workspace(int64_t ntris, int32_t width, int32_t max_levels);
+/* crt_wide_build_gpu without any host readback (no reference counterpart; for
+ * a per-frame rebuild captured into a graph with crt_build_bvh_gpu and the
+ * traces).  It launches max_levels tree levels (1 to CRT_WIDE_GPU_MAX_LEVELS);
+ * a level past the tree's depth returns at once.  The output is
+ * crt_wide_build_gpu's, byte for byte.  wide_nodes (int64 [2]) and status
+ * (uint32) are DEVICE pointers written at the end: the node count and the
+ * number of levels the tree needs (max_levels at least that gives status 0;
+ * a caller that rebuilds every frame can size the next frame's launch from
+ * it), and the CRT_WIDE_GPU_* bits.  Nothing is read back, so errors in the data show only
+ * in *status; argument errors return CRT_E_INVALID.  A trace can run before
+ * the node count is known: the scene's node count may be ntris (the capacity
+ * of `groups`), which bounds every valid link.  workspace: 256-byte aligned,
+ * crt_wide_build_gpu_async_workspace bytes. */
+int crt_wide_build_gpu_async(const crt_stack_node* nodes, int64_t node_count, const crt_triangle* triangles,
+                             int64_t ntris, const crt_vertex* vertices, int64_t nverts, int32_t width,
+                             int32_t max_levels, crt_wide_group* groups, crt_tri_record* records,
+                             int64_t* wide_nodes, uint32_t* status, void* workspace, size_t workspace_bytes,
+                             void* stream);
 
 /* ------------------------------------------------------------------------- */
 /* RayIntersector<T> -- Intersector.h:60-351                                  */

@@ -187,3 +187,132 @@ def test_wide_collapse_errors(gpu):
                                          w.groups.ptr, w.records.ptr, C.byref(n), w.workspace.ptr,
                                          w.workspace.nbytes, None))
     assert lib.crt_wide_build_gpu_workspace(1, 4) == 0 and lib.crt_wide_build_gpu_workspace(100, 5) == 0
+
+
+@pytest.mark.parametrize("name", ["2", "3", "1000", "200000", "boxes", "cloud", "same", "flat"])
+def test_wide_collapse_async(gpu, name):
+    """crt_wide_build_gpu_async (64 levels, nothing read back) writes crt_wide_build_gpu's node count, groups and
+    records byte for byte, twice over the same workspace; its scene, which spans ntris nodes, traces to the same
+    hit records as the exact one."""
+    log(f"[case] async wide collapse {name}")
+    verts, idx, mids = _scene(name)
+    b = gpu.GpuBVH(verts, idx, mids)
+    lo, hi = ([-60, -60, -60], [60, 60, 60]) if name.isdigit() or name == "flat" else ([-4, -4, -4], [64, 64, 64])
+    rays = random_rays(20000, lo, hi, seed=37)
+    for width in (4, 8):
+        w = gpu.GpuWide(b, width)
+        a = gpu.GpuWide(b, width, max_levels=64)
+        for rep in range(2):
+            if rep:
+                a.groups.fill_bytes(0x7F)
+                a.build()
+            n, status = a.result()
+            assert status == 0, f"status {status:#x} ({name}, width {width})"
+            assert n == w.node_count
+            per = width // 4
+            assert a.groups.download()[:n * per].tobytes() == w.groups.download()[:n * per].tobytes()
+            assert a.records.download().tobytes() == w.records.download().tobytes()
+        for q in (0, 1):
+            ga = gpu_trace(gpu, a.scene, q, rays, kernel=3, variant=0)
+            gw = gpu_trace(gpu, w.scene, q, rays, kernel=3, variant=0)
+            assert ga.tobytes() == gw.tobytes(), f"async scene traces differ ({name}, width {width}, q={q})"
+        log(f"[parity] async wide collapse {name} width {width}: {n} nodes, identical to the synchronous collapse")
+
+
+def test_wide_collapse_async_errors(gpu):
+    import ctypes as C
+    lib = gpu.lib()
+    verts, idx, _ = _soup(1000, 3)
+    b = gpu.GpuBVH(verts, idx)
+    a = gpu.GpuWide(b, 4, max_levels=64)
+    n_ok, st = a.result()
+    assert st == 0 and n_ok == gpu.GpuWide(b, 4).node_count
+    args = (b.nodes.ptr, b.nodes.count, b.triangles.ptr, b.ntris, b.vertices.ptr, b.vertices.count)
+
+    def call(*mid, nodes=None, nverts=None, ws=None):
+        a0 = list(args)
+        if nodes is not None:
+            a0[0] = nodes
+        if nverts is not None:
+            a0[5] = nverts
+        return lib.crt_wide_build_gpu_async(*a0, *mid, a.groups.ptr, a.records.ptr, a.out.ptr, a.out.ptr + 16,
+                                            a.workspace.ptr, a.workspace.nbytes if ws is None else ws, None)
+
+    for bad in ((6, 64), (4, 0), (4, gpu.WIDE_GPU_MAX_LEVELS + 1)):
+        with pytest.raises(gpu.CrtError):
+            gpu.check(call(*bad))
+    with pytest.raises(gpu.CrtError):           # workspace too small
+        gpu.check(call(4, 64, ws=64))
+    assert lib.crt_wide_build_gpu_async_workspace(1, 4, 8) == 0
+    assert lib.crt_wide_build_gpu_async_workspace(100, 4, 0) == 0
+    assert lib.crt_wide_build_gpu_async(None, 1, None, 2, None, 1, 4, 8, None, None, None, None, None, 0, None) < 0
+    # data errors come back in the status word
+    gpu.check(call(4, 64, nverts=2))
+    assert a.result()[1] & gpu.WIDE_GPU_BAD_VERTEX
+    gpu.check(call(4, 64))
+    assert a.result()[1] == 0
+    need = a.levels
+    assert 2 <= need < 64
+    gpu.check(call(4, need))                    # exactly the levels the tree needs
+    assert a.result() == (n_ok, 0)
+    gpu.check(call(4, need - 1))
+    assert a.result()[1] == gpu.WIDE_GPU_TOO_DEEP
+    gpu.check(call(4, 1))
+    assert a.result()[1] == gpu.WIDE_GPU_TOO_DEEP
+    nodes = b.nodes.download()
+    assert nodes[0]["left"]["min"][3].view(np.int32) == -1          # the root's left child is inner
+    nodes[0]["left"]["max"][3] = np.array([b.nodes.count + 5], np.int32).view(np.float32)[0]
+    bad_nodes = gpu.DeviceArray.from_host(nodes)
+    gpu.check(call(4, 64, nodes=bad_nodes.ptr))
+    assert a.result()[1] & gpu.WIDE_GPU_BAD_LINK
+    gpu.check(call(4, 64))
+    assert a.result()[1] == 0
+
+
+def test_rebuild_graph(gpu):
+    """The per-frame rebuild in one graph: crt_build_bvh_gpu, crt_wide_build_gpu_async and a closest-hit and an
+    any-hit trace over the new layout, captured once and replayed on moving geometry (eager uploads and
+    synchronizations between replays).  Every replay gives the eager synchronous pipeline's hit records."""
+    from candela_amd import scenes
+    from conftest import concat_meshes
+    verts, idx, mids = concat_meshes(scenes.icosphere_cloud(count=150, subdivisions=3, seed=8, box=50.0))
+    rays = random_rays(30000, [-4, -4, -4], [54, 54, 54], seed=41)
+    s = gpu.Stream()
+    b = gpu.GpuBVH(verts, idx, mids, stream=s)
+    a = gpu.GpuWide(b, 4, stream=s, max_levels=64)
+    d_rays = gpu.DeviceArray.from_host(rays)
+    hits = [gpu.DeviceArray(gpu.HIT_DTYPE, rays.size) for _ in range(2)]
+    for q in (0, 1):   # eager warm-up
+        gpu.trace(a.scene, q, d_rays, hits[q], kernel=3, variant=0, stream=s)
+    s.sync()
+    g = gpu.Graph(s)
+    with g:
+        b.build(s)
+        a.build(s)
+        for q in (0, 1):
+            gpu.trace(a.scene, q, d_rays, hits[q], kernel=3, variant=0, stream=s)
+    assert g.handle
+    rng = np.random.default_rng(9)
+    ref_b = gpu.GpuBVH(verts, idx, mids)
+    for frame in range(4):
+        moved = verts.copy()
+        moved["position"][:, :3] += rng.normal(scale=0.3, size=(verts.size, 3)).astype(np.float32)
+        b.vertices.upload(moved)
+        for h in hits:
+            h.fill_bytes(0xFF)
+        s.sync()
+        g.launch()
+        s.sync()
+        got = [h.download() for h in hits]
+        n, status = a.result()
+        # the eager synchronous pipeline on the same vertices
+        ref_b.vertices.upload(moved)
+        ref_b.build()
+        w = gpu.GpuWide(ref_b, 4)
+        assert (n, status) == (w.node_count, 0)
+        for q in (0, 1):
+            ref = gpu_trace(gpu, w.scene, q, rays, kernel=3, variant=0)
+            assert got[q].tobytes() == ref.tobytes(), f"replay {frame} query {q} differs from the eager pipeline"
+        log(f"[parity] rebuild graph replay {frame}: {n} wide nodes, hit records identical to the eager pipeline")
+    g.close()
+    s.close()
