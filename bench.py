@@ -767,6 +767,8 @@ def main(argv=None) -> int:
 
         plain_choice = {}
         b2 = best_of(lambda r: r["layout"] == "bvh2")
+        if b2 is None:   # --only named a wide layout: the BVH2 plain launches take the wave kernel, preset 1
+            b2 = bvh2(crt.KERNEL_WAVE, 1)
         plain_choice[None] = (b2["kernel"], b2["variant"], *ao_of(b2), 0)
         for w in (4, 8):
             bw = best_of(lambda r: r["layout"] == f"wide{w}" and r["max_leaf"] == 2)

@@ -35,37 +35,47 @@ __device__ __forceinline__ uint32_t block_rank(bool flag, uint32_t* wave_counts,
 }
 
 // Exclusive scan of nb counts by one workgroup of kScanBlock threads; the
-// total goes to *total.
+// total goes to *total.  A pass covers kScanBlock * kScanPer counts: each
+// thread scans kScanPer consecutive ones in registers, the workgroup scans
+// the per-thread sums, and the running carry links the passes.
+constexpr int kScanPer = 16;
+
 [[maybe_unused]] static __global__ void __launch_bounds__(kScanBlock)
     exclusive_scan(const uint32_t* __restrict__ counts, int64_t nb, int64_t* __restrict__ offsets,
                    int64_t* __restrict__ total) {
     __shared__ int64_t wave_sums[kScanBlock / 64];
-    __shared__ int64_t carry_sh;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    if (threadIdx.x == 0) carry_sh = 0;
-    __syncthreads();
-    for (int64_t base = 0; base < nb; base += kScanBlock) {
-        const int64_t k = base + threadIdx.x;
-        const int64_t v = k < nb ? int64_t(counts[k]) : 0;
-        int64_t incl = v;
+    int64_t carry = 0;
+    for (int64_t pass = 0; pass < nb; pass += int64_t(kScanBlock) * kScanPer) {
+        const int64_t base = pass + int64_t(threadIdx.x) * kScanPer;
+        int64_t loc[kScanPer];
+        int64_t run = 0;
+#pragma unroll
+        for (int j = 0; j < kScanPer; ++j) {
+            loc[j] = run;
+            run += base + j < nb ? int64_t(counts[base + j]) : 0;
+        }
+        int64_t incl = run;
         for (int d = 1; d < 64; d <<= 1) {
             const int64_t o = __shfl_up(incl, d, 64);
             if (lane >= d) incl += o;
         }
         if (lane == 63) wave_sums[wave] = incl;
         __syncthreads();
-        int64_t before = carry_sh;
-        int64_t chunk = 0;
+        int64_t before = carry, chunk = 0;
         for (int w = 0; w < kScanBlock / 64; ++w) {
-            before += w < wave ? wave_sums[w] : 0;
-            chunk += wave_sums[w];
+            const int64_t ws = wave_sums[w];
+            before += w < wave ? ws : 0;
+            chunk += ws;
         }
-        if (k < nb) offsets[k] = before + incl - v;
-        __syncthreads();
-        if (threadIdx.x == 0) carry_sh += chunk;
-        __syncthreads();
+        before += incl - run;
+#pragma unroll
+        for (int j = 0; j < kScanPer; ++j)
+            if (base + j < nb) offsets[base + j] = before + loc[j];
+        carry += chunk;
+        __syncthreads();   // wave_sums is rewritten by the next pass
     }
-    if (threadIdx.x == 0) *total = carry_sh;
+    if (threadIdx.x == 0) *total = carry;
 }
 
 }  // namespace scan
