@@ -20,9 +20,12 @@
 //   4. Karras's radix tree: internal node i finds its range and split by
 //      binary searches over the longest common prefix of neighbouring codes
 //      (ties broken by the sorted index, so every code is distinct);
-//   5. boxes bottom up: each leaf walks up; at every internal node the second
-//      of its two children to arrive forms the union (workgroup-local subtrees
-//      in LDS, then the nodes spanning workgroups in one more workgroup);
+//   5. boxes: a node's box is the min / max over the triangles of its leaf
+//      range.  Nodes inside one workgroup's 1024 leaves are united bottom up
+//      in LDS; the workgroup also writes its leaves' prefix and suffix boxes
+//      and lists the nodes whose range spans workgroups.  A wave then forms
+//      each listed node's box from one suffix, one prefix and the whole
+//      chunks between them (no chain of dependent steps up the tree);
 //   6. the stack layout: entry i holds internal node i's two children; a
 //      child that covers two leaves is written as one 2-triangle leaf (the
 //      reference's leaf size), so its own entry is never reached.
@@ -240,8 +243,9 @@ __device__ __forceinline__ int delta(const uint32_t* __restrict__ keys, int64_t 
 // range, and the parent of every node (parent[internal c] at c, parent[leaf l] at n - 1 + l).
 __global__ void __launch_bounds__(kBlock) karras(const uint32_t* __restrict__ keys, int64_t n,
                                                  int2* __restrict__ child, int2* __restrict__ range,
-                                                 int32_t* __restrict__ parent) {
+                                                 int32_t* __restrict__ parent, unsigned* __restrict__ span_count) {
     const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (i == 0) *span_count = 0u;   // the boxes pass appends to the span list
     if (i >= n - 1) return;
     const int d = delta(keys, n, i, i + 1) - delta(keys, n, i, i - 1) >= 0 ? 1 : -1;
     const int dmin = delta(keys, n, i, i - d);
@@ -268,22 +272,24 @@ __global__ void __launch_bounds__(kBlock) karras(const uint32_t* __restrict__ ke
     parent[right] = int32_t(i);
 }
 
-// Boxes bottom up, in two passes.  A thread per leaf climbs while it is the
-// second of a node's two children to finish, so every internal node's box is
-// written once, after both children's.
+// Boxes.  A node's box is the min / max over the triangle boxes of its leaf
+// range, which min / max give exactly in any order, so the boxes equal a
+// bottom-up union's bit for bit.
 //
 // Karras's tree gives internal node i a leaf range with one end at i, so a
 // node whose range lies in one workgroup's kBoxBlock leaves has its index in
 // that workgroup's leaf index range, and its whole subtree is computed by that
-// workgroup.  The local pass keeps those nodes' counters and boxes in LDS
-// (workgroup scope).  A climb that reaches a node spanning workgroups counts
-// its arrival in global memory (relaxed; nothing is read back in this pass),
-// and the second arrival appends the node to the ready list.  The upper pass
-// (one workgroup) then finishes the spanning nodes from that list, reading the
-// local pass's boxes after the kernel boundary and synchronising among its
-// own threads at workgroup scope.  No pass needs agent-scope fences, which on
-// this chip write back and invalidate an XCD's L2.
+// workgroup: a thread per leaf climbs while it is the second of a node's two
+// children to arrive (counters and boxes in LDS, workgroup scope).  The same
+// pass writes each leaf's prefix box (the chunk's leaves up to it) and suffix
+// box (from it to the chunk's end) and lists, a thread per internal node, the
+// nodes whose range spans chunks.  The span pass then gives each listed node
+// [a, b] the union of suffix(a), prefix(b) and the whole chunks between them
+// (a chunk's box is the suffix box of its first leaf).  No pass has a chain
+// of dependent steps through the upper tree, and none needs agent-scope
+// fences, which on this chip write back and invalidate an XCD's L2.
 constexpr int kBoxBlock = 1024;
+constexpr int kSpanGrid = 512;   // workgroups of the span pass (kBlock threads, a node per wave)
 
 __device__ __forceinline__ Box unite(const Box& a, const Box& b) {
     Box u;
@@ -294,32 +300,81 @@ __device__ __forceinline__ Box unite(const Box& a, const Box& b) {
     return u;
 }
 
-// flags [n]: arrival counters of the internal nodes [0, n - 1) and, at n - 1,
-// the length of the ready list.
+__device__ __forceinline__ Box empty_box() {
+    Box b;
+    for (int q = 0; q < 3; ++q) {
+        b.lo[q] = __builtin_inff();
+        b.hi[q] = -__builtin_inff();
+    }
+    return b;
+}
+
+__device__ __forceinline__ Box shfl_box(const Box& b, int src_lane) {
+    Box o;
+    for (int q = 0; q < 3; ++q) {
+        o.lo[q] = __shfl(b.lo[q], src_lane, 64);
+        o.hi[q] = __shfl(b.hi[q], src_lane, 64);
+    }
+    return o;
+}
+
 __global__ void __launch_bounds__(kBoxBlock) boxes_local(const crt_vertex* __restrict__ v,
                                                          const uint32_t* __restrict__ idx,
                                                          const uint32_t* __restrict__ order, int64_t n,
                                                          const int2* __restrict__ child, const int2* __restrict__ range,
-                                                         const int32_t* __restrict__ parent,
-                                                         unsigned* __restrict__ flags, Box* __restrict__ boxes,
-                                                         int32_t* __restrict__ ready) {
+                                                         const int32_t* __restrict__ parent, Box* __restrict__ boxes,
+                                                         Box* __restrict__ pre, Box* __restrict__ suf,
+                                                         int32_t* __restrict__ span,
+                                                         unsigned* __restrict__ span_count) {
     __shared__ unsigned lflag[kBoxBlock];
     __shared__ Box lbox[kBoxBlock];
+    __shared__ Box wave_pre[kBoxBlock / 64], wave_suf[kBoxBlock / 64];
     const int64_t base = int64_t(blockIdx.x) * kBoxBlock;
-    lflag[threadIdx.x] = 0u;
-    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int64_t l = base + threadIdx.x;
-    if (l >= n) return;   // no barrier below
+    const bool valid = l < n;
+    lflag[threadIdx.x] = 0u;
+    // prefix and suffix boxes of the chunk: wave scans, then the wave totals
+    const Box tb = valid ? tri_box(v, idx, int64_t(order[l])) : empty_box();
+    Box p = tb, q = tb;
+    for (int d = 1; d < 64; d <<= 1) {
+        const Box op = shfl_box(p, lane >= d ? lane - d : lane);
+        const Box oq = shfl_box(q, lane + d < 64 ? lane + d : lane);
+        if (lane >= d) p = unite(p, op);
+        if (lane + d < 64) q = unite(q, oq);
+    }
+    if (lane == 63) wave_pre[wave] = p;
+    if (lane == 0) wave_suf[wave] = q;
+    __syncthreads();
+    for (int w = 0; w < kBoxBlock / 64; ++w) {
+        if (w < wave) p = unite(p, wave_pre[w]);
+        if (w > wave) q = unite(q, wave_suf[w]);
+    }
+    // internal node l (l < n - 1) goes to the span list when its range leaves
+    // one chunk: a ballot per wave, one append per wave
+    bool spans = false;
+    if (l < n - 1) {
+        const int2 r = range[l];
+        spans = r.x / kBoxBlock != r.y / kBoxBlock;
+    }
+    const uint64_t ballot = __ballot(spans);
+    if (ballot) {
+        unsigned at = 0;
+        if (lane == 0)
+            at = __hip_atomic_fetch_add(span_count, unsigned(__popcll(ballot)), __ATOMIC_RELAXED,
+                                        __HIP_MEMORY_SCOPE_AGENT);
+        at = __shfl(at, 0, 64);
+        const unsigned below =
+            __builtin_amdgcn_mbcnt_hi(uint32_t(ballot >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(ballot), 0u));
+        if (spans) span[at + below] = int32_t(l);
+    }
+    if (!valid) return;   // no barrier below
+    pre[l] = p;
+    suf[l] = q;
     int32_t node = parent[(n - 1) + l];
     while (node >= 0) {
         const int2 r = range[node];
-        if (r.x / kBoxBlock != r.y / kBoxBlock) {
-            if (__hip_atomic_fetch_add(&flags[node], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 1u) {
-                const unsigned k = __hip_atomic_fetch_add(&flags[n - 1], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                ready[k] = node;
-            }
-            return;
-        }
+        if (r.x / kBoxBlock != r.y / kBoxBlock) return;   // the span pass forms it
         const int32_t li = node - int32_t(base);   // in [0, kBoxBlock): node lies in this workgroup's leaf range
         if (__hip_atomic_fetch_add(&lflag[li], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP) == 0u) return;
         const int2 c = child[node];
@@ -332,31 +387,24 @@ __global__ void __launch_bounds__(kBoxBlock) boxes_local(const crt_vertex* __res
     }
 }
 
-__global__ void __launch_bounds__(kBoxBlock) boxes_upper(const crt_vertex* __restrict__ v,
-                                                         const uint32_t* __restrict__ idx,
-                                                         const uint32_t* __restrict__ order, int64_t n,
-                                                         const int2* __restrict__ child,
-                                                         const int32_t* __restrict__ parent,
-                                                         unsigned* __restrict__ flags, Box* __restrict__ boxes,
-                                                         const int32_t* __restrict__ ready) {
-    const unsigned count = flags[n - 1];
-    for (unsigned i = threadIdx.x; i < count; i += kBoxBlock) {
-        int32_t node = ready[i];
-        while (true) {
-            const int2 c = child[node];
-            const Box b0 = c.x >= n - 1 ? tri_box(v, idx, int64_t(order[c.x - (n - 1)])) : boxes[c.x];
-            const Box b1 = c.y >= n - 1 ? tri_box(v, idx, int64_t(order[c.y - (n - 1)])) : boxes[c.y];
-            boxes[node] = unite(b0, b1);
-            if (node == 0) break;
-            node = parent[node];
-            if (__hip_atomic_fetch_add(&flags[node], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP) == 0u) break;
-        }
+// A wave per listed node: suffix(a), prefix(b) and the whole chunks of
+// [a, b] between them, united across the wave.
+__global__ void __launch_bounds__(kBlock) boxes_span(const int2* __restrict__ range, const int32_t* __restrict__ span,
+                                                     const unsigned* __restrict__ span_count,
+                                                     const Box* __restrict__ pre, const Box* __restrict__ suf,
+                                                     Box* __restrict__ boxes) {
+    const unsigned count = *span_count;
+    const int lane = threadIdx.x & 63;
+    const unsigned waves = gridDim.x * (kBlock / 64);
+    for (unsigned e = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); e < count; e += waves) {
+        const int32_t node = span[e];
+        const int2 r = range[node];
+        const int64_t ca = r.x / kBoxBlock, cb = r.y / kBoxBlock;
+        Box acc = lane == 0 ? suf[r.x] : lane == 1 ? pre[r.y] : empty_box();
+        for (int64_t k = ca + 1 + lane; k < cb; k += 64) acc = unite(acc, suf[k * kBoxBlock]);
+        for (int d = 32; d > 0; d >>= 1) acc = unite(acc, shfl_box(acc, lane ^ d));
+        if (lane == 0) boxes[node] = acc;
     }
-}
-
-__global__ void __launch_bounds__(kBlock) zero_u32(unsigned* __restrict__ p, int64_t n) {
-    const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
-    if (i < n) p[i] = 0u;
 }
 
 // Packed leaf word (start << 4 | count) as the stack layout stores it.
@@ -410,7 +458,8 @@ __global__ void __launch_bounds__(kBlock) emit(const crt_vertex* __restrict__ v,
 inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
 struct Layout {
-    size_t keys[2], vals[2], counts, offsets, total, partial, frame, child, range, parent, flags, boxes, ready, bytes;
+    size_t keys[2], vals[2], counts, offsets, total, partial, frame, child, range, parent, boxes, pre, suf, span,
+        span_count, bytes;
 };
 
 inline Layout layout(int64_t n) {
@@ -436,9 +485,11 @@ inline Layout layout(int64_t n) {
     L.child = take(un * 8);
     L.range = take(un * 8);
     L.parent = take(2 * un * 4);
-    L.flags = take(un * 4);
     L.boxes = take(un * sizeof(Box));
-    L.ready = take(un * 4);
+    L.pre = take(un * sizeof(Box));
+    L.suf = take(un * sizeof(Box));
+    L.span = take(un * 4);
+    L.span_count = take(4);
     L.bytes = off;
     return L;
 }
@@ -480,9 +531,11 @@ int crt_build_bvh_gpu(const crt_vertex* vertices, int64_t nverts, const uint32_t
     auto* child = static_cast<int2*>(at(L.child));
     auto* range = static_cast<int2*>(at(L.range));
     auto* parent = static_cast<int32_t*>(at(L.parent));
-    auto* flags = static_cast<unsigned*>(at(L.flags));
     auto* boxes = static_cast<Box*>(at(L.boxes));
-    auto* ready = static_cast<int32_t*>(at(L.ready));
+    auto* pre = static_cast<Box*>(at(L.pre));
+    auto* suf = static_cast<Box*>(at(L.suf));
+    auto* span = static_cast<int32_t*>(at(L.span));
+    auto* span_count = static_cast<unsigned*>(at(L.span_count));
     const hipStream_t st = static_cast<hipStream_t>(stream);
     const int64_t n = ntris;
     const unsigned nb = unsigned((n + kBlock - 1) / kBlock);
@@ -501,14 +554,12 @@ int crt_build_bvh_gpu(const crt_vertex* vertices, int64_t nverts, const uint32_t
         cur = 1 - cur;
     }
     const unsigned ni = unsigned((n - 1 + kBlock - 1) / kBlock);
-    hipLaunchKernelGGL(karras, dim3(ni), dim3(kBlock), 0, st, keys[cur], n, child, range, parent);
-    // flags and the ready count must start at zero: a fill kernel (no memset
-    // node, so the chain stays capturable)
-    hipLaunchKernelGGL(zero_u32, dim3(nb), dim3(kBlock), 0, st, flags, n);
+    // karras also zeroes the span count (a kernel store, no memset node, so
+    // the chain stays capturable)
+    hipLaunchKernelGGL(karras, dim3(ni), dim3(kBlock), 0, st, keys[cur], n, child, range, parent, span_count);
     hipLaunchKernelGGL(boxes_local, dim3(unsigned((n + kBoxBlock - 1) / kBoxBlock)), dim3(kBoxBlock), 0, st, vertices,
-                       indices, vals[cur], n, child, range, parent, flags, boxes, ready);
-    hipLaunchKernelGGL(boxes_upper, dim3(1), dim3(kBoxBlock), 0, st, vertices, indices, vals[cur], n, child, parent,
-                       flags, boxes, ready);
+                       indices, vals[cur], n, child, range, parent, boxes, pre, suf, span, span_count);
+    hipLaunchKernelGGL(boxes_span, dim3(kSpanGrid), dim3(kBlock), 0, st, range, span, span_count, pre, suf, boxes);
     hipLaunchKernelGGL(emit, dim3(nb), dim3(kBlock), 0, st, vertices, indices, mesh_ids, mesh_id, vals[cur], n,
                        t_offset, child, range, boxes, nodes, triangles);
     const hipError_t e = hipGetLastError();
