@@ -96,12 +96,15 @@ __global__ void __launch_bounds__(kBlock) centroid_bounds(const crt_vertex* __re
 }
 
 // One workgroup folds the partials into (lo[3], scale[3]): scale = 1 / extent, 0 for a flat axis.
-__global__ void __launch_bounds__(kBlock) finish_bounds(const float* __restrict__ partial, int64_t nb,
-                                                        float* __restrict__ frame) {
-    __shared__ float sh[6][kBlock];
+constexpr int kBoundsBlock = 1024;   // the one workgroup that reduces the partial bounds
+
+__global__ void __launch_bounds__(kBoundsBlock) finish_bounds(const float* __restrict__ partial, int64_t nb,
+                                                              float* __restrict__ frame) {
+    __shared__ float sh[6][kBoundsBlock];
     float lo[3] = {__builtin_inff(), __builtin_inff(), __builtin_inff()};
     float hi[3] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
-    for (int64_t b = threadIdx.x; b < nb; b += kBlock)
+#pragma unroll 4
+    for (int64_t b = threadIdx.x; b < nb; b += kBoundsBlock)
         for (int k = 0; k < 3; ++k) {
             lo[k] = fminf(lo[k], partial[6 * b + k]);
             hi[k] = fmaxf(hi[k], partial[6 * b + 3 + k]);
@@ -111,7 +114,7 @@ __global__ void __launch_bounds__(kBlock) finish_bounds(const float* __restrict_
         sh[3 + k][threadIdx.x] = hi[k];
     }
     __syncthreads();
-    for (int s = kBlock / 2; s > 0; s >>= 1) {
+    for (int s = kBoundsBlock / 2; s > 0; s >>= 1) {
         if (int(threadIdx.x) < s)
             for (int k = 0; k < 3; ++k) {
                 sh[k][threadIdx.x] = fminf(sh[k][threadIdx.x], sh[k][threadIdx.x + s]);
@@ -324,7 +327,7 @@ __global__ void __launch_bounds__(kBoxBlock) boxes_local(const crt_vertex* __res
                                                          const int2* __restrict__ child, const int2* __restrict__ range,
                                                          const int32_t* __restrict__ parent, Box* __restrict__ boxes,
                                                          Box* __restrict__ pre, Box* __restrict__ suf,
-                                                         int32_t* __restrict__ span,
+                                                         Box* __restrict__ leaf_box, int32_t* __restrict__ span,
                                                          unsigned* __restrict__ span_count) {
     __shared__ unsigned lflag[kBoxBlock];
     __shared__ Box lbox[kBoxBlock];
@@ -371,6 +374,7 @@ __global__ void __launch_bounds__(kBoxBlock) boxes_local(const crt_vertex* __res
     if (!valid) return;   // no barrier below
     pre[l] = p;
     suf[l] = q;
+    leaf_box[l] = tb;   // emit writes the leaf children from these
     int32_t node = parent[(n - 1) + l];
     while (node >= 0) {
         const int2 r = range[node];
@@ -426,8 +430,8 @@ __global__ void __launch_bounds__(kBlock) emit(const crt_vertex* __restrict__ v,
                                                const int32_t* __restrict__ mesh, int32_t mesh_all,
                                                const uint32_t* __restrict__ order, int64_t n, int32_t t_offset,
                                                const int2* __restrict__ child, const int2* __restrict__ range,
-                                               const Box* __restrict__ boxes, crt_stack_node* __restrict__ nodes,
-                                               crt_triangle* __restrict__ tris) {
+                                               const Box* __restrict__ boxes, const Box* __restrict__ leaf_box,
+                                               crt_stack_node* __restrict__ nodes, crt_triangle* __restrict__ tris) {
     const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
     if (i >= n) return;
     const uint32_t t = order[i];
@@ -445,7 +449,7 @@ __global__ void __launch_bounds__(kBlock) emit(const crt_vertex* __restrict__ v,
         crt_bounds& out = k == 0 ? e.left : e.right;
         if (cc >= n - 1) {
             const int32_t leaf = cc - int32_t(n - 1);
-            put_child(out, tri_box(v, idx, int64_t(order[leaf])), leaf_word(int64_t(t_offset) + leaf, 1), 0);
+            put_child(out, leaf_box[leaf], leaf_word(int64_t(t_offset) + leaf, 1), 0);
         } else if (range[cc].y - range[cc].x == 1) {
             put_child(out, boxes[cc], leaf_word(int64_t(t_offset) + range[cc].x, 2), 0);
         } else {
@@ -458,8 +462,8 @@ __global__ void __launch_bounds__(kBlock) emit(const crt_vertex* __restrict__ v,
 inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
 struct Layout {
-    size_t keys[2], vals[2], counts, offsets, total, partial, frame, child, range, parent, boxes, pre, suf, span,
-        span_count, bytes;
+    size_t keys[2], vals[2], counts, offsets, total, partial, frame, child, range, parent, boxes, pre, suf, leaf_box,
+        span, span_count, bytes;
 };
 
 inline Layout layout(int64_t n) {
@@ -488,6 +492,7 @@ inline Layout layout(int64_t n) {
     L.boxes = take(un * sizeof(Box));
     L.pre = take(un * sizeof(Box));
     L.suf = take(un * sizeof(Box));
+    L.leaf_box = take(un * sizeof(Box));
     L.span = take(un * 4);
     L.span_count = take(4);
     L.bytes = off;
@@ -534,6 +539,7 @@ int crt_build_bvh_gpu(const crt_vertex* vertices, int64_t nverts, const uint32_t
     auto* boxes = static_cast<Box*>(at(L.boxes));
     auto* pre = static_cast<Box*>(at(L.pre));
     auto* suf = static_cast<Box*>(at(L.suf));
+    auto* leaf_box = static_cast<Box*>(at(L.leaf_box));
     auto* span = static_cast<int32_t*>(at(L.span));
     auto* span_count = static_cast<unsigned*>(at(L.span_count));
     const hipStream_t st = static_cast<hipStream_t>(stream);
@@ -541,7 +547,7 @@ int crt_build_bvh_gpu(const crt_vertex* vertices, int64_t nverts, const uint32_t
     const unsigned nb = unsigned((n + kBlock - 1) / kBlock);
 
     hipLaunchKernelGGL(centroid_bounds, dim3(nb), dim3(kBlock), 0, st, vertices, indices, n, partial);
-    hipLaunchKernelGGL(finish_bounds, dim3(1), dim3(kBlock), 0, st, partial, int64_t(nb), frame);
+    hipLaunchKernelGGL(finish_bounds, dim3(1), dim3(kBoundsBlock), 0, st, partial, int64_t(nb), frame);
     hipLaunchKernelGGL(morton, dim3(nb), dim3(kBlock), 0, st, vertices, indices, n, frame, keys[0], vals[0]);
     int cur = 0;
     const unsigned nt = unsigned((n + kTile - 1) / kTile);
@@ -558,10 +564,10 @@ int crt_build_bvh_gpu(const crt_vertex* vertices, int64_t nverts, const uint32_t
     // the chain stays capturable)
     hipLaunchKernelGGL(karras, dim3(ni), dim3(kBlock), 0, st, keys[cur], n, child, range, parent, span_count);
     hipLaunchKernelGGL(boxes_local, dim3(unsigned((n + kBoxBlock - 1) / kBoxBlock)), dim3(kBoxBlock), 0, st, vertices,
-                       indices, vals[cur], n, child, range, parent, boxes, pre, suf, span, span_count);
+                       indices, vals[cur], n, child, range, parent, boxes, pre, suf, leaf_box, span, span_count);
     hipLaunchKernelGGL(boxes_span, dim3(kSpanGrid), dim3(kBlock), 0, st, range, span, span_count, pre, suf, boxes);
     hipLaunchKernelGGL(emit, dim3(nb), dim3(kBlock), 0, st, vertices, indices, mesh_ids, mesh_id, vals[cur], n,
-                       t_offset, child, range, boxes, nodes, triangles);
+                       t_offset, child, range, boxes, leaf_box, nodes, triangles);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess)
         return crt::set_error(CRT_E_HIP, std::string("crt_build_bvh_gpu: kernel launch: ") + hipGetErrorString(e));
