@@ -331,6 +331,7 @@ __global__ void __launch_bounds__(kBoxBlock) boxes_local(const crt_vertex* __res
                                                          unsigned* __restrict__ span_count) {
     __shared__ unsigned lflag[kBoxBlock];
     __shared__ Box lbox[kBoxBlock];
+    __shared__ Box ltri[kBoxBlock];   // the chunk's leaf boxes, for the climb
     __shared__ Box wave_pre[kBoxBlock / 64], wave_suf[kBoxBlock / 64];
     const int64_t base = int64_t(blockIdx.x) * kBoxBlock;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -339,6 +340,7 @@ __global__ void __launch_bounds__(kBoxBlock) boxes_local(const crt_vertex* __res
     lflag[threadIdx.x] = 0u;
     // prefix and suffix boxes of the chunk: wave scans, then the wave totals
     const Box tb = valid ? tri_box(v, idx, int64_t(order[l])) : empty_box();
+    ltri[threadIdx.x] = tb;
     Box p = tb, q = tb;
     for (int d = 1; d < 64; d <<= 1) {
         const Box op = shfl_box(p, lane >= d ? lane - d : lane);
@@ -382,8 +384,9 @@ __global__ void __launch_bounds__(kBoxBlock) boxes_local(const crt_vertex* __res
         const int32_t li = node - int32_t(base);   // in [0, kBoxBlock): node lies in this workgroup's leaf range
         if (__hip_atomic_fetch_add(&lflag[li], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP) == 0u) return;
         const int2 c = child[node];
-        const Box b0 = c.x >= n - 1 ? tri_box(v, idx, int64_t(order[c.x - (n - 1)])) : lbox[c.x - int32_t(base)];
-        const Box b1 = c.y >= n - 1 ? tri_box(v, idx, int64_t(order[c.y - (n - 1)])) : lbox[c.y - int32_t(base)];
+        // a local node's children lie in this chunk: leaves in ltri, nodes in lbox
+        const Box b0 = c.x >= n - 1 ? ltri[c.x - int32_t(n - 1) - int32_t(base)] : lbox[c.x - int32_t(base)];
+        const Box b1 = c.y >= n - 1 ? ltri[c.y - int32_t(n - 1) - int32_t(base)] : lbox[c.y - int32_t(base)];
         const Box u = unite(b0, b1);
         lbox[li] = u;
         boxes[node] = u;
