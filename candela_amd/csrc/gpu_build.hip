@@ -15,8 +15,8 @@
 //   2. a 30-bit Morton code of the centroid quantized to 1024 cells per axis;
 //   3. a stable LSD radix sort of (code, triangle) by 4-bit digits, 8 passes:
 //      each pass counts the digits per 1024-code tile, scans the digit-major
-//      counts (scan.h) and scatters each tile's codes in input order within
-//      their digit;
+//      counts (a workgroup per digit) and scatters each tile's codes in input
+//      order within their digit;
 //   4. Karras's radix tree: internal node i finds its range and split by
 //      binary searches over the longest common prefix of neighbouring codes
 //      (ties broken by the sorted index, so every code is distinct);
@@ -215,6 +215,49 @@ __global__ void __launch_bounds__(kBlock) count_digits(const uint32_t* __restric
     uint32_t key[kItems], digit[kItems], rank[kItems];
     tile_ranks(keys, n, shift, key, digit, rank, wc, run);
     if (threadIdx.x < kRadix) counts[int64_t(threadIdx.x) * gridDim.x + blockIdx.x] = run[threadIdx.x];
+}
+
+// Exclusive scan of the digit-major tile counts (counts[d * nt + tile]) by
+// kRadix workgroups, one per digit: workgroup d sums the counts of the digits
+// below it (a contiguous prefix of the array) for its base, then scans its
+// own nt counts in coalesced passes.  One workgroup scanning all kRadix * nt
+// counts ran on one CU and took a third of the build.
+constexpr int kDigitScanBlock = 1024;
+
+__global__ void __launch_bounds__(kDigitScanBlock) scan_digits(const uint32_t* __restrict__ counts, int64_t nt,
+                                                               int64_t* __restrict__ offsets) {
+    __shared__ int64_t wave_sums[kDigitScanBlock / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t seg = int64_t(blockIdx.x) * nt;
+    // base: the sum of counts [0, seg)
+    int64_t part = 0;
+    for (int64_t k = threadIdx.x; k < seg; k += kDigitScanBlock) part += counts[k];
+    for (int d = 32; d > 0; d >>= 1) part += __shfl_xor(part, d, 64);
+    if (lane == 0) wave_sums[wave] = part;
+    __syncthreads();
+    int64_t carry = 0;
+    for (int w = 0; w < kDigitScanBlock / 64; ++w) carry += wave_sums[w];
+    __syncthreads();
+    for (int64_t b = 0; b < nt; b += kDigitScanBlock) {
+        const int64_t k = b + threadIdx.x;
+        const int64_t v = k < nt ? int64_t(counts[seg + k]) : 0;
+        int64_t incl = v;
+        for (int d = 1; d < 64; d <<= 1) {
+            const int64_t o = __shfl_up(incl, d, 64);
+            if (lane >= d) incl += o;
+        }
+        if (lane == 63) wave_sums[wave] = incl;
+        __syncthreads();
+        int64_t before = carry, chunk = 0;
+        for (int w = 0; w < kDigitScanBlock / 64; ++w) {
+            const int64_t ws = wave_sums[w];
+            before += w < wave ? ws : 0;
+            chunk += ws;
+        }
+        if (k < nt) offsets[seg + k] = before + incl - v;
+        carry += chunk;
+        __syncthreads();
+    }
 }
 
 __global__ void __launch_bounds__(kBlock) scatter_digits(const uint32_t* __restrict__ kin,
@@ -465,7 +508,7 @@ __global__ void __launch_bounds__(kBlock) emit(const crt_vertex* __restrict__ v,
 inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
 struct Layout {
-    size_t keys[2], vals[2], counts, offsets, total, partial, frame, child, range, parent, boxes, pre, suf, leaf_box,
+    size_t keys[2], vals[2], counts, offsets, partial, frame, child, range, parent, boxes, pre, suf, leaf_box,
         span, span_count, bytes;
 };
 
@@ -486,7 +529,6 @@ inline Layout layout(int64_t n) {
     const size_t nt = size_t((n + kTile - 1) / kTile);
     L.counts = take(kRadix * nt * 4);
     L.offsets = take(kRadix * nt * 8);
-    L.total = take(8);
     L.partial = take(nb * 6 * 4);
     L.frame = take(6 * 4);
     L.child = take(un * 8);
@@ -533,7 +575,6 @@ int crt_build_bvh_gpu(const crt_vertex* vertices, int64_t nverts, const uint32_t
     uint32_t* vals[2] = {static_cast<uint32_t*>(at(L.vals[0])), static_cast<uint32_t*>(at(L.vals[1]))};
     auto* counts = static_cast<uint32_t*>(at(L.counts));
     auto* offsets = static_cast<int64_t*>(at(L.offsets));
-    auto* total = static_cast<int64_t*>(at(L.total));
     auto* partial = static_cast<float*>(at(L.partial));
     auto* frame = static_cast<float*>(at(L.frame));
     auto* child = static_cast<int2*>(at(L.child));
@@ -556,8 +597,7 @@ int crt_build_bvh_gpu(const crt_vertex* vertices, int64_t nverts, const uint32_t
     const unsigned nt = unsigned((n + kTile - 1) / kTile);
     for (int shift = 0; shift < kMortonBits; shift += kDigitBits) {
         hipLaunchKernelGGL(count_digits, dim3(nt), dim3(kBlock), 0, st, keys[cur], n, shift, counts);
-        hipLaunchKernelGGL(crt::scan::exclusive_scan, dim3(1), dim3(kScanBlock), 0, st, counts,
-                           int64_t(kRadix) * nt, offsets, total);
+        hipLaunchKernelGGL(scan_digits, dim3(kRadix), dim3(kDigitScanBlock), 0, st, counts, int64_t(nt), offsets);
         hipLaunchKernelGGL(scatter_digits, dim3(nt), dim3(kBlock), 0, st, keys[cur], vals[cur], n, shift, offsets,
                            keys[1 - cur], vals[1 - cur]);
         cur = 1 - cur;
