@@ -230,8 +230,14 @@ __global__ void __launch_bounds__(kDigitScanBlock) scan_digits(const uint32_t* _
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int64_t seg = int64_t(blockIdx.x) * nt;
     // base: the sum of counts [0, seg)
-    int64_t part = 0;
-    for (int64_t k = threadIdx.x; k < seg; k += kDigitScanBlock) part += counts[k];
+    int64_t p0 = 0, p1 = 0, p2 = 0, p3 = 0;   // four loads in flight per thread
+    for (int64_t k = threadIdx.x; k < seg; k += 4 * kDigitScanBlock) {
+        p0 += counts[k];
+        p1 += k + kDigitScanBlock < seg ? counts[k + kDigitScanBlock] : 0u;
+        p2 += k + 2 * kDigitScanBlock < seg ? counts[k + 2 * kDigitScanBlock] : 0u;
+        p3 += k + 3 * kDigitScanBlock < seg ? counts[k + 3 * kDigitScanBlock] : 0u;
+    }
+    int64_t part = (p0 + p1) + (p2 + p3);
     for (int d = 32; d > 0; d >>= 1) part += __shfl_xor(part, d, 64);
     if (lane == 0) wave_sums[wave] = part;
     __syncthreads();
