@@ -82,3 +82,26 @@ def test_context_host_side(crt):
     assert np.array_equal(t1[:, 3], b1.triangles["packed"][:, 3])
     with pytest.raises(crt.CrtError):
         r.scene()   # nothing buffered yet
+
+
+def test_device_build_workspaces_and_arguments(crt):
+    """Workspace sizes and argument checks of the device builder and both device collapses: host-side only,
+    rejected before any device work."""
+    lib = crt.lib()
+    n = 100000
+    assert lib.crt_build_bvh_gpu_workspace(1) == 0
+    assert lib.crt_build_bvh_gpu_workspace(n) >= n * 140
+    assert lib.crt_wide_build_gpu_workspace(n, 4) > 0 and lib.crt_wide_build_gpu_workspace(n, 5) == 0
+    a8, a64 = lib.crt_wide_build_gpu_async_workspace(n, 4, 8), lib.crt_wide_build_gpu_async_workspace(n, 4, 64)
+    assert 0 < a8 < a64 and a64 - a8 >= 56 * 16        # two int64 of level state per extra level
+    assert lib.crt_wide_build_gpu_async_workspace(n, 8, 64) > a64    # eight child slots per node
+    for bad in ((1, 4, 8), (n, 6, 8), (n, 4, 0), (n, 4, crt.WIDE_GPU_MAX_LEVELS + 1)):
+        assert lib.crt_wide_build_gpu_async_workspace(*bad) == 0
+    assert lib.crt_wide_build_gpu_async(None, 1, None, 2, None, 1, 4, 8, None, None, None, None, None, 0, None) == -1
+    assert b"null" in lib.crt_last_error()
+    assert (crt.WIDE_GPU_BAD_LINK, crt.WIDE_GPU_NOT_TREE, crt.WIDE_GPU_BAD_VERTEX, crt.WIDE_GPU_TOO_DEEP) == (1, 2, 4, 8)
+    # graph capture needs a created stream and an output
+    assert lib.crt_graph_begin(None) == -1
+    assert lib.crt_graph_end(None, None) == -1
+    assert lib.crt_graph_launch(None, None) == -1
+    assert lib.crt_graph_destroy(None) == 0
