@@ -130,7 +130,7 @@ def trace(fmt: int, nodes: np.ndarray, tris: np.ndarray, verts: np.ndarray, ents
     hits = np.empty(rays.size, HIT_DTYPE)
     tests = np.zeros(rays.size, np.uint32) if with_tests else None
     if threads <= 0:
-        threads = min(16, os.cpu_count() or 1)   # a one-GPU box's CPU share
+        threads = min(16, os.cpu_count() or 1)   # at most 16 worker threads
     rc = lib().oracle_trace(fmt, ents.size, nodes.size, _p(nodes), _p(tris), _p(verts), _p(ents), query, flags,
                             _p(rays), _p(hits), rays.size, threads, _p(tests) if tests is not None else None,
                             max_iter)

@@ -4,7 +4,7 @@ Every comparison runs the same BVH buffers and the same ray records through
 crt_trace / crt_intersect / crt_intersect_primary (GPU) and oracle_trace (CPU).
 Both do the same FP32 operations in the same order, so the bar is exact
 equality of t, u, v, w, triangle, mesh, entity and status on every ray; the
-north-star tolerance (>= 99.999 % hit-ID agreement, t and barycentrics within
+path's stated tolerance (>= 99.999 % hit-ID agreement, t and barycentrics within
 1e-5 relative) is asserted as well so that a failure reports both.
 """
 import sys
