@@ -458,7 +458,7 @@ class GpuWide:
     """crt_wide_build_gpu: the wide layout (2-triangle leaves) of a GpuBVH, collapsed on the device and numbered
     breadth-first; `scene` is its crt_scene_view (one identity entity over the wide nodes).
 
-    max_levels=None collapses with crt_wide_build_gpu (reads each level's size back; node_count is exact).
+    max_levels=None collapses with crt_wide_build_gpu (reads the node count back; node_count is exact).
     An integer uses crt_wide_build_gpu_async with that many levels: nothing is read back, so build() can be
     captured into a Graph; the scene then spans ntris nodes (the capacity of `groups`, which bounds every
     valid link), and result() reads the node count and status words."""

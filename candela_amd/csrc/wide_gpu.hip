@@ -14,13 +14,11 @@
 // walks visit the same boxes in the same order, so hit records, visit and
 // test counts are identical.
 //
-// A level is three launches: expand (a thread per wide node writes its slots
-// and lists the BVH2 entries of its inner children, with per-workgroup
-// totals), one workgroup scans the totals (scan.h), and assign (a
-// workgroup-local scan gives each node the index of its first child; it writes
-// the child fields and the next level's list).  The host reads each level's
-// size back, and reads the root entry first: a tree that is one 2-triangle
-// leaf gets its one-slot root from the host.
+// Nothing is read back between levels: the host launches a fixed number of
+// levels (two launches each: expand, then assign), the level sizes live on the
+// device, and a level past the tree's depth returns at once
+// (crt_wide_build_gpu_async).  crt_wide_build_gpu runs the same launches and
+// reads the node count and status back once.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -31,7 +29,6 @@
 
 #include "candela_rt.h"
 #include "crt_internal.h"
-#include "scan.h"
 
 namespace crt {
 namespace wg {
@@ -114,12 +111,6 @@ __device__ __forceinline__ uint32_t block_excl(uint32_t v, uint32_t* wave_sums, 
     return before + incl - v;
 }
 
-__global__ void __launch_bounds__(kBlock) seed(int32_t* entry, int32_t* wide, unsigned* err) {
-    entry[0] = 0;
-    wide[0] = 0;
-    *err = 0u;
-}
-
 // The slots of one wide node w, the expansion of stack entry `entry`: writes
 // its groups and kids[slot] (the stack entry of an inner child, -1
 // otherwise); returns its number of inner children.
@@ -194,47 +185,6 @@ __device__ __forceinline__ bool assign_one(int64_t w, const int32_t* __restrict_
         ++r;
     }
     return true;
-}
-
-// Slots of the wide nodes of one level.  kids[item * W + slot]: the stack
-// entry of an inner child, -1 otherwise; counts[item]: inner children;
-// totals[block]: their sum per workgroup.
-template <int W>
-__global__ void __launch_bounds__(kBlock) expand(const crt_stack_node* __restrict__ nodes, int64_t node_count,
-                                                 const int32_t* __restrict__ entries,
-                                                 const int32_t* __restrict__ wides, int32_t nf,
-                                                 crt_wide_group* __restrict__ groups, int32_t* __restrict__ kids_out,
-                                                 uint32_t* __restrict__ counts, uint32_t* __restrict__ totals,
-                                                 unsigned* __restrict__ err) {
-    __shared__ uint32_t wave_sums[kBlock / 64];
-    const int32_t item = int32_t(blockIdx.x) * kBlock + int32_t(threadIdx.x);
-    uint32_t m = 0;
-    if (item < nf) {
-        m = expand_one<W>(nodes, node_count, entries[item], wides[item], groups, kids_out + int64_t(item) * W, err);
-        counts[item] = m;
-    }
-    uint32_t total;
-    (void)block_excl(m, wave_sums, total);
-    if (threadIdx.x == 0) totals[blockIdx.x] = total;
-}
-
-// Child indices: a node's inner children are next_base + its offset + 0, 1,
-// ... in slot order; they form the next level's list in that order.
-template <int W>
-__global__ void __launch_bounds__(kBlock) assign(const int32_t* __restrict__ wides, int32_t nf,
-                                                 const int32_t* __restrict__ kids, const uint32_t* __restrict__ counts,
-                                                 const int64_t* __restrict__ block_offsets, int64_t next_base,
-                                                 int64_t cap, crt_wide_group* __restrict__ groups,
-                                                 int32_t* __restrict__ next_entries, int32_t* __restrict__ next_wides,
-                                                 unsigned* __restrict__ err) {
-    __shared__ uint32_t wave_sums[kBlock / 64];
-    const int32_t item = int32_t(blockIdx.x) * kBlock + int32_t(threadIdx.x);
-    const uint32_t m = item < nf ? counts[item] : 0u;
-    uint32_t total;
-    const uint32_t local = block_excl(m, wave_sums, total);
-    if (item >= nf) return;
-    (void)assign_one<W>(wides[item], kids + int64_t(item) * W, block_offsets[blockIdx.x] + local, next_base, cap,
-                        groups, next_entries, next_wides, err);
 }
 
 // ---- the asynchronous collapse (crt_wide_build_gpu_async) ----------------
@@ -444,12 +394,12 @@ __global__ void __launch_bounds__(kBlock) records(const crt_triangle* __restrict
 inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
 struct Layout {
-    size_t entries[2], wides[2], kids, counts, totals, offsets, total, err, lvl, bytes;
+    size_t entries[2], wides[2], kids, counts, totals, err, lvl, out, bytes;
 };
 
 // Levels hold at most `cap` wide nodes (a valid tree has fewer wide nodes
-// than triangles); `levels` > 0 adds the asynchronous collapse's level state.
-inline Layout layout(int64_t cap, int32_t width, int32_t levels = 0) {
+// than triangles); the level state has room for `levels` levels.
+inline Layout layout(int64_t cap, int32_t width, int32_t levels) {
     const size_t n = size_t(cap), nb = size_t((cap + kBlock - 1) / kBlock);
     Layout L{};
     size_t off = 0;
@@ -465,94 +415,11 @@ inline Layout layout(int64_t cap, int32_t width, int32_t levels = 0) {
     L.kids = take(n * size_t(width) * 4);
     L.counts = take(n * 4);
     L.totals = take(nb * 4);
-    L.offsets = take(nb * 8);
-    L.total = take(8);
     L.err = take(4);
-    L.lvl = take(levels > 0 ? size_t(2) * size_t(levels + 1) * 8 : 0);
+    L.lvl = take(size_t(2) * size_t(levels + 1) * 8);
+    L.out = take(3 * 8);   // crt_wide_build_gpu's node count, level count and status
     L.bytes = off;
     return L;
-}
-
-inline int32_t host_bits(float f) {
-    int32_t i;
-    std::memcpy(&i, &f, 4);
-    return i;
-}
-
-// A tree whose root entry holds two adjacent 1-triangle leaves is one
-// 2-triangle leaf under a wide root of one slot (crt_wide_build's rule); the
-// root entry was read back, so the host writes that node.
-template <int W>
-int single_leaf(const crt_stack_node& root, crt_wide_group* groups, int64_t* wide_nodes, hipStream_t st) {
-    const int32_t a = host_bits(root.left.min[3]), c = host_bits(root.right.min[3]);
-    crt_wide_group g[W / 4] = {};
-    for (auto& x : g)
-        for (int j = 0; j < 4; ++j) x.count[j] = -1;
-    g[0].lo_x[0] = fminf(root.left.min[0], root.right.min[0]);
-    g[0].lo_y[0] = fminf(root.left.min[1], root.right.min[1]);
-    g[0].lo_z[0] = fminf(root.left.min[2], root.right.min[2]);
-    g[0].hi_x[0] = fmaxf(root.left.max[0], root.right.max[0]);
-    g[0].hi_y[0] = fmaxf(root.left.max[1], root.right.max[1]);
-    g[0].hi_z[0] = fmaxf(root.left.max[2], root.right.max[2]);
-    g[0].child[0] = std::min(a >> 4, c >> 4);
-    g[0].count[0] = (a & 0xF) + (c & 0xF);
-    if (hipMemcpyAsync(groups, g, sizeof(g), hipMemcpyHostToDevice, st) != hipSuccess ||
-        hipStreamSynchronize(st) != hipSuccess)
-        return set_error(CRT_E_HIP, "crt_wide_build_gpu: single-leaf root");
-    *wide_nodes = 1;
-    return CRT_OK;
-}
-
-template <int W>
-int collapse(const crt_stack_node* nodes, int64_t node_count, int64_t cap, crt_wide_group* groups,
-             int64_t* wide_nodes, char* ws, const Layout& L, hipStream_t st) {
-    crt_stack_node root;
-    if (hipMemcpyAsync(&root, nodes, sizeof(root), hipMemcpyDeviceToHost, st) != hipSuccess ||
-        hipStreamSynchronize(st) != hipSuccess)
-        return set_error(CRT_E_HIP, "crt_wide_build_gpu: reading the root entry");
-    {
-        const int32_t a = host_bits(root.left.min[3]), c = host_bits(root.right.min[3]);
-        if (a != -1 && c != -1) {
-            const int32_t sa = a >> 4, na = a & 0xF, sc = c >> 4, nc = c & 0xF;
-            if (na + nc <= 2 && std::max(sa + na, sc + nc) - std::min(sa, sc) == na + nc)
-                return single_leaf<W>(root, groups, wide_nodes, st);
-        }
-    }
-    int32_t* entries[2] = {reinterpret_cast<int32_t*>(ws + L.entries[0]), reinterpret_cast<int32_t*>(ws + L.entries[1])};
-    int32_t* wides[2] = {reinterpret_cast<int32_t*>(ws + L.wides[0]), reinterpret_cast<int32_t*>(ws + L.wides[1])};
-    auto* kids = reinterpret_cast<int32_t*>(ws + L.kids);
-    auto* counts = reinterpret_cast<uint32_t*>(ws + L.counts);
-    auto* totals = reinterpret_cast<uint32_t*>(ws + L.totals);
-    auto* offsets = reinterpret_cast<int64_t*>(ws + L.offsets);
-    auto* total = reinterpret_cast<int64_t*>(ws + L.total);
-    auto* err = reinterpret_cast<unsigned*>(ws + L.err);
-    int64_t nf = 1, next_base = 1;
-    int cur = 0;
-    while (nf > 0) {
-        const unsigned nb = unsigned((nf + kBlock - 1) / kBlock);
-        hipLaunchKernelGGL(expand<W>, dim3(nb), dim3(kBlock), 0, st, nodes, node_count, entries[cur], wides[cur],
-                           int32_t(nf), groups, kids, counts, totals, err);
-        hipLaunchKernelGGL(scan::exclusive_scan, dim3(1), dim3(scan::kScanBlock), 0, st, totals, int64_t(nb), offsets,
-                           total);
-        hipLaunchKernelGGL(assign<W>, dim3(nb), dim3(kBlock), 0, st, wides[cur], int32_t(nf), kids, counts, offsets,
-                           next_base, cap, groups, entries[1 - cur], wides[1 - cur], err);
-        int64_t next = 0;
-        unsigned bad = 0;
-        if (hipMemcpyAsync(&next, total, 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
-            hipMemcpyAsync(&bad, err, 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
-            hipStreamSynchronize(st) != hipSuccess)
-            return set_error(CRT_E_HIP, std::string("crt_wide_build_gpu: ") + hipGetErrorString(hipGetLastError()));
-        if (bad & kErrVertex) return set_error(CRT_E_INVALID, "crt_wide_build_gpu: a vertex index leaves the vertex array");
-        if (bad & kErrLink) return set_error(CRT_E_INVALID, "crt_wide_build_gpu: a stack link leaves the node array");
-        if ((bad & kErrCap) || next_base + next > cap)
-            return set_error(CRT_E_INVALID, "crt_wide_build_gpu: more wide nodes than triangles (the links do not "
-                                            "form a tree)");
-        nf = next;
-        next_base += next;
-        cur = 1 - cur;
-    }
-    *wide_nodes = next_base;
-    return CRT_OK;
 }
 
 template <int W>
@@ -588,32 +455,57 @@ extern "C" {
 
 size_t crt_wide_build_gpu_workspace(int64_t ntris, int32_t width) {
     if (ntris < 2 || (width != 4 && width != 8)) return 0;
-    return crt::wg::layout(ntris, width).bytes;
+    return crt::wg::layout(ntris, width, CRT_WIDE_GPU_MAX_LEVELS).bytes;
 }
 
+// The asynchronous collapse with kFirstLevels levels, and again with every
+// level CRT_WIDE_GPU_MAX_LEVELS allows when the tree is deeper; one readback
+// of the node count and status per run.
 int crt_wide_build_gpu(const crt_stack_node* nodes, int64_t node_count, const crt_triangle* triangles, int64_t ntris,
                        const crt_vertex* vertices, int64_t nverts, int32_t width, crt_wide_group* groups,
                        crt_tri_record* records, int64_t* wide_nodes, void* workspace, size_t workspace_bytes,
                        void* stream) {
     using namespace crt::wg;
+    constexpr int32_t kFirstLevels = 32;
     if (!nodes || !triangles || !vertices || !groups || !records || !wide_nodes || !workspace)
         return crt::set_error(CRT_E_INVALID, "crt_wide_build_gpu: null argument");
     if (width != 4 && width != 8) return crt::set_error(CRT_E_INVALID, "crt_wide_build_gpu: width must be 4 or 8");
     if (ntris < 2 || node_count < 1 || nverts < 1 || ntris > 0x7FFFFFFFll)
         return crt::set_error(CRT_E_INVALID, "crt_wide_build_gpu: needs 2 to 2^31 - 1 triangles and a root entry");
-    const Layout L = layout(ntris, width);
+    const Layout L = layout(ntris, width, CRT_WIDE_GPU_MAX_LEVELS);
     if (workspace_bytes < L.bytes || reinterpret_cast<uintptr_t>(workspace) % 256 != 0)
         return crt::set_error(CRT_E_INVALID, "crt_wide_build_gpu: workspace must be 256-byte aligned and hold "
                                              "crt_wide_build_gpu_workspace(ntris, width) = " +
                                                  std::to_string(L.bytes) + " bytes");
     const hipStream_t st = static_cast<hipStream_t>(stream);
     char* ws = static_cast<char*>(workspace);
-    hipLaunchKernelGGL(seed, dim3(1), dim3(1), 0, st, reinterpret_cast<int32_t*>(ws + L.entries[0]),
-                       reinterpret_cast<int32_t*>(ws + L.wides[0]), reinterpret_cast<unsigned*>(ws + L.err));
-    hipLaunchKernelGGL(crt::wg::records, dim3(unsigned((ntris + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, triangles,
-                       vertices, nverts, ntris, records, reinterpret_cast<unsigned*>(ws + L.err));
-    return width == 4 ? collapse<4>(nodes, node_count, ntris, groups, wide_nodes, ws, L, st)
-                      : collapse<8>(nodes, node_count, ntris, groups, wide_nodes, ws, L, st);
+    auto* out = reinterpret_cast<int64_t*>(ws + L.out);
+    auto* status = reinterpret_cast<uint32_t*>(out + 2);
+    int64_t host_out[3] = {0, 0, 0};
+    for (const int32_t levels : {kFirstLevels, int32_t(CRT_WIDE_GPU_MAX_LEVELS)}) {
+        if (width == 4)
+            collapse_async<4>(nodes, node_count, triangles, vertices, nverts, ntris, levels, groups, records, out,
+                              status, ws, L, st);
+        else
+            collapse_async<8>(nodes, node_count, triangles, vertices, nverts, ntris, levels, groups, records, out,
+                              status, ws, L, st);
+        if (hipMemcpyAsync(host_out, out, sizeof(host_out), hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess)
+            return crt::set_error(CRT_E_HIP, std::string("crt_wide_build_gpu: ") + hipGetErrorString(hipGetLastError()));
+        if (!(uint32_t(host_out[2]) & kErrDepth)) break;
+    }
+    const uint32_t bad = uint32_t(host_out[2]);
+    if (bad & kErrVertex)
+        return crt::set_error(CRT_E_INVALID, "crt_wide_build_gpu: a vertex index leaves the vertex array");
+    if (bad & kErrLink) return crt::set_error(CRT_E_INVALID, "crt_wide_build_gpu: a stack link leaves the node array");
+    if (bad & kErrCap)
+        return crt::set_error(CRT_E_INVALID,
+                              "crt_wide_build_gpu: more wide nodes than triangles (the links do not form a tree)");
+    if (bad & kErrDepth)
+        return crt::set_error(CRT_E_INVALID, "crt_wide_build_gpu: the tree has more than " +
+                                                 std::to_string(CRT_WIDE_GPU_MAX_LEVELS) + " wide levels");
+    *wide_nodes = host_out[0];
+    return CRT_OK;
 }
 
 size_t crt_wide_build_gpu_async_workspace(int64_t ntris, int32_t width, int32_t max_levels) {

@@ -463,10 +463,11 @@ size_t crt_wide_build_gpu_workspace(int64_t ntris, int32_t width);
  * than depth-first, so walks of either layout give identical hit records and
  * counters.  groups: device room for ntris * width / 4 groups; records
  * [ntris]; *wide_nodes (host) receives the node count (the entity's
- * node_count, root 0).  Reads each tree level's size back, so it
- * synchronizes `stream`; links and vertex indices are checked on the device
- * (CRT_E_INVALID).  workspace: 256-byte aligned, crt_wide_build_gpu_workspace
- * bytes. */
+ * node_count, root 0).  Runs crt_wide_build_gpu_async with 32 levels (again
+ * with CRT_WIDE_GPU_MAX_LEVELS for a deeper tree) and reads the result back,
+ * so it synchronizes `stream`; links and vertex indices are checked on the
+ * device (CRT_E_INVALID).  workspace: 256-byte aligned,
+ * crt_wide_build_gpu_workspace bytes. */
 int crt_wide_build_gpu(const crt_stack_node* nodes, int64_t node_count, const crt_triangle* triangles, int64_t ntris,
                        const crt_vertex* vertices, int64_t nverts, int32_t width, crt_wide_group* groups,
                        crt_tri_record* records, int64_t* wide_nodes, void* workspace, size_t workspace_bytes,
