@@ -11,6 +11,7 @@ disabled.  Objects go to candela_amd/_build/, the library next to this file.
 from __future__ import annotations
 
 import argparse
+import concurrent.futures
 import os
 import shutil
 import subprocess
@@ -64,15 +65,17 @@ def build_library(verbose: bool = True) -> str:
     cxx = shutil.which("g++") or "g++"
     os.makedirs(BUILD, exist_ok=True)
     inc = [f"-I{os.path.join(ROOT, 'include')}", f"-I{CSRC}"]
-    objs = []
+    jobs = []
     for src in _sources(HOST_SOURCES):
-        obj = os.path.join(BUILD, src + ".o")
-        _run([cxx, *HOST_FLAGS, *inc, "-c", os.path.join(CSRC, src), "-o", obj])
-        objs.append(obj)
+        jobs.append([cxx, *HOST_FLAGS, *inc, "-c", os.path.join(CSRC, src), "-o", os.path.join(BUILD, src + ".o")])
     for src in _sources(HIP_SOURCES):
-        obj = os.path.join(BUILD, src + ".o")
-        _run([hipcc, *HIP_FLAGS, *inc, "-c", os.path.join(CSRC, src), "-o", obj])
-        objs.append(obj)
+        jobs.append([hipcc, *HIP_FLAGS, *inc, "-c", os.path.join(CSRC, src), "-o", os.path.join(BUILD, src + ".o")])
+    objs = [j[-1] for j in jobs]
+    # translation units are independent: compile them side by side
+    workers = max(1, min(len(jobs), int(os.environ.get("MAX_JOBS", "0") or 0) or (os.cpu_count() or 1), 8))
+    with concurrent.futures.ThreadPoolExecutor(workers) as pool:
+        for f in [pool.submit(_run, j) for j in jobs]:
+            f.result()
     tmp = LIB + ".tmp"
     _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, "-lpthread"])
     os.replace(tmp, LIB)

@@ -96,6 +96,8 @@ def lib() -> C.CDLL:
         l.oracle_bruteforce_closest.argtypes = [vp, vp, vp, i64, i64, vp, vp, vp, i64]
         l.oracle_primary_rays.restype = None
         l.oracle_primary_rays.argtypes = [C.c_int, C.c_int, vp, vp, f32, vp]
+        l.oracle_barycentrics.restype = None
+        l.oracle_barycentrics.argtypes = [vp, vp, vp, vp, vp]
         l.oracle_ray_triangle.restype = None
         l.oracle_ray_triangle.argtypes = [vp, vp, vp, vp, vp, vp]
         l.oracle_bounds_stackless.restype = f32
@@ -241,6 +243,14 @@ def ray_triangle(ro, rd, a, b, c):
     arrs = [np.ascontiguousarray(x, np.float32) for x in (ro, rd, a, b, c)]
     out = np.empty(3, np.float32)
     lib().oracle_ray_triangle(*[_p(x) for x in arrs], _p(out))
+    return out
+
+
+def barycentrics(p, a, b, c) -> np.ndarray:
+    """ComputeBarycentrics (Include/TraverseBVHStackless.glsl:156-172): (u, v, w)."""
+    arrs = [np.ascontiguousarray(x, np.float32) for x in (p, a, b, c)]
+    out = np.empty(3, np.float32)
+    lib().oracle_barycentrics(*[_p(x) for x in arrs], _p(out))
     return out
 
 

@@ -169,6 +169,14 @@ static inline v3 barycentrics(v3 p, v3 a, v3 b, v3 c) {
     return mk(u, v, w);
 }
 
+/* Exported for the closed-form tests of ComputeBarycentrics. */
+void oracle_barycentrics(const float* p, const float* a, const float* b, const float* c, float* out) {
+    v3 r = barycentrics(mk(p[0], p[1], p[2]), mk(a[0], a[1], a[2]), mk(b[0], b[1], b[2]), mk(c[0], c[1], c[2]));
+    out[0] = r.x;
+    out[1] = r.y;
+    out[2] = r.z;
+}
+
 typedef struct {
     int32_t format;
     int32_t entity_count;
