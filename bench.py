@@ -81,6 +81,9 @@ def bytes_per_ray(fmt: int, visits: np.ndarray, tests: np.ndarray) -> np.ndarray
 
 
 def _kernel_name(layout: str, kernel: int, variant: int) -> str:
+    if layout == "stack":
+        return {1: "trace_kernel (stack walk)", 3: "stack_wave, preset %d" % variant,
+                2: "stack_persistent (lane refill), preset %d" % variant}.get(kernel, "trace_kernel")
     if layout == "bvh2":
         return "%s, preset %d" % (KERNEL_NAMES[kernel], variant)
     return {3: "wide_grid, preset %d" % variant, 2: "wide_persistent (lane refill), preset %d" % variant}.get(
@@ -91,6 +94,9 @@ def kernel_label(c: dict, gi_only: bool = False) -> str:
     name = _kernel_name(c["layout"], c["kernel"], c["variant"])
     if c["layout"] == "bvh2":
         label = "plain (one batch per launch): %s, BVH2 stackless" % name
+    elif c["layout"] == "stack":
+        label = "plain (one batch per launch): %s, BVH2 stack layout (FlattenStackBVH), LDS stack %d" % (
+            name, c.get("lds", 16))
     else:
         label = "plain (one batch per launch): %s over the BVH%d layout derived from the BVH2, %d-triangle leaves" % (
             name, c["width"], c["max_leaf"])
@@ -107,6 +113,8 @@ def choice_name(c: dict) -> str:
     AO walks take children in slot order (CRT_FLAG_SLOT_ORDER)."""
     if c["layout"] == "bvh2":
         name = f"{c['kernel']}:{c['variant']}"
+    elif c["layout"] == "stack":
+        name = f"s{c.get('lds', 16)}:{c['kernel']}:{c['variant']}"
     else:
         name = f"w{c['width']}l{c['max_leaf']}:{c['kernel']}:{c['variant']}"
     if "ao_kernel" in c:
@@ -415,6 +423,24 @@ def time_large(crt, args) -> dict:
     return out_d
 
 
+def host_cpu_info() -> dict:
+    """The host's logical CPUs, the ones this process may run on, and the CPU model."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = None
+    return {"host_logical_cpus": os.cpu_count(), "usable_cpus": usable, "cpu_model": model}
+
+
 def copy_probe(crt, args) -> int:
     """Streaming copy of a buffer of known size: measured HBM read+write bandwidth."""
     nbytes = (args.copy_probe // 16) * 16
@@ -456,6 +482,8 @@ def main(argv=None) -> int:
                          "w4l2:3:0 the wave walk preset 0 over the BVH4 layout with 2-triangle leaves; a suffix "
                          "+ao:K:V runs the AO launches with kernel K, variant V)")
     ap.add_argument("--no-wide", action="store_true", help="time the BVH2 kernels only")
+    ap.add_argument("--no-stack", action="store_true",
+                    help="skip the kernels over the stack layout (FlattenStackBVH) of the same meshes")
     ap.add_argument("--no-large", action="store_true",
                     help="skip the large-scene path-tracing workload (15.6M triangles, 3840x2160x4 paths)")
     ap.add_argument("--large-scale", type=int, default=1, metavar="D",
@@ -464,6 +492,8 @@ def main(argv=None) -> int:
                     help="time only path-kernel preset V on the large scene (default: every preset)")
     ap.add_argument("--large-only", action="store_true",
                     help="run only the large-scene path-tracing workload and print its record as one JSON line")
+    ap.add_argument("--detail", default="", metavar="PATH",
+                    help="also write every candidate's timing and the side workloads' records to PATH (JSON)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="instead of the workload, time the streaming copy kernel over BYTES (HBM probe and "
                          "counter calibration)")
@@ -520,9 +550,23 @@ def main(argv=None) -> int:
 
     wides = {}
 
+    stack_ctx = {}
+
     def scene_of(c: dict):
         if c["layout"] == "bvh2":
             return scene
+        if c["layout"] == "stack":
+            if "r" not in stack_ctx:   # the same meshes through FlattenStackBVH (the reference's stack layout)
+                t = time.perf_counter()
+                rs = crt.RayIntersector(crt.FORMAT_STACK)
+                rs.add_object(0, meshes)
+                rs.push_entity(0)
+                rs.buffer_entities()
+                rs.buffer_data()
+                stack_ctx["r"], stack_ctx["scene"] = rs, rs.scene()
+                stack_ctx["hb"] = rs.host_buffers()
+                log(f"stack layout: {stack_ctx['hb']['nodes'].size} entries, built in {time.perf_counter() - t:.2f}s")
+            return stack_ctx["scene"]
         key = (c["width"], c["max_leaf"])
         if key not in wides:
             t = time.perf_counter()
@@ -536,7 +580,8 @@ def main(argv=None) -> int:
         """W warm-up + K timed steps of (AO any-hit, GI closest-hit) with one kernel choice."""
         kernel, variant = c["kernel"], c["variant"]
         scene = scene_of(c)
-        kw = dict(block_size=args.block, lds_stack=args.lds_stack, kernel=kernel, variant=variant, stream=stream)
+        kw = dict(block_size=c.get("block", args.block), lds_stack=c.get("lds", args.lds_stack), kernel=kernel,
+                  variant=variant, stream=stream)
         ak, av = ao_of(c)
         kw_ao = dict(kw, kernel=ak, variant=av, flags=ao_flags(crt, c))
         for _ in range(args.warmup):
@@ -571,6 +616,9 @@ def main(argv=None) -> int:
     def bvh2(k, v):
         return dict(layout="bvh2", kernel=k, variant=v)
 
+    def stack(k, v, lds=16):
+        return dict(layout="stack", kernel=k, variant=v, lds=lds)
+
     def wide(w, leaf, kernel=crt.KERNEL_WAVE, variant=0):
         return dict(layout=f"wide{w}", width=w, max_leaf=leaf, kernel=kernel, variant=variant)
 
@@ -581,12 +629,16 @@ def main(argv=None) -> int:
             candidates += [wide(w, 2, crt.KERNEL_THREAD_PER_RAY) for w in (4, 8)]
             candidates += [wide(w, leaf, k, v) for w in (4, 8) for leaf in (2, 4)
                            for k in (crt.KERNEL_WAVE, crt.KERNEL_PERSISTENT) for v in range(10)]
+        if not args.no_stack:   # the reference's other layout over the same meshes
+            candidates += [stack(crt.KERNEL_THREAD_PER_RAY, 0, lds) for lds in (16, 32)]
     if args.only:
         main_part, _, ao_part = args.only.partition("+ao:")
         parts = main_part.split(":")
         if parts[0].startswith("w"):
             w, leaf = (int(x) for x in parts[0][1:].split("l"))
             only = wide(w, leaf, int(parts[1]), int(parts[2]))
+        elif parts[0].startswith("s"):
+            only = stack(int(parts[1]), int(parts[2]), int(parts[0][1:]))
         else:
             only = bvh2(int(parts[0]), int(parts[1]))
         if ao_part:
@@ -604,14 +656,15 @@ def main(argv=None) -> int:
     if "ao_kernel" not in fastest_gi and \
             (fastest_ao["kernel"], fastest_ao["variant"]) != (fastest_gi["kernel"], fastest_gi["variant"]):
         mixed = {k: v for k, v in fastest_gi.items()
-                 if k in ("layout", "width", "max_leaf", "kernel", "variant")}
+                 if k in ("layout", "width", "max_leaf", "kernel", "variant", "lds", "block")}
         mixed.update(ao_kernel=fastest_ao["kernel"], ao_variant=fastest_ao["variant"])
         results.append(run(mixed))
     # child order of the AO walks: the fastest AO choice of the wide layouts again with slot order
     if not args.only and fastest_gi["layout"] != "bvh2":
         base = min((r for r in results if scene_of(r) is scene_of(fastest_gi)), key=lambda r: r["ms_step"])
         results.append(run(dict({k: v for k, v in base.items() if k in (
-            "layout", "width", "max_leaf", "kernel", "variant", "ao_kernel", "ao_variant")}, ao_order=1)))
+            "layout", "width", "max_leaf", "kernel", "variant", "lds", "block", "ao_kernel", "ao_variant")},
+            ao_order=1)))
     thread = results[0]
     best = max(results, key=lambda r: r["mrays"])
     best_scene = scene_of(best)
@@ -625,7 +678,8 @@ def main(argv=None) -> int:
     for name, dr, dh, q, n in (("ao", d_ao, h_ao, crt.QUERY_ANY, n_ao), ("gi", d_gi, h_gi, crt.QUERY_CLOSEST, n_gi)):
         cnt = crt.DeviceArray(np.uint32, n)
         k_, v_ = ao_of(best) if q == crt.QUERY_ANY else (best["kernel"], best["variant"])
-        crt.trace(best_scene, q, dr, dh, block_size=args.block, lds_stack=args.lds_stack, kernel=k_, variant=v_,
+        crt.trace(best_scene, q, dr, dh, block_size=best.get("block", args.block),
+                  lds_stack=best.get("lds", args.lds_stack), kernel=k_, variant=v_,
                   tri_tests=cnt, flags=ao_flags(crt, best) if q == crt.QUERY_ANY else 0)
         crt.lib().crt_device_sync()
         hits = dh.download()
@@ -786,7 +840,11 @@ def main(argv=None) -> int:
     parity = None
     if not args.no_cpu:
         import oracle
-        threads = max(1, min(args.cpu_threads, os.cpu_count() or 1))
+        try:
+            usable = len(os.sched_getaffinity(0))
+        except (AttributeError, OSError):
+            usable = os.cpu_count() or 1
+        threads = max(1, min(args.cpu_threads, usable))
         nodes, tris, verts, ents = hb["nodes"], hb["triangles"], hb["vertices"], hb["entities"]
         probe = 4096
         ts = time.perf_counter()
@@ -796,12 +854,17 @@ def main(argv=None) -> int:
         # cpu_seconds in total; one thread: the same over a sample a `threads`-th of that size
         n_s = int(min(n_gi, max(probe, args.cpu_seconds / 5 / 2 / max(per_ray, 1e-9))))
 
+        def spread(a, ng):
+            """ng rays at an even stride over the whole image (rays are in pixel order)."""
+            return np.ascontiguousarray(a[np.linspace(0, a.size - 1, ng).astype(np.int64)])
+
         def best_of_3(ng, nthreads):
             rates, hits = [], None
+            sg, sa = spread(gi, ng), spread(ao, ng)
             for rep in range(4):
                 ts = time.perf_counter()
-                hg = oracle.trace(fmt, nodes, tris, verts, ents, 0, gi[:ng], threads=nthreads)
-                ha = oracle.trace(fmt, nodes, tris, verts, ents, 1, ao[:ng], threads=nthreads)
+                hg = oracle.trace(fmt, nodes, tris, verts, ents, 0, sg, threads=nthreads)
+                ha = oracle.trace(fmt, nodes, tris, verts, ents, 1, sa, threads=nthreads)
                 dt = time.perf_counter() - ts
                 if rep > 0:
                     rates.append(2 * ng / dt / 1e6)
@@ -811,17 +874,25 @@ def main(argv=None) -> int:
         cpu_value, _ = best_of_3(n_s, threads)
         n_1 = max(probe // 4, n_s // threads)
         cpu_1, _ = best_of_3(n_1, 1)
+        host = host_cpu_info()
         cpu = {"value": round(cpu_value, 4), "unit": "Mrays/s", "cores": threads, "kind": "port",
-               "value_1thread": round(cpu_1, 4),
-               "sample": f"first {n_s} GI closest-hit + first {n_s} RTAO any-hit rays of the same step on "
-                         f"{threads} threads, first {n_1} + {n_1} on 1 thread; best of 3 passes after 1 warm-up "
-                         f"(C restatement of the GLSL walk, -O3); BVH build excluded (bvh_build_s)"}
+               "value_1thread": round(cpu_1, 4), **host,
+               "sample": f"{n_s} GI closest-hit + {n_s} RTAO any-hit rays taken at an even stride over the whole "
+                         f"image, on {threads} worker threads; {n_1} + {n_1} on 1 thread; best of 3 passes after "
+                         f"1 warm-up (C restatement of the GLSL walk, -O3); BVH build excluded "
+                         f"({round(t2 - t1, 2)} s)"}
         # parity over every ray of the step
         g, a = out["gi"]["hits"], out["ao"]["hits"]
         ohg = oracle.trace(fmt, nodes, tris, verts, ents, 0, gi, threads=threads)
         oha = oracle.trace(fmt, nodes, tris, verts, ents, 1, ao, threads=threads)
         if best["layout"] == "bvh2":
             twin_g, twin_a = ohg, oha
+        elif best["layout"] == "stack":
+            sh_ = stack_ctx["hb"]
+            twin_g = oracle.trace(crt.FORMAT_STACK, sh_["nodes"], sh_["triangles"], sh_["vertices"], sh_["entities"],
+                                  0, gi, threads=threads)
+            twin_a = oracle.trace(crt.FORMAT_STACK, sh_["nodes"], sh_["triangles"], sh_["vertices"], sh_["entities"],
+                                  1, ao, threads=threads)
         else:   # the host twin of the wide walk, on the same wide layout
             w = wides[(best["width"], best["max_leaf"])].host
             twin_g = oracle.trace_wide(w.width, w.groups, w.records, w.entities, 0, gi, threads=threads)
@@ -841,6 +912,50 @@ def main(argv=None) -> int:
         log(f"cpu baseline {cpu_value:.3f} Mrays/s on {threads} threads, {cpu_1:.3f} Mrays/s on 1 thread; "
             f"parity {parity}")
 
+    def short_choice(c):
+        return None if c is None else {"choice": choice_name(c), "mrays_s": round(c["mrays"], 2),
+                                       "ms_step": round(c["ms_step"], 4), "gi_ms": round(c["gi_ms"], 4),
+                                       "ao_ms": round(c["ao_ms"], 4)}
+
+    def best_where(pred):
+        rs = [r for r in results if pred(r)]
+        return max(rs, key=lambda r: r["mrays"]) if rs else None
+
+    per_format = {
+        "stackless": short_choice(best_where(lambda r: r["layout"] == "bvh2" and fmt == crt.FORMAT_STACKLESS)),
+        "stack": short_choice(best_where(lambda r: (r["layout"] == "bvh2" and fmt == crt.FORMAT_STACK)
+                                         or r["layout"] == "stack")),
+        "bvh4": short_choice(best_where(lambda r: r["layout"] == "wide4")),
+        "bvh8": short_choice(best_where(lambda r: r["layout"] == "wide8")),
+    }
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "kernel": gi_choice_name(best) + " (GI closest hit)", "launch_ms": round(gi_s * 1e3, 4),
+                "bytes_per_launch": out["gi"]["bytes"], "bytes_per_ray": round(out["gi"]["bytes"] / n_gi, 1),
+                "traffic_source": traffic_src}
+    if traffic:
+        roofline["traffic_frac_of_peak"] = round(traffic / gi_s / 1e9 / HBM_PEAK_GBS, 4)
+    full = {
+        "kernels": [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()} for r in results],
+        "kernel_label": kernel_label(best),
+        "batched_vs_plain": batched,
+        "c2_batched": c2_batched,
+        "attributes": attrs,
+        "raygen": raygen,
+        "gpu_build": gpu_build,
+        "large_scene": large,
+        "cpu_baseline": cpu,
+        "parity": parity,
+        "detail": {k: {kk: vv for kk, vv in v.items() if kk != "hits"} for k, v in out.items()},
+    }
+    log("detail " + json.dumps(full))
+    if args.detail:
+        with open(args.detail, "w") as f:
+            json.dump(full, f, indent=1)
+
+    def pick(d, keys):
+        return None if not isinstance(d, dict) else {k: d[k] for k in keys if k in d}
+
     line = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -859,30 +974,23 @@ def main(argv=None) -> int:
                         "2000 icospheres (2.56M tris)",
             "triangles": ntri,
             "rays_per_step": rays_step,
-            "gi_rays": n_gi,
-            "ao_rays": n_ao,
-            "format": args.format,
-            "kernel": kernel_label(best),
+            "kernel": choice_name(best),
             "parallelism": "single GPU",
         },
-        "plain_mrays_s": round(value, 2),
-        "thread_per_ray_mrays_s": round(thread["mrays"], 2),
+        "plain_by_format": per_format,
         "batched_mrays_s": (round(rays_step / (c2_batched["ms"] * 1e-3) / 1e6, 2) if c2_batched else None),
-        "batched_vs_plain": batched,
-        "kernels": [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()} for r in results],
-        "roofline": {"bound": "hbm", "kernel": kernel_label(best, gi_only=True) + " (GI closest hit)", "achieved": round(achieved, 1),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic, "traffic_source": traffic_src,
-                     "bytes_per_launch": out["gi"]["bytes"], "launch_ms": round(gi_s * 1e3, 4),
-                     "bytes_per_ray": round(out["gi"]["bytes"] / n_gi, 1)},
-        "attributes": attrs,
-        "raygen": raygen,
-        "bvh_build_s": round(t2 - t1, 3),
-        "gpu_build": gpu_build,
-        "large_scene": large,
+        "batched_vs_plain": pick(batched, ("plain_launches", "plain_ms", "plain_layout", "batched_ms",
+                                           "batched_layout", "batched_mrays_s", "speedup", "speedup_same_layout",
+                                           "hits_identical")),
+        "roofline": roofline,
         "cpu_baseline": cpu,
         "parity": parity,
-        "detail": {k: {kk: vv for kk, vv in v.items() if kk != "hits"} for k, v in out.items()},
+        "large_scene": pick(large, ("triangles", "paths", "segments", "ms_per_frame", "mrays_s", "parity",
+                                    "error")),
+        "gpu_build": pick(gpu_build, ("triangles", "ms", "wide_collapse_async_ms", "rebuild_frame_graph_ms",
+                                      "error")),
+        "attributes_mhits_s": attrs.get("mhits_s") if attrs else None,
+        "raygen_mrays_s": raygen.get("mrays_s") if raygen else None,
     }
     print(json.dumps(line), flush=True)
     return 0

@@ -21,7 +21,7 @@ from ._lib import (KERNEL_AUTO, KERNEL_PERSISTENT, KERNEL_THREAD_PER_RAY, KERNEL
                    STATUS_VISITS_MASK, SWAP_NONE, SWAP_SEEDED, TRIANGLE_DTYPE, VERTEX_DTYPE, BuildOptions,
                    BvhStats, CrtError, Mesh, SceneView, TraceConfig, Batch, BatchConfig, ShadeParams,
                    SHADE_RECORD_DTYPE, RAYGEN_PARAMS_DTYPE, PROBE_GRID_DTYPE, CacheHeader, E_CORRUPT, E_IO, E_STALE, WIDE_GPU_BAD_LINK, WIDE_GPU_NOT_TREE,
-                   WIDE_GPU_BAD_VERTEX, WIDE_GPU_TOO_DEEP, WIDE_GPU_MAX_LEVELS, check, lib, node_dtype, ptr)
+                   WIDE_GPU_BAD_VERTEX, WIDE_GPU_TOO_DEEP, WIDE_GPU_BAD_LEAF, WIDE_GPU_MAX_LEVELS, check, lib, node_dtype, ptr)
 
 __all__ = [
     "VERTEX_DTYPE", "TRIANGLE_DTYPE", "STACKLESS_NODE_DTYPE", "STACK_NODE_DTYPE", "BOUNDS_DTYPE", "ENTITY_DTYPE",
@@ -37,7 +37,7 @@ __all__ = [
     "PATH_PARAMS_DTYPE", "PATH_SAMPLE_DTYPE", "path_params", "trace_paths", "hash64", "bvh_key", "load_bvh",
     "CacheHeader", "E_IO", "E_CORRUPT", "E_STALE", "RAYGEN_PARAMS_DTYPE", "PROBE_GRID_DTYPE", "raygen_params",
     "RayGen", "probe_grid", "gen_probe_rays", "GpuBVH", "Graph", "WIDE_GPU_BAD_LINK", "WIDE_GPU_NOT_TREE",
-    "WIDE_GPU_BAD_VERTEX", "WIDE_GPU_TOO_DEEP", "WIDE_GPU_MAX_LEVELS",
+    "WIDE_GPU_BAD_VERTEX", "WIDE_GPU_TOO_DEEP", "WIDE_GPU_BAD_LEAF", "WIDE_GPU_MAX_LEVELS",
 ]
 
 
@@ -428,8 +428,10 @@ class GpuBVH:
     """crt_build_bvh_gpu: the stack layout of one object built on the device (a linear BVH over Morton codes;
     not BuildBVH's tree).  `scene` is a crt_scene_view over it with one identity entity."""
 
-    def __init__(self, vertices: np.ndarray, indices: np.ndarray, mesh_ids=None, mesh_id: int = 0, stream=None):
+    def __init__(self, vertices: np.ndarray, indices: np.ndarray, mesh_ids=None, mesh_id: int = 0, stream=None,
+                 t_offset: int = 0):
         v = np.ascontiguousarray(vertices, VERTEX_DTYPE)
+        self.t_offset = int(t_offset)
         idx = np.ascontiguousarray(indices, np.uint32).reshape(-1)
         if idx.size % 3 or idx.size < 6 or int(idx.max()) >= v.size:
             raise ValueError("GpuBVH needs >= 2 triangles whose vertex indices lie in the vertex array")
@@ -449,7 +451,8 @@ class GpuBVH:
     def build(self, stream=None) -> None:
         """Rebuilds from the current vertex buffer (asynchronous)."""
         check(lib().crt_build_bvh_gpu(self.vertices.ptr, self.vertices.count, self.indices.ptr, self.ntris,
-                                      self.mesh_ids.ptr if self.mesh_ids is not None else None, self.mesh_id, 0,
+                                      self.mesh_ids.ptr if self.mesh_ids is not None else None, self.mesh_id,
+                                      self.t_offset,
                                       self.nodes.ptr, self.triangles.ptr, self.workspace.ptr, self.workspace.nbytes,
                                       _stream_handle(stream)))
 

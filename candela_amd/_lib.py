@@ -35,7 +35,7 @@ KERNEL_AUTO = 0
 KERNEL_THREAD_PER_RAY = 1
 KERNEL_PERSISTENT = 2
 KERNEL_WAVE = 3
-WIDE_GPU_BAD_LINK, WIDE_GPU_NOT_TREE, WIDE_GPU_BAD_VERTEX, WIDE_GPU_TOO_DEEP = 1, 2, 4, 8
+WIDE_GPU_BAD_LINK, WIDE_GPU_NOT_TREE, WIDE_GPU_BAD_VERTEX, WIDE_GPU_TOO_DEEP, WIDE_GPU_BAD_LEAF = 1, 2, 4, 8, 16
 WIDE_GPU_MAX_LEVELS = 4096
 
 # --- record dtypes (byte layouts of the reference buffers) ---------------------

@@ -45,10 +45,18 @@ struct Kid {
 
 // Bits of *err: a link leaves the node array; the links reach more wide
 // nodes than the output holds (they do not form a tree); a vertex index
-// leaves the vertex array.
-constexpr unsigned kErrLink = 1u, kErrCap = 2u, kErrVertex = 4u;
+// leaves the vertex array; a leaf range is empty or leaves [0, ntris) (a tree
+// built with a triangle offset, or damaged entries).
+constexpr unsigned kErrLink = 1u, kErrCap = 2u, kErrVertex = 4u, kErrLeaf = 16u;
 
-__device__ __forceinline__ Kid make_kid(const crt_stack_node* __restrict__ nodes, int64_t node_count,
+// A leaf's triangle range indexes the records: it must lie in [0, ntris).
+__device__ __forceinline__ bool leaf_ok(int32_t start, int32_t count, int64_t ntris) {
+    return count >= 1 && start >= 0 && int64_t(start) + count <= ntris;
+}
+
+// A child slot that fails a check (bad link, bad leaf range) comes back as a
+// leaf of count 0, which expand_one writes as an empty slot (count -1).
+__device__ __forceinline__ Kid make_kid(const crt_stack_node* __restrict__ nodes, int64_t node_count, int64_t ntris,
                                         const crt_bounds& b, unsigned* err) {
     Kid k;
     for (int q = 0; q < 3; ++q) {
@@ -62,6 +70,10 @@ __device__ __forceinline__ Kid make_kid(const crt_stack_node* __restrict__ nodes
     if (wmin != -1) {
         k.start = wmin >> 4;
         k.count = wmin & 0xF;
+        if (!leaf_ok(k.start, k.count, ntris)) {
+            atomicOr(err, kErrLeaf);
+            k.start = k.count = 0;
+        }
         return k;
     }
     const int32_t e = __float_as_int(b.max[3]);
@@ -79,6 +91,10 @@ __device__ __forceinline__ Kid make_kid(const crt_stack_node* __restrict__ nodes
         if (na + nc <= 2 && last - first == na + nc) {
             k.start = first;
             k.count = na + nc;
+            if (!leaf_ok(k.start, k.count, ntris)) {
+                atomicOr(err, kErrLeaf);
+                k.start = k.count = 0;
+            }
             return k;
         }
     }
@@ -116,13 +132,13 @@ __device__ __forceinline__ uint32_t block_excl(uint32_t v, uint32_t* wave_sums, 
 // otherwise); returns its number of inner children.
 template <int W>
 __device__ __forceinline__ uint32_t expand_one(const crt_stack_node* __restrict__ nodes, int64_t node_count,
-                                               int32_t entry, int64_t w, crt_wide_group* __restrict__ groups,
+                                               int64_t ntris, int32_t entry, int64_t w, crt_wide_group* __restrict__ groups,
                                                int32_t* __restrict__ kids_out, unsigned* __restrict__ err) {
     const crt_stack_node& e = nodes[entry];
     Kid kids[W];
     int n = 0;
-    kids[n++] = make_kid(nodes, node_count, e.left, err);
-    kids[n++] = make_kid(nodes, node_count, e.right, err);
+    kids[n++] = make_kid(nodes, node_count, ntris, e.left, err);
+    kids[n++] = make_kid(nodes, node_count, ntris, e.right, err);
     while (n < W) {
         int best = -1;
         float best_area = -1.0f;
@@ -137,8 +153,8 @@ __device__ __forceinline__ uint32_t expand_one(const crt_stack_node* __restrict_
         if (best < 0) break;
         const crt_stack_node& c = nodes[kids[best].entry];
         for (int k = n; k > best + 1; --k) kids[k] = kids[k - 1];
-        kids[best + 1] = make_kid(nodes, node_count, c.right, err);
-        kids[best] = make_kid(nodes, node_count, c.left, err);
+        kids[best + 1] = make_kid(nodes, node_count, ntris, c.right, err);
+        kids[best] = make_kid(nodes, node_count, ntris, c.left, err);
         ++n;
     }
     uint32_t m = 0;
@@ -154,8 +170,9 @@ __device__ __forceinline__ uint32_t expand_one(const crt_stack_node* __restrict_
         g.hi_y[j] = used ? k.hi[1] : 0.0f;
         g.hi_z[j] = used ? k.hi[2] : 0.0f;
         const bool inner = used && k.entry >= 0;
-        g.child[j] = used && !inner ? k.start : 0;   // inner: assign writes it
-        g.count[j] = !used ? -1 : (inner ? 0 : k.count);
+        const bool leaf = used && !inner && k.count >= 1;   // else an empty slot (unused or failed a check)
+        g.child[j] = leaf ? k.start : 0;                     // inner: assign writes it
+        g.count[j] = inner ? 0 : (leaf ? k.count : -1);
         kids_out[s] = inner ? k.entry : -1;
         m += inner ? 1u : 0u;
     }
@@ -200,7 +217,8 @@ __device__ __forceinline__ bool assign_one(int64_t w, const int32_t* __restrict_
 // Bit of *err: the tree has more levels than were launched.
 constexpr unsigned kErrDepth = 8u;
 static_assert(kErrLink == CRT_WIDE_GPU_BAD_LINK && kErrCap == CRT_WIDE_GPU_NOT_TREE &&
-                  kErrVertex == CRT_WIDE_GPU_BAD_VERTEX && kErrDepth == CRT_WIDE_GPU_TOO_DEEP,
+                  kErrVertex == CRT_WIDE_GPU_BAD_VERTEX && kErrDepth == CRT_WIDE_GPU_TOO_DEEP &&
+                  kErrLeaf == CRT_WIDE_GPU_BAD_LEAF,
               "status bits are the header's");
 constexpr int kGridMax = 2048;
 
@@ -237,7 +255,8 @@ __device__ __forceinline__ void chunk_of(int64_t nf, int32_t grid, int64_t& begi
 // (crt_wide_build's rule): this kernel writes that root, and level 0 is then
 // empty.  Otherwise level 0 is the root entry.
 template <int W>
-__global__ void __launch_bounds__(64) root_async(const crt_stack_node* __restrict__ nodes, int32_t* __restrict__ entry,
+__global__ void __launch_bounds__(64) root_async(const crt_stack_node* __restrict__ nodes, int64_t ntris,
+                                                 int32_t* __restrict__ entry,
                                                  int32_t* __restrict__ wide, int64_t* __restrict__ lvl,
                                                  crt_wide_group* __restrict__ groups, unsigned* __restrict__ err) {
     if (threadIdx.x != 0) return;
@@ -248,6 +267,8 @@ __global__ void __launch_bounds__(64) root_async(const crt_stack_node* __restric
     const int32_t a = __float_as_int(l.min[3]), c = __float_as_int(r.min[3]);
     const int32_t sa = a >> 4, na = a & 0xF, sc = c >> 4, nc = c & 0xF;
     const bool one = a != -1 && c != -1 && na + nc <= 2 && max(sa + na, sc + nc) - min(sa, sc) == na + nc;
+    const bool one_ok = one && leaf_ok(min(sa, sc), na + nc, ntris);
+    if (one && !one_ok) *err = kErrLeaf;   // finish_async leaves the root empty
     if (one) {
         for (int q = 0; q < W / 4; ++q) {
             crt_wide_group& g = groups[q];
@@ -259,8 +280,8 @@ __global__ void __launch_bounds__(64) root_async(const crt_stack_node* __restric
                 g.hi_x[j] = s0 ? fmaxf(l.max[0], r.max[0]) : 0.0f;
                 g.hi_y[j] = s0 ? fmaxf(l.max[1], r.max[1]) : 0.0f;
                 g.hi_z[j] = s0 ? fmaxf(l.max[2], r.max[2]) : 0.0f;
-                g.child[j] = s0 ? min(sa, sc) : 0;
-                g.count[j] = s0 ? na + nc : -1;
+                g.child[j] = s0 && one_ok ? min(sa, sc) : 0;
+                g.count[j] = s0 && one_ok ? na + nc : -1;
             }
         }
     }
@@ -272,7 +293,7 @@ __global__ void __launch_bounds__(64) root_async(const crt_stack_node* __restric
 // the grid when the level is not empty.
 template <int W>
 __global__ void __launch_bounds__(kBlock) expand_async(const crt_stack_node* __restrict__ nodes, int64_t node_count,
-                                                       const int64_t* __restrict__ lvl, int32_t level, int32_t grid,
+                                                       int64_t ntris, const int64_t* __restrict__ lvl, int32_t level, int32_t grid,
                                                        const int32_t* __restrict__ entries,
                                                        const int32_t* __restrict__ wides,
                                                        crt_wide_group* __restrict__ groups,
@@ -288,7 +309,8 @@ __global__ void __launch_bounds__(kBlock) expand_async(const crt_stack_node* __r
         const int64_t item = base + threadIdx.x;
         uint32_t m = 0;
         if (item < end) {
-            m = expand_one<W>(nodes, node_count, entries[item], wides[item], groups, kids_out + item * W, err);
+            m = expand_one<W>(nodes, node_count, ntris, entries[item], wides[item], groups, kids_out + item * W,
+                              err);
             counts[item] = m;
         }
         uint32_t total;
@@ -349,10 +371,24 @@ __global__ void __launch_bounds__(kBlock) assign_async(int64_t* __restrict__ lvl
 }
 
 // The node count, the number of levels the tree needs (the first empty
-// level; max_levels when none was) and the status word for the caller.
+// level; max_levels when none was) and the status word for the caller.  A
+// layout with any status bit is incomplete (levels never expanded hold stale
+// memory, a failed slot reads as empty): its root is rewritten with every
+// slot empty, so a trace that runs after it, in the same graph say, ends at
+// the root with no hit.
+template <int W>
 __global__ void __launch_bounds__(64) finish_async(const int64_t* __restrict__ lvl, int32_t levels,
                                                    const unsigned* __restrict__ err, int64_t* __restrict__ wide_nodes,
-                                                   uint32_t* __restrict__ status) {
+                                                   uint32_t* __restrict__ status, crt_wide_group* __restrict__ groups) {
+    const uint32_t st = *err | (lvl[2 * levels + 1] > 0 ? kErrDepth : 0u);
+    if (st != 0u) {
+        // 64 lanes: W / 4 groups of 8 fields of 4 words (32 words each)
+        for (int k = int(threadIdx.x); k < (W / 4) * 32; k += 64) {
+            int32_t* g = reinterpret_cast<int32_t*>(groups + k / 32);
+            const int word = k % 32;
+            g[word] = word >= 28 ? -1 : 0;   // count[4] is the last field
+        }
+    }
     if (threadIdx.x != 0) return;
     int32_t used = levels;
     for (int32_t l = 0; l < levels; ++l)
@@ -362,7 +398,7 @@ __global__ void __launch_bounds__(64) finish_async(const int64_t* __restrict__ l
         }
     wide_nodes[0] = lvl[2 * levels];
     wide_nodes[1] = used;
-    *status = *err | (lvl[2 * levels + 1] > 0 ? kErrDepth : 0u);
+    *status = st;
 }
 
 // Triangle records in leaf order (crt_wide_build's).
@@ -435,17 +471,17 @@ void collapse_async(const crt_stack_node* nodes, int64_t node_count, const crt_t
     auto* lvl = reinterpret_cast<int64_t*>(ws + L.lvl);
     auto* err = reinterpret_cast<unsigned*>(ws + L.err);
     const int32_t grid = int32_t(std::min<int64_t>(kGridMax, (cap + kBlock - 1) / kBlock));
-    hipLaunchKernelGGL(root_async<W>, dim3(1), dim3(64), 0, st, nodes, entries[0], wides[0], lvl, groups, err);
+    hipLaunchKernelGGL(root_async<W>, dim3(1), dim3(64), 0, st, nodes, cap, entries[0], wides[0], lvl, groups, err);
     hipLaunchKernelGGL(records, dim3(unsigned((cap + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, tris, verts, nverts,
                        cap, recs, err);
     for (int32_t l = 0; l < levels; ++l) {
         const int cur = l & 1;
-        hipLaunchKernelGGL(expand_async<W>, dim3(grid), dim3(kBlock), 0, st, nodes, node_count, lvl, l, grid,
+        hipLaunchKernelGGL(expand_async<W>, dim3(grid), dim3(kBlock), 0, st, nodes, node_count, cap, lvl, l, grid,
                            entries[cur], wides[cur], groups, kids, counts, totals, err);
         hipLaunchKernelGGL(assign_async<W>, dim3(grid), dim3(kBlock), 0, st, lvl, l, grid, cap, wides[cur], kids, counts,
                            totals, groups, entries[1 - cur], wides[1 - cur], err);
     }
-    hipLaunchKernelGGL(finish_async, dim3(1), dim3(64), 0, st, lvl, levels, err, wide_nodes, status);
+    hipLaunchKernelGGL(finish_async<W>, dim3(1), dim3(64), 0, st, lvl, levels, err, wide_nodes, status, groups);
 }
 
 }  // namespace wg
@@ -498,6 +534,9 @@ int crt_wide_build_gpu(const crt_stack_node* nodes, int64_t node_count, const cr
     if (bad & kErrVertex)
         return crt::set_error(CRT_E_INVALID, "crt_wide_build_gpu: a vertex index leaves the vertex array");
     if (bad & kErrLink) return crt::set_error(CRT_E_INVALID, "crt_wide_build_gpu: a stack link leaves the node array");
+    if (bad & kErrLeaf)
+        return crt::set_error(CRT_E_INVALID, "crt_wide_build_gpu: a leaf range is empty or leaves [0, ntris) (the "
+                                             "tree must be built with triangle offset 0)");
     if (bad & kErrCap)
         return crt::set_error(CRT_E_INVALID,
                               "crt_wide_build_gpu: more wide nodes than triangles (the links do not form a tree)");

@@ -463,7 +463,8 @@ size_t crt_wide_build_gpu_workspace(int64_t ntris, int32_t width);
  * than depth-first, so walks of either layout give identical hit records and
  * counters.  groups: device room for ntris * width / 4 groups; records
  * [ntris]; *wide_nodes (host) receives the node count (the entity's
- * node_count, root 0).  Runs crt_wide_build_gpu_async with 32 levels (again
+ * node_count, root 0).  The tree must be built with t_offset 0: every leaf range
+ * must lie in [0, ntris), else CRT_E_INVALID.  Runs crt_wide_build_gpu_async with 32 levels (again
  * with CRT_WIDE_GPU_MAX_LEVELS for a deeper tree) and reads the result back,
  * so it synchronizes `stream`; links and vertex indices are checked on the
  * device (CRT_E_INVALID).  workspace: 256-byte aligned,
@@ -478,6 +479,8 @@ int crt_wide_build_gpu(const crt_stack_node* nodes, int64_t node_count, const cr
 #define CRT_WIDE_GPU_NOT_TREE 2u     /* more wide nodes than triangles */
 #define CRT_WIDE_GPU_BAD_VERTEX 4u   /* a vertex index leaves the vertex array */
 #define CRT_WIDE_GPU_TOO_DEEP 8u     /* the wide tree has more than max_levels levels */
+#define CRT_WIDE_GPU_BAD_LEAF 16u    /* a leaf range is empty or leaves [0, ntris): the tree
+                                        was built with a triangle offset, or is damaged */
 #define CRT_WIDE_GPU_MAX_LEVELS 4096
 
 /* Bytes of device workspace crt_wide_build_gpu_async needs (0 for invalid
@@ -492,7 +495,10 @@ size_t crt_wide_build_gpu_async_workspace(int64_t ntris, int32_t width, int32_t 
  * number of levels the tree needs (max_levels at least that gives status 0;
  * a caller that rebuilds every frame can size the next frame's launch from
  * it), and the CRT_WIDE_GPU_* bits.  Nothing is read back, so errors in the data show only
- * in *status; argument errors return CRT_E_INVALID.  A trace can run before
+ * in *status; argument errors return CRT_E_INVALID.  When any status bit is set the
+ * root is written with every slot empty, so a trace launched after the collapse (in
+ * the same graph, before the status is read) ends at the root with no hit and reads
+ * nothing else of the incomplete layout.  A trace can run before
  * the node count is known: the scene's node count may be ntris (the capacity
  * of `groups`), which bounds every valid link.  workspace: 256-byte aligned,
  * crt_wide_build_gpu_async_workspace bytes. */

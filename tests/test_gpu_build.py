@@ -269,6 +269,53 @@ def test_wide_collapse_async_errors(gpu):
     assert a.result()[1] == 0
 
 
+def test_wide_collapse_error_layouts_trace_safely(gpu):
+    """Every layout the collapse flags is safe to trace: a tree built with a triangle offset (leaf starts past the
+    records), a truncated collapse (max_levels one short) and a bad link each set their status bit, and closest-
+    and any-hit traces launched right after, over the scene that spans ntris nodes, end normally with no hit and no
+    guard bit (the root is written empty)."""
+    verts, idx, _ = _soup(3000, 12)
+    rays = random_rays(5000, [-60, -60, -60], [60, 60, 60], seed=43)
+
+    def traced_empty(a, what):
+        for q in (0, 1):
+            for kernel, variant in ((3, 0), (2, 8), (1, 0)):
+                h = gpu_trace(gpu, a.scene, q, rays, kernel=kernel, variant=variant)
+                assert (h["t"] == -1).all() and (h["tri"] == -1).all(), f"{what}: hits after an error (q={q})"
+                assert (h["status"] & 0xFF000000 == 0).all(), f"{what}: guard bits after an error"
+
+    log("[case] collapse of a tree built with t_offset 100")
+    off = gpu.GpuBVH(verts, idx, t_offset=100)
+    with pytest.raises(gpu.CrtError, match="leaf range"):
+        gpu.GpuWide(off, 4)
+    for width in (4, 8):
+        a = gpu.GpuWide(off, width, max_levels=64)
+        n, status = a.result()
+        assert status & gpu.WIDE_GPU_BAD_LEAF, f"status {status:#x}"
+        traced_empty(a, f"t_offset width {width}")
+
+    log("[case] truncated collapse")
+    b = gpu.GpuBVH(verts, idx)
+    full = gpu.GpuWide(b, 4, max_levels=64)
+    assert full.result()[1] == 0
+    need = full.levels
+    for width in (4, 8):
+        a = gpu.GpuWide(b, width, max_levels=need - 1)
+        a.groups.fill_bytes(0x3C)        # stale memory in the levels that are never expanded
+        a.build()
+        assert a.result()[1] & gpu.WIDE_GPU_TOO_DEEP
+        traced_empty(a, f"truncated width {width}")
+
+    log("[case] bad link")
+    nodes = b.nodes.download()
+    nodes[0]["left"]["max"][3] = np.array([b.nodes.count + 5], np.int32).view(np.float32)[0]
+    b.nodes.upload(nodes)
+    a = gpu.GpuWide(b, 4, max_levels=64)
+    assert a.result()[1] & gpu.WIDE_GPU_BAD_LINK
+    traced_empty(a, "bad link")
+    log("[parity] error layouts trace to misses")
+
+
 def test_rebuild_graph(gpu):
     """The per-frame rebuild in one graph: crt_build_bvh_gpu, crt_wide_build_gpu_async and a closest-hit and an
     any-hit trace over the new layout, captured once and replayed on moving geometry (eager uploads and
