@@ -769,11 +769,12 @@ def main(argv=None) -> int:
             crt.trace(ga.scene, crt.QUERY_CLOSEST, d_gi, dh_w, kernel=k_w, variant=v_w, stream=stream)
         stream.sync()
         graph = crt.Graph(stream)
+        q_ao, q_gi = crt.queue_block(), crt.queue_block()   # the graph owns its work-queue counters
         with graph:
             gb.build(stream)
             ga.build(stream)
-            crt.trace(ga.scene, crt.QUERY_ANY, d_ao, dh_ao, kernel=k_w, variant=v_w, stream=stream)
-            crt.trace(ga.scene, crt.QUERY_CLOSEST, d_gi, dh_w, kernel=k_w, variant=v_w, stream=stream)
+            crt.trace(ga.scene, crt.QUERY_ANY, d_ao, dh_ao, kernel=k_w, variant=v_w, stream=stream, queue=q_ao)
+            crt.trace(ga.scene, crt.QUERY_CLOSEST, d_gi, dh_w, kernel=k_w, variant=v_w, stream=stream, queue=q_gi)
         graph.launch()
         stream.sync()
         ev = Events(crt, 2)

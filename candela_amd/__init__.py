@@ -21,7 +21,7 @@ from ._lib import (KERNEL_AUTO, KERNEL_PERSISTENT, KERNEL_THREAD_PER_RAY, KERNEL
                    STATUS_VISITS_MASK, SWAP_NONE, SWAP_SEEDED, TRIANGLE_DTYPE, VERTEX_DTYPE, BuildOptions,
                    BvhStats, CrtError, Mesh, SceneView, TraceConfig, Batch, BatchConfig, ShadeParams,
                    SHADE_RECORD_DTYPE, RAYGEN_PARAMS_DTYPE, PROBE_GRID_DTYPE, CacheHeader, E_CORRUPT, E_IO, E_STALE, WIDE_GPU_BAD_LINK, WIDE_GPU_NOT_TREE,
-                   WIDE_GPU_BAD_VERTEX, WIDE_GPU_TOO_DEEP, WIDE_GPU_BAD_LEAF, WIDE_GPU_MAX_LEVELS, check, lib, node_dtype, ptr)
+                   WIDE_GPU_BAD_VERTEX, WIDE_GPU_TOO_DEEP, WIDE_GPU_BAD_LEAF, WIDE_GPU_MAX_LEVELS, QUEUE_BYTES, check, lib, node_dtype, ptr)
 
 __all__ = [
     "VERTEX_DTYPE", "TRIANGLE_DTYPE", "STACKLESS_NODE_DTYPE", "STACK_NODE_DTYPE", "BOUNDS_DTYPE", "ENTITY_DTYPE",
@@ -37,7 +37,8 @@ __all__ = [
     "PATH_PARAMS_DTYPE", "PATH_SAMPLE_DTYPE", "path_params", "trace_paths", "hash64", "bvh_key", "load_bvh",
     "CacheHeader", "E_IO", "E_CORRUPT", "E_STALE", "RAYGEN_PARAMS_DTYPE", "PROBE_GRID_DTYPE", "raygen_params",
     "RayGen", "probe_grid", "gen_probe_rays", "GpuBVH", "Graph", "WIDE_GPU_BAD_LINK", "WIDE_GPU_NOT_TREE",
-    "WIDE_GPU_BAD_VERTEX", "WIDE_GPU_TOO_DEEP", "WIDE_GPU_BAD_LEAF", "WIDE_GPU_MAX_LEVELS",
+    "WIDE_GPU_BAD_VERTEX", "WIDE_GPU_TOO_DEEP", "WIDE_GPU_BAD_LEAF", "WIDE_GPU_MAX_LEVELS", "QUEUE_BYTES",
+    "queue_block",
 ]
 
 
@@ -85,6 +86,8 @@ class Graph:
         if exc_type is None:
             check(rc)
             self.handle = h.value
+        elif rc == 0 and h.value:   # the capture failed part way: drop what it recorded
+            lib().crt_graph_destroy(h.value)
         return False
 
     def launch(self, stream: "Stream | None" = None) -> None:
@@ -334,13 +337,14 @@ def path_params(width: int, height: int, spp: int, inv_view, inv_proj, albedo=No
 
 
 def trace_paths(scene: SceneView, params: np.ndarray, out: DeviceArray, npaths: int | None = None, variant: int = 0,
-                grid: int = 0, stream=None) -> None:
-    """crt_trace_paths: paths [0, npaths) into out (PATH_SAMPLE_DTYPE), asynchronous."""
+                grid: int = 0, stream=None, queue: "DeviceArray | None" = None) -> None:
+    """crt_trace_paths: paths [0, npaths) into out (PATH_SAMPLE_DTYPE), asynchronous.  queue: a work-queue block
+    (queue_block()) the launch uses instead of one leased from the library's pool (required under capture)."""
     params = np.ascontiguousarray(params, PATH_PARAMS_DTYPE).reshape(1)
     npaths = out.count if npaths is None else int(npaths)
     if npaths > out.count:
         raise ValueError("output smaller than npaths")
-    cfg = TraceConfig(0, 0, 0, variant, grid, 0, None)
+    cfg = TraceConfig(0, 0, 0, variant, grid, 0, None, queue.ptr if queue is not None else None)
     check(lib().crt_trace_paths(C.byref(scene), ptr(params), out.ptr, npaths, C.byref(cfg), _stream_handle(stream)))
 
 
@@ -549,14 +553,24 @@ def _colmajor(m) -> np.ndarray:
     return np.ascontiguousarray(m.T.reshape(16))
 
 
+def queue_block() -> DeviceArray:
+    """Device memory for one launch's work-queue counters (CRT_QUEUE_BYTES, 256-byte aligned)."""
+    return DeviceArray(np.uint8, QUEUE_BYTES)
+
+
 def trace(scene: SceneView, query: int, rays: DeviceArray, hits: DeviceArray, n: int | None = None, flags: int = 0,
           block_size: int = 0, lds_stack: int = 0, kernel: int = KERNEL_AUTO, variant: int = 0, grid: int = 0,
-          tri_tests: DeviceArray | None = None, stream=None) -> None:
-    """crt_trace: closest- or any-hit for every ray of a device ray buffer (asynchronous)."""
+          tri_tests: DeviceArray | None = None, stream=None, queue: "DeviceArray | None" = None) -> None:
+    """crt_trace: closest- or any-hit for every ray of a device ray buffer (asynchronous).  queue: a work-queue
+    block (queue_block()) for the persistent kernels instead of one leased from the library's pool (required when
+    the launch is captured into a graph)."""
     n = rays.count if n is None else int(n)
     if n > rays.count or n > hits.count or (tri_tests is not None and n > tri_tests.count):
         raise ValueError("ray/hit/counter buffers smaller than n")
-    cfg = TraceConfig(block_size, lds_stack, kernel, variant, grid, 0, tri_tests.ptr if tri_tests is not None else None)
+    if queue is not None and queue.nbytes < QUEUE_BYTES:
+        raise ValueError("queue block smaller than CRT_QUEUE_BYTES")
+    cfg = TraceConfig(block_size, lds_stack, kernel, variant, grid, 0, tri_tests.ptr if tri_tests is not None else None,
+                      queue.ptr if queue is not None else None)
     check(lib().crt_trace(C.byref(scene), query, flags, rays.ptr, hits.ptr, n, C.byref(cfg), _stream_handle(stream)))
 
 

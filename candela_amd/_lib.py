@@ -118,7 +118,10 @@ class WideStats(C.Structure):
 
 class TraceConfig(C.Structure):
     _fields_ = [("block_size", C.c_int32), ("lds_stack", C.c_int32), ("kernel", C.c_int32), ("variant", C.c_int32),
-                ("grid", C.c_int32), ("reserved", C.c_int32), ("tri_tests", C.c_void_p)]
+                ("grid", C.c_int32), ("reserved", C.c_int32), ("tri_tests", C.c_void_p), ("queue", C.c_void_p)]
+
+
+QUEUE_BYTES = 2048   # CRT_QUEUE_BYTES
 
 
 class Batch(C.Structure):

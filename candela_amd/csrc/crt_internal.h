@@ -38,16 +38,19 @@ int rebase_bvh(crt_bvh& b, int32_t t_offset);
 // True when the header's toolchain is this library's (CRT_SWAP_SEEDED draws match).
 bool same_toolchain(const crt_cache_header& hd);
 
-// Counters of one sharded work queue (queue.h): kQueueBlockWords u64, zeroed
-// by the launch that uses them; blocks are handed out round robin from a
-// per-device pool of kQueueSlots.
+// Counters of one persistent launch (queue.h): kQueueBlockWords u64, zeroed
+// by the launch that uses them.  queue_acquire takes config->queue when the
+// caller brings one, else leases a block from the per-device pool (refused
+// under stream capture); queue_release, called once the launch is enqueued,
+// records the event that frees a leased block.
 constexpr int kQueueBlockWords = 256;
-constexpr int kQueueSlots = 64;
-int queue_block(unsigned long long** out);
-
-// A zeroed-by-the-launch device counter from the per-device pool of the
-// persistent kernels (one slot per launch, handed out round robin).
-int counter_slot(unsigned long long** out);
+struct QueueLease {
+    unsigned long long* ptr = nullptr;
+    int dev = 0;
+    int32_t slot = -1;   // pool block, or -1 for the caller's own
+};
+int queue_acquire(const crt_trace_config* cfg, void* stream, const char* who, QueueLease& out);
+int queue_release(const QueueLease& lease, void* stream);
 
 }  // namespace crt
 

@@ -368,9 +368,10 @@ extern "C" int crt_trace_paths(const crt_scene_view* scene, const crt_path_param
     P.sky[2] = p.sky[2];
     P.albedo = p.albedo;
     P.nmesh = p.nmesh;
-    unsigned long long* counters = nullptr;
-    int rc = crt::queue_block(&counters);
+    crt::QueueLease lease;
+    int rc = crt::queue_acquire(config, stream, "crt_trace_paths", lease);
     if (rc) return rc;
+    unsigned long long* counters = lease.ptr;
     const void* fn = scene->format == CRT_FORMAT_WIDE8 ? crt::path::path_fn<8>(variant) : crt::path::path_fn<4>(variant);
     int grid = config && config->grid > 0 ? config->grid : 0;
     if (grid == 0) {
@@ -385,7 +386,10 @@ extern "C" int crt_trace_paths(const crt_scene_view* scene, const crt_path_param
     hipStream_t st = static_cast<hipStream_t>(stream);
     hipLaunchKernelGGL(crt::reset_queue, dim3(1), dim3(64), 0, st, counters);
     hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return crt::set_error(CRT_E_HIP, std::string("crt_trace_paths: reset: ") + hipGetErrorString(e));
+    if (e != hipSuccess) {
+        (void)crt::queue_release(lease, stream);
+        return crt::set_error(CRT_E_HIP, std::string("crt_trace_paths: reset: ") + hipGetErrorString(e));
+    }
     crt::SceneArgs a = crt::scene_args(*scene);
     int64_t n = npaths;
     float tmax = p.tmax;
@@ -393,6 +397,7 @@ extern "C" int crt_trace_paths(const crt_scene_view* scene, const crt_path_param
     crt_path_sample* o = out;
     void* args[] = {&a, &P, &n, &tmax, &max_iter, &counters, &o};
     e = hipLaunchKernel(fn, dim3(unsigned(grid)), dim3(256), args, 0, st);
+    rc = crt::queue_release(lease, stream);
     if (e != hipSuccess) return crt::set_error(CRT_E_HIP, std::string("crt_trace_paths: launch: ") + hipGetErrorString(e));
-    return CRT_OK;
+    return rc;
 }

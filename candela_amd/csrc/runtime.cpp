@@ -67,48 +67,94 @@ static int validate_scene(const crt_scene_view* s) {
     return CRT_OK;
 }
 
-// Device counters of the persistent kernel: a small pool handed out round
-// robin, one slot per launch, zeroed by the launch itself on its stream.
-// Launches in flight on different streams get different slots unless more
-// than kCounterSlots of them overlap.
-constexpr int kCounterSlots = 4096;
-static std::mutex g_counter_mutex;
-static unsigned long long* g_counters[64] = {};
-static std::atomic<uint32_t> g_counter_next{0};
+// Work-queue counters of the persistent kernels (queue.h).  A launch either
+// brings its own block (crt_trace_config.queue: a graph that captures the
+// launch owns its counters) or leases one from a per-device pool.  A pool
+// block goes back to the pool when the launch has been enqueued, with a HIP
+// event recorded after it; it is leased again only once that event has
+// completed, so launches in flight on any number of streams never share
+// counters.  When every block is busy the pool grows by kQueueChunk blocks,
+// up to kQueueMaxBlocks, and then waits for the block leased longest ago.
+constexpr int kQueueChunk = 64;
+constexpr int kQueueMaxBlocks = 4096;
+static_assert(size_t(kQueueBlockWords) * 8 == CRT_QUEUE_BYTES, "queue block size is the header's");
 
-int counter_slot(unsigned long long** out) {
+namespace {
+struct QueuePool {
+    std::mutex m;
+    std::vector<unsigned long long*> blocks;
+    std::vector<hipEvent_t> events;
+    std::vector<uint8_t> state;   // 0 never used, 1 leased (not yet recorded), 2 recorded
+    size_t next = 0;
+};
+QueuePool g_pools[64];
+}  // namespace
+
+int queue_acquire(const crt_trace_config* cfg, void* stream, const char* who, QueueLease& out) {
+    out = QueueLease{};
+    if (cfg && cfg->queue) {
+        if (reinterpret_cast<uintptr_t>(cfg->queue) % 256 != 0)
+            return set_error(CRT_E_INVALID, std::string(who) + ": config->queue must be 256-byte aligned");
+        out.ptr = static_cast<unsigned long long*>(cfg->queue);
+        return CRT_OK;
+    }
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    CRT_HIP(hipStreamIsCapturing(static_cast<hipStream_t>(stream), &cs), "hipStreamIsCapturing");
+    if (cs != hipStreamCaptureStatusNone)
+        return set_error(CRT_E_INVALID, std::string(who) +
+                                            ": a persistent launch under stream capture needs config->queue "
+                                            "(CRT_QUEUE_BYTES of device memory that the graph owns)");
     int dev = 0;
     CRT_HIP(hipGetDevice(&dev), "hipGetDevice");
     if (dev < 0 || dev >= 64) return set_error(CRT_E_INVALID, "device index out of range");
-    if (!g_counters[dev]) {
-        std::lock_guard<std::mutex> lock(g_counter_mutex);
-        if (!g_counters[dev]) {
-            void* p = nullptr;
-            CRT_HIP(hipMalloc(&p, kCounterSlots * sizeof(unsigned long long)), "hipMalloc(counters)");
-            g_counters[dev] = static_cast<unsigned long long*>(p);
+    QueuePool& P = g_pools[dev];
+    std::lock_guard<std::mutex> lock(P.m);
+    const size_t n = P.blocks.size();
+    for (size_t k = 0; k < n; ++k) {
+        const size_t i = (P.next + k) % n;
+        if (P.state[i] == 1) continue;
+        if (P.state[i] == 2) {
+            const hipError_t q = hipEventQuery(P.events[i]);
+            if (q == hipErrorNotReady) continue;
+            if (q != hipSuccess) return hip_fail(q, "hipEventQuery(queue block)");
         }
+        P.state[i] = 1;
+        P.next = i + 1;
+        out = QueueLease{P.blocks[i], dev, int32_t(i)};
+        return CRT_OK;
     }
-    *out = g_counters[dev] + (g_counter_next.fetch_add(1) % kCounterSlots);
+    if (n < size_t(kQueueMaxBlocks)) {
+        void* p = nullptr;
+        CRT_HIP(hipMalloc(&p, size_t(kQueueChunk) * CRT_QUEUE_BYTES), "hipMalloc(queue blocks)");
+        for (int j = 0; j < kQueueChunk; ++j) {
+            hipEvent_t e = nullptr;
+            CRT_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate(queue block)");
+            P.blocks.push_back(reinterpret_cast<unsigned long long*>(static_cast<char*>(p) + size_t(j) * CRT_QUEUE_BYTES));
+            P.events.push_back(e);
+            P.state.push_back(0);
+        }
+        P.state[n] = 1;
+        P.next = n + 1;
+        out = QueueLease{P.blocks[n], dev, int32_t(n)};
+        return CRT_OK;
+    }
+    // every block is in flight: wait for the next one in lease order
+    const size_t i = P.next % n;
+    CRT_HIP(hipEventSynchronize(P.events[i]), "hipEventSynchronize(queue block)");
+    P.state[i] = 1;
+    P.next = i + 1;
+    out = QueueLease{P.blocks[i], dev, int32_t(i)};
     return CRT_OK;
 }
 
-static unsigned long long* g_queues[64] = {};
-static std::atomic<uint32_t> g_queue_next{0};
-
-int queue_block(unsigned long long** out) {
-    int dev = 0;
-    CRT_HIP(hipGetDevice(&dev), "hipGetDevice");
-    if (dev < 0 || dev >= 64) return set_error(CRT_E_INVALID, "device index out of range");
-    if (!g_queues[dev]) {
-        std::lock_guard<std::mutex> lock(g_counter_mutex);
-        if (!g_queues[dev]) {
-            void* p = nullptr;
-            CRT_HIP(hipMalloc(&p, size_t(kQueueSlots) * kQueueBlockWords * sizeof(unsigned long long)),
-                    "hipMalloc(queues)");
-            g_queues[dev] = static_cast<unsigned long long*>(p);
-        }
-    }
-    *out = g_queues[dev] + size_t(g_queue_next.fetch_add(1) % kQueueSlots) * kQueueBlockWords;
+int queue_release(const QueueLease& lease, void* stream) {
+    if (lease.slot < 0) return CRT_OK;   // the caller's own block
+    QueuePool& P = g_pools[lease.dev];
+    std::lock_guard<std::mutex> lock(P.m);
+    const hipError_t e = hipEventRecord(P.events[size_t(lease.slot)], static_cast<hipStream_t>(stream));
+    // an unrecorded block stays leased (never reused) rather than be shared
+    if (e != hipSuccess) return hip_fail(e, "hipEventRecord(queue block)");
+    P.state[size_t(lease.slot)] = 2;
     return CRT_OK;
 }
 
@@ -162,25 +208,28 @@ int trace(const crt_scene_view* scene, int query, uint32_t flags, const crt_ray*
     if (persistent && is_wide(scene->format)) {
         if (n >= (int64_t(1) << 31))
             return set_error(CRT_E_INVALID, "crt_trace: the persistent kernel takes < 2^31 rays per launch");
-        unsigned long long* counters = nullptr;
-        rc = queue_block(&counters);
+        QueueLease q;
+        rc = queue_acquire(cfg, stream, "crt_trace", q);
         if (rc) return rc;
-        CRT_HIP(launch_wide_persistent(a, query, rays, hits, n, flags, kMaxIterations, counters, tri_tests,
-                                       cfg ? cfg->variant : 0, cfg ? cfg->grid : 0, static_cast<hipStream_t>(stream)),
-                "crt_trace: wide persistent kernel launch");
-        return CRT_OK;
+        const hipError_t e = launch_wide_persistent(a, query, rays, hits, n, flags, kMaxIterations, q.ptr, tri_tests,
+                                                    cfg ? cfg->variant : 0, cfg ? cfg->grid : 0,
+                                                    static_cast<hipStream_t>(stream));
+        rc = queue_release(q, stream);
+        CRT_HIP(e, "crt_trace: wide persistent kernel launch");
+        return rc;
     }
     if (persistent) {
         if (n >= (int64_t(1) << 31))
             return set_error(CRT_E_INVALID, "crt_trace: the persistent kernel takes < 2^31 rays per launch");
-        unsigned long long* counter = nullptr;
-        rc = counter_slot(&counter);
+        QueueLease q;
+        rc = queue_acquire(cfg, stream, "crt_trace", q);
         if (rc) return rc;
         PersistentShape shape{cfg ? cfg->variant : 0, cfg && cfg->grid ? cfg->grid : persistent_fill_grid(query)};
-        CRT_HIP(launch_persistent(a, query, rays, hits, n, flags, kMaxIterations, counter, tri_tests, shape,
-                                  static_cast<hipStream_t>(stream)),
-                "crt_trace: persistent kernel launch");
-        return CRT_OK;
+        const hipError_t e = launch_persistent(a, query, rays, hits, n, flags, kMaxIterations, q.ptr, tri_tests,
+                                               shape, static_cast<hipStream_t>(stream));
+        rc = queue_release(q, stream);
+        CRT_HIP(e, "crt_trace: persistent kernel launch");
+        return rc;
     }
     CRT_HIP(launch_trace(a, query, rays, hits, n, flags, kMaxIterations, block, depth, tri_tests,
                          static_cast<hipStream_t>(stream)),

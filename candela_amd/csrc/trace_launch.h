@@ -57,7 +57,7 @@ hipError_t launch_grid(const SceneArgs& s, int query, const crt_ray* rays, crt_h
                        int32_t max_iter, uint32_t* tri_tests, int variant, hipStream_t stream);
 
 // Refilling persistent walk of the wide layouts (variant 0..5; grid 0 = fill
-// every CU); `counters` is a queue block (queue_block) the launch resets itself.
+// every CU); `counters` is a queue block (queue_acquire) the launch resets itself.
 hipError_t launch_wide_persistent(const SceneArgs& s, int query, const crt_ray* rays, crt_hit* hits, int64_t n,
                                   uint32_t flags, int32_t max_iter, unsigned long long* counters, uint32_t* tri_tests,
                                   int variant, int grid, hipStream_t stream);

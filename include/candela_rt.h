@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define CRT_ABI_VERSION 1
+#define CRT_ABI_VERSION 2
 
 /* ------------------------------------------------------------------------- */
 /* Status codes                                                              */
@@ -384,7 +384,19 @@ typedef struct crt_trace_config {
     uint32_t* tri_tests;   /* optional device array [nrays]: the counting build of the
                               kernel writes the triangle tests of each ray (node
                               visits are in hit.status); NULL = no counting */
+    void* queue;           /* persistent kernels and crt_trace_paths: optional device
+                              memory of CRT_QUEUE_BYTES, 256-byte aligned, for the
+                              launch's work-queue counters (the launch zeroes it on
+                              its stream; launches that share one must not overlap).
+                              NULL = a block leased from the library's per-device
+                              pool, reused only after the launch that last held it
+                              has completed.  Required under stream capture: a
+                              graph owns its counters, and NULL there is
+                              CRT_E_INVALID. */
 } crt_trace_config;
+
+/* Bytes of one work-queue block (crt_trace_config.queue). */
+#define CRT_QUEUE_BYTES 2048
 
 /* IntersectScene / IntersectSceneIgnoreTransparent (closest) or the occlusion
  * IntersectScene (any) for `nrays` rays; all pointers are device pointers.

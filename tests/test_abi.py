@@ -27,7 +27,7 @@ def test_header_symbols_exported(crt):
 
 
 def test_abi_version(crt):
-    assert crt.lib().crt_abi_version() == 1
+    assert crt.lib().crt_abi_version() == 2
 
 
 def test_struct_sizes(crt):
