@@ -298,46 +298,59 @@ struct crt_context {
 
 namespace {
 
-// 4x4 inverse (column-major) by cofactor expansion in double, rounded once.
-bool invert4(const float* m, float* out) {
-    double a[16];
-    for (int i = 0; i < 16; ++i) a[i] = m[i];
-    double inv[16];
-    inv[0] = a[5] * a[10] * a[15] - a[5] * a[11] * a[14] - a[9] * a[6] * a[15] + a[9] * a[7] * a[14] +
-             a[13] * a[6] * a[11] - a[13] * a[7] * a[10];
-    inv[4] = -a[4] * a[10] * a[15] + a[4] * a[11] * a[14] + a[8] * a[6] * a[15] - a[8] * a[7] * a[14] -
-             a[12] * a[6] * a[11] + a[12] * a[7] * a[10];
-    inv[8] = a[4] * a[9] * a[15] - a[4] * a[11] * a[13] - a[8] * a[5] * a[15] + a[8] * a[7] * a[13] +
-             a[12] * a[5] * a[11] - a[12] * a[7] * a[9];
-    inv[12] = -a[4] * a[9] * a[14] + a[4] * a[10] * a[13] + a[8] * a[5] * a[14] - a[8] * a[6] * a[13] -
-              a[12] * a[5] * a[10] + a[12] * a[6] * a[9];
-    inv[1] = -a[1] * a[10] * a[15] + a[1] * a[11] * a[14] + a[9] * a[2] * a[15] - a[9] * a[3] * a[14] -
-             a[13] * a[2] * a[11] + a[13] * a[3] * a[10];
-    inv[5] = a[0] * a[10] * a[15] - a[0] * a[11] * a[14] - a[8] * a[2] * a[15] + a[8] * a[3] * a[14] +
-             a[12] * a[2] * a[11] - a[12] * a[3] * a[10];
-    inv[9] = -a[0] * a[9] * a[15] + a[0] * a[11] * a[13] + a[8] * a[1] * a[15] - a[8] * a[3] * a[13] -
-             a[12] * a[1] * a[11] + a[12] * a[3] * a[9];
-    inv[13] = a[0] * a[9] * a[14] - a[0] * a[10] * a[13] - a[8] * a[1] * a[14] + a[8] * a[2] * a[13] +
-              a[12] * a[1] * a[10] - a[12] * a[2] * a[9];
-    inv[2] = a[1] * a[6] * a[15] - a[1] * a[7] * a[14] - a[5] * a[2] * a[15] + a[5] * a[3] * a[14] +
-             a[13] * a[2] * a[7] - a[13] * a[3] * a[6];
-    inv[6] = -a[0] * a[6] * a[15] + a[0] * a[7] * a[14] + a[4] * a[2] * a[15] - a[4] * a[3] * a[14] -
-             a[12] * a[2] * a[7] + a[12] * a[3] * a[6];
-    inv[10] = a[0] * a[5] * a[15] - a[0] * a[7] * a[13] - a[4] * a[1] * a[15] + a[4] * a[3] * a[13] +
-              a[12] * a[1] * a[7] - a[12] * a[3] * a[5];
-    inv[14] = -a[0] * a[5] * a[14] + a[0] * a[6] * a[13] + a[4] * a[1] * a[14] - a[4] * a[2] * a[13] -
-              a[12] * a[1] * a[6] + a[12] * a[2] * a[5];
-    inv[3] = -a[1] * a[6] * a[11] + a[1] * a[7] * a[10] + a[5] * a[2] * a[11] - a[5] * a[3] * a[10] -
-             a[9] * a[2] * a[7] + a[9] * a[3] * a[6];
-    inv[7] = a[0] * a[6] * a[11] - a[0] * a[7] * a[10] - a[4] * a[2] * a[11] + a[4] * a[3] * a[10] +
-             a[8] * a[2] * a[7] - a[8] * a[3] * a[6];
-    inv[11] = -a[0] * a[5] * a[11] + a[0] * a[7] * a[9] + a[4] * a[1] * a[11] - a[4] * a[3] * a[9] -
-              a[8] * a[1] * a[7] + a[8] * a[3] * a[5];
-    inv[15] = a[0] * a[5] * a[10] - a[0] * a[6] * a[9] - a[4] * a[1] * a[10] + a[4] * a[2] * a[9] +
-              a[8] * a[1] * a[6] - a[8] * a[2] * a[5];
-    const double det = a[0] * inv[0] + a[1] * inv[4] + a[2] * inv[8] + a[3] * inv[12];
-    if (det == 0.0 || !std::isfinite(det)) return false;
-    for (int i = 0; i < 16; ++i) out[i] = float(inv[i] / det);
+// glm::inverse of a float mat4, as PushEntity stores it (Intersector.h:209):
+// the steps of glm 0.9.8.5's generic compute_inverse<tmat4x4>
+// (glm/detail/func_matrix.inl:297-351) in float, in the same order, so the
+// inverse is byte-identical to the reference's.  The 2x2 minors of the lower
+// rows (coef), the cofactor columns (inv_k = v1*f0 - v2*f1 + v3*f2 and the
+// like, signs alternating), the determinant as the first column of the input
+// against the first row of the adjugate summed pairwise, then one reciprocal
+// multiplied into every entry.  `a` and `out` are column-major (a[c * 4 + r]).
+// The reference pushes the inf / NaN entries of a singular matrix; this
+// returns false instead.
+bool invert4(const float* a, float* out) {
+    auto m = [a](int c, int r) { return a[c * 4 + r]; };
+    const float c00 = m(2, 2) * m(3, 3) - m(3, 2) * m(2, 3);
+    const float c02 = m(1, 2) * m(3, 3) - m(3, 2) * m(1, 3);
+    const float c03 = m(1, 2) * m(2, 3) - m(2, 2) * m(1, 3);
+    const float c04 = m(2, 1) * m(3, 3) - m(3, 1) * m(2, 3);
+    const float c06 = m(1, 1) * m(3, 3) - m(3, 1) * m(1, 3);
+    const float c07 = m(1, 1) * m(2, 3) - m(2, 1) * m(1, 3);
+    const float c08 = m(2, 1) * m(3, 2) - m(3, 1) * m(2, 2);
+    const float c10 = m(1, 1) * m(3, 2) - m(3, 1) * m(1, 2);
+    const float c11 = m(1, 1) * m(2, 2) - m(2, 1) * m(1, 2);
+    const float c12 = m(2, 0) * m(3, 3) - m(3, 0) * m(2, 3);
+    const float c14 = m(1, 0) * m(3, 3) - m(3, 0) * m(1, 3);
+    const float c15 = m(1, 0) * m(2, 3) - m(2, 0) * m(1, 3);
+    const float c16 = m(2, 0) * m(3, 2) - m(3, 0) * m(2, 2);
+    const float c18 = m(1, 0) * m(3, 2) - m(3, 0) * m(1, 2);
+    const float c19 = m(1, 0) * m(2, 2) - m(2, 0) * m(1, 2);
+    const float c20 = m(2, 0) * m(3, 1) - m(3, 0) * m(2, 1);
+    const float c22 = m(1, 0) * m(3, 1) - m(3, 0) * m(1, 1);
+    const float c23 = m(1, 0) * m(2, 1) - m(2, 0) * m(1, 1);
+    const float f0[4] = {c00, c00, c02, c03}, f1[4] = {c04, c04, c06, c07}, f2[4] = {c08, c08, c10, c11};
+    const float f3[4] = {c12, c12, c14, c15}, f4[4] = {c16, c16, c18, c19}, f5[4] = {c20, c20, c22, c23};
+    float v[4][4];   // v[r] = (m[1][r], m[0][r], m[0][r], m[0][r])
+    for (int r = 0; r < 4; ++r) {
+        v[r][0] = m(1, r);
+        v[r][1] = v[r][2] = v[r][3] = m(0, r);
+    }
+    float inv[4][4];   // adjugate columns
+    for (int i = 0; i < 4; ++i) {
+        const float sa = (i % 2 == 0) ? 1.0f : -1.0f;   // (+1, -1, +1, -1)
+        const float sb = -sa;                             // (-1, +1, -1, +1)
+        inv[0][i] = (v[1][i] * f0[i] - v[2][i] * f1[i] + v[3][i] * f2[i]) * sa;
+        inv[1][i] = (v[0][i] * f0[i] - v[2][i] * f3[i] + v[3][i] * f4[i]) * sb;
+        inv[2][i] = (v[0][i] * f1[i] - v[1][i] * f3[i] + v[3][i] * f5[i]) * sa;
+        inv[3][i] = (v[0][i] * f2[i] - v[1][i] * f4[i] + v[2][i] * f5[i]) * sb;
+    }
+    float d[4];
+    for (int c = 0; c < 4; ++c) d[c] = m(0, c) * inv[c][0];
+    const float det = (d[0] + d[1]) + (d[2] + d[3]);
+    if (det == 0.0f || !std::isfinite(det)) return false;
+    const float one = 1.0f / det;
+    for (int c = 0; c < 4; ++c)
+        for (int r = 0; r < 4; ++r) out[c * 4 + r] = inv[c][r] * one;
     return true;
 }
 
@@ -628,15 +641,26 @@ int crt_push_entity(crt_context* ctx, int32_t object_id, const float model[16], 
         return crt::set_error(CRT_E_INVALID,
                               "Trying to push entity whose parent object hasn't been added to global BVH");
     crt_entity e;
+    const int rc = crt_make_entity(model, emissive, translucency, int32_t(it->second.node_offset),
+                                   int32_t(it->second.node_count), &e);
+    if (rc) return rc;
+    ctx->pending.push_back(e);
+    return CRT_OK;
+}
+
+int crt_make_entity(const float model[16], float emissive, float translucency, int32_t node_offset,
+                    int32_t node_count, crt_entity* out) {
+    if (!model || !out) return crt::set_error(CRT_E_INVALID, "crt_make_entity: null argument");
+    crt_entity e;
     std::memset(&e, 0, sizeof(e));
     std::memcpy(e.model, model, sizeof(e.model));
-    if (!invert4(model, e.inverse)) return crt::set_error(CRT_E_INVALID, "crt_push_entity: singular model matrix");
-    e.node_offset = int32_t(it->second.node_offset);
-    e.node_count = int32_t(it->second.node_count);
+    if (!invert4(model, e.inverse)) return crt::set_error(CRT_E_INVALID, "crt_make_entity: singular model matrix");
+    e.node_offset = node_offset;
+    e.node_count = node_count;
     const float alpha = 1.0f - translucency;
     std::memcpy(&e.data[0], &emissive, 4);
     std::memcpy(&e.data[1], &alpha, 4);
-    ctx->pending.push_back(e);
+    *out = e;
     return CRT_OK;
 }
 

@@ -552,9 +552,19 @@ int crt_add_object_prebuilt(crt_context* ctx, int32_t object_id, const void* nod
 int crt_add_object_cached(crt_context* ctx, int32_t object_id, const crt_mesh* meshes, int mesh_count,
                           const crt_build_options* options, const char* cache_dir, int32_t* cache_hit);
 /* PushEntity (Intersector.h:200-216): model matrix (column-major), emissive,
- * translucency; inverse is computed here. */
+ * translucency; the record is crt_make_entity's with the object's node
+ * offset and count.  A singular model is CRT_E_INVALID (the reference pushes
+ * its inf / NaN inverse). */
 int crt_push_entity(crt_context* ctx, int32_t object_id, const float model[16], float emissive,
                     float translucency);
+/* The BVHEntity record PushEntity builds (Intersector.h:207-213), for callers
+ * that assemble their own entity buffers for crt_trace: model, inverse =
+ * glm::inverse(model) computed with glm 0.9.8.5's float steps (byte-identical
+ * to the reference's), node offset and count, data[0] = float bits of
+ * emissive, data[1] = float bits of 1 - translucency, the rest zero.
+ * CRT_E_INVALID for a singular model. */
+int crt_make_entity(const float model[16], float emissive, float translucency, int32_t node_offset,
+                    int32_t node_count, crt_entity* out);
 /* BufferEntities (Intersector.h:226-239) */
 int crt_buffer_entities(crt_context* ctx);
 /* BufferData (Intersector.h:321-351) */
