@@ -631,6 +631,7 @@ def main(argv=None) -> int:
                            for k in (crt.KERNEL_WAVE, crt.KERNEL_PERSISTENT) for v in range(10)]
         if not args.no_stack:   # the reference's other layout over the same meshes
             candidates += [stack(crt.KERNEL_THREAD_PER_RAY, 0, lds) for lds in (16, 32)]
+            candidates += [stack(k, v, 0) for k in (crt.KERNEL_WAVE, crt.KERNEL_PERSISTENT) for v in range(8)]
     if args.only:
         main_part, _, ao_part = args.only.partition("+ao:")
         parts = main_part.split(":")

@@ -172,10 +172,9 @@ int trace(const crt_scene_view* scene, int query, uint32_t flags, const crt_ray*
     const int kernel = cfg ? cfg->kernel : CRT_KERNEL_AUTO;
     if (kernel < CRT_KERNEL_AUTO || kernel > CRT_KERNEL_WAVE)
         return set_error(CRT_E_INVALID, "crt_trace: unknown kernel");
-    if (kernel == CRT_KERNEL_PERSISTENT && scene->format == CRT_FORMAT_STACK)
-        return set_error(CRT_E_INVALID, "crt_trace: the persistent kernels walk the stackless and wide formats");
-    if (kernel == CRT_KERNEL_WAVE && scene->format == CRT_FORMAT_STACK)
-        return set_error(CRT_E_INVALID, "crt_trace: the wave kernels walk the stackless and wide formats");
+    if ((kernel == CRT_KERNEL_PERSISTENT || kernel == CRT_KERNEL_WAVE) && scene->format == CRT_FORMAT_STACK && cfg &&
+        cfg->variant > 7)
+        return set_error(CRT_E_INVALID, "crt_trace: the stack-layout wave kernels take variant 0..7");
     if (is_wide(scene->format) && (depth != 8 && depth != 16 && depth != 32))
         return set_error(CRT_E_INVALID, "crt_trace: wide formats take lds_stack 8, 16 or 32");
     if (is_wide(scene->format) && size_t(2 * depth + 8) * size_t(block) * 4 > 64 * 1024)
@@ -189,6 +188,27 @@ int trace(const crt_scene_view* scene, int query, uint32_t flags, const crt_ray*
     const bool wave = kernel == CRT_KERNEL_WAVE ||
                       (kernel == CRT_KERNEL_AUTO && (scene->format == CRT_FORMAT_STACKLESS || is_wide(scene->format)));
     if (n == 0) return CRT_OK;
+    if (wave && scene->format == CRT_FORMAT_STACK) {
+        if (n >= (int64_t(1) << 31) * 256 - 2048)
+            return set_error(CRT_E_INVALID, "crt_trace: too many rays for one launch");
+        CRT_HIP(launch_stack_grid(a, query, rays, hits, n, flags, kMaxIterations, tri_tests, cfg ? cfg->variant : 0,
+                                  static_cast<hipStream_t>(stream)),
+                "crt_trace: stack wave kernel launch");
+        return CRT_OK;
+    }
+    if (persistent && scene->format == CRT_FORMAT_STACK) {
+        if (n >= (int64_t(1) << 31))
+            return set_error(CRT_E_INVALID, "crt_trace: the persistent kernel takes < 2^31 rays per launch");
+        QueueLease q;
+        rc = queue_acquire(cfg, stream, "crt_trace", q);
+        if (rc) return rc;
+        const hipError_t e = launch_stack_persistent(a, query, rays, hits, n, flags, kMaxIterations, q.ptr, tri_tests,
+                                                     cfg ? cfg->variant : 0, cfg ? cfg->grid : 0,
+                                                     static_cast<hipStream_t>(stream));
+        rc = queue_release(q, stream);
+        CRT_HIP(e, "crt_trace: stack persistent kernel launch");
+        return rc;
+    }
     if (wave && is_wide(scene->format)) {
         if (n >= (int64_t(1) << 31) * 256 - 2048)
             return set_error(CRT_E_INVALID, "crt_trace: too many rays for one launch");

@@ -66,6 +66,16 @@ hipError_t launch_wide_persistent(const SceneArgs& s, int query, const crt_ray* 
 hipError_t launch_wide_grid(const SceneArgs& s, int query, const crt_ray* rays, crt_hit* hits, int64_t n,
                             uint32_t flags, int32_t max_iter, uint32_t* tri_tests, int variant, hipStream_t stream);
 
+// Wave-scheduled walk of the stack layout (variant 0..7).
+hipError_t launch_stack_grid(const SceneArgs& s, int query, const crt_ray* rays, crt_hit* hits, int64_t n,
+                             uint32_t flags, int32_t max_iter, uint32_t* tri_tests, int variant, hipStream_t stream);
+
+// Refilling persistent walk of the stack layout (variant 0..7; grid 0 = fill
+// every CU); `counters` is a queue block the launch resets itself.
+hipError_t launch_stack_persistent(const SceneArgs& s, int query, const crt_ray* rays, crt_hit* hits, int64_t n,
+                                   uint32_t flags, int32_t max_iter, unsigned long long* counters,
+                                   uint32_t* tri_tests, int variant, int grid, hipStream_t stream);
+
 // Workgroups of 256 threads that fill every CU with the persistent kernel.
 int persistent_fill_grid(int query);
 

@@ -67,9 +67,9 @@ def gpu_trace(crt, scene, query, rays, flags=0, block=0, lds=0, kernel=0, varian
     return dh.download()
 
 
-# (format, kernel) pairs: the one-ray-per-lane kernel walks both formats, the
-# persistent refill kernel the stackless one.
-WALKERS = [(0, 1), (0, 2), (0, 3), (1, 1)]
+# (format, kernel) pairs: the one-ray-per-lane, persistent and wave kernels
+# over both layouts of the reference.
+WALKERS = [(0, 1), (0, 2), (0, 3), (1, 1), (1, 2), (1, 3)]
 
 
 @pytest.mark.parametrize("fmt", [0, 1])
@@ -182,7 +182,7 @@ def test_guards_on_gpu(gpu, orc):
     for fmt, nodes, flag in cases:
         ents = oracle.identity_entity(nodes.size)
         s, keep = device_scene(gpu, fmt, nodes, tris, v, ents)
-        for kernel, lds in (((1, 0), (2, 0), (3, 0)) if fmt == 0 else ((1, 8), (1, 16), (1, 64))):
+        for kernel, lds in (((1, 0), (2, 0), (3, 0)) if fmt == 0 else ((1, 8), (1, 16), (1, 64), (2, 0), (3, 0))):
             gh = gpu_trace(gpu, s, 0, ray, lds=lds, kernel=kernel)
             oh = orc.trace(fmt, nodes, tris, v, ents, 0, ray)
             assert (oh["status"] & 0xFF000000 == flag).all()
