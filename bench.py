@@ -86,8 +86,9 @@ def _kernel_name(layout: str, kernel: int, variant: int) -> str:
                 2: "stack_persistent (lane refill), preset %d" % variant}.get(kernel, "trace_kernel")
     if layout == "bvh2":
         return "%s, preset %d" % (KERNEL_NAMES[kernel], variant)
-    return {3: "wide_grid, preset %d" % variant, 2: "wide_persistent (lane refill), preset %d" % variant}.get(
-        kernel, "trace_kernel")
+    return {3: "wide_grid, preset %d" % variant, 2: "wide_persistent (lane refill), preset %d" % variant,
+            4: "spec_grid (speculative, wave-spread leaf tests), preset %d" % variant,
+            5: "spec_persistent (speculative, lane refill), preset %d" % variant}.get(kernel, "trace_kernel")
 
 
 def kernel_label(c: dict, gi_only: bool = False) -> str:
@@ -237,6 +238,16 @@ def batched_vs_plain(crt, args, plain_choice: dict, stream) -> dict:
         log(f"{name}: plain {2 * len(work.scenes)} launches {plain_ms:.3f} ms ({work.rays / plain_ms / 1e3:.1f} "
             f"Mrays/s), batched one launch {b['ms']:.3f} ms ({work.rays / b['ms'] / 1e3:.1f} Mrays/s) -> "
             f"{plain_ms / b['ms']:.2f}x; identical hits: {same}")
+    # the library's own node-width choice: crt_autotune_batched_layouts over every layout built above
+    keys = list(plain_choice.keys())
+    t0 = time.perf_counter()
+    idx, cfg, ms = crt.autotune_batched_layouts([work.batches(crt, k) for k in keys], default_candidates(crt, True),
+                                                repeats=2, stream=stream)
+    lib_choice = {"layout": "bvh2" if keys[idx] is None else f"wide{keys[idx][0]}", "ms": round(ms, 4),
+                  "config": {k: getattr(cfg, k) for k in ("variant", "tiles_per_claim", "waves_per_group", "order",
+                                                          "shards", "child_order")},
+                  "tune_s": round(time.perf_counter() - t0, 1)}
+    log(f"library layout autotuner: {lib_choice}")
     best_plain = min(per_layout.items(), key=lambda kv: kv[1]["plain_ms"])
     best_batched = min(per_layout.items(), key=lambda kv: kv[1]["batched_ms"])
     return {"workload": f"{len(work.scenes)} icosphere clouds, {work.triangles} triangles, {work.rays} rays "
@@ -249,6 +260,7 @@ def batched_vs_plain(crt, args, plain_choice: dict, stream) -> dict:
             "speedup": round(best_plain[1]["plain_ms"] / best_batched[1]["batched_ms"], 3),
             "speedup_same_layout": {k: v["speedup"] for k, v in per_layout.items()},
             "hits_identical": all(v["hits_identical"] for v in per_layout.values()),
+            "library_autotuner": lib_choice,
             "layouts": per_layout}
 
 
@@ -629,6 +641,8 @@ def main(argv=None) -> int:
             candidates += [wide(w, 2, crt.KERNEL_THREAD_PER_RAY) for w in (4, 8)]
             candidates += [wide(w, leaf, k, v) for w in (4, 8) for leaf in (2, 4)
                            for k in (crt.KERNEL_WAVE, crt.KERNEL_PERSISTENT) for v in range(10)]
+            candidates += [wide(w, 2, k, v) for w in (4, 8)
+                           for k in (crt.KERNEL_SPECULATIVE, crt.KERNEL_SPECULATIVE_REFILL) for v in range(6)]
         if not args.no_stack:   # the reference's other layout over the same meshes
             candidates += [stack(crt.KERNEL_THREAD_PER_RAY, 0, lds) for lds in (16, 32)]
             candidates += [stack(k, v, 0) for k in (crt.KERNEL_WAVE, crt.KERNEL_PERSISTENT) for v in range(8)]
@@ -897,12 +911,30 @@ def main(argv=None) -> int:
                                   1, ao, threads=threads)
         else:   # the host twin of the wide walk, on the same wide layout
             w = wides[(best["width"], best["max_leaf"])].host
-            twin_g = oracle.trace_wide(w.width, w.groups, w.records, w.entities, 0, gi, threads=threads)
-            twin_a = oracle.trace_wide(w.width, w.groups, w.records, w.entities, 1, ao, threads=threads,
-                                       flags=ao_flags(crt, best))
+
+            def wide_twin(q, rays_, kernel, variant, flags):
+                # the speculative grid kernel's twin replays its wave schedule; the refilling one has no fixed
+                # schedule, so it is held to the sequential walk's nearest t and occlusion below
+                if kernel == crt.KERNEL_SPECULATIVE:
+                    return oracle.trace_wide_spec(w.width, w.groups, w.records, w.entities, q, rays_,
+                                                  variant=variant, flags=flags, threads=threads)
+                return oracle.trace_wide(w.width, w.groups, w.records, w.entities, q, rays_, threads=threads,
+                                         flags=flags)
+
+            twin_g = wide_twin(0, gi, best["kernel"], best["variant"], 0)
+            twin_a = wide_twin(1, ao, *ao_of(best), ao_flags(crt, best))
+        schedule_free = {crt.KERNEL_SPECULATIVE_REFILL}
+        gi_exact = best["kernel"] not in schedule_free or best["layout"] in ("bvh2", "stack")
+        ao_exact = ao_of(best)[0] not in schedule_free or best["layout"] in ("bvh2", "stack")
         gi_miss = g["tri"] != ohg["tri"]
         parity = {"rays": int(g.size + a.size),
-                  "exact": float((np.concatenate([g, a]).tobytes() == np.concatenate([twin_g, twin_a]).tobytes())),
+                  # bit-identical records against the twin of each launch's kernel (kernels with a fixed schedule)
+                  "exact": float((np.concatenate([g, a]).tobytes() == np.concatenate([twin_g, twin_a]).tobytes()))
+                  if gi_exact and ao_exact else None,
+                  "gi_exact_vs_twin": bool(g.tobytes() == twin_g.tobytes()) if gi_exact else None,
+                  "ao_exact_vs_twin": bool(a.tobytes() == twin_a.tobytes()) if ao_exact else None,
+                  "gi_t_equal_to_twin": float(np.mean(g["t"] == twin_g["t"])),
+                  "ao_occlusion_equal_to_twin": float(np.mean((a["t"] > 0) == (twin_a["t"] > 0))),
                   # closest hits: the triangle is unique up to exact distance ties (shared edges and vertices)
                   "gi_id_agreement_vs_reference_walk": float(1.0 - gi_miss.mean()),
                   "gi_id_mismatches": int(gi_miss.sum()),
@@ -983,7 +1015,7 @@ def main(argv=None) -> int:
         "batched_mrays_s": (round(rays_step / (c2_batched["ms"] * 1e-3) / 1e6, 2) if c2_batched else None),
         "batched_vs_plain": pick(batched, ("plain_launches", "plain_ms", "plain_layout", "batched_ms",
                                            "batched_layout", "batched_mrays_s", "speedup", "speedup_same_layout",
-                                           "hits_identical")),
+                                           "hits_identical", "library_autotuner")),
         "roofline": roofline,
         "cpu_baseline": cpu,
         "parity": parity,

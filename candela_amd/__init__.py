@@ -13,7 +13,8 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from ._lib import (KERNEL_AUTO, KERNEL_PERSISTENT, KERNEL_THREAD_PER_RAY, KERNEL_WAVE, BOUNDS_DTYPE, ENTITY_DTYPE, FLAG_IGNORE_TRANSPARENT, FLAG_TRI0_IS_MISS, FLAG_SLOT_ORDER, FORMAT_STACK,
+from ._lib import (KERNEL_AUTO, KERNEL_PERSISTENT, KERNEL_THREAD_PER_RAY, KERNEL_WAVE, KERNEL_SPECULATIVE,
+                   KERNEL_SPECULATIVE_REFILL, BOUNDS_DTYPE, ENTITY_DTYPE, FLAG_IGNORE_TRANSPARENT, FLAG_TRI0_IS_MISS, FLAG_SLOT_ORDER, FORMAT_STACK,
                    FORMAT_STACKLESS, FORMAT_WIDE4, FORMAT_WIDE8, WIDE_GROUP_DTYPE, TRI_RECORD_DTYPE, WideStats,
                    TEXTURE_REF_DTYPE, TEXTURE_DTYPE, HIT_ATTR_DTYPE, PATH_PARAMS_DTYPE, PATH_SAMPLE_DTYPE,
                    HIT_DTYPE, QUERY_ANY, QUERY_CLOSEST, RAY_DTYPE, STACK_NODE_DTYPE,
@@ -38,7 +39,7 @@ __all__ = [
     "CacheHeader", "E_IO", "E_CORRUPT", "E_STALE", "RAYGEN_PARAMS_DTYPE", "PROBE_GRID_DTYPE", "raygen_params",
     "RayGen", "probe_grid", "gen_probe_rays", "GpuBVH", "Graph", "WIDE_GPU_BAD_LINK", "WIDE_GPU_NOT_TREE",
     "WIDE_GPU_BAD_VERTEX", "WIDE_GPU_TOO_DEEP", "WIDE_GPU_BAD_LEAF", "WIDE_GPU_MAX_LEVELS", "QUEUE_BYTES",
-    "queue_block",
+    "queue_block", "autotune_batched_layouts", "KERNEL_SPECULATIVE", "KERNEL_SPECULATIVE_REFILL",
 ]
 
 
@@ -720,6 +721,23 @@ def autotune_batched(batches, candidates=None, repeats: int = 2, stream=None):
         check(lib().crt_autotune_batched(arr, len(batches), None, 0, repeats, _stream_handle(stream), C.byref(best),
                                          C.byref(ms)))
     return best, float(ms.value)
+
+
+def autotune_batched_layouts(layouts, candidates=None, repeats: int = 2, stream=None):
+    """crt_autotune_batched_layouts: `layouts` is a list of batch lists, the same batches over different node
+    layouts (BVH2 / BVH4 / BVH8 scene views); returns (index of the fastest layout, its BatchConfig, ms)."""
+    n = len(layouts[0])
+    if any(len(b) != n for b in layouts):
+        raise ValueError("every layout needs the same number of batches")
+    arr = _batch_array([b for batches in layouts for b in batches])
+    best = BatchConfig()
+    ms = C.c_float()
+    layout = C.c_int32()
+    carr = (BatchConfig * len(candidates))(*candidates) if candidates else None
+    check(lib().crt_autotune_batched_layouts(arr, n, len(layouts), carr, len(candidates) if candidates else 0,
+                                             repeats, _stream_handle(stream), C.byref(layout), C.byref(best),
+                                             C.byref(ms)))
+    return int(layout.value), best, float(ms.value)
 
 
 class ShadingModel:

@@ -35,6 +35,7 @@ KERNEL_AUTO = 0
 KERNEL_THREAD_PER_RAY = 1
 KERNEL_PERSISTENT = 2
 KERNEL_WAVE = 3
+KERNEL_SPECULATIVE, KERNEL_SPECULATIVE_REFILL = 4, 5
 WIDE_GPU_BAD_LINK, WIDE_GPU_NOT_TREE, WIDE_GPU_BAD_VERTEX, WIDE_GPU_TOO_DEEP, WIDE_GPU_BAD_LEAF = 1, 2, 4, 8, 16
 WIDE_GPU_MAX_LEVELS = 4096
 
@@ -238,6 +239,9 @@ SIGNATURES = {
                                      C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "crt_autotune_batched": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_void_p,
                                        C.POINTER(BatchConfig), C.POINTER(C.c_float)]),
+    "crt_autotune_batched_layouts": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_int,
+                                               C.c_void_p, C.POINTER(C.c_int32), C.POINTER(BatchConfig),
+                                               C.POINTER(C.c_float)]),
     "crt_wide_build": (C.c_int, [C.c_int, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64,
                                  C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_void_p)]),
     "crt_wide_free": (None, [C.c_void_p]),

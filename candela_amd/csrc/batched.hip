@@ -21,6 +21,7 @@
 #include <cstring>
 #include <new>
 #include <numeric>
+#include <string>
 #include <vector>
 
 #include "candela_rt.h"
@@ -558,6 +559,41 @@ int crt_autotune_batched(const crt_batch* batches, int nbatches, const crt_batch
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (rc) return rc;
+    *best_ms = best_t;
+    return CRT_OK;
+}
+
+int crt_autotune_batched_layouts(const crt_batch* batches, int nbatches, int nlayouts,
+                                 const crt_batch_config* candidates, int ncandidates, int repeats, void* stream,
+                                 int32_t* best_layout, crt_batch_config* best, float* best_ms) {
+    if (!batches || !best_layout || !best || !best_ms)
+        return crt::set_error(CRT_E_INVALID, "crt_autotune_batched_layouts: null argument");
+    if (nbatches < 1 || nlayouts < 1)
+        return crt::set_error(CRT_E_INVALID, "crt_autotune_batched_layouts: needs batches and layouts");
+    // every layout must hold the same work: same queries, flags, rays, hits and counts, batch by batch
+    for (int l = 1; l < nlayouts; ++l)
+        for (int i = 0; i < nbatches; ++i) {
+            const crt_batch& a = batches[i];
+            const crt_batch& b = batches[int64_t(l) * nbatches + i];
+            if (a.query != b.query || a.flags != b.flags || a.rays != b.rays || a.hits != b.hits ||
+                a.count != b.count || a.tri_tests != b.tri_tests)
+                return crt::set_error(CRT_E_INVALID, "crt_autotune_batched_layouts: layout " + std::to_string(l) +
+                                                         " batch " + std::to_string(i) +
+                                                         " differs from layout 0 in more than its scene");
+        }
+    float best_t = -1.0f;
+    for (int l = 0; l < nlayouts; ++l) {
+        crt_batch_config c{};
+        float ms = 0.0f;
+        const int rc = crt_autotune_batched(batches + int64_t(l) * nbatches, nbatches, candidates, ncandidates,
+                                            repeats, stream, &c, &ms);
+        if (rc) return rc;
+        if (best_t < 0.0f || ms < best_t) {
+            best_t = ms;
+            *best = c;
+            *best_layout = l;
+        }
+    }
     *best_ms = best_t;
     return CRT_OK;
 }

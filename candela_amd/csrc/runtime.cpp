@@ -170,8 +170,13 @@ int trace(const crt_scene_view* scene, int query, uint32_t flags, const crt_ray*
     rc = validate_trace_config(cfg, block, depth);
     if (rc) return rc;
     const int kernel = cfg ? cfg->kernel : CRT_KERNEL_AUTO;
-    if (kernel < CRT_KERNEL_AUTO || kernel > CRT_KERNEL_WAVE)
+    if (kernel < CRT_KERNEL_AUTO || kernel > CRT_KERNEL_SPECULATIVE_REFILL)
         return set_error(CRT_E_INVALID, "crt_trace: unknown kernel");
+    const bool speculative = kernel == CRT_KERNEL_SPECULATIVE || kernel == CRT_KERNEL_SPECULATIVE_REFILL;
+    if (speculative && !is_wide(scene->format))
+        return set_error(CRT_E_INVALID, "crt_trace: the speculative kernels walk the wide formats");
+    if (speculative && cfg && cfg->variant > 5)
+        return set_error(CRT_E_INVALID, "crt_trace: the speculative kernels take variant 0..5");
     if ((kernel == CRT_KERNEL_PERSISTENT || kernel == CRT_KERNEL_WAVE) && scene->format == CRT_FORMAT_STACK && cfg &&
         cfg->variant > 7)
         return set_error(CRT_E_INVALID, "crt_trace: the stack-layout wave kernels take variant 0..7");
@@ -188,6 +193,27 @@ int trace(const crt_scene_view* scene, int query, uint32_t flags, const crt_ray*
     const bool wave = kernel == CRT_KERNEL_WAVE ||
                       (kernel == CRT_KERNEL_AUTO && (scene->format == CRT_FORMAT_STACKLESS || is_wide(scene->format)));
     if (n == 0) return CRT_OK;
+    if (kernel == CRT_KERNEL_SPECULATIVE) {
+        if (n >= (int64_t(1) << 31) * 256 - 2048)
+            return set_error(CRT_E_INVALID, "crt_trace: too many rays for one launch");
+        CRT_HIP(launch_spec_grid(a, query, rays, hits, n, flags, kMaxIterations, tri_tests, cfg ? cfg->variant : 0,
+                                 static_cast<hipStream_t>(stream)),
+                "crt_trace: speculative kernel launch");
+        return CRT_OK;
+    }
+    if (kernel == CRT_KERNEL_SPECULATIVE_REFILL) {
+        if (n >= (int64_t(1) << 31))
+            return set_error(CRT_E_INVALID, "crt_trace: the persistent kernel takes < 2^31 rays per launch");
+        QueueLease q;
+        rc = queue_acquire(cfg, stream, "crt_trace", q);
+        if (rc) return rc;
+        const hipError_t e = launch_spec_persistent(a, query, rays, hits, n, flags, kMaxIterations, q.ptr, tri_tests,
+                                                    cfg ? cfg->variant : 0, cfg ? cfg->grid : 0,
+                                                    static_cast<hipStream_t>(stream));
+        rc = queue_release(q, stream);
+        CRT_HIP(e, "crt_trace: speculative persistent kernel launch");
+        return rc;
+    }
     if (wave && scene->format == CRT_FORMAT_STACK) {
         if (n >= (int64_t(1) << 31) * 256 - 2048)
             return set_error(CRT_E_INVALID, "crt_trace: too many rays for one launch");

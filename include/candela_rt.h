@@ -364,9 +364,22 @@ enum {
                                    stackless, 64 at a time (leaf batching + node
                                    window); wide formats, one per idle lane whenever
                                    enough lanes are idle                            */
-    CRT_KERNEL_WAVE = 3         /* one ray per lane, leaf batching, XCD-aware block
+    CRT_KERNEL_WAVE = 3,        /* one ray per lane, leaf batching, XCD-aware block
                                    order (stackless: + node window; wide formats:
                                    leaves of a node parked and tested by the wave) */
+    CRT_KERNEL_SPECULATIVE = 4, /* wide formats only: one ray per lane, lanes never
+                                   wait for leaf tests; the wave tests the queued
+                                   triangles 64 at a time between node steps, with
+                                   culling by the tmax of the last such phase.
+                                   Grid launch, 64 consecutive rays per wave: the
+                                   records are deterministic; nearest t and hit /
+                                   miss equal the other wide walks', the triangle
+                                   up to exact distance ties; visit and test
+                                   counts depend on the wave's schedule (variant 0..5) */
+    CRT_KERNEL_SPECULATIVE_REFILL = 5 /* the same walk in device-filling waves that
+                                   refill idle lanes from a work queue (uses
+                                   config->queue like CRT_KERNEL_PERSISTENT); counts
+                                   vary from run to run (variant 0..5) */
 };
 
 typedef struct crt_trace_config {
@@ -804,6 +817,17 @@ int64_t crt_batch_plan_rays(const crt_batch_plan* plan);
  * hold any-hit queries. */
 int crt_autotune_batched(const crt_batch* batches, int nbatches, const crt_batch_config* candidates,
                          int ncandidates, int repeats, void* stream, crt_batch_config* best, float* best_ms);
+
+/* The node-width axis of the autotuner: `batches` holds nlayouts consecutive
+ * copies of the same nbatches batches, copy l over one layout of the same
+ * scenes (say the BVH2, its BVH4 and its BVH8 collapse: same queries, flags,
+ * rays, hits and counts, only the scene views differ; CRT_E_INVALID
+ * otherwise).  Each layout is tuned as crt_autotune_batched tunes it
+ * (candidates == NULL: its built-in grid); *best_layout receives the index of
+ * the fastest layout, *best and *best_ms its configuration and time. */
+int crt_autotune_batched_layouts(const crt_batch* batches, int nbatches, int nlayouts,
+                                 const crt_batch_config* candidates, int ncandidates, int repeats, void* stream,
+                                 int32_t* best_layout, crt_batch_config* best, float* best_ms);
 
 /* ------------------------------------------------------------------------- */
 /* Differentiable shading: Cook-Torrance direct light + one diffuse bounce    */

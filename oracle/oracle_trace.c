@@ -39,6 +39,7 @@
  * walks (tests/test_wide.py), and to check the GPU wide walk bit for bit.
  */
 #include <math.h>
+#include <stddef.h>
 #include <pthread.h>
 #include <stdint.h>
 #include <stdlib.h>
@@ -778,4 +779,362 @@ void oracle_primary_rays(int width, int height, const float* inv_view, const flo
             r->dir[2] = d.z * il;
             r->id = (uint32_t)((int64_t)y * width + x);
         }
+}
+
+/* ---------------------------------------------------------------------------
+ * Host twin of the speculative wave walk (libcandela_rt's spec_walk.h, grid
+ * launch: rays [64 w, 64 w + 64) share wave w).  The wave's schedule is
+ * replayed exactly, so hit records, visits and tests equal the kernel's:
+ *   round      up to `inner` node-step iterations; in each, every active lane
+ *              that is neither done nor parked takes one step, then the
+ *              wave appends the waiting leaves of all lanes in lane order
+ *              (a lane whose triangles do not fit the list stays parked); the
+ *              iterations stop early when no lane can step or `trigger` items
+ *              wait; then, with items waiting, the leaf phase and a second
+ *              append; then every lane whose walk is over, with nothing
+ *              parked or waiting, goes to its next entity or writes its hit;
+ *   step       pop (culling by the current tmax), guards, node: hit leaves
+ *              are kept for the list, every hit inner child but the nearest
+ *              is pushed and the walk goes on to the nearest;
+ *   leaf phase each owner keeps the accepted test (0 < t < its tmax) with the
+ *              smallest (t, item) -- any hit: (item, t) -- and ends an any-hit
+ *              walk.
+ * ------------------------------------------------------------------------- */
+#define SPEC_MAX_RECORD (1 << 26)
+
+typedef struct {
+    v3 ro, rd, inv;
+    float tmax, ray_tmax;
+    int32_t root, count, cur, sp, it, tri, mesh, slot, ent, hit_ent;
+    uint32_t tests, status, visits;
+    int32_t np, leaves[8];
+    int active, walk_done, parked, inlist;
+    int32_t st_node[256];
+    float st_t[256];
+} spec_lane_t;
+
+typedef struct {
+    const scene_t* s;
+    int any;
+    uint32_t flags;
+    int inner, trigger, list;
+    const crt_ray* rays;
+    crt_hit* hits;
+    uint32_t* tri_tests;
+    int64_t n, wave_begin, wave_end;
+} spec_job_t;
+
+static int spec_begin_walk(const spec_job_t* J, spec_lane_t* L) {
+    const scene_t* s = J->s;
+    int32_t e = L->ent + 1;
+    while (e < s->entity_count && (J->flags & CRT_FLAG_IGNORE_TRANSPARENT) && as_float(s->ents[e].data[1]) < 0.99f)
+        ++e;
+    L->ent = e;
+    if (e >= s->entity_count) return 0;
+    const crt_entity* en = &s->ents[e];
+    const crt_ray* r = &J->rays[L->slot];
+    L->ro = xform(en->inverse, mk(r->org[0], r->org[1], r->org[2]), 1.0f);
+    L->rd = xform(en->inverse, mk(r->dir[0], r->dir[1], r->dir[2]), 0.0f);
+    L->inv = mk(1.0f / L->rd.x, 1.0f / L->rd.y, 1.0f / L->rd.z);
+    if (J->any) L->tmax = L->ray_tmax;
+    L->root = en->node_offset;
+    L->count = en->node_count;
+    L->cur = L->root;
+    L->sp = 0;
+    L->it = 0;
+    L->np = 0;
+    L->walk_done = L->parked = L->inlist = 0;
+    return 1;
+}
+
+static void spec_finish(const spec_job_t* J, spec_lane_t* L) {
+    const scene_t* s = J->s;
+    crt_hit* h = &J->hits[L->slot];
+    h->t = h->u = h->v = h->w = -1.0f;
+    const float closest = L->tri >= 0 ? L->tmax : -1.0f;
+    if (J->any) {
+        if (L->tri >= 0) h->t = closest;
+    } else {
+        int ok = closest > 0.0f && ((J->flags & CRT_FLAG_TRI0_IS_MISS) ? L->tri > 0 : L->tri >= 0);
+        if (ok) {
+            const crt_entity* e = &s->ents[L->hit_ent];
+            const crt_ray* r = &J->rays[L->slot];
+            v3 lro = xform(e->inverse, mk(r->org[0], r->org[1], r->org[2]), 1.0f);
+            v3 lrd = xform(e->inverse, mk(r->dir[0], r->dir[1], r->dir[2]), 0.0f);
+            const crt_tri_record* q = &s->recs[L->tri];
+            v3 p = mk(lro.x + lrd.x * closest, lro.y + lrd.y * closest, lro.z + lrd.z * closest);
+            v3 uvw = barycentrics(p, mk(q->v0[0], q->v0[1], q->v0[2]), mk(q->v1[0], q->v1[1], q->v1[2]),
+                                  mk(q->v2[0], q->v2[1], q->v2[2]));
+            h->t = closest;
+            h->u = uvw.x;
+            h->v = uvw.y;
+            h->w = uvw.z;
+        }
+    }
+    h->tri = L->tri;
+    h->mesh = L->mesh;
+    h->entity = L->tri >= 0 ? L->hit_ent : -1;
+    h->status = pack_status(L->visits, L->status);
+    if (J->tri_tests) J->tri_tests[L->slot] = L->tests;
+}
+
+static int spec_step(const spec_job_t* J, spec_lane_t* L) {
+    const scene_t* s = J->s;
+    const int G = s->format == CRT_FORMAT_WIDE8 ? 2 : 1;
+    const crt_wide_group* groups = (const crt_wide_group*)s->nodes;
+    if (L->cur < 0) {
+        int found = 0;
+        while (L->sp > 0) {
+            --L->sp;
+            if (!J->any && L->st_t[L->sp] > L->tmax * WIDE_GROW) continue;
+            L->cur = L->st_node[L->sp];
+            found = 1;
+            break;
+        }
+        if (!found) return 0;
+    }
+    if (L->it >= s->max_iter) {
+        L->status |= CRT_STATUS_ITER_CAP;
+        return 0;
+    }
+    if (L->cur < L->root || L->cur >= L->root + L->count || (int64_t)L->cur >= s->total_nodes) {
+        L->status |= CRT_STATUS_BAD_INDEX;
+        return 0;
+    }
+    L->it++;
+    float ct[8];
+    int32_t cn[8];
+    int nl = 0;
+    const float tgrow = L->tmax * WIDE_GROW;
+    for (int j = 0; j < 8; ++j) {
+        ct[j] = INFINITY;
+        cn[j] = -1;
+    }
+    for (int g = 0; g < G; ++g) {
+        const crt_wide_group* gp = &groups[(int64_t)L->cur * G + g];
+        for (int k = 0; k < 4; ++k) {
+            v3 ro = L->ro, inv = L->inv;
+            float fx = (gp->hi_x[k] - ro.x) * inv.x, fy = (gp->hi_y[k] - ro.y) * inv.y,
+                  fz = (gp->hi_z[k] - ro.z) * inv.z;
+            float nx = (gp->lo_x[k] - ro.x) * inv.x, ny = (gp->lo_y[k] - ro.y) * inv.y,
+                  nz = (gp->lo_z[k] - ro.z) * inv.z;
+            float t1 = mn(mn(mx(fx, nx), mn(mx(fy, ny), mx(fz, nz))) * WIDE_GROW, tgrow);
+            float t0 = mx(mx(mn(fx, nx), mx(mn(fy, ny), mn(fz, nz))), 0.0f);
+            int32_t cnt = gp->count[k], chi = gp->child[k];
+            int hit = cnt >= 0 && t1 >= t0 && t1 > 0.0f;
+            if (hit && cnt > 0) {
+                if (chi >= 0 && chi + cnt <= SPEC_MAX_RECORD)
+                    L->leaves[nl++] = (chi << 4) | cnt;
+                else
+                    L->status |= CRT_STATUS_BAD_INDEX;
+            }
+            int inner = hit && cnt == 0;
+            ct[4 * g + k] = inner ? t0 : INFINITY;
+            cn[4 * g + k] = inner ? chi : -1;
+        }
+    }
+    int m = 0;
+    for (int j = 0; j < 8; ++j) m += cn[j] >= 0;
+    int pushes = m > 0 ? m - 1 : 0;
+    if (L->sp + pushes > s->stack_cap) {
+        L->status |= CRT_STATUS_STACK_OVERFLOW;
+        return 0;
+    }
+    if (G == 1) {
+        static const int net4[5][2] = {{0, 1}, {2, 3}, {0, 2}, {1, 3}, {1, 2}};
+        for (int e = 0; e < 5; ++e) cx(ct, cn, net4[e][0], net4[e][1]);
+    } else {
+        static const int net8[19][2] = {{0, 1}, {2, 3}, {4, 5}, {6, 7}, {0, 2}, {1, 3}, {4, 6},
+                                        {5, 7}, {1, 2}, {5, 6}, {0, 4}, {1, 5}, {2, 6}, {3, 7},
+                                        {2, 4}, {3, 5}, {1, 2}, {3, 4}, {5, 6}};
+        for (int e = 0; e < 19; ++e) cx(ct, cn, net8[e][0], net8[e][1]);
+    }
+    for (int j = 4 * G - 1; j >= 1; --j)
+        if (j < m) {
+            L->st_node[L->sp] = cn[j];
+            L->st_t[L->sp] = ct[j];
+            L->sp++;
+        }
+    L->cur = cn[0];
+    L->np = nl;
+    return 1;
+}
+
+/* Appends the waiting leaves of every lane in lane order while they fit. */
+static void spec_append(const spec_job_t* J, spec_lane_t* lanes, uint32_t* items, int32_t* nitems) {
+    int32_t room = J->list - *nitems, excl = 0, stop = 0;
+    for (int l = 0; l < 64; ++l) {
+        spec_lane_t* L = &lanes[l];
+        int32_t n = 0;
+        for (int j = 0; j < L->np; ++j) n += L->leaves[j] & 0xF;
+        if (n == 0) continue;
+        if (!stop && excl + n <= room) {
+            int32_t k = *nitems + excl;
+            for (int j = 0; j < L->np; ++j) {
+                int32_t first = L->leaves[j] >> 4, len = L->leaves[j] & 0xF;
+                for (int32_t x = 0; x < len; ++x) items[k++] = ((uint32_t)(first + x) << 6) | (uint32_t)l;
+            }
+            L->tests += (uint32_t)n;
+            L->np = 0;
+            L->parked = 0;
+            L->inlist = 1;
+            excl += n;
+        } else {
+            stop = 1;   /* later lanes' exclusive sums only grow */
+            L->parked = 1;
+        }
+    }
+    *nitems += excl;
+}
+
+static void spec_leaf_phase(const spec_job_t* J, spec_lane_t* lanes, const uint32_t* items, int32_t* nitems) {
+    const scene_t* s = J->s;
+    uint64_t keys[64];
+    for (int l = 0; l < 64; ++l) keys[l] = ~0ull;
+    for (int32_t idx = 0; idx < *nitems; ++idx) {
+        uint32_t item = items[idx];
+        spec_lane_t* O = &lanes[item & 63u];
+        const crt_tri_record* r = &s->recs[item >> 6];
+        float t = tri_t(O->ro, O->rd, mk(r->v0[0], r->v0[1], r->v0[2]), mk(r->v1[0], r->v1[1], r->v1[2]),
+                        mk(r->v2[0], r->v2[1], r->v2[2]));
+        if (t > 0.0f && t < O->tmax) {
+            uint32_t tb;
+            memcpy(&tb, &t, 4);
+            uint64_t key = J->any ? (((uint64_t)(uint32_t)idx << 32) | tb) : (((uint64_t)tb << 32) | (uint32_t)idx);
+            if (key < keys[item & 63u]) keys[item & 63u] = key;
+        }
+    }
+    for (int l = 0; l < 64; ++l) {
+        spec_lane_t* L = &lanes[l];
+        if (keys[l] != ~0ull && L->inlist) {
+            uint32_t ib = J->any ? (uint32_t)(keys[l] >> 32) : (uint32_t)keys[l];
+            uint32_t tb = J->any ? (uint32_t)keys[l] : (uint32_t)(keys[l] >> 32);
+            const crt_tri_record* r = &s->recs[items[ib] >> 6];
+            memcpy(&L->tmax, &tb, 4);
+            L->tri = r->tri;
+            L->mesh = r->mesh;
+            L->hit_ent = L->ent;
+            if (J->any) {
+                L->walk_done = 1;
+                L->np = 0;
+                L->parked = 0;
+            }
+        }
+        L->inlist = 0;
+    }
+    *nitems = 0;
+}
+
+static void* spec_run_job(void* arg) {
+    const spec_job_t* J = (const spec_job_t*)arg;
+    spec_lane_t* lanes = (spec_lane_t*)malloc(sizeof(spec_lane_t) * 64);
+    uint32_t* items = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)J->list);
+    if (!lanes || !items) {
+        free(lanes);
+        free(items);
+        return NULL;
+    }
+    for (int64_t w = J->wave_begin; w < J->wave_end; ++w) {
+        int32_t nitems = 0;
+        for (int l = 0; l < 64; ++l) {
+            spec_lane_t* L = &lanes[l];
+            memset(L, 0, offsetof(spec_lane_t, st_node));
+            int64_t i = w * 64 + l;
+            if (i >= J->n) continue;
+            L->active = 1;
+            L->slot = (int32_t)i;
+            L->ray_tmax = J->rays[i].tmax;
+            L->tmax = L->ray_tmax;
+            L->ent = -1;
+            L->hit_ent = -1;
+            L->tri = L->mesh = -1;
+            if (!spec_begin_walk(J, L)) {
+                spec_finish(J, L);
+                L->active = 0;
+            }
+        }
+        for (;;) {
+            int act = 0;
+            for (int l = 0; l < 64; ++l) act |= lanes[l].active;
+            if (!act) break;
+            for (int k = 0; k < J->inner; ++k) {
+                for (int l = 0; l < 64; ++l) {
+                    spec_lane_t* L = &lanes[l];
+                    if (L->active && !L->walk_done && !L->parked)
+                        if (!spec_step(J, L)) L->walk_done = 1;
+                }
+                spec_append(J, lanes, items, &nitems);
+                int stepping = 0;
+                for (int l = 0; l < 64; ++l) stepping |= lanes[l].active && !lanes[l].walk_done && !lanes[l].parked;
+                if (!stepping || nitems >= J->trigger) break;
+            }
+            if (nitems > 0) {
+                spec_leaf_phase(J, lanes, items, &nitems);
+                spec_append(J, lanes, items, &nitems);
+            }
+            for (int l = 0; l < 64; ++l) {
+                spec_lane_t* L = &lanes[l];
+                if (L->active && L->walk_done && !L->parked && !L->inlist) {
+                    L->visits += (uint32_t)L->it;
+                    int fin = J->any && L->tri >= 0;
+                    if (!fin) fin = !spec_begin_walk(J, L);
+                    if (fin) {
+                        spec_finish(J, L);
+                        L->active = 0;
+                    }
+                }
+            }
+        }
+    }
+    free(lanes);
+    free(items);
+    return NULL;
+}
+
+/* Schedule parameters of the kernel's preset: stack_cap = its LDS stack
+ * entries + 64, inner node steps per round, the leaf-phase trigger and the
+ * list capacity (items). */
+int oracle_trace_wide_spec(int32_t width, int32_t entity_count, int64_t total_nodes, const crt_wide_group* groups,
+                           const crt_tri_record* records, const crt_entity* ents, int query, uint32_t flags,
+                           const crt_ray* rays, crt_hit* hits, int64_t n, int threads, uint32_t* tri_tests,
+                           int32_t max_iter, int32_t stack_cap, int32_t inner, int32_t trigger, int32_t list) {
+    scene_t s;
+    memset(&s, 0, sizeof(s));
+    if ((width != 4 && width != 8) || inner < 1 || trigger < 1 || list < trigger || list < width * 15 ||
+        stack_cap < 1 || stack_cap > 248)
+        return -1;
+    s.format = width == 8 ? CRT_FORMAT_WIDE8 : CRT_FORMAT_WIDE4;
+    s.entity_count = entity_count;
+    s.total_nodes = total_nodes;
+    s.nodes = groups;
+    s.recs = records;
+    s.ents = ents;
+    s.stack_cap = stack_cap;
+    s.max_iter = max_iter > 0 ? max_iter : MAX_ITER_DEFAULT;
+    int64_t waves = (n + 63) / 64;
+    if (threads < 1) threads = 1;
+    if (threads > 256) threads = 256;
+    if ((int64_t)threads > waves) threads = (int)(waves > 0 ? waves : 1);
+    pthread_t th[256];
+    spec_job_t jobs[256];
+    int64_t per = (waves + threads - 1) / threads;
+    for (int t = 0; t < threads; ++t) {
+        spec_job_t* J = &jobs[t];
+        J->s = &s;
+        J->any = query == CRT_QUERY_ANY;
+        J->flags = flags;
+        J->inner = inner;
+        J->trigger = trigger;
+        J->list = list;
+        J->rays = rays;
+        J->hits = hits;
+        J->tri_tests = tri_tests;
+        J->n = n;
+        J->wave_begin = (int64_t)t * per < waves ? (int64_t)t * per : waves;
+        J->wave_end = J->wave_begin + per < waves ? J->wave_begin + per : waves;
+    }
+    for (int t = 1; t < threads; ++t) pthread_create(&th[t], NULL, spec_run_job, &jobs[t]);
+    spec_run_job(&jobs[0]);
+    for (int t = 1; t < threads; ++t) pthread_join(th[t], NULL);
+    return 0;
 }

@@ -105,3 +105,18 @@ def test_device_build_workspaces_and_arguments(crt):
     assert lib.crt_graph_end(None, None) == -1
     assert lib.crt_graph_launch(None, None) == -1
     assert lib.crt_graph_destroy(None) == 0
+
+
+def test_layout_autotuner_rejects_mismatched_layouts(crt):
+    """crt_autotune_batched_layouts checks that every layout holds the same work before any device call."""
+    import ctypes as C
+    lib = crt.lib()
+    sv = crt.SceneView(0, 1, 10, 1, 1, 1, 1, None)
+    b = (crt.Batch * 2)(crt.Batch(sv, 0, 0, 0x1000, 0x2000, 64, None), crt.Batch(sv, 0, 0, 0x1100, 0x2000, 64, None))
+    best = crt.BatchConfig()
+    ms, idx = C.c_float(), C.c_int32()
+    rc = lib.crt_autotune_batched_layouts(b, 1, 2, None, 0, 1, None, C.byref(idx), C.byref(best), C.byref(ms))
+    assert rc == -1 and b"differs from layout 0" in lib.crt_last_error()
+    assert lib.crt_autotune_batched_layouts(b, 0, 2, None, 0, 1, None, C.byref(idx), C.byref(best), C.byref(ms)) == -1
+    assert lib.crt_autotune_batched_layouts(None, 1, 1, None, 0, 1, None, None, None, None) == -1
+

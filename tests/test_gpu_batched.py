@@ -226,3 +226,34 @@ def test_batched_stack_format(orc):
                     assert np.array_equal(dt.download(), ot)
     log(f"[parity] batched stack format: {sum(b[2].count for b in batches)} rays, 8 presets x 3 shapes, "
         "eager and graph, bit-identical")
+
+
+def test_autotune_picks_a_layout(work, orc):
+    """crt_autotune_batched_layouts over the BVH2, BVH4 and BVH8 layouts of the same scenes: the chosen layout's
+    plan gives that layout's host-oracle records."""
+    import candela_amd as crt
+    for key in ((4, 2), (8, 2)):
+        work.build_wide(crt, *key)
+    keys = [None, (4, 2), (8, 2)]
+    layouts = [work.batches(crt, k) for k in keys]
+    cands = [crt.BatchConfig(v, 1, 4, 0, 1, 16, co) for v in (0, 6) for co in (0, 1)]
+    idx, best, ms = crt.autotune_batched_layouts(layouts, cands, repeats=1)
+    log(f"[autotune] layouts: best {keys[idx]} variant={best.variant} child_order={best.child_order} {ms:.3f} ms")
+    assert 0 <= idx < 3 and ms > 0
+    clear_hits(work)
+    crt.BatchPlan(layouts[idx], best).launch()
+    crt.lib().crt_device_sync()
+    key = keys[idx]
+    for s in work.scenes:
+        if key is None:
+            hb = s.host
+            rc = orc.trace(0, hb["nodes"], hb["triangles"], hb["vertices"], hb["entities"], 0, s.closest)
+            ra = orc.trace(0, hb["nodes"], hb["triangles"], hb["vertices"], hb["entities"], 1, s.shadow)
+        else:
+            w = s.wide[key].host
+            rc = orc.trace_wide(w.width, w.groups, w.records, w.entities, 0, s.closest)
+            ra = orc.trace_wide(w.width, w.groups, w.records, w.entities, 1, s.shadow,
+                                flags=4 if best.child_order == 1 else 0)
+        assert s.h_closest.download().tobytes() == rc.tobytes(), f"scene {s.index} closest ({key})"
+        assert s.h_shadow.download().tobytes() == ra.tobytes(), f"scene {s.index} shadow ({key})"
+    log(f"[parity] autotuned layout {key}: every batch bit-identical to its oracle")
