@@ -89,7 +89,8 @@ struct PathWave {
         ent = e;
         if (e >= s.entity_count) return false;
         const crt_entity& en = s.entities[e];
-        c.begin(xform(en.inverse, org, 1.0f), xform(en.inverse, dir, 0.0f), c.tmax, en.node_offset, en.node_count);
+        c.begin(xform(en.inverse, org, 1.0f), xform(en.inverse, dir, 0.0f), c.tmax, en.node_offset, en.node_count,
+                s.total_nodes);
         walk_done = false;
         return true;
     }

@@ -76,14 +76,19 @@ struct Stack {
     float* spill_t;
 
     __device__ __forceinline__ LdsInt* col() const { return (LdsInt*)lds; }   // see LdsInt
+    // Stack entry sp < LDS_DEPTH sits on row LDS_DEPTH - 1 - sp (its node, and its entry distance one
+    // column stride later): the stack grows toward row 0, so the entries one node step pushes lie on
+    // consecutive rows above one base row and are stored at constant offsets from one address.
+    __device__ __forceinline__ LdsInt* row(int32_t r) const { return col() + 2 * r * stride; }
 
     __device__ __forceinline__ void put_leaf(int32_t j, int32_t v) { col()[(2 * LDS_DEPTH + j) * stride] = v; }
     __device__ __forceinline__ int32_t leaf(int32_t j) const { return col()[(2 * LDS_DEPTH + j) * stride]; }
 
     __device__ __forceinline__ void put(int32_t sp, int32_t node, float t) {
         if (sp < LDS_DEPTH) {
-            col()[(2 * sp) * stride] = node;
-            col()[(2 * sp + 1) * stride] = __float_as_int(t);
+            LdsInt* r = row(LDS_DEPTH - 1 - sp);
+            r[0] = node;
+            r[stride] = __float_as_int(t);
         } else {
             spill_node[sp - LDS_DEPTH] = node;
             spill_t[sp - LDS_DEPTH] = t;
@@ -91,8 +96,9 @@ struct Stack {
     }
     __device__ __forceinline__ void get(int32_t sp, int32_t& node, float& t) const {
         if (sp < LDS_DEPTH) {
-            node = col()[(2 * sp) * stride];
-            t = __int_as_float(col()[(2 * sp + 1) * stride]);
+            const LdsInt* r = row(LDS_DEPTH - 1 - sp);
+            node = r[0];
+            t = __int_as_float(r[stride]);
         } else {
             node = spill_node[sp - LDS_DEPTH];
             t = spill_t[sp - LDS_DEPTH];
@@ -113,20 +119,27 @@ struct Core {
 
     f3 ro, rd, inv;
     float tmax;          // closest: shrinks with every accepted hit
-    int32_t root, count;
+    int32_t root;
+    uint32_t span;       // node indices root .. root + span - 1 pass the index guard
     int32_t cur;         // next node, or -1: pop first
     int32_t sp, it;
     int32_t np, pi;      // pending leaves (in LDS, start << 4 | count): count, next to test
     int32_t tri, mesh;   // accepted hit of this walk
     uint32_t tests, status;
 
-    __device__ __forceinline__ void begin(f3 lro, f3 lrd, float walk_tmax, int32_t r, int32_t c) {
+    // The index guard passes cur in [r, min(r + c, total_nodes)), with r + c in 32-bit arithmetic as the
+    // host walk computes it; span is that range's length (at most 2^31 - 1 - r), so the guard is one
+    // unsigned compare of cur - r mod 2^32 per step: for cur < r that difference is at least 2^31 - r.
+    __device__ __forceinline__ void begin(f3 lro, f3 lrd, float walk_tmax, int32_t r, int32_t c,
+                                          int64_t total_nodes) {
         ro = lro;
         rd = lrd;
         inv = mk(1.0f / rd.x, 1.0f / rd.y, 1.0f / rd.z);
         tmax = walk_tmax;
         root = r;
-        count = c;
+        const int64_t end = int32_t(uint32_t(r) + uint32_t(c));
+        const int64_t lim = end < total_nodes ? end : total_nodes;
+        span = lim > r ? uint32_t(lim - r) : 0u;
         cur = r;
         sp = 0;
         it = 0;
@@ -158,13 +171,14 @@ struct Core {
             status |= CRT_STATUS_ITER_CAP;
             return kEnd;
         }
-        if (cur < root || cur >= root + count || int64_t(cur) >= s.total_nodes) {
+        if (uint32_t(cur) - uint32_t(root) >= span) {
             status |= CRT_STATUS_BAD_INDEX;
             return kEnd;
         }
         it++;
         np = 0;
         pi = 0;
+        int32_t m = 0;   // hit inner children with a node index >= 0 (the entries the sort puts first)
         const float4* __restrict__ groups = reinterpret_cast<const float4*>(s.nodes);
         // hit inner children as (entry distance, node) in named registers t0..t7 /
         // n0..n7 (no arrays: every index is a compile-time name); misses and
@@ -189,6 +203,7 @@ struct Core {
         const bool inner = hit && cnt == 0;                                                                 \
         T = inner ? e0 : inf;                                                                               \
         N = inner ? chi : -1;                                                                               \
+        m += N >= 0 ? 1 : 0;                                                                                \
     }
 #define CRT_WIDE_GROUP(G_, TA, NA, TB, NB, TC, NC, TD, ND)                                                  \
     {                                                                                                       \
@@ -215,8 +230,6 @@ struct Core {
         if constexpr (W == 8) CRT_WIDE_GROUP(1, t4, n4, t5, n5, t6, n6, t7, n7)
 #undef CRT_WIDE_GROUP
 #undef CRT_WIDE_CHILD
-        const int32_t m = (n0 >= 0) + (n1 >= 0) + (n2 >= 0) + (n3 >= 0) + (n4 >= 0) + (n5 >= 0) + (n6 >= 0) +
-                          (n7 >= 0);
         // with leaves pending the first child is pushed too (on top)
         const int32_t pushes = np ? m : (m > 0 ? m - 1 : 0);
         if (sp + pushes > kCap) {
@@ -272,8 +285,37 @@ struct Core {
             CRT_CX(t1, n1, t2, n2) CRT_CX(t3, n3, t4, n4) CRT_CX(t5, n5, t6, n6)
         }
 #undef CRT_CX
+        // farthest first, so that pops come nearest first: sorted entry k (k < m) goes to stack
+        // index sp + m - 1 - k; entry 0 only with leaves pending
+        if (sp + m <= LDS_DEPTH) {
+            // the whole run stays in LDS: entry k on row LDS_DEPTH - sp - m + k, all from one base
+            LdsInt* b = st.row(LDS_DEPTH - sp - m);
+            const int32_t rs = 2 * st.stride;   // one row
+#define CRT_WIDE_PUT(K, N, T)                           \
+    if (K < m) {                                        \
+        b[K * rs] = N;                                  \
+        b[K * rs + st.stride] = __float_as_int(T);      \
+    }
+            if constexpr (W == 8) {
+                CRT_WIDE_PUT(7, n7, t7)
+                CRT_WIDE_PUT(6, n6, t6)
+                CRT_WIDE_PUT(5, n5, t5)
+                CRT_WIDE_PUT(4, n4, t4)
+            }
+            CRT_WIDE_PUT(3, n3, t3)
+            CRT_WIDE_PUT(2, n2, t2)
+            CRT_WIDE_PUT(1, n1, t1)
+            if (np) CRT_WIDE_PUT(0, n0, t0)
+#undef CRT_WIDE_PUT
+            sp += pushes;
+            if (np) {
+                cur = -1;
+                return kLeaves;
+            }
+            cur = n0;   // -1 (no inner child hit): pop first
+            return kNext;
+        }
         const int32_t lowest = np ? 0 : 1;   // sorted index pushed last
-        // farthest first, so that pops come nearest first
         if (W == 8) {
             if (7 >= lowest && 7 < m) st.put(sp++, n7, t7);
             if (6 >= lowest && 6 < m) st.put(sp++, n6, t6);
@@ -324,7 +366,7 @@ __device__ __forceinline__ uint32_t walk_wide_thread(const SceneArgs& s, f3 ro, 
     st.stride = stride;
     st.spill_node = spill_node;
     st.spill_t = spill_t;
-    c.begin(ro, rd, tmax, root, count);
+    c.begin(ro, rd, tmax, root, count, s.total_nodes);
     c.tri = -1;
     c.mesh = -1;
     c.tests = 0;
@@ -388,7 +430,7 @@ struct WideWave {
         const float4* rp = reinterpret_cast<const float4*>(rays + slot);
         const float4 a = rp[0], b = rp[1];
         c.begin(xform(en.inverse, mk(a.x, a.y, a.z), 1.0f), xform(en.inverse, mk(b.x, b.y, b.z), 0.0f),
-                c.any() ? ray_tmax : c.tmax, en.node_offset, en.node_count);
+                c.any() ? ray_tmax : c.tmax, en.node_offset, en.node_count, s.total_nodes);
         walk_done = false;
         return true;
     }
@@ -467,33 +509,46 @@ struct WideWave {
             if (stepping == 0 || __popcll(__ballot(leafing)) >= LEAF_BATCH) break;
         }
         // --- leaf phase: pending triangles, two records per fetch ----------------
+        // Straight-line code for every lane of the wave, the state updated by
+        // selects: a lane that is not testing reads record 0 (which exists
+        // whenever some lane has a leaf) and a clamped leaf slot, and its
+        // results are discarded.  Branches around the tests would make the
+        // compiler copy the loop-carried state on every iteration.
         while (__ballot(leafing)) {
-            if (leafing) {
-                const int32_t first = (leaf >> 4) + off, left = (leaf & 0xF) - off;
-                const float4* p = rec + 3 * int64_t(first);
-                // the second record, or the first again when one is left (no
-                // branch, no copies, and never past the leaf)
-                const float4* p1 = p + (left > 1 ? 3 : 0);
-                const float4 a0 = p[0], b0 = p[1], c0 = p[2];
-                const float4 a1 = p1[0], b1 = p1[1], c1 = p1[2];
-                const float before = c.tmax;
-                bool done = c.test(first, a0, b0, c0);
-                if (!done && left > 1) done = c.test(first + 1, a1, b1, c1);
-                if (c.tmax < before) hit_ent = ent;
-                off += left > 1 ? 2 : 1;
-                if (done) {
-                    leafing = false;
-                    walk_done = true;
-                } else if (off >= (leaf & 0xF)) {
-                    if (c.pi < c.np) {
-                        leaf = st.leaf(c.pi);
-                        ++c.pi;
-                        off = 0;
-                    } else {
-                        leafing = false;
-                    }
-                }
-            }
+            const bool act = leafing;
+            const int32_t len = leaf & 0xF;
+            const int32_t first = (leaf >> 4) + off, left = len - off;
+            const bool two = act && left > 1;   // an idle lane reads record 0 only
+            const float4* p = rec + 3 * int64_t(act ? first : 0);
+            // the second record, or the first again when one is left (never past the leaf)
+            const float4* p1 = p + (two ? 3 : 0);
+            const float4 a0 = p[0], b0 = p[1], c0 = p[2];
+            const float4 a1 = p1[0], b1 = p1[1], c1 = p1[2];
+            const float ta = tri_t(c.ro, c.rd, mk(a0.x, a0.y, a0.z), mk(b0.x, b0.y, b0.z), mk(c0.x, c0.y, c0.z));
+            const float tb = tri_t(c.ro, c.rd, mk(a1.x, a1.y, a1.z), mk(b1.x, b1.y, b1.z), mk(c1.x, c1.y, c1.z));
+            // first record, then the second unless the first ended an any-hit walk
+            // (acceptance 0 < t < tmax, the first found wins a tie: Core::test)
+            const bool acc_a = act && ta > 0.0f && ta < c.tmax;
+            const float tm = acc_a ? ta : c.tmax;
+            const bool test_b = act && two && !(c.any() && acc_a);
+            const bool acc_b = test_b && tb > 0.0f && tb < tm;
+            c.tmax = acc_b ? tb : tm;
+            c.tri = acc_b ? __float_as_int(a1.w) : (acc_a ? __float_as_int(a0.w) : c.tri);
+            c.mesh = acc_b ? __float_as_int(b1.w) : (acc_a ? __float_as_int(b0.w) : c.mesh);
+            hit_ent = (acc_a || acc_b) ? ent : hit_ent;
+            c.tests += act ? (test_b ? 2u : 1u) : 0u;
+            const bool done = c.any() && (acc_a || acc_b);
+            // next: more of this leaf, the next pending leaf, or the end of the phase
+            const int32_t off2 = off + (two ? 2 : 1);
+            const bool leaf_end = off2 >= len;
+            const bool more = c.pi < c.np;
+            const int32_t nl = st.leaf(uint32_t(c.pi) < uint32_t(W) ? c.pi : W - 1);
+            const bool adv = act && !done && leaf_end && more;
+            leaf = adv ? nl : leaf;
+            c.pi += adv ? 1 : 0;
+            off = adv ? 0 : (act ? off2 : off);
+            walk_done = walk_done || (act && done);
+            leafing = act && !done && !(leaf_end && !more);
         }
         // --- walk ends: next entity or the ray's result --------------------------
         if (__ballot(walk_done)) {
