@@ -212,6 +212,8 @@ SIGNATURES = {
     "crt_event_elapsed_ms": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_float)]),
     "crt_trace": (C.c_int, [C.POINTER(SceneView), C.c_int, C.c_uint32, C.c_void_p, C.c_void_p, C.c_int64,
                             C.POINTER(TraceConfig), C.c_void_p]),
+    "crt_trace_noalloc": (C.c_int, [C.POINTER(SceneView), C.c_int, C.c_uint32, C.c_void_p, C.c_void_p, C.c_int64,
+                                    C.POINTER(TraceConfig), C.c_void_p]),
     "crt_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
     "crt_destroy": (None, [C.c_void_p]),
     "crt_context_format": (C.c_int, [C.c_void_p]),

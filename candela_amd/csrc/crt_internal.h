@@ -52,6 +52,10 @@ struct QueueLease {
 int queue_acquire(const crt_trace_config* cfg, void* stream, const char* who, QueueLease& out);
 int queue_release(const QueueLease& lease, void* stream);
 
+// crt_trace's body (runtime.cpp); crt_trace_noalloc (noalloc.hip) calls it with the static queue block.
+int trace(const crt_scene_view* scene, int query, uint32_t flags, const crt_ray* rays, crt_hit* hits, int64_t n,
+          const crt_trace_config* cfg, void* stream);
+
 }  // namespace crt
 
 // A built object: its flattened buffers and what they were built with.

@@ -418,6 +418,21 @@ int crt_trace(const crt_scene_view* scene, int query, uint32_t flags,
               const crt_ray* rays, crt_hit* hits, int64_t nrays,
               const crt_trace_config* config, void* stream);
 
+/* crt_trace that allocates nothing.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, hosts that forbid allocation after start-up).
+ * The persistent kernels' work-queue counters come from one static
+ * CRT_QUEUE_BYTES block per GPU, held from module load to process end,
+ * instead of a pool block or config->queue (which must be NULL).  The kernels
+ * zero the counters on the call's stream, so nothing needs clearing between
+ * calls; calls whose persistent launches overlap on one GPU share that block
+ * and race.  At most CRT_NOALLOC_MAX_RAYS rays per call (CRT_E_INVALID
+ * beyond).  Otherwise identical to crt_trace: same kernels, same records. */
+#define CRT_NOALLOC_MAX_RAYS 2147483647LL
+int crt_trace_noalloc(const crt_scene_view* scene, int query, uint32_t flags,
+                      const crt_ray* rays, crt_hit* hits, int64_t nrays,
+                      const crt_trace_config* config, void* stream);
+
 /* ------------------------------------------------------------------------- */
 /* Wide BVH (BVH4 / BVH8) derived from the reference's BVH2                   */
 /* ------------------------------------------------------------------------- */
