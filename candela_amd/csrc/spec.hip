@@ -19,17 +19,24 @@
 namespace crt {
 namespace sp {
 
-template <int W, int D, int LIST>
+// Queue slots per lane: the preset's Q for BVH4, Q + 4 (at most 16) for BVH8,
+// so that a lane keeps stepping after a node with leaves.
+template <int W, int Q>
+constexpr int queue_slots() {
+    return W == 4 ? Q : (Q + 4 < 16 ? Q + 4 : 16);
+}
+
+template <int W, int D, int Q>
 struct GroupLds {
-    int32_t cols[wide::lds_ints<W, D>() * 256];
-    spec::WaveLds<LIST> waves[4];
+    int32_t cols[spec::lds_ints<D, queue_slots<W, Q>()>() * 256];
+    spec::WaveLds<queue_slots<W, Q>()> waves[4];
 };
 
-template <bool ANY, int W, int D, int I, int TRIG, int LIST>
+template <bool ANY, int W, int D, int I, int Q>
 __device__ __forceinline__ void grid_body(const SceneArgs& s, const crt_ray* __restrict__ rays,
                                           crt_hit* __restrict__ hits, int64_t n, uint32_t flags, int32_t max_iter,
                                           uint32_t* __restrict__ tri_tests, int32_t blocks,
-                                          GroupLds<W, D, LIST>& lds) {
+                                          GroupLds<W, D, Q>& lds) {
     const int32_t per = (blocks + 7) / 8;  // the grid is padded to a multiple of 8
     const int32_t logical = int32_t(blockIdx.x % 8u) * per + int32_t(blockIdx.x / 8u);
     if (logical >= blocks) return;
@@ -41,24 +48,24 @@ __device__ __forceinline__ void grid_body(const SceneArgs& s, const crt_ray* __r
     st.stride = 256;
     st.spill_node = spill_node;
     st.spill_t = spill_t;
-    spec::SpecWave<ANY, W, D, I, TRIG, LIST> w;
+    spec::SpecWave<ANY, W, D, I, queue_slots<W, Q>()> w;
     w.run(s, rays, hits, i < n ? i : -1, flags, max_iter, tri_tests, st, lds.waves[threadIdx.x >> 6]);
 }
 
-template <bool ANY, int W, int D, int I, int TRIG, int LIST, int OCC>
+template <bool ANY, int W, int D, int I, int Q, int OCC>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1, 8)))
 spec_grid(SceneArgs s, const crt_ray* __restrict__ rays, crt_hit* __restrict__ hits, int64_t n, uint32_t flags,
           int32_t max_iter, uint32_t* __restrict__ tri_tests, int32_t blocks) {
-    __shared__ GroupLds<W, D, LIST> lds;
-    grid_body<ANY, W, D, I, TRIG, LIST>(s, rays, hits, n, flags, max_iter, tri_tests, blocks, lds);
+    __shared__ GroupLds<W, D, Q> lds;
+    grid_body<ANY, W, D, I, Q>(s, rays, hits, n, flags, max_iter, tri_tests, blocks, lds);
 }
 
-template <bool ANY, int W, int D, int I, int TRIG, int LIST, int REFILL, int OCC>
+template <bool ANY, int W, int D, int I, int Q, int REFILL, int OCC>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1, 8)))
 spec_persistent(SceneArgs s, const crt_ray* __restrict__ rays, crt_hit* __restrict__ hits, int64_t n,
                 uint32_t flags, int32_t max_iter, uint32_t* __restrict__ tri_tests,
                 unsigned long long* __restrict__ counters) {
-    __shared__ GroupLds<W, D, LIST> lds;
+    __shared__ GroupLds<W, D, Q> lds;
     int32_t spill_node[wide::kSpill];
     float spill_t[wide::kSpill];
     wide::Stack<D> st;
@@ -66,36 +73,36 @@ spec_persistent(SceneArgs s, const crt_ray* __restrict__ rays, crt_hit* __restri
     st.stride = 256;
     st.spill_node = spill_node;
     st.spill_t = spill_t;
-    spec::SpecWave<ANY, W, D, I, TRIG, LIST> w;
+    spec::SpecWave<ANY, W, D, I, queue_slots<W, Q>()> w;
     w.run_refill(s, rays, hits, n, flags, max_iter, tri_tests, st, lds.waves[threadIdx.x >> 6], counters, REFILL);
 }
 
 }  // namespace sp
 
 // Presets of the grid walk (variant, LDS stack entries, INNER node steps per
-// round, LIST_TRIGGER items, list capacity, OCC waves per SIMD).
+// round, Q queue slots per lane (BVH4), OCC waves per SIMD).
 #define CRT_SPEC_PRESETS(X) \
-    X(0, 8, 4, 64, 256, 0)  \
-    X(1, 8, 4, 128, 256, 0) \
-    X(2, 8, 8, 128, 256, 0) \
-    X(3, 8, 4, 64, 256, 5)  \
-    X(4, 8, 4, 128, 256, 5) \
-    X(5, 8, 2, 64, 256, 5)
+    X(0, 8, 4, 8, 0)        \
+    X(1, 8, 8, 8, 0)        \
+    X(2, 8, 16, 8, 0)       \
+    X(3, 8, 4, 8, 5)        \
+    X(4, 8, 8, 8, 5)        \
+    X(5, 8, 16, 8, 5)
 
-// Presets of the refilling walk (variant, LDS stack entries, INNER,
-// LIST_TRIGGER, list capacity, REFILL idle lanes, OCC).
+// Presets of the refilling walk (variant, LDS stack entries, INNER, Q,
+// REFILL idle lanes, OCC).
 #define CRT_SPEC_P_PRESETS(X)   \
-    X(0, 8, 4, 64, 256, 16, 0)  \
-    X(1, 8, 4, 128, 256, 8, 0)  \
-    X(2, 8, 4, 128, 256, 16, 0) \
-    X(3, 8, 4, 64, 256, 16, 5)  \
-    X(4, 8, 4, 128, 256, 8, 5)  \
-    X(5, 8, 8, 128, 256, 8, 5)
+    X(0, 8, 4, 8, 16, 0)        \
+    X(1, 8, 8, 8, 16, 0)        \
+    X(2, 8, 8, 8, 8, 0)         \
+    X(3, 8, 4, 8, 16, 5)        \
+    X(4, 8, 8, 8, 16, 5)        \
+    X(5, 8, 16, 8, 16, 5)
 
 template <bool ANY, int W>
 static const void* spec_grid_fn(int variant) {
-#define CRT_SPEC_G_FN(V, D, I, TRIG, LIST, OCC) \
-    case V: return reinterpret_cast<const void*>(&sp::spec_grid<ANY, W, D, I, TRIG, LIST, OCC>);
+#define CRT_SPEC_G_FN(V, D, I, Q, OCC) \
+    case V: return reinterpret_cast<const void*>(&sp::spec_grid<ANY, W, D, I, Q, OCC>);
     switch (variant) {
         CRT_SPEC_PRESETS(CRT_SPEC_G_FN)
         default: return nullptr;
@@ -105,8 +112,8 @@ static const void* spec_grid_fn(int variant) {
 
 template <bool ANY, int W>
 static const void* spec_persistent_fn(int variant) {
-#define CRT_SPEC_P_FN(V, D, I, TRIG, LIST, R, OCC) \
-    case V: return reinterpret_cast<const void*>(&sp::spec_persistent<ANY, W, D, I, TRIG, LIST, R, OCC>);
+#define CRT_SPEC_P_FN(V, D, I, Q, R, OCC) \
+    case V: return reinterpret_cast<const void*>(&sp::spec_persistent<ANY, W, D, I, Q, R, OCC>);
     switch (variant) {
         CRT_SPEC_P_PRESETS(CRT_SPEC_P_FN)
         default: return nullptr;

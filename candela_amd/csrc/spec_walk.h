@@ -1,5 +1,5 @@
 // Speculative wave walk of the wide layout (BVH4 / BVH8): lanes never wait
-// for their leaf tests, and the wave tests leaves 64 triangles at a time.
+// for their leaf tests, and the wave tests leaves 64 at a time.
 //
 // The wave-scheduled walk of wide_walk.h parks a lane whose node step found
 // leaves until the wave's leaf phase, because the lane's next box tests need
@@ -11,29 +11,31 @@
 //              grown by kWideGrow, children sorted nearest first), but every
 //              hit inner child except the nearest is pushed and the lane goes
 //              straight on to the nearest, whether or not the node had leaves;
-//              the hit leaves' triangles go to the wave's item list;
-//   items      each item is one triangle test, (record << 6 | owner lane),
-//              appended in lane order, then slot order, then triangle order
-//              (an exclusive scan of the per-lane counts by ballot bit planes);
-//              a lane whose triangles do not fit in the list waits (parked)
-//              until the next leaf phase;
-//   leaf phase after INNER node steps, once LIST_TRIGGER items wait or when no
-//              lane can step: item i of the list is tested by lane i % 64
-//              with the owner's ray (lane shuffles); an accepted test
-//              (0 < t < the owner's tmax) enters a 64-bit LDS atomic minimum
-//              per owner: (t, item) for closest hits, (item, t) for any hits;
-//              then each owner takes its winner (closest: nearer hit; any:
-//              the walk ends);
+//              the hit leaves go to the lane's own queue (Q slots in its LDS
+//              column, no work across lanes); a lane steps only while its
+//              queue has room for a whole node's leaves
+//   round      up to INNER node steps (fewer once no lane can step), then the
+//              leaf phase whenever some queue holds a leaf, then the walk ends
+//   leaf phase the queued leaves of all lanes become one list: lane order,
+//              then queue order (an exclusive scan of the queue lengths by
+//              ballot bit planes, then each lane writes (lane, slot) entries);
+//              list entry i is tested by lane i % 64 with the owner's ray
+//              (lane shuffles), the triangles of a leaf two per fetch; an
+//              accepted test (0 < t < the owner's tmax) enters a 64-bit LDS
+//              atomic minimum per owner over (t, order) for closest hits and
+//              (order, t) for any hits, where order = entry * 16 + triangle
+//              of the leaf; then each owner takes its winner (closest: nearer
+//              hit; any: the walk ends) and every queue is empty
 //   culling    box tests and pops use the tmax of the last leaf phase, which
 //              only errs toward visiting more nodes, so a closest-hit walk
-//              still reports the nearest triangle that any wide walk finds.
+//              still reports the nearest triangle that any wide walk finds
 //
 // A lane's visits and tests depend on the leaf phases, which depend on the
 // rays sharing its wave: with the grid launch (64 consecutive rays per wave)
-// the records are deterministic and the host twin (oracle_spec.c) replays the
-// schedule; with the refilling launch the nearest t, the hit / miss answer
-// and any-hit occlusion are still fixed, the triangle is fixed up to exact
-// distance ties, and the counters vary.
+// the records are deterministic and the host twin (oracle_trace.c:
+// oracle_trace_wide_spec) replays the schedule; with the refilling launch the
+// nearest t, the hit / miss answer and any-hit occlusion are still fixed, the
+// triangle is fixed up to exact distance ties, and the counters vary.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -53,10 +55,17 @@ using namespace ::crt::dm;
 using wide::kWideGrow;
 using wide::Stack;
 
-// LDS of one wave: its item list and the 64 owners' keys.
-template <int LIST>
+// Ints per lane column: the stack's 2 * LDS_DEPTH, then the Q queue slots.
+template <int LDS_DEPTH, int Q>
+constexpr int lds_ints() {
+    return 2 * LDS_DEPTH + Q;
+}
+
+// LDS of one wave: the leaf-phase list ((lane << 4) | slot per queued leaf)
+// and the 64 owners' keys.
+template <int Q>
 struct WaveLds {
-    uint32_t items[LIST];
+    uint16_t list[64 * Q];
     unsigned long long keys[64];
 };
 
@@ -64,25 +73,31 @@ __device__ __forceinline__ int32_t lanes_below(unsigned long long mask) {
     return int32_t(__builtin_amdgcn_mbcnt_hi(uint32_t(mask >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(mask), 0u)));
 }
 
-// An item packs a triangle record index above the 6 owner-lane bits.
+// Orders the LDS accesses of the lanes of one wave (list and queue entries
+// written by one lane and read by another) without a workgroup barrier.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// A queued leaf packs its first record above 4 count bits.
 constexpr int32_t kMaxRecord = 1 << 26;
 
-template <bool ANY, int W, int LDS_DEPTH, int INNER, int LIST_TRIGGER, int LIST>
+template <bool ANY, int W, int LDS_DEPTH, int INNER, int Q>
 struct SpecWave {
     static constexpr int G = W / 4;
     static constexpr int32_t kCap = LDS_DEPTH + wide::kSpill;
-    static_assert(LIST >= LIST_TRIGGER && LIST >= W * 15, "a lane's leaves of one step always fit an empty list");
+    static_assert(Q >= W && Q <= 16, "a queue holds one node's leaves; slots fit 4 bits");
 
     // per lane
     f3 ro, rd, inv;
     float tmax, ray_tmax;
-    int32_t root, count, cur, sp, it;
+    int32_t root, cur, sp, it;
+    uint32_t span;       // node indices root .. root + span - 1 pass the index guard
     int32_t tri, mesh, slot, ent, hit_ent;
     uint32_t tests, status, visits;
-    int32_t np;   // leaves of the last step waiting for the list (in the lane's LDS column)
-    bool active, walk_done, parked, inlist;
-    // per wave (uniform)
-    int32_t nitems;
+    int32_t nq;          // queued leaves (in the lane's LDS column)
+    bool active, walk_done;
 
     __device__ __forceinline__ bool begin_walk(const SceneArgs& s, const crt_ray* __restrict__ rays, uint32_t flags) {
         int32_t e = ent + 1;
@@ -99,14 +114,15 @@ struct SpecWave {
         inv = mk(1.0f / rd.x, 1.0f / rd.y, 1.0f / rd.z);
         if (ANY) tmax = ray_tmax;
         root = en.node_offset;
-        count = en.node_count;
+        // the index guard of wide_walk.h's Core::begin
+        const int64_t end = int32_t(uint32_t(root) + uint32_t(en.node_count));
+        const int64_t lim = end < s.total_nodes ? end : s.total_nodes;
+        span = lim > root ? uint32_t(lim - root) : 0u;
         cur = root;
         sp = 0;
         it = 0;
-        np = 0;
+        nq = 0;
         walk_done = false;
-        parked = false;
-        inlist = false;
         return true;
     }
 
@@ -156,9 +172,11 @@ struct SpecWave {
         }
     }
 
+    // A lane steps while its walk goes on and its queue has room for W leaves.
+    __device__ __forceinline__ bool can_step() const { return active && !walk_done && nq <= Q - W; }
+
     // Pop (culling by the current tmax), guards and one node; the node's hit
-    // leaves go to the lane's LDS column (np of them).  False when the walk
-    // is over.
+    // leaves are appended to the lane's queue.  False when the walk is over.
     __device__ __forceinline__ bool step(const SceneArgs& s, int32_t max_iter, Stack<LDS_DEPTH>& st) {
         if (cur < 0) {
             bool found = false;
@@ -178,12 +196,13 @@ struct SpecWave {
             status |= CRT_STATUS_ITER_CAP;
             return false;
         }
-        if (cur < root || cur >= root + count || int64_t(cur) >= s.total_nodes) {
+        if (uint32_t(cur) - uint32_t(root) >= span) {
             status |= CRT_STATUS_BAD_INDEX;
             return false;
         }
         it++;
-        int32_t nl = 0;
+        int32_t nl = 0;   // leaves of this node, written at queue slots nq, nq + 1, ...
+        int32_t m = 0;    // hit inner children with a node index >= 0
         const float4* __restrict__ groups = reinterpret_cast<const float4*>(s.nodes);
         const float inf = __builtin_inff();
         const float tgrow = tmax * kWideGrow;
@@ -199,15 +218,16 @@ struct SpecWave {
         const bool hit = cnt >= 0 && e1 >= e0 && e1 > 0.0f;                                                 \
         if (hit && cnt > 0) {                                                                               \
             if (chi >= 0 && chi + cnt <= kMaxRecord) {                                                      \
-                st.put_leaf(nl, (chi << 4) | cnt);                                                          \
+                st.put_leaf(nq + nl, (chi << 4) | cnt);                                                     \
                 ++nl;                                                                                       \
             } else {                                                                                        \
-                status |= CRT_STATUS_BAD_INDEX;   /* not representable in an item */                        \
+                status |= CRT_STATUS_BAD_INDEX;   /* not representable in a queue slot */                   \
             }                                                                                               \
         }                                                                                                   \
         const bool inner = hit && cnt == 0;                                                                 \
         T = inner ? e0 : inf;                                                                               \
         N = inner ? chi : -1;                                                                               \
+        m += N >= 0 ? 1 : 0;                                                                                \
     }
 #define CRT_SPEC_GROUP(G_, TA, NA, TB, NB, TC, NC, TD, ND)                                                  \
     {                                                                                                       \
@@ -231,8 +251,6 @@ struct SpecWave {
     }
         CRT_SPEC_GROUP(0, t0, n0, t1, n1, t2, n2, t3, n3)
         if constexpr (W == 8) CRT_SPEC_GROUP(1, t4, n4, t5, n5, t6, n6, t7, n7)
-        const int32_t m = (n0 >= 0) + (n1 >= 0) + (n2 >= 0) + (n3 >= 0) + (n4 >= 0) + (n5 >= 0) + (n6 >= 0) +
-                          (n7 >= 0);
         const int32_t pushes = m > 0 ? m - 1 : 0;
         if (sp + pushes > kCap) {   // the node's leaves are dropped with the walk
             status |= CRT_STATUS_STACK_OVERFLOW;
@@ -253,121 +271,127 @@ struct SpecWave {
 #undef CRT_SPEC_CX
 #undef CRT_SPEC_GROUP
 #undef CRT_SPEC_CHILD
-        // farthest first, so that pops come nearest first; the nearest is next
-        if constexpr (W == 8) {
-            if (7 < m) st.put(sp++, n7, t7);
-            if (6 < m) st.put(sp++, n6, t6);
-            if (5 < m) st.put(sp++, n5, t5);
-            if (4 < m) st.put(sp++, n4, t4);
+        // farthest first, so that pops come nearest first; the nearest is next.
+        // Sorted entry k (1 <= k < m) goes to stack index sp + m - 1 - k.
+        if (sp + m <= LDS_DEPTH) {
+            // the run stays in LDS: entry k on row LDS_DEPTH - sp - m + k, all from one base
+            LdsInt* b = st.row(LDS_DEPTH - sp - m);
+            const int32_t rs = 2 * st.stride;
+#define CRT_SPEC_PUT(K, N, T)                           \
+    if (K < m) {                                        \
+        b[K * rs] = N;                                  \
+        b[K * rs + st.stride] = __float_as_int(T);      \
+    }
+            if constexpr (W == 8) {
+                CRT_SPEC_PUT(7, n7, t7)
+                CRT_SPEC_PUT(6, n6, t6)
+                CRT_SPEC_PUT(5, n5, t5)
+                CRT_SPEC_PUT(4, n4, t4)
+            }
+            CRT_SPEC_PUT(3, n3, t3)
+            CRT_SPEC_PUT(2, n2, t2)
+            CRT_SPEC_PUT(1, n1, t1)
+#undef CRT_SPEC_PUT
+            sp += pushes;
+        } else {
+            if constexpr (W == 8) {
+                if (7 < m) st.put(sp++, n7, t7);
+                if (6 < m) st.put(sp++, n6, t6);
+                if (5 < m) st.put(sp++, n5, t5);
+                if (4 < m) st.put(sp++, n4, t4);
+            }
+            if (3 < m) st.put(sp++, n3, t3);
+            if (2 < m) st.put(sp++, n2, t2);
+            if (1 < m) st.put(sp++, n1, t1);
         }
-        if (3 < m) st.put(sp++, n3, t3);
-        if (2 < m) st.put(sp++, n2, t2);
-        if (1 < m) st.put(sp++, n1, t1);
         cur = n0;   // -1 (no inner child hit): pop first
-        np = nl;
+        nq += nl;
         return true;
     }
 
-    // Appends the waiting leaves of every lane, in lane order, as far as the
-    // list has room; a lane that does not fit stays parked.
-    __device__ __forceinline__ void append(Stack<LDS_DEPTH>& st, WaveLds<LIST>& L) {
-        int32_t n = 0;
-        for (int32_t j = 0; j < np; ++j) n += st.leaf(j) & 0xF;
-        // exclusive scan of n over the wave by bit planes
-        int32_t excl = 0;
-        for (int b = 0; b < 8; ++b) {
-            const unsigned long long plane = __ballot((n >> b) & 1);
-            if (plane == 0ull && __ballot(n >> b) == 0ull) break;
-            excl += lanes_below(plane) << b;
-        }
-        const int32_t room = LIST - nitems;
-        const bool fits = n > 0 && excl + n <= room;
-        const unsigned long long fit_mask = __ballot(fits);
-        if (fits) {
-            int32_t k = nitems + excl;
-            const uint32_t lane = uint32_t(__lane_id());
-            for (int32_t j = 0; j < np; ++j) {
-                const int32_t leaf = st.leaf(j);
-                const int32_t first = leaf >> 4, len = leaf & 0xF;
-                for (int32_t x = 0; x < len; ++x) L.items[k++] = (uint32_t(first + x) << 6) | lane;
-            }
-            tests += uint32_t(n);
-            np = 0;
-            parked = false;
-            inlist = true;
-        } else if (n > 0) {
-            parked = true;
-        }
-        if (fit_mask) {
-            const int32_t last = 63 - __builtin_clzll(fit_mask);
-            nitems += __shfl(excl + n, last);
-        }
-    }
-
-    // Tests every item of the list, 64 at a time, and gives each owner its winner.
-    __device__ __forceinline__ void leaf_phase(const SceneArgs& s, WaveLds<LIST>& L) {
+    // Tests every queued leaf of the wave, 64 at a time, and gives each owner
+    // its winner; the queues are empty afterwards.
+    __device__ __forceinline__ void leaf_phase(const SceneArgs& s, Stack<LDS_DEPTH>& st, WaveLds<Q>& L) {
         const int32_t lane = __lane_id();
-        L.keys[lane] = ~0ull;
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         const float4* __restrict__ rec = reinterpret_cast<const float4*>(s.tri_records);
-        for (int32_t base = 0; base < nitems; base += 64) {
+        // exclusive scan of the queue lengths (at most 16) by bit planes
+        int32_t excl = 0, total = 0;
+#pragma unroll
+        for (int b = 0; b < 5; ++b) {
+            const unsigned long long plane = __ballot((nq >> b) & 1);
+            excl += lanes_below(plane) << b;
+            total += __popcll(plane) << b;
+        }
+        for (int32_t j = 0; j < nq; ++j) {
+            L.list[excl + j] = uint16_t((lane << 4) | j);
+            tests += uint32_t(st.leaf(j) & 0xF);
+        }
+        L.keys[lane] = ~0ull;
+        wave_lds_sync();
+        // the owner's queue slot j sits (owner - lane) columns from this lane's
+        const LdsInt* col = st.col();
+        const int32_t qrow = 2 * LDS_DEPTH * st.stride;
+        for (int32_t base = 0; base < total; base += 64) {
             const int32_t idx = base + lane;
-            const bool valid = idx < nitems;
-            const uint32_t item = valid ? L.items[idx] : 0u;
-            const int32_t owner = int32_t(item & 63u);
+            const bool valid = idx < total;
+            const uint32_t e = valid ? uint32_t(L.list[idx]) : 0u;
+            const int32_t owner = int32_t(e >> 4), j = int32_t(e & 15u);
+            const int32_t leaf = valid ? col[(owner - lane) + qrow + j * st.stride] : 0;
             const float ox = __shfl(ro.x, owner), oy = __shfl(ro.y, owner), oz = __shfl(ro.z, owner);
             const float dx = __shfl(rd.x, owner), dy = __shfl(rd.y, owner), dz = __shfl(rd.z, owner);
             const float otmax = __shfl(tmax, owner);
-            if (valid) {
-                const float4* p = rec + 3 * int64_t(item >> 6);
-                const float4 a = p[0], b = p[1], c = p[2];
-                const float t = tri_t(mk(ox, oy, oz), mk(dx, dy, dz), mk(a.x, a.y, a.z), mk(b.x, b.y, b.z),
-                                      mk(c.x, c.y, c.z));
-                if (t > 0.0f && t < otmax) {
-                    const unsigned long long tb = __float_as_uint(t), ib = uint32_t(idx);
-                    atomicMin(&L.keys[owner], ANY ? ((ib << 32) | tb) : ((tb << 32) | ib));
+            const int32_t first = leaf >> 4, len = leaf & 0xF;
+            const f3 o = mk(ox, oy, oz), d = mk(dx, dy, dz);
+            for (int32_t x = 0; __ballot(x < len); x += 2) {
+                const bool v0 = x < len, v1 = x + 1 < len;
+                // a lane past its leaf reads record 0, which exists whenever a leaf does
+                const float4* p = rec + 3 * int64_t(v0 ? first + x : 0);
+                const float4* p1 = p + (v1 ? 3 : 0);
+                const float4 a0 = p[0], b0 = p[1], c0 = p[2];
+                const float4 a1 = p1[0], b1 = p1[1], c1 = p1[2];
+                const float ta = tri_t(o, d, mk(a0.x, a0.y, a0.z), mk(b0.x, b0.y, b0.z), mk(c0.x, c0.y, c0.z));
+                const float tb = tri_t(o, d, mk(a1.x, a1.y, a1.z), mk(b1.x, b1.y, b1.z), mk(c1.x, c1.y, c1.z));
+                const unsigned long long order = (unsigned long long)(uint32_t(idx) << 4 | uint32_t(x));
+                if (v0 && ta > 0.0f && ta < otmax) {
+                    const unsigned long long tbits = __float_as_uint(ta);
+                    atomicMin(&L.keys[owner], ANY ? ((order << 32) | tbits) : ((tbits << 32) | order));
+                }
+                if (v1 && tb > 0.0f && tb < otmax) {
+                    const unsigned long long tbits = __float_as_uint(tb);
+                    atomicMin(&L.keys[owner], ANY ? (((order + 1) << 32) | tbits) : ((tbits << 32) | (order + 1)));
                 }
             }
         }
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        wave_lds_sync();
         const unsigned long long k = L.keys[lane];
-        if (k != ~0ull && inlist) {
-            const uint32_t ib = ANY ? uint32_t(k >> 32) : uint32_t(k);
+        if (k != ~0ull) {
+            const uint32_t order = ANY ? uint32_t(k >> 32) : uint32_t(k);
             const uint32_t tb = ANY ? uint32_t(k) : uint32_t(k >> 32);
-            const float4* p = rec + 3 * int64_t(L.items[ib] >> 6);
+            const int32_t j = int32_t(L.list[order >> 4] & 15u);
+            const float4* p = rec + 3 * int64_t((st.leaf(j) >> 4) + int32_t(order & 15u));
             tmax = __uint_as_float(tb);
             tri = __float_as_int(p[0].w);
             mesh = __float_as_int(p[1].w);
             hit_ent = ent;
-            if (ANY) {   // the first accepted triangle ends an any-hit walk
-                walk_done = true;
-                np = 0;
-                parked = false;
-            }
+            if (ANY) walk_done = true;   // the first accepted triangle ends an any-hit walk
         }
-        inlist = false;
-        nitems = 0;
+        nq = 0;
+        wave_lds_sync();   // the list and keys are reused by the next phase
     }
 
     // One round of the wave: node steps, leaf phase, walk ends.
     __device__ __forceinline__ void round(const SceneArgs& s, const crt_ray* __restrict__ rays,
                                           crt_hit* __restrict__ hits, uint32_t flags, int32_t max_iter,
-                                          uint32_t* __restrict__ tri_tests, Stack<LDS_DEPTH>& st,
-                                          WaveLds<LIST>& L) {
+                                          uint32_t* __restrict__ tri_tests, Stack<LDS_DEPTH>& st, WaveLds<Q>& L) {
         for (int k = 0; k < INNER; ++k) {
-            if (active && !walk_done && !parked) {
+            if (can_step()) {
                 if (!step(s, max_iter, st)) walk_done = true;
             }
-            append(st, L);
-            const unsigned long long stepping = __ballot(active && !walk_done && !parked);
-            if (stepping == 0 || nitems >= LIST_TRIGGER) break;
+            if (!__ballot(can_step())) break;
         }
-        if (nitems > 0) {
-            leaf_phase(s, L);
-            append(st, L);   // parked lanes' leaves go first into the empty list
-        }
-        // a walk ends once it has no step left and no triangle waiting
-        const bool ends = active && walk_done && !parked && !inlist;
+        if (__ballot(nq > 0)) leaf_phase(s, st, L);
+        // every queue is empty here: a walk ends once it has no step left
+        const bool ends = active && walk_done;
         if (__ballot(ends)) {
             if (ends) {
                 visits += uint32_t(it);
@@ -383,11 +407,10 @@ struct SpecWave {
 
     __device__ __forceinline__ void run(const SceneArgs& s, const crt_ray* __restrict__ rays,
                                         crt_hit* __restrict__ hits, int64_t i, uint32_t flags, int32_t max_iter,
-                                        uint32_t* __restrict__ tri_tests, Stack<LDS_DEPTH> st, WaveLds<LIST>& L) {
+                                        uint32_t* __restrict__ tri_tests, Stack<LDS_DEPTH> st, WaveLds<Q>& L) {
         active = false;
-        walk_done = parked = inlist = false;
-        np = 0;
-        nitems = 0;
+        walk_done = false;
+        nq = 0;
         if (i >= 0) begin_ray(s, rays, hits, i, flags, tri_tests);
         while (__ballot(active)) round(s, rays, hits, flags, max_iter, tri_tests, st, L);
     }
@@ -397,12 +420,11 @@ struct SpecWave {
     __device__ __forceinline__ void run_refill(const SceneArgs& s, const crt_ray* __restrict__ rays,
                                                crt_hit* __restrict__ hits, int64_t n, uint32_t flags,
                                                int32_t max_iter, uint32_t* __restrict__ tri_tests,
-                                               Stack<LDS_DEPTH> st, WaveLds<LIST>& L,
+                                               Stack<LDS_DEPTH> st, WaveLds<Q>& L,
                                                unsigned long long* __restrict__ counters, int32_t refill) {
         active = false;
-        walk_done = parked = inlist = false;
-        np = 0;
-        nitems = 0;
+        walk_done = false;
+        nq = 0;
         ShardQueue q;
         q.init(counters, n);
         while (true) {

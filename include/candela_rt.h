@@ -368,8 +368,9 @@ enum {
                                    order (stackless: + node window; wide formats:
                                    leaves of a node parked and tested by the wave) */
     CRT_KERNEL_SPECULATIVE = 4, /* wide formats only: one ray per lane, lanes never
-                                   wait for leaf tests; the wave tests the queued
-                                   triangles 64 at a time between node steps, with
+                                   wait for leaf tests: hit leaves go to a per-lane
+                                   queue, and after a round of node steps the wave
+                                   tests the queued leaves 64 at a time, with
                                    culling by the tmax of the last such phase.
                                    Grid launch, 64 consecutive rays per wave: the
                                    records are deterministic; nearest t and hit /

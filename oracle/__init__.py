@@ -94,7 +94,7 @@ def lib() -> C.CDLL:
         l.oracle_trace_wide.argtypes = [i32, i32, i64, vp, vp, vp, C.c_int, u32, vp, vp, i64, C.c_int, vp, i32, i32]
         l.oracle_trace_wide_spec.restype = C.c_int
         l.oracle_trace_wide_spec.argtypes = [i32, i32, i64, vp, vp, vp, C.c_int, u32, vp, vp, i64, C.c_int, vp, i32,
-                                             i32, i32, i32, i32]
+                                             i32, i32, i32]
         l.oracle_bruteforce_closest.restype = C.c_int
         l.oracle_bruteforce_closest.argtypes = [vp, vp, vp, i64, i64, vp, vp, vp, i64]
         l.oracle_primary_rays.restype = None
@@ -164,18 +164,21 @@ def trace_wide(width: int, groups: np.ndarray, records: np.ndarray, ents: np.nda
     return (hits, tests) if with_tests else hits
 
 
-# Schedules of libcandela_rt's speculative wide kernels by variant: (LDS stack entries, inner node steps,
-# leaf-phase trigger, list capacity).
-SPEC_PRESETS = {0: (8, 4, 64, 256), 1: (8, 4, 128, 256), 2: (8, 8, 128, 256), 3: (8, 4, 64, 256),
-                4: (8, 4, 128, 256), 5: (8, 2, 64, 256)}
+# Schedules of libcandela_rt's speculative wide kernels by variant: (LDS stack entries, inner node steps per round,
+# queue slots per lane over the BVH4; the BVH8 kernels use 4 more, at most 16).
+SPEC_PRESETS = {0: (8, 4, 8), 1: (8, 8, 8), 2: (8, 16, 8), 3: (8, 4, 8), 4: (8, 8, 8), 5: (8, 16, 8)}
+
+
+def spec_queue_slots(width: int, q4: int) -> int:
+    return q4 if width == 4 else min(q4 + 4, 16)
 
 
 def trace_wide_spec(width: int, groups: np.ndarray, records: np.ndarray, ents: np.ndarray, query: int,
                     rays: np.ndarray, variant: int = 0, flags: int = 0, threads: int = 0, max_iter: int = 0,
                     with_tests: bool = False, schedule=None):
     """Host twin of the speculative wide kernel's grid launch (64 consecutive rays per wave), replaying the
-    wave schedule of preset `variant` (or `schedule` = (lds_depth, inner, trigger, list))."""
-    depth, inner, trigger, lst = schedule if schedule is not None else SPEC_PRESETS[variant]
+    wave schedule of preset `variant` (or `schedule` = (lds_depth, inner, queue slots over the BVH4))."""
+    depth, inner, q4 = schedule if schedule is not None else SPEC_PRESETS[variant]
     groups = np.ascontiguousarray(groups, WIDE_GROUP_DTYPE)
     records = np.ascontiguousarray(records, TRI_RECORD_DTYPE)
     ents = np.ascontiguousarray(ents, ENTITY_DTYPE)
@@ -188,7 +191,7 @@ def trace_wide_spec(width: int, groups: np.ndarray, records: np.ndarray, ents: n
     rc = lib().oracle_trace_wide_spec(width, ents.size, nodes, _p(groups), _p(records), _p(ents), query, flags,
                                       _p(rays), _p(hits), rays.size, threads,
                                       _p(tests) if tests is not None else None, max_iter, depth + 64, inner,
-                                      trigger, lst)
+                                      spec_queue_slots(width, q4))
     if rc != 0:
         raise RuntimeError("oracle_trace_wide_spec failed")
     return (hits, tests) if with_tests else hits

@@ -786,19 +786,19 @@ void oracle_primary_rays(int width, int height, const float* inv_view, const flo
  * launch: rays [64 w, 64 w + 64) share wave w).  The wave's schedule is
  * replayed exactly, so hit records, visits and tests equal the kernel's:
  *   round      up to `inner` node-step iterations; in each, every active lane
- *              that is neither done nor parked takes one step, then the
- *              wave appends the waiting leaves of all lanes in lane order
- *              (a lane whose triangles do not fit the list stays parked); the
- *              iterations stop early when no lane can step or `trigger` items
- *              wait; then, with items waiting, the leaf phase and a second
- *              append; then every lane whose walk is over, with nothing
- *              parked or waiting, goes to its next entity or writes its hit;
+ *              whose walk goes on and whose queue has room for `width` more
+ *              leaves takes one step; the iterations stop early once no lane
+ *              can step; then, when some queue holds a leaf, the leaf phase;
+ *              then every lane whose walk is over goes to its next entity or
+ *              writes its hit;
  *   step       pop (culling by the current tmax), guards, node: hit leaves
- *              are kept for the list, every hit inner child but the nearest
- *              is pushed and the walk goes on to the nearest;
- *   leaf phase each owner keeps the accepted test (0 < t < its tmax) with the
- *              smallest (t, item) -- any hit: (item, t) -- and ends an any-hit
- *              walk.
+ *              are appended to the lane's queue, every hit inner child but
+ *              the nearest is pushed and the walk goes on to the nearest;
+ *   leaf phase the queued leaves of all lanes in lane order, then queue
+ *              order, are list entries 0, 1, ...; order = entry * 16 +
+ *              triangle of its leaf; each owner keeps the accepted test
+ *              (0 < t < its tmax) with the smallest (t, order) -- any hit:
+ *              (order, t) -- which ends an any-hit walk; every queue empties.
  * ------------------------------------------------------------------------- */
 #define SPEC_MAX_RECORD (1 << 26)
 
@@ -807,8 +807,8 @@ typedef struct {
     float tmax, ray_tmax;
     int32_t root, count, cur, sp, it, tri, mesh, slot, ent, hit_ent;
     uint32_t tests, status, visits;
-    int32_t np, leaves[8];
-    int active, walk_done, parked, inlist;
+    int32_t nq, queue[16];
+    int active, walk_done;
     int32_t st_node[256];
     float st_t[256];
 } spec_lane_t;
@@ -817,7 +817,7 @@ typedef struct {
     const scene_t* s;
     int any;
     uint32_t flags;
-    int inner, trigger, list;
+    int inner, qcap;
     const crt_ray* rays;
     crt_hit* hits;
     uint32_t* tri_tests;
@@ -842,8 +842,8 @@ static int spec_begin_walk(const spec_job_t* J, spec_lane_t* L) {
     L->cur = L->root;
     L->sp = 0;
     L->it = 0;
-    L->np = 0;
-    L->walk_done = L->parked = L->inlist = 0;
+    L->nq = 0;
+    L->walk_done = 0;
     return 1;
 }
 
@@ -878,6 +878,11 @@ static void spec_finish(const spec_job_t* J, spec_lane_t* L) {
     if (J->tri_tests) J->tri_tests[L->slot] = L->tests;
 }
 
+static int spec_can_step(const spec_job_t* J, const spec_lane_t* L) {
+    const int width = J->s->format == CRT_FORMAT_WIDE8 ? 8 : 4;
+    return L->active && !L->walk_done && L->nq <= J->qcap - width;
+}
+
 static int spec_step(const spec_job_t* J, spec_lane_t* L) {
     const scene_t* s = J->s;
     const int G = s->format == CRT_FORMAT_WIDE8 ? 2 : 1;
@@ -904,6 +909,7 @@ static int spec_step(const spec_job_t* J, spec_lane_t* L) {
     L->it++;
     float ct[8];
     int32_t cn[8];
+    int32_t leaves[8];
     int nl = 0;
     const float tgrow = L->tmax * WIDE_GROW;
     for (int j = 0; j < 8; ++j) {
@@ -924,7 +930,7 @@ static int spec_step(const spec_job_t* J, spec_lane_t* L) {
             int hit = cnt >= 0 && t1 >= t0 && t1 > 0.0f;
             if (hit && cnt > 0) {
                 if (chi >= 0 && chi + cnt <= SPEC_MAX_RECORD)
-                    L->leaves[nl++] = (chi << 4) | cnt;
+                    leaves[nl++] = (chi << 4) | cnt;
                 else
                     L->status |= CRT_STATUS_BAD_INDEX;
             }
@@ -936,7 +942,7 @@ static int spec_step(const spec_job_t* J, spec_lane_t* L) {
     int m = 0;
     for (int j = 0; j < 8; ++j) m += cn[j] >= 0;
     int pushes = m > 0 ? m - 1 : 0;
-    if (L->sp + pushes > s->stack_cap) {
+    if (L->sp + pushes > s->stack_cap) {   /* the node's leaves are dropped with the walk */
         L->status |= CRT_STATUS_STACK_OVERFLOW;
         return 0;
     }
@@ -956,86 +962,69 @@ static int spec_step(const spec_job_t* J, spec_lane_t* L) {
             L->sp++;
         }
     L->cur = cn[0];
-    L->np = nl;
+    for (int j = 0; j < nl; ++j) L->queue[L->nq++] = leaves[j];
     return 1;
 }
 
-/* Appends the waiting leaves of every lane in lane order while they fit. */
-static void spec_append(const spec_job_t* J, spec_lane_t* lanes, uint32_t* items, int32_t* nitems) {
-    int32_t room = J->list - *nitems, excl = 0, stop = 0;
+/* Tests the queued leaves of the wave in list order and gives each owner its
+ * winner; every queue is empty afterwards. */
+static void spec_leaf_phase(const spec_job_t* J, spec_lane_t* lanes, uint16_t* list) {
+    const scene_t* s = J->s;
+    int32_t total = 0;
     for (int l = 0; l < 64; ++l) {
         spec_lane_t* L = &lanes[l];
-        int32_t n = 0;
-        for (int j = 0; j < L->np; ++j) n += L->leaves[j] & 0xF;
-        if (n == 0) continue;
-        if (!stop && excl + n <= room) {
-            int32_t k = *nitems + excl;
-            for (int j = 0; j < L->np; ++j) {
-                int32_t first = L->leaves[j] >> 4, len = L->leaves[j] & 0xF;
-                for (int32_t x = 0; x < len; ++x) items[k++] = ((uint32_t)(first + x) << 6) | (uint32_t)l;
-            }
-            L->tests += (uint32_t)n;
-            L->np = 0;
-            L->parked = 0;
-            L->inlist = 1;
-            excl += n;
-        } else {
-            stop = 1;   /* later lanes' exclusive sums only grow */
-            L->parked = 1;
+        for (int j = 0; j < L->nq; ++j) {
+            list[total++] = (uint16_t)((l << 4) | j);
+            L->tests += (uint32_t)(L->queue[j] & 0xF);
         }
     }
-    *nitems += excl;
-}
-
-static void spec_leaf_phase(const spec_job_t* J, spec_lane_t* lanes, const uint32_t* items, int32_t* nitems) {
-    const scene_t* s = J->s;
     uint64_t keys[64];
     for (int l = 0; l < 64; ++l) keys[l] = ~0ull;
-    for (int32_t idx = 0; idx < *nitems; ++idx) {
-        uint32_t item = items[idx];
-        spec_lane_t* O = &lanes[item & 63u];
-        const crt_tri_record* r = &s->recs[item >> 6];
-        float t = tri_t(O->ro, O->rd, mk(r->v0[0], r->v0[1], r->v0[2]), mk(r->v1[0], r->v1[1], r->v1[2]),
-                        mk(r->v2[0], r->v2[1], r->v2[2]));
-        if (t > 0.0f && t < O->tmax) {
-            uint32_t tb;
-            memcpy(&tb, &t, 4);
-            uint64_t key = J->any ? (((uint64_t)(uint32_t)idx << 32) | tb) : (((uint64_t)tb << 32) | (uint32_t)idx);
-            if (key < keys[item & 63u]) keys[item & 63u] = key;
+    for (int32_t idx = 0; idx < total; ++idx) {
+        const int owner = list[idx] >> 4;
+        spec_lane_t* O = &lanes[owner];
+        const int32_t leaf = O->queue[list[idx] & 15];
+        const int32_t first = leaf >> 4, len = leaf & 0xF;
+        for (int32_t x = 0; x < len; ++x) {
+            const crt_tri_record* r = &s->recs[first + x];
+            float t = tri_t(O->ro, O->rd, mk(r->v0[0], r->v0[1], r->v0[2]), mk(r->v1[0], r->v1[1], r->v1[2]),
+                            mk(r->v2[0], r->v2[1], r->v2[2]));
+            if (t > 0.0f && t < O->tmax) {
+                uint32_t tb;
+                memcpy(&tb, &t, 4);
+                const uint64_t order = ((uint64_t)(uint32_t)idx << 4) | (uint32_t)x;
+                uint64_t key = J->any ? ((order << 32) | tb) : (((uint64_t)tb << 32) | order);
+                if (key < keys[owner]) keys[owner] = key;
+            }
         }
     }
     for (int l = 0; l < 64; ++l) {
         spec_lane_t* L = &lanes[l];
-        if (keys[l] != ~0ull && L->inlist) {
-            uint32_t ib = J->any ? (uint32_t)(keys[l] >> 32) : (uint32_t)keys[l];
-            uint32_t tb = J->any ? (uint32_t)keys[l] : (uint32_t)(keys[l] >> 32);
-            const crt_tri_record* r = &s->recs[items[ib] >> 6];
+        if (keys[l] != ~0ull) {
+            const uint32_t order = J->any ? (uint32_t)(keys[l] >> 32) : (uint32_t)keys[l];
+            const uint32_t tb = J->any ? (uint32_t)keys[l] : (uint32_t)(keys[l] >> 32);
+            const int32_t leaf = L->queue[list[order >> 4] & 15];
+            const crt_tri_record* r = &s->recs[(leaf >> 4) + (int32_t)(order & 15u)];
             memcpy(&L->tmax, &tb, 4);
             L->tri = r->tri;
             L->mesh = r->mesh;
             L->hit_ent = L->ent;
-            if (J->any) {
-                L->walk_done = 1;
-                L->np = 0;
-                L->parked = 0;
-            }
+            if (J->any) L->walk_done = 1;
         }
-        L->inlist = 0;
     }
-    *nitems = 0;
+    for (int l = 0; l < 64; ++l) lanes[l].nq = 0;
 }
 
 static void* spec_run_job(void* arg) {
     const spec_job_t* J = (const spec_job_t*)arg;
     spec_lane_t* lanes = (spec_lane_t*)malloc(sizeof(spec_lane_t) * 64);
-    uint32_t* items = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)J->list);
-    if (!lanes || !items) {
+    uint16_t* list = (uint16_t*)malloc(sizeof(uint16_t) * 64 * 16);
+    if (!lanes || !list) {
         free(lanes);
-        free(items);
+        free(list);
         return NULL;
     }
     for (int64_t w = J->wave_begin; w < J->wave_end; ++w) {
-        int32_t nitems = 0;
         for (int l = 0; l < 64; ++l) {
             spec_lane_t* L = &lanes[l];
             memset(L, 0, offsetof(spec_lane_t, st_node));
@@ -1060,21 +1049,19 @@ static void* spec_run_job(void* arg) {
             for (int k = 0; k < J->inner; ++k) {
                 for (int l = 0; l < 64; ++l) {
                     spec_lane_t* L = &lanes[l];
-                    if (L->active && !L->walk_done && !L->parked)
+                    if (spec_can_step(J, L))
                         if (!spec_step(J, L)) L->walk_done = 1;
                 }
-                spec_append(J, lanes, items, &nitems);
                 int stepping = 0;
-                for (int l = 0; l < 64; ++l) stepping |= lanes[l].active && !lanes[l].walk_done && !lanes[l].parked;
-                if (!stepping || nitems >= J->trigger) break;
+                for (int l = 0; l < 64; ++l) stepping |= spec_can_step(J, &lanes[l]);
+                if (!stepping) break;
             }
-            if (nitems > 0) {
-                spec_leaf_phase(J, lanes, items, &nitems);
-                spec_append(J, lanes, items, &nitems);
-            }
+            int queued = 0;
+            for (int l = 0; l < 64; ++l) queued |= lanes[l].nq > 0;
+            if (queued) spec_leaf_phase(J, lanes, list);
             for (int l = 0; l < 64; ++l) {
                 spec_lane_t* L = &lanes[l];
-                if (L->active && L->walk_done && !L->parked && !L->inlist) {
+                if (L->active && L->walk_done) {
                     L->visits += (uint32_t)L->it;
                     int fin = J->any && L->tri >= 0;
                     if (!fin) fin = !spec_begin_walk(J, L);
@@ -1087,21 +1074,21 @@ static void* spec_run_job(void* arg) {
         }
     }
     free(lanes);
-    free(items);
+    free(list);
     return NULL;
 }
 
 /* Schedule parameters of the kernel's preset: stack_cap = its LDS stack
- * entries + 64, inner node steps per round, the leaf-phase trigger and the
- * list capacity (items). */
+ * entries + 64, inner node steps per round and the queue slots per lane
+ * (width..16). */
 int oracle_trace_wide_spec(int32_t width, int32_t entity_count, int64_t total_nodes, const crt_wide_group* groups,
                            const crt_tri_record* records, const crt_entity* ents, int query, uint32_t flags,
                            const crt_ray* rays, crt_hit* hits, int64_t n, int threads, uint32_t* tri_tests,
-                           int32_t max_iter, int32_t stack_cap, int32_t inner, int32_t trigger, int32_t list) {
+                           int32_t max_iter, int32_t stack_cap, int32_t inner, int32_t queue_slots) {
     scene_t s;
     memset(&s, 0, sizeof(s));
-    if ((width != 4 && width != 8) || inner < 1 || trigger < 1 || list < trigger || list < width * 15 ||
-        stack_cap < 1 || stack_cap > 248)
+    if ((width != 4 && width != 8) || inner < 1 || queue_slots < width || queue_slots > 16 || stack_cap < 1 ||
+        stack_cap > 248)
         return -1;
     s.format = width == 8 ? CRT_FORMAT_WIDE8 : CRT_FORMAT_WIDE4;
     s.entity_count = entity_count;
@@ -1124,8 +1111,7 @@ int oracle_trace_wide_spec(int32_t width, int32_t entity_count, int64_t total_no
         J->any = query == CRT_QUERY_ANY;
         J->flags = flags;
         J->inner = inner;
-        J->trigger = trigger;
-        J->list = list;
+        J->qcap = queue_slots;
         J->rays = rays;
         J->hits = hits;
         J->tri_tests = tri_tests;

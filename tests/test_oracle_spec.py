@@ -5,16 +5,17 @@ as the sequential walk's, so a closest-hit walk must find the nearest t the
 sequential wide walk (oracle_trace_wide) finds, on every ray; the triangle may
 differ only where both walks report the same t (an exact tie between
 triangles), and then it is one the sequential walk can reach; any-hit walks
-must agree on occlusion.  Checked over every preset schedule and over tiny
-lists that force lanes to wait for room, on clouds with random rays, on
-instanced entities with IgnoreTransparent, and against brute force.
+must agree on occlusion.  Checked over every preset schedule and over the
+smallest queues (a lane that queued any leaf waits for the leaf phase), on
+clouds with random rays, on instanced entities with IgnoreTransparent, and
+against brute force.
 """
 import numpy as np
 import pytest
 
 from oracle_cases import axis_aligned_rays, random_rays
 
-SCHEDULES = [None, (8, 4, 8, 120), (8, 1, 1, 120), (8, 16, 256, 512)]
+SCHEDULES = [None, (8, 4, 4), (8, 1, 4), (8, 16, 16)]
 
 
 def _cloud(crt, orc, width):
