@@ -80,24 +80,25 @@ spec_persistent(SceneArgs s, const crt_ray* __restrict__ rays, crt_hit* __restri
 }  // namespace sp
 
 // Presets of the grid walk (variant, LDS stack entries, INNER node steps per
-// round, Q queue slots per lane (BVH4), OCC waves per SIMD).
+// round, Q queue slots per lane (BVH4), OCC waves per SIMD).  Preset 5 keeps
+// the workgroup's LDS under 26 KiB so that six workgroups share a CU.
 #define CRT_SPEC_PRESETS(X) \
     X(0, 8, 4, 8, 0)        \
     X(1, 8, 8, 8, 0)        \
-    X(2, 8, 16, 8, 0)       \
+    X(2, 8, 2, 8, 5)        \
     X(3, 8, 4, 8, 5)        \
     X(4, 8, 8, 8, 5)        \
-    X(5, 8, 16, 8, 5)
+    X(5, 7, 6, 6, 6)
 
 // Presets of the refilling walk (variant, LDS stack entries, INNER, Q,
 // REFILL idle lanes, OCC).
 #define CRT_SPEC_P_PRESETS(X)   \
     X(0, 8, 4, 8, 16, 0)        \
     X(1, 8, 8, 8, 16, 0)        \
-    X(2, 8, 8, 8, 8, 0)         \
-    X(3, 8, 4, 8, 16, 5)        \
+    X(2, 8, 8, 8, 8, 5)         \
+    X(3, 8, 6, 8, 16, 5)        \
     X(4, 8, 8, 8, 16, 5)        \
-    X(5, 8, 16, 8, 16, 5)
+    X(5, 7, 8, 6, 16, 6)
 
 template <bool ANY, int W>
 static const void* spec_grid_fn(int variant) {

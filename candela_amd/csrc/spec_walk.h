@@ -216,14 +216,13 @@ struct SpecWave {
         const float e0 = mx(mx(mn(fx, nx), mx(mn(fy, ny), mn(fz, nz))), 0.0f);                              \
         const int32_t cnt = __float_as_int(CN), chi = __float_as_int(CH);                                   \
         const bool hit = cnt >= 0 && e1 >= e0 && e1 > 0.0f;                                                 \
-        if (hit && cnt > 0) {                                                                               \
-            if (chi >= 0 && chi + cnt <= kMaxRecord) {                                                      \
-                st.put_leaf(nq + nl, (chi << 4) | cnt);                                                     \
-                ++nl;                                                                                       \
-            } else {                                                                                        \
-                status |= CRT_STATUS_BAD_INDEX;   /* not representable in a queue slot */                   \
-            }                                                                                               \
-        }                                                                                                   \
+        /* the next free queue slot is written whatever the test says; only a */                            \
+        /* hit leaf whose records fit a slot's packing advances the count      */                            \
+        const bool take = hit && cnt > 0;                                                                   \
+        const bool fits = chi >= 0 && chi <= kMaxRecord - cnt;                                              \
+        st.put_leaf(nq + nl, (chi << 4) | cnt);                                                             \
+        nl += take && fits ? 1 : 0;                                                                         \
+        status |= take && !fits ? uint32_t(CRT_STATUS_BAD_INDEX) : 0u;                                      \
         const bool inner = hit && cnt == 0;                                                                 \
         T = inner ? e0 : inf;                                                                               \
         N = inner ? chi : -1;                                                                               \

@@ -196,10 +196,10 @@ struct Core {
         const int32_t cnt = __float_as_int(CN), chi = __float_as_int(CH);                                   \
         /* e0 >= 0, so (e0 > 0 || e1 > 0) given e1 >= e0 is e1 > 0 */                                      \
         const bool hit = cnt >= 0 && e1 >= e0 && e1 > 0.0f;                                                 \
-        if (hit && cnt > 0) {                                                                               \
-            st.put_leaf(np, (chi << 4) | cnt);                                                              \
-            ++np;                                                                                           \
-        }                                                                                                   \
+        /* the next free pending slot is written whatever the test says; a */                               \
+        /* hit leaf advances the count                                      */                               \
+        st.put_leaf(np, (chi << 4) | cnt);                                                                  \
+        np += hit && cnt > 0 ? 1 : 0;                                                                       \
         const bool inner = hit && cnt == 0;                                                                 \
         T = inner ? e0 : inf;                                                                               \
         N = inner ? chi : -1;                                                                               \

@@ -166,7 +166,7 @@ def trace_wide(width: int, groups: np.ndarray, records: np.ndarray, ents: np.nda
 
 # Schedules of libcandela_rt's speculative wide kernels by variant: (LDS stack entries, inner node steps per round,
 # queue slots per lane over the BVH4; the BVH8 kernels use 4 more, at most 16).
-SPEC_PRESETS = {0: (8, 4, 8), 1: (8, 8, 8), 2: (8, 16, 8), 3: (8, 4, 8), 4: (8, 8, 8), 5: (8, 16, 8)}
+SPEC_PRESETS = {0: (8, 4, 8), 1: (8, 8, 8), 2: (8, 2, 8), 3: (8, 4, 8), 4: (8, 8, 8), 5: (7, 6, 6)}
 
 
 def spec_queue_slots(width: int, q4: int) -> int:

@@ -929,7 +929,7 @@ static int spec_step(const spec_job_t* J, spec_lane_t* L) {
             int32_t cnt = gp->count[k], chi = gp->child[k];
             int hit = cnt >= 0 && t1 >= t0 && t1 > 0.0f;
             if (hit && cnt > 0) {
-                if (chi >= 0 && chi + cnt <= SPEC_MAX_RECORD)
+                if (chi >= 0 && chi <= SPEC_MAX_RECORD - cnt)
                     leaves[nl++] = (chi << 4) | cnt;
                 else
                     L->status |= CRT_STATUS_BAD_INDEX;
