@@ -150,8 +150,9 @@ def measured_traffic(best: dict, args, fmt: int):
             d = json.load(f)
         if d.get("kernel_choice") == choice and d.get("workload") == "cloud2000_s3_gi" and args.spheres == 2000 \
                 and fmt == 0:
-            return float(d["traffic_bytes"]), os.path.relpath(path, ROOT)
-    return None, None
+            extra = {k: d[k] for k in ("l2_hit_rate", "valu_issue_frac") if k in d}
+            return float(d["traffic_bytes"]), os.path.relpath(path, ROOT), extra
+    return None, None, {}
 
 def time_batched(crt, batches, steps: int, warmup: int, stream, wide: bool) -> dict:
     """Autotunes the batched kernel on `batches`, then times `steps` hipGraph replays of the best plan."""
@@ -213,14 +214,32 @@ def batched_vs_plain(crt, args, plain_choice: dict, stream) -> dict:
         ev.record(1, stream.handle)
         plain_ms = ev.ms(0, 1) / K
         ev.close()
+        timed_hits = [[s.h_closest.download(), s.h_shadow.download()] for s in work.scenes]
+        schedule_dep = {crt.KERNEL_SPECULATIVE, crt.KERNEL_SPECULATIVE_REFILL}
+        if kernel in schedule_dep or ao_kernel in schedule_dep:
+            # the speculative walks' visit counts (and exact-tie triangles) follow their wave schedule: the
+            # batched records are checked byte for byte against plain launches of the wave walk they share
+            for s in work.scenes:
+                sc = s.scene_for(layout)
+                crt.trace(sc, crt.QUERY_CLOSEST, s.d_closest, s.h_closest, kernel=crt.KERNEL_WAVE, stream=stream)
+                crt.trace(sc, crt.QUERY_ANY, s.d_shadow, s.h_shadow, kernel=crt.KERNEL_WAVE, stream=stream,
+                          flags=kw_any["flags"])
+            stream.sync()
         plain_hits = [[s.h_closest.download(), s.h_shadow.download()] for s in work.scenes]
         b = time_batched(crt, work.batches(crt, layout), K, args.warmup, stream, wide=layout is not None)
         batched_hits = [(s.h_closest.download(), s.h_shadow.download()) for s in work.scenes]
+        # the timed plain kernel against the batched records: nearest t and occlusion
+        equivalent = all(np.array_equal(bh[0]["t"], th[0]["t"]) and
+                         np.array_equal(bh[1]["t"] > 0, th[1]["t"] > 0)
+                         for bh, th in zip(batched_hits, timed_hits))
         if layout is not None and b["config"]["child_order"] != ao_order:
             # the batched any-hit walks took the other child order: compare with plain shadow hits in that order
             f = crt.FLAG_SLOT_ORDER if b["config"]["child_order"] == 1 else 0
+            kw_ref = dict(kw_any, flags=f)
+            if ao_kernel in schedule_dep:
+                kw_ref.update(kernel=crt.KERNEL_WAVE, variant=0)
             for s in work.scenes:
-                crt.trace(s.scene_for(layout), crt.QUERY_ANY, s.d_shadow, s.h_shadow, **dict(kw_any, flags=f))
+                crt.trace(s.scene_for(layout), crt.QUERY_ANY, s.d_shadow, s.h_shadow, **kw_ref)
             stream.sync()
             for s, ph in zip(work.scenes, plain_hits):
                 ph[1] = s.h_shadow.download()
@@ -234,7 +253,7 @@ def batched_vs_plain(crt, args, plain_choice: dict, stream) -> dict:
                             "batched_ms": round(b["ms"], 4),
                             "batched_mrays_s": round(work.rays / b["ms"] / 1e3, 2),
                             "speedup": round(plain_ms / b["ms"], 3), "batched_config": b["config"],
-                            "hits_identical": bool(same)}
+                            "hits_identical": bool(same), "timed_plain_t_and_occlusion_equal": bool(equivalent)}
         log(f"{name}: plain {2 * len(work.scenes)} launches {plain_ms:.3f} ms ({work.rays / plain_ms / 1e3:.1f} "
             f"Mrays/s), batched one launch {b['ms']:.3f} ms ({work.rays / b['ms'] / 1e3:.1f} Mrays/s) -> "
             f"{plain_ms / b['ms']:.2f}x; identical hits: {same}")
@@ -260,6 +279,8 @@ def batched_vs_plain(crt, args, plain_choice: dict, stream) -> dict:
             "speedup": round(best_plain[1]["plain_ms"] / best_batched[1]["batched_ms"], 3),
             "speedup_same_layout": {k: v["speedup"] for k, v in per_layout.items()},
             "hits_identical": all(v["hits_identical"] for v in per_layout.values()),
+            "timed_plain_t_and_occlusion_equal": all(v["timed_plain_t_and_occlusion_equal"]
+                                                     for v in per_layout.values()),
             "library_autotuner": lib_choice,
             "layouts": per_layout}
 
@@ -674,12 +695,14 @@ def main(argv=None) -> int:
                  if k in ("layout", "width", "max_leaf", "kernel", "variant", "lds", "block")}
         mixed.update(ao_kernel=fastest_ao["kernel"], ao_variant=fastest_ao["variant"])
         results.append(run(mixed))
-    # child order of the AO walks: the fastest AO choice of the wide layouts again with slot order
+    # child order of the AO walks: the fastest AO choice of the wide layouts again with slot order (the
+    # speculative kernels always walk nearest first, so they have no second order to time)
     if not args.only and fastest_gi["layout"] != "bvh2":
         base = min((r for r in results if scene_of(r) is scene_of(fastest_gi)), key=lambda r: r["ms_step"])
-        results.append(run(dict({k: v for k, v in base.items() if k in (
-            "layout", "width", "max_leaf", "kernel", "variant", "lds", "block", "ao_kernel", "ao_variant")},
-            ao_order=1)))
+        if ao_of(base)[0] not in (crt.KERNEL_SPECULATIVE, crt.KERNEL_SPECULATIVE_REFILL):
+            results.append(run(dict({k: v for k, v in base.items() if k in (
+                "layout", "width", "max_leaf", "kernel", "variant", "lds", "block", "ao_kernel", "ao_variant")},
+                ao_order=1)))
     thread = results[0]
     best = max(results, key=lambda r: r["mrays"])
     best_scene = scene_of(best)
@@ -706,7 +729,7 @@ def main(argv=None) -> int:
                          hit_rate=float(np.mean(hits["t"] > 0)))
     gi_s = best["gi_ms"] * 1e-3
     achieved = out["gi"]["bytes"] / gi_s / 1e9
-    traffic, traffic_src = measured_traffic(best, args, fmt)
+    traffic, traffic_src, counters = measured_traffic(best, args, fmt)
     log(f"GI: {out['gi']['visits']:.1f} node visits, {out['gi']['tests']:.1f} tri tests per ray, "
         f"{out['gi']['bytes'] / n_gi:.0f} B/ray, {achieved:.0f} GB/s algorithmic; "
         f"iteration-capped {out['gi']['capped'] * 100:.2f}%")
@@ -968,7 +991,9 @@ def main(argv=None) -> int:
                 "bytes_per_launch": out["gi"]["bytes"], "bytes_per_ray": round(out["gi"]["bytes"] / n_gi, 1),
                 "traffic_source": traffic_src}
     if traffic:
+        # algorithmic bytes price every node and record fetch as HBM bytes; the caches serve most of them
         roofline["traffic_frac_of_peak"] = round(traffic / gi_s / 1e9 / HBM_PEAK_GBS, 4)
+        roofline.update(counters)
     full = {
         "kernels": [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()} for r in results],
         "kernel_label": kernel_label(best),
@@ -1015,7 +1040,8 @@ def main(argv=None) -> int:
         "batched_mrays_s": (round(rays_step / (c2_batched["ms"] * 1e-3) / 1e6, 2) if c2_batched else None),
         "batched_vs_plain": pick(batched, ("plain_launches", "plain_ms", "plain_layout", "batched_ms",
                                            "batched_layout", "batched_mrays_s", "speedup", "speedup_same_layout",
-                                           "hits_identical", "library_autotuner")),
+                                           "hits_identical", "timed_plain_t_and_occlusion_equal",
+                                           "library_autotuner")),
         "roofline": roofline,
         "cpu_baseline": cpu,
         "parity": parity,

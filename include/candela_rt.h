@@ -141,8 +141,9 @@ enum {
     CRT_FLAG_TRI0_IS_MISS = 1u << 1,
     /* Child ordering of any-hit walks of the wide layouts (no reference
      * counterpart): the hit children of a node are taken in slot order
-     * instead of nearest first, which saves the sort.  Closest-hit walks and
-     * the BVH2 formats ignore it. */
+     * instead of nearest first, which saves the sort.  Closest-hit walks, the
+     * BVH2 formats and the speculative kernels (always nearest first) ignore
+     * it. */
     CRT_FLAG_SLOT_ORDER = 1u << 2
 };
 

@@ -8,11 +8,16 @@ pixel one cosine RTAO any-hit ray (tmax 2) and one cosine GI closest-hit ray
 traced on the GPU and compared with the host oracle over the same buffers:
 
 - BVH2 kernels (stackless layout): bit-identical to the reference walk;
-- the BVH4 kernels the benchmark picks (refilling walk for GI, grid walk in
-  slot order for RTAO): bit-identical to the host twin of the wide walk, and
-  GI triangle ids agree with the reference walk on at least 99.999 % of rays,
+- the sequential BVH4 walks (refilling walk for GI, grid walk in slot order
+  for RTAO): bit-identical to the host twin of the wide walk, and GI
+  triangle ids agree with the reference walk on at least 99.999 % of rays,
   every mismatch at an exactly equal t (the same distance reached through
   another triangle of a shared edge), RTAO occlusion on every ray;
+- the speculative BVH4 kernels the benchmark picks (refilling speculative
+  walk for GI, speculative grid walk for RTAO): RTAO bit-identical to the
+  host twin that replays the grid schedule; GI nearest t equal to the
+  sequential wide walk's on every ray (its visit counts follow the refill
+  schedule), and the same bar against the reference walk as above;
 - the stack layout (FlattenStackBVH of the same meshes): bit-identical to the
   reference's stack walk.
 """
@@ -98,6 +103,31 @@ def test_config2_bvh4_benchmark_choice(gpu, orc, work, ref):
     assert np.array_equal(g["t"] > 0, rg["t"] > 0)
     assert np.array_equal(a["t"] > 0, ra["t"] > 0), "RTAO occlusion differs from the reference walk"
     # barycentrics within 1e-5 of the reference walk where the triangle agrees
+    same = ~miss & (g["t"] > 0)
+    for k in ("u", "v", "w"):
+        assert np.abs(g[k][same] - rg[k][same]).max() <= 1e-5
+
+
+def test_config2_bvh4_speculative_choice(gpu, orc, work, ref):
+    log("[case] config 2 BVH4: GI refilling speculative preset 4, AO speculative grid preset 5")
+    dw = gpu.upload_wide(gpu.FORMAT_STACKLESS, work.host, 4, 2)
+    w = dw.host
+    g = _trace(gpu, dw.scene, 0, work.gi, gpu.KERNEL_SPECULATIVE_REFILL, 4)
+    a = _trace(gpu, dw.scene, 1, work.ao, gpu.KERNEL_SPECULATIVE, 5)
+    ta = orc.trace_wide_spec(w.width, w.groups, w.records, w.entities, 1, work.ao, variant=5)
+    assert _differs(a, ta) == 0, "AO records differ from the twin of the speculative grid walk"
+    tg = orc.trace_wide(w.width, w.groups, w.records, w.entities, 0, work.gi)
+    assert np.array_equal(g["t"], tg["t"]), "GI nearest t differs from the sequential wide walk"
+    assert (g["status"] >> 24 == 0).all()
+    rg, ra = ref
+    miss = g["tri"] != rg["tri"]
+    agree = 1.0 - miss.mean()
+    log(f"[parity] config 2 speculative BVH4: AO exact vs twin; GI id agreement vs reference walk {agree:.7f} "
+        f"({int(miss.sum())} mismatches, {int((miss & (g['t'] == rg['t'])).sum())} at equal t)")
+    assert agree >= 0.99999
+    assert (g["t"][miss] == rg["t"][miss]).all(), "an id mismatch at a different distance"
+    assert np.array_equal(g["t"] > 0, rg["t"] > 0)
+    assert np.array_equal(a["t"] > 0, ra["t"] > 0), "RTAO occlusion differs from the reference walk"
     same = ~miss & (g["t"] > 0)
     for k in ("u", "v", "w"):
         assert np.abs(g[k][same] - rg[k][same]).max() <= 1e-5
