@@ -173,11 +173,16 @@ def time_batched(crt, batches, steps: int, warmup: int, stream, wide: bool) -> d
     return {"ms": ms, "config": cfg, "tune_ms": tune_ms, "rays": plan.rays}
 
 
+# batched presets 8..11 (the speculative walk per tile) -> crt_trace's CRT_KERNEL_SPECULATIVE presets
+BATCHED_SPEC_PRESET = {8: 0, 9: 1, 10: 3, 11: 4}
+
+
 def default_candidates(crt, wide: bool):
     """Autotuner grid: preset, tiles per claim, waves per group, tile order, queue shards and, over the wide
-    layouts, the child order of the any-hit walks."""
-    return [crt.BatchConfig(v, t, w, 0, o, q, co) for v in (0, 1, 4, 6, 7) for t in (1, 2) for w in (1, 4)
-            for o in (0, 1) for q in (1, 16) for co in ((0, 1) if wide else (0,))]
+    layouts, the speculative walk (presets 8..11) and the child order of the sequential any-hit walks."""
+    presets = (0, 1, 4, 6, 7) + ((8, 9, 10, 11) if wide else ())
+    return [crt.BatchConfig(v, t, w, 0, o, q, co) for v in presets for t in (1, 2) for w in (1, 4)
+            for o in (0, 1) for q in (1, 16) for co in ((0, 1) if wide and v < 8 else (0,))]
 
 
 def batched_vs_plain(crt, args, plain_choice: dict, stream) -> dict:
@@ -215,34 +220,24 @@ def batched_vs_plain(crt, args, plain_choice: dict, stream) -> dict:
         plain_ms = ev.ms(0, 1) / K
         ev.close()
         timed_hits = [[s.h_closest.download(), s.h_shadow.download()] for s in work.scenes]
-        schedule_dep = {crt.KERNEL_SPECULATIVE, crt.KERNEL_SPECULATIVE_REFILL}
-        if kernel in schedule_dep or ao_kernel in schedule_dep:
-            # the speculative walks' visit counts (and exact-tie triangles) follow their wave schedule: the
-            # batched records are checked byte for byte against plain launches of the wave walk they share
-            for s in work.scenes:
-                sc = s.scene_for(layout)
-                crt.trace(sc, crt.QUERY_CLOSEST, s.d_closest, s.h_closest, kernel=crt.KERNEL_WAVE, stream=stream)
-                crt.trace(sc, crt.QUERY_ANY, s.d_shadow, s.h_shadow, kernel=crt.KERNEL_WAVE, stream=stream,
-                          flags=kw_any["flags"])
-            stream.sync()
-        plain_hits = [[s.h_closest.download(), s.h_shadow.download()] for s in work.scenes]
         b = time_batched(crt, work.batches(crt, layout), K, args.warmup, stream, wide=layout is not None)
         batched_hits = [(s.h_closest.download(), s.h_shadow.download()) for s in work.scenes]
+        # the batched records byte for byte against plain launches of the walk the chosen plan runs: the
+        # speculative grid launch with the matching preset (a tile is one of its waves), else the wave walk
+        v_b = b["config"]["variant"]
+        ref_kw = dict(kernel=crt.KERNEL_SPECULATIVE, variant=BATCHED_SPEC_PRESET[v_b]) if v_b >= 8 else \
+            dict(kernel=crt.KERNEL_WAVE, variant=0)
+        ref_flags = crt.FLAG_SLOT_ORDER if (layout is not None and b["config"]["child_order"] == 1) else 0
+        for s in work.scenes:
+            sc = s.scene_for(layout)
+            crt.trace(sc, crt.QUERY_CLOSEST, s.d_closest, s.h_closest, stream=stream, **ref_kw)
+            crt.trace(sc, crt.QUERY_ANY, s.d_shadow, s.h_shadow, stream=stream, flags=ref_flags, **ref_kw)
+        stream.sync()
+        plain_hits = [[s.h_closest.download(), s.h_shadow.download()] for s in work.scenes]
         # the timed plain kernel against the batched records: nearest t and occlusion
         equivalent = all(np.array_equal(bh[0]["t"], th[0]["t"]) and
                          np.array_equal(bh[1]["t"] > 0, th[1]["t"] > 0)
                          for bh, th in zip(batched_hits, timed_hits))
-        if layout is not None and b["config"]["child_order"] != ao_order:
-            # the batched any-hit walks took the other child order: compare with plain shadow hits in that order
-            f = crt.FLAG_SLOT_ORDER if b["config"]["child_order"] == 1 else 0
-            kw_ref = dict(kw_any, flags=f)
-            if ao_kernel in schedule_dep:
-                kw_ref.update(kernel=crt.KERNEL_WAVE, variant=0)
-            for s in work.scenes:
-                crt.trace(s.scene_for(layout), crt.QUERY_ANY, s.d_shadow, s.h_shadow, **kw_ref)
-            stream.sync()
-            for s, ph in zip(work.scenes, plain_hits):
-                ph[1] = s.h_shadow.download()
         same = all(bh[0].tobytes() == ph[0].tobytes() and bh[1].tobytes() == ph[1].tobytes()
                    for bh, ph in zip(batched_hits, plain_hits))
         name = "bvh2" if layout is None else f"wide{layout[0]}"

@@ -30,6 +30,7 @@
 #include "trace_launch.h"
 #include "wave_walk.h"
 #include "wide_walk.h"
+#include "spec_walk.h"
 #include "thread_walk.h"
 
 namespace crt {
@@ -163,6 +164,58 @@ batched_wide_occ(CRT_BATCH_ARGS) {
     __shared__ int32_t lds[wide::lds_ints<W, LDS_DEPTH>() * 64 * WPG];
     wide_body<W, LDS_DEPTH, INNER, LEAF_BATCH, WPG>(descs, tiles, ntiles, per_claim, counters, max_iter, shards, lds);
 }
+// The speculative wide walk (spec_walk.h) on the same tile queue: a tile is
+// rays [64 k, 64 k + 64) of its batch, exactly one wave of crt_trace's
+// speculative grid launch on that batch, so with the same preset every hit
+// record, counters included, equals crt_trace's kernel 4.
+template <int W, int LDS_DEPTH, int INNER, int Q, int WPG>
+struct SpecGroupLds {
+    int32_t cols[spec::lds_ints<LDS_DEPTH, Q>() * 64 * WPG];
+    spec::WaveLds<Q> waves[WPG];
+};
+
+template <int W, int LDS_DEPTH, int INNER, int Q, int WPG>
+__device__ __forceinline__ void spec_body(const BatchDesc* __restrict__ descs, const int2* __restrict__ tiles,
+                                          int32_t ntiles, int32_t per_claim, unsigned long long* __restrict__ counters,
+                                          int32_t max_iter, int32_t shards,
+                                          SpecGroupLds<W, LDS_DEPTH, INNER, Q, WPG>& lds) {
+    int32_t spill_node[wide::kSpill];
+    float spill_t[wide::kSpill];
+    wide::Stack<LDS_DEPTH> st;
+    st.lds = lds.cols + threadIdx.x;
+    st.stride = 64 * WPG;
+    st.spill_node = spill_node;
+    st.spill_t = spill_t;
+    spec::WaveLds<Q>& wl = lds.waves[threadIdx.x >> 6];
+    ShardQueue q;
+    q.init(counters, ntiles, true, shards);
+    while (true) {
+        int32_t got = 0;
+        const int64_t base = q.claim(per_claim, got);
+        if (got == 0) break;
+        for (int32_t k = 0; k < got; ++k) {
+            const int32_t t = __builtin_amdgcn_readfirstlane(int32_t(q.at(base + k)));
+            const int2 tile = tiles[t];
+            const int32_t b = __builtin_amdgcn_readfirstlane(tile.x);
+            const int32_t first = __builtin_amdgcn_readfirstlane(tile.y);
+            const BatchDesc d = load_desc(descs, b);
+            const int64_t i = int64_t(first) + __lane_id();
+            const int64_t slot = i < d.count ? i : -1;
+            // one walk for both queries (the query is a per-tile value)
+            spec::SpecWave<false, W, LDS_DEPTH, INNER, Q, true> w;
+            w.dyn_any = d.query == CRT_QUERY_ANY;
+            w.run(d.scene, d.rays, d.hits, slot, d.flags, max_iter, d.tri_tests, st, wl);
+        }
+    }
+}
+
+template <int W, int LDS_DEPTH, int INNER, int Q, int WPG, int OCC>
+__global__ void __launch_bounds__(64 * WPG) __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1, 8)))
+batched_spec(CRT_BATCH_ARGS) {
+    __shared__ SpecGroupLds<W, LDS_DEPTH, INNER, Q, WPG> lds;
+    spec_body<W, LDS_DEPTH, INNER, Q, WPG>(descs, tiles, ntiles, per_claim, counters, max_iter, shards, lds);
+}
+
 // The stack format (FlattenStackBVH): each claimed tile runs the one-ray-per-
 // lane stack walk of crt_trace's thread kernel (thread_walk.h), its first
 // LDS_DEPTH entries in LDS; the walk's results do not depend on LDS_DEPTH, so
@@ -231,8 +284,23 @@ static const void* batched_symbol(int variant) {
 
 // Wide presets (LDS stack entries, INNER, LEAF_BATCH) -- the same as the
 // plain wide wave kernel's (wide.hip).
+// Speculative presets 8..11 = crt_trace's speculative grid presets 0, 1, 3
+// and 4 (LDS stack entries, INNER, queue slots, OCC); the BVH8 walk takes 4
+// more queue slots, as crt_trace's does.
+template <int W, int WPG>
+static const void* batched_spec_symbol(int variant) {
+    constexpr int Q = W == 4 ? 8 : 12;
+    switch (variant) {
+        case 9: return reinterpret_cast<const void*>(&pdev::batched_spec<W, 8, 8, Q, WPG, 0>);
+        case 10: return reinterpret_cast<const void*>(&pdev::batched_spec<W, 8, 4, Q, WPG, 5>);
+        case 11: return reinterpret_cast<const void*>(&pdev::batched_spec<W, 8, 8, Q, WPG, 5>);
+        default: return reinterpret_cast<const void*>(&pdev::batched_spec<W, 8, 4, Q, WPG, 0>);
+    }
+}
+
 template <int W, int WPG>
 static const void* batched_wide_symbol(int variant) {
+    if (variant >= 8) return batched_spec_symbol<W, WPG>(variant);
     switch (variant) {
         case 1: return reinterpret_cast<const void*>(&pdev::batched_wide<W, 8, 8, 16, WPG>);
         case 2: return reinterpret_cast<const void*>(&pdev::batched_wide<W, 8, 2, 8, WPG>);
@@ -316,7 +384,8 @@ int hip_err(hipError_t e, const char* what) {
 static_assert(sizeof(crt_batch_config) == 28, "crt_batch_config is 7 int32 fields");
 
 int validate_config(const crt_batch_config& c) {
-    if (c.variant < 0 || c.variant > 7) return crt::set_error(CRT_E_INVALID, "batch config: variant must be 0..7");
+    if (c.variant < 0 || c.variant > 11)
+        return crt::set_error(CRT_E_INVALID, "batch config: variant must be 0..11 (8..11: wide formats only)");
     if (c.tiles_per_claim != 1 && c.tiles_per_claim != 2 && c.tiles_per_claim != 4)
         return crt::set_error(CRT_E_INVALID, "batch config: tiles_per_claim must be 1, 2 or 4");
     if (c.waves_per_group != 1 && c.waves_per_group != 2 && c.waves_per_group != 4)
@@ -352,6 +421,8 @@ int build_plan(const crt_batch* batches, int nbatches, const crt_batch_config& c
     std::vector<crt::BatchDesc> descs(static_cast<size_t>(nbatches));
     const int format = nbatches > 0 ? batches[0].scene.format : CRT_FORMAT_STACKLESS;
     const bool wide = format == CRT_FORMAT_WIDE4 || format == CRT_FORMAT_WIDE8;
+    if (cfg.variant >= 8 && !wide)
+        return crt::set_error(CRT_E_INVALID, "batch config: variants 8..11 (speculative walk) need a wide format");
     for (int i = 0; i < nbatches; ++i) {
         const crt_batch& b = batches[i];
         if (b.scene.format != CRT_FORMAT_STACKLESS && b.scene.format != CRT_FORMAT_STACK &&
@@ -515,7 +586,10 @@ int crt_autotune_batched(const crt_batch* batches, int nbatches, const crt_batch
         for (int i = 0; i < nbatches; ++i)
             orders |= batches[i].query == CRT_QUERY_ANY &&
                       (batches[i].scene.format == CRT_FORMAT_WIDE4 || batches[i].scene.format == CRT_FORMAT_WIDE8);
-        for (int v = 0; v < 8; ++v)
+        bool wide_fmt = nbatches > 0;
+        for (int i = 0; i < nbatches; ++i)
+            wide_fmt &= batches[i].scene.format == CRT_FORMAT_WIDE4 || batches[i].scene.format == CRT_FORMAT_WIDE8;
+        for (int v = 0; v < (wide_fmt ? 12 : 8); ++v)
             for (int t : {1, 2, 4})
                 for (int w : {1, 4})
                     for (int o : {0, 1})
@@ -537,7 +611,11 @@ int crt_autotune_batched(const crt_batch* batches, int nbatches, const crt_batch
     PLAN_HIP(hipEventCreate(&e1), "hipEventCreate");
     float best_t = -1.0f;
     int rc = CRT_OK;
+    bool wide_batches = nbatches > 0;
+    for (int i = 0; i < nbatches; ++i)
+        wide_batches &= batches[i].scene.format == CRT_FORMAT_WIDE4 || batches[i].scene.format == CRT_FORMAT_WIDE8;
     for (const crt_batch_config& c : cands) {
+        if (c.variant >= 8 && !wide_batches) continue;   // the speculative presets walk the wide formats only
         crt_batch_plan* p = nullptr;
         rc = crt_batch_plan_create(batches, nbatches, &c, &p);
         if (rc) break;
@@ -559,6 +637,8 @@ int crt_autotune_batched(const crt_batch* batches, int nbatches, const crt_batch
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (rc) return rc;
+    if (best_t < 0.0f)
+        return crt::set_error(CRT_E_INVALID, "crt_autotune_batched: no candidate applies to these batches");
     *best_ms = best_t;
     return CRT_OK;
 }

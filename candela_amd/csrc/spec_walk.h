@@ -83,11 +83,16 @@ __device__ __forceinline__ void wave_lds_sync() {
 // A queued leaf packs its first record above 4 count bits.
 constexpr int32_t kMaxRecord = 1 << 26;
 
-template <bool ANY, int W, int LDS_DEPTH, int INNER, int Q>
+// The query is ANY, or with DYN the member dyn_any, set per ray batch (one
+// compiled walk for kernels that serve both queries: the batched kernel).
+template <bool ANY, int W, int LDS_DEPTH, int INNER, int Q, bool DYN = false>
 struct SpecWave {
     static constexpr int G = W / 4;
     static constexpr int32_t kCap = LDS_DEPTH + wide::kSpill;
     static_assert(Q >= W && Q <= 16, "a queue holds one node's leaves; slots fit 4 bits");
+
+    bool dyn_any;
+    __device__ __forceinline__ bool any() const { return DYN ? dyn_any : ANY; }
 
     // per lane
     f3 ro, rd, inv;
@@ -112,7 +117,7 @@ struct SpecWave {
         ro = xform(en.inverse, mk(a.x, a.y, a.z), 1.0f);
         rd = xform(en.inverse, mk(b.x, b.y, b.z), 0.0f);
         inv = mk(1.0f / rd.x, 1.0f / rd.y, 1.0f / rd.z);
-        if (ANY) tmax = ray_tmax;
+        if (any()) tmax = ray_tmax;
         root = en.node_offset;
         // the index guard of wide_walk.h's Core::begin
         const int64_t end = int32_t(uint32_t(root) + uint32_t(en.node_count));
@@ -131,7 +136,7 @@ struct SpecWave {
                                            uint32_t flags) {
         float4 tuvw = make_float4(-1.0f, -1.0f, -1.0f, -1.0f);
         const float closest = tri >= 0 ? tmax : -1.0f;
-        if (ANY) {
+        if (any()) {
             if (tri >= 0) tuvw.x = closest;
         } else {
             const bool ok = closest > 0.0f && ((flags & CRT_FLAG_TRI0_IS_MISS) ? tri > 0 : tri >= 0);
@@ -185,7 +190,7 @@ struct SpecWave {
                 int32_t node;
                 float t;
                 st.get(sp, node, t);
-                if (!ANY && t > tmax * kWideGrow) continue;
+                if (!any() && t > tmax * kWideGrow) continue;
                 cur = node;
                 found = true;
                 break;
@@ -353,26 +358,26 @@ struct SpecWave {
                 const unsigned long long order = (unsigned long long)(uint32_t(idx) << 4 | uint32_t(x));
                 if (v0 && ta > 0.0f && ta < otmax) {
                     const unsigned long long tbits = __float_as_uint(ta);
-                    atomicMin(&L.keys[owner], ANY ? ((order << 32) | tbits) : ((tbits << 32) | order));
+                    atomicMin(&L.keys[owner], any() ? ((order << 32) | tbits) : ((tbits << 32) | order));
                 }
                 if (v1 && tb > 0.0f && tb < otmax) {
                     const unsigned long long tbits = __float_as_uint(tb);
-                    atomicMin(&L.keys[owner], ANY ? (((order + 1) << 32) | tbits) : ((tbits << 32) | (order + 1)));
+                    atomicMin(&L.keys[owner], any() ? (((order + 1) << 32) | tbits) : ((tbits << 32) | (order + 1)));
                 }
             }
         }
         wave_lds_sync();
         const unsigned long long k = L.keys[lane];
         if (k != ~0ull) {
-            const uint32_t order = ANY ? uint32_t(k >> 32) : uint32_t(k);
-            const uint32_t tb = ANY ? uint32_t(k) : uint32_t(k >> 32);
+            const uint32_t order = any() ? uint32_t(k >> 32) : uint32_t(k);
+            const uint32_t tb = any() ? uint32_t(k) : uint32_t(k >> 32);
             const int32_t j = int32_t(L.list[order >> 4] & 15u);
             const float4* p = rec + 3 * int64_t((st.leaf(j) >> 4) + int32_t(order & 15u));
             tmax = __uint_as_float(tb);
             tri = __float_as_int(p[0].w);
             mesh = __float_as_int(p[1].w);
             hit_ent = ent;
-            if (ANY) walk_done = true;   // the first accepted triangle ends an any-hit walk
+            if (any()) walk_done = true;   // the first accepted triangle ends an any-hit walk
         }
         nq = 0;
         wave_lds_sync();   // the list and keys are reused by the next phase
@@ -394,7 +399,7 @@ struct SpecWave {
         if (__ballot(ends)) {
             if (ends) {
                 visits += uint32_t(it);
-                bool fin = ANY && tri >= 0;
+                bool fin = any() && tri >= 0;
                 if (!fin) fin = !begin_walk(s, rays, flags);
                 if (fin) {
                     finish(s, rays, hits, tri_tests, flags);

@@ -798,7 +798,10 @@ typedef struct crt_batch {
 typedef struct crt_batch_config {
     int32_t variant;        /* walk preset 0..7 (stackless: inner steps, leaf batch, window;
                                wide: LDS stack depth, inner steps, leaf batch; stack: LDS stack
-                               depth; 6 and 7 are compiled for at least 5 or 6 waves per SIMD) */
+                               depth; 6 and 7 are compiled for at least 5 or 6 waves per SIMD);
+                               wide formats also 8..11: the speculative walk with crt_trace's
+                               CRT_KERNEL_SPECULATIVE presets 0, 1, 3 and 4 (a tile is one wave of
+                               that launch, so its records equal crt_trace's) */
     int32_t tiles_per_claim;/* 64-ray tiles a wave claims per atomic: 1, 2 or 4 */
     int32_t waves_per_group;/* 1, 2 or 4 waves per workgroup */
     int32_t grid;           /* workgroups; 0 = enough to fill every CU */

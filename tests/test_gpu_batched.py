@@ -159,7 +159,47 @@ def test_batched_wide(work, orc, width):
     with pytest.raises(crt.CrtError):
         crt.BatchPlan(work.batches(crt, layout), crt.BatchConfig(0, 1, 4, 0, 1, 16, 2))
     with pytest.raises(crt.CrtError):
-        crt.BatchPlan(work.batches(crt, layout), crt.BatchConfig(8, 1, 4, 0, 1, 16, 0))
+        crt.BatchPlan(work.batches(crt, layout), crt.BatchConfig(12, 1, 4, 0, 1, 16, 0))
+    # the speculative presets walk the wide formats only
+    with pytest.raises(crt.CrtError, match="wide format"):
+        crt.BatchPlan(work.batches(crt), crt.BatchConfig(8, 1, 4, 0, 1, 16, 0))
+
+
+# batched speculative presets -> crt_trace's CRT_KERNEL_SPECULATIVE presets (the host twin's schedules)
+SPEC_OF_BATCHED = {8: 0, 9: 1, 10: 3, 11: 4}
+
+
+@pytest.mark.parametrize("width", [4, 8])
+def test_batched_wide_speculative(work, orc, width):
+    """Presets 8..11 run the speculative walk per 64-ray tile: each tile is one wave of crt_trace's speculative
+    grid launch on its batch, so every record, counters included, equals the host twin that replays that
+    schedule -- eager and graph, for several claim sizes, group sizes, tile orders and shard counts."""
+    import candela_amd as crt
+    layout = (width, 2)
+    work.build_wide(crt, width, 2)
+    ref = {}
+    for variant, tpc, wpg, order, shards, co in ((8, 1, 4, 1, 16, 0), (9, 2, 1, 0, 1, 1), (10, 4, 2, 1, 16, 0),
+                                                 (11, 1, 4, 0, 16, 1)):
+        for s in work.scenes:
+            s.h_closest.fill_bytes(0xAB)
+            s.h_shadow.fill_bytes(0xAB)
+        plan = crt.BatchPlan(work.batches(crt, layout), crt.BatchConfig(variant, tpc, wpg, 0, order, shards, co))
+        for replay in range(2):
+            plan.launch(graph=replay == 1)
+            crt.lib().crt_device_sync()
+            for s in work.scenes:
+                w = s.wide[layout].host
+                for which, h, q, rays in (("closest", s.h_closest, 0, s.closest), ("shadow", s.h_shadow, 1, s.shadow)):
+                    key = (s.index, which, variant)
+                    if key not in ref:
+                        ref[key] = orc.trace_wide_spec(w.width, w.groups, w.records, w.entities, q, rays,
+                                                       variant=SPEC_OF_BATCHED[variant])
+                    got = h.download()
+                    assert got.tobytes() == ref[key].tobytes(), \
+                        f"speculative wide{width} v{variant} scene {s.index} {which} (replay {replay})"
+        log(f"[parity] batched speculative wide{width} variant={variant} tpc={tpc} wpg={wpg} order={order} "
+            f"shards={shards}: {work.rays} rays bit-identical to the twin of the speculative grid launch "
+            "(eager and graph)")
 
 
 def test_autotune_wide_child_order(work, orc):
