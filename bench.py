@@ -406,7 +406,7 @@ def time_large(crt, args) -> dict:
     stream = crt.Stream()
     scene = dw.scene
     best = None
-    for v in (range(6) if args.large_preset < 0 else [args.large_preset]):
+    for v in (range(10) if args.large_preset < 0 else [args.large_preset]):
         crt.trace_paths(scene, p, out, n, variant=v, stream=stream)   # warm-up
         stream.sync()
         ev = Events(crt, 2)
@@ -433,7 +433,17 @@ def time_large(crt, args) -> dict:
         w = dw.host
         idx = np.random.default_rng(6).choice(n, 2000, replace=False)
         ref = oracle.trace_paths(p, 4, w.groups, w.records, w.entities, albedo, idx)
-        parity = {"paths": int(idx.size), "exact": bool(res[idx].tobytes() == ref.tobytes())}
+        if best[0] >= 6:
+            # speculative presets: radiance, segments and status are the twin's; visit and test counts follow
+            # the wave schedule
+            got = res[idx]
+            parity = {"paths": int(idx.size), "exact": None,
+                      "radiance_segments_status_exact": bool(
+                          got["radiance"].tobytes() == ref["radiance"].tobytes() and
+                          np.array_equal(got["segments"], ref["segments"]) and
+                          np.array_equal(got["status"], ref["status"]))}
+        else:
+            parity = {"paths": int(idx.size), "exact": bool(res[idx].tobytes() == ref.tobytes())}
     except ImportError:
         parity = None
     ms = best[1]

@@ -41,6 +41,7 @@
 #include "crt_internal.h"
 #include "trace_launch.h"
 #include "sampling.h"
+#include "spec_walk.h"
 #include "wide_walk.h"
 
 #pragma clang fp contract(off)
@@ -303,6 +304,202 @@ path_kernel_occ(SceneArgs s, Params P, int64_t npaths, float tmax, int32_t max_i
     path_body<W, LDS_DEPTH, INNER, LEAF_BATCH, REFILL>(s, P, npaths, tmax, max_iter, counters, out, lds);
 }
 
+// The same paths over the speculative walk (spec_walk.h): a segment's leaves go
+// to the lane's queue and the wave tests every queued leaf once per round, so
+// no lane waits for its leaf tests.  The walk state is the SpecWave's (its ray
+// fields hold the segment's ray in the current entity's space, `active` marks
+// a live path); the path state and shading are PathWave's.  Each segment ends
+// with the nearest hit the sequential walk finds, so radiance and segment
+// counts equal the host twin's; visit and test counts follow the schedule.
+template <int W, int LDS_DEPTH, int INNER, int Q>
+struct PathSpecWave {
+    spec::SpecWave<false, W, LDS_DEPTH, INNER, Q> w;
+    int64_t path;
+    f3 org, dir;
+    f3 thr, rad;
+    uint32_t rng;
+    int32_t bounce, segs;
+    uint32_t visits, tests;
+
+    __device__ __forceinline__ bool begin_walk(const SceneArgs& s) {
+        const int32_t e = w.ent + 1;
+        w.ent = e;
+        if (e >= s.entity_count) return false;
+        const crt_entity& en = s.entities[e];
+        w.begin_local(xform(en.inverse, org, 1.0f), xform(en.inverse, dir, 0.0f), en.node_offset, en.node_count,
+                      s.total_nodes);
+        return true;
+    }
+
+    __device__ __forceinline__ bool start_segment(const SceneArgs& s, float tmax) {
+        w.ent = -1;
+        w.hit_ent = -1;
+        w.tmax = tmax;
+        w.tri = w.mesh = -1;
+        w.tests = 0;
+        segs++;
+        return begin_walk(s);
+    }
+
+    // PathWave::shade over the walk's hit.
+    __device__ __forceinline__ bool shade(const SceneArgs& s, const Params& P) {
+        tests += w.tests;
+        if (w.tri < 0) {
+            rad = add(rad, mk(thr.x * P.sky[0], thr.y * P.sky[1], thr.z * P.sky[2]));
+            return false;
+        }
+        const float t = w.tmax;
+        const int32_t mesh = w.mesh;
+        f3 alb = mk(0.0f, 0.0f, 0.0f);
+        if (mesh >= 0 && mesh < P.nmesh) alb = mk(P.albedo[3 * mesh], P.albedo[3 * mesh + 1], P.albedo[3 * mesh + 2]);
+        thr = mk(thr.x * alb.x, thr.y * alb.y, thr.z * alb.z);
+        if (bounce >= P.max_bounces) return false;
+        if (bounce >= 1) {
+            if (!(draw(rng) < P.rr)) return false;
+            thr = mk(thr.x / P.rr, thr.y / P.rr, thr.z / P.rr);
+        }
+        const float4* r = reinterpret_cast<const float4*>(s.tri_records) + 3 * int64_t(w.tri);
+        const float4 va = r[0], vb = r[1], vc = r[2];
+        const f3 a = mk(va.x, va.y, va.z);
+        const f3 no = cross(sub(mk(vb.x, vb.y, vb.z), a), sub(mk(vc.x, vc.y, vc.z), a));
+        const float* inv = s.entities[w.hit_ent].inverse;
+        f3 n = normalize(mk(inv[0] * no.x + inv[1] * no.y + inv[2] * no.z, inv[4] * no.x + inv[5] * no.y + inv[6] * no.z,
+                            inv[8] * no.x + inv[9] * no.y + inv[10] * no.z));
+        if (dot(n, dir) > 0.0f) n = neg(n);
+        const f3 p = add(org, scale(dir, t));
+        const float u1 = draw(rng), u2 = draw(rng);
+        org = add(p, scale(n, 1e-4f));
+        dir = cosine_dir(n, u1, u2);
+        bounce++;
+        return true;
+    }
+
+    __device__ __forceinline__ void finish(crt_path_sample* __restrict__ out) {
+        float4* o = reinterpret_cast<float4*>(out + path);
+        o[0] = make_float4(rad.x, rad.y, rad.z, __int_as_float(segs));
+        o[1] = make_float4(__uint_as_float(visits), __uint_as_float(tests), __uint_as_float(w.status), 0.0f);
+        w.active = false;
+    }
+
+    __device__ __forceinline__ void segment_done(const SceneArgs& s, const Params& P, float tmax,
+                                                 crt_path_sample* __restrict__ out) {
+        while (shade(s, P)) {
+            if (start_segment(s, tmax)) return;
+            w.tri = -1;   // no entity: the new segment misses
+        }
+        finish(out);
+    }
+
+    // PathWave::start_path (same camera ray and random stream).
+    __device__ __forceinline__ void start_path(const SceneArgs& s, const Params& P, int64_t p, float tmax,
+                                               crt_path_sample* __restrict__ out) {
+        w.active = true;
+        w.walk_done = false;
+        w.nq = 0;
+        path = p;
+        rng = pcg(uint32_t(p) ^ (P.seed * 0x9E3779B9u));
+        const int64_t pixel = p / P.spp;
+        const int32_t x = int32_t(pixel % P.width), y = int32_t(pixel / P.width);
+        const float jx = draw(rng), jy = draw(rng);
+        const float u = (float(x) + jx) / float(P.width), v = (float(y) + jy) / float(P.height);
+        const float cx = u * 2.0f - 1.0f, cy = v * 2.0f - 1.0f;
+        const float* ip = P.cam.inv_proj;
+        const float ex = ip[0] * cx + ip[4] * cy + ip[8] * -1.0f + ip[12] * 1.0f;
+        const float ey = ip[1] * cx + ip[5] * cy + ip[9] * -1.0f + ip[13] * 1.0f;
+        const f3 d = xform(P.cam.inv_view, mk(ex, ey, -1.0f), 0.0f);
+        dir = scale(d, 1.0f / sqrtf(dot(d, d)));
+        org = mk(P.cam.inv_view[12], P.cam.inv_view[13], P.cam.inv_view[14]);
+        thr = mk(1.0f, 1.0f, 1.0f);
+        rad = mk(0.0f, 0.0f, 0.0f);
+        bounce = 0;
+        segs = 0;
+        visits = 0;
+        tests = 0;
+        w.status = 0;
+        if (!start_segment(s, tmax)) {
+            w.tri = -1;
+            segment_done(s, P, tmax, out);
+        }
+    }
+
+    __device__ __forceinline__ void round(const SceneArgs& s, const Params& P, float tmax, int32_t max_iter,
+                                          Stack<LDS_DEPTH>& st, spec::WaveLds<Q>& L,
+                                          crt_path_sample* __restrict__ out) {
+        for (int k = 0; k < INNER; ++k) {
+            if (w.can_step()) {
+                if (!w.step(s, max_iter, st)) w.walk_done = true;
+            }
+            if (!__ballot(w.can_step())) break;
+        }
+        if (__ballot(w.nq > 0)) w.leaf_phase(s, st, L);
+        // every queue is empty here: a segment's walk of this entity is over
+        const bool ends = w.active && w.walk_done;
+        if (__ballot(ends)) {
+            if (ends) {
+                visits += uint32_t(w.it);
+                if (!begin_walk(s)) segment_done(s, P, tmax, out);
+            }
+        }
+    }
+
+    __device__ __forceinline__ void run(const SceneArgs& s, const Params& P, int64_t npaths, float tmax,
+                                        int32_t max_iter, Stack<LDS_DEPTH> st, spec::WaveLds<Q>& L,
+                                        unsigned long long* __restrict__ counters, int32_t refill,
+                                        crt_path_sample* __restrict__ out) {
+        w.active = false;
+        w.walk_done = false;
+        w.nq = 0;
+        ShardQueue q;
+        q.init(counters, npaths);
+        while (true) {
+            if (q.more) {
+                const unsigned long long idle = __ballot(!w.active);
+                const int32_t cnt = __popcll(idle);
+                if (cnt >= refill) {
+                    int32_t got = 0;
+                    const int64_t first = q.claim(cnt, got);
+                    if (!w.active) {
+                        const int32_t rank = spec::lanes_below(idle);
+                        if (rank < got) start_path(s, P, q.at(first + rank), tmax, out);
+                    }
+                }
+            }
+            if (!__ballot(w.active)) {
+                if (!q.more) break;
+                continue;
+            }
+            round(s, P, tmax, max_iter, st, L, out);
+        }
+    }
+};
+
+template <int W, int Q>
+constexpr int spec_queue_slots() {
+    return W == 4 ? Q : (Q + 4 < 16 ? Q + 4 : 16);
+}
+
+template <int W, int D, int Q>
+struct SpecPathLds {
+    int32_t cols[spec::lds_ints<D, spec_queue_slots<W, Q>()>() * 256];
+    spec::WaveLds<spec_queue_slots<W, Q>()> waves[4];
+};
+
+template <int W, int LDS_DEPTH, int INNER, int Q, int REFILL, int OCC>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1, 8)))
+spec_path_kernel(SceneArgs s, Params P, int64_t npaths, float tmax, int32_t max_iter,
+                 unsigned long long* __restrict__ counters, crt_path_sample* __restrict__ out) {
+    __shared__ SpecPathLds<W, LDS_DEPTH, Q> lds;
+    int32_t spill_node[wide::kSpill];
+    float spill_t[wide::kSpill];
+    Stack<LDS_DEPTH> st;
+    st.lds = lds.cols + threadIdx.x;
+    st.stride = 256;
+    st.spill_node = spill_node;
+    st.spill_t = spill_t;
+    PathSpecWave<W, LDS_DEPTH, INNER, spec_queue_slots<W, Q>()> pw;
+    pw.run(s, P, npaths, tmax, max_iter, st, lds.waves[threadIdx.x >> 6], counters, REFILL, out);
+}
+
 template <int W, int D, int I, int LB, int R, int OCC>
 const void* path_kernel_ptr() {
     if constexpr (OCC == 0) {
@@ -322,14 +519,26 @@ const void* path_kernel_ptr() {
     X(4, 8, 4, 32, 16, 4)   \
     X(5, 8, 4, 48, 24, 4)
 
+// Presets of the speculative path walk (variant, LDS stack entries, INNER,
+// Q queue slots over the BVH4, REFILL, OCC).
+#define CRT_SPEC_PATH_PRESETS(X) \
+    X(6, 8, 8, 8, 16, 0)         \
+    X(7, 8, 8, 8, 16, 5)         \
+    X(8, 8, 4, 8, 16, 5)         \
+    X(9, 8, 8, 8, 8, 5)
+
 template <int W>
 const void* path_fn(int variant) {
 #define CRT_PATH_FN(V, D, I, LB, R, OCC) \
     case V: return path_kernel_ptr<W, D, I, LB, R, OCC>();
+#define CRT_SPEC_PATH_FN(V, D, I, Q, R, OCC) \
+    case V: return reinterpret_cast<const void*>(&spec_path_kernel<W, D, I, Q, R, OCC>);
     switch (variant) {
         CRT_PATH_PRESETS(CRT_PATH_FN)
+        CRT_SPEC_PATH_PRESETS(CRT_SPEC_PATH_FN)
         default: return nullptr;
     }
+#undef CRT_SPEC_PATH_FN
 #undef CRT_PATH_FN
 }
 
@@ -352,7 +561,7 @@ extern "C" int crt_trace_paths(const crt_scene_view* scene, const crt_path_param
     if (npaths > 0 && !out) return crt::set_error(CRT_E_INVALID, "crt_trace_paths: out is null");
     if (npaths >= (int64_t(1) << 32)) return crt::set_error(CRT_E_INVALID, "crt_trace_paths: >= 2^32 paths");
     const int variant = config ? config->variant : 0;
-    if (variant < 0 || variant > 5) return crt::set_error(CRT_E_INVALID, "crt_trace_paths: variant must be 0..5");
+    if (variant < 0 || variant > 9) return crt::set_error(CRT_E_INVALID, "crt_trace_paths: variant must be 0..9");
     if (npaths == 0) return CRT_OK;
     crt::path::Params P;
     std::memcpy(P.cam.inv_view, p.inv_view, sizeof(P.cam.inv_view));

@@ -114,21 +114,28 @@ struct SpecWave {
         const crt_entity& en = s.entities[e];
         const float4* rp = reinterpret_cast<const float4*>(rays + slot);
         const float4 a = rp[0], b = rp[1];
-        ro = xform(en.inverse, mk(a.x, a.y, a.z), 1.0f);
-        rd = xform(en.inverse, mk(b.x, b.y, b.z), 0.0f);
-        inv = mk(1.0f / rd.x, 1.0f / rd.y, 1.0f / rd.z);
         if (any()) tmax = ray_tmax;
-        root = en.node_offset;
-        // the index guard of wide_walk.h's Core::begin
-        const int64_t end = int32_t(uint32_t(root) + uint32_t(en.node_count));
-        const int64_t lim = end < s.total_nodes ? end : s.total_nodes;
-        span = lim > root ? uint32_t(lim - root) : 0u;
-        cur = root;
+        begin_local(xform(en.inverse, mk(a.x, a.y, a.z), 1.0f), xform(en.inverse, mk(b.x, b.y, b.z), 0.0f),
+                    en.node_offset, en.node_count, s.total_nodes);
+        return true;
+    }
+
+    // A walk of one entity's tree from the ray (lro, lrd) in its local space,
+    // with the current tmax: empty stack and queue, the index guard of
+    // wide_walk.h's Core::begin.  (The path kernel starts its segments here.)
+    __device__ __forceinline__ void begin_local(f3 lro, f3 lrd, int32_t r, int32_t c, int64_t total_nodes) {
+        ro = lro;
+        rd = lrd;
+        inv = mk(1.0f / rd.x, 1.0f / rd.y, 1.0f / rd.z);
+        root = r;
+        const int64_t end = int32_t(uint32_t(r) + uint32_t(c));
+        const int64_t lim = end < total_nodes ? end : total_nodes;
+        span = lim > r ? uint32_t(lim - r) : 0u;
+        cur = r;
         sp = 0;
         it = 0;
         nq = 0;
         walk_done = false;
-        return true;
     }
 
     __device__ __forceinline__ void finish(const SceneArgs& s, const crt_ray* __restrict__ rays,

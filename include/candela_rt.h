@@ -707,8 +707,12 @@ typedef struct crt_path_sample {
  * path a jittered camera ray, closest hits, diffuse bounces with Russian
  * roulette, sky radiance on escape; out[p] per path.  Device-filling waves
  * claim new paths whenever enough lanes are idle, so segments of different
- * paths and bounces share the waves.  config: variant 0..5 (stack depth,
- * inner steps, leaf batch, refill threshold), grid (0 = fill the device). */
+ * paths and bounces share the waves.  config: variant 0..5 (the sequential
+ * wide walk: stack depth, inner steps, leaf batch, refill threshold) or 6..9
+ * (the speculative walk of CRT_KERNEL_SPECULATIVE: stack depth, inner steps,
+ * queue slots, refill threshold; radiance, segments and status equal the
+ * other presets', visit and test counts follow the wave schedule), grid
+ * (0 = fill the device). */
 int crt_trace_paths(const crt_scene_view* scene, const crt_path_params* params, crt_path_sample* out,
                     int64_t npaths, const crt_trace_config* config, void* stream);
 

@@ -7,7 +7,8 @@ bounces, Russian roulette p = 0.8 from the second bounce, all in one
 persistent crt_trace_paths launch.  2,000 sampled paths must equal the host
 twin bit for bit, every path must end without a guard flag, and the segment
 counts must follow the roulette (camera segment always, a third segment on
-about 0.8 of the paths that reach it).
+about 0.8 of the paths that reach it).  The speculative walk (preset 7) must
+give every path the same radiance, segments and status.
 """
 import sys
 
@@ -65,3 +66,12 @@ def test_config5_paths_match_host_twin(crt, orc):
     assert (segs >= 1).all() and (segs <= 3).all()
     # roulette: of the paths that reach a second bounce (3 segments possible), about 0.8 continue
     assert counts[3] > 0 and counts[2] > 0
+    # the speculative walk (preset 7): radiance, segments and status of every path equal preset 4's
+    log("[case] config 5: speculative preset 7")
+    out.fill_bytes(0xAB)
+    crt.trace_paths(dw.scene, p, out, n, variant=7)
+    crt.lib().crt_device_sync()
+    spec = out.download()
+    assert spec["radiance"].tobytes() == got["radiance"].tobytes(), "speculative radiance differs"
+    assert np.array_equal(spec["segments"], got["segments"]) and np.array_equal(spec["status"], got["status"])
+    log(f"[parity] config 5: speculative preset 7 gives every path's radiance, segments and status of preset 4")

@@ -75,6 +75,29 @@ def test_presets_and_grids_agree(crt, scene):
     log(f"[parity] paths: presets 0..3 and grids 1, 3 give identical records for {base.size} paths")
 
 
+def _path_fields(a):
+    """Radiance bits, segment counts and status: what every preset must agree on (visit and test counts of the
+    speculative presets follow their wave schedule)."""
+    return a["radiance"].tobytes(), a["segments"].tobytes(), a["status"].tobytes()
+
+
+def test_speculative_presets(crt, orc, scene):
+    """Presets 6..9 walk each segment with the speculative walk: radiance, segments and status of every path
+    equal the sequential presets', sampled paths equal the host twin's, and the counters are live."""
+    r, w, albedo = scene
+    p, base = _run(crt, r, albedo, 256, 144, 2)
+    idx = np.random.default_rng(14).choice(base.size, 3000, replace=False)
+    ref = orc.trace_paths(p, 4, w.groups, w.records, w.entities, albedo, idx)
+    for variant, grid in ((6, 0), (7, 0), (8, 3), (9, 0)):
+        log(f"[case] speculative path preset {variant} grid {grid}")
+        _, got = _run(crt, r, albedo, 256, 144, 2, variant=variant, grid=grid)
+        assert _path_fields(got) == _path_fields(base), f"speculative preset {variant} grid {grid}"
+        assert _path_fields(got[idx]) == _path_fields(ref)
+        assert (got["visits"] > 0).all() and (got["tests"] > 0).mean() > 0.5
+    log(f"[parity] paths: speculative presets 6..9 give the sequential presets' radiance, segments and status "
+        f"for {base.size} paths, and the host twin's on 3000 sampled paths")
+
+
 def test_errors(crt, scene):
     r, w, albedo = scene
     from candela_amd import scenes
@@ -85,4 +108,4 @@ def test_errors(crt, scene):
     with pytest.raises(crt.CrtError):
         crt.trace_paths(r.wide_scene(), crt.path_params(4, 4, 1, iv, ip, rr=0.0), out)
     with pytest.raises(crt.CrtError):
-        crt.trace_paths(r.wide_scene(), crt.path_params(4, 4, 1, iv, ip), out, variant=6)
+        crt.trace_paths(r.wide_scene(), crt.path_params(4, 4, 1, iv, ip), out, variant=10)
