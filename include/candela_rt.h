@@ -444,7 +444,12 @@ int crt_trace_noalloc(const crt_scene_view* scene, int query, uint32_t flags,
  * node two consecutive groups.  count[k]: 0 = inner child (child[k] = wide
  * node index, absolute), n > 0 = leaf of n triangles starting at record
  * child[k], -1 = empty slot.  The child boxes are the BVH2 node boxes,
- * unchanged. */
+ * unchanged.  Leaves hold at most 15 triangles (crt_wide_build's max_leaf).
+ * The walks refuse inner children outside the entity's node range
+ * (CRT_STATUS_BAD_INDEX) and treat a negative inner child as an empty slot;
+ * the sequential kernels read leaf records unchecked, the speculative ones
+ * refuse a leaf whose start lies outside 0 .. 2^26 - n (CRT_STATUS_BAD_INDEX,
+ * its records unread). */
 typedef struct crt_wide_group {
     float lo_x[4], hi_x[4];
     float lo_y[4], hi_y[4];

@@ -99,3 +99,99 @@ def crafted_stack_chain(depth: int = 70):
     n["left"]["min"][last, 3] = _fbits(1)
     n["right"]["min"][last, 3] = _fbits(1)
     return n
+
+
+WIDE_KINDS = ("cycle", "cycle_leaf", "fan", "bad_child", "bad_record")
+WIDE_MAX_RECORD = 1 << 26   # leaf starts the speculative walk's queue entries can pack (start << 4 | count)
+
+
+def crafted_wide(kind: str, width: int):
+    """Hand-written BVH4 / BVH8 buffers (crt_wide_group layout) that exercise one traversal guard each.
+
+    Every used slot has the box [0,1]^3; three one-triangle records lie in the planes x = 0.5 (record 0,
+    triangle 0), x = 0.25 (record 1, triangle 1) and x = 0.75 (record 2, triangle 2), covering the box's
+    y/z extent.  Returns (groups, records, entity node count).
+      'cycle'       root slot 0 is an inner child pointing at the root -> ITER_CAP after 1024 visits
+      'cycle_leaf'  as 'cycle' plus a leaf of record 0 in slot 1 -> the leaf is tested on the way
+                    (the sequential walk pushes the inner child, tests the leaf and pops it each visit;
+                    the speculative walk queues it each visit), then ITER_CAP
+      'fan'         every slot an inner child pointing at the root -> STACK_OVERFLOW after
+                    capacity // (width - 1) + 1 visits
+      'bad_child'   leaf of record 0, an inner child outside the entity's node range (node 1 exists in
+                    the buffer), one at -5 (an empty slot: inner children need an index >= 0), one far
+                    outside the buffer; BVH8: a leaf of record 1 in the second group -> BAD_INDEX,
+                    leaves still tested
+      'bad_record'  leaves whose start is negative or too large to pack beside their count, next to a
+                    leaf of record 1 -> BAD_INDEX from the speculative walk, which never reads them;
+                    the sequential walks read leaf records unchecked, so they must not be given it
+    """
+    from oracle import TRI_RECORD_DTYPE, WIDE_GROUP_DTYPE
+    if kind not in WIDE_KINDS:
+        raise ValueError(kind)
+    per = width // 4
+    nodes = 2 if kind == "bad_child" else 1
+    g = np.zeros(nodes * per, WIDE_GROUP_DTYPE)
+    g["child"] = -1
+    g["count"] = -1
+    for f in ("lo_x", "lo_y", "lo_z"):
+        g[f] = 0.0
+    for f in ("hi_x", "hi_y", "hi_z"):
+        g[f] = 1.0
+    slots = [(0, k) for k in range(4)] + ([(1, k) for k in range(4)] if width == 8 else [])
+
+    def put(i, child, count):
+        grp, k = slots[i]
+        g["child"][grp, k] = child
+        g["count"][grp, k] = count
+
+    if kind == "cycle":
+        put(0, 0, 0)
+    elif kind == "cycle_leaf":
+        put(0, 0, 0)
+        put(1, 0, 1)
+    elif kind == "fan":
+        for i in range(width):
+            put(i, 0, 0)
+    elif kind == "bad_child":
+        put(0, 0, 1)
+        put(1, 1, 0)
+        put(2, -5, 0)
+        put(3, 1000, 0)
+        if width == 8:
+            put(4, 1, 1)
+        g["child"][per] = [2, -1, -1, -1]     # node 1: a leaf of record 2 (outside the entity; never walked)
+        g["count"][per] = [1, -1, -1, -1]
+    else:
+        put(0, -3, 1)
+        put(1, WIDE_MAX_RECORD, 1)
+        put(2, 1, 1)
+        put(3, WIDE_MAX_RECORD - 1, 2)
+    r = np.zeros(3, TRI_RECORD_DTYPE)
+    for i, x in enumerate((0.5, 0.25, 0.75)):
+        r["v0"][i] = [x, -1, -1]
+        r["v1"][i] = [x, 3, -1]
+        r["v2"][i] = [x, -1, 3]
+        r["tri"][i] = i
+        r["mesh"][i] = 0
+    return g, r, 1
+
+
+def crafted_wide_rays(seed: int = 5) -> np.ndarray:
+    """300 rays for crafted_wide: 200 from x = -1 toward +x (most enter the box), 60 from inside the box in
+    random directions, 40 that miss it (y = 5, toward +x)."""
+    rng = np.random.default_rng(seed)
+    r = np.zeros(300, RAY_DTYPE)
+    o = np.empty((300, 3))
+    d = np.empty((300, 3))
+    o[:200] = np.c_[np.full(200, -1.0), rng.uniform(0.02, 0.98, (200, 2))]
+    d[:200] = np.c_[np.ones(200), rng.uniform(-0.2, 0.2, (200, 2))]
+    o[200:260] = rng.uniform(0.05, 0.95, (60, 3))
+    d[200:260] = rng.normal(size=(60, 3))
+    o[260:] = np.c_[np.full(40, -1.0), np.full(40, 5.0), rng.uniform(0, 1, 40)]
+    d[260:] = np.c_[np.ones(40), np.zeros(40), np.zeros(40)]
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    r["org"] = o.astype(np.float32)
+    r["dir"] = d.astype(np.float32)
+    r["tmax"] = np.float32(1e6)
+    r["id"] = np.arange(300, dtype=np.uint32)
+    return r
