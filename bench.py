@@ -99,8 +99,9 @@ def kernel_label(c: dict, gi_only: bool = False) -> str:
         label = "plain (one batch per launch): %s, BVH2 stack layout (FlattenStackBVH), LDS stack %d" % (
             name, c.get("lds", 16))
     else:
-        label = "plain (one batch per launch): %s over the BVH%d layout derived from the BVH2, %d-triangle leaves" % (
-            name, c["width"], c["max_leaf"])
+        label = "plain (one batch per launch): %s over the BVH%d layout derived from the BVH2 (%s collapse), " \
+                "leaves of up to %d triangles" % (name, c["width"], "SAH" if c.get("collapse") else "greedy",
+                                                  c["max_leaf"])
     if "ao_kernel" in c and not gi_only:
         label += "; AO launch: " + _kernel_name(c["layout"], c["ao_kernel"], c["ao_variant"])
     if c.get("ao_order") == 1 and not gi_only:
@@ -110,19 +111,28 @@ def kernel_label(c: dict, gi_only: bool = False) -> str:
 
 def choice_name(c: dict) -> str:
     """'K:V' for a BVH2 kernel choice, 'wWlL:K:V' for kernel K, variant V over the wide layout of width W
-    and leaf size L; a choice whose AO launch uses another kernel gets '+ao:K:V', and ':s' after it when the
+    and leaf size L ('wWsL' with the SAH collapse); a choice whose AO launch uses another kernel gets '+ao:K:V', and ':s' after it when the
     AO walks take children in slot order (CRT_FLAG_SLOT_ORDER)."""
     if c["layout"] == "bvh2":
         name = f"{c['kernel']}:{c['variant']}"
     elif c["layout"] == "stack":
         name = f"s{c.get('lds', 16)}:{c['kernel']}:{c['variant']}"
     else:
-        name = f"w{c['width']}l{c['max_leaf']}:{c['kernel']}:{c['variant']}"
+        name = f"w{c['width']}{'s' if c.get('collapse') else 'l'}{c['max_leaf']}:{c['kernel']}:{c['variant']}"
     if "ao_kernel" in c:
         name += f"+ao:{c['ao_kernel']}:{c['ao_variant']}"
     if c.get("ao_order") == 1:
         name += ":s" if "ao_kernel" in c else f"+ao:{c['kernel']}:{c['variant']}:s"
     return name
+
+
+# Cost of one triangle test against one wide-node visit for the SAH-collapsed layouts (crt_wide_build_ex).
+SAH_TRI_COST = 0.3
+
+
+def wide_key(c: dict) -> tuple:
+    """(width, max_leaf, collapse) of a wide-layout choice."""
+    return c["width"], c["max_leaf"], c.get("collapse", 0)
 
 
 def gi_choice_name(c: dict) -> str:
@@ -605,10 +615,11 @@ def main(argv=None) -> int:
                 stack_ctx["hb"] = rs.host_buffers()
                 log(f"stack layout: {stack_ctx['hb']['nodes'].size} entries, built in {time.perf_counter() - t:.2f}s")
             return stack_ctx["scene"]
-        key = (c["width"], c["max_leaf"])
+        key = wide_key(c)
         if key not in wides:
             t = time.perf_counter()
-            wides[key] = crt.upload_wide(fmt, hb, *key, triangles=scene.triangles, vertices=scene.vertices)
+            wides[key] = crt.upload_wide(fmt, hb, key[0], key[1], scene.triangles, scene.vertices, key[2], 0.0,
+                                         SAH_TRI_COST if key[2] else 0.0)
             st = wides[key].host.stats
             log(f"wide layout {key}: {st['node_count']} nodes, fill {st['mean_fill']:.2f}, depth {st['max_depth']}, "
                 f"built in {time.perf_counter() - t:.2f}s")
@@ -657,8 +668,8 @@ def main(argv=None) -> int:
     def stack(k, v, lds=16):
         return dict(layout="stack", kernel=k, variant=v, lds=lds)
 
-    def wide(w, leaf, kernel=crt.KERNEL_WAVE, variant=0):
-        return dict(layout=f"wide{w}", width=w, max_leaf=leaf, kernel=kernel, variant=variant)
+    def wide(w, leaf, kernel=crt.KERNEL_WAVE, variant=0, collapse=0):
+        return dict(layout=f"wide{w}", width=w, max_leaf=leaf, kernel=kernel, variant=variant, collapse=collapse)
 
     candidates = [bvh2(crt.KERNEL_THREAD_PER_RAY, 0)]
     if fmt == crt.FORMAT_STACKLESS:
@@ -669,6 +680,8 @@ def main(argv=None) -> int:
                            for k in (crt.KERNEL_WAVE, crt.KERNEL_PERSISTENT) for v in range(10)]
             candidates += [wide(w, 2, k, v) for w in (4, 8)
                            for k in (crt.KERNEL_SPECULATIVE, crt.KERNEL_SPECULATIVE_REFILL) for v in range(6)]
+            candidates += [wide(w, leaf, k, v, crt.COLLAPSE_SAH) for w, leaf in ((4, 2), (4, 4), (8, 2))
+                           for k in (crt.KERNEL_SPECULATIVE, crt.KERNEL_SPECULATIVE_REFILL) for v in range(6)]
         if not args.no_stack:   # the reference's other layout over the same meshes
             candidates += [stack(crt.KERNEL_THREAD_PER_RAY, 0, lds) for lds in (16, 32)]
             candidates += [stack(k, v, 0) for k in (crt.KERNEL_WAVE, crt.KERNEL_PERSISTENT) for v in range(8)]
@@ -676,8 +689,9 @@ def main(argv=None) -> int:
         main_part, _, ao_part = args.only.partition("+ao:")
         parts = main_part.split(":")
         if parts[0].startswith("w"):
-            w, leaf = (int(x) for x in parts[0][1:].split("l"))
-            only = wide(w, leaf, int(parts[1]), int(parts[2]))
+            sah = "s" in parts[0]
+            w, leaf = (int(x) for x in parts[0][1:].split("s" if sah else "l"))
+            only = wide(w, leaf, int(parts[1]), int(parts[2]), crt.COLLAPSE_SAH if sah else 0)
         elif parts[0].startswith("s"):
             only = stack(int(parts[1]), int(parts[2]), int(parts[0][1:]))
         else:
@@ -697,7 +711,7 @@ def main(argv=None) -> int:
     if "ao_kernel" not in fastest_gi and \
             (fastest_ao["kernel"], fastest_ao["variant"]) != (fastest_gi["kernel"], fastest_gi["variant"]):
         mixed = {k: v for k, v in fastest_gi.items()
-                 if k in ("layout", "width", "max_leaf", "kernel", "variant", "lds", "block")}
+                 if k in ("layout", "width", "max_leaf", "collapse", "kernel", "variant", "lds", "block")}
         mixed.update(ao_kernel=fastest_ao["kernel"], ao_variant=fastest_ao["variant"])
         results.append(run(mixed))
     # child order of the AO walks: the fastest AO choice of the wide layouts again with slot order (the
@@ -706,7 +720,8 @@ def main(argv=None) -> int:
         base = min((r for r in results if scene_of(r) is scene_of(fastest_gi)), key=lambda r: r["ms_step"])
         if ao_of(base)[0] not in (crt.KERNEL_SPECULATIVE, crt.KERNEL_SPECULATIVE_REFILL):
             results.append(run(dict({k: v for k, v in base.items() if k in (
-                "layout", "width", "max_leaf", "kernel", "variant", "lds", "block", "ao_kernel", "ao_variant")},
+                "layout", "width", "max_leaf", "collapse", "kernel", "variant", "lds", "block", "ao_kernel",
+                "ao_variant")},
                 ao_order=1)))
     thread = results[0]
     best = max(results, key=lambda r: r["mrays"])
@@ -869,7 +884,7 @@ def main(argv=None) -> int:
             b2 = bvh2(crt.KERNEL_WAVE, 1)
         plain_choice[None] = (b2["kernel"], b2["variant"], *ao_of(b2), 0)
         for w in (4, 8):
-            bw = best_of(lambda r: r["layout"] == f"wide{w}" and r["max_leaf"] == 2)
+            bw = best_of(lambda r: r["layout"] == f"wide{w}" and r["max_leaf"] == 2 and not r.get("collapse"))
             if bw is not None:
                 plain_choice[(w, 2)] = (bw["kernel"], bw["variant"], *ao_of(bw), bw.get("ao_order", 0))
         c2_batched = time_batched(crt, [(best_scene, crt.QUERY_ANY, d_ao, h_ao),
@@ -938,7 +953,7 @@ def main(argv=None) -> int:
             twin_a = oracle.trace(crt.FORMAT_STACK, sh_["nodes"], sh_["triangles"], sh_["vertices"], sh_["entities"],
                                   1, ao, threads=threads)
         else:   # the host twin of the wide walk, on the same wide layout
-            w = wides[(best["width"], best["max_leaf"])].host
+            w = wides[wide_key(best)].host
 
             def wide_twin(q, rays_, kernel, variant, flags):
                 # the speculative grid kernel's twin replays its wave schedule; the refilling one has no fixed

@@ -15,7 +15,8 @@ import numpy as np
 
 from ._lib import (KERNEL_AUTO, KERNEL_PERSISTENT, KERNEL_THREAD_PER_RAY, KERNEL_WAVE, KERNEL_SPECULATIVE,
                    KERNEL_SPECULATIVE_REFILL, BOUNDS_DTYPE, ENTITY_DTYPE, FLAG_IGNORE_TRANSPARENT, FLAG_TRI0_IS_MISS, FLAG_SLOT_ORDER, FORMAT_STACK,
-                   FORMAT_STACKLESS, FORMAT_WIDE4, FORMAT_WIDE8, WIDE_GROUP_DTYPE, TRI_RECORD_DTYPE, WideStats,
+                   FORMAT_STACKLESS, FORMAT_WIDE4, FORMAT_WIDE8, WIDE_GROUP_DTYPE, TRI_RECORD_DTYPE, WideStats, WideOptions,
+                   COLLAPSE_GREEDY, COLLAPSE_SAH,
                    TEXTURE_REF_DTYPE, TEXTURE_DTYPE, HIT_ATTR_DTYPE, PATH_PARAMS_DTYPE, PATH_SAMPLE_DTYPE,
                    HIT_DTYPE, QUERY_ANY, QUERY_CLOSEST, RAY_DTYPE, STACK_NODE_DTYPE,
                    STACKLESS_NODE_DTYPE, STATUS_BAD_INDEX, STATUS_ITER_CAP, STATUS_STACK_OVERFLOW,
@@ -34,6 +35,7 @@ __all__ = [
     "GpuWide",
     "autotune_batched", "ShadeParams", "SHADE_RECORD_DTYPE", "ShadingModel", "FORMAT_WIDE4", "FORMAT_WIDE8",
     "WIDE_GROUP_DTYPE", "TRI_RECORD_DTYPE", "WideBVH", "wide_build", "DeviceWide", "upload_wide",
+    "COLLAPSE_GREEDY", "COLLAPSE_SAH", "WideOptions",
     "TEXTURE_REF_DTYPE", "TEXTURE_DTYPE", "HIT_ATTR_DTYPE", "MaterialTable", "hit_attributes",
     "PATH_PARAMS_DTYPE", "PATH_SAMPLE_DTYPE", "path_params", "trace_paths", "hash64", "bvh_key", "load_bvh",
     "CacheHeader", "E_IO", "E_CORRUPT", "E_STALE", "RAYGEN_PARAMS_DTYPE", "PROBE_GRID_DTYPE", "raygen_params",
@@ -278,15 +280,22 @@ def _view(addr: int, count: int, dt: np.dtype) -> np.ndarray:
 
 
 def wide_build(src_format: int, nodes: np.ndarray, triangles: np.ndarray, vertices: np.ndarray,
-               entities: np.ndarray, width: int = 4, max_leaf: int = 2) -> WideBVH:
-    """crt_wide_build: the BVH4 / BVH8 layout derived from BVH2 buffers (host copies)."""
+               entities: np.ndarray, width: int = 4, max_leaf: int = 2, collapse: int = COLLAPSE_GREEDY,
+               node_cost: float = 0.0, tri_cost: float = 0.0) -> WideBVH:
+    """crt_wide_build / crt_wide_build_ex: the BVH4 / BVH8 layout derived from BVH2 buffers (host copies).
+    collapse: COLLAPSE_GREEDY (crt_wide_build's) or COLLAPSE_SAH with the node / triangle costs (0 = defaults)."""
     nodes = np.ascontiguousarray(nodes, dtype=node_dtype(src_format))
     triangles = np.ascontiguousarray(triangles, dtype=TRIANGLE_DTYPE)
     vertices = np.ascontiguousarray(vertices, dtype=VERTEX_DTYPE)
     entities = np.ascontiguousarray(entities, dtype=ENTITY_DTYPE)
     h = C.c_void_p()
-    check(lib().crt_wide_build(src_format, ptr(nodes), nodes.size, ptr(triangles), triangles.size, ptr(vertices),
-                               vertices.size, ptr(entities), entities.size, int(width), int(max_leaf), C.byref(h)))
+    args = (src_format, ptr(nodes), nodes.size, ptr(triangles), triangles.size, ptr(vertices), vertices.size,
+            ptr(entities), entities.size)
+    if collapse == COLLAPSE_GREEDY and node_cost == 0.0 and tri_cost == 0.0:
+        check(lib().crt_wide_build(*args, int(width), int(max_leaf), C.byref(h)))
+    else:
+        opt = WideOptions(int(width), int(max_leaf), int(collapse), float(node_cost), float(tri_cost))
+        check(lib().crt_wide_build_ex(*args, C.byref(opt), C.byref(h)))
     try:
         st = WideStats()
         check(lib().crt_wide_get_stats(h, C.byref(st)))
@@ -539,11 +548,12 @@ class DeviceWide:
 
 
 def upload_wide(src_format: int, host_buffers: dict, width: int = 4, max_leaf: int = 2, triangles=None,
-                vertices=None) -> DeviceWide:
+                vertices=None, collapse: int = COLLAPSE_GREEDY, node_cost: float = 0.0,
+                tri_cost: float = 0.0) -> DeviceWide:
     """Derives the wide layout from a context's host buffers and uploads it."""
     hb = host_buffers
     return DeviceWide(wide_build(src_format, hb["nodes"], hb["triangles"], hb["vertices"], hb["entities"], width,
-                                 max_leaf), triangles, vertices)
+                                 max_leaf, collapse, node_cost, tri_cost), triangles, vertices)
 
 
 def _colmajor(m) -> np.ndarray:

@@ -117,6 +117,14 @@ class WideStats(C.Structure):
                 ("width", C.c_int32), ("max_depth", C.c_int32), ("mean_fill", C.c_double)]
 
 
+class WideOptions(C.Structure):
+    _fields_ = [("width", C.c_int32), ("max_leaf", C.c_int32), ("collapse", C.c_int32), ("node_cost", C.c_float),
+                ("tri_cost", C.c_float)]
+
+
+COLLAPSE_GREEDY, COLLAPSE_SAH = 0, 1
+
+
 class TraceConfig(C.Structure):
     _fields_ = [("block_size", C.c_int32), ("lds_stack", C.c_int32), ("kernel", C.c_int32), ("variant", C.c_int32),
                 ("grid", C.c_int32), ("reserved", C.c_int32), ("tri_tests", C.c_void_p), ("queue", C.c_void_p)]
@@ -246,6 +254,8 @@ SIGNATURES = {
                                                C.POINTER(C.c_float)]),
     "crt_wide_build": (C.c_int, [C.c_int, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64,
                                  C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_void_p)]),
+    "crt_wide_build_ex": (C.c_int, [C.c_int, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64,
+                                    C.c_void_p, C.c_int32, C.POINTER(WideOptions), C.POINTER(C.c_void_p)]),
     "crt_wide_free": (None, [C.c_void_p]),
     "crt_wide_get_stats": (C.c_int, [C.c_void_p, C.POINTER(WideStats)]),
     "crt_wide_groups": (C.c_void_p, [C.c_void_p]),

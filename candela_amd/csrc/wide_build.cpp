@@ -191,9 +191,17 @@ struct Collapser {
     int32_t width;
     int32_t max_leaf;
     std::vector<crt_wide_group>& groups;
+    int32_t mode = CRT_COLLAPSE_GREEDY;
+    float node_cost = 1.0f;   // CRT_COLLAPSE_SAH: cost of one wide-node visit ...
+    float tri_cost = 0.5f;    // ... and of one triangle test, both per unit of half area
     int64_t leaves = 0;
     int64_t used_slots = 0;
     int32_t max_depth = 0;
+    // CRT_COLLAPSE_SAH tables, per decoded node: best[n * width + i - 1] is the
+    // least cost of covering the subtree with at most i slots; as_leaf[n] says
+    // whether one slot holds it best as a leaf (else as a wide node)
+    std::vector<float> best;
+    std::vector<uint8_t> as_leaf;
 
     int32_t groups_per_node() const { return width / 4; }
 
@@ -215,6 +223,76 @@ struct Collapser {
         return first;
     }
 
+    bool slot_is_leaf(const Decoded& d, int32_t k) const {
+        if (mode == CRT_COLLAPSE_SAH) return as_leaf[size_t(k)] != 0;
+        return is_leaf(d.nodes[size_t(k)]);
+    }
+
+    // Least cost of covering the subtree of n with at most j >= 2 slots when n
+    // itself takes none (its two children share the j slots); k_out = the
+    // left child's share.
+    float split_cost(const Node2& n, int32_t j, int32_t* k_out) const {
+        float c = std::numeric_limits<float>::infinity();
+        int32_t bk = 1;
+        for (int32_t k = 1; k < j; ++k) {
+            const float v = best[size_t(n.left) * size_t(width) + size_t(k - 1)] +
+                            best[size_t(n.right) * size_t(width) + size_t(j - k - 1)];
+            if (v < c) {
+                c = v;
+                bk = k;
+            }
+        }
+        if (k_out) *k_out = bk;
+        return c;
+    }
+
+    // Fills best / as_leaf bottom-up (children have larger ids than parents).
+    // A subtree in one slot costs area * tri_cost * triangles as a leaf (only
+    // where the greedy collapse could make it one: a BVH2 leaf, or at most
+    // max_leaf contiguous triangles) or area * node_cost plus its best cover
+    // by `width` slots as a wide node; with more slots it may instead hand
+    // them to its two children.
+    void plan_sah(const Decoded& d) {
+        const size_t n = d.nodes.size();
+        best.assign(n * size_t(width), 0.0f);
+        as_leaf.assign(n, 0);
+        for (int64_t i = int64_t(n) - 1; i >= 0; --i) {
+            const Node2& c = d.nodes[size_t(i)];
+            float* b = &best[size_t(i) * size_t(width)];
+            const float area = std::max(half_area(c), 0.0f);
+            if (c.left < 0) {
+                const float leaf = area * tri_cost * float(c.count);
+                for (int32_t j = 0; j < width; ++j) b[j] = leaf;
+                as_leaf[size_t(i)] = 1;
+                continue;
+            }
+            const float leaf = is_leaf(c) ? area * tri_cost * float(c.tris) : std::numeric_limits<float>::infinity();
+            const float node = area * node_cost + split_cost(c, width, nullptr);
+            as_leaf[size_t(i)] = leaf <= node ? 1 : 0;
+            b[0] = std::min(leaf, node);
+            for (int32_t j = 2; j <= width; ++j) b[j - 1] = std::min(b[0], split_cost(c, j, nullptr));
+        }
+    }
+
+    // The slots of a wide node at n under the SAH plan: the best cover of n's
+    // subtree by `width` slots with n itself opened.
+    void split_sah(const Decoded& d, const Node2& r, int32_t j, std::vector<int32_t>& out) const {
+        int32_t k = 1;
+        split_cost(r, j, &k);
+        cover_sah(d, r.left, k, out);
+        cover_sah(d, r.right, j - k, out);
+    }
+
+    void cover_sah(const Decoded& d, int32_t id, int32_t j, std::vector<int32_t>& out) const {
+        const Node2& c = d.nodes[size_t(id)];
+        const float* b = &best[size_t(id) * size_t(width)];
+        if (j == 1 || c.left < 0 || b[j - 1] >= b[0]) {
+            out.push_back(id);
+            return;
+        }
+        split_sah(d, c, j, out);
+    }
+
     // Collapses the decoded tree into wide nodes; returns the root's index.
     int collapse(const Decoded& d, int64_t& root) {
         struct Item {
@@ -223,6 +301,7 @@ struct Collapser {
             int32_t depth;
         };
         std::vector<Item> work;
+        if (mode == CRT_COLLAPSE_SAH) plan_sah(d);
         root = alloc(1);
         work.push_back({root, 0, 1});
         std::vector<int32_t> kids;
@@ -232,13 +311,19 @@ struct Collapser {
             max_depth = std::max(max_depth, it.depth);
             const Node2& r = d.nodes[size_t(it.node)];
             kids.clear();
-            if (is_leaf(r)) {
+            if (mode == CRT_COLLAPSE_SAH) {
+                if (r.left < 0) {
+                    kids.push_back(it.node);
+                } else {
+                    split_sah(d, r, width, kids);
+                }
+            } else if (is_leaf(r)) {
                 kids.push_back(it.node);
             } else {
                 kids.push_back(r.left);
                 kids.push_back(r.right);
             }
-            while (int32_t(kids.size()) < width) {
+            while (mode == CRT_COLLAPSE_GREEDY && int32_t(kids.size()) < width) {
                 int best = -1;
                 float best_area = -1.0f;
                 for (int k = 0; k < int(kids.size()); ++k) {
@@ -256,7 +341,7 @@ struct Collapser {
                 kids.insert(kids.begin() + best + 1, c.right);
             }
             int64_t inner = 0;
-            for (int32_t k : kids) inner += is_leaf(d.nodes[size_t(k)]) ? 0 : 1;
+            for (int32_t k : kids) inner += slot_is_leaf(d, k) ? 0 : 1;
             const int64_t base = inner ? alloc(inner) : 0;
             int64_t next = base;
             for (int slot = 0; slot < int(kids.size()); ++slot) {
@@ -269,7 +354,7 @@ struct Collapser {
                 g.hi_x[j] = c.hi[0];
                 g.hi_y[j] = c.hi[1];
                 g.hi_z[j] = c.hi[2];
-                if (is_leaf(c)) {
+                if (slot_is_leaf(d, kids[size_t(slot)])) {
                     g.child[j] = c.left < 0 ? c.start : c.first;
                     g.count[j] = int32_t(c.left < 0 ? c.count : c.tris);
                     ++leaves;
@@ -297,9 +382,26 @@ int crt_wide_build(int src_format, const void* nodes, int64_t node_count, const 
                    int64_t triangle_count, const crt_vertex* vertices, int64_t vertex_count,
                    const crt_entity* entities, int32_t entity_count, int32_t width, int32_t max_leaf,
                    crt_wide** out) {
+    const crt_wide_options o{width, max_leaf, CRT_COLLAPSE_GREEDY, 0.0f, 0.0f};
+    return crt_wide_build_ex(src_format, nodes, node_count, triangles, triangle_count, vertices, vertex_count,
+                             entities, entity_count, &o, out);
+}
+
+int crt_wide_build_ex(int src_format, const void* nodes, int64_t node_count, const crt_triangle* triangles,
+                      int64_t triangle_count, const crt_vertex* vertices, int64_t vertex_count,
+                      const crt_entity* entities, int32_t entity_count, const crt_wide_options* options,
+                      crt_wide** out) {
     using namespace crt;
     if (!out) return set_error(CRT_E_INVALID, "crt_wide_build: out is null");
     *out = nullptr;
+    if (!options) return set_error(CRT_E_INVALID, "crt_wide_build: options is null");
+    const int32_t width = options->width, max_leaf = options->max_leaf;
+    if (options->collapse != CRT_COLLAPSE_GREEDY && options->collapse != CRT_COLLAPSE_SAH)
+        return set_error(CRT_E_INVALID, "crt_wide_build: collapse must be CRT_COLLAPSE_GREEDY or CRT_COLLAPSE_SAH");
+    const float node_cost = options->node_cost == 0.0f ? 1.0f : options->node_cost;
+    const float tri_cost = options->tri_cost == 0.0f ? 0.5f : options->tri_cost;
+    if (!(node_cost > 0.0f) || !(tri_cost > 0.0f) || !std::isfinite(node_cost) || !std::isfinite(tri_cost))
+        return set_error(CRT_E_INVALID, "crt_wide_build: node_cost and tri_cost must be positive and finite");
     if (src_format != CRT_FORMAT_STACKLESS && src_format != CRT_FORMAT_STACK)
         return set_error(CRT_E_INVALID, "crt_wide_build: source format must be STACKLESS or STACK");
     if (width != 4 && width != 8) return set_error(CRT_E_INVALID, "crt_wide_build: width must be 4 or 8");
@@ -332,6 +434,9 @@ int crt_wide_build(int src_format, const void* nodes, int64_t node_count, const 
             r.pad = 0;
         }
         Collapser col{width, max_leaf, w->groups};
+        col.mode = options->collapse;
+        col.node_cost = node_cost;
+        col.tri_cost = tri_cost;
         std::map<std::pair<int32_t, int32_t>, std::pair<int32_t, int32_t>> built;  // (offset, count) -> (root, nodes)
         w->entities.assign(entities, entities + entity_count);
         Decoded d;

@@ -493,6 +493,32 @@ int crt_wide_build(int src_format, const void* nodes, int64_t node_count, const 
                    int64_t triangle_count, const crt_vertex* vertices, int64_t vertex_count,
                    const crt_entity* entities, int32_t entity_count, int32_t width, int32_t max_leaf,
                    crt_wide** out);
+
+/* Collapse policies of crt_wide_build_ex. */
+#define CRT_COLLAPSE_GREEDY 0   /* crt_wide_build's: open the child of largest area */
+#define CRT_COLLAPSE_SAH 1      /* least surface-area cost over all collapses */
+
+typedef struct crt_wide_options {
+    int32_t width;      /* 4 or 8 */
+    int32_t max_leaf;   /* 2..15 */
+    int32_t collapse;   /* CRT_COLLAPSE_GREEDY or CRT_COLLAPSE_SAH */
+    float node_cost;    /* SAH: cost of visiting one wide node (0 = 1.0) */
+    float tri_cost;     /* SAH: cost of testing one triangle (0 = 0.5) */
+} crt_wide_options;
+
+/* crt_wide_build with a choice of collapse (no reference counterpart).
+ * CRT_COLLAPSE_SAH: dynamic programming over the BVH2, bottom-up, finds the
+ * collapse of least expected cost, where a wide node costs
+ * half_area * node_cost and a leaf half_area * tri_cost * triangles.  A
+ * subtree may still become a leaf only where the greedy collapse allows it
+ * (a BVH2 leaf, or at most max_leaf contiguous triangles), but the cost
+ * decides whether it does; each wide node's slots are the cheapest cover of
+ * its subtree by `width` subtrees.  Child boxes, leaf ranges and records are
+ * those of crt_wide_build, so the walks report the same nearest hits. */
+int crt_wide_build_ex(int src_format, const void* nodes, int64_t node_count, const crt_triangle* triangles,
+                      int64_t triangle_count, const crt_vertex* vertices, int64_t vertex_count,
+                      const crt_entity* entities, int32_t entity_count, const crt_wide_options* options,
+                      crt_wide** out);
 void crt_wide_free(crt_wide* wide);
 int crt_wide_get_stats(const crt_wide* wide, crt_wide_stats* out);
 const crt_wide_group* crt_wide_groups(const crt_wide* wide);       /* node_count * width / 4 */

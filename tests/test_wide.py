@@ -44,13 +44,15 @@ def _slots(w):
     return lo, hi, g["child"].reshape(n, w.width), g["count"].reshape(n, w.width)
 
 
+@pytest.mark.parametrize("collapse", [0, 1])    # COLLAPSE_GREEDY, COLLAPSE_SAH
 @pytest.mark.parametrize("fmt", [0, 1])
 @pytest.mark.parametrize("width,max_leaf", [(4, 2), (8, 2), (4, 6), (8, 4)])
-def test_structure(crt, fmt, width, max_leaf):
+def test_structure(crt, fmt, width, max_leaf, collapse):
     from candela_amd import scenes
     meshes = scenes.icosphere_cloud(count=60, subdivisions=2, seed=11, box=20.0)
     _, hb = _context(crt, fmt, meshes)
-    w = crt.wide_build(fmt, hb["nodes"], hb["triangles"], hb["vertices"], hb["entities"], width, max_leaf)
+    w = crt.wide_build(fmt, hb["nodes"], hb["triangles"], hb["vertices"], hb["entities"], width, max_leaf,
+                       collapse)
     lo, hi, child, count = _slots(w)
     n = lo.shape[0]
     assert w.stats["node_count"] == n and w.stats["width"] == width
