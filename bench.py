@@ -400,7 +400,13 @@ def time_large(crt, args) -> dict:
     r.push_entity(0)
     r.buffer_entities()                              # materialises the entity record
     hb = r.host_buffers()
-    dw = crt.upload_wide(crt.FORMAT_STACKLESS, hb, 4, 2)     # one host collapse, reused by the parity sample
+    lay = args.large_layout
+    if len(lay) < 2 or lay[0] not in "ls" or not lay[1:].isdigit():
+        raise SystemExit(f"--large-layout {lay}: expected lN or sN")
+    sah = lay[0] == "s"
+    # one host collapse, reused by the parity sample
+    dw = crt.upload_wide(crt.FORMAT_STACKLESS, hb, 4, int(lay[1:]), collapse=crt.COLLAPSE_SAH if sah else 0,
+                         tri_cost=SAH_TRI_COST if sah else 0.0)
     t3 = time.perf_counter()
     ntri = int(hb["triangles"].size)
     nmesh = int(hb["triangles"]["packed"][:, 3].max()) + 1
@@ -458,7 +464,7 @@ def time_large(crt, args) -> dict:
         parity = None
     ms = best[1]
     out_d = {"workload": f"terrain {2048 // d}^2 + {1400 // d} icospheres ({ntri} triangles), {W}x{H}x{spp} paths, "
-                         "2 diffuse bounces, roulette 0.8, BVH4 layout",
+                         f"2 diffuse bounces, roulette 0.8, BVH4 layout {lay} ({dw.host.stats['node_count']} nodes)",
              "triangles": ntri, "paths": n, "segments": segs, "ms_per_frame": round(ms, 3),
              "mrays_s": round(segs / ms / 1e3, 1), "mpaths_s": round(n / ms / 1e3, 1), "preset": best[0],
              "mean_segments": round(segs / n, 4), "bytes_per_frame": nbytes,
@@ -538,6 +544,9 @@ def main(argv=None) -> int:
                     help="divide the large scene's terrain grid, sphere count and image size by D (tests)")
     ap.add_argument("--large-preset", type=int, default=-1, metavar="V",
                     help="time only path-kernel preset V on the large scene (default: every preset)")
+    ap.add_argument("--large-layout", default="s2", metavar="L",
+                    help="the large scene's BVH4: lN = greedy collapse with leaves of up to N triangles, sN = SAH "
+                         "collapse (default s2)")
     ap.add_argument("--large-only", action="store_true",
                     help="run only the large-scene path-tracing workload and print its record as one JSON line")
     ap.add_argument("--detail", default="", metavar="PATH",
