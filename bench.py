@@ -135,6 +135,13 @@ def wide_key(c: dict) -> tuple:
     return c["width"], c["max_leaf"], c.get("collapse", 0)
 
 
+def layout_name(layout) -> str:
+    """bvh2, wide4, wide8, or wide4_sah for the SAH collapse, of a batched-workload layout key."""
+    if layout is None:
+        return "bvh2"
+    return f"wide{layout[0]}" + ("_sah" if len(layout) > 2 and layout[2] else "")
+
+
 def gi_choice_name(c: dict) -> str:
     """The choice of the GI launch alone."""
     return choice_name({k: v for k, v in c.items() if not k.startswith("ao_")})
@@ -250,7 +257,7 @@ def batched_vs_plain(crt, args, plain_choice: dict, stream) -> dict:
                          for bh, th in zip(batched_hits, timed_hits))
         same = all(bh[0].tobytes() == ph[0].tobytes() and bh[1].tobytes() == ph[1].tobytes()
                    for bh, ph in zip(batched_hits, plain_hits))
-        name = "bvh2" if layout is None else f"wide{layout[0]}"
+        name = layout_name(layout)
         per_layout[name] = {"plain_ms": round(plain_ms, 4), "plain_mrays_s": round(work.rays / plain_ms / 1e3, 2),
                             "plain_kernel": f"{kernel}:{variant}" + (
                                 f"+any:{ao_kernel}:{ao_variant}" + (":s" if ao_order == 1 else "")
@@ -267,7 +274,7 @@ def batched_vs_plain(crt, args, plain_choice: dict, stream) -> dict:
     t0 = time.perf_counter()
     idx, cfg, ms = crt.autotune_batched_layouts([work.batches(crt, k) for k in keys], default_candidates(crt, True),
                                                 repeats=2, stream=stream)
-    lib_choice = {"layout": "bvh2" if keys[idx] is None else f"wide{keys[idx][0]}", "ms": round(ms, 4),
+    lib_choice = {"layout": layout_name(keys[idx]), "ms": round(ms, 4),
                   "config": {k: getattr(cfg, k) for k in ("variant", "tiles_per_claim", "waves_per_group", "order",
                                                           "shards", "child_order")},
                   "tune_s": round(time.perf_counter() - t0, 1)}
@@ -896,6 +903,9 @@ def main(argv=None) -> int:
             bw = best_of(lambda r: r["layout"] == f"wide{w}" and r["max_leaf"] == 2 and not r.get("collapse"))
             if bw is not None:
                 plain_choice[(w, 2)] = (bw["kernel"], bw["variant"], *ao_of(bw), bw.get("ao_order", 0))
+        bs = best_of(lambda r: r["layout"] == "wide4" and r["max_leaf"] == 2 and r.get("collapse"))
+        if bs is not None:   # the SAH-collapsed BVH4 with 2-triangle leaves
+            plain_choice[(4, 2, crt.COLLAPSE_SAH)] = (bs["kernel"], bs["variant"], *ao_of(bs), bs.get("ao_order", 0))
         c2_batched = time_batched(crt, [(best_scene, crt.QUERY_ANY, d_ao, h_ao),
                                         (best_scene, crt.QUERY_CLOSEST, d_gi, h_gi)],
                                   K, args.warmup, stream, wide=best["layout"] != "bvh2")

@@ -110,14 +110,16 @@ class BatchedWorkload:
             out.append((sc, QUERY_ANY, s.d_shadow, s.h_shadow))
         return out
 
-    def build_wide(self, crt, width: int, max_leaf: int = 2) -> float:
-        """Derives and uploads the wide layout of every scene; returns the host seconds spent."""
+    def build_wide(self, crt, width: int, max_leaf: int = 2, collapse: int = 0, tri_cost: float = 0.0) -> float:
+        """Derives and uploads the wide layout of every scene (key (width, max_leaf), with the collapse as a
+        third element when it is not the greedy one); returns the host seconds spent."""
         import time
         t0 = time.perf_counter()
+        key = (width, max_leaf) if not collapse else (width, max_leaf, collapse)
         for s in self.scenes:
-            if (width, max_leaf) not in s.wide:
-                s.wide[(width, max_leaf)] = crt.upload_wide(FORMAT_STACKLESS, s.host, width, max_leaf,
-                                                            s.scene.triangles, s.scene.vertices)
+            if key not in s.wide:
+                s.wide[key] = crt.upload_wide(FORMAT_STACKLESS, s.host, width, max_leaf, s.scene.triangles,
+                                              s.scene.vertices, collapse, 0.0, tri_cost)
         return time.perf_counter() - t0
 
 
