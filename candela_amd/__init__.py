@@ -656,9 +656,14 @@ class RayIntersector:
         check(lib().crt_context_scene(self.handle, C.byref(s)))
         return s
 
-    def build_wide(self, width: int = 4, max_leaf: int = 2) -> None:
-        """crt_context_build_wide: derive and upload the BVH4 / BVH8 layout of the buffered scene."""
-        check(lib().crt_context_build_wide(self.handle, int(width), int(max_leaf)))
+    def build_wide(self, width: int = 4, max_leaf: int = 2, collapse: int = COLLAPSE_GREEDY, node_cost: float = 0.0,
+                   tri_cost: float = 0.0) -> None:
+        """crt_context_build_wide(_ex): derive and upload the BVH4 / BVH8 layout of the buffered scene."""
+        if collapse == COLLAPSE_GREEDY and node_cost == 0.0 and tri_cost == 0.0:
+            check(lib().crt_context_build_wide(self.handle, int(width), int(max_leaf)))
+        else:
+            opt = WideOptions(int(width), int(max_leaf), int(collapse), float(node_cost), float(tri_cost))
+            check(lib().crt_context_build_wide_ex(self.handle, C.byref(opt)))
 
     def wide_scene(self) -> SceneView:
         s = SceneView()

@@ -262,6 +262,7 @@ SIGNATURES = {
     "crt_wide_records": (C.c_void_p, [C.c_void_p]),
     "crt_wide_entities": (C.c_void_p, [C.c_void_p]),
     "crt_context_build_wide": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
+    "crt_context_build_wide_ex": (C.c_int, [C.c_void_p, C.POINTER(WideOptions)]),
     "crt_hit_attributes": (C.c_int, [C.POINTER(SceneView), C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, C.c_void_p,
                                      C.c_int64, C.c_void_p, C.c_void_p]),
     "crt_trace_paths": (C.c_int, [C.POINTER(SceneView), C.c_void_p, C.c_void_p, C.c_int64, C.POINTER(TraceConfig),

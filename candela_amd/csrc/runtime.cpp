@@ -772,7 +772,14 @@ int crt_context_scene(const crt_context* ctx, crt_scene_view* out) {
 }
 
 int crt_context_build_wide(crt_context* ctx, int32_t width, int32_t max_leaf) {
+    const crt_wide_options o{width, max_leaf, CRT_COLLAPSE_GREEDY, 0.0f, 0.0f};
+    return crt_context_build_wide_ex(ctx, &o);
+}
+
+int crt_context_build_wide_ex(crt_context* ctx, const crt_wide_options* options) {
     if (!ctx) return crt::set_error(CRT_E_INVALID, "crt_context_build_wide: ctx is null");
+    if (!options) return crt::set_error(CRT_E_INVALID, "crt_context_build_wide: options is null");
+    const int32_t width = options->width;
     if (!ctx->data_buffered || !ctx->d_entities)
         return crt::set_error(CRT_E_STATE, "crt_context_build_wide: call crt_buffer_data and crt_buffer_entities first");
     if (ctx->node_count() != ctx->nodes_buffered)
@@ -781,9 +788,9 @@ int crt_context_build_wide(crt_context* ctx, int32_t width, int32_t max_leaf) {
     const void* nodes = ctx->format == CRT_FORMAT_STACKLESS ? static_cast<const void*>(ctx->stackless.data())
                                                             : static_cast<const void*>(ctx->stack.data());
     crt_wide* w = nullptr;
-    int rc = crt_wide_build(ctx->format, nodes, ctx->node_count(), ctx->triangles.data(),
-                            int64_t(ctx->triangles.size()), ctx->vertices.data(), int64_t(ctx->vertices.size()),
-                            ctx->entities.data(), int32_t(ctx->entities.size()), width, max_leaf, &w);
+    int rc = crt_wide_build_ex(ctx->format, nodes, ctx->node_count(), ctx->triangles.data(),
+                               int64_t(ctx->triangles.size()), ctx->vertices.data(), int64_t(ctx->vertices.size()),
+                               ctx->entities.data(), int32_t(ctx->entities.size()), options, &w);
     if (rc) return rc;
     crt_wide_stats st;
     crt_wide_get_stats(w, &st);

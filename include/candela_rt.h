@@ -644,6 +644,8 @@ int crt_context_scene(const crt_context* ctx, crt_scene_view* out);
  * the buffered data and entities and uploads it (needs buffer_data without
  * clearing the CPU data, and buffer_entities).  No reference counterpart. */
 int crt_context_build_wide(crt_context* ctx, int32_t width, int32_t max_leaf);
+/* The same with a choice of collapse (crt_wide_build_ex's options). */
+int crt_context_build_wide_ex(crt_context* ctx, const crt_wide_options* options);
 /* Device scene of the wide layout (CRT_FORMAT_WIDE4 / WIDE8). */
 int crt_context_wide_scene(const crt_context* ctx, crt_scene_view* out);
 

@@ -95,6 +95,37 @@ def test_wide_cloud_incoherent(gpu, orc, src, width):
         assert np.array_equal(dt.download(), ot)
 
 
+@pytest.mark.parametrize("width,max_leaf", [(4, 2), (4, 4), (8, 2)])
+def test_wide_sah_context(gpu, orc, width, max_leaf):
+    """crt_context_build_wide_ex with the SAH collapse: the context's layout walks (wave kernel, speculative grid
+    and refilling kernels) against the host twins over crt_wide_build_ex's layout of the same buffers."""
+    from candela_amd import scenes
+    log(f"[case] SAH context width={width} max_leaf={max_leaf}")
+    r = gpu.RayIntersector(0)
+    r.add_object(0, scenes.icosphere_cloud(count=150, subdivisions=3, seed=29, box=35.0))
+    r.push_entity(0)
+    r.buffer_entities()
+    r.buffer_data()
+    r.build_wide(width, max_leaf, gpu.COLLAPSE_SAH, 0.0, 0.3)
+    hb = r.host_buffers()
+    w = gpu.wide_build(0, hb["nodes"], hb["triangles"], hb["vertices"], hb["entities"], width, max_leaf,
+                       gpu.COLLAPSE_SAH, 0.0, 0.3)
+    rays = np.concatenate([random_rays(40000, [-2, -2, -2], [37, 37, 37], seed=30),
+                           axis_aligned_rays(3000, [0, 0, 0], [35, 35, 35], seed=31)])
+    scene = r.wide_scene()
+    for q in (0, 1):
+        o = orc.trace_wide(width, w.groups, w.records, w.entities, q, rays, stack_cap=8 + 64)
+        compare(gpu_trace(gpu, scene, q, rays, kernel=3, variant=0), o, f"SAH w{width} l{max_leaf} wave q={q}")
+        os_ = orc.trace_wide_spec(width, w.groups, w.records, w.entities, q, rays, variant=1)
+        compare(gpu_trace(gpu, scene, q, rays, kernel=gpu.KERNEL_SPECULATIVE, variant=1), os_,
+                f"SAH w{width} l{max_leaf} speculative q={q}")
+        g = gpu_trace(gpu, scene, q, rays, kernel=gpu.KERNEL_SPECULATIVE_REFILL, variant=4)
+        assert np.array_equal(g["t"] > 0, o["t"] > 0)
+        if q == 0:
+            assert np.array_equal(g["t"], o["t"])
+    log(f"[parity] SAH context layout width {width} leaves {max_leaf}: equal to the twins")
+
+
 @pytest.mark.parametrize("width", [4, 8])
 def test_wide_wave_presets(gpu, orc, width):
     """The wave-scheduled and the refilling wide walks (every preset, including the occupancy-hinted builds

@@ -173,3 +173,10 @@ def test_sah_errors(crt):
     for nc, tc in ((-1.0, 0.5), (1.0, -0.5), (float("inf"), 0.5), (1.0, float("nan"))):
         with pytest.raises(crt.CrtError, match="cost"):
             crt.wide_build(*args, crt.COLLAPSE_SAH, nc, tc)
+    import ctypes as C
+    h = C.c_void_p()
+    rc = crt.lib().crt_wide_build_ex(1, crt.ptr(b.nodes), b.nodes.size, crt.ptr(b.triangles), b.triangles.size,
+                                     crt.ptr(b.vertices), b.vertices.size, crt.ptr(ent), ent.size, None, C.byref(h))
+    assert rc == -1 and b"options" in crt.lib().crt_last_error() and not h.value   # CRT_E_INVALID
+    with pytest.raises(crt.CrtError, match="width"):
+        crt.wide_build(1, b.nodes, b.triangles, b.vertices, ent, 5, 2, crt.COLLAPSE_SAH)
