@@ -330,6 +330,55 @@ def time_attributes(crt, scene, d_hits, n: int, hits: np.ndarray, stream, steps:
     return res
 
 
+def time_shading(crt, stream, steps: int, size: int = 1024) -> dict:
+    """Config 4: the differentiable Cook-Torrance + one-bounce shading of the boxes scene at size x size with 8
+    point lights.  The forward pass traces, per pixel with a primary hit, one closest-hit bounce ray and 8 any-hit
+    shadow rays from the hit, plus 8 from the bounce hit when there is one; the backward pass re-traces the 16
+    shadow rays (visibility recomputed, not stored) and adds the gradients.  HIP events around `steps` launches
+    of each pass on one stream."""
+    from candela_amd import workloads
+    pb = workloads.shading_problem(crt, size=size)
+    n = size * size
+    color = crt.DeviceArray(np.float32, 3 * n)
+    target = crt.DeviceArray(np.float32, 3 * n)
+    rec = crt.DeviceArray(crt.SHADE_RECORD_DTYPE, n)
+    pb.model.set(pb.target_albedo, pb.target_roughness, pb.target_intensity)
+    pb.model.forward(pb.scene, pb.rays, pb.hits, target, rec, stream=stream)
+    pb.model.set(pb.albedo, pb.roughness, pb.intensity)
+    ntri = pb.roughness.size
+    ga = crt.DeviceArray(np.float32, 3 * ntri)
+    gr = crt.DeviceArray(np.float32, ntri)
+    gi = crt.DeviceArray(np.float32, pb.intensity.size)
+    for d in (ga, gr, gi):
+        d.fill_bytes(0)
+    pb.model.forward(pb.scene, pb.rays, pb.hits, color, rec, stream=stream)
+    pb.model.backward(pb.scene, rec, color, target, ga, gr, gi, stream=stream)
+    stream.sync()
+    r = rec.download()
+    n_p = int(np.count_nonzero(r["tri0"] >= 0))
+    n_q = int(np.count_nonzero(r["tri1"] >= 0))
+    nl = int(pb.intensity.size)
+    fwd_rays = n_p + nl * (n_p + n_q)
+    bwd_rays = nl * (n_p + n_q)
+    ev = Events(crt, 3)
+    ev.record(0, stream.handle)
+    for _ in range(steps):
+        pb.model.forward(pb.scene, pb.rays, pb.hits, color, rec, stream=stream)
+    ev.record(1, stream.handle)
+    for _ in range(steps):
+        pb.model.backward(pb.scene, rec, color, target, ga, gr, gi, stream=stream)
+    ev.record(2, stream.handle)
+    stream.sync()
+    f_ms, b_ms = ev.ms(0, 1) / steps, ev.ms(1, 2) / steps
+    ev.close()
+    res = {"pixels": n, "lights": nl, "triangles": ntri, "forward_ms": round(f_ms, 4), "backward_ms": round(b_ms, 4),
+           "step_ms": round(f_ms + b_ms, 4), "forward_rays": fwd_rays, "backward_rays": bwd_rays,
+           "forward_mrays_s": round(fwd_rays / f_ms / 1e3, 1), "backward_mrays_s": round(bwd_rays / b_ms / 1e3, 1)}
+    log(f"config 4 shading: forward {f_ms:.3f} ms ({res['forward_mrays_s']} Mrays/s), backward {b_ms:.3f} ms "
+        f"({res['backward_mrays_s']} Mrays/s) at {size}x{size}, {nl} lights")
+    return res
+
+
 def time_raygen(crt, scene, d_rays, d_hits, rays: np.ndarray, hits: np.ndarray, hb: dict, stream, steps: int) -> dict:
     """crt_gen_diffuse_rays over the 1920x1080 primary G-buffer: AO + GI rays of every pixel that hit, compacted
     (count, scan, emit).  Algorithmic bytes: 32 (hit, count pass) + 64 (ray + hit, emit pass) per record, and
@@ -877,6 +926,15 @@ def main(argv=None) -> int:
     # --- wavefront ray generation from the primary G-buffer: one launch chain, HIP events -------
     raygen = time_raygen(crt, scene, drays, dhits, prays, phits, hb, stream, K)
 
+    # --- config 4: differentiable shading, forward and backward passes ------------------------
+    shading = None
+    if not args.only:
+        try:
+            shading = time_shading(crt, stream, K)
+        except Exception as e:   # reported, never fatal to the headline line
+            shading = {"error": f"{type(e).__name__}: {e}"}
+            log(f"shading failed: {shading['error']}")
+
     # --- large scene: multi-bounce paths with in-kernel regeneration --------------------------
     large = None
     if not args.no_large and not args.only:
@@ -1040,6 +1098,7 @@ def main(argv=None) -> int:
         "c2_batched": c2_batched,
         "attributes": attrs,
         "raygen": raygen,
+        "shading": shading,
         "gpu_build": gpu_build,
         "large_scene": large,
         "cpu_baseline": cpu,
@@ -1090,6 +1149,7 @@ def main(argv=None) -> int:
                                       "error")),
         "attributes_mhits_s": attrs.get("mhits_s") if attrs else None,
         "raygen_mrays_s": raygen.get("mrays_s") if raygen else None,
+        "shading": pick(shading, ("forward_ms", "backward_ms", "forward_mrays_s", "backward_mrays_s", "error")),
     }
     print(json.dumps(line), flush=True)
     return 0

@@ -273,23 +273,33 @@ __device__ __forceinline__ void wave_scatter_add(float* __restrict__ base, int32
         const int32_t lidx = __shfl(idx, leader);
         const bool mine = idx == lidx && idx >= 0;
         const unsigned long long m = __ballot(mine);
+        if (m == (1ull << leader)) {
+            // the leader alone holds this index: its own values, no reduction
+            if (lane == unsigned(leader)) {
 #pragma unroll
-        for (int k = 0; k < NV; ++k) {
-            const float sum = wave_sum(mine ? v[k] : 0.0f);
-            if (lane == unsigned(leader)) atomicAdd(base + int64_t(lidx) * NV + k, sum);
+                for (int k = 0; k < NV; ++k) atomicAdd(base + int64_t(lidx) * NV + k, v[k]);
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < NV; ++k) {
+                const float sum = wave_sum(mine ? v[k] : 0.0f);
+                if (lane == unsigned(leader)) atomicAdd(base + int64_t(lidx) * NV + k, sum);
+            }
         }
         pending &= ~m;
     }
 }
 
+// The adjoint of one pixel (a tile of 128 pixels per workgroup: every lane
+// of a wave calls it, with i >= npix for lanes past the end): albedo and
+// roughness gradients are added to the triangles' entries, and each light's
+// intensity gradient, summed over the wave, into the workgroup's LDS sum.
 template <int FORMAT>
-__global__ void __launch_bounds__(128) backward_kernel(SceneArgs s, const crt_shade_record* __restrict__ rec,
-                                                       int64_t npix, Params P, const float* __restrict__ color,
-                                                       const float* __restrict__ target, float* __restrict__ g_albedo,
-                                                       float* __restrict__ g_rough, float* __restrict__ g_inten) {
-    __shared__ int32_t lds[16 * 128];
-    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    int32_t* my = lds + threadIdx.x;
+__device__ __forceinline__ void backward_pixel(const SceneArgs& s, const crt_shade_record* __restrict__ rec,
+                                               int64_t i, int64_t npix, const Params& P,
+                                               const float* __restrict__ color, const float* __restrict__ target,
+                                               float* __restrict__ g_albedo, float* __restrict__ g_rough,
+                                               float* inten_sum, int32_t* my) {
     crt_shade_record R;
     R.tri0 = -1;
     R.tri1 = -1;
@@ -311,7 +321,7 @@ __global__ void __launch_bounds__(128) backward_kernel(SceneArgs s, const crt_sh
     }
     float SAkd0 = 0.0f, Sd0 = 0.0f, B1 = 0.0f;
     // the light loop is wave-uniform so each light's intensity gradient is
-    // reduced in-wave and added with one atomic
+    // reduced in-wave and added to the workgroup's sum with one LDS atomic
     for (int32_t l = 0; l < P.nl; ++l) {
         float dI = 0.0f;
         if (valid) {
@@ -331,7 +341,7 @@ __global__ void __launch_bounds__(128) backward_kernel(SceneArgs s, const crt_sh
             }
         }
         const float sum = wave_sum(dI);
-        if (__lane_id() == 0) atomicAdd(g_inten + l, sum);
+        if (__lane_id() == 0) atomicAdd(inten_sum + l, sum);
     }
     const float a0[3] = {g.x * (SAkd0 + alb1.x * B1), g.y * (SAkd0 + alb1.y * B1), g.z * (SAkd0 + alb1.z * B1)};
     const float r0g[1] = {(g.x + g.y + g.z) * Sd0};
@@ -339,6 +349,28 @@ __global__ void __launch_bounds__(128) backward_kernel(SceneArgs s, const crt_sh
     wave_scatter_add<3>(g_albedo, valid ? t0 : -1, a0);
     wave_scatter_add<1>(g_rough, valid ? t0 : -1, r0g);
     wave_scatter_add<3>(g_albedo, t1, a1);
+}
+
+// Workgroups of 128 lanes take tiles of 128 pixels in a grid-stride loop
+// (the launch caps the grid at a few workgroups per CU), so the intensity
+// gradients, which every pixel adds to, leave each workgroup as one global
+// atomic per light instead of one per wave: the few addresses they share are
+// otherwise the kernel's bottleneck.
+template <int FORMAT>
+__global__ void __launch_bounds__(128) backward_kernel(SceneArgs s, const crt_shade_record* __restrict__ rec,
+                                                       int64_t npix, Params P, const float* __restrict__ color,
+                                                       const float* __restrict__ target, float* __restrict__ g_albedo,
+                                                       float* __restrict__ g_rough, float* __restrict__ g_inten) {
+    __shared__ int32_t lds[16 * 128];
+    __shared__ float inten_sum[kMaxLights];
+    if (threadIdx.x < unsigned(kMaxLights)) inten_sum[threadIdx.x] = 0.0f;
+    __syncthreads();
+    int32_t* my = lds + threadIdx.x;
+    const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+    for (int64_t base = int64_t(blockIdx.x) * blockDim.x; base < npix; base += stride)
+        backward_pixel<FORMAT>(s, rec, base + threadIdx.x, npix, P, color, target, g_albedo, g_rough, inten_sum, my);
+    __syncthreads();
+    if (threadIdx.x < unsigned(P.nl)) atomicAdd(g_inten + threadIdx.x, inten_sum[threadIdx.x]);
 }
 
 }  // namespace shade
@@ -398,7 +430,14 @@ int crt_shade_backward(const crt_scene_view* scene, const crt_shade_record* reco
                                   !grad_intensity)))
         return crt::set_error(CRT_E_INVALID, "crt_shade_backward: bad buffers");
     if (npix == 0) return CRT_OK;
-    const unsigned grid = unsigned((npix + 127) / 128);
+    // at most 16 workgroups of 128 lanes per CU (more than fit at once), so
+    // each workgroup walks several tiles
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+        cus = 256;
+    const int64_t tiles = (npix + 127) / 128;
+    const unsigned grid = unsigned(tiles < int64_t(cus) * 16 ? tiles : int64_t(cus) * 16);
     hipStream_t st = static_cast<hipStream_t>(stream);
     if (a.format == CRT_FORMAT_STACKLESS)
         hipLaunchKernelGGL(crt::shade::backward_kernel<CRT_FORMAT_STACKLESS>, dim3(grid), dim3(128), 0, st, a,
