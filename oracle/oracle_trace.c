@@ -801,6 +801,7 @@ void oracle_primary_rays(int width, int height, const float* inv_view, const flo
  *              (order, t) -- which ends an any-hit walk; every queue empties.
  * ------------------------------------------------------------------------- */
 #define SPEC_MAX_RECORD (1 << 26)
+#define SPEC_END_BATCH 32
 
 typedef struct {
     v3 ro, rd, inv;
@@ -1059,6 +1060,14 @@ static void* spec_run_job(void* arg) {
             int queued = 0;
             for (int l = 0; l < 64; ++l) queued |= lanes[l].nq > 0;
             if (queued) spec_leaf_phase(J, lanes, list);
+            /* the walk ends run once SPEC_END_BATCH lanes wait or no lane can
+               step (spec_walk.h: kEndBatch) */
+            int ending = 0, can = 0;
+            for (int l = 0; l < 64; ++l) {
+                ending += lanes[l].active && lanes[l].walk_done;
+                can |= spec_can_step(J, &lanes[l]);
+            }
+            if (!ending || (ending < SPEC_END_BATCH && can)) continue;
             for (int l = 0; l < 64; ++l) {
                 spec_lane_t* L = &lanes[l];
                 if (L->active && L->walk_done) {
