@@ -89,7 +89,7 @@ constexpr int32_t kMaxRecord = 1 << 26;
 // Lanes whose walks are over wait for the walk-end code until this many of
 // them do, or until no lane can step (measured on config 2's GI rays: 32
 // takes the refilling launch from 0.590 to 0.555 ms; 16 gives 0.585, 48 0.562).
-constexpr int32_t kEndBatch = 32;
+constexpr int32_t kEndBatch = kWalkEndBatch;
 
 // The query is ANY, or with DYN the member dyn_any, set per ray batch (one
 // compiled walk for kernels that serve both queries: the batched kernel).

@@ -288,7 +288,9 @@ struct StackWave {
             }
         }
         // --- walk ends: next entity or the ray's result --------------------------
-        if (__ballot(walk_done)) {
+        // the ends run once kWalkEndBatch lanes wait or no lane can step
+        const unsigned long long ends = __ballot(walk_done);
+        if (ends && (__popcll(ends) >= kWalkEndBatch || !__ballot(active && !walk_done && !leafing))) {
             if (walk_done) {
                 walk_done = false;
                 visits += uint32_t(it);

@@ -258,7 +258,9 @@ struct WaveWalk {
             }
 
             // --- walk ends: next entity or the ray's result ----------------------
-            if (__ballot(walk_done)) {
+            // the ends run once kWalkEndBatch lanes wait or no lane can step
+            const unsigned long long ends = __ballot(walk_done);
+            if (ends && (__popcll(ends) >= kWalkEndBatch || !__ballot(active && !parked && !walk_done))) {
                 if (walk_done) {
                     walk_done = false;
                     L.visits += uint32_t(L.it);
