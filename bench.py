@@ -38,7 +38,8 @@ sys.path.insert(0, ROOT)
 
 METRIC = "incoherent Mrays/s, batched vs plain launch on 1 MI355X; % of HBM roofline"
 HBM_PEAK_GBS = 8000.0
-KERNEL_NAMES = {1: "trace_kernel", 2: "persistent_stackless", 3: "grid_stackless"}
+KERNEL_NAMES = {1: "trace_kernel", 2: "persistent_stackless", 3: "grid_stackless", 4: "sbvh2::grid (speculative)",
+                5: "sbvh2::persistent (speculative, lane refill)"}
 
 
 def log(msg: str) -> None:
@@ -739,6 +740,7 @@ def main(argv=None) -> int:
     candidates = [bvh2(crt.KERNEL_THREAD_PER_RAY, 0)]
     if fmt == crt.FORMAT_STACKLESS:
         candidates += [bvh2(crt.KERNEL_WAVE, v) for v in range(6)] + [bvh2(crt.KERNEL_PERSISTENT, v) for v in range(6)]
+        candidates += [bvh2(k, v) for k in (crt.KERNEL_SPECULATIVE, crt.KERNEL_SPECULATIVE_REFILL) for v in range(4)]
         if not args.no_wide:
             candidates += [wide(w, 2, crt.KERNEL_THREAD_PER_RAY) for w in (4, 8)]
             candidates += [wide(w, leaf, k, v) for w in (4, 8) for leaf in (2, 4)

@@ -27,7 +27,7 @@ ARCH = os.environ.get("CRT_OFFLOAD_ARCH", "gfx950")
 
 HOST_SOURCES = ["bvh_build.cpp", "runtime.cpp", "wide_build.cpp", "scene_cache.cpp"]
 HIP_SOURCES = ["trace.hip", "persistent.hip", "stack.hip", "spec.hip", "batched.hip", "shade.hip", "wide.hip", "attr.hip", "path.hip",
-               "raygen.hip", "gpu_build.hip", "wide_gpu.hip", "noalloc.hip"]
+               "raygen.hip", "gpu_build.hip", "wide_gpu.hip", "noalloc.hip", "spec_bvh2.hip"]
 
 HIP_FLAGS = [
     f"--offload-arch={ARCH}",

@@ -368,20 +368,22 @@ enum {
     CRT_KERNEL_WAVE = 3,        /* one ray per lane, leaf batching, XCD-aware block
                                    order (stackless: + node window; wide formats:
                                    leaves of a node parked and tested by the wave) */
-    CRT_KERNEL_SPECULATIVE = 4, /* wide formats only: one ray per lane, lanes never
-                                   wait for leaf tests: hit leaves go to a per-lane
-                                   queue, and after a round of node steps the wave
-                                   tests the queued leaves 64 at a time, with
-                                   culling by the tmax of the last such phase.
-                                   Grid launch, 64 consecutive rays per wave: the
-                                   records are deterministic; nearest t and hit /
-                                   miss equal the other wide walks', the triangle
-                                   up to exact distance ties; visit and test
-                                   counts depend on the wave's schedule (variant 0..5) */
+    CRT_KERNEL_SPECULATIVE = 4, /* wide and stackless formats: one ray per lane,
+                                   lanes never wait for leaf tests: hit leaves go
+                                   to a per-lane queue, and after a round of node
+                                   steps the wave tests the queued leaves 64 at a
+                                   time, with culling by the tmax of the last such
+                                   phase.  Grid launch, 64 consecutive rays per
+                                   wave: the records are deterministic; nearest t
+                                   and hit / miss equal the other walks' of the
+                                   format, the triangle up to exact distance ties;
+                                   visit and test counts depend on the wave's
+                                   schedule (variant 0..5 over the wide formats,
+                                   0..3 over the stackless format) */
     CRT_KERNEL_SPECULATIVE_REFILL = 5 /* the same walk in device-filling waves that
                                    refill idle lanes from a work queue (uses
                                    config->queue like CRT_KERNEL_PERSISTENT); counts
-                                   vary from run to run (variant 0..5) */
+                                   vary from run to run (variants as kernel 4) */
 };
 
 typedef struct crt_trace_config {

@@ -823,6 +823,7 @@ typedef struct {
     crt_hit* hits;
     uint32_t* tri_tests;
     int64_t n, wave_begin, wave_end;
+    int stackless; /* the stackless BVH2 walk (spec_stackless.h) instead of the wide one */
 } spec_job_t;
 
 static int spec_begin_walk(const spec_job_t* J, spec_lane_t* L) {
@@ -844,7 +845,8 @@ static int spec_begin_walk(const spec_job_t* J, spec_lane_t* L) {
     L->sp = 0;
     L->it = 0;
     L->nq = 0;
-    L->walk_done = 0;
+    /* the stackless loop `while (Pointer >= 0 ...)` never runs from a negative root */
+    L->walk_done = J->stackless ? L->cur < 0 : 0;
     return 1;
 }
 
@@ -862,10 +864,16 @@ static void spec_finish(const spec_job_t* J, spec_lane_t* L) {
             const crt_ray* r = &J->rays[L->slot];
             v3 lro = xform(e->inverse, mk(r->org[0], r->org[1], r->org[2]), 1.0f);
             v3 lrd = xform(e->inverse, mk(r->dir[0], r->dir[1], r->dir[2]), 0.0f);
-            const crt_tri_record* q = &s->recs[L->tri];
             v3 p = mk(lro.x + lrd.x * closest, lro.y + lrd.y * closest, lro.z + lrd.z * closest);
-            v3 uvw = barycentrics(p, mk(q->v0[0], q->v0[1], q->v0[2]), mk(q->v1[0], q->v1[1], q->v1[2]),
-                                  mk(q->v2[0], q->v2[1], q->v2[2]));
+            v3 uvw;
+            if (J->stackless) {
+                const crt_triangle* tr = &s->tris[L->tri];
+                uvw = barycentrics(p, vpos(s, tr->packed[0]), vpos(s, tr->packed[1]), vpos(s, tr->packed[2]));
+            } else {
+                const crt_tri_record* q = &s->recs[L->tri];
+                uvw = barycentrics(p, mk(q->v0[0], q->v0[1], q->v0[2]), mk(q->v1[0], q->v1[1], q->v1[2]),
+                                   mk(q->v2[0], q->v2[1], q->v2[2]));
+            }
             h->t = closest;
             h->u = uvw.x;
             h->v = uvw.y;
@@ -880,8 +888,38 @@ static void spec_finish(const spec_job_t* J, spec_lane_t* L) {
 }
 
 static int spec_can_step(const spec_job_t* J, const spec_lane_t* L) {
+    if (J->stackless) return L->active && !L->walk_done && L->nq < J->qcap;   /* one leaf per node */
     const int width = J->s->format == CRT_FORMAT_WIDE8 ? 8 : 4;
     return L->active && !L->walk_done && L->nq <= J->qcap - width;
+}
+
+/* One node of the stackless walk (walk_stackless's step), with a hit leaf
+ * queued instead of tested: the lane follows the leaf's miss link at once.
+ * Returns 0 when the walk is over. */
+static int sls_step(const spec_job_t* J, spec_lane_t* L) {
+    const scene_t* s = J->s;
+    if (L->it >= s->max_iter) {
+        L->status |= CRT_STATUS_ITER_CAP;
+        return 0;
+    }
+    if (L->cur < L->root || L->cur > L->root + L->count || (int64_t)L->cur >= s->total_nodes) {
+        L->status |= CRT_STATUS_BAD_INDEX;
+        return 0;
+    }
+    L->it++;
+    const crt_stackless_node* nd = &((const crt_stackless_node*)s->nodes)[L->cur];
+    float tb = bounds_stackless(nd->min, nd->max, L->ro, L->inv, 0.0001f, L->tmax);
+    const int32_t link = as_int(nd->max[3]);
+    const int32_t packed = as_int(nd->min[3]);
+    const int hit = tb > 0.0f && tb < L->tmax;
+    if (hit && packed == -1) {
+        L->cur++;
+        return 1;
+    }
+    if (hit) L->queue[L->nq++] = packed;
+    if (link < 0) return 0;
+    L->cur = link + L->root;
+    return 1;
 }
 
 static int spec_step(const spec_job_t* J, spec_lane_t* L) {
@@ -987,9 +1025,15 @@ static void spec_leaf_phase(const spec_job_t* J, spec_lane_t* lanes, uint16_t* l
         const int32_t leaf = O->queue[list[idx] & 15];
         const int32_t first = leaf >> 4, len = leaf & 0xF;
         for (int32_t x = 0; x < len; ++x) {
-            const crt_tri_record* r = &s->recs[first + x];
-            float t = tri_t(O->ro, O->rd, mk(r->v0[0], r->v0[1], r->v0[2]), mk(r->v1[0], r->v1[1], r->v1[2]),
-                            mk(r->v2[0], r->v2[1], r->v2[2]));
+            float t;
+            if (J->stackless) {
+                const crt_triangle* tr = &s->tris[first + x];
+                t = tri_t(O->ro, O->rd, vpos(s, tr->packed[0]), vpos(s, tr->packed[1]), vpos(s, tr->packed[2]));
+            } else {
+                const crt_tri_record* r = &s->recs[first + x];
+                t = tri_t(O->ro, O->rd, mk(r->v0[0], r->v0[1], r->v0[2]), mk(r->v1[0], r->v1[1], r->v1[2]),
+                          mk(r->v2[0], r->v2[1], r->v2[2]));
+            }
             if (t > 0.0f && t < O->tmax) {
                 uint32_t tb;
                 memcpy(&tb, &t, 4);
@@ -1005,10 +1049,15 @@ static void spec_leaf_phase(const spec_job_t* J, spec_lane_t* lanes, uint16_t* l
             const uint32_t order = J->any ? (uint32_t)(keys[l] >> 32) : (uint32_t)keys[l];
             const uint32_t tb = J->any ? (uint32_t)keys[l] : (uint32_t)(keys[l] >> 32);
             const int32_t leaf = L->queue[list[order >> 4] & 15];
-            const crt_tri_record* r = &s->recs[(leaf >> 4) + (int32_t)(order & 15u)];
+            const int32_t idx = (leaf >> 4) + (int32_t)(order & 15u);
             memcpy(&L->tmax, &tb, 4);
-            L->tri = r->tri;
-            L->mesh = r->mesh;
+            if (J->stackless) {
+                L->tri = idx;
+                L->mesh = s->tris[idx].packed[3];
+            } else {
+                L->tri = s->recs[idx].tri;
+                L->mesh = s->recs[idx].mesh;
+            }
             L->hit_ent = L->ent;
             if (J->any) L->walk_done = 1;
         }
@@ -1051,7 +1100,7 @@ static void* spec_run_job(void* arg) {
                 for (int l = 0; l < 64; ++l) {
                     spec_lane_t* L = &lanes[l];
                     if (spec_can_step(J, L))
-                        if (!spec_step(J, L)) L->walk_done = 1;
+                        if (!(J->stackless ? sls_step(J, L) : spec_step(J, L))) L->walk_done = 1;
                 }
                 int stepping = 0;
                 for (int l = 0; l < 64; ++l) stepping |= spec_can_step(J, &lanes[l]);
@@ -1127,6 +1176,53 @@ int oracle_trace_wide_spec(int32_t width, int32_t entity_count, int64_t total_no
         J->n = n;
         J->wave_begin = (int64_t)t * per < waves ? (int64_t)t * per : waves;
         J->wave_end = J->wave_begin + per < waves ? J->wave_begin + per : waves;
+        J->stackless = 0;
+    }
+    for (int t = 1; t < threads; ++t) pthread_create(&th[t], NULL, spec_run_job, &jobs[t]);
+    spec_run_job(&jobs[0]);
+    for (int t = 1; t < threads; ++t) pthread_join(th[t], NULL);
+    return 0;
+}
+
+/* The host twin of the speculative stackless walk's grid launch
+ * (spec_stackless.h): 64 consecutive rays per wave, `inner` node steps per
+ * round, `queue_slots` queued leaves per lane (1..16). */
+int oracle_trace_stackless_spec(int32_t entity_count, int64_t total_nodes, const crt_stackless_node* nodes,
+                                const crt_triangle* tris, const crt_vertex* verts, const crt_entity* ents, int query,
+                                uint32_t flags, const crt_ray* rays, crt_hit* hits, int64_t n, int threads,
+                                uint32_t* tri_tests, int32_t max_iter, int32_t inner, int32_t queue_slots) {
+    scene_t s;
+    memset(&s, 0, sizeof(s));
+    if (inner < 1 || queue_slots < 1 || queue_slots > 16) return -1;
+    s.format = CRT_FORMAT_STACKLESS;
+    s.entity_count = entity_count;
+    s.total_nodes = total_nodes;
+    s.nodes = nodes;
+    s.tris = tris;
+    s.verts = verts;
+    s.ents = ents;
+    s.max_iter = max_iter > 0 ? max_iter : MAX_ITER_DEFAULT;
+    int64_t waves = (n + 63) / 64;
+    if (threads < 1) threads = 1;
+    if (threads > 256) threads = 256;
+    if ((int64_t)threads > waves) threads = (int)(waves > 0 ? waves : 1);
+    pthread_t th[256];
+    spec_job_t jobs[256];
+    int64_t per = (waves + threads - 1) / threads;
+    for (int t = 0; t < threads; ++t) {
+        spec_job_t* J = &jobs[t];
+        J->s = &s;
+        J->any = query == CRT_QUERY_ANY;
+        J->flags = flags;
+        J->inner = inner;
+        J->qcap = queue_slots;
+        J->rays = rays;
+        J->hits = hits;
+        J->tri_tests = tri_tests;
+        J->n = n;
+        J->wave_begin = (int64_t)t * per < waves ? (int64_t)t * per : waves;
+        J->wave_end = J->wave_begin + per < waves ? J->wave_begin + per : waves;
+        J->stackless = 1;
     }
     for (int t = 1; t < threads; ++t) pthread_create(&th[t], NULL, spec_run_job, &jobs[t]);
     spec_run_job(&jobs[0]);

@@ -1,0 +1,155 @@
+"""The host twin of the speculative stackless walk (oracle_trace_stackless_spec), without a GPU.
+
+The walk is the reference's stackless walk (TraverseBVHStackless.glsl:175-278)
+with hit leaves queued instead of tested: culling uses the TMax of the last leaf
+phase, which is never below the reference walk's at the same node.  A box the
+reference walk enters is therefore entered too, and the pre-order sequence of
+nodes only gains whole subtrees.  So, on rays where the reference walk's culling
+is exact: the nearest t equals the reference walk's, the triangle differs only at
+an exact tie of t, node visits are never fewer (closest hits), any-hit occlusion
+is the same, and guard status bits agree on the crafted buffers.  Checked over every preset and small schedules, on
+clouds with random and axis-parallel rays, on instanced entities with the scene
+flags, and against brute force.  Where boxes touch, the reference walk's culling
+can skip a nearer triangle; there the speculative walk may find it (see
+test_touching_boxes_against_brute_force).
+"""
+import numpy as np
+import pytest
+
+from oracle_cases import axis_aligned_rays, crafted_stackless, random_rays
+
+SCHEDULES = [None, (1, 1), (4, 2), (64, 16)]
+
+
+def _cloud(crt, orc):
+    from candela_amd import scenes
+    b = crt.build_bvh(0, scenes.icosphere_cloud(count=120, subdivisions=2, seed=13, box=30.0))
+    return b, orc.identity_entity(b.nodes.size)
+
+
+@pytest.mark.parametrize("schedule", SCHEDULES)
+def test_nearest_t_visits_and_occlusion(crt, orc, schedule):
+    b, ent = _cloud(crt, orc)
+    rays = np.concatenate([random_rays(8000, [-3, -3, -3], [33, 33, 33], seed=91),
+                           axis_aligned_rays(1000, [0, 0, 0], [30, 30, 30], seed=92)])
+    ref = orc.trace(0, b.nodes, b.triangles, b.vertices, ent, 0, rays)
+    ref_any = orc.trace(0, b.nodes, b.triangles, b.vertices, ent, 1, rays)
+    variants = sorted(orc.SBVH2_PRESETS) if schedule is None else [0]
+    for v in variants:
+        h = orc.trace_stackless_spec(b.nodes, b.triangles, b.vertices, ent, 0, rays, variant=v, schedule=schedule)
+        assert np.array_equal(h["t"], ref["t"]), f"nearest t differs (variant {v}, schedule {schedule})"
+        same = h["tri"] == ref["tri"]
+        assert (same | (h["t"] == ref["t"])).all()
+        for k in ("u", "v", "w"):
+            assert np.array_equal(h[k][same], ref[k][same])
+        vis, rvis = h["status"] & 0xFFFFFF, ref["status"] & 0xFFFFFF
+        assert (vis >= rvis).all(), "the speculative walk skipped a node the reference walk visits"
+        assert (h["status"] >> 24 == 0).all()
+        a = orc.trace_stackless_spec(b.nodes, b.triangles, b.vertices, ent, 1, rays, variant=v, schedule=schedule)
+        assert np.array_equal(a["t"] > 0, ref_any["t"] > 0), "occlusion differs"
+        hit = a["t"] > 0
+        assert (a["t"][hit] < rays["tmax"][hit]).all()
+
+
+def _near(t):
+    return np.where(t > 0, t, np.inf)
+
+
+def test_touching_boxes_against_brute_force(crt, orc):
+    """On the boxes scene (boxes touching each other and the ground) the reference walk misses the nearest hit on
+    about 1% of rays: its culling has no margin, and a triangle on a box face can lie a rounding error before the
+    box's computed entry distance.  The speculative walk culls with a TMax that is never smaller, so it recovers
+    some of those hits: wherever it differs from the reference walk, its hit is nearer and is the brute-force
+    nearest hit, and it is never nearer than brute force."""
+    from candela_amd import scenes
+    b = crt.build_bvh(0, scenes.boxes_scene())
+    ent = orc.identity_entity(b.nodes.size)
+    rays = random_rays(20000, [-4, 0.2, -4], [20, 8, 20], seed=3)
+    bt, _ = orc.bruteforce_closest(b.triangles, b.vertices, ent, rays)
+    ref = orc.trace(0, b.nodes, b.triangles, b.vertices, ent, 0, rays)
+    assert (ref["t"] != bt).sum() > 50                  # the reference walk's misses the check relies on
+    for v in orc.SBVH2_PRESETS:
+        h = orc.trace_stackless_spec(b.nodes, b.triangles, b.vertices, ent, 0, rays, variant=v)
+        d = h["t"] != ref["t"]
+        assert (_near(h["t"]) <= _near(ref["t"])).all() and (_near(h["t"]) >= _near(bt)).all()
+        assert np.array_equal(h["t"][d], bt[d]), f"variant {v}: a differing hit is not the nearest"
+        assert (h["status"] & 0xFFFFFF >= ref["status"] & 0xFFFFFF).all()
+
+
+def _instanced(crt):
+    """Two objects (boxes, an icosphere cloud) concatenated as AddObject does, six entities with rotations and
+    translations, one of them translucent."""
+    from candela_amd import scenes
+    b0 = crt.build_bvh(0, scenes.boxes_scene())
+    b3 = crt.build_bvh(0, scenes.icosphere_cloud(count=20, subdivisions=2, seed=3, box=16.0),
+                       t_offset=b0.triangles.size)
+    nodes = np.concatenate([b0.nodes, b3.nodes])
+    t3 = b3.triangles.copy()
+    t3["packed"][:, :3] += b0.vertices.size
+    tris = np.concatenate([b0.triangles, t3])
+    verts = np.concatenate([b0.vertices, b3.vertices])
+    ents = np.zeros(6, crt.ENTITY_DTYPE)
+    rng = np.random.default_rng(5)
+    for k in range(6):
+        ang = rng.uniform(0, 2 * np.pi)
+        m = np.eye(4)
+        m[:3, :3] = [[np.cos(ang), 0, np.sin(ang)], [0, 1, 0], [-np.sin(ang), 0, np.cos(ang)]]
+        m[:3, 3] = rng.uniform(-10, 10, 3)
+        off, cnt = (0, b0.nodes.size) if k % 2 == 0 else (b0.nodes.size, b3.nodes.size)
+        mm = np.ascontiguousarray(m.astype(np.float32).T.reshape(16))
+        crt.check(crt.lib().crt_make_entity(mm.ctypes.data, 0.0, 0.5 if k == 4 else 0.0, off, cnt,
+                                            ents[k:k + 1].ctypes.data))
+    return nodes, tris, verts, ents
+
+
+def test_instances_and_flags(crt, orc):
+    nodes, tris, verts, ents = _instanced(crt)
+    rays = random_rays(6000, [-25, -5, -25], [25, 20, 25], seed=103, toward=[0, 2, 0])
+    for flags in (0, crt.FLAG_IGNORE_TRANSPARENT, crt.FLAG_TRI0_IS_MISS):
+        ref = orc.trace(0, nodes, tris, verts, ents, 0, rays, flags=flags)
+        refa = orc.trace(0, nodes, tris, verts, ents, 1, rays, flags=flags)
+        for v in (0, 3):
+            h = orc.trace_stackless_spec(nodes, tris, verts, ents, 0, rays, variant=v, flags=flags)
+            # the boxes object has touching boxes: a hit may only be nearer than the reference walk's
+            same = h["t"] == ref["t"]
+            assert (_near(h["t"]) <= _near(ref["t"])).all() and same.mean() > 0.98
+            assert np.array_equal(h["entity"][same], ref["entity"][same])
+            a = orc.trace_stackless_spec(nodes, tris, verts, ents, 1, rays, variant=v, flags=flags)
+            assert np.array_equal(a["t"] > 0, refa["t"] > 0)
+    assert (ref["t"] > 0).mean() > 0.2
+
+
+@pytest.mark.parametrize("kind", ["bad_link", "cycle"])
+def test_guards(crt, orc, kind):
+    from oracle_cases import single_triangle_scene
+    v, tris = single_triangle_scene()
+    nodes = crafted_stackless(kind)
+    ent = orc.identity_entity(nodes.size)
+    rays = random_rays(300, [-0.5, -0.5, -1.0], [1.0, 1.0, -0.5], seed=4, toward=[0.3, 0.3, 0.0])
+    for q in (0, 1):
+        ref = orc.trace(0, nodes, tris, v, ent, q, rays)
+        for var in orc.SBVH2_PRESETS:
+            h = orc.trace_stackless_spec(nodes, tris, v, ent, q, rays, variant=var)
+            hf, rf = h["status"] >> 24, ref["status"] >> 24
+            if q == 0:
+                assert np.array_equal(hf, rf), f"{kind} v={var}"
+            else:
+                # an any-hit walk of the reference stops at its first hit; the speculative walk finishes its round
+                # of node steps first and can meet a guard on the way
+                hit = ref["t"] > 0
+                assert np.array_equal(hf[~hit], rf[~hit]) and ((hf[hit] & rf[hit]) == rf[hit]).all()
+            assert np.array_equal(h["t"] > 0, ref["t"] > 0)
+
+
+def test_planted_mistake_fails(crt, orc):
+    """The nearest-t check notices a twin that loses leaves: planted here by truncating every leaf to no
+    triangles in a copy of the buffers given to the twin alone."""
+    b, ent = _cloud(crt, orc)
+    rays = random_rays(2000, [-3, -3, -3], [33, 33, 33], seed=91)
+    ref = orc.trace(0, b.nodes, b.triangles, b.vertices, ent, 0, rays)
+    bad = b.nodes.copy()
+    packed = bad["min"][:, 3].view(np.int32)
+    leaf = packed != -1
+    packed[leaf] = packed[leaf] & ~0xF
+    h = orc.trace_stackless_spec(bad, b.triangles, b.vertices, ent, 0, rays, variant=0)
+    assert not np.array_equal(h["t"], ref["t"])
