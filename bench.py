@@ -84,7 +84,10 @@ def bytes_per_ray(fmt: int, visits: np.ndarray, tests: np.ndarray) -> np.ndarray
 def _kernel_name(layout: str, kernel: int, variant: int) -> str:
     if layout == "stack":
         return {1: "trace_kernel (stack walk)", 3: "stack_wave, preset %d" % variant,
-                2: "stack_persistent (lane refill), preset %d" % variant}.get(kernel, "trace_kernel")
+                2: "stack_persistent (lane refill), preset %d" % variant,
+                4: "sbvh2::grid (speculative, stack layout), preset %d" % variant,
+                5: "sbvh2::persistent (speculative, stack layout, lane refill), preset %d" % variant}.get(
+                    kernel, "trace_kernel")
     if layout == "bvh2":
         return "%s, preset %d" % (KERNEL_NAMES[kernel], variant)
     return {3: "wide_grid, preset %d" % variant, 2: "wide_persistent (lane refill), preset %d" % variant,
@@ -752,6 +755,8 @@ def main(argv=None) -> int:
         if not args.no_stack:   # the reference's other layout over the same meshes
             candidates += [stack(crt.KERNEL_THREAD_PER_RAY, 0, lds) for lds in (16, 32)]
             candidates += [stack(k, v, 0) for k in (crt.KERNEL_WAVE, crt.KERNEL_PERSISTENT) for v in range(8)]
+            candidates += [stack(k, v, 0) for k in (crt.KERNEL_SPECULATIVE, crt.KERNEL_SPECULATIVE_REFILL)
+                           for v in range(4)]
     if args.only:
         main_part, _, ao_part = args.only.partition("+ao:")
         parts = main_part.split(":")
@@ -1023,14 +1028,28 @@ def main(argv=None) -> int:
         g, a = out["gi"]["hits"], out["ao"]["hits"]
         ohg = oracle.trace(fmt, nodes, tris, verts, ents, 0, gi, threads=threads)
         oha = oracle.trace(fmt, nodes, tris, verts, ents, 1, ao, threads=threads)
-        if best["layout"] == "bvh2":
-            twin_g, twin_a = ohg, oha
-        elif best["layout"] == "stack":
-            sh_ = stack_ctx["hb"]
-            twin_g = oracle.trace(crt.FORMAT_STACK, sh_["nodes"], sh_["triangles"], sh_["vertices"], sh_["entities"],
-                                  0, gi, threads=threads)
-            twin_a = oracle.trace(crt.FORMAT_STACK, sh_["nodes"], sh_["triangles"], sh_["vertices"], sh_["entities"],
-                                  1, ao, threads=threads)
+        def bvh2_twin(stack_layout, hb_, q, rays_, kernel, variant, walk):
+            # the speculative grid kernel's twin replays its wave schedule; the other kernels' records are the
+            # reference walk's (`walk`, computed on demand)
+            if kernel == crt.KERNEL_SPECULATIVE:
+                fn = oracle.trace_stack_spec if stack_layout else oracle.trace_stackless_spec
+                return fn(hb_["nodes"], hb_["triangles"], hb_["vertices"], hb_["entities"], q, rays_,
+                          variant=variant, threads=threads)
+            return walk()
+
+        if best["layout"] in ("bvh2", "stack"):
+            on_stack = best["layout"] == "stack" or fmt == crt.FORMAT_STACK
+            hb_ = stack_ctx["hb"] if best["layout"] == "stack" else dict(nodes=nodes, triangles=tris,
+                                                                          vertices=verts, entities=ents)
+
+            def walk(q, rays_, base):
+                if best["layout"] == "bvh2":
+                    return base
+                return oracle.trace(crt.FORMAT_STACK, hb_["nodes"], hb_["triangles"], hb_["vertices"],
+                                    hb_["entities"], q, rays_, threads=threads)
+
+            twin_g = bvh2_twin(on_stack, hb_, 0, gi, best["kernel"], best["variant"], lambda: walk(0, gi, ohg))
+            twin_a = bvh2_twin(on_stack, hb_, 1, ao, *ao_of(best), lambda: walk(1, ao, oha))
         else:   # the host twin of the wide walk, on the same wide layout
             w = wides[wide_key(best)].host
 
@@ -1046,8 +1065,8 @@ def main(argv=None) -> int:
             twin_g = wide_twin(0, gi, best["kernel"], best["variant"], 0)
             twin_a = wide_twin(1, ao, *ao_of(best), ao_flags(crt, best))
         schedule_free = {crt.KERNEL_SPECULATIVE_REFILL}
-        gi_exact = best["kernel"] not in schedule_free or best["layout"] in ("bvh2", "stack")
-        ao_exact = ao_of(best)[0] not in schedule_free or best["layout"] in ("bvh2", "stack")
+        gi_exact = best["kernel"] not in schedule_free
+        ao_exact = ao_of(best)[0] not in schedule_free
         gi_miss = g["tri"] != ohg["tri"]
         parity = {"rays": int(g.size + a.size),
                   # bit-identical records against the twin of each launch's kernel (kernels with a fixed schedule)

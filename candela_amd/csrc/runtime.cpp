@@ -173,12 +173,11 @@ int trace(const crt_scene_view* scene, int query, uint32_t flags, const crt_ray*
     if (kernel < CRT_KERNEL_AUTO || kernel > CRT_KERNEL_SPECULATIVE_REFILL)
         return set_error(CRT_E_INVALID, "crt_trace: unknown kernel");
     const bool speculative = kernel == CRT_KERNEL_SPECULATIVE || kernel == CRT_KERNEL_SPECULATIVE_REFILL;
-    if (speculative && !is_wide(scene->format) && scene->format != CRT_FORMAT_STACKLESS)
-        return set_error(CRT_E_INVALID, "crt_trace: the speculative kernels walk the wide and stackless formats");
+    const bool bvh2 = scene->format == CRT_FORMAT_STACKLESS || scene->format == CRT_FORMAT_STACK;
     if (speculative && cfg && is_wide(scene->format) && cfg->variant > 5)
         return set_error(CRT_E_INVALID, "crt_trace: the speculative wide kernels take variant 0..5");
-    if (speculative && cfg && scene->format == CRT_FORMAT_STACKLESS && cfg->variant > 3)
-        return set_error(CRT_E_INVALID, "crt_trace: the speculative stackless kernels take variant 0..3");
+    if (speculative && cfg && bvh2 && cfg->variant > 3)
+        return set_error(CRT_E_INVALID, "crt_trace: the speculative stack and stackless kernels take variant 0..3");
     if ((kernel == CRT_KERNEL_PERSISTENT || kernel == CRT_KERNEL_WAVE) && scene->format == CRT_FORMAT_STACK && cfg &&
         cfg->variant > 7)
         return set_error(CRT_E_INVALID, "crt_trace: the stack-layout wave kernels take variant 0..7");
@@ -200,9 +199,9 @@ int trace(const crt_scene_view* scene, int query, uint32_t flags, const crt_ray*
             return set_error(CRT_E_INVALID, "crt_trace: too many rays for one launch");
         const int v = cfg ? cfg->variant : 0;
         hipStream_t st = static_cast<hipStream_t>(stream);
-        CRT_HIP(scene->format == CRT_FORMAT_STACKLESS
-                    ? launch_spec_stackless_grid(a, query, rays, hits, n, flags, kMaxIterations, tri_tests, v, st)
-                    : launch_spec_grid(a, query, rays, hits, n, flags, kMaxIterations, tri_tests, v, st),
+        CRT_HIP(bvh2 ? launch_spec_bvh2_grid(a, scene->format == CRT_FORMAT_STACK, query, rays, hits, n, flags,
+                                             kMaxIterations, tri_tests, v, st)
+                     : launch_spec_grid(a, query, rays, hits, n, flags, kMaxIterations, tri_tests, v, st),
                 "crt_trace: speculative kernel launch");
         return CRT_OK;
     }
@@ -215,10 +214,9 @@ int trace(const crt_scene_view* scene, int query, uint32_t flags, const crt_ray*
         const int v = cfg ? cfg->variant : 0, g = cfg ? cfg->grid : 0;
         hipStream_t st = static_cast<hipStream_t>(stream);
         const hipError_t e =
-            scene->format == CRT_FORMAT_STACKLESS
-                ? launch_spec_stackless_persistent(a, query, rays, hits, n, flags, kMaxIterations, q.ptr, tri_tests,
-                                                   v, g, st)
-                : launch_spec_persistent(a, query, rays, hits, n, flags, kMaxIterations, q.ptr, tri_tests, v, g, st);
+            bvh2 ? launch_spec_bvh2_persistent(a, scene->format == CRT_FORMAT_STACK, query, rays, hits, n, flags,
+                                               kMaxIterations, q.ptr, tri_tests, v, g, st)
+                 : launch_spec_persistent(a, query, rays, hits, n, flags, kMaxIterations, q.ptr, tri_tests, v, g, st);
         rc = queue_release(q, stream);
         CRT_HIP(e, "crt_trace: speculative persistent kernel launch");
         return rc;

@@ -1,19 +1,29 @@
-// Speculative wave walk of the reference's stackless BVH2 layout
-// (IntersectBVHStackless / IntersectBVHStacklessOcclusion,
-// Source/Core/Shaders/Intersectors/Include/TraverseBVHStackless.glsl:175-278,
-// :463-556, in the IntersectScene entity loop :280-366, :558-579).
+// Speculative wave walks of the reference's two BVH2 layouts, one template:
 //
-// The pre-order walk follows a node's miss link after testing a leaf, whatever
-// the test found: the leaf's result only shortens TMax for the boxes that
-// follow.  The wave walk of wave_walk.h parks a lane at each hit leaf until
-// enough lanes wait, then each lane tests its own leaf.  Here, as in the
-// speculative wide walk (spec_walk.h):
+//   D == 0  the stackless layout (IntersectBVHStackless /
+//           IntersectBVHStacklessOcclusion,
+//           Source/Core/Shaders/Intersectors/Include/TraverseBVHStackless.glsl:175-278,
+//           :463-556, in the IntersectScene entity loop :280-366, :558-579);
+//   D > 0   the stack layout (IntersectBVHStack / IntersectBVHStackOcclusion,
+//           Include/TraverseBVHStack.glsl:168-324, :509-657) with the first D
+//           stack entries in LDS and the rest in per-lane scratch.
+//
+// Both reference walks decide the next node from box distances alone: the
+// stackless walk follows a node's miss link after testing a leaf, whatever
+// the test found, and the stack walk tests a node's leaf children and then
+// goes on by the inner children's box distances.  A leaf's result only
+// shortens TMax for the boxes that follow.  The wave walks of wave_walk.h and
+// stack_walk.h park a lane at its leaves until enough lanes wait.  Here, as in
+// the speculative wide walk (spec_walk.h):
 //
 //   node step  the reference's step (iteration cap, index guard, RayBounds
-//              with the current TMax, inner hit -> next node, otherwise the
-//              miss link), except that a hit leaf goes to the lane's queue
-//              (Q slots in its LDS column) and the lane goes straight on
-//              along the miss link; a lane steps while its queue has room
+//              with the current TMax, then the next node), except that leaves
+//              go to the lane's queue (Q slots in its LDS column) and the lane
+//              goes straight on: a hit leaf of the stackless walk, every
+//              non-empty leaf child (left, then right) of the stack walk.  A
+//              lane steps while its queue has room for one more node's leaves.
+//              A stack walk whose push finds 63 entries ends with
+//              CRT_STATUS_STACK_OVERFLOW after its node's leaves are queued
 //   round      up to INNER node steps (fewer once no lane can step), the
 //              leaf phase whenever some queue holds a leaf, then the walk ends
 //              once kWalkEndBatch lanes wait or no lane can step
@@ -24,12 +34,15 @@
 //              owner over (t, order) for closest hits and (order, t) for any
 //              hits, order = entry * 16 + triangle of the leaf
 //   culling    boxes are tested with the TMax of the last leaf phase, which
-//              only errs toward visiting more nodes: a closest-hit walk still
-//              ends with the reference walk's nearest t
+//              only errs toward visiting more nodes; box distances, which
+//              order the stack walk, do not depend on TMax, so the reference
+//              walk's node sequence only gains whole subtrees and a
+//              closest-hit walk still ends with the reference walk's nearest t
 //
 // With the grid launch the schedule is fixed and the host twin
-// (oracle_trace.c: oracle_trace_stackless_spec) replays it, counters included;
-// with the refilling launch the nearest t, hit / miss and occlusion are fixed.
+// (oracle_trace.c: oracle_trace_stackless_spec, oracle_trace_stack_spec)
+// replays it, counters included; with the refilling launch the nearest t,
+// hit / miss and occlusion are fixed.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -40,6 +53,7 @@
 #include "device_math.h"
 #include "queue.h"
 #include "spec_walk.h"
+#include "stack_walk.h"
 #include "trace_launch.h"
 
 namespace crt {
@@ -50,19 +64,23 @@ using spec::lanes_below;
 using spec::wave_lds_sync;
 using spec::WaveLds;
 
-template <bool ANY, int INNER, int Q>
-struct SpecStacklessWave {
-    static_assert(Q >= 1 && Q <= 16, "queue slots fit 4 bits");
+template <bool ANY, int D, int INNER, int Q>
+struct SpecBvh2Wave {
+    static_assert(Q >= 2 && Q <= 16, "queue slots fit 4 bits");
+    static_assert(D >= 0 && D <= sdev::kStackCap, "LDS stack entries");
+    static constexpr bool kStack = D > 0;
+    static constexpr int32_t kPerNode = kStack ? 2 : 1;   // leaves one node step can queue
 
     // per lane
     f3 ro, rd, inv;
     float tmax, ray_tmax;
-    int32_t start, count, ptr, it;
+    int32_t start, count, ptr, sp, it;
     int32_t tri, mesh, slot, ent, hit_ent;
     uint32_t tests, status, visits;
     int32_t nq;          // queued leaves (this lane's LDS column, stride 256)
     bool active, walk_done;
     LdsInt* q;
+    sdev::NodeStack<kStack ? D : 1> st;   // the stack walk's node stack
 
     __device__ __forceinline__ bool begin_walk(const SceneArgs& s, const crt_ray* __restrict__ rays, uint32_t flags) {
         int32_t e = ent + 1;
@@ -81,9 +99,12 @@ struct SpecStacklessWave {
         start = en.node_offset;
         count = en.node_count;
         ptr = start;
+        sp = 0;
         it = 0;
         nq = 0;
-        walk_done = ptr < 0;   // `while (Pointer >= 0 ...)` never runs
+        // the stackless `while (Pointer >= 0 ...)` never runs from a negative
+        // root; the stack walk's index guard stops it instead
+        walk_done = !kStack && ptr < 0;
         return true;
     }
 
@@ -133,10 +154,63 @@ struct SpecStacklessWave {
         }
     }
 
-    __device__ __forceinline__ bool can_step() const { return active && !walk_done && nq < Q; }
+    __device__ __forceinline__ bool can_step() const { return active && !walk_done && nq <= Q - kPerNode; }
 
-    // One node of the walk.  False when the walk is over.
+    // One entry of the stack layout (TraverseBVHStack.glsl:168-324): both
+    // children in 64 bytes, inner children box-tested, leaf children queued
+    // without a box test.  False when the walk is over.
+    __device__ __forceinline__ bool step_stack(const SceneArgs& s, int32_t max_iter) {
+        if (it >= max_iter) {
+            status |= CRT_STATUS_ITER_CAP;
+            return false;
+        }
+        if (ptr < start || ptr > start + count || ptr < 0 || int64_t(ptr) >= s.total_nodes) {
+            status |= CRT_STATUS_BAD_INDEX;
+            return false;
+        }
+        it++;
+        const float4* np = reinterpret_cast<const float4*>(s.nodes) + 4 * int64_t(ptr);
+        const float4 lmin = np[0], lmax = np[1], rmin = np[2], rmax = np[3];
+        const int32_t lp = __float_as_int(lmin.w), rp = __float_as_int(rmin.w);
+        const float lt = lp != -1 ? -1.0f : bounds_stack(lmin, lmax, ro, inv, tmax);
+        const float rt = rp != -1 ? -1.0f : bounds_stack(rmin, rmax, ro, inv, tmax);
+        // slots nq and nq + 1 exist (can_step); each is kept only for a leaf
+        // with triangles
+        q[nq * 256] = lp;
+        nq += sdev::leaf_nonempty(lp) ? 1 : 0;
+        q[nq * 256] = rp;
+        nq += sdev::leaf_nonempty(rp) ? 1 : 0;
+        if (lt > 0.0f && rt > 0.0f) {
+            int32_t next = __float_as_int(lmax.w) + start;
+            int32_t post = __float_as_int(rmax.w) + start;
+            if (rt < lt) {
+                const int32_t tmp = next;
+                next = post;
+                post = tmp;
+            }
+            if (sp >= sdev::kStackCap) {
+                status |= CRT_STATUS_STACK_OVERFLOW;
+                return false;
+            }
+            st.put(sp, post);
+            sp++;
+            ptr = next;
+        } else if (lt > 0.0f) {
+            ptr = __float_as_int(lmax.w) + start;
+        } else if (rt > 0.0f) {
+            ptr = __float_as_int(rmax.w) + start;
+        } else if (sp <= 0) {
+            return false;
+        } else {
+            --sp;
+            ptr = st.get(sp);
+        }
+        return true;
+    }
+
+    // One node of the stackless walk.  False when the walk is over.
     __device__ __forceinline__ bool step(const SceneArgs& s, int32_t max_iter) {
+        if constexpr (kStack) return step_stack(s, max_iter);
         if (it >= max_iter) {
             status |= CRT_STATUS_ITER_CAP;
             return false;

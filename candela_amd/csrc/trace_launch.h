@@ -94,14 +94,14 @@ hipError_t launch_spec_persistent(const SceneArgs& s, int query, const crt_ray* 
                                   uint32_t flags, int32_t max_iter, unsigned long long* counters,
                                   uint32_t* tri_tests, int variant, int grid, hipStream_t stream);
 
-// Speculative walk of the stackless BVH2 layout (spec_bvh2.hip; variant 0..3).
-hipError_t launch_spec_stackless_grid(const SceneArgs& s, int query, const crt_ray* rays, crt_hit* hits, int64_t n,
-                                      uint32_t flags, int32_t max_iter, uint32_t* tri_tests, int variant,
-                                      hipStream_t stream);
-hipError_t launch_spec_stackless_persistent(const SceneArgs& s, int query, const crt_ray* rays, crt_hit* hits,
-                                            int64_t n, uint32_t flags, int32_t max_iter,
-                                            unsigned long long* counters, uint32_t* tri_tests, int variant, int grid,
-                                            hipStream_t stream);
+// Speculative walks of the BVH2 layouts (spec_bvh2.hip; variant 0..3):
+// stack = true for CRT_FORMAT_STACK, false for CRT_FORMAT_STACKLESS.
+hipError_t launch_spec_bvh2_grid(const SceneArgs& s, bool stack, int query, const crt_ray* rays, crt_hit* hits,
+                                 int64_t n, uint32_t flags, int32_t max_iter, uint32_t* tri_tests, int variant,
+                                 hipStream_t stream);
+hipError_t launch_spec_bvh2_persistent(const SceneArgs& s, bool stack, int query, const crt_ray* rays, crt_hit* hits,
+                                       int64_t n, uint32_t flags, int32_t max_iter, unsigned long long* counters,
+                                       uint32_t* tri_tests, int variant, int grid, hipStream_t stream);
 
 // Workgroups of 256 threads that fill every CU with the persistent kernel.
 int persistent_fill_grid(int query);

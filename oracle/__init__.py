@@ -98,6 +98,9 @@ def lib() -> C.CDLL:
         l.oracle_trace_stackless_spec.restype = C.c_int
         l.oracle_trace_stackless_spec.argtypes = [i32, i64, vp, vp, vp, vp, C.c_int, u32, vp, vp, i64, C.c_int, vp,
                                                   i32, i32, i32]
+        l.oracle_trace_stack_spec.restype = C.c_int
+        l.oracle_trace_stack_spec.argtypes = [i32, i64, vp, vp, vp, vp, C.c_int, u32, vp, vp, i64, C.c_int, vp, i32,
+                                              i32, i32]
         l.oracle_bruteforce_closest.restype = C.c_int
         l.oracle_bruteforce_closest.argtypes = [vp, vp, vp, i64, i64, vp, vp, vp, i64]
         l.oracle_primary_rays.restype = None
@@ -200,18 +203,15 @@ def trace_wide_spec(width: int, groups: np.ndarray, records: np.ndarray, ents: n
     return (hits, tests) if with_tests else hits
 
 
-# Schedules of libcandela_rt's speculative stackless kernels by variant: (inner node steps per round, queue slots
-# per lane).
+# Schedules of libcandela_rt's speculative kernels over the two BVH2 layouts (stackless and stack) by variant:
+# (inner node steps per round, queue slots per lane).
 SBVH2_PRESETS = {0: (8, 8), 1: (16, 8), 2: (16, 12), 3: (32, 16)}
 
 
-def trace_stackless_spec(nodes: np.ndarray, tris: np.ndarray, verts: np.ndarray, ents: np.ndarray, query: int,
-                         rays: np.ndarray, variant: int = 0, flags: int = 0, threads: int = 0, max_iter: int = 0,
-                         with_tests: bool = False, schedule=None):
-    """Host twin of the speculative stackless kernel's grid launch (64 consecutive rays per wave), replaying the
-    wave schedule of preset `variant` (or `schedule` = (inner, queue slots))."""
+def _trace_bvh2_spec(stack, nodes, tris, verts, ents, query, rays, variant, flags, threads, max_iter, with_tests,
+                     schedule):
     inner, q = schedule if schedule is not None else SBVH2_PRESETS[variant]
-    nodes = np.ascontiguousarray(nodes, STACKLESS_NODE_DTYPE)
+    nodes = np.ascontiguousarray(nodes, STACK_NODE_DTYPE if stack else STACKLESS_NODE_DTYPE)
     tris = np.ascontiguousarray(tris, TRIANGLE_DTYPE)
     verts = np.ascontiguousarray(verts, VERTEX_DTYPE)
     ents = np.ascontiguousarray(ents, ENTITY_DTYPE)
@@ -220,12 +220,30 @@ def trace_stackless_spec(nodes: np.ndarray, tris: np.ndarray, verts: np.ndarray,
     tests = np.zeros(rays.size, np.uint32) if with_tests else None
     if threads <= 0:
         threads = min(16, os.cpu_count() or 1)
-    rc = lib().oracle_trace_stackless_spec(ents.size, nodes.size, _p(nodes), _p(tris), _p(verts), _p(ents), query,
-                                           flags, _p(rays), _p(hits), rays.size, threads,
-                                           _p(tests) if tests is not None else None, max_iter, inner, q)
+    fn = lib().oracle_trace_stack_spec if stack else lib().oracle_trace_stackless_spec
+    rc = fn(ents.size, nodes.size, _p(nodes), _p(tris), _p(verts), _p(ents), query, flags, _p(rays), _p(hits),
+            rays.size, threads, _p(tests) if tests is not None else None, max_iter, inner, q)
     if rc != 0:
-        raise RuntimeError("oracle_trace_stackless_spec failed")
+        raise RuntimeError("oracle_trace_%s_spec failed" % ("stack" if stack else "stackless"))
     return (hits, tests) if with_tests else hits
+
+
+def trace_stackless_spec(nodes: np.ndarray, tris: np.ndarray, verts: np.ndarray, ents: np.ndarray, query: int,
+                         rays: np.ndarray, variant: int = 0, flags: int = 0, threads: int = 0, max_iter: int = 0,
+                         with_tests: bool = False, schedule=None):
+    """Host twin of the speculative stackless kernel's grid launch (64 consecutive rays per wave), replaying the
+    wave schedule of preset `variant` (or `schedule` = (inner, queue slots))."""
+    return _trace_bvh2_spec(False, nodes, tris, verts, ents, query, rays, variant, flags, threads, max_iter,
+                            with_tests, schedule)
+
+
+def trace_stack_spec(nodes: np.ndarray, tris: np.ndarray, verts: np.ndarray, ents: np.ndarray, query: int,
+                     rays: np.ndarray, variant: int = 0, flags: int = 0, threads: int = 0, max_iter: int = 0,
+                     with_tests: bool = False, schedule=None):
+    """Host twin of the speculative stack-layout kernel's grid launch, as trace_stackless_spec (queue slots
+    2..16)."""
+    return _trace_bvh2_spec(True, nodes, tris, verts, ents, query, rays, variant, flags, threads, max_iter,
+                            with_tests, schedule)
 
 
 def trace_paths(params: np.ndarray, width: int, groups, records, ents, albedo, paths, stack_cap: int = 72):

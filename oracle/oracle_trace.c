@@ -823,8 +823,12 @@ typedef struct {
     crt_hit* hits;
     uint32_t* tri_tests;
     int64_t n, wave_begin, wave_end;
-    int stackless; /* the stackless BVH2 walk (spec_stackless.h) instead of the wide one */
+    int stackless; /* the stackless BVH2 walk (spec_bvh2.h, D == 0) instead of the wide one */
+    int stack;     /* the stack-layout BVH2 walk (spec_bvh2.h, D > 0) */
 } spec_job_t;
+
+/* Both BVH2 walks read the reference's Triangle and Vertex records. */
+static inline int spec_bvh2(const spec_job_t* J) { return J->stackless || J->stack; }
 
 static int spec_begin_walk(const spec_job_t* J, spec_lane_t* L) {
     const scene_t* s = J->s;
@@ -866,7 +870,7 @@ static void spec_finish(const spec_job_t* J, spec_lane_t* L) {
             v3 lrd = xform(e->inverse, mk(r->dir[0], r->dir[1], r->dir[2]), 0.0f);
             v3 p = mk(lro.x + lrd.x * closest, lro.y + lrd.y * closest, lro.z + lrd.z * closest);
             v3 uvw;
-            if (J->stackless) {
+            if (spec_bvh2(J)) {
                 const crt_triangle* tr = &s->tris[L->tri];
                 uvw = barycentrics(p, vpos(s, tr->packed[0]), vpos(s, tr->packed[1]), vpos(s, tr->packed[2]));
             } else {
@@ -888,7 +892,8 @@ static void spec_finish(const spec_job_t* J, spec_lane_t* L) {
 }
 
 static int spec_can_step(const spec_job_t* J, const spec_lane_t* L) {
-    if (J->stackless) return L->active && !L->walk_done && L->nq < J->qcap;   /* one leaf per node */
+    if (J->stackless) return L->active && !L->walk_done && L->nq < J->qcap;        /* one leaf per node */
+    if (J->stack) return L->active && !L->walk_done && L->nq <= J->qcap - 2;   /* two leaf children */
     const int width = J->s->format == CRT_FORMAT_WIDE8 ? 8 : 4;
     return L->active && !L->walk_done && L->nq <= J->qcap - width;
 }
@@ -919,6 +924,54 @@ static int sls_step(const spec_job_t* J, spec_lane_t* L) {
     if (hit) L->queue[L->nq++] = packed;
     if (link < 0) return 0;
     L->cur = link + L->root;
+    return 1;
+}
+
+/* One entry of the stack walk (walk_stack's step), with the non-empty leaf
+ * children queued (left, then right) instead of tested; the next entry is
+ * decided from the inner children's box distances as in walk_stack.  A push
+ * at 63 entries ends the walk with STACK_OVERFLOW after the node's leaves are
+ * queued.  Returns 0 when the walk is over. */
+static int stk_step(const spec_job_t* J, spec_lane_t* L) {
+    const scene_t* s = J->s;
+    if (L->it >= s->max_iter) {
+        L->status |= CRT_STATUS_ITER_CAP;
+        return 0;
+    }
+    if (L->cur < L->root || L->cur > L->root + L->count || L->cur < 0 || (int64_t)L->cur >= s->total_nodes) {
+        L->status |= CRT_STATUS_BAD_INDEX;
+        return 0;
+    }
+    L->it++;
+    const crt_stack_node* nd = &((const crt_stack_node*)s->nodes)[L->cur];
+    const int32_t lp = as_int(nd->left.min[3]), rp = as_int(nd->right.min[3]);
+    const float lt = lp != -1 ? -1.0f : bounds_stack(&nd->left, L->ro, L->inv, L->tmax);
+    const float rt = rp != -1 ? -1.0f : bounds_stack(&nd->right, L->ro, L->inv, L->tmax);
+    if (lp != -1 && (lp & 0xF) > 0) L->queue[L->nq++] = lp;
+    if (rp != -1 && (rp & 0xF) > 0) L->queue[L->nq++] = rp;
+    if (lt > 0.0f && rt > 0.0f) {
+        int32_t next = as_int(nd->left.max[3]) + L->root;
+        int32_t post = as_int(nd->right.max[3]) + L->root;
+        if (rt < lt) {
+            const int32_t tmp = next;
+            next = post;
+            post = tmp;
+        }
+        if (L->sp >= 63) {
+            L->status |= CRT_STATUS_STACK_OVERFLOW;
+            return 0;
+        }
+        L->st_node[L->sp++] = post;
+        L->cur = next;
+    } else if (lt > 0.0f) {
+        L->cur = as_int(nd->left.max[3]) + L->root;
+    } else if (rt > 0.0f) {
+        L->cur = as_int(nd->right.max[3]) + L->root;
+    } else if (L->sp <= 0) {
+        return 0;
+    } else {
+        L->cur = L->st_node[--L->sp];
+    }
     return 1;
 }
 
@@ -1026,7 +1079,7 @@ static void spec_leaf_phase(const spec_job_t* J, spec_lane_t* lanes, uint16_t* l
         const int32_t first = leaf >> 4, len = leaf & 0xF;
         for (int32_t x = 0; x < len; ++x) {
             float t;
-            if (J->stackless) {
+            if (spec_bvh2(J)) {
                 const crt_triangle* tr = &s->tris[first + x];
                 t = tri_t(O->ro, O->rd, vpos(s, tr->packed[0]), vpos(s, tr->packed[1]), vpos(s, tr->packed[2]));
             } else {
@@ -1051,7 +1104,7 @@ static void spec_leaf_phase(const spec_job_t* J, spec_lane_t* lanes, uint16_t* l
             const int32_t leaf = L->queue[list[order >> 4] & 15];
             const int32_t idx = (leaf >> 4) + (int32_t)(order & 15u);
             memcpy(&L->tmax, &tb, 4);
-            if (J->stackless) {
+            if (spec_bvh2(J)) {
                 L->tri = idx;
                 L->mesh = s->tris[idx].packed[3];
             } else {
@@ -1100,7 +1153,8 @@ static void* spec_run_job(void* arg) {
                 for (int l = 0; l < 64; ++l) {
                     spec_lane_t* L = &lanes[l];
                     if (spec_can_step(J, L))
-                        if (!(J->stackless ? sls_step(J, L) : spec_step(J, L))) L->walk_done = 1;
+                        if (!(J->stackless ? sls_step(J, L) : J->stack ? stk_step(J, L) : spec_step(J, L)))
+                            L->walk_done = 1;
                 }
                 int stepping = 0;
                 for (int l = 0; l < 64; ++l) stepping |= spec_can_step(J, &lanes[l]);
@@ -1177,6 +1231,7 @@ int oracle_trace_wide_spec(int32_t width, int32_t entity_count, int64_t total_no
         J->wave_begin = (int64_t)t * per < waves ? (int64_t)t * per : waves;
         J->wave_end = J->wave_begin + per < waves ? J->wave_begin + per : waves;
         J->stackless = 0;
+        J->stack = 0;
     }
     for (int t = 1; t < threads; ++t) pthread_create(&th[t], NULL, spec_run_job, &jobs[t]);
     spec_run_job(&jobs[0]);
@@ -1184,17 +1239,16 @@ int oracle_trace_wide_spec(int32_t width, int32_t entity_count, int64_t total_no
     return 0;
 }
 
-/* The host twin of the speculative stackless walk's grid launch
- * (spec_stackless.h): 64 consecutive rays per wave, `inner` node steps per
- * round, `queue_slots` queued leaves per lane (1..16). */
-int oracle_trace_stackless_spec(int32_t entity_count, int64_t total_nodes, const crt_stackless_node* nodes,
-                                const crt_triangle* tris, const crt_vertex* verts, const crt_entity* ents, int query,
-                                uint32_t flags, const crt_ray* rays, crt_hit* hits, int64_t n, int threads,
-                                uint32_t* tri_tests, int32_t max_iter, int32_t inner, int32_t queue_slots) {
+/* Runs the BVH2 twin: stack = 0 for the stackless layout, 1 for the stack
+ * layout. */
+static int spec_bvh2_run(int stack, int32_t entity_count, int64_t total_nodes, const void* nodes,
+                         const crt_triangle* tris, const crt_vertex* verts, const crt_entity* ents, int query,
+                         uint32_t flags, const crt_ray* rays, crt_hit* hits, int64_t n, int threads,
+                         uint32_t* tri_tests, int32_t max_iter, int32_t inner, int32_t queue_slots) {
     scene_t s;
     memset(&s, 0, sizeof(s));
-    if (inner < 1 || queue_slots < 1 || queue_slots > 16) return -1;
-    s.format = CRT_FORMAT_STACKLESS;
+    if (inner < 1 || queue_slots < (stack ? 2 : 1) || queue_slots > 16) return -1;
+    s.format = stack ? CRT_FORMAT_STACK : CRT_FORMAT_STACKLESS;
     s.entity_count = entity_count;
     s.total_nodes = total_nodes;
     s.nodes = nodes;
@@ -1222,10 +1276,33 @@ int oracle_trace_stackless_spec(int32_t entity_count, int64_t total_nodes, const
         J->n = n;
         J->wave_begin = (int64_t)t * per < waves ? (int64_t)t * per : waves;
         J->wave_end = J->wave_begin + per < waves ? J->wave_begin + per : waves;
-        J->stackless = 1;
+        J->stackless = !stack;
+        J->stack = stack;
     }
     for (int t = 1; t < threads; ++t) pthread_create(&th[t], NULL, spec_run_job, &jobs[t]);
     spec_run_job(&jobs[0]);
     for (int t = 1; t < threads; ++t) pthread_join(th[t], NULL);
     return 0;
+}
+
+/* The host twin of the speculative stackless walk's grid launch
+ * (spec_bvh2.h, D == 0): 64 consecutive rays per wave, `inner` node steps per
+ * round, `queue_slots` queued leaves per lane (1..16). */
+int oracle_trace_stackless_spec(int32_t entity_count, int64_t total_nodes, const crt_stackless_node* nodes,
+                                const crt_triangle* tris, const crt_vertex* verts, const crt_entity* ents, int query,
+                                uint32_t flags, const crt_ray* rays, crt_hit* hits, int64_t n, int threads,
+                                uint32_t* tri_tests, int32_t max_iter, int32_t inner, int32_t queue_slots) {
+    return spec_bvh2_run(0, entity_count, total_nodes, nodes, tris, verts, ents, query, flags, rays, hits, n, threads,
+                         tri_tests, max_iter, inner, queue_slots);
+}
+
+/* The host twin of the speculative stack-layout walk's grid launch
+ * (spec_bvh2.h, D > 0): as above with `queue_slots` in 2..16 (a node queues
+ * up to two leaf children). */
+int oracle_trace_stack_spec(int32_t entity_count, int64_t total_nodes, const crt_stack_node* nodes,
+                            const crt_triangle* tris, const crt_vertex* verts, const crt_entity* ents, int query,
+                            uint32_t flags, const crt_ray* rays, crt_hit* hits, int64_t n, int threads,
+                            uint32_t* tri_tests, int32_t max_iter, int32_t inner, int32_t queue_slots) {
+    return spec_bvh2_run(1, entity_count, total_nodes, nodes, tris, verts, ents, query, flags, rays, hits, n, threads,
+                         tri_tests, max_iter, inner, queue_slots);
 }

@@ -195,3 +195,56 @@ def crafted_wide_rays(seed: int = 5) -> np.ndarray:
     r["tmax"] = np.float32(1e6)
     r["id"] = np.arange(300, dtype=np.uint32)
     return r
+
+
+def instanced_bvh2(crt, fmt: int):
+    """Two objects (boxes, an icosphere cloud) built in BVH2 format `fmt` (0 stackless, 1 stack) and concatenated
+    as AddObject does, with six entities (rotations about y and translations, one of them translucent).  Returns
+    (nodes, triangles, vertices, entities)."""
+    from candela_amd import scenes
+    b0 = crt.build_bvh(fmt, scenes.boxes_scene())
+    b3 = crt.build_bvh(fmt, scenes.icosphere_cloud(count=20, subdivisions=2, seed=3, box=16.0),
+                       t_offset=b0.triangles.size)
+    nodes = np.concatenate([b0.nodes, b3.nodes])
+    t3 = b3.triangles.copy()
+    t3["packed"][:, :3] += b0.vertices.size
+    tris = np.concatenate([b0.triangles, t3])
+    verts = np.concatenate([b0.vertices, b3.vertices])
+    ents = np.zeros(6, crt.ENTITY_DTYPE)
+    rng = np.random.default_rng(5)
+    for k in range(6):
+        ang = rng.uniform(0, 2 * np.pi)
+        m = np.eye(4)
+        m[:3, :3] = [[np.cos(ang), 0, np.sin(ang)], [0, 1, 0], [-np.sin(ang), 0, np.cos(ang)]]
+        m[:3, 3] = rng.uniform(-10, 10, 3)
+        off, cnt = (0, b0.nodes.size) if k % 2 == 0 else (b0.nodes.size, b3.nodes.size)
+        mm = np.ascontiguousarray(m.astype(np.float32).T.reshape(16))
+        crt.check(crt.lib().crt_make_entity(mm.ctypes.data, 0.0, 0.5 if k == 4 else 0.0, off, cnt,
+                                            ents[k:k + 1].ctypes.data))
+    return nodes, tris, verts, ents
+
+
+def crafted_stack(kind: str):
+    """Small hand-written stack-layout buffers that exercise one traversal guard each; every box is [-1, 2]^3 and
+    every leaf holds triangle 0.
+      'cycle'      entry 0: left an inner child that is entry 0 again, right a leaf -> ITER_CAP at 1024
+      'bad_child'  entry 0: left a leaf, right an inner child at entry 1000 -> BAD_INDEX
+    (STACK_OVERFLOW: crafted_stack_chain.)"""
+    bd = np.dtype([("min", "<f4", 4), ("max", "<f4", 4)])
+    sd = np.dtype([("left", bd), ("right", bd)])
+    n = np.zeros(1, sd)
+    for side in ("left", "right"):
+        n[side]["min"][:, :3] = [-1, -1, -1]
+        n[side]["max"][:, :3] = [2, 2, 2]
+    leaf = (0 << 4) | 1
+    if kind == "cycle":
+        n["left"]["min"][0, 3] = _fbits(-1)
+        n["left"]["max"][0, 3] = _fbits(0)
+        n["right"]["min"][0, 3] = _fbits(leaf)
+    elif kind == "bad_child":
+        n["left"]["min"][0, 3] = _fbits(leaf)
+        n["right"]["min"][0, 3] = _fbits(-1)
+        n["right"]["max"][0, 3] = _fbits(1000)
+    else:
+        raise ValueError(kind)
+    return n

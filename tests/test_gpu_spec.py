@@ -143,16 +143,8 @@ def test_instances_flags_and_ragged(gpu, orc):
 
 
 def test_errors(gpu, layouts):
-    from candela_amd import scenes
-    r = gpu.RayIntersector(1)          # the stack layout has no speculative walk
-    r.add_object(0, scenes.boxes_scene())
-    r.push_entity(0)
-    r.buffer_entities()
-    r.buffer_data()
     rays = gpu.DeviceArray.from_host(random_rays(64, [0, 1, 0], [1, 2, 1], seed=1))
     hits = gpu.DeviceArray(gpu.HIT_DTYPE, 64)
-    with pytest.raises(gpu.CrtError, match="wide"):
-        gpu.trace(r.scene(), 0, rays, hits, kernel=gpu.KERNEL_SPECULATIVE)
     with pytest.raises(gpu.CrtError, match="variant 0..5"):
         gpu.trace(layouts[4].scene, 0, rays, hits, kernel=gpu.KERNEL_SPECULATIVE, variant=6)
     s = gpu.Stream()

@@ -16,7 +16,7 @@ test_touching_boxes_against_brute_force).
 import numpy as np
 import pytest
 
-from oracle_cases import axis_aligned_rays, crafted_stackless, random_rays
+from oracle_cases import axis_aligned_rays, crafted_stackless, instanced_bvh2, random_rays
 
 SCHEDULES = [None, (1, 1), (4, 2), (64, 16)]
 
@@ -76,34 +76,8 @@ def test_touching_boxes_against_brute_force(crt, orc):
         assert (h["status"] & 0xFFFFFF >= ref["status"] & 0xFFFFFF).all()
 
 
-def _instanced(crt):
-    """Two objects (boxes, an icosphere cloud) concatenated as AddObject does, six entities with rotations and
-    translations, one of them translucent."""
-    from candela_amd import scenes
-    b0 = crt.build_bvh(0, scenes.boxes_scene())
-    b3 = crt.build_bvh(0, scenes.icosphere_cloud(count=20, subdivisions=2, seed=3, box=16.0),
-                       t_offset=b0.triangles.size)
-    nodes = np.concatenate([b0.nodes, b3.nodes])
-    t3 = b3.triangles.copy()
-    t3["packed"][:, :3] += b0.vertices.size
-    tris = np.concatenate([b0.triangles, t3])
-    verts = np.concatenate([b0.vertices, b3.vertices])
-    ents = np.zeros(6, crt.ENTITY_DTYPE)
-    rng = np.random.default_rng(5)
-    for k in range(6):
-        ang = rng.uniform(0, 2 * np.pi)
-        m = np.eye(4)
-        m[:3, :3] = [[np.cos(ang), 0, np.sin(ang)], [0, 1, 0], [-np.sin(ang), 0, np.cos(ang)]]
-        m[:3, 3] = rng.uniform(-10, 10, 3)
-        off, cnt = (0, b0.nodes.size) if k % 2 == 0 else (b0.nodes.size, b3.nodes.size)
-        mm = np.ascontiguousarray(m.astype(np.float32).T.reshape(16))
-        crt.check(crt.lib().crt_make_entity(mm.ctypes.data, 0.0, 0.5 if k == 4 else 0.0, off, cnt,
-                                            ents[k:k + 1].ctypes.data))
-    return nodes, tris, verts, ents
-
-
 def test_instances_and_flags(crt, orc):
-    nodes, tris, verts, ents = _instanced(crt)
+    nodes, tris, verts, ents = instanced_bvh2(crt, 0)
     rays = random_rays(6000, [-25, -5, -25], [25, 20, 25], seed=103, toward=[0, 2, 0])
     for flags in (0, crt.FLAG_IGNORE_TRANSPARENT, crt.FLAG_TRI0_IS_MISS):
         ref = orc.trace(0, nodes, tris, verts, ents, 0, rays, flags=flags)
