@@ -194,14 +194,15 @@ def time_batched(crt, batches, steps: int, warmup: int, stream, wide: bool) -> d
     return {"ms": ms, "config": cfg, "tune_ms": tune_ms, "rays": plan.rays}
 
 
-# batched presets 8..11 (the speculative walk per tile) -> crt_trace's CRT_KERNEL_SPECULATIVE presets
+# batched presets 8..11 (the speculative walk per tile) -> crt_trace's CRT_KERNEL_SPECULATIVE presets over the wide
+# layouts (over the BVH2 layouts: presets 0..3)
 BATCHED_SPEC_PRESET = {8: 0, 9: 1, 10: 3, 11: 4}
 
 
 def default_candidates(crt, wide: bool):
-    """Autotuner grid: preset, tiles per claim, waves per group, tile order, queue shards and, over the wide
-    layouts, the speculative walk (presets 8..11) and the child order of the sequential any-hit walks."""
-    presets = (0, 1, 4, 6, 7) + ((8, 9, 10, 11) if wide else ())
+    """Autotuner grid: preset (8..11: the speculative walk), tiles per claim, waves per group, tile order, queue
+    shards and, over the wide layouts, the child order of the sequential any-hit walks."""
+    presets = (0, 1, 4, 6, 7, 8, 9, 10, 11)
     return [crt.BatchConfig(v, t, w, 0, o, q, co) for v in presets for t in (1, 2) for w in (1, 4)
             for o in (0, 1) for q in (1, 16) for co in ((0, 1) if wide and v < 8 else (0,))]
 
@@ -246,7 +247,8 @@ def batched_vs_plain(crt, args, plain_choice: dict, stream) -> dict:
         # the batched records byte for byte against plain launches of the walk the chosen plan runs: the
         # speculative grid launch with the matching preset (a tile is one of its waves), else the wave walk
         v_b = b["config"]["variant"]
-        ref_kw = dict(kernel=crt.KERNEL_SPECULATIVE, variant=BATCHED_SPEC_PRESET[v_b]) if v_b >= 8 else \
+        spec_preset = BATCHED_SPEC_PRESET.get(v_b) if layout is not None else v_b - 8
+        ref_kw = dict(kernel=crt.KERNEL_SPECULATIVE, variant=spec_preset) if v_b >= 8 else \
             dict(kernel=crt.KERNEL_WAVE, variant=0)
         ref_flags = crt.FLAG_SLOT_ORDER if (layout is not None and b["config"]["child_order"] == 1) else 0
         for s in work.scenes:

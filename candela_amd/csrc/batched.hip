@@ -30,6 +30,7 @@
 #include "trace_launch.h"
 #include "wave_walk.h"
 #include "wide_walk.h"
+#include "spec_bvh2.h"
 #include "spec_walk.h"
 #include "thread_walk.h"
 
@@ -216,6 +217,56 @@ batched_spec(CRT_BATCH_ARGS) {
     spec_body<W, LDS_DEPTH, INNER, Q, WPG>(descs, tiles, ntiles, per_claim, counters, max_iter, shards, lds);
 }
 
+// The speculative walks of the two BVH2 layouts (spec_bvh2.h; D = 0 the
+// stackless layout, D > 0 the stack layout with D LDS stack entries) on the
+// same tile queue: a tile is one wave of crt_trace's speculative grid launch
+// on its batch, so with the same preset every hit record, counters included,
+// equals crt_trace's kernel 4.
+template <int D, int Q, int WPG>
+struct Bvh2SpecLds {
+    int32_t queue[Q * 64 * WPG];
+    int32_t stack[(D > 0 ? D : 1) * 64 * WPG];
+    spec::WaveLds<Q> waves[WPG];
+};
+
+template <int D, int INNER, int Q, int WPG>
+__device__ __forceinline__ void bvh2_spec_body(const BatchDesc* __restrict__ descs, const int2* __restrict__ tiles,
+                                               int32_t ntiles, int32_t per_claim,
+                                               unsigned long long* __restrict__ counters, int32_t max_iter,
+                                               int32_t shards, Bvh2SpecLds<D, Q, WPG>& lds) {
+    constexpr int kCols = 64 * WPG;
+    int32_t spill[D > 0 ? sdev::spill_entries<D>() : 1];
+    spec::WaveLds<Q>& wl = lds.waves[threadIdx.x >> 6];
+    ShardQueue q;
+    q.init(counters, ntiles, true, shards);
+    while (true) {
+        int32_t got = 0;
+        const int64_t base = q.claim(per_claim, got);
+        if (got == 0) break;
+        for (int32_t k = 0; k < got; ++k) {
+            const int32_t t = __builtin_amdgcn_readfirstlane(int32_t(q.at(base + k)));
+            const int2 tile = tiles[t];
+            const int32_t b = __builtin_amdgcn_readfirstlane(tile.x);
+            const int32_t first = __builtin_amdgcn_readfirstlane(tile.y);
+            const BatchDesc d = load_desc(descs, b);
+            const int64_t i = int64_t(first) + __lane_id();
+            const int64_t slot = i < d.count ? i : -1;
+            // one walk for both queries (the query is a per-tile value)
+            sbvh2::SpecBvh2Wave<false, D, INNER, Q, true, kCols> w;
+            w.dyn_any = d.query == CRT_QUERY_ANY;
+            w.q = (LdsInt*)(lds.queue + threadIdx.x);
+            if constexpr (D > 0) w.st = sdev::NodeStack<D>{lds.stack + threadIdx.x, kCols, spill};
+            w.run(d.scene, d.rays, d.hits, slot, d.flags, max_iter, d.tri_tests, wl);
+        }
+    }
+}
+
+template <int D, int INNER, int Q, int WPG>
+__global__ void __launch_bounds__(64 * WPG) batched_bvh2_spec(CRT_BATCH_ARGS) {
+    __shared__ Bvh2SpecLds<D, Q, WPG> lds;
+    bvh2_spec_body<D, INNER, Q, WPG>(descs, tiles, ntiles, per_claim, counters, max_iter, shards, lds);
+}
+
 // The stack format (FlattenStackBVH): each claimed tile runs the one-ray-per-
 // lane stack walk of crt_trace's thread kernel (thread_walk.h), its first
 // LDS_DEPTH entries in LDS; the walk's results do not depend on LDS_DEPTH, so
@@ -268,8 +319,22 @@ batched_stack_occ(CRT_BATCH_ARGS) {
 
 using BatchedFn = void (*)(const BatchDesc*, const int2*, int32_t, int32_t, unsigned long long*, int32_t, int32_t);
 
+// Speculative BVH2 presets 8..11 = crt_trace's speculative grid presets 0..3
+// over the same layout (INNER, queue slots; the stack layout keeps 8 stack
+// entries in LDS).
+template <int D, int WPG>
+static const void* batched_bvh2_spec_symbol(int variant) {
+    switch (variant) {
+        case 9: return reinterpret_cast<const void*>(&pdev::batched_bvh2_spec<D, 16, 8, WPG>);
+        case 10: return reinterpret_cast<const void*>(&pdev::batched_bvh2_spec<D, 16, 12, WPG>);
+        case 11: return reinterpret_cast<const void*>(&pdev::batched_bvh2_spec<D, 32, 16, WPG>);
+        default: return reinterpret_cast<const void*>(&pdev::batched_bvh2_spec<D, 8, 8, WPG>);
+    }
+}
+
 template <int WPG>
 static const void* batched_symbol(int variant) {
+    if (variant >= 8) return batched_bvh2_spec_symbol<0, WPG>(variant);
     switch (variant) {
         case 1: return reinterpret_cast<const void*>(&pdev::batched_stackless<8, 8, 1, WPG>);
         case 2: return reinterpret_cast<const void*>(&pdev::batched_stackless<8, 8, 4, WPG>);
@@ -317,6 +382,7 @@ static const void* batched_wide_symbol(int variant) {
 // for at least 5 (6) or 6 (7) waves per SIMD.
 template <int WPG>
 static const void* batched_stack_symbol(int variant) {
+    if (variant >= 8) return batched_bvh2_spec_symbol<8, WPG>(variant);
     switch (variant) {
         case 1: return reinterpret_cast<const void*>(&pdev::batched_stack<16, WPG>);
         case 2: return reinterpret_cast<const void*>(&pdev::batched_stack<32, WPG>);
@@ -385,7 +451,7 @@ static_assert(sizeof(crt_batch_config) == 28, "crt_batch_config is 7 int32 field
 
 int validate_config(const crt_batch_config& c) {
     if (c.variant < 0 || c.variant > 11)
-        return crt::set_error(CRT_E_INVALID, "batch config: variant must be 0..11 (8..11: wide formats only)");
+        return crt::set_error(CRT_E_INVALID, "batch config: variant must be 0..11");
     if (c.tiles_per_claim != 1 && c.tiles_per_claim != 2 && c.tiles_per_claim != 4)
         return crt::set_error(CRT_E_INVALID, "batch config: tiles_per_claim must be 1, 2 or 4");
     if (c.waves_per_group != 1 && c.waves_per_group != 2 && c.waves_per_group != 4)
@@ -421,8 +487,6 @@ int build_plan(const crt_batch* batches, int nbatches, const crt_batch_config& c
     std::vector<crt::BatchDesc> descs(static_cast<size_t>(nbatches));
     const int format = nbatches > 0 ? batches[0].scene.format : CRT_FORMAT_STACKLESS;
     const bool wide = format == CRT_FORMAT_WIDE4 || format == CRT_FORMAT_WIDE8;
-    if (cfg.variant >= 8 && !wide)
-        return crt::set_error(CRT_E_INVALID, "batch config: variants 8..11 (speculative walk) need a wide format");
     for (int i = 0; i < nbatches; ++i) {
         const crt_batch& b = batches[i];
         if (b.scene.format != CRT_FORMAT_STACKLESS && b.scene.format != CRT_FORMAT_STACK &&
@@ -586,10 +650,7 @@ int crt_autotune_batched(const crt_batch* batches, int nbatches, const crt_batch
         for (int i = 0; i < nbatches; ++i)
             orders |= batches[i].query == CRT_QUERY_ANY &&
                       (batches[i].scene.format == CRT_FORMAT_WIDE4 || batches[i].scene.format == CRT_FORMAT_WIDE8);
-        bool wide_fmt = nbatches > 0;
-        for (int i = 0; i < nbatches; ++i)
-            wide_fmt &= batches[i].scene.format == CRT_FORMAT_WIDE4 || batches[i].scene.format == CRT_FORMAT_WIDE8;
-        for (int v = 0; v < (wide_fmt ? 12 : 8); ++v)
+        for (int v = 0; v < 12; ++v)
             for (int t : {1, 2, 4})
                 for (int w : {1, 4})
                     for (int o : {0, 1})
@@ -611,11 +672,7 @@ int crt_autotune_batched(const crt_batch* batches, int nbatches, const crt_batch
     PLAN_HIP(hipEventCreate(&e1), "hipEventCreate");
     float best_t = -1.0f;
     int rc = CRT_OK;
-    bool wide_batches = nbatches > 0;
-    for (int i = 0; i < nbatches; ++i)
-        wide_batches &= batches[i].scene.format == CRT_FORMAT_WIDE4 || batches[i].scene.format == CRT_FORMAT_WIDE8;
     for (const crt_batch_config& c : cands) {
-        if (c.variant >= 8 && !wide_batches) continue;   // the speculative presets walk the wide formats only
         crt_batch_plan* p = nullptr;
         rc = crt_batch_plan_create(batches, nbatches, &c, &p);
         if (rc) break;

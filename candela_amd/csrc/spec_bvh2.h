@@ -64,7 +64,10 @@ using spec::lanes_below;
 using spec::wave_lds_sync;
 using spec::WaveLds;
 
-template <bool ANY, int D, int INNER, int Q>
+// ANY: the query; DYN: the query is the run-time value dyn_any instead (one
+// walk for both queries, as the batched tiles need); COLS: the LDS column
+// stride, the threads of the workgroup.
+template <bool ANY, int D, int INNER, int Q, bool DYN = false, int COLS = 256>
 struct SpecBvh2Wave {
     static_assert(Q >= 2 && Q <= 16, "queue slots fit 4 bits");
     static_assert(D >= 0 && D <= sdev::kStackCap, "LDS stack entries");
@@ -77,10 +80,13 @@ struct SpecBvh2Wave {
     int32_t start, count, ptr, sp, it;
     int32_t tri, mesh, slot, ent, hit_ent;
     uint32_t tests, status, visits;
-    int32_t nq;          // queued leaves (this lane's LDS column, stride 256)
+    int32_t nq;          // queued leaves (this lane's LDS column, stride COLS)
     bool active, walk_done;
+    bool dyn_any;        // the query when DYN (wave-uniform)
     LdsInt* q;
     sdev::NodeStack<kStack ? D : 1> st;   // the stack walk's node stack
+
+    __device__ __forceinline__ bool any() const { return DYN ? dyn_any : ANY; }
 
     __device__ __forceinline__ bool begin_walk(const SceneArgs& s, const crt_ray* __restrict__ rays, uint32_t flags) {
         int32_t e = ent + 1;
@@ -92,7 +98,7 @@ struct SpecBvh2Wave {
         const crt_entity& en = s.entities[e];
         const float4* rp = reinterpret_cast<const float4*>(rays + slot);
         const float4 a = rp[0], b = rp[1];
-        if (ANY) tmax = ray_tmax;
+        if (any()) tmax = ray_tmax;
         ro = xform(en.inverse, mk(a.x, a.y, a.z), 1.0f);
         rd = xform(en.inverse, mk(b.x, b.y, b.z), 0.0f);
         inv = mk(1.0f / rd.x, 1.0f / rd.y, 1.0f / rd.z);
@@ -113,7 +119,7 @@ struct SpecBvh2Wave {
                                            uint32_t flags) {
         float4 tuvw = make_float4(-1.0f, -1.0f, -1.0f, -1.0f);
         const float closest = tri >= 0 ? tmax : -1.0f;
-        if (ANY) {
+        if (any()) {
             if (tri >= 0) tuvw.x = closest;
         } else {
             const bool ok = closest > 0.0f && ((flags & CRT_FLAG_TRI0_IS_MISS) ? tri > 0 : tri >= 0);
@@ -176,9 +182,9 @@ struct SpecBvh2Wave {
         const float rt = rp != -1 ? -1.0f : bounds_stack(rmin, rmax, ro, inv, tmax);
         // slots nq and nq + 1 exist (can_step); each is kept only for a leaf
         // with triangles
-        q[nq * 256] = lp;
+        q[nq * COLS] = lp;
         nq += sdev::leaf_nonempty(lp) ? 1 : 0;
-        q[nq * 256] = rp;
+        q[nq * COLS] = rp;
         nq += sdev::leaf_nonempty(rp) ? 1 : 0;
         if (lt > 0.0f && rt > 0.0f) {
             int32_t next = __float_as_int(lmax.w) + start;
@@ -232,7 +238,7 @@ struct SpecBvh2Wave {
         }
         // the next free queue slot is written whatever the test says; a hit
         // leaf advances the count
-        q[nq * 256] = packed;
+        q[nq * COLS] = packed;
         nq += hit ? 1 : 0;
         if (link < 0) return false;
         ptr = link + start;
@@ -253,7 +259,7 @@ struct SpecBvh2Wave {
         }
         for (int32_t j = 0; j < nq; ++j) {
             L.list[excl + j] = uint16_t((lane << 4) | j);
-            tests += uint32_t(q[j * 256] & 0xF);
+            tests += uint32_t(q[j * COLS] & 0xF);
         }
         L.keys[lane] = ~0ull;
         wave_lds_sync();
@@ -263,7 +269,7 @@ struct SpecBvh2Wave {
             const uint32_t e = valid ? uint32_t(L.list[idx]) : 0u;
             const int32_t owner = int32_t(e >> 4), j = int32_t(e & 15u);
             // the owner's queue slot j sits (owner - lane) columns from this lane's
-            const int32_t leaf = valid ? q[(owner - lane) + j * 256] : 0;
+            const int32_t leaf = valid ? q[(owner - lane) + j * COLS] : 0;
             const float ox = __shfl(ro.x, owner), oy = __shfl(ro.y, owner), oz = __shfl(ro.z, owner);
             const float dx = __shfl(rd.x, owner), dy = __shfl(rd.y, owner), dz = __shfl(rd.z, owner);
             const float otmax = __shfl(tmax, owner);
@@ -278,22 +284,22 @@ struct SpecBvh2Wave {
                 if (v && t > 0.0f && t < otmax) {
                     const unsigned long long order = (unsigned long long)(uint32_t(idx) << 4 | uint32_t(x));
                     const unsigned long long tbits = __float_as_uint(t);
-                    atomicMin(&L.keys[owner], ANY ? ((order << 32) | tbits) : ((tbits << 32) | order));
+                    atomicMin(&L.keys[owner], any() ? ((order << 32) | tbits) : ((tbits << 32) | order));
                 }
             }
         }
         wave_lds_sync();
         const unsigned long long k = L.keys[lane];
         if (k != ~0ull) {
-            const uint32_t order = ANY ? uint32_t(k >> 32) : uint32_t(k);
-            const uint32_t tb = ANY ? uint32_t(k) : uint32_t(k >> 32);
+            const uint32_t order = any() ? uint32_t(k >> 32) : uint32_t(k);
+            const uint32_t tb = any() ? uint32_t(k) : uint32_t(k >> 32);
             const int32_t j = int32_t(L.list[order >> 4] & 15u);
-            const int32_t idx = (q[j * 256] >> 4) + int32_t(order & 15u);
+            const int32_t idx = (q[j * COLS] >> 4) + int32_t(order & 15u);
             tmax = __uint_as_float(tb);
             tri = idx;
             mesh = s.tris[idx].packed[3];
             hit_ent = ent;
-            if (ANY) walk_done = true;   // the first accepted triangle ends an any-hit walk
+            if (any()) walk_done = true;   // the first accepted triangle ends an any-hit walk
         }
         nq = 0;
         wave_lds_sync();   // the list and keys are reused by the next phase
@@ -316,7 +322,7 @@ struct SpecBvh2Wave {
         if (em && (__popcll(em) >= kWalkEndBatch || !__ballot(can_step()))) {
             if (ends) {
                 visits += uint32_t(it);
-                bool fin = ANY && tri >= 0;
+                bool fin = any() && tri >= 0;
                 if (!fin) fin = !begin_walk(s, rays, flags);
                 if (fin) {
                     finish(s, rays, hits, tri_tests, flags);

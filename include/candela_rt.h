@@ -368,18 +368,21 @@ enum {
     CRT_KERNEL_WAVE = 3,        /* one ray per lane, leaf batching, XCD-aware block
                                    order (stackless: + node window; wide formats:
                                    leaves of a node parked and tested by the wave) */
-    CRT_KERNEL_SPECULATIVE = 4, /* wide and stackless formats: one ray per lane,
-                                   lanes never wait for leaf tests: hit leaves go
-                                   to a per-lane queue, and after a round of node
-                                   steps the wave tests the queued leaves 64 at a
-                                   time, with culling by the tmax of the last such
-                                   phase.  Grid launch, 64 consecutive rays per
-                                   wave: the records are deterministic; nearest t
-                                   and hit / miss equal the other walks' of the
-                                   format, the triangle up to exact distance ties;
-                                   visit and test counts depend on the wave's
-                                   schedule (variant 0..5 over the wide formats,
-                                   0..3 over the stackless format) */
+    CRT_KERNEL_SPECULATIVE = 4, /* every format: one ray per lane, lanes never
+                                   wait for leaf tests: leaves go to a per-lane
+                                   queue, and after a round of node steps the wave
+                                   tests the queued leaves 64 at a time, with
+                                   culling by the tmax of the last such phase.
+                                   Grid launch, 64 consecutive rays per wave: the
+                                   records are deterministic; nearest t and hit /
+                                   miss equal the other walks' of the format, the
+                                   triangle up to exact distance ties (stackless
+                                   format: where boxes touch, a hit the reference
+                                   walk's culling skips can be found, so t is then
+                                   nearer); visit and test counts depend on the
+                                   wave's schedule (variant 0..5 over the wide
+                                   formats, 0..3 over the stack and stackless
+                                   formats) */
     CRT_KERNEL_SPECULATIVE_REFILL = 5 /* the same walk in device-filling waves that
                                    refill idle lanes from a work queue (uses
                                    config->queue like CRT_KERNEL_PERSISTENT); counts
@@ -838,9 +841,10 @@ typedef struct crt_batch_config {
     int32_t variant;        /* walk preset 0..7 (stackless: inner steps, leaf batch, window;
                                wide: LDS stack depth, inner steps, leaf batch; stack: LDS stack
                                depth; 6 and 7 are compiled for at least 5 or 6 waves per SIMD);
-                               wide formats also 8..11: the speculative walk with crt_trace's
-                               CRT_KERNEL_SPECULATIVE presets 0, 1, 3 and 4 (a tile is one wave of
-                               that launch, so its records equal crt_trace's) */
+                               8..11: the speculative walk with crt_trace's
+                               CRT_KERNEL_SPECULATIVE presets 0, 1, 3 and 4 (wide formats) or
+                               0..3 (stack and stackless formats); a tile is one wave of that
+                               launch, so its records equal crt_trace's */
     int32_t tiles_per_claim;/* 64-ray tiles a wave claims per atomic: 1, 2 or 4 */
     int32_t waves_per_group;/* 1, 2 or 4 waves per workgroup */
     int32_t grid;           /* workgroups; 0 = enough to fill every CU */

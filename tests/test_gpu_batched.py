@@ -160,9 +160,15 @@ def test_batched_wide(work, orc, width):
         crt.BatchPlan(work.batches(crt, layout), crt.BatchConfig(0, 1, 4, 0, 1, 16, 2))
     with pytest.raises(crt.CrtError):
         crt.BatchPlan(work.batches(crt, layout), crt.BatchConfig(12, 1, 4, 0, 1, 16, 0))
-    # the speculative presets walk the wide formats only
-    with pytest.raises(crt.CrtError, match="wide format"):
-        crt.BatchPlan(work.batches(crt), crt.BatchConfig(8, 1, 4, 0, 1, 16, 0))
+
+
+def span_twin(s, which, fn):
+    """The host twin `fn(query, rays)` replayed per batch, as the plan's tiles run: the closest-hit rays of a scene
+    are two batches (primary, then GI) whose tiles start at each batch's first ray."""
+    if which == "shadow":
+        return fn(1, s.shadow)
+    parts = [s.closest[:s.primary], s.closest[s.primary:s.primary + s.gi]]
+    return np.concatenate([fn(0, p) for p in parts if p.size])
 
 
 # batched speculative presets -> crt_trace's CRT_KERNEL_SPECULATIVE presets (the host twin's schedules)
@@ -189,17 +195,42 @@ def test_batched_wide_speculative(work, orc, width):
             crt.lib().crt_device_sync()
             for s in work.scenes:
                 w = s.wide[layout].host
-                for which, h, q, rays in (("closest", s.h_closest, 0, s.closest), ("shadow", s.h_shadow, 1, s.shadow)):
+                for which, h in (("closest", s.h_closest), ("shadow", s.h_shadow)):
                     key = (s.index, which, variant)
                     if key not in ref:
-                        ref[key] = orc.trace_wide_spec(w.width, w.groups, w.records, w.entities, q, rays,
-                                                       variant=SPEC_OF_BATCHED[variant])
+                        ref[key] = span_twin(s, which, lambda q_, r_: orc.trace_wide_spec(
+                            w.width, w.groups, w.records, w.entities, q_, r_, variant=SPEC_OF_BATCHED[variant]))
                     got = h.download()
                     assert got.tobytes() == ref[key].tobytes(), \
                         f"speculative wide{width} v{variant} scene {s.index} {which} (replay {replay})"
         log(f"[parity] batched speculative wide{width} variant={variant} tpc={tpc} wpg={wpg} order={order} "
             f"shards={shards}: {work.rays} rays bit-identical to the twin of the speculative grid launch "
             "(eager and graph)")
+
+
+def test_batched_stackless_speculative(work, orc):
+    """Presets 8..11 over the stackless layout run its speculative walk per 64-ray tile: one wave of crt_trace's
+    speculative grid launch (presets 0..3), so every record, counters included, equals the host twin of that
+    launch -- eager and graph, for several claim sizes, group sizes, tile orders and shard counts."""
+    import candela_amd as crt
+    ref = {}
+    for variant, tpc, wpg, order, shards in ((8, 1, 4, 1, 16), (9, 2, 1, 0, 1), (10, 4, 2, 1, 16), (11, 1, 4, 0, 16)):
+        clear_hits(work)
+        plan = crt.BatchPlan(work.batches(crt), crt.BatchConfig(variant, tpc, wpg, 0, order, shards, 0))
+        for replay in range(2):
+            plan.launch(graph=replay == 1)
+            crt.lib().crt_device_sync()
+            for s in work.scenes:
+                hb = s.host
+                for which, h in (("closest", s.h_closest), ("shadow", s.h_shadow)):
+                    key = (s.index, which, variant)
+                    if key not in ref:
+                        ref[key] = span_twin(s, which, lambda q_, r_: orc.trace_stackless_spec(
+                            hb["nodes"], hb["triangles"], hb["vertices"], hb["entities"], q_, r_, variant=variant - 8))
+                    assert h.download().tobytes() == ref[key].tobytes(), \
+                        f"speculative stackless v{variant} scene {s.index} {which} (replay {replay})"
+        log(f"[parity] batched speculative stackless variant={variant} tpc={tpc} wpg={wpg} order={order} "
+            f"shards={shards}: {work.rays} rays bit-identical to the twin (eager and graph)")
 
 
 def test_autotune_wide_child_order(work, orc):
@@ -266,6 +297,27 @@ def test_batched_stack_format(orc):
                     assert np.array_equal(dt.download(), ot)
     log(f"[parity] batched stack format: {sum(b[2].count for b in batches)} rays, 8 presets x 3 shapes, "
         "eager and graph, bit-identical")
+    # presets 8..11: the speculative stack walk per tile, equal to the twin of crt_trace's speculative grid launch
+    # (presets 0..3), counters included
+    twins = []
+    for k, (scene, q, dr, _, _, _) in enumerate(batches):
+        hb = ctxs[k // 2].host_buffers()
+        rays = dr.download()
+        twins.append([orc.trace_stack_spec(hb["nodes"], hb["triangles"], hb["vertices"], hb["entities"], q, rays,
+                                           variant=v, with_tests=True) for v in range(4)])
+    for variant, wpg, tpc in ((8, 4, 1), (9, 1, 2), (10, 2, 4), (11, 4, 2)):
+        for _, dh, dt, _ in bufs:
+            dh.fill_bytes(0xAB)
+            dt.fill_bytes(0xCD)
+        plan = crt.BatchPlan(batches, crt.BatchConfig(variant, tpc, wpg, 0, 1, 16))
+        for replay in range(2):
+            plan.launch(graph=replay == 1)
+            crt.lib().crt_device_sync()
+            for (_, dh, dt, _), tw in zip(bufs, twins):
+                o, ot = tw[variant - 8]
+                assert dh.download().tobytes() == o.tobytes(), f"speculative stack v{variant} wpg{wpg}"
+                assert np.array_equal(dt.download(), ot)
+    log("[parity] batched speculative stack format: presets 8..11, eager and graph, bit-identical to the twin")
 
 
 def test_autotune_picks_a_layout(work, orc):
