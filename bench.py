@@ -1031,12 +1031,14 @@ def main(argv=None) -> int:
         ohg = oracle.trace(fmt, nodes, tris, verts, ents, 0, gi, threads=threads)
         oha = oracle.trace(fmt, nodes, tris, verts, ents, 1, ao, threads=threads)
         def bvh2_twin(stack_layout, hb_, q, rays_, kernel, variant, walk):
-            # the speculative grid kernel's twin replays its wave schedule; the other kernels' records are the
-            # reference walk's (`walk`, computed on demand)
-            if kernel == crt.KERNEL_SPECULATIVE:
+            # the speculative kernels' twin replays the preset's schedule (a ray's record does not depend on
+            # which rays share its wave, so this holds for the refilling launch too); the other kernels' records
+            # are the reference walk's (`walk`, computed on demand)
+            if kernel in (crt.KERNEL_SPECULATIVE, crt.KERNEL_SPECULATIVE_REFILL):
                 fn = oracle.trace_stack_spec if stack_layout else oracle.trace_stackless_spec
+                presets = oracle.SBVH2_PRESETS if kernel == crt.KERNEL_SPECULATIVE else oracle.SBVH2_P_PRESETS
                 return fn(hb_["nodes"], hb_["triangles"], hb_["vertices"], hb_["entities"], q, rays_,
-                          variant=variant, threads=threads)
+                          schedule=presets[variant], threads=threads)
             return walk()
 
         if best["layout"] in ("bvh2", "stack"):
@@ -1056,26 +1058,22 @@ def main(argv=None) -> int:
             w = wides[wide_key(best)].host
 
             def wide_twin(q, rays_, kernel, variant, flags):
-                # the speculative grid kernel's twin replays its wave schedule; the refilling one has no fixed
-                # schedule, so it is held to the sequential walk's nearest t and occlusion below
-                if kernel == crt.KERNEL_SPECULATIVE:
+                # the speculative kernels' twin replays the preset's schedule, for the refilling launch too
+                if kernel in (crt.KERNEL_SPECULATIVE, crt.KERNEL_SPECULATIVE_REFILL):
+                    presets = oracle.SPEC_PRESETS if kernel == crt.KERNEL_SPECULATIVE else oracle.SPEC_P_PRESETS
                     return oracle.trace_wide_spec(w.width, w.groups, w.records, w.entities, q, rays_,
-                                                  variant=variant, flags=flags, threads=threads)
+                                                  schedule=presets[variant], flags=flags, threads=threads)
                 return oracle.trace_wide(w.width, w.groups, w.records, w.entities, q, rays_, threads=threads,
                                          flags=flags)
 
             twin_g = wide_twin(0, gi, best["kernel"], best["variant"], 0)
             twin_a = wide_twin(1, ao, *ao_of(best), ao_flags(crt, best))
-        schedule_free = {crt.KERNEL_SPECULATIVE_REFILL}
-        gi_exact = best["kernel"] not in schedule_free
-        ao_exact = ao_of(best)[0] not in schedule_free
         gi_miss = g["tri"] != ohg["tri"]
         parity = {"rays": int(g.size + a.size),
-                  # bit-identical records against the twin of each launch's kernel (kernels with a fixed schedule)
-                  "exact": float((np.concatenate([g, a]).tobytes() == np.concatenate([twin_g, twin_a]).tobytes()))
-                  if gi_exact and ao_exact else None,
-                  "gi_exact_vs_twin": bool(g.tobytes() == twin_g.tobytes()) if gi_exact else None,
-                  "ao_exact_vs_twin": bool(a.tobytes() == twin_a.tobytes()) if ao_exact else None,
+                  # bit-identical records against the twin of each launch's kernel
+                  "exact": float((np.concatenate([g, a]).tobytes() == np.concatenate([twin_g, twin_a]).tobytes())),
+                  "gi_exact_vs_twin": bool(g.tobytes() == twin_g.tobytes()),
+                  "ao_exact_vs_twin": bool(a.tobytes() == twin_a.tobytes()),
                   "gi_t_equal_to_twin": float(np.mean(g["t"] == twin_g["t"])),
                   "ao_occlusion_equal_to_twin": float(np.mean((a["t"] > 0) == (twin_a["t"] > 0))),
                   # closest hits: the triangle is unique up to exact distance ties (shared edges and vertices)

@@ -39,10 +39,10 @@
 //              walk's node sequence only gains whole subtrees and a
 //              closest-hit walk still ends with the reference walk's nearest t
 //
-// With the grid launch the schedule is fixed and the host twin
-// (oracle_trace.c: oracle_trace_stackless_spec, oracle_trace_stack_spec)
-// replays it, counters included; with the refilling launch the nearest t,
-// hit / miss and occlusion are fixed.
+// As in spec_walk.h, a ray's record does not depend on which rays share its
+// wave.  The host twin (oracle_trace.c: oracle_trace_stackless_spec,
+// oracle_trace_stack_spec) replays the grid launch; run with a refilling
+// preset's (INNER, Q) it gives that launch's records, counters included.
 #pragma once
 
 #include <hip/hip_runtime.h>

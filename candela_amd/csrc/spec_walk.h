@@ -33,12 +33,15 @@
 //              only errs toward visiting more nodes, so a closest-hit walk
 //              still reports the nearest triangle that any wide walk finds
 //
-// A lane's visits and tests depend on the leaf phases, which depend on the
-// rays sharing its wave: with the grid launch (64 consecutive rays per wave)
-// the records are deterministic and the host twin (oracle_trace.c:
-// oracle_trace_wide_spec) replays the schedule; with the refilling launch the
-// nearest t, the hit / miss answer and any-hit occlusion are still fixed, the
-// triangle is fixed up to exact distance ties, and the counters vary.
+// A ray's record does not depend on which rays share its wave.  In a round a
+// lane that can step always steps, and a lane that cannot (queue full, walk
+// over) cannot become able again before the leaf phase; so every lane takes
+// INNER node steps, or fewer when its own queue fills or its walk ends, and
+// then its own leaves are tested, whatever the other lanes do.  Walk ends and
+// refills happen between rounds and only delay a lane.  The host twin
+// (oracle_trace.c: oracle_trace_wide_spec) replays the grid launch; run with a
+// refilling preset's (LDS depth, INNER, Q) it gives that launch's records too,
+// counters included, whichever rays the waves claimed.
 #pragma once
 
 #include <hip/hip_runtime.h>

@@ -373,20 +373,23 @@ enum {
                                    queue, and after a round of node steps the wave
                                    tests the queued leaves 64 at a time, with
                                    culling by the tmax of the last such phase.
-                                   Grid launch, 64 consecutive rays per wave: the
-                                   records are deterministic; nearest t and hit /
-                                   miss equal the other walks' of the format, the
-                                   triangle up to exact distance ties (stackless
-                                   format: where boxes touch, a hit the reference
-                                   walk's culling skips can be found, so t is then
-                                   nearer); visit and test counts depend on the
-                                   wave's schedule (variant 0..5 over the wide
-                                   formats, 0..3 over the stack and stackless
-                                   formats) */
+                                   Grid launch, 64 consecutive rays per wave.
+                                   Records are deterministic, counters included;
+                                   nearest t and hit / miss equal the other walks'
+                                   of the format, the triangle up to exact
+                                   distance ties (stackless format: where boxes
+                                   touch, a hit the reference walk's culling skips
+                                   can be found, so t is then nearer); visit and
+                                   test counts follow the preset's schedule
+                                   (variant 0..5 over the wide formats, 0..3 over
+                                   the stack and stackless formats) */
     CRT_KERNEL_SPECULATIVE_REFILL = 5 /* the same walk in device-filling waves that
                                    refill idle lanes from a work queue (uses
-                                   config->queue like CRT_KERNEL_PERSISTENT); counts
-                                   vary from run to run (variants as kernel 4) */
+                                   config->queue like CRT_KERNEL_PERSISTENT); a
+                                   ray's record does not depend on which rays
+                                   share its wave, so records are deterministic
+                                   too (variants as kernel 4, with their own
+                                   schedules) */
 };
 
 typedef struct crt_trace_config {

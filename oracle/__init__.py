@@ -173,6 +173,10 @@ def trace_wide(width: int, groups: np.ndarray, records: np.ndarray, ents: np.nda
 # Schedules of libcandela_rt's speculative wide kernels by variant: (LDS stack entries, inner node steps per round,
 # queue slots per lane over the BVH4; the BVH8 kernels use 4 more, at most 16).
 SPEC_PRESETS = {0: (8, 4, 8), 1: (8, 8, 8), 2: (8, 2, 8), 3: (8, 4, 8), 4: (8, 8, 8), 5: (7, 6, 6)}
+# The refilling speculative kernel's presets as schedules of the same twin.  A lane's sequence of node steps and
+# leaf phases does not depend on which rays share its wave, so the grid twin with the refilling preset's
+# (LDS depth, inner steps, queue slots) gives that launch's records, counters included.
+SPEC_P_PRESETS = {0: (8, 4, 8), 1: (8, 8, 8), 2: (8, 8, 8), 3: (8, 6, 8), 4: (8, 8, 8), 5: (7, 8, 6)}
 
 
 def spec_queue_slots(width: int, q4: int) -> int:
@@ -206,6 +210,8 @@ def trace_wide_spec(width: int, groups: np.ndarray, records: np.ndarray, ents: n
 # Schedules of libcandela_rt's speculative kernels over the two BVH2 layouts (stackless and stack) by variant:
 # (inner node steps per round, queue slots per lane).
 SBVH2_PRESETS = {0: (8, 8), 1: (16, 8), 2: (16, 12), 3: (32, 16)}
+# ... and of the refilling kernels over those layouts (see SPEC_P_PRESETS)
+SBVH2_P_PRESETS = {0: (16, 8), 1: (32, 16), 2: (16, 12), 3: (32, 16)}
 
 
 def _trace_bvh2_spec(stack, nodes, tris, verts, ents, query, rays, variant, flags, threads, max_iter, with_tests,

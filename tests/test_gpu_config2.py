@@ -14,10 +14,12 @@ traced on the GPU and compared with the host oracle over the same buffers:
   every mismatch at an exactly equal t (the same distance reached through
   another triangle of a shared edge), RTAO occlusion on every ray;
 - the speculative BVH4 kernels the benchmark picks (refilling speculative
-  walk for GI, speculative grid walk for RTAO): RTAO bit-identical to the
-  host twin that replays the grid schedule; GI nearest t equal to the
-  sequential wide walk's on every ray (its visit counts follow the refill
-  schedule), and the same bar against the reference walk as above;
+  walk for GI, speculative grid walk for RTAO, over the greedy and the SAH
+  collapse): bit-identical to the host twin run with each preset's schedule
+  (a ray's record does not depend on which rays share its wave, so this
+  holds for the refilling launch as well); GI nearest t equal to the
+  sequential wide walk's on every ray, and the same bar against the
+  reference walk as above;
 - the stack layout (FlattenStackBVH of the same meshes): bit-identical to the
   reference's stack walk.
 """
@@ -116,6 +118,8 @@ def test_config2_bvh4_speculative_choice(gpu, orc, work, ref):
     a = _trace(gpu, dw.scene, 1, work.ao, gpu.KERNEL_SPECULATIVE, 5)
     ta = orc.trace_wide_spec(w.width, w.groups, w.records, w.entities, 1, work.ao, variant=5)
     assert _differs(a, ta) == 0, "AO records differ from the twin of the speculative grid walk"
+    tw = orc.trace_wide_spec(w.width, w.groups, w.records, w.entities, 0, work.gi, schedule=orc.SPEC_P_PRESETS[4])
+    assert _differs(g, tw) == 0, "GI records differ from the twin run with the refilling preset's schedule"
     tg = orc.trace_wide(w.width, w.groups, w.records, w.entities, 0, work.gi)
     assert np.array_equal(g["t"], tg["t"]), "GI nearest t differs from the sequential wide walk"
     assert (g["status"] >> 24 == 0).all()
@@ -131,6 +135,28 @@ def test_config2_bvh4_speculative_choice(gpu, orc, work, ref):
     same = ~miss & (g["t"] > 0)
     for k in ("u", "v", "w"):
         assert np.abs(g[k][same] - rg[k][same]).max() <= 1e-5
+
+
+def test_config2_sah_bvh4_choice(gpu, orc, work, ref):
+    """The headline choice w4s2:5:2+ao:4:5: the SAH-collapsed BVH4 with 2-triangle leaves (tri_cost 0.3), GI
+    refilling speculative preset 2, RTAO speculative grid preset 5 -- every record of both launches bit-identical
+    to the twin with each preset's schedule."""
+    log("[case] config 2 SAH BVH4: GI refilling speculative preset 2, AO speculative grid preset 5")
+    dw = gpu.upload_wide(gpu.FORMAT_STACKLESS, work.host, 4, 2, None, None, gpu.COLLAPSE_SAH, 0.0, 0.3)
+    w = dw.host
+    g = _trace(gpu, dw.scene, 0, work.gi, gpu.KERNEL_SPECULATIVE_REFILL, 2)
+    a = _trace(gpu, dw.scene, 1, work.ao, gpu.KERNEL_SPECULATIVE, 5)
+    tg = orc.trace_wide_spec(w.width, w.groups, w.records, w.entities, 0, work.gi, schedule=orc.SPEC_P_PRESETS[2])
+    ta = orc.trace_wide_spec(w.width, w.groups, w.records, w.entities, 1, work.ao, variant=5)
+    assert _differs(g, tg) == 0, "GI records differ from the twin"
+    assert _differs(a, ta) == 0, "AO records differ from the twin"
+    rg, ra = ref
+    miss = g["tri"] != rg["tri"]
+    log(f"[parity] config 2 SAH BVH4 (w4s2:5:2+ao:4:5): GI and AO bit-identical to the twin; GI id agreement vs "
+        f"reference walk {1.0 - miss.mean():.7f}")
+    assert (g["t"][miss] == rg["t"][miss]).all(), "an id mismatch at a different distance"
+    assert np.array_equal(g["t"] > 0, rg["t"] > 0)
+    assert np.array_equal(a["t"] > 0, ra["t"] > 0), "RTAO occlusion differs from the reference walk"
 
 
 def test_config2_stack_layout(gpu, orc, work):

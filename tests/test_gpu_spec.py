@@ -4,9 +4,12 @@ Grid launch (kernel 4): 64 consecutive rays share a wave, so the schedule is
 fixed and every record -- t, barycentrics, triangle, mesh, entity, visits,
 status and the triangle-test count -- must equal the host twin that replays it
 (oracle_trace_wide_spec), for every preset, both widths and both queries.
-Refilling launch (kernel 5): which rays share a wave varies, so the nearest t,
-hit / miss and any-hit occlusion must equal the sequential wide walk's
-(oracle_trace_wide) and the triangle may differ only at exactly equal t.
+Refilling launch (kernel 5): which rays share a wave varies from run to run,
+but a ray's record does not depend on it (tests/test_oracle_spec_regroup.py),
+so every record must equal the same twin run with the refilling preset's
+schedule (oracle.SPEC_P_PRESETS), for several grid sizes; the nearest t,
+hit / miss and any-hit occlusion also equal the sequential wide walk's
+(oracle_trace_wide), the triangle up to exactly equal t.
 """
 import sys
 
@@ -79,16 +82,23 @@ def test_grid_equals_twin(gpu, orc, layouts, width, q):
 
 
 @pytest.mark.parametrize("width", [4, 8])
-def test_refill_nearest_and_occlusion(gpu, orc, layouts, width):
+def test_refill_equals_twin(gpu, orc, layouts, width):
     dw = layouts[width]
     w = dw.host
     for q in (0, 1):
         rays = _rays(q)
         seq = orc.trace_wide(width, w.groups, w.records, w.entities, q, rays)
+        twin = {}
         for v in range(6):
             for grid in ((0, 1, 7) if v == 4 else (0,)):
                 log(f"[case] speculative refill width {width} q={q} preset {v} grid {grid}")
-                g, _ = _trace(gpu, dw.scene, q, rays, gpu.KERNEL_SPECULATIVE_REFILL, v, grid=grid)
+                g, gt = _trace(gpu, dw.scene, q, rays, gpu.KERNEL_SPECULATIVE_REFILL, v, grid=grid)
+                if grid == 0:
+                    twin[v] = orc.trace_wide_spec(width, w.groups, w.records, w.entities, q, rays,
+                                                  schedule=orc.SPEC_P_PRESETS[v], with_tests=True)
+                o, ot = twin[v]
+                compare(g, o, f"spec refill w{width} q={q} v={v} grid={grid}")
+                assert g.tobytes() == o.tobytes() and np.array_equal(gt, ot), "not bit-identical to the twin"
                 assert np.array_equal(g["t"] > 0, seq["t"] > 0), "hit / miss differs"
                 assert (g["status"] & 0xFF000000 == 0).all()
                 if q == 0:
@@ -102,7 +112,8 @@ def test_refill_nearest_and_occlusion(gpu, orc, layouts, width):
                     # 1e6 for the axis-aligned ones)
                     hit = g["t"] > 0
                     assert (g["t"][hit] < rays["tmax"][hit]).all()
-    log(f"[parity] speculative refill width {width}: nearest t, hits and occlusion equal the sequential walk")
+    log(f"[parity] speculative refill width {width}: every record bit-identical to the twin; nearest t, hits and "
+        "occlusion equal the sequential walk")
 
 
 def test_instances_flags_and_ragged(gpu, orc):

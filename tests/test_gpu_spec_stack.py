@@ -6,9 +6,10 @@ barycentrics, triangle, mesh, entity, visits, status and the triangle-test count
 replays it (oracle_trace_stack_spec), for every preset and both queries, on a cloud, on the boxes scene, on instanced
 entities with the scene flags and on the crafted guard buffers (the overflow chain reaches the stack entries kept in
 per-lane scratch).
-Refilling launch (kernel 5): which rays share a wave varies, so the nearest t must equal the reference walk's
-(oracle_trace), the triangle may differ only at exactly equal t, and an any-hit ray reports the reference walk's
-first accepted triangle.
+Refilling launch (kernel 5): which rays share a wave varies from run to run, but a ray's record does not depend on
+it (tests/test_oracle_spec_regroup.py), so every record must equal the twin run with the refilling preset's schedule
+(oracle.SBVH2_P_PRESETS); the nearest t also equals the reference walk's (oracle_trace), the triangle up to exactly
+equal t, and an any-hit ray reports the reference walk's first accepted triangle.
 """
 import sys
 
@@ -90,7 +91,11 @@ def test_refill_equals_reference(gpu, orc, cloud):
         for v in orc.SBVH2_PRESETS:
             for grid in ((0, 1, 7) if v == 1 else (0,)):
                 log(f"[case] speculative stack refill q={q} preset {v} grid {grid}")
-                g, _ = _trace(gpu, r.scene(), q, rays, gpu.KERNEL_SPECULATIVE_REFILL, v, grid=grid)
+                g, gt = _trace(gpu, r.scene(), q, rays, gpu.KERNEL_SPECULATIVE_REFILL, v, grid=grid)
+                o, ot = orc.trace_stack_spec(hb["nodes"], hb["triangles"], hb["vertices"], hb["entities"], q, rays,
+                                 schedule=orc.SBVH2_P_PRESETS[v], with_tests=True)
+                compare(g, o, f"refill q={q} v={v} grid={grid}")
+                assert g.tobytes() == o.tobytes() and np.array_equal(gt, ot), "not bit-identical to the twin"
                 assert np.array_equal(g["t"], ref["t"]), "t differs"
                 assert (g["status"] & 0xFF000000 == 0).all()
                 if q == 0:
@@ -100,7 +105,8 @@ def test_refill_equals_reference(gpu, orc, cloud):
                         assert np.array_equal(g[k][same], ref[k][same])
                 else:
                     assert np.array_equal(g["tri"], ref["tri"]), "any hit: not the reference walk's first triangle"
-    log("[parity] speculative stack refill: nearest t, any-hit triangle and occlusion equal the reference walk")
+    log("[parity] speculative stack refill: every record bit-identical to the twin; nearest t, any-hit triangle and "
+        "occlusion equal the reference walk")
 
 
 def test_boxes_instances_flags_and_ragged(gpu, orc):

@@ -5,9 +5,10 @@ Grid launch (kernel 4): 64 consecutive rays share a wave, so the schedule is fix
 barycentrics, triangle, mesh, entity, visits, status and the triangle-test count -- must equal the host twin that
 replays it (oracle_trace_stackless_spec), for every preset and both queries, on a cloud, on instanced entities
 with the scene flags, and on the boxes scene with touching boxes.
-Refilling launch (kernel 5): which rays share a wave varies, so on the cloud (no touching boxes) the nearest t,
-hit / miss and any-hit occlusion must equal the reference walk's (oracle_trace) and the triangle may differ only at
-exactly equal t.
+Refilling launch (kernel 5): which rays share a wave varies from run to run, but a ray's record does not depend on
+it (tests/test_oracle_spec_regroup.py), so every record must equal the twin run with the refilling preset's schedule
+(oracle.SBVH2_P_PRESETS); on the cloud (no touching boxes) the nearest t, hit / miss and any-hit occlusion also equal
+the reference walk's (oracle_trace) and the triangle may differ only at exactly equal t.
 """
 import sys
 
@@ -85,7 +86,11 @@ def test_refill_nearest_and_occlusion(gpu, orc, cloud):
         for v in orc.SBVH2_PRESETS:
             for grid in ((0, 1, 7) if v == 1 else (0,)):
                 log(f"[case] speculative stackless refill q={q} preset {v} grid {grid}")
-                g, _ = _trace(gpu, r.scene(), q, rays, gpu.KERNEL_SPECULATIVE_REFILL, v, grid=grid)
+                g, gt = _trace(gpu, r.scene(), q, rays, gpu.KERNEL_SPECULATIVE_REFILL, v, grid=grid)
+                o, ot = orc.trace_stackless_spec(hb["nodes"], hb["triangles"], hb["vertices"], hb["entities"], q, rays,
+                                 schedule=orc.SBVH2_P_PRESETS[v], with_tests=True)
+                compare(g, o, f"refill q={q} v={v} grid={grid}")
+                assert g.tobytes() == o.tobytes() and np.array_equal(gt, ot), "not bit-identical to the twin"
                 assert np.array_equal(g["t"] > 0, ref["t"] > 0), "hit / miss differs"
                 assert (g["status"] & 0xFF000000 == 0).all()
                 if q == 0:
@@ -97,7 +102,8 @@ def test_refill_nearest_and_occlusion(gpu, orc, cloud):
                 else:
                     hit = g["t"] > 0
                     assert (g["t"][hit] < rays["tmax"][hit]).all()
-    log("[parity] speculative stackless refill: nearest t, hits and occlusion equal the reference walk")
+    log("[parity] speculative stackless refill: every record bit-identical to the twin; nearest t, hits and occlusion "
+        "equal the reference walk")
 
 
 def test_instances_flags_touching_boxes_and_ragged(gpu, orc):
