@@ -4,8 +4,10 @@ crt_trace_noalloc allocates nothing: the persistent kernels' work-queue
 counters live in a static device block per GPU instead of a pool block.  Its
 records must equal crt_trace's bit for bit, on two calls in a row of which the
 second traces fewer rays (the block keeps the first call's counters, which the
-launch must clear), for the persistent kernels over the stackless and BVH4
-layouts and for a kernel that needs no counters.
+launch must clear), for the persistent kernels over the stackless, stack and
+BVH4 layouts -- the refilling speculative kernels included, whose records do
+not depend on which rays share a wave -- and for a kernel that needs no
+counters.
 """
 import ctypes as C
 import sys
@@ -39,7 +41,12 @@ def scenes(gpu):
     r.buffer_entities()
     r.buffer_data()
     wide = gpu.upload_wide(gpu.FORMAT_STACKLESS, r.host_buffers(), 4, 2)
-    return {"stackless": (r, r.scene()), "bvh4": (wide, wide.scene)}
+    rs = gpu.RayIntersector(gpu.FORMAT_STACK)
+    rs.add_object(0, sc.icosphere_cloud(count=300, subdivisions=3, seed=21, box=40.0))
+    rs.push_entity(0)
+    rs.buffer_entities()
+    rs.buffer_data()
+    return {"stackless": (r, r.scene()), "bvh4": (wide, wide.scene), "stack": (rs, rs.scene())}
 
 
 def _call(gpu, fn, scene, q, rays, n, kernel, variant, tri_tests):
@@ -53,7 +60,8 @@ def _call(gpu, fn, scene, q, rays, n, kernel, variant, tri_tests):
 
 
 @pytest.mark.parametrize("layout,kernel,variant", [("stackless", 2, 0), ("stackless", 2, 3), ("bvh4", 2, 8),
-                                                   ("bvh4", 2, 0), ("bvh4", 3, 7)])
+                                                   ("bvh4", 2, 0), ("bvh4", 3, 7), ("bvh4", 5, 2),
+                                                   ("stackless", 5, 3), ("stack", 5, 2), ("stack", 2, 4)])
 def test_noalloc_equals_crt_trace(gpu, scenes, layout, kernel, variant):
     _, scene = scenes[layout]
     lib = gpu.lib()
