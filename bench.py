@@ -110,13 +110,16 @@ def kernel_label(c: dict, gi_only: bool = False) -> str:
         label += "; AO launch: " + _kernel_name(c["layout"], c["ao_kernel"], c["ao_variant"])
     if c.get("ao_order") == 1 and not gi_only:
         label += "; AO children in slot order"
+    if c.get("streams") == 2 and not gi_only:
+        label += "; AO and GI launches overlapped on two streams"
     return label
 
 
 def choice_name(c: dict) -> str:
     """'K:V' for a BVH2 kernel choice, 'wWlL:K:V' for kernel K, variant V over the wide layout of width W
     and leaf size L ('wWsL' with the SAH collapse); a choice whose AO launch uses another kernel gets '+ao:K:V', and ':s' after it when the
-    AO walks take children in slot order (CRT_FLAG_SLOT_ORDER)."""
+    AO walks take children in slot order (CRT_FLAG_SLOT_ORDER); '+overlap' when the AO and GI launches run on two
+    streams."""
     if c["layout"] == "bvh2":
         name = f"{c['kernel']}:{c['variant']}"
     elif c["layout"] == "stack":
@@ -127,6 +130,8 @@ def choice_name(c: dict) -> str:
         name += f"+ao:{c['ao_kernel']}:{c['ao_variant']}"
     if c.get("ao_order") == 1:
         name += ":s" if "ao_kernel" in c else f"+ao:{c['kernel']}:{c['variant']}:s"
+    if c.get("streams") == 2:
+        name += "+overlap"
     return name
 
 
@@ -148,7 +153,7 @@ def layout_name(layout) -> str:
 
 def gi_choice_name(c: dict) -> str:
     """The choice of the GI launch alone."""
-    return choice_name({k: v for k, v in c.items() if not k.startswith("ao_")})
+    return choice_name({k: v for k, v in c.items() if not k.startswith("ao_") and k != "streams"})
 
 
 def ao_of(c: dict) -> tuple:
@@ -733,6 +738,70 @@ def main(argv=None) -> int:
             f"AO {ao_ms:.3f} ms, GI {gi_ms:.3f} ms -> {res['mrays']:.1f} Mrays/s")
         return res
 
+    def run_overlapped(c: dict) -> dict:
+        """The same step with the AO launch on a second stream: the two launches are independent, so each
+        fills the other's tail.  A fork and a join event per step keep every step's launches together; events
+        on each stream give each launch's duration inside the timed region.  The records must equal those of a
+        sequential step byte for byte."""
+        import ctypes as C
+        kernel, variant = c["kernel"], c["variant"]
+        scene = scene_of(c)
+        kw = dict(block_size=c.get("block", args.block), lds_stack=c.get("lds", args.lds_stack), kernel=kernel,
+                  variant=variant, stream=stream)
+        ak, av = ao_of(c)
+        s2 = crt.Stream()
+        kw_ao = dict(kw, kernel=ak, variant=av, flags=ao_flags(crt, c), stream=s2)
+        lib = crt.lib()
+        # a sequential step's records, to compare with
+        crt.trace(scene, crt.QUERY_ANY, d_ao, h_ao, **dict(kw_ao, stream=stream))
+        crt.trace(scene, crt.QUERY_CLOSEST, d_gi, h_gi, **kw)
+        stream.sync()
+        seq = (h_ao.download(), h_gi.download())
+        h_ao.fill_bytes(0xAB)
+        h_gi.fill_bytes(0xAB)
+        crt.lib().crt_device_sync()
+
+        def step(ev, k):
+            ev.record(2 + 3 * k, sh)                      # fork
+            crt._lib.check(lib.crt_stream_wait_event(s2.handle, C.c_void_p(ev.ev[2 + 3 * k])))
+            crt.trace(scene, crt.QUERY_ANY, d_ao, h_ao, **kw_ao)
+            crt.trace(scene, crt.QUERY_CLOSEST, d_gi, h_gi, **kw)
+            ev.record(3 + 3 * k, sh)                      # GI done
+            ev.record(4 + 3 * k, s2.handle)               # AO done; join
+            crt._lib.check(lib.crt_stream_wait_event(sh, C.c_void_p(ev.ev[4 + 3 * k])))
+
+        warm = Events(crt, 3 * max(1, args.warmup) + 2)
+        for k in range(args.warmup):
+            step(warm, k)
+        stream.sync()
+        crt.lib().crt_device_sync()
+        warm.close()
+        ev = Events(crt, 3 * K + 2)
+        w0 = time.perf_counter()
+        ev.record(0, sh)
+        for k in range(K):
+            step(ev, k)
+        ev.record(1, sh)
+        stream.sync()
+        crt.lib().crt_device_sync()
+        w1 = time.perf_counter()
+        ms_step = ev.ms(0, 1) / K
+        gi_ms = float(np.mean([ev.ms(2 + 3 * k, 3 + 3 * k) for k in range(K)]))
+        ao_ms = float(np.mean([ev.ms(2 + 3 * k, 4 + 3 * k) for k in range(K)]))
+        ev.close()
+        s2.close()
+        same = h_ao.download().tobytes() == seq[0].tobytes() and h_gi.download().tobytes() == seq[1].tobytes()
+        res = dict(c, streams=2, ms_step=ms_step, wall_ms_step=(w1 - w0) * 1e3 / K, ao_ms=ao_ms, gi_ms=gi_ms,
+                   ao_ms_alone=c["ao_ms"], gi_ms_alone=c["gi_ms"], records_equal_sequential=same,
+                   mrays=(n_ao + n_gi) / (ms_step * 1e-3) / 1e6)
+        res["choice"] = choice_name(res)
+        log(f"{res['choice']}: {ms_step:.3f} ms/step (wall {res['wall_ms_step']:.3f}) with AO and GI on two streams "
+            f"(AO {ao_ms:.3f} ms, GI {gi_ms:.3f} ms from the fork) -> {res['mrays']:.1f} Mrays/s; records equal a "
+            f"sequential step: {same}")
+        if not same:
+            raise RuntimeError("overlapped launches wrote other records than a sequential step")
+        return res
+
     def bvh2(k, v):
         return dict(layout="bvh2", kernel=k, variant=v)
 
@@ -760,7 +829,7 @@ def main(argv=None) -> int:
             candidates += [stack(k, v, 0) for k in (crt.KERNEL_SPECULATIVE, crt.KERNEL_SPECULATIVE_REFILL)
                            for v in range(4)]
     if args.only:
-        main_part, _, ao_part = args.only.partition("+ao:")
+        main_part, _, ao_part = args.only.replace("+overlap", "").partition("+ao:")
         parts = main_part.split(":")
         if parts[0].startswith("w"):
             sah = "s" in parts[0]
@@ -798,6 +867,8 @@ def main(argv=None) -> int:
                 "ao_variant")},
                 ao_order=1)))
     thread = results[0]
+    # the fastest choice again with its two launches overlapped on two streams
+    results.append(run_overlapped(max(results, key=lambda r: r["mrays"])))
     best = max(results, key=lambda r: r["mrays"])
     best_scene = scene_of(best)
     best_fmt = best_scene.format
@@ -1108,6 +1179,11 @@ def main(argv=None) -> int:
                 "kernel": gi_choice_name(best) + " (GI closest hit)", "launch_ms": round(gi_s * 1e3, 4),
                 "bytes_per_launch": out["gi"]["bytes"], "bytes_per_ray": round(out["gi"]["bytes"] / n_gi, 1),
                 "traffic_source": traffic_src}
+    if best.get("streams") == 2:
+        # launch_ms is the GI launch's duration inside the overlapped timed region (sharing the GPU with the
+        # AO launch); the same launch alone, as timed in the sequential step:
+        roofline["launch_ms_alone"] = round(best["gi_ms_alone"], 4)
+        roofline["achieved_alone"] = round(out["gi"]["bytes"] / (best["gi_ms_alone"] * 1e-3) / 1e9, 1)
     if traffic:
         # algorithmic bytes price every node and record fetch as HBM bytes; the caches serve most of them
         roofline["traffic_frac_of_peak"] = round(traffic / gi_s / 1e9 / HBM_PEAK_GBS, 4)
@@ -1154,6 +1230,7 @@ def main(argv=None) -> int:
             "rays_per_step": rays_step,
             "kernel": choice_name(best),
             "parallelism": "single GPU",
+            "streams": best.get("streams", 1),
         },
         "plain_by_format": per_format,
         "batched_mrays_s": (round(rays_step / (c2_batched["ms"] * 1e-3) / 1e6, 2) if c2_batched else None),

@@ -217,6 +217,7 @@ SIGNATURES = {
     "crt_event_create": (C.c_int, [C.POINTER(C.c_void_p)]),
     "crt_event_destroy": (C.c_int, [C.c_void_p]),
     "crt_event_record": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "crt_stream_wait_event": (C.c_int, [C.c_void_p, C.c_void_p]),
     "crt_event_elapsed_ms": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_float)]),
     "crt_trace": (C.c_int, [C.POINTER(SceneView), C.c_int, C.c_uint32, C.c_void_p, C.c_void_p, C.c_int64,
                             C.POINTER(TraceConfig), C.c_void_p]),

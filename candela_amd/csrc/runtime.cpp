@@ -588,6 +588,13 @@ int crt_event_record(void* event, void* stream) {
     return CRT_OK;
 }
 
+int crt_stream_wait_event(void* stream, void* event) {
+    if (!event) return crt::set_error(CRT_E_INVALID, "crt_stream_wait_event: null event");
+    CRT_HIP(hipStreamWaitEvent(static_cast<hipStream_t>(stream), static_cast<hipEvent_t>(event), 0),
+            "hipStreamWaitEvent");
+    return CRT_OK;
+}
+
 int crt_event_elapsed_ms(void* start, void* end, float* ms) {
     if (!start || !end || !ms) return crt::set_error(CRT_E_INVALID, "crt_event_elapsed_ms: null argument");
     CRT_HIP(hipEventSynchronize(static_cast<hipEvent_t>(end)), "hipEventSynchronize");

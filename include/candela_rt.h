@@ -332,6 +332,10 @@ int crt_graph_destroy(void* graph);
 int crt_event_create(void** event);
 int crt_event_destroy(void* event);
 int crt_event_record(void* event, void* stream);
+/* Work enqueued on `stream` after this call waits for `event` (recorded on
+ * another stream): independent ray batches can then run on two streams and
+ * fill each other's launch tails (no reference counterpart). */
+int crt_stream_wait_event(void* stream, void* event);
 /* Waits for `end` and returns the milliseconds between the two events. */
 int crt_event_elapsed_ms(void* start, void* end, float* ms);
 
