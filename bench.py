@@ -764,8 +764,10 @@ def main(argv=None) -> int:
         def step(ev, k):
             ev.record(2 + 3 * k, sh)                      # fork
             crt._lib.check(lib.crt_stream_wait_event(s2.handle, C.c_void_p(ev.ev[2 + 3 * k])))
-            crt.trace(scene, crt.QUERY_ANY, d_ao, h_ao, **kw_ao)
+            # GI first: its persistent waves take the device, the AO grid's waves fill the slots they leave
+            # (0.711 against 0.722 ms per step with AO enqueued first)
             crt.trace(scene, crt.QUERY_CLOSEST, d_gi, h_gi, **kw)
+            crt.trace(scene, crt.QUERY_ANY, d_ao, h_ao, **kw_ao)
             ev.record(3 + 3 * k, sh)                      # GI done
             ev.record(4 + 3 * k, s2.handle)               # AO done; join
             crt._lib.check(lib.crt_stream_wait_event(sh, C.c_void_p(ev.ev[4 + 3 * k])))
