@@ -871,7 +871,15 @@ def main(argv=None) -> int:
     thread = results[0]
     # the three fastest choices again with their two launches overlapped on two streams (the best pairing on one
     # stream need not be the best on two)
-    for c in sorted(results, key=lambda r: -r["mrays"])[:3]:
+    top = sorted(results, key=lambda r: -r["mrays"])[:3]
+    # ... and the best choice of each of the reference's own layouts, for their per-layout figures
+    for lay in ("bvh2", "stack"):
+        own = [r for r in results if r["layout"] == lay]
+        if own:
+            b_ = max(own, key=lambda r: r["mrays"])
+            if b_ not in top:
+                top.append(b_)
+    for c in top:
         results.append(run_overlapped(c))
     best = max(results, key=lambda r: r["mrays"])
     best_scene = scene_of(best)
